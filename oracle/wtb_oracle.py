@@ -1,0 +1,530 @@
+"""CPU oracle for the WanTiBEXOS BSE path — TEST INFRASTRUCTURE ONLY.
+
+This module restates, in numpy, the reference algorithm of the BSE pipeline
+(bsesolver + bsedielraw) function by function, each citing the reference
+file:line it follows.  It is the checker for the HIP product path: only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+may import it, and never as the thing measured or shipped.
+
+Float32 quirks of the reference are reproduced on purpose:
+  * phases k.R and Cartesian R components pass through COMPLEX(4)
+    (``cmplx(0.0, x)`` has default kind), hamiltonian_tb.f90:64-66,
+    optics.f90:39-47;
+  * pi = acos(-1.) in single precision, mhkpack_subs.f90:413;
+  * Coulomb constant -(0.0904756)*10**3 evaluated in single precision,
+    coulomb_pot.f90:6 (and every potential);
+  * cmplx(-efermi,0.0), cmplx(0.0,sme), 1.0/sqrt(2.), gama = 180.90,
+    factor = 0.15 are single precision (hamiltonian_tb.f90:383,
+    optics.f90:139,188,335, bse_solver-tool-diel.f90:376-377).
+  * optsp stores 1/(Ec-Ev+i*sme) into a REAL variable, so only its real part
+    multiplies the dipole (optics.f90:130,149).
+
+Pinning (how each function is checked; see tests/test_oracle_*.py):
+  k grid, stt, Delta tables     bit-exact vs the compiled reference dump
+  bands / vectors (gauge)       vs reference dump (MKL ZHEEV), <=1e-12 eV
+  dipoles                       vs reference dump, <=1e-10 relative
+  BSE matrix                    vs reference dump (full upper triangle, small N)
+  Coulomb potentials            vs reference dump (matrix rows) + closed forms
+  BSE eigenvalues/activities    vs reference dump; sum rule; Hermiticity
+  eps(omega), imagrenorm        vs reference dump (F15.4 path and full precision)
+The per-k and BSE eigensolves use LAPACK through numpy (zheevd) in place of
+MKL ZHEEV; eigenvalues are basis independent and the per-k eigenvectors are
+put into ZHEEV('L')'s gauge (component on orbital 1 real), which is all the
+reference's outputs depend on.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+F32 = np.float32
+
+
+def fl32(x):
+    """Round a double to float32 and back (what a default-kind REAL/COMPLEX does)."""
+    return np.asarray(x, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+PI32 = float(F32(math.pi))                                  # acos(-1.)  mhkpack_subs.f90:413
+CIC = float(F32(-F32(0.0904756) * F32(1000.0)))             # coulomb_pot.f90:6
+GAMA = float(F32(180.90))                                   # optics.f90:188
+RENORM_FACTOR = float(F32(0.15))                            # optics.f90:335
+INV_SQRT2_F32 = float(F32(1.0) / np.sqrt(F32(2.0)))         # bse_solver-tool-diel.f90:376
+
+
+# ---------------------------------------------------------------------------
+# geometry and k grid
+# ---------------------------------------------------------------------------
+
+def prodvec(v1, v2):
+    """general_subs.f90:183-193"""
+    return np.array([(v1[1] * v2[2]) - (v1[2] * v2[1]),
+                     (v1[2] * v2[0]) - (v1[0] * v2[2]),
+                     (v1[0] * v2[1]) - (v1[1] * v2[0])])
+
+
+def vecsize(v):
+    """general_subs.f90:286-294"""
+    return math.sqrt((v[0] ** 2) + (v[1] ** 2) + (v[2] ** 2))
+
+
+def vcell2D(rlat):
+    """general_subs.f90:195-206: |a1 x a2|"""
+    return vecsize(prodvec(rlat[0], rlat[1]))
+
+
+def vcell3D(rlat):
+    """general_subs.f90:208-221: |(a1 x a2).a3|"""
+    vx = prodvec(rlat[0], rlat[1])
+    return abs(vx[0] * rlat[2][0] + vx[1] * rlat[2][1] + vx[2] * rlat[2][2])
+
+
+def alat2D(rlat):
+    """general_subs.f90:248-260"""
+    return 0.5 * (vecsize(rlat[0]) + vecsize(rlat[1]))
+
+
+def recvec(rlat):
+    """mhkpack_subs.f90:409-432 (single-precision pi)"""
+    a1, a2, a3 = (np.asarray(r, dtype=np.float64) for r in rlat)
+    v23, v31, v12 = prodvec(a2, a3), prodvec(a3, a1), prodvec(a1, a2)
+    vol = abs((a1[0] * v23[0]) + (a1[1] * v23[1]) + (a1[2] * v23[2]))
+    f = (2.0 * PI32) / vol
+    return np.array([f * v23, f * v31, f * v12])
+
+
+def monkhorst_pack(ngrid, shift, rlat):
+    """mhkpack_subs.f90:359-407 -> kpt (nk,3); index = i*n2*n3 + j*n3 + l (l fastest)."""
+    n1, n2, n3 = (int(v) for v in ngrid)
+    b = recvec(rlat)
+    sh = [float(v) for v in shift]
+    kshift = [b[0][c] * sh[0] + b[1][c] * sh[1] + b[2][c] * sh[2] for c in range(3)]
+    s1 = [b[0][c] / float(n1) for c in range(3)]
+    s2 = [b[1][c] / float(n2) for c in range(3)]
+    s3 = [b[2][c] / float(n3) for c in range(3)]
+    out = np.empty((n1 * n2 * n3, 3))
+    cnt = 0
+    for i in range(n1):
+        for j in range(n2):
+            for l in range(n3):
+                for c in range(3):
+                    out[cnt, c] = s1[c] * float(i) + s2[c] * float(j) + s3[c] * float(l) + kshift[c]
+                cnt += 1
+    return out
+
+
+def rkmesh2D(rk, rlat):
+    """mhkpack_subs.f90:30-52"""
+    b = recvec(rlat)
+    pi = PI32  # acos(-1.0) single precision, used in double arithmetic
+    vs = [vecsize(b[i]) / (2 * pi) for i in range(3)]
+    return [int(max(1.0, rk * vs[0] + 0.5)), int(max(1.0, rk * vs[1] + 0.5)), 1]
+
+
+# ---------------------------------------------------------------------------
+# stages (a)+(b): H(k), per-k eigensolve
+# ---------------------------------------------------------------------------
+
+def phases(kpt, rvec, rlat):
+    """fl32(k.R) of hamiltonian_tb.f90:62-68, shape (nk, nvec)."""
+    r = np.asarray(rvec, dtype=np.float64)
+    L = np.asarray(rlat, dtype=np.float64)
+    rx = (r[:, 0] * L[0, 0] + r[:, 1] * L[1, 0]) + r[:, 2] * L[2, 0]
+    ry = (r[:, 0] * L[0, 1] + r[:, 1] * L[1, 1]) + r[:, 2] * L[2, 1]
+    rz = (r[:, 0] * L[0, 2] + r[:, 1] * L[1, 2]) + r[:, 2] * L[2, 2]
+    k = np.asarray(kpt, dtype=np.float64)
+    x = (k[:, 0:1] * rx[None, :] + k[:, 1:2] * ry[None, :]) + k[:, 2:3] * rz[None, :]
+    return fl32(x)
+
+
+def cart_r(rvec, rlat):
+    """fl32 Cartesian R components of optics.f90:41-45, shape (nvec, 3)."""
+    r = np.asarray(rvec, dtype=np.float64)
+    L = np.asarray(rlat, dtype=np.float64)
+    out = np.empty((len(r), 3))
+    for a in range(3):
+        out[:, a] = (r[:, 0] * L[0, a] + r[:, 1] * L[1, a]) + r[:, 2] * L[2, a]
+    return fl32(out)
+
+
+def hk_stack(kpt, model):
+    """H(k) = sum_R e^{i fl32(k.R)} H(R)/ndeg  (hamiltonian_tb.f90:71-77), (nk,nw,nw)."""
+    ph = np.exp(1j * phases(kpt, model.rvec, model.rlat))
+    hr = (model.hop + 1j * model.ihop) * (1.0 / model.ndeg.astype(np.float64))[:, None, None]
+    return np.einsum("kr,rmn->kmn", ph, hr)
+
+
+def dhk_stack(kpt, model):
+    """dH/dk_alpha = sum_R i fl32(R_alpha) e^{i fl32(k.R)} H(R)/ndeg (optics.f90:1-97),
+    shape (3, nk, nw, nw).  The R=0 diagonal is zeroed as in optics.f90:61-66
+    (its weight fl32(R_alpha) is 0 anyway)."""
+    ph = np.exp(1j * phases(kpt, model.rvec, model.rlat))
+    rc = cart_r(model.rvec, model.rlat)
+    hr = (model.hop + 1j * model.ihop) * (1.0 / model.ndeg.astype(np.float64))[:, None, None]
+    zero = np.where((model.rvec == 0).all(axis=1))[0]
+    for i0 in zero:
+        hr[i0] = hr[i0] - np.diag(np.diag(hr[i0]))
+    return np.stack([np.einsum("kr,rmn->kmn", ph * (1j * rc[None, :, a]), hr) for a in range(3)])
+
+
+def gauge_fix(U):
+    """Put eigenvector columns in ZHEEV('L')'s gauge: component 1 real and >= 0."""
+    x1 = U[..., 0:1, :]
+    mag = np.abs(x1)
+    ph = np.where(mag > 0, np.conj(x1) / np.where(mag > 0, mag, 1.0), 1.0)
+    return U * ph
+
+
+def eigsys(kpt, model):
+    """hamiltonian_tb.f90:2-137 for all k: returns (E (nk,nw), U (nk,nw,nw) columns =
+    eigenvectors in ZHEEV('L') gauge, nocp (nk,))."""
+    H = hk_stack(kpt, model)
+    ef = float(fl32(-model.efermi))                      # cmplx(-efermi,0.0), :383
+    H = H + ef * np.eye(model.nw)[None]
+    E, U = np.linalg.eigh(H, UPLO="L")
+    U = gauge_fix(U)
+    pos = E > 0.0
+    if not pos.any(axis=1).all():
+        raise ValueError("eigsys: no band above E_F at some k (nocp undefined in the reference)")
+    nocp = np.argmax(pos, axis=1)                        # first E>0 (0-based) = nocp
+    E = E.copy()
+    for k in range(len(E)):
+        E[k, nocp[k]:] += model.scs                      # :129-133
+    return E, U, nocp.astype(np.int32)
+
+
+def select_bands(E, U, nocp, nc, nv):
+    """bse_solver-tool-diel.f90:262-303: eigv (nk, nv+nc), vector (nk, nv+nc, nw)."""
+    nk = len(E)
+    idx = nocp[:, None] - nv + np.arange(nc + nv)[None, :]
+    if (idx < 0).any() or (idx >= E.shape[1]).any():
+        raise ValueError("quantumnumbers2: band window outside 1..nw")
+    eigv = np.take_along_axis(E, idx, axis=1)
+    vector = np.stack([U[k][:, idx[k]].T for k in range(nk)])
+    return eigv, vector
+
+
+def quantumnumbers2(nk, nc, nv, nocp, nw):
+    """general_subs.f90:501-567: stt (N,4) 1-based (I, v, c, k); k outer, c, v inner."""
+    if (nv > nocp).any():
+        raise ValueError("Number of valence states higher then ocuppied states")
+    if (nc > (nw - nocp)).any():
+        raise ValueError("Number of conduction states higher then unocuppied states")
+    k, c, v = np.meshgrid(np.arange(1, nk + 1), np.arange(nv + 1, nv + nc + 1),
+                          np.arange(1, nv + 1), indexing="ij")
+    n = nk * nc * nv
+    stt = np.empty((n, 4), dtype=np.int32)
+    stt[:, 0] = np.arange(1, n + 1)
+    stt[:, 1] = v.ravel()
+    stt[:, 2] = c.ravel()
+    stt[:, 3] = k.ravel()
+    return stt
+
+
+def dipoles(kpt, model, eigv, vector, stt, sme):
+    """optsp/hlm/sandwich per transition (optics.f90:102-156, general_subs.f90:69-86),
+    driver bse_solver-tool-diel.f90:365-377.  Returns (5, N): hrx, hry, hrz, hrsp, hrsm."""
+    dH = dhk_stack(kpt, model)                           # (3, nk, nw, nw)
+    ki = stt[:, 3] - 1
+    vi = stt[:, 1] - 1
+    ci = stt[:, 2] - 1
+    vv = vector[ki, vi]                                  # (N, nw)
+    vc = vector[ki, ci]
+    ev = eigv[ki, vi]
+    ec = eigv[ki, ci]
+    esp = (ec - ev) + 1j * float(fl32(sme))
+    factor = (1.0 / esp).real                            # REAL factor = 1./esp, optics.f90:130,149
+    out = np.empty((5, len(stt)), dtype=np.complex128)
+    for a in range(3):
+        aux = np.einsum("nmj,nj->nm", dH[a][ki], vv)     # matvec
+        out[a] = factor * np.einsum("nm,nm->n", np.conj(vc), aux)
+    out[3] = (out[0] + 1j * out[1]) * INV_SQRT2_F32
+    out[4] = (out[0] - 1j * out[1]) * INV_SQRT2_F32
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Coulomb potentials (coulomb_pot.f90), V(k1,k2) for arrays of k-differences
+# ---------------------------------------------------------------------------
+
+def _modk(dk):
+    return np.sqrt((dk[..., 0] ** 2 + dk[..., 1] ** 2) + dk[..., 2] ** 2)
+
+
+def coulomb(coultype, dk, ngrid, rlat, ediel, lc, ez, w, r0, tolr):
+    """V for k1-k2 = dk (..., 3), selecting the potential as matrizelbse does
+    (bse_subs.f90:45-91)."""
+    dk = np.asarray(dk, dtype=np.float64)
+    nkt = int(ngrid[0]) * int(ngrid[1]) * int(ngrid[2])
+    modk = _modk(dk)
+    small = modk < tolr
+    safe = np.where(small, 1.0, modk)
+    ct = coultype.strip().upper()
+    if ct == "V2DRK":                                    # coulomb_pot.f90:276-340
+        vc = vcell2D(rlat)
+        vbz = 1.0 / (nkt * vc)
+        epar, et, eb = float(ediel[1]), float(ediel[0]), float(ediel[2])
+        eta = math.sqrt(epar / ez)
+        kappa = math.sqrt(epar * ez)
+        pb = (eb - kappa) / (eb + kappa)
+        pt = (et - kappa) / (et + kappa)
+        aux1 = (1.0 - (pb * pt * np.exp(-2.0 * safe * eta * lc))) * kappa
+        aux2 = (1.0 - (pt * np.exp(-eta * safe * lc))) * (1.0 - (pb * np.exp(-eta * safe * lc)))
+        aux3 = r0 * safe * np.exp(-safe * w)
+        ew = (aux1 / aux2) + aux3
+        v = (vbz * CIC) * np.exp(-safe * w) * (1.0 / ew) * (1.0 / safe)
+        return np.where(small, 0.0, v)
+    if ct in ("V3D", "V3DL"):                            # coulomb_pot.f90:55-122
+        vc = vcell3D(rlat)
+        ed = 1.0 if ct == "V3D" else float(ediel[1])
+        vbz = 1.0 / (nkt * vc)
+        v = vbz * (CIC / ed) * (1.0 / (safe * safe))
+        return np.where(small, 0.0, v)
+    if ct == "V2DK":                                     # coulomb_pot.f90:2-51
+        a0 = alat2D(rlat)
+        vc = vcell2D(rlat)
+        rr0 = ((ediel[1] - 1.0) * lc) / (ediel[0] + ediel[2])
+        vbz = 1.0 / (nkt * vc)
+        g1 = float(int(ngrid[0]) * int(ngrid[1]))
+        auxi = (2.0 * PI32 * rr0) / (a0 * math.sqrt(g1))
+        ed = (ediel[0] + ediel[2]) / 2.0
+        alpha1, alpha2, alpha3 = float(F32(1.76)), 1.0, 0.0
+        v0 = vbz * (CIC / ed) * (a0 * math.sqrt(g1) / (2.0 * PI32)) * (alpha1 + auxi * alpha2 + alpha3 * auxi ** 2)
+        v = vbz * (CIC / ed) * (1.0 / (safe * (1 + (rr0 * safe))))
+        return np.where(small, v0, v)
+    if ct == "V2DT":                                     # coulomb_pot.f90:126-183
+        vc = vcell3D(rlat)
+        vbz = 1.0 / (nkt * vc)
+        gz = np.abs(dk[..., 2])
+        gpar = np.sqrt((dk[..., 0] * dk[..., 0]) + (dk[..., 1] * dk[..., 1]))
+        rc = 0.5 * rlat[2][2]
+        c0 = (vbz * CIC) * (-0.5 * rc * rc)
+        c1 = (vbz * CIC) * (1.0 / (safe * safe)) * (1.0 - np.cos(gz * rc) - (gz * rc * np.sin(gz * rc)))
+        gps = np.where(gpar < tolr, 1.0, gpar)
+        aux1 = gz / gps
+        aux2 = gps * rc
+        aux3 = gz * rc
+        c2 = (vbz * CIC) * (1.0 / (safe * safe)) * (1.0 + (np.exp(-aux2) * (aux1 * np.sin(aux3) - np.cos(aux3))))
+        return np.where((gpar < tolr) & (gz < tolr), c0, np.where(gpar < tolr, c1, c2))
+    if ct == "V2DT2":                                    # coulomb_pot.f90:233-272
+        vc = vcell3D(rlat)
+        vbz = 1.0 / (nkt * vc)
+        qxy = np.sqrt((dk[..., 0] * dk[..., 0]) + (dk[..., 1] * dk[..., 1]))
+        v = (vbz * CIC) * (1.0 / (safe * safe)) * (1.0 - np.exp(-0.5 * qxy * lc) * np.cos(0.5 * lc * dk[..., 2]))
+        return np.where(small, 0.0, v)
+    if ct == "V0DT":                                     # coulomb_pot.f90:187-228
+        vc = vcell3D(rlat)
+        vbz = 1.0 / (nkt * vc)
+        cr = 0.5 * min(vecsize(rlat[0]), vecsize(rlat[1]), vecsize(rlat[2]))
+        v0 = (vbz * CIC) * 0.5 * cr * cr
+        v = (vbz * CIC) * (1.0 / (safe * safe)) * (1.0 - np.cos(cr * safe))
+        return np.where(small, v0, v)
+    if ct == "V2DOH":                                    # coulomb_pot.f90:344-377
+        vc = vcell2D(rlat)
+        vbz = 1.0 / (nkt * vc)
+        v = (vbz * CIC) * np.exp(-w * safe) * (1.0 / (ez * safe))
+        return np.where(small, 0.0, v)
+    if ct == "V1DT":                                     # coulomb_pot.f90:381-432
+        from scipy.special import j0, j1, k0, k1
+        vc = vcell3D(rlat)
+        vbz = 1.0 / (nkt * vc)
+        ax = np.abs(dk[..., 0])
+        qyz = np.sqrt((dk[..., 1] * dk[..., 1]) + (dk[..., 2] * dk[..., 2]))
+        out = np.empty(np.shape(modk))
+        a_small = ax < tolr
+        q_small = qyz < tolr
+        out = np.where(a_small & q_small, (vbz * CIC) * ((lc * lc) / 16.0) * (2.0 * math.log(lc) - 1.0), 0.0)
+        for idx in zip(*np.nonzero(a_small & ~q_small)):
+            out[idx] = (vbz * CIC) * integral_j0pot(lc, qyz[idx])
+        gen = ~a_small
+        axs = np.where(gen, ax, 1.0)
+        c1 = qyz * lc * j1(qyz * lc) * k0(axs * lc)
+        c2 = axs * lc * j0(qyz * lc) * k1(axs * lc)
+        out = np.where(gen, (vbz * CIC) * (1.0 + c1 + c2), out)
+        return out
+    raise ValueError("Wrong Coulomb Potential")
+
+
+def integral_j0pot(rmax, gperp):
+    """coulomb_pot.f90:437-469: trapezoid of r J0(g r) log r on 100 points."""
+    from scipy.special import j0
+    n = 100
+    eta = float(F32(1e-4))
+    r = ((np.arange(1, n + 1) - 1.0) / (n - 1.0)) * rmax + eta
+    f = r * j0(r * gperp) * np.log(r)
+    h = r[1] - r[0]
+    res = 0.0
+    for i in range(n - 1):
+        res = res + (h / 2.0) * (f[i] + f[i + 1])
+    return res
+
+
+# ---------------------------------------------------------------------------
+# stage (c): BSE matrix
+# ---------------------------------------------------------------------------
+
+def bse_matrix(kpt, eigv, vector, stt, params, ngrid, rlat):
+    """Full Hermitian BSE matrix whose upper triangle is matrizelbse's
+    (bse_subs.f90:2-124, loop bse_solver-tool-diel.f90:431-458):
+      H_II = (Ec-Ev) + V(0);  H_IJ = V(k_I-k_J) <c_I|c_J> <v_I|v_J>  (J>I)."""
+    ki = stt[:, 3] - 1
+    vi = stt[:, 1] - 1
+    ci = stt[:, 2] - 1
+    vc = vector[ki, ci]                                  # (N, nw)
+    vv = vector[ki, vi]
+    oc = np.conj(vc) @ vc.T                              # <c_I|c_J>
+    ov = np.conj(vv) @ vv.T
+    kI = kpt[ki]
+    dk = kI[:, None, :] - kI[None, :, :]
+    V = coulomb(params["coultype"], dk, ngrid, rlat, params["ediel"], params["lc"],
+                params["ez"], params["w"], params["r0"], params["ktol"])
+    H = V * oc * ov
+    H = np.triu(H, 1)
+    H = H + np.conj(H.T)
+    diag = (eigv[ki, ci] - eigv[ki, vi]) + np.diag(V)
+    H[np.arange(len(H)), np.arange(len(H))] = diag
+    return H
+
+
+def delta_index(ngrid):
+    """Integer k-k' table (Delta1, Delta2, Delta3) for the BSE basis order (k outer):
+    row/col = k index, entries = (i-i', j-j', l-l'); the reference never folds
+    k-k' (general_subs.f90:297-308)."""
+    n1, n2, n3 = ngrid
+    i, j, l = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
+    g = np.stack([i.ravel(), j.ravel(), l.ravel()], axis=1)
+    return g[:, None, :] - g[None, :, :]
+
+
+# ---------------------------------------------------------------------------
+# stage (d)+(e1): eigensolve and oscillator strengths
+# ---------------------------------------------------------------------------
+
+def bse_eig(H):
+    """ZHEEV('V','U') (bse_solver-tool-diel.f90:485-488): ascending eigenpairs."""
+    return np.linalg.eigh(H, UPLO="U")
+
+
+def activities(X, hvec):
+    """dielbsev/dielbsep (bse_subs.f90:278-378): f_n = |sum_j X_jn h_j|^2 (no conjugate
+    on X) for xx,yy,zz,sp,sm and Re[(X^T h_a)(X^T h_b)^*] for xy,xz,yz.
+    hvec = (5,N) [x,y,z,sp,sm]; returns (8,N) in the order xx,yy,zz,xy,xz,yz,sp,sm."""
+    t = X.T @ hvec.T                                     # (N, 5)
+    tx, ty, tz, tp, tm = (t[:, i] for i in range(5))
+    sq = lambda a: (a * np.conj(a)).real
+    cr = lambda a, b: (a * np.conj(b)).real
+    return np.stack([sq(tx), sq(ty), sq(tz), cr(tx, ty), cr(tx, tz), cr(ty, tz), sq(tp), sq(tm)])
+
+
+def f15_4(x):
+    """Value after a write "(F15.4)" and list-directed read back
+    (bse_solver-tool-diel.f90:610, bse_diel-tool.f90:117-123)."""
+    x = np.asarray(x, dtype=np.float64)
+    flat = [float(f"{v:.4f}") for v in x.ravel()]
+    return np.array(flat, dtype=np.float64).reshape(x.shape)
+
+
+# ---------------------------------------------------------------------------
+# stage (e2): eps(omega)
+# ---------------------------------------------------------------------------
+
+def avgsme(E):
+    """optics.f90:302-324: 10 * (sum of successive differences) / N."""
+    s = 0.0
+    for i in range(1, len(E)):
+        s = s + E[i] - E[i - 1]
+    return 10.0 * (s / float(len(E)))
+
+
+def optdiel(rtype, vc, ngrid, elux, E, f, sme, smeavg=None):
+    """optics.f90:160-244 for an array of photon energies elux (vectorised over elux)."""
+    baux = float(F32(float(int(ngrid[0]) * int(ngrid[1]) * int(ngrid[2])))) * vc
+    caux = GAMA / (baux * 1.0)
+    delta = 1.0 if rtype == 1 else 0.0
+    if smeavg is None:
+        smeavg = avgsme(E)
+    smef = smeavg + sme
+    el = np.asarray(elux, dtype=np.float64)[:, None]
+    eaux = ((el - E[None, :]) ** 2) + (smef ** 2)
+    raux = f[None, :] * ((E[None, :] - el) / eaux)
+    iaux = f[None, :] * (smef / eaux)
+    rpart = raux.sum(axis=1)
+    ipart = iaux.sum(axis=1)
+    return delta + (rpart * caux), ipart * caux
+
+
+def imagrenorm(E, omega, eps_im):
+    """optics.f90:327-390 (in place on a copy; returns the new imaginary part)."""
+    out = np.array(eps_im, dtype=np.float64, copy=True)
+    aux = E[0] - RENORM_FACTOR
+    if aux <= 0.0:
+        return out
+    ibegin = 0
+    for i in range(len(omega)):
+        if omega[i] <= aux:
+            ibegin = i
+        else:
+            break
+    if ibegin + 1 >= len(omega):
+        # the reference reads one element past the grid here; the oracle zeroes all
+        out[:] = 0.0
+        return out
+    aux3 = out[ibegin + 1]
+    out[:ibegin + 1] = 0.0
+    rest = out[ibegin + 1:] - aux3
+    out[ibegin + 1:] = np.where(rest < 0.0, 0.0, rest)
+    return out
+
+
+def omega_grid(ebse0, ebsef, numbse):
+    """bse_diel-tool.f90:137"""
+    n = int(numbse)
+    return np.array([ebse0 + ((ebsef - ebse0) * (j - 1)) / (numbse - 1.0) for j in range(1, n + 1)])
+
+
+def spectrum(E, act6, rlat, ngrid, ebse0, ebsef, numbse, sme, renorm=True):
+    """bsedielraw loop (bse_diel-tool.f90:111,135-201) without the final spinf/REAL
+    output conversion.  act6 = (6,N) in order xx,yy,zz,xy,xz,yz.  Returns
+    (omega, re (6,nw), im (6,nw)) with components in the same order."""
+    vc = vcell3D(rlat)
+    om = omega_grid(ebse0, ebsef, numbse)
+    smeavg = avgsme(E)
+    re = np.empty((6, len(om)))
+    im = np.empty((6, len(om)))
+    for c in range(6):
+        r, i = optdiel(1 if c < 3 else 0, vc, ngrid, om, E, act6[c], sme, smeavg)
+        re[c] = r
+        im[c] = imagrenorm(E, om, i) if renorm else i
+    return om, re, im
+
+
+# ---------------------------------------------------------------------------
+# the whole pipeline
+# ---------------------------------------------------------------------------
+
+DEFAULT_PARAMS = dict(nc=2, nv=2, coultype="V2DRK", ediel=(1.0, 1.0, 1.0), lc=1.0, ez=1.0,
+                      w=0.0, r0=40.0, ktol=0.001, cshift=0.05, shift=(0.0, 0.0, 0.0),
+                      ebse0=0.0, ebsef=6.0, numbse=1001, renorm=True, dtfull=True, f154=True)
+
+
+def run_pipeline(model, ngrid, **kw):
+    """bsesolver (bse_solver-tool-diel.f90:223-620) + bsedielraw (bse_diel-tool.f90)."""
+    p = dict(DEFAULT_PARAMS)
+    p.update(kw)
+    kpt = monkhorst_pack(ngrid, p["shift"], model.rlat)
+    E, U, nocp = eigsys(kpt, model)
+    eigv, vector = select_bands(E, U, nocp, p["nc"], p["nv"])
+    stt = quantumnumbers2(len(kpt), p["nc"], p["nv"], nocp, model.nw)
+    h = dipoles(kpt, model, eigv, vector, stt, p["cshift"])
+    H = bse_matrix(kpt, eigv, vector, stt, p, ngrid, model.rlat)
+    W, X = bse_eig(H)
+    act = activities(X, h)
+    if not p["dtfull"]:
+        act[3:6] = 0.0
+    Wq, actq = (f15_4(W), f15_4(act[:6])) if p["f154"] else (W, act[:6])
+    om, re, im = spectrum(Wq, actq, model.rlat, ngrid, p["ebse0"], p["ebsef"], p["numbse"],
+                          p["cshift"], p["renorm"])
+    spinf = 2.0 if model.systype.strip() == "NP" else 1.0
+    return dict(kpt=kpt, E=E, U=U, nocp=nocp, eigv=eigv, vector=vector, stt=stt, h=h, H=H,
+                W=W, X=X, act=act, Wq=Wq, actq=actq, omega=om, eps_re=re, eps_im=im, spinf=spinf)

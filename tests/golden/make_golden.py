@@ -1,0 +1,168 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_dump.x (the reference's own subroutines, built by
+oracle/Makefile from the reference sources) on seeded synthetic TB models and
+stores the full-precision intermediates it dumps as compressed .npz files:
+inputs (the TB model arrays and the run parameters) and expected outputs.
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Needs the reference build (this container only); the fixtures it writes are
+plain data and travel with the repository.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+from wantibexos_amd import tbmodel  # noqa: E402
+
+HARNESS = ROOT / "oracle" / "_ref" / "ref_dump.x"
+OUTDIR = Path(__file__).resolve().parent
+
+
+def read_dump(path):
+    """Parse the harness's binary stream: records of (tag[8], n int64, type int32, data)."""
+    raw = Path(path).read_bytes()
+    out = {}
+    pos = 0
+    while pos < len(raw):
+        tag = raw[pos:pos + 8].decode().strip()
+        n = int(np.frombuffer(raw, np.int64, 1, pos + 8)[0])
+        typ = int(np.frombuffer(raw, np.int32, 1, pos + 16)[0])
+        pos += 20
+        if typ == 1:
+            a = np.frombuffer(raw, np.float64, n, pos).copy()
+            pos += 8 * n
+        elif typ == 2:
+            a = np.frombuffer(raw, np.complex128, n, pos).copy()
+            pos += 16 * n
+        else:
+            a = np.frombuffer(raw, np.int32, n, pos).copy()
+            pos += 4 * n
+        out[tag] = a
+    return out
+
+
+def input_text(p, tbname, nthreads=4):
+    T = lambda b: "T" if b else "F"
+    lines = [
+        f"NTHREADS= {nthreads}", 'OUTPUT= "./out/"', 'CALC_DATA= "./out/"',
+        f'PARAMS_FILE= "{tbname}"',
+        f"NGX= {p['ngrid'][0]}", f"NGY= {p['ngrid'][1]}", f"NGZ= {p['ngrid'][2]}",
+        "BSE= T", "SPEC= T", f"DTDIAG= {T(p['dtfull'])}", "CPOL= T",
+        f"COULOMB_POT= {p['coultype']}",
+        f"R_0= {p['r0']!r}", f"LC= {p['lc']!r}", f"W_COUL= {p['w']!r}",
+        f"EDIEL_T= {p['ediel'][0]!r}", f"EDIEL= {p['ediel'][1]!r}", f"EDIEL_B= {p['ediel'][2]!r}",
+        f"EDIEL_Z= {p['ez']!r}",
+        f"NBANDSC= {p['nc']}", f"NBANDSV= {p['nv']}", f"KTOL= {p['ktol']!r}",
+        f"CSHIFT= {p['cshift']!r}", f"ENSPECI= {p['ebse0']!r}", f"ENSPECF= {p['ebsef']!r}",
+        f"NESPEC= {p['numbse']}", f"RNMD= {T(p['renorm'])}",
+        f"SHIFT_1= {p['shift'][0]!r}", f"SHIFT_2= {p['shift'][1]!r}", f"SHIFT_3= {p['shift'][2]!r}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+BASE = dict(nc=2, nv=2, coultype="V2DRK", ediel=(1.0, 1.0, 1.0), lc=1.0, ez=1.0, w=0.0,
+            r0=40.0, ktol=0.001, cshift=0.05, shift=(0.0, 0.0, 0.0), ebse0=0.0, ebsef=6.0,
+            numbse=1001, renorm=True, dtfull=True)
+
+
+def cases():
+    """(name, model factory, params, dump full matrix?)"""
+    out = []
+    out.append(("hex8_g3_v2drk", lambda: tbmodel.hex2d(11, 8, (3, 3, 1)), dict(BASE, ngrid=(3, 3, 1)), True))
+    out.append(("hex8_g6_v2drk", lambda: tbmodel.hex2d(12, 8, (6, 6, 1)), dict(BASE, ngrid=(6, 6, 1)), True))
+    # the small CPU-runnable configuration: nw=8, 12x12x1, N=576
+    out.append(("hex8_g12_v2drk_c1", lambda: tbmodel.hex2d(1, 8, (12, 12, 1)),
+                dict(BASE, ngrid=(12, 12, 1)), False))
+    out.append(("cub6_g3_v3dl", lambda: tbmodel.cubic3d(21, 6, (3, 3, 3)),
+                dict(BASE, ngrid=(3, 3, 3), coultype="V3DL", ediel=(1.0, 6.0, 1.0)), True))
+    # scissor, Fermi level, grid shift, NP spin factor, dtfull off, other screening
+    def hex_np():
+        m = tbmodel.hex2d(13, 6, (4, 4, 1), systype="NP")
+        m.scs = 0.25
+        m.efermi = 0.123456789
+        return m
+    out.append(("hex6_g4_np_scs_shift", hex_np,
+                dict(BASE, ngrid=(4, 4, 1), shift=(0.1, 0.2, 0.0), dtfull=False, cshift=0.08,
+                     ediel=(2.0, 5.0, 3.0), ez=2.5, w=0.7, lc=6.5, r0=25.0, nc=1, nv=2), True))
+    pots = [("v2dk", "V2DK", dict(ediel=(1.5, 4.0, 2.0), lc=6.0)),
+            ("v3d", "V3D", {}),
+            ("v2dt", "V2DT", {}),
+            ("v2dt2", "V2DT2", dict(lc=8.0)),
+            ("v0dt", "V0DT", {}),
+            ("v2doh", "V2DOH", dict(w=0.5, ez=3.0)),
+            ("v1dt", "V1DT", dict(lc=4.0))]
+    for tag, ct, extra in pots:
+        out.append((f"hex6_g3_{tag}", (lambda s=len(out): tbmodel.hex2d(100 + s, 6, (3, 3, 1))),
+                    dict(BASE, ngrid=(3, 3, 1), coultype=ct, **extra), True))
+    return out
+
+
+def run_case(name, factory, p, full):
+    model = factory()
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "out").mkdir()
+        tbmodel.write_tb(model, td / "tb.dat")
+        (td / "input.dat").write_text(input_text(p, "tb.dat"))
+        env = dict(os.environ, WTB_DUMP=str(td / "dump.bin"), WTB_DUMP_HBSE="1" if full else "0",
+                   OMP_NUM_THREADS="4")
+        r = subprocess.run([str(HARNESS)], stdin=open(td / "input.dat"), cwd=td, env=env,
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: harness failed\n{r.stdout}\n{r.stderr}")
+        d = read_dump(td / "dump.bin")
+    nk = int(np.prod(p["ngrid"]))
+    nw = model.nw
+    nb = p["nc"] + p["nv"]
+    N = nk * p["nc"] * p["nv"]
+    fx = dict(
+        model_systype=np.array(model.systype), model_scs=np.float64(model.scs),
+        model_efermi=np.float64(model.efermi), model_rlat=model.rlat, model_rvec=model.rvec,
+        model_ndeg=model.ndeg, model_hop=model.hop, model_ihop=model.ihop,
+        params=np.array(json.dumps(dict(p, ngrid=list(p["ngrid"]))), dtype=object).astype(str),
+        kpt=d["kpt"].reshape(3, nk).T,
+        eall=d["eall"].reshape(nk, nw),
+        nocpk=d["nocpk"] - 0,                                  # reference count (1-based nocp)
+        eigv=d["eigv"].reshape(nb, nk).T,
+        vector=d["vector"].reshape(nw, nb, nk).transpose(2, 1, 0),
+        egap=d["egap"],
+        stt=d["stt"].reshape(4, N).T,
+        hrx=d["hrx"], hry=d["hry"], hrz=d["hrz"], hrsp=d["hrsp"], hrsm=d["hrsm"],
+        hrow1=d["hrow1"], hrow2=d["hrow2"], hrowm=d["hrowm"],
+        wbse=d["wbse"], xvec1=d["xvec1"],
+        act=d["act"].reshape(8, N),
+        wq=d["wq"], actq=d["actq"].reshape(6, N),
+        dielq=d["dielq"].reshape(6, 3, int(p["numbse"]))[:, 1:, :],
+        dielf=d["dielf"].reshape(6, 3, int(p["numbse"]))[:, 1:, :],
+        vc3d=d["vc3d"], spinf=d["spinf"],
+    )
+    if full:
+        fx["hbse_up"] = d["hbse_up"]
+    np.savez_compressed(OUTDIR / f"{name}.npz", **fx)
+    return N
+
+
+def main():
+    if not HARNESS.exists():
+        sys.exit("build the reference harness first: make -C oracle")
+    only = set(sys.argv[1:])
+    for name, factory, p, full in cases():
+        if only and name not in only:
+            continue
+        N = run_case(name, factory, p, full)
+        print(f"{name}: N={N} -> {(OUTDIR / (name + '.npz')).stat().st_size} bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,242 @@
+"""Tight-binding model I/O and seeded synthetic models.
+
+The text format is the one `hamiltonian_input_read` parses
+(subroutines/module_input_read.f90:32-176):
+
+    systype                     (e.g. SOC, NP, 2D; "SP" = spin blocks, read only)
+    scissor
+    E_F
+    a1x a1y a1z                 (rows are lattice vectors, Angstrom)
+    a2x a2y a2z
+    a3x a3y a3z
+    comment
+    nw
+    nvec
+    ndegen(1..nvec)             (free format, any number per line)
+    R1 R2 R3 m n Re Im          (nvec*nw*nw lines; R outer, column n, row m fastest)
+
+Arrays use the reference's index order: ``hop[i, m, n]`` = Re H(R_i)_{mn}
+(`hopmatrices(i,k,j)`, module_input_read.f90:147-172), ``rvec[i]`` the
+integer lattice vector, ``rlat[i]`` the i-th lattice vector.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from pathlib import Path
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class TBModel:
+    systype: str
+    scs: float
+    efermi: float
+    rlat: np.ndarray      # (3,3) float64, row i = lattice vector a_i
+    rvec: np.ndarray      # (nvec,3) int32
+    ndeg: np.ndarray      # (nvec,) int32
+    hop: np.ndarray       # (nvec,nw,nw) float64, real part of H(R)
+    ihop: np.ndarray      # (nvec,nw,nw) float64, imaginary part
+    comment: str = "written by wantibexos_amd.tbmodel"
+
+    @property
+    def nw(self) -> int:
+        return int(self.hop.shape[1])
+
+    @property
+    def nvec(self) -> int:
+        return int(self.hop.shape[0])
+
+    def hr(self) -> np.ndarray:
+        """Complex H(R) stack, shape (nvec, nw, nw)."""
+        return self.hop + 1j * self.ihop
+
+
+def _fmt(x: float) -> str:
+    return repr(float(x))
+
+
+def write_tb(model: TBModel, path) -> None:
+    """Write `model` in the reference's TB text format (exact round trip)."""
+    lines = [model.systype, _fmt(model.scs), _fmt(model.efermi)]
+    for i in range(3):
+        lines.append(" ".join(_fmt(v) for v in model.rlat[i]))
+    lines.append(model.comment)
+    lines.append(str(model.nw))
+    lines.append(str(model.nvec))
+    nd = [str(int(v)) for v in model.ndeg]
+    for s in range(0, len(nd), 15):
+        lines.append(" ".join(nd[s:s + 15]))
+    out = ["\n".join(lines)]
+    nw = model.nw
+    rows = []
+    for i in range(model.nvec):
+        r1, r2, r3 = (int(v) for v in model.rvec[i])
+        for n in range(nw):
+            for m in range(nw):
+                rows.append(f"{r1} {r2} {r3} {m + 1} {n + 1} "
+                            f"{_fmt(model.hop[i, m, n])} {_fmt(model.ihop[i, m, n])}")
+    out.append("\n".join(rows))
+    Path(path).write_text("\n".join(out) + "\n")
+
+
+def read_tb(path) -> TBModel:
+    """Parse a TB file the way hamiltonian_input_read does (module_input_read.f90:32-176)."""
+    toks_lines = Path(path).read_text().splitlines()
+    pos = 0
+
+    def line():
+        nonlocal pos
+        s = toks_lines[pos]
+        pos += 1
+        return s
+
+    systype = line().split()[0].strip("'\"")
+    scs = float(line().split()[0])
+    efermi = float(line().split()[0])
+    rlat = np.array([[float(v) for v in line().split()[:3]] for _ in range(3)])
+
+    def read_block():
+        comment = line()
+        nw = int(line().split()[0])
+        nvec = int(line().split()[0])
+        nd = []
+        while len(nd) < nvec:
+            nd.extend(int(v) for v in line().split())
+        nd = np.array(nd[:nvec], dtype=np.int32)
+        rvec = np.zeros((nvec, 3), dtype=np.int32)
+        hop = np.zeros((nvec, nw, nw))
+        ihop = np.zeros((nvec, nw, nw))
+        body = np.array([line().split() for _ in range(nvec * nw * nw)])
+        body = body.reshape(nvec, nw, nw, 7)   # [i, n(col), m(row), field]
+        rvec[:] = body[:, 0, 0, 0:3].astype(np.int32)
+        hop[:] = body[:, :, :, 5].astype(np.float64).transpose(0, 2, 1)
+        ihop[:] = body[:, :, :, 6].astype(np.float64).transpose(0, 2, 1)
+        return comment, nd, rvec, hop, ihop
+
+    if systype == "SP":
+        # spin-up and spin-down blocks concatenated (module_input_read.f90:56-142)
+        c1, ndu, rvu, hu, ihu = read_block()
+        _, ndd, rvd, hd, ihd = read_block()
+        nwu, nwd = hu.shape[1], hd.shape[1]
+        nw = nwu + nwd
+        nvec = len(ndu) + len(ndd)
+        hop = np.zeros((nvec, nw, nw))
+        ihop = np.zeros((nvec, nw, nw))
+        hop[:len(ndu), :nwu, :nwu] = hu
+        ihop[:len(ndu), :nwu, :nwu] = ihu
+        hop[len(ndu):, nwu:, nwu:] = hd
+        ihop[len(ndu):, nwu:, nwu:] = ihd
+        return TBModel(systype, scs, efermi, rlat, np.vstack([rvu, rvd]),
+                       np.concatenate([ndu, ndd]), hop, ihop, c1)
+    comment, nd, rvec, hop, ihop = read_block()
+    return TBModel(systype, scs, efermi, rlat, rvec, nd, hop, ihop, comment)
+
+
+# ---------------------------------------------------------------------------
+# seeded synthetic models
+# ---------------------------------------------------------------------------
+
+def _rset(rlat, cutoff, nmax, three_d):
+    out = []
+    rng3 = range(-nmax, nmax + 1) if three_d else [0]
+    for n1 in range(-nmax, nmax + 1):
+        for n2 in range(-nmax, nmax + 1):
+            for n3 in rng3:
+                v = n1 * rlat[0] + n2 * rlat[1] + n3 * rlat[2]
+                if np.linalg.norm(v) <= cutoff * (1 + 1e-4):
+                    out.append((n1, n2, n3))
+    out.sort(key=lambda t: (np.linalg.norm(t[0] * rlat[0] + t[1] * rlat[1] + t[2] * rlat[2]), t))
+    return np.array(out, dtype=np.int32)
+
+
+def _random_hr(rng, rvec, rlat, nw, decay):
+    """H(R) = (N(0,1)+iN(0,1))*exp(-|R|/decay), with H(-R) = H(R)^dagger."""
+    nvec = len(rvec)
+    index = {tuple(r): i for i, r in enumerate(rvec)}
+    hr = np.zeros((nvec, nw, nw), dtype=np.complex128)
+    done = set()
+    for i, r in enumerate(rvec):
+        key = tuple(int(v) for v in r)
+        if key in done:
+            continue
+        neg = tuple(-v for v in key)
+        dist = float(np.linalg.norm(r @ rlat))
+        blk = (rng.standard_normal((nw, nw)) + 1j * rng.standard_normal((nw, nw))) * math.exp(-dist / decay)
+        if key == neg:
+            hr[i] = 0.5 * (blk + blk.conj().T)
+            done.add(key)
+        else:
+            hr[i] = blk
+            hr[index[neg]] = blk.conj().T
+            done.add(key)
+            done.add(neg)
+    return hr
+
+
+def _bands_on_grid(rlat, rvec, hr, ngrid):
+    from numpy.linalg import eigvalsh
+    b = 2 * np.pi * np.linalg.inv(rlat).T          # rows b_i (exact pi is fine here)
+    n1, n2, n3 = ngrid
+    ks = np.array([(i / n1) * b[0] + (j / n2) * b[1] + (l / n3) * b[2]
+                   for i in range(n1) for j in range(n2) for l in range(n3)])
+    rc = rvec @ rlat
+    ph = np.exp(1j * ks @ rc.T)                     # (nk, nvec)
+    hk = np.einsum("kr,rmn->kmn", ph, hr)
+    return eigvalsh(hk)
+
+
+def _place_gap(rlat, rvec, hr, ngrid, nw, gap, rng_max=60.0):
+    """Split on-site energies by +-E0 (first half of orbitals down, second half up),
+    choose E0 so that the indirect gap on `ngrid` is `gap` eV, and move mid-gap to 0."""
+    i0 = int(np.where((rvec == 0).all(axis=1))[0][0])
+    split = np.concatenate([-np.ones(nw // 2), np.ones(nw - nw // 2)])
+    half = nw // 2
+
+    def gap_of(e0):
+        h = hr.copy()
+        h[i0] += np.diag(e0 * split)
+        ev = _bands_on_grid(rlat, rvec, h, ngrid)
+        return ev[:, half].min() - ev[:, half - 1].max(), ev
+
+    lo, hi = 0.0, rng_max
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        g, _ = gap_of(mid)
+        if g < gap:
+            lo = mid
+        else:
+            hi = mid
+    g, ev = gap_of(hi)
+    mid_gap = 0.5 * (ev[:, half].min() + ev[:, half - 1].max())
+    hr = hr.copy()
+    hr[i0] += np.diag(hi * split) - mid_gap * np.eye(nw)
+    return hr
+
+
+def hex2d(seed: int, nw: int, ngrid=(12, 12, 1), a: float = 3.2, vacuum: float = 20.0,
+          rcut_cells: float = 3.0, decay: float = 1.2, gap: float = 2.0,
+          systype: str = "SOC") -> TBModel:
+    """Seeded random 2D hexagonal Wannier model (37 R vectors for rcut = 3a)."""
+    rng = np.random.default_rng(seed)
+    rlat = np.array([[a, 0.0, 0.0], [-0.5 * a, round(a * math.sqrt(3) / 2, 4), 0.0], [0.0, 0.0, vacuum]])
+    rvec = _rset(rlat, rcut_cells * a, int(math.ceil(rcut_cells)) + 2, False)
+    hr = _random_hr(rng, rvec, rlat, nw, decay)
+    hr = _place_gap(rlat, rvec, hr, ngrid, nw, gap)
+    return TBModel(systype, 0.0, 0.0, rlat, rvec, np.ones(len(rvec), np.int32),
+                   np.ascontiguousarray(hr.real), np.ascontiguousarray(hr.imag),
+                   f"hex2d seed={seed} nw={nw}")
+
+
+def cubic3d(seed: int, nw: int, ngrid=(4, 4, 4), a: float = 5.4, rcut_cells: float = 2.0,
+            decay: float = 1.2, gap: float = 2.0, systype: str = "SOC") -> TBModel:
+    """Seeded random simple-cubic Wannier model (33 R vectors for rcut = 2a)."""
+    rng = np.random.default_rng(seed)
+    rlat = np.eye(3) * a
+    rvec = _rset(rlat, rcut_cells * a, int(math.ceil(rcut_cells)) + 1, True)
+    hr = _random_hr(rng, rvec, rlat, nw, decay)
+    hr = _place_gap(rlat, rvec, hr, ngrid, nw, gap)
+    return TBModel(systype, 0.0, 0.0, rlat, rvec, np.ones(len(rvec), np.int32),
+                   np.ascontiguousarray(hr.real), np.ascontiguousarray(hr.imag),
+                   f"cubic3d seed={seed} nw={nw}")

@@ -1,0 +1,196 @@
+/*
+ * wtb_bse.h -- C-ABI of the MI355X (gfx950) BSE pipeline.
+ *
+ * Drop-in boundary for the WanTiBEXOS Bethe-Salpeter path.  Every entry point
+ * replaces one reference subroutine (or loop of calls) and is callable from the
+ * reference's Fortran drivers through ISO_C_BINDING (see
+ * wantibexos_amd/fortran/wtb_gpu.f90) or from C / ctypes.
+ *
+ * Conventions
+ *  - Arrays are passed in the reference's Fortran (column-major) order and
+ *    index meaning, unchanged.  Complex arrays are interleaved (re, im) doubles
+ *    (COMPLEX*16).  Layout changes happen on the device.
+ *  - Flags are plain int (never Fortran LOGICAL).
+ *  - Every call returns a status (WTB_OK = 0); wtb_last_error() gives the text.
+ *    Status codes map onto the reference's STOP messages.
+ *  - A context owns one HIP stream on one device; calls on a context are not
+ *    re-entrant.  Host arrays are borrowed for the duration of the call only.
+ *  - All work runs on the GPU.  There is no CPU fallback: without a usable
+ *    gfx950 device wtb_create fails.
+ */
+#ifndef WTB_BSE_H
+#define WTB_BSE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  WTB_OK = 0,
+  WTB_ERR_ARG = 1,       /* invalid argument (size, pointer, enum)                          */
+  WTB_ERR_HIP = 2,       /* HIP runtime error                                               */
+  WTB_ERR_NOMEM = 3,     /* device allocation failed                                        */
+  WTB_ERR_COULOMB = 4,   /* "Wrong Coulomb Potential"            bse_subs.f90:88-91         */
+  WTB_ERR_NVAL = 5,      /* "Number of valence states higher then ocuppied states"
+                            general_subs.f90:521-524                                        */
+  WTB_ERR_NCOND = 6,     /* "Number of conduction states higher then unocuppied states"
+                            general_subs.f90:525-529                                        */
+  WTB_ERR_NOCP = 7,      /* no band above E_F at some k: nocp undefined in the reference
+                            (hamiltonian_tb.f90:112-127)                                    */
+  WTB_ERR_EIG = 8,       /* "The algorithm failed to compute eigenvalues."
+                            hamiltonian_tb.f90:98-101, bse_solver-tool-diel.f90:489-492     */
+  WTB_ERR_LIMIT = 9,     /* a documented size limit is exceeded (nw > 64, ...)              */
+  WTB_ERR_NONFINITE = 10 /* NaN or Inf in the input                                         */
+};
+
+/* ---- COULOMB_POT selector (bse_subs.f90:45-91) ------------------------------ */
+enum {
+  WTB_V2DK = 1, WTB_V3D = 2, WTB_V3DL = 3, WTB_V2DT = 4, WTB_V2DT2 = 5,
+  WTB_V2DOH = 6, WTB_V2DRK = 7, WTB_V1DT = 8, WTB_V0DT = 9
+};
+
+/* ---- arithmetic forms ------------------------------------------------------- */
+enum {
+  WTB_FP64 = 0,   /* plain: everything in FP64 (drop-in for the reference)            */
+  WTB_MIXED = 1   /* precision form: FP32 assembly / tridiagonal reduction, FP64
+                     tridiagonal solve and FP64 refinement of the lowest excitons    */
+};
+
+typedef struct wtb_ctx wtb_ctx;
+typedef struct wtb_bse_plan wtb_bse_plan;
+
+/* Tight-binding model as hamiltonian_input_read leaves it in the module
+ * hamiltonian_input_variables (subroutines/module_input_read.f90:1-30,32-176). */
+typedef struct {
+  int nw;                /* w90basis                                                  */
+  int nvec;              /* number of R vectors                                       */
+  const double* rlat;    /* rlat(3,3): rlat[i + 3*c] = component c of lattice vector i */
+  const int* rvec;       /* rvec(nvec,3): rvec[i + nvec*c]                              */
+  const int* ffactor;    /* ndegen(nvec)                                              */
+  const double* hop;     /* hopmatrices(nvec,nw,nw): hop[i + nvec*(m + nw*n)] = Re H(R_i)_mn */
+  const double* ihop;    /* ihopmatrices(nvec,nw,nw), imaginary parts                 */
+  double efermi;         /* E_F (the reference subtracts its float32 rounding)          */
+  double scs;            /* scissor added to the bands above E_F                      */
+} wtb_tb_model;
+
+/* The bsesolver / bsedielraw arguments that change the arithmetic
+ * (bse_solver-tool-diel.f90:2-5, bse_diel-tool.f90:5, input keys of
+ * module_input_read.f90:364-658). */
+typedef struct {
+  int ngrid[3];          /* NGX NGY NGZ                                               */
+  double mshift[3];      /* SHIFT_1..3                                                */
+  int nc, nv;            /* NBANDSC NBANDSV                                           */
+  int coultype;          /* COULOMB_POT (WTB_V2DRK, ...)                              */
+  double ediel[3];       /* EDIEL_T EDIEL EDIEL_B                                     */
+  double lc, ez, w, r0;  /* LC EDIEL_Z W_COUL R_0                                     */
+  double ktol;           /* KTOL                                                      */
+  double cshift;         /* CSHIFT (passed in bsesolver's sme slot and to bsedielraw)  */
+  double ebse0, ebsef;   /* ENSPECI ENSPECF                                           */
+  int numbse;            /* NESPEC                                                    */
+  int dtfull;            /* DTDIAG: xy,xz,yz activities (else zero)                   */
+  int cpol;              /* CPOL: sp,sm activities (else zero)                        */
+  int renorm;            /* RNMD: imagrenorm on eps2                                  */
+  int f154;              /* 1: round energies/activities as the bse_opt_diel.dat
+                            "(7F15.4)" write/read does before eps(omega)              */
+  int precision;         /* WTB_FP64 or WTB_MIXED                                     */
+} wtb_bse_params;
+
+/* ---- context ---------------------------------------------------------------- */
+int wtb_create(int device, wtb_ctx** ctx);
+int wtb_destroy(wtb_ctx* ctx);
+const char* wtb_last_error(void);
+int wtb_version(void);
+void* wtb_stream(wtb_ctx* ctx);             /* the context's hipStream_t             */
+
+/* ---- single stages ---------------------------------------------------------- */
+
+/* monhkhorst_pack (subroutines/mhkpack_subs.f90:359-407, recvec :409-432):
+ * kpt(nk,3), nk = ngrid[0]*ngrid[1]*ngrid[2]; bit-identical to the reference. */
+int wtb_monkhorst_pack(wtb_ctx* ctx, const int ngrid[3], const double shift[3],
+                       const double* rlat, double* kpt);
+
+/* eigsys (subroutines/hamiltonian_tb.f90:2-137) for nk k-points in one launch.
+ * energias(nw,nk): ascending band energies, scissor applied above E_F.
+ * autovetores(nw,nw,nk) complex: autovetores(i,:,k) = eigenvector i (the
+ * reference's transposed layout), gauge of ZHEEV('V','L'): component 1 real.
+ * nocp(nk): number of bands with E <= 0.  Limit: nw <= 64. */
+int wtb_eigsys(wtb_ctx* ctx, const wtb_tb_model* model, int nk, const double* kpt,
+               double* energias, double* autovetores, int* nocp);
+
+/* ZHEEV('N' or 'V', 'U', n, a, lda, w) for the BSE matrix
+ * (subprograms/bse_solver-tool-diel.f90:485-492).  The upper triangle of
+ * a(lda,n) is read.  want_vectors = 0: eigenvalues only.  want_vectors = 1:
+ * a is overwritten with the orthonormal eigenvectors (column j <-> w[j]). */
+int wtb_zheev(wtb_ctx* ctx, int want_vectors, int n, double* a, int lda, double* w);
+
+/* dielbsev (subroutines/bse_subs.f90:336-378): act(i) = |sum_j x(j,i) h(j)|^2 */
+int wtb_dielbsev(wtb_ctx* ctx, int n, const double* x, const double* h, double* act);
+/* dielbsep (bse_subs.f90:278-332): act(i) = Re[(x^T h1)_i conj((x^T h2)_i)]     */
+int wtb_dielbsep(wtb_ctx* ctx, int n, const double* x, const double* h1, const double* h2,
+                 double* act);
+
+/* The bsedielraw loop (subprograms/bse_diel-tool.f90:111,135-201): optdiel
+ * (optics.f90:160-244) with avgsme (:302-324) for xx,yy,zz (delta = 1) and
+ * xy,xz,yz (delta = 0) on numbse photon energies, then imagrenorm
+ * (optics.f90:327-390) if renorm.  e(n), act6(n,6) in order xx,yy,zz,xy,xz,yz.
+ * eps(numbse,2,6): eps[j + numbse*(part + 2*comp)], part 0 = Re, 1 = Im,
+ * without the spin factor and float32 conversion of the file write. */
+int wtb_bse_spectrum(wtb_ctx* ctx, int n, const double* e, const double* act6,
+                     const double* rlat, const int ngrid[3], double ebse0, double ebsef,
+                     int numbse, double sme, int renorm, double* eps);
+
+/* ---- the whole pipeline -------------------------------------------------------
+ * bsesolver (subprograms/bse_solver-tool-diel.f90:2-660) followed by the
+ * spectrum of bsedielraw (subprograms/bse_diel-tool.f90:5-245), on the GPU:
+ * k grid, H(k) phase GEMM, per-k Jacobi eigensolves, dipoles, W(k-k') table,
+ * BSE assembly, Hermitian eigensolve, activities, eps(omega).
+ *   w(N)           exciton energies, ascending (N = nk*nc*nv)
+ *   act(N,8)       xx,yy,zz,xy,xz,yz,sp,sm activities (zero where the reference
+ *                  zeroes them: dtfull/cpol off)
+ *   eps(numbse,2,6) as wtb_bse_spectrum (may be NULL)
+ *   egap           direct gap logged by the reference (:305-317) (may be NULL)  */
+int wtb_bsesolver(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm,
+                  double* w, double* act, double* eps, double* egap);
+
+/* Plan form of wtb_bsesolver for repeated runs: create uploads the model and
+ * allocates every device buffer; execute enqueues the whole pipeline on the
+ * context stream (inputs resident in HBM, no host synchronisation, no
+ * allocation); results synchronises and copies out. */
+int wtb_bse_plan_create(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm,
+                        wtb_bse_plan** plan);
+int wtb_bse_plan_dim(const wtb_bse_plan* plan);
+int wtb_bse_plan_execute(wtb_bse_plan* plan);
+int wtb_bse_plan_results(wtb_bse_plan* plan, double* w, double* act, double* eps,
+                         double* egap);
+/* Copy an intermediate device buffer to the host (tests, diagnostics).
+ * what: see WTB_BUF_*; bytes must equal the buffer size (wtb_bse_plan_buffer_bytes). */
+enum {
+  WTB_BUF_KPT = 0,       /* kpt(nk,3) double                                          */
+  WTB_BUF_EALL = 1,      /* energias per k: [k][nw] double                            */
+  WTB_BUF_NOCP = 2,      /* nocp [k] int                                              */
+  WTB_BUF_EIGV = 3,      /* selected bands [k][nc+nv] double                          */
+  WTB_BUF_VECTOR = 4,    /* selected vectors [k][nc+nv][nw] complex                   */
+  WTB_BUF_DIPOLE = 5,    /* hrx,hry,hrz [3][N] complex                                */
+  WTB_BUF_HBSE = 6,      /* BSE matrix, column-major N x N complex (before the solve) */
+  WTB_BUF_WTAB = 7,      /* W(Delta) table, [D1][D2][D3] double                       */
+  WTB_BUF_TRIDIAG = 8,   /* d[N] then e[N] double (after the reduction)                */
+  WTB_BUF_NBUF = 9
+};
+size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* plan, int what);
+int wtb_bse_plan_fetch(wtb_bse_plan* plan, int what, void* dst, size_t bytes);
+/* Per-stage device time of the last execute (ms), measured with HIP events on
+ * the context stream when timing is on: [0] k grid + H(k) + per-k eig,
+ * [1] dipoles, [2] W table + assembly, [3] tridiagonal reduction,
+ * [4] tridiagonal eigensolve, [5] activities + eps(omega). */
+enum { WTB_NSTAGES = 6 };
+int wtb_bse_plan_set_timing(wtb_bse_plan* plan, int on);
+int wtb_bse_plan_stage_ms(const wtb_bse_plan* plan, double* ms);
+int wtb_bse_plan_destroy(wtb_bse_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WTB_BSE_H */

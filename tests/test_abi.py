@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every entry point include/wtb_bse.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "wtb_bse.h"
+LIB = ROOT / "wantibexos_amd" / "libwtb_bse.so"
+
+
+def declared():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wtb_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for n in ["wtb_create", "wtb_eigsys", "wtb_zheev", "wtb_bsesolver", "wtb_bse_spectrum",
+              "wtb_dielbsev", "wtb_dielbsep", "wtb_monkhorst_pack"]:
+        assert n in names
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built (python -m wantibexos_amd.build)")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(LIB))
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_no_device_fails_loudly():
+    """Without a usable gfx950 device the library refuses to create a context."""
+    from wantibexos_amd import bse
+    try:
+        c = bse.Context(0)
+    except bse.WTBError:
+        return
+    c.close()
+    pytest.skip("a GPU is present")

@@ -1,0 +1,345 @@
+"""Python host interface over the C-ABI of libwtb_bse.so (include/wtb_bse.h).
+
+The functions mirror the reference subroutines they replace (same names,
+argument meaning, and STOP conditions raised as exceptions):
+
+    monkhorst_pack   subroutines/mhkpack_subs.f90:359-407
+    eigsys           subroutines/hamiltonian_tb.f90:2-137 (all k at once)
+    zheev            ZHEEV('N','U') of subprograms/bse_solver-tool-diel.f90:485-492
+    dielbsev/p       subroutines/bse_subs.f90:278-378
+    bse_spectrum     the bsedielraw loop, subprograms/bse_diel-tool.f90:135-201
+    bsesolver        subprograms/bse_solver-tool-diel.f90:2-660 + spectrum
+
+Everything runs on the GPU; the library refuses to start without a gfx950
+device, and importing this module without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+
+from .tbmodel import TBModel
+
+_LIB_PATH = Path(__file__).resolve().parent / "libwtb_bse.so"
+
+WTB_OK = 0
+ERRORS = {
+    1: "invalid argument", 2: "HIP runtime error", 3: "device allocation failed",
+    4: "Wrong Coulomb Potential",
+    5: "Number of valence states higher then ocuppied states",
+    6: "Number of conduction states higher then unocuppied states",
+    7: "no band above the Fermi level at some k",
+    8: "The algorithm failed to compute eigenvalues.",
+    9: "size limit exceeded", 10: "NaN or Inf in the input",
+}
+COULOMB = {"V2DK": 1, "V3D": 2, "V3DL": 3, "V2DT": 4, "V2DT2": 5, "V2DOH": 6, "V2DRK": 7,
+           "V1DT": 8, "V0DT": 9}
+FP64, MIXED = 0, 1
+BUF = dict(kpt=0, eall=1, nocp=2, eigv=3, vector=4, dipole=5, hbse=6, wtab=7, tridiag=8)
+NSTAGES = 6
+STAGES = ["eig", "dipoles", "build", "tridiag", "dc", "spectrum"]
+
+
+class WTBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[wtb {code}] {msg}")
+        self.code = code
+
+
+class _Model(C.Structure):
+    _fields_ = [("nw", C.c_int), ("nvec", C.c_int), ("rlat", C.c_void_p), ("rvec", C.c_void_p),
+                ("ffactor", C.c_void_p), ("hop", C.c_void_p), ("ihop", C.c_void_p),
+                ("efermi", C.c_double), ("scs", C.c_double)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("ngrid", C.c_int * 3), ("mshift", C.c_double * 3), ("nc", C.c_int), ("nv", C.c_int),
+                ("coultype", C.c_int), ("ediel", C.c_double * 3), ("lc", C.c_double),
+                ("ez", C.c_double), ("w", C.c_double), ("r0", C.c_double), ("ktol", C.c_double),
+                ("cshift", C.c_double), ("ebse0", C.c_double), ("ebsef", C.c_double),
+                ("numbse", C.c_int), ("dtfull", C.c_int), ("cpol", C.c_int), ("renorm", C.c_int),
+                ("f154", C.c_int), ("precision", C.c_int)]
+
+
+def _load():
+    if not _LIB_PATH.exists():
+        raise ImportError(f"{_LIB_PATH} is not built: run python -m wantibexos_amd.build "
+                          "(there is no CPU fallback)")
+    lib = C.CDLL(str(_LIB_PATH))
+    vp, ip, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
+    sig = {
+        "wtb_create": (ip, [ip, C.POINTER(vp)]),
+        "wtb_destroy": (ip, [vp]),
+        "wtb_last_error": (C.c_char_p, []),
+        "wtb_version": (ip, []),
+        "wtb_stream": (vp, [vp]),
+        "wtb_monkhorst_pack": (ip, [vp, vp, vp, vp, vp]),
+        "wtb_eigsys": (ip, [vp, vp, ip, vp, vp, vp, vp]),
+        "wtb_zheev": (ip, [vp, ip, ip, vp, ip, vp]),
+        "wtb_dielbsev": (ip, [vp, ip, vp, vp, vp]),
+        "wtb_dielbsep": (ip, [vp, ip, vp, vp, vp, vp]),
+        "wtb_bse_spectrum": (ip, [vp, ip, vp, vp, vp, vp, C.c_double, C.c_double, ip, C.c_double,
+                                  ip, vp]),
+        "wtb_bsesolver": (ip, [vp, vp, vp, vp, vp, vp, vp]),
+        "wtb_bse_plan_create": (ip, [vp, vp, vp, C.POINTER(vp)]),
+        "wtb_bse_plan_dim": (ip, [vp]),
+        "wtb_bse_plan_execute": (ip, [vp]),
+        "wtb_bse_plan_results": (ip, [vp, vp, vp, vp, vp]),
+        "wtb_bse_plan_buffer_bytes": (C.c_size_t, [vp, ip]),
+        "wtb_bse_plan_fetch": (ip, [vp, ip, vp, C.c_size_t]),
+        "wtb_bse_plan_set_timing": (ip, [vp, ip]),
+        "wtb_bse_plan_stage_ms": (ip, [vp, vp]),
+        "wtb_bse_plan_destroy": (ip, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _check(rc):
+    if rc != WTB_OK:
+        msg = lib().wtb_last_error().decode(errors="replace") or ERRORS.get(rc, "error")
+        raise WTBError(rc, msg)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclasses.dataclass
+class BSEParams:
+    """bsesolver/bsedielraw arguments (input keys of module_input_read.f90:364-658)."""
+    ngrid: tuple = (1, 1, 1)           # NGX NGY NGZ
+    mshift: tuple = (0.0, 0.0, 0.0)    # SHIFT_1..3
+    nc: int = 1                        # NBANDSC
+    nv: int = 1                        # NBANDSV
+    coultype: str = "V3D"              # COULOMB_POT
+    ediel: tuple = (1.0, 1.0, 1.0)     # EDIEL_T EDIEL EDIEL_B
+    lc: float = 1.0                    # LC
+    ez: float = 1.0                    # EDIEL_Z
+    w: float = 0.0                     # W_COUL
+    r0: float = 1.0                    # R_0
+    ktol: float = float(np.float32(0.001))   # KTOL default is the REAL literal 0.001
+    cshift: float = float(np.float32(0.01))  # CSHIFT default 0.01 (REAL literal)
+    ebse0: float = 0.0                 # ENSPECI
+    ebsef: float = 3.0                 # ENSPECF
+    numbse: int = 6001                 # NESPEC
+    dtfull: bool = False               # DTDIAG
+    cpol: bool = False                 # CPOL
+    renorm: bool = True                # RNMD
+    f154: bool = True                  # bse_opt_diel.dat F15.4 round trip
+    precision: int = FP64
+
+    @property
+    def nk(self):
+        return int(np.prod(self.ngrid))
+
+    @property
+    def dim(self):
+        return self.nk * self.nc * self.nv
+
+    def to_c(self) -> _Params:
+        ct = self.coultype.strip().upper()
+        p = _Params()
+        p.ngrid[:] = [int(v) for v in self.ngrid]
+        p.mshift[:] = [float(v) for v in self.mshift]
+        p.nc, p.nv = int(self.nc), int(self.nv)
+        p.coultype = COULOMB.get(ct, -1)
+        p.ediel[:] = [float(v) for v in self.ediel]
+        p.lc, p.ez, p.w, p.r0, p.ktol = map(float, (self.lc, self.ez, self.w, self.r0, self.ktol))
+        p.cshift, p.ebse0, p.ebsef = map(float, (self.cshift, self.ebse0, self.ebsef))
+        p.numbse = int(self.numbse)
+        p.dtfull, p.cpol, p.renorm, p.f154 = (int(bool(v)) for v in
+                                              (self.dtfull, self.cpol, self.renorm, self.f154))
+        p.precision = int(self.precision)
+        return p
+
+
+class _CModel:
+    """Keeps Fortran-ordered copies of a TBModel alive for one call."""
+
+    def __init__(self, m: TBModel):
+        self.rlat = np.asfortranarray(np.asarray(m.rlat, dtype=np.float64))
+        self.rvec = np.asfortranarray(np.asarray(m.rvec, dtype=np.int32))
+        self.ff = np.ascontiguousarray(np.asarray(m.ndeg, dtype=np.int32))
+        self.hop = np.asfortranarray(np.asarray(m.hop, dtype=np.float64))
+        self.ihop = np.asfortranarray(np.asarray(m.ihop, dtype=np.float64))
+        self.s = _Model(m.nw, m.nvec, _p(self.rlat), _p(self.rvec), _p(self.ff), _p(self.hop),
+                        _p(self.ihop), float(m.efermi), float(m.scs))
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().wtb_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().wtb_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return lib().wtb_stream(self.h)
+
+
+def monkhorst_pack(ctx: Context, ngrid, shift, rlat):
+    ng = np.array(ngrid, dtype=np.int32)
+    sh = np.array(shift, dtype=np.float64)
+    rl = np.asfortranarray(np.asarray(rlat, dtype=np.float64))
+    kpt = np.zeros((int(np.prod(ng)), 3), dtype=np.float64, order="F")
+    _check(lib().wtb_monkhorst_pack(ctx.h, _p(ng), _p(sh), _p(rl), _p(kpt)))
+    return np.ascontiguousarray(kpt)
+
+
+def eigsys(ctx: Context, model: TBModel, kpt):
+    """Returns (E (nk,nw) ascending incl. scissor, U (nk,nw,nw) with U[k,i,:] =
+    eigenvector i (the reference's autovetores(i,:)), nocp (nk,))."""
+    cm = _CModel(model)
+    k = np.asfortranarray(np.asarray(kpt, dtype=np.float64))
+    nk, nw = k.shape[0], model.nw
+    E = np.zeros((nk, nw))
+    U = np.zeros((nk, nw, nw), dtype=np.complex128)     # [k][m][i] in C order = autovetores(i,m,k)
+    nocp = np.zeros(nk, dtype=np.int32)
+    _check(lib().wtb_eigsys(ctx.h, C.byref(cm.s), nk, _p(k), _p(E), _p(U), _p(nocp)))
+    return E, np.ascontiguousarray(U.transpose(0, 2, 1)), nocp
+
+
+def zheev(ctx: Context, a):
+    """Eigenvalues of the Hermitian matrix whose upper triangle is `a` (ZHEEV('N','U'))."""
+    a = np.asfortranarray(np.asarray(a, dtype=np.complex128))
+    n = a.shape[0]
+    w = np.zeros(n)
+    _check(lib().wtb_zheev(ctx.h, 0, n, _p(a), n, _p(w)))
+    return w
+
+
+def dielbsev(ctx: Context, x, h):
+    x = np.asfortranarray(np.asarray(x, dtype=np.complex128))
+    h = np.ascontiguousarray(h, dtype=np.complex128)
+    act = np.zeros(x.shape[0])
+    _check(lib().wtb_dielbsev(ctx.h, x.shape[0], _p(x), _p(h), _p(act)))
+    return act
+
+
+def dielbsep(ctx: Context, x, h1, h2):
+    x = np.asfortranarray(np.asarray(x, dtype=np.complex128))
+    h1 = np.ascontiguousarray(h1, dtype=np.complex128)
+    h2 = np.ascontiguousarray(h2, dtype=np.complex128)
+    act = np.zeros(x.shape[0])
+    _check(lib().wtb_dielbsep(ctx.h, x.shape[0], _p(x), _p(h1), _p(h2), _p(act)))
+    return act
+
+
+def bse_spectrum(ctx: Context, e, act6, rlat, ngrid, ebse0, ebsef, numbse, sme, renorm=True):
+    """eps (6 components xx,yy,zz,xy,xz,yz; 2 parts re,im; numbse)."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    a6 = np.asfortranarray(np.asarray(act6, dtype=np.float64).T)   # act6(n,6)
+    rl = np.asfortranarray(np.asarray(rlat, dtype=np.float64))
+    ng = np.array(ngrid, dtype=np.int32)
+    eps = np.zeros((6, 2, int(numbse)))
+    _check(lib().wtb_bse_spectrum(ctx.h, len(e), _p(e), _p(a6), _p(rl), _p(ng), float(ebse0),
+                                  float(ebsef), int(numbse), float(sme), int(bool(renorm)), _p(eps)))
+    return eps
+
+
+class BSEPlan:
+    """Device-resident pipeline: create once, execute many times."""
+
+    def __init__(self, ctx: Context, model: TBModel, params: BSEParams):
+        self.ctx = ctx
+        self.params = params
+        self._cm = _CModel(model)
+        self._cp = params.to_c()
+        h = C.c_void_p()
+        _check(lib().wtb_bse_plan_create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), C.byref(h)))
+        self.h = h
+        self.N = lib().wtb_bse_plan_dim(h)
+
+    def execute(self):
+        _check(lib().wtb_bse_plan_execute(self.h))
+
+    def set_timing(self, on=True):
+        _check(lib().wtb_bse_plan_set_timing(self.h, int(on)))
+
+    def results(self):
+        N, nb = self.N, int(self.params.numbse)
+        W = np.zeros(N)
+        act = np.zeros((8, N))
+        eps = np.zeros((6, 2, nb))
+        egap = np.zeros(1)
+        _check(lib().wtb_bse_plan_results(self.h, _p(W), _p(act), _p(eps), _p(egap)))
+        return dict(W=W, act=act, eps=eps, egap=float(egap[0]))
+
+    def stage_ms(self):
+        ms = np.zeros(NSTAGES)
+        _check(lib().wtb_bse_plan_stage_ms(self.h, _p(ms)))
+        return dict(zip(STAGES, ms))
+
+    def fetch(self, what: str):
+        code = BUF[what]
+        nbytes = lib().wtb_bse_plan_buffer_bytes(self.h, code)
+        raw = np.zeros(nbytes, dtype=np.uint8)
+        _check(lib().wtb_bse_plan_fetch(self.h, code, _p(raw), nbytes))
+        p, nk, N = self.params, self.params.nk, self.N
+        nb = p.nc + p.nv
+        if what == "kpt":
+            return raw.view(np.float64).reshape(3, nk).T.copy()
+        if what == "eall":
+            return raw.view(np.float64).reshape(nk, -1)
+        if what == "nocp":
+            return raw.view(np.int32).copy()
+        if what == "eigv":
+            return raw.view(np.float64).reshape(nk, nb)
+        if what == "vector":
+            return raw.view(np.complex128).reshape(nk, nb, -1)
+        if what == "dipole":
+            return raw.view(np.complex128).reshape(3, N)
+        if what == "hbse":
+            return raw.view(np.complex128).reshape(N, N).T     # column-major N x N (lower valid)
+        if what == "wtab":
+            return raw.view(np.float64).reshape(2 * p.ngrid[0] - 1, 2 * p.ngrid[1] - 1, 2 * p.ngrid[2] - 1)
+        if what == "tridiag":
+            return raw.view(np.float64).reshape(2, N)
+        return raw
+
+    def close(self):
+        if self.h:
+            lib().wtb_bse_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def bsesolver(ctx: Context, model: TBModel, params: BSEParams):
+    """The whole BSE pipeline (bsesolver + the bsedielraw spectrum) on the GPU.
+    Returns W (N,), act (8,N) [xx,yy,zz,xy,xz,yz,sp,sm], eps (6,2,numbse), egap."""
+    with BSEPlan(ctx, model, params) as plan:
+        plan.execute()
+        return plan.results()

@@ -1,0 +1,493 @@
+// Stage (d2): eigenvalues of the real symmetric tridiagonal (d, e) by
+// divide and conquer, carrying Z^T x for a few tracked vectors instead of the
+// eigenvector matrix Z.  The algorithm is the one of LAPACK DSTEDC/DLAED0-4
+// (rank-one tearing, deflation of small z components and of close poles by
+// Givens rotations, secular equation, Gu-Eisenstat recomputation of z):
+//
+//   T = diag(T1, T2) + rho u u^T,  Ti = Zi Li Zi^T,  z = [last row Z1; +-first row Z2]
+//   D + rho z z^T = U L U^T  =>  Z = diag(Z1, Z2) U
+//
+// Each node keeps, for its index range, its eigenvalues (ascending), the first
+// and last rows of its Z (needed by its parent's z) and Z^T x for the tracked
+// vectors x.  The parent applies U^T to [first1; 0], [0; last2], [x1; x2],
+// which costs O(n^2) per merge, so the whole solve is O(N^2).
+//
+// Leaves are single elements.  All nodes of one tree level are processed by the
+// same kernel launches (grid.y = node).  The secular equation is solved by
+// bisection over the bit patterns of the root's offset from the nearer pole,
+// which reaches the last unit of that offset in at most 64 steps.
+#include "wtb_kernels.h"
+#include <algorithm>
+#include <vector>
+
+namespace wtb {
+
+struct DCNode {
+  int lo, mid, hi;
+  int pad;
+};
+
+struct DCArgs {
+  int N, nt;            // size, tracked vectors (<= 8)
+  const DCNode* nodes;  // this level
+  int nnodes;
+  const double* e;      // off-diagonal
+  double* lam;          // [N]
+  double* FR;           // [N]
+  double* LR;           // [N]
+  double* T;            // [nt][N]
+  // work
+  double* sD;           // [N] sorted poles
+  double* sz;           // [N]
+  double* sx;           // [(2+nt)][N]: xF, xL, T...
+  double* cD;           // [N] compacted: dlamda (first K), deflated after
+  double* cw;           // [N]
+  double* cx;           // [(2+nt)][N]
+  int* slot;            // [N]
+  int* org;             // [N]
+  double* tau;          // [N]
+  double* what;         // [N]
+  double* nl;           // [N] new eigenvalues (unsorted)
+  double* nx;           // [(2+nt)][N]
+  int* K;               // [nnodes]
+  double* rho;          // [nnodes]
+};
+
+__device__ __constant__ double kEps = 1.1102230246251565e-16;   // DLAMCH('E')
+
+// d_i -= |e_{i-1}| + |e_i| (every coupling is a split point when leaves are size 1);
+// leaves: lambda = d_i, first = last row = 1, T = x
+__global__ void k_dc_leaves(int N, const double* __restrict__ d, const double* __restrict__ e,
+                            double* lam, double* FR, double* LR) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double v = d[i];
+  if (i > 0) v -= fabs(e[i - 1]);
+  if (i < N - 1) v -= fabs(e[i]);
+  lam[i] = v;
+  FR[i] = 1.0;
+  LR[i] = 1.0;
+}
+
+// M1: merge the children's sorted eigenvalues, build z and the tracked sources
+__global__ void k_dc_gather(DCArgs a) {
+  const DCNode nd = a.nodes[blockIdx.y];
+  const int n = nd.hi - nd.lo, n1 = nd.mid - nd.lo;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double beta = a.e[nd.mid - 1];
+  const double sgn = beta < 0 ? -1.0 : 1.0;
+  const double isq2 = 0.70710678118654752440;
+  int src, rank;
+  double val;
+  if (i < n1) {
+    src = nd.lo + i;
+    val = a.lam[src];
+    int l = nd.mid, h = nd.hi;       // count child-2 values < val
+    while (l < h) {
+      int m = (l + h) >> 1;
+      if (a.lam[m] < val) l = m + 1; else h = m;
+    }
+    rank = i + (l - nd.mid);
+  } else {
+    src = nd.mid + (i - n1);
+    val = a.lam[src];
+    int l = nd.lo, h = nd.mid;       // count child-1 values <= val
+    while (l < h) {
+      int m = (l + h) >> 1;
+      if (a.lam[m] <= val) l = m + 1; else h = m;
+    }
+    rank = (i - n1) + (l - nd.lo);
+  }
+  const int dst = nd.lo + rank;
+  const int N = a.N;
+  a.sD[dst] = val;
+  a.sz[dst] = (i < n1 ? a.LR[src] : sgn * a.FR[src]) * isq2;
+  a.sx[dst] = (i < n1) ? a.FR[src] : 0.0;              // first row of [Z1 0; 0 Z2]
+  a.sx[(size_t)N + dst] = (i < n1) ? 0.0 : a.LR[src];  // last row
+  for (int q = 0; q < a.nt; ++q) a.sx[(size_t)(2 + q) * N + dst] = a.T[(size_t)q * N + src];
+}
+
+// M2: deflation scan (DLAED2), one thread per node.  Decides, for each sorted
+// position, its slot (non-deflated first), applies the Givens rotations of
+// close-pole deflation to the tracked sources in place, and writes the poles
+// and weights of the secular equation.
+__global__ void k_dc_deflate(DCArgs a) {
+  const int nodeid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nodeid >= a.nnodes) return;
+  const DCNode nd = a.nodes[nodeid];
+  const int lo = nd.lo, n = nd.hi - nd.lo;
+  const int N = a.N, nx = 2 + a.nt;
+  const double rho = fabs(2.0 * a.e[nd.mid - 1]);
+  double zmax = 0.0, dmax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    zmax = fmax(zmax, fabs(a.sz[lo + i]));
+    dmax = fmax(dmax, fabs(a.sD[lo + i]));
+  }
+  const double tol = 8.0 * kEps * fmax(dmax, zmax);
+  a.rho[nodeid] = rho;
+  int K = 0, nd_ = 0;
+  auto deflate = [&](int i) {
+    int s = n - 1 - nd_;
+    ++nd_;
+    a.slot[lo + i] = s;
+    a.cD[lo + s] = a.sD[lo + i];
+  };
+  auto keep = [&](int i) {
+    int s = K;
+    ++K;
+    a.slot[lo + i] = s;
+    a.cD[lo + s] = a.sD[lo + i];
+    a.cw[lo + s] = a.sz[lo + i];
+  };
+  if (rho * zmax <= tol) {
+    for (int i = 0; i < n; ++i) deflate(i);
+    a.K[nodeid] = 0;
+    return;
+  }
+  int j = 0, pj = -1;
+  for (; j < n; ++j) {
+    if (rho * fabs(a.sz[lo + j]) <= tol) deflate(j);
+    else {
+      pj = j;
+      ++j;
+      break;
+    }
+  }
+  if (pj >= 0) {
+    for (; j < n; ++j) {
+      const int nj = j;
+      double znj = a.sz[lo + nj];
+      if (rho * fabs(znj) <= tol) {
+        deflate(nj);
+        continue;
+      }
+      double s = a.sz[lo + pj], c = znj;
+      double tau = hypot(c, s);
+      double t = a.sD[lo + nj] - a.sD[lo + pj];
+      c = c / tau;
+      s = -s / tau;
+      if (fabs(t * c * s) <= tol) {
+        a.sz[lo + nj] = tau;
+        a.sz[lo + pj] = 0.0;
+        for (int q = 0; q < nx; ++q) {
+          double* x = a.sx + (size_t)q * N + lo;
+          double xp = x[pj], xn = x[nj];
+          x[pj] = c * xp + s * xn;
+          x[nj] = c * xn - s * xp;
+        }
+        double dp = a.sD[lo + pj], dn = a.sD[lo + nj];
+        double t2 = dp * c * c + dn * s * s;
+        a.sD[lo + nj] = dp * s * s + dn * c * c;
+        a.sD[lo + pj] = t2;
+        deflate(pj);
+        pj = nj;
+      } else {
+        keep(pj);
+        pj = nj;
+      }
+    }
+    keep(pj);
+  }
+  a.K[nodeid] = K;
+}
+
+// M2b: move the tracked sources to their slots
+__global__ void k_dc_compact(DCArgs a) {
+  const DCNode nd = a.nodes[blockIdx.y];
+  const int n = nd.hi - nd.lo;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = a.slot[nd.lo + i];
+  const int N = a.N;
+  for (int q = 0; q < 2 + a.nt; ++q) a.cx[(size_t)q * N + nd.lo + s] = a.sx[(size_t)q * N + nd.lo + i];
+}
+
+// secular function f(d_o + tau) = 1 + rho sum_i w_i^2 / ((d_i - d_o) - tau), summed by a
+// 16-lane group
+__device__ inline double secular16(const double* __restrict__ dl, const double* __restrict__ w, int K,
+                                   int o, double tau, double rho, int sub) {
+  const double dor = dl[o];
+  double s = 0.0;
+  for (int i = sub; i < K; i += 16) {
+    double wi = w[i];
+    s += wi * wi / ((dl[i] - dor) - tau);
+  }
+  for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+  return 1.0 + rho * s;
+}
+
+// M3: roots of the secular equation, 16 lanes per root
+__global__ __launch_bounds__(256) void k_dc_secular(DCArgs a) {
+  const int node = blockIdx.y;
+  const DCNode nd = a.nodes[node];
+  const int K = a.K[node];
+  const int jr = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
+  if (blockIdx.x * (blockDim.x >> 4) >= K) return;   // whole block idle (uniform)
+  const bool active = jr < K;
+  const int j = active ? jr : K - 1;
+  const double* dl = a.cD + nd.lo;
+  const double* w = a.cw + nd.lo;
+  const double rho = a.rho[node];
+  int o;
+  bool neg;
+  double hi;
+  if (j < K - 1) {
+    double gap = dl[j + 1] - dl[j];
+    double mid = 0.5 * gap;
+    double fm = secular16(dl, w, K, j, mid, rho, sub);
+    if (fm >= 0.0) {
+      o = j;
+      neg = false;
+      hi = mid;
+    } else {
+      o = j + 1;
+      neg = true;
+      hi = gap - mid;
+    }
+  } else {
+    double ws = 0.0;
+    for (int i = sub; i < K; i += 16) ws += w[i] * w[i];
+    for (int off = 8; off > 0; off >>= 1) ws += __shfl_xor(ws, off, 16);
+    o = K - 1;
+    neg = false;
+    hi = rho * ws;
+  }
+  // bisection on the magnitude m of tau (tau = +-m) over the bit patterns of m
+  unsigned long long lob = 0ull, hib = __double_as_longlong(hi);
+  for (int it = 0; it < 64; ++it) {
+    if (hib - lob <= 1ull) break;
+    unsigned long long mb = lob + ((hib - lob) >> 1);
+    double m = __longlong_as_double(mb);
+    double f = secular16(dl, w, K, o, neg ? -m : m, rho, sub);
+    // f increases with tau: for tau = +m the root lies above m iff f < 0;
+    // for tau = -m it lies at larger m iff f > 0
+    bool up = neg ? (f > 0.0) : (f < 0.0);
+    if (up) lob = mb; else hib = mb;
+  }
+  double m = __longlong_as_double(hib);
+  if (active && sub == 0) {
+    a.org[nd.lo + j] = o;
+    a.tau[nd.lo + j] = neg ? -m : m;
+  }
+}
+
+// d_i - lambda_j with lambda_j = d_{o_j} + tau_j
+__device__ inline double dmin_lam(const double* dl, const int* org, const double* tau, int i, int j) {
+  return (dl[i] - dl[org[j]]) - tau[j];
+}
+
+// M4: Gu-Eisenstat weights (DLAED3): w_i = sign(z_i) sqrt(-prod_j (d_i-l_j) / prod_{j!=i} (d_i-d_j))
+__global__ __launch_bounds__(256) void k_dc_lowner(DCArgs a) {
+  const int node = blockIdx.y;
+  const DCNode nd = a.nodes[node];
+  const int K = a.K[node];
+  if (blockIdx.x * (blockDim.x >> 4) >= K) return;
+  const int ir = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
+  const bool active = ir < K;
+  const int i = active ? ir : K - 1;
+  const double* dl = a.cD + nd.lo;
+  const int* org = a.org + nd.lo;
+  const double* tau = a.tau + nd.lo;
+  double p = 1.0;
+  for (int j = sub; j < K; j += 16) {
+    double dlj = dmin_lam(dl, org, tau, i, j);
+    p *= (j == i) ? dlj : dlj / (dl[i] - dl[j]);
+  }
+  for (int off = 8; off > 0; off >>= 1) p *= __shfl_xor(p, off, 16);
+  if (active && sub == 0) a.what[nd.lo + i] = copysign(sqrt(fmax(-p, 0.0)), a.cw[nd.lo + i]);
+}
+
+// M5: new eigenvalues and U^T x for the tracked sources; deflated slots copy through
+__global__ __launch_bounds__(256) void k_dc_apply(DCArgs a) {
+  const int node = blockIdx.y;
+  const DCNode nd = a.nodes[node];
+  const int n = nd.hi - nd.lo;
+  const int K = a.K[node];
+  const int jr = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
+  const int N = a.N, nx = 2 + a.nt;
+  if (blockIdx.x * (blockDim.x >> 4) >= n) return;
+  if (jr >= n) return;                       // lanes of a 16-group share jr
+  const int lo = nd.lo;
+  if (jr >= K) {
+    if (sub == 0) {
+      a.nl[lo + jr] = a.cD[lo + jr];
+      for (int q = 0; q < nx; ++q) a.nx[(size_t)q * N + lo + jr] = a.cx[(size_t)q * N + lo + jr];
+    }
+    return;
+  }
+  const int j = jr;
+  const double* dl = a.cD + lo;
+  const int* org = a.org + lo;
+  const double* tau = a.tau + lo;
+  const double* wh = a.what + lo;
+  double acc[10];
+  for (int q = 0; q < nx; ++q) acc[q] = 0.0;
+  double nrm = 0.0;
+  for (int i = sub; i < K; i += 16) {
+    double s = wh[i] / dmin_lam(dl, org, tau, i, j);
+    nrm += s * s;
+    for (int q = 0; q < nx; ++q) acc[q] += s * a.cx[(size_t)q * N + lo + i];
+  }
+  for (int off = 8; off > 0; off >>= 1) {
+    nrm += __shfl_xor(nrm, off, 16);
+    for (int q = 0; q < nx; ++q) acc[q] += __shfl_xor(acc[q], off, 16);
+  }
+  if (sub == 0) {
+    double inv = 1.0 / sqrt(nrm);
+    a.nl[lo + j] = dl[org[j]] + tau[j];
+    for (int q = 0; q < nx; ++q) a.nx[(size_t)q * N + lo + j] = acc[q] * inv;
+  }
+}
+
+// M6: sort the node's eigenvalues ascending and write lam/FR/LR/T
+__global__ void k_dc_sort(DCArgs a) {
+  const DCNode nd = a.nodes[blockIdx.y];
+  const int n = nd.hi - nd.lo, lo = nd.lo;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = a.nl[lo + i];
+  int rank = 0;
+  for (int k = 0; k < n; ++k) {
+    double u = a.nl[lo + k];
+    rank += (u < v) || (u == v && k < i);
+  }
+  const int N = a.N, dst = lo + rank;
+  a.lam[dst] = v;
+  a.FR[dst] = a.nx[lo + i];
+  a.LR[dst] = a.nx[(size_t)N + lo + i];
+  for (int q = 0; q < a.nt; ++q) a.T[(size_t)q * N + dst] = a.nx[(size_t)(2 + q) * N + lo + i];
+}
+
+// ---------------------------------------------------------------------------
+// host side: tree and launches
+// ---------------------------------------------------------------------------
+static void build_levels(int N, std::vector<std::vector<DCNode>>& levels) {
+  struct Item { int lo, hi, depth; };
+  std::vector<Item> st;
+  st.push_back({0, N, 0});
+  std::vector<std::pair<int, DCNode>> merges;
+  int maxd = 0;
+  while (!st.empty()) {
+    Item it = st.back();
+    st.pop_back();
+    if (it.hi - it.lo <= 1) continue;
+    int mid = it.lo + (it.hi - it.lo) / 2;
+    merges.push_back({it.depth, DCNode{it.lo, mid, it.hi, 0}});
+    maxd = it.depth > maxd ? it.depth : maxd;
+    st.push_back({it.lo, mid, it.depth + 1});
+    st.push_back({mid, it.hi, it.depth + 1});
+  }
+  levels.assign(maxd + 1, {});
+  for (auto& m : merges) levels[maxd - m.first].push_back(m.second);   // deepest first
+}
+
+size_t dc_work_bytes(int N, int ntrack) {
+  size_t nx = 2 + ntrack;
+  size_t b = 0;
+  b += sizeof(double) * N * 3;             // lam FR LR
+  b += sizeof(double) * N * 2;             // sD sz
+  b += sizeof(double) * N * nx * 3;        // sx cx nx
+  b += sizeof(double) * N * 5;             // cD cw tau what nl
+  b += sizeof(int) * N * 2;                // slot org
+  b += (sizeof(int) + sizeof(double)) * (N / 2 + 2);   // K rho
+  b += sizeof(DCNode) * (N + 2);           // nodes
+  return b + 64 * 256;
+}
+
+struct DCLayout {
+  DCArgs a;
+  DCNode* dnodes;
+};
+
+static DCLayout dc_layout(void* work, int N, int ntrack) {
+  char* p = static_cast<char*>(work);
+  auto take = [&](size_t bytes) {
+    char* q = p;
+    p += (bytes + 255) & ~size_t(255);
+    return q;
+  };
+  const size_t nx = 2 + ntrack;
+  DCLayout L;
+  DCArgs& a = L.a;
+  a.N = N;
+  a.nt = ntrack;
+  a.FR = (double*)take(sizeof(double) * N);
+  a.LR = (double*)take(sizeof(double) * N);
+  a.sD = (double*)take(sizeof(double) * N);
+  a.sz = (double*)take(sizeof(double) * N);
+  a.sx = (double*)take(sizeof(double) * N * nx);
+  a.cD = (double*)take(sizeof(double) * N);
+  a.cw = (double*)take(sizeof(double) * N);
+  a.cx = (double*)take(sizeof(double) * N * nx);
+  a.slot = (int*)take(sizeof(int) * N);
+  a.org = (int*)take(sizeof(int) * N);
+  a.tau = (double*)take(sizeof(double) * N);
+  a.what = (double*)take(sizeof(double) * N);
+  a.nl = (double*)take(sizeof(double) * N);
+  a.nx = (double*)take(sizeof(double) * N * nx);
+  a.K = (int*)take(sizeof(int) * (N / 2 + 2));
+  a.rho = (double*)take(sizeof(double) * (N / 2 + 2));
+  L.dnodes = (DCNode*)take(sizeof(DCNode) * (N + 2));
+  return L;
+}
+
+int dc_prepare(void* work, int N, int ntrack, DCTree& tree) {
+  std::vector<std::vector<DCNode>> levels;
+  build_levels(N, levels);
+  std::vector<DCNode> flat;
+  tree.level_off.clear();
+  tree.level_cnt.clear();
+  tree.level_maxn.clear();
+  for (auto& L : levels) {
+    if (L.empty()) continue;
+    int maxn = 0;
+    for (auto& nd : L) maxn = std::max(maxn, nd.hi - nd.lo);
+    tree.level_off.push_back((int)flat.size());
+    tree.level_cnt.push_back((int)L.size());
+    tree.level_maxn.push_back(maxn);
+    flat.insert(flat.end(), L.begin(), L.end());
+  }
+  tree.N = N;
+  tree.ntrack = ntrack;
+  DCLayout lay = dc_layout(work, N, ntrack);
+  if (!flat.empty()) {
+    WTB_HIP(hipMemcpy(lay.dnodes, flat.data(), sizeof(DCNode) * flat.size(), hipMemcpyHostToDevice));
+  }
+  return WTB_OK;
+}
+
+int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, const double* d_e,
+           double* d_track, double* d_lambda) {
+  const int N = tree.N;
+  if (tree.ntrack > 8) return WTB_ERR_LIMIT;
+  DCLayout lay = dc_layout(work, N, tree.ntrack);
+  DCArgs a = lay.a;
+  a.e = d_e;
+  a.lam = d_lambda;
+  a.T = d_track;
+  hipLaunchKernelGGL(k_dc_leaves, dim3((N + 255) / 256), dim3(256), 0, s, N, d_d, d_e, d_lambda, a.FR,
+                     a.LR);
+  WTB_LAUNCH_CHECK();
+  for (size_t l = 0; l < tree.level_off.size(); ++l) {
+    const int cnt = tree.level_cnt[l], maxn = tree.level_maxn[l];
+    a.nodes = lay.dnodes + tree.level_off[l];
+    a.nnodes = cnt;
+    dim3 g1((maxn + 255) / 256, cnt);
+    dim3 g16((maxn * 16 + 255) / 256, cnt);
+    hipLaunchKernelGGL(k_dc_gather, g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dc_deflate, dim3((cnt + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_dc_compact, g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dc_secular, g16, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dc_lowner, g16, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dc_apply, g16, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dc_sort, g1, dim3(256), 0, s, a);
+    WTB_LAUNCH_CHECK();
+  }
+  return WTB_OK;
+}
+
+}  // namespace wtb

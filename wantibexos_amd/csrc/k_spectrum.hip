@@ -1,0 +1,203 @@
+// Stage (e): oscillator strengths and eps(omega).
+//
+// Reference: dielbsev/dielbsep (subroutines/bse_subs.f90:278-378) and their
+// driver (subprograms/bse_solver-tool-diel.f90:540-602); the "(7F15.4)" write
+// of bse_opt_diel.dat and its read-back (:607-620, bse_diel-tool.f90:117-123);
+// optdiel/avgsme/imagrenorm (subroutines/optics.f90:160-244,302-390) in the
+// bsedielraw loop (subprograms/bse_diel-tool.f90:111,135-201).
+#include "wtb_kernels.h"
+
+namespace wtb {
+
+// G = Z^T (Q^T h) as 6 real vectors [x.re, x.im, y.re, y.im, z.re, z.im]; the
+// activities are |t|^2 and Re(t_a conj(t_b)) with t = X^T h (no conjugate on X).
+__global__ void k_activities(int N, const double* __restrict__ G, int dtfull, int cpol,
+                             double* __restrict__ act) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  zc tx = mkz(G[n], G[(size_t)N + n]);
+  zc ty = mkz(G[2 * (size_t)N + n], G[3 * (size_t)N + n]);
+  zc tz = mkz(G[4 * (size_t)N + n], G[5 * (size_t)N + n]);
+  auto sq = [](zc a) { return a.re * a.re + a.im * a.im; };
+  auto cr = [](zc a, zc b) { return a.re * b.re + a.im * b.im; };   // Re(a conj(b))
+  act[n] = sq(tx);
+  act[(size_t)N + n] = sq(ty);
+  act[2 * (size_t)N + n] = sq(tz);
+  act[3 * (size_t)N + n] = dtfull ? cr(tx, ty) : 0.0;
+  act[4 * (size_t)N + n] = dtfull ? cr(tx, tz) : 0.0;
+  act[5 * (size_t)N + n] = dtfull ? cr(ty, tz) : 0.0;
+  // hrsp = (hrx + i hry) * REAL(1./sqrt(2.)), hrsm = (hrx - i hry) * ... (:376-377)
+  const double c = (double)(1.0f / sqrtf(2.0f));
+  zc tp = mkz(tx.re - ty.im, tx.im + ty.re) * c;
+  zc tm = mkz(tx.re + ty.im, tx.im - ty.re) * c;
+  act[6 * (size_t)N + n] = cpol ? sq(tp) : 0.0;
+  act[7 * (size_t)N + n] = cpol ? sq(tm) : 0.0;
+}
+
+// x -> the value a "(F15.4)" write and a list-directed read give back: the
+// decimal rounding of the exact binary value to 4 places, then the nearest
+// double of that decimal.
+__device__ inline double f15_4(double x) {
+  double y = x * 1e4;
+  double err = fma(x, 1e4, -y);          // x*1e4 = y + err exactly
+  double fl = floor(y);
+  double r;
+  if (y - fl == 0.5) r = err > 0 ? fl + 1.0 : (err < 0 ? fl : rint(y));
+  else r = rint(y);
+  return r / 1e4;
+}
+
+__global__ void k_quantize(int n, const double* __restrict__ src, double* __restrict__ dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = f15_4(src[i]);
+}
+
+// avgsme (optics.f90:302-324): sequential sum of successive differences
+__global__ void k_avgsme(int N, const double* __restrict__ E, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 1; i < N; ++i) s = s + E[i] - E[i - 1];
+  out[0] = 10.0 * (s / (double)N);
+}
+
+// optdiel for 4 photon energies per workgroup and all six components:
+//   eps2 = caux sum f g/((w-E)^2+g^2),  eps1 = delta + caux sum f (E-w)/((w-E)^2+g^2)
+// with g = sme + avgsme; partial sums reduced in a fixed tree order.
+#define OMB 4
+__global__ __launch_bounds__(256) void k_optdiel(int N, const double* __restrict__ E,
+                                                 const double* __restrict__ f6, double caux,
+                                                 double ebse0, double ebsef, int numbse, double sme,
+                                                 const double* __restrict__ smeavg,
+                                                 double* __restrict__ eps) {
+#pragma clang fp contract(off)
+  __shared__ double red[256];
+  const int j0 = blockIdx.x * OMB;
+  const double smef = smeavg[0] + sme;
+  double om[OMB];
+  for (int q = 0; q < OMB; ++q) {
+    int j = j0 + q;
+    om[q] = ebse0 + ((ebsef - ebse0) * (double)j) / ((double)numbse - 1.0);
+  }
+  double sr[OMB][6], si[OMB][6];
+  for (int q = 0; q < OMB; ++q)
+    for (int c = 0; c < 6; ++c) sr[q][c] = si[q][c] = 0.0;
+  for (int n = threadIdx.x; n < N; n += 256) {
+    double e = E[n];
+    double f[6];
+    for (int c = 0; c < 6; ++c) f[c] = f6[(size_t)c * N + n];
+    for (int q = 0; q < OMB; ++q) {
+      double dw = om[q] - e;
+      double eaux = (dw * dw) + (smef * smef);
+      double rr = (e - om[q]) / eaux;
+      double ii = smef / eaux;
+      for (int c = 0; c < 6; ++c) {
+        sr[q][c] += f[c] * rr;
+        si[q][c] += f[c] * ii;
+      }
+    }
+  }
+  for (int q = 0; q < OMB; ++q)
+    for (int c = 0; c < 6; ++c)
+      for (int part = 0; part < 2; ++part) {
+        double v = part ? si[q][c] : sr[q][c];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+          if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+          __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+          int j = j0 + q;
+          if (j < numbse) {
+            double out = part ? red[0] * caux : (c < 3 ? 1.0 : 0.0) + red[0] * caux;
+            eps[j + (size_t)numbse * (part + 2 * c)] = out;
+          }
+        }
+        __syncthreads();
+      }
+}
+
+// imagrenorm (optics.f90:327-390) on eps2 of each component; one thread per component
+__global__ void k_imagrenorm(const double* __restrict__ E, double ebse0, double ebsef, int numbse,
+                             double* __restrict__ eps) {
+#pragma clang fp contract(off)
+  const int c = threadIdx.x;
+  if (c >= 6) return;
+  const double factor = (double)0.15f;
+  const double aux = E[0] - factor;
+  if (aux <= 0.0) return;
+  double* im = eps + (size_t)numbse * (1 + 2 * c);
+  int ib = 0;
+  for (int j = 0; j < numbse; ++j) {
+    double w = ebse0 + ((ebsef - ebse0) * (double)j) / ((double)numbse - 1.0);
+    if (w <= aux) ib = j;
+    else break;
+  }
+  if (ib + 1 >= numbse) {      // the reference would read past the grid; zero it all
+    for (int j = 0; j < numbse; ++j) im[j] = 0.0;
+    return;
+  }
+  const double aux3 = im[ib + 1];
+  for (int j = 0; j <= ib; ++j) im[j] = 0.0;
+  for (int j = ib + 1; j < numbse; ++j) {
+    double v = im[j] - aux3;
+    im[j] = v < 0.0 ? 0.0 : v;
+  }
+}
+
+// t_i = sum_j X(j,i) h(j) for up to two vectors (dielbsev / dielbsep), one wave per column i
+__global__ __launch_bounds__(256) void k_xth(int n, const zc* __restrict__ X, const zc* __restrict__ h1,
+                                             const zc* __restrict__ h2, double* __restrict__ act) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  zc a = mkz(0, 0), b = mkz(0, 0);
+  const zc* col = X + (size_t)i * n;
+  for (int j = lane; j < n; j += 64) {
+    zc x = col[j];
+    a = fma_z(x, h1[j], a);
+    if (h2) b = fma_z(x, h2[j], b);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a.re += __shfl_xor(a.re, off);
+    a.im += __shfl_xor(a.im, off);
+    b.re += __shfl_xor(b.re, off);
+    b.im += __shfl_xor(b.im, off);
+  }
+  if (lane == 0) act[i] = h2 ? (a.re * b.re + a.im * b.im) : (a.re * a.re + a.im * a.im);
+}
+
+int launch_activities(hipStream_t s, int N, const double* d_G, int dtfull, int cpol, double* d_act) {
+  hipLaunchKernelGGL(k_activities, dim3((N + 255) / 256), dim3(256), 0, s, N, d_G, dtfull, cpol, d_act);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int launch_quantize(hipStream_t s, int n, const double* src, double* dst) {
+  hipLaunchKernelGGL(k_quantize, dim3((n + 255) / 256), dim3(256), 0, s, n, src, dst);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+// d_work: >= 1 double.  caux = fl32(180.90) / (REAL(nk) * vc) (optics.f90:188-196)
+int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act6, double vc,
+                    double nkf32, double ebse0, double ebsef, int numbse, double sme, int renorm,
+                    double* d_work, double* d_eps) {
+  const double gama = (double)180.90f;
+  const double baux = nkf32 * vc;
+  const double caux = gama / (baux * 1.0);
+  hipLaunchKernelGGL(k_avgsme, dim3(1), dim3(64), 0, s, N, d_e, d_work);
+  hipLaunchKernelGGL(k_optdiel, dim3((numbse + OMB - 1) / OMB), dim3(256), 0, s, N, d_e, d_act6, caux,
+                     ebse0, ebsef, numbse, sme, d_work, d_eps);
+  if (renorm) hipLaunchKernelGGL(k_imagrenorm, dim3(1), dim3(64), 0, s, d_e, ebse0, ebsef, numbse, d_eps);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int launch_xth(hipStream_t s, int n, const zc* X, const zc* h1, const zc* h2, double* act) {
+  hipLaunchKernelGGL(k_xth, dim3((n + 3) / 4), dim3(256), 0, s, n, X, h1, h2, act);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+}  // namespace wtb

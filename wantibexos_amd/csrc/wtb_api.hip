@@ -1,0 +1,714 @@
+// C-ABI of the BSE pipeline (include/wtb_bse.h): context, plan, entry points.
+#include "wtb_kernels.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace wtb {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+int launch_xth(hipStream_t s, int n, const zc* X, const zc* h1, const zc* h2, double* act);
+
+__global__ void k_zero_int(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_split_g(int N, const zc* __restrict__ g, double* __restrict__ track) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int q = 0; q < 3; ++q) {
+    zc v = g[(size_t)q * N + i];
+    track[(size_t)(2 * q) * N + i] = v.re;
+    track[(size_t)(2 * q + 1) * N + i] = v.im;
+  }
+}
+// lower triangle of the Hermitian matrix whose upper triangle is given (ZHEEV 'U')
+__global__ void k_upper_to_lower(int n, const zc* __restrict__ a, int lda, zc* __restrict__ h, int ldh) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= n || i < j) return;
+  zc v = a[(size_t)j + (size_t)lda * i];      // a(j,i), j <= i
+  v = conj(v);
+  if (i == j) v.im = 0.0;
+  h[(size_t)i + (size_t)ldh * j] = v;
+}
+
+static int status_to_code(int st) {
+  if (st & (1 << WTB_ERR_NOCP)) return WTB_ERR_NOCP;
+  if (st & (1 << WTB_ERR_NVAL)) return WTB_ERR_NVAL;
+  if (st & (1 << WTB_ERR_NCOND)) return WTB_ERR_NCOND;
+  if (st & (1 << WTB_ERR_EIG)) return WTB_ERR_EIG;
+  return WTB_OK;
+}
+static const char* status_text(int code) {
+  switch (code) {
+    case WTB_ERR_NOCP: return "no band above the Fermi level at some k (nocp undefined)";
+    case WTB_ERR_NVAL: return "Number of valence states higher then ocuppied states";
+    case WTB_ERR_NCOND: return "Number of conduction states higher then unocuppied states";
+    case WTB_ERR_EIG: return "The algorithm failed to compute eigenvalues.";
+    default: return "";
+  }
+}
+
+template <class T>
+static int dmalloc(T** p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc(") + std::to_string(sizeof(T) * count) + " bytes): " +
+              hipGetErrorString(e));
+    *p = nullptr;
+    return WTB_ERR_NOMEM;
+  }
+  return WTB_OK;
+}
+
+}  // namespace wtb
+
+using namespace wtb;
+
+struct wtb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct wtb_bse_plan {
+  wtb_ctx* ctx = nullptr;
+  wtb_bse_params prm{};
+  int nk = 0, nw = 0, nvec = 0, nb = 0, N = 0;
+  double ef32 = 0, scs = 0, sme32 = 0, vc3d = 0, nkf32 = 0;
+  CoulombArgs ca{};
+  DevModel dm;
+  double* kpt = nullptr;
+  zc* phase = nullptr;
+  zc* hk4 = nullptr;
+  double* eall = nullptr;
+  int* nocp = nullptr;
+  double* eigv = nullptr;
+  zc* vec = nullptr;
+  zc* dip = nullptr;
+  zc* g = nullptr;
+  double* gapk = nullptr;
+  double* wtab = nullptr;
+  int wtab_n = 0;
+  zc* H = nullptr;
+  size_t ldh = 0;
+  void* triwork = nullptr;
+  TridiagWork tw;
+  void* dcwork = nullptr;
+  DCTree tree;
+  double* track = nullptr;
+  double* lambda = nullptr;
+  double* act = nullptr;
+  double* wq = nullptr;
+  double* actq = nullptr;
+  double* eps = nullptr;
+  double* smeavg = nullptr;
+  int* status = nullptr;
+  bool timing = false;
+  hipEvent_t ev[WTB_NSTAGES + 1] = {};
+  double stage_ms[WTB_NSTAGES] = {};
+};
+
+extern "C" {
+
+const char* wtb_last_error(void) { return g_err.c_str(); }
+int wtb_version(void) { return 10000; }
+
+int wtb_create(int device, wtb_ctx** out) {
+  if (!out) {
+    set_error("wtb_create: null output pointer");
+    return WTB_ERR_ARG;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    set_error("wtb_create: no HIP device available (the library runs on MI355X/gfx950 only)");
+    return WTB_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("wtb_create: device index out of range");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  WTB_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
+    set_error(std::string("wtb_create: device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    return WTB_ERR_HIP;
+  }
+  wtb_ctx* c = new wtb_ctx;
+  c->device = device;
+  WTB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c;
+  return WTB_OK;
+}
+
+int wtb_destroy(wtb_ctx* c) {
+  if (!c) return WTB_OK;
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return WTB_OK;
+}
+
+void* wtb_stream(wtb_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+int wtb_monkhorst_pack(wtb_ctx* c, const int ngrid[3], const double shift[3], const double* rlat,
+                       double* kpt) {
+  if (!c || !ngrid || !shift || !rlat || !kpt) {
+    set_error("wtb_monkhorst_pack: null argument");
+    return WTB_ERR_ARG;
+  }
+  if (ngrid[0] < 1 || ngrid[1] < 1 || ngrid[2] < 1) {
+    set_error("wtb_monkhorst_pack: grid sizes must be >= 1");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  int nk = ngrid[0] * ngrid[1] * ngrid[2];
+  double *d_rl = nullptr, *d_k = nullptr;
+  int rc;
+  if ((rc = dmalloc(&d_rl, 9)) || (rc = dmalloc(&d_k, (size_t)nk * 3))) {
+    hipFree(d_rl);
+    return rc;
+  }
+  hipMemcpyAsync(d_rl, rlat, 9 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  rc = launch_kgrid(c->stream, d_rl, ngrid, shift, d_k);
+  if (rc == WTB_OK) {
+    hipMemcpyAsync(kpt, d_k, sizeof(double) * nk * 3, hipMemcpyDeviceToHost, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = WTB_ERR_HIP;
+  }
+  hipFree(d_rl);
+  hipFree(d_k);
+  return rc;
+}
+
+static int check_model(const wtb_tb_model* m) {
+  if (!m || !m->rlat || !m->rvec || !m->ffactor || !m->hop || !m->ihop) {
+    set_error("tight-binding model: null array");
+    return WTB_ERR_ARG;
+  }
+  if (m->nw < 1 || m->nvec < 1) {
+    set_error("tight-binding model: nw and nvec must be >= 1");
+    return WTB_ERR_ARG;
+  }
+  if (m->nw > 64) {
+    set_error("tight-binding model: nw > 64 exceeds the LDS-resident per-k eigensolver");
+    return WTB_ERR_LIMIT;
+  }
+  size_t cnt = (size_t)m->nvec * m->nw * m->nw;
+  for (size_t i = 0; i < cnt; ++i)
+    if (!std::isfinite(m->hop[i]) || !std::isfinite(m->ihop[i])) {
+      set_error("tight-binding model: NaN or Inf in H(R)");
+      return WTB_ERR_NONFINITE;
+    }
+  for (int i = 0; i < m->nvec; ++i)
+    if (m->ffactor[i] == 0) {
+      set_error("tight-binding model: zero degeneracy factor");
+      return WTB_ERR_ARG;
+    }
+  if (!std::isfinite(m->efermi) || !std::isfinite(m->scs)) {
+    set_error("tight-binding model: non-finite E_F or scissor");
+    return WTB_ERR_NONFINITE;
+  }
+  return WTB_OK;
+}
+
+static int upload_model(hipStream_t s, const wtb_tb_model* m, DevModel& dm) {
+  int rc;
+  dm.nw = m->nw;
+  dm.nvec = m->nvec;
+  if ((rc = dmalloc(&dm.rlat, 9)) || (rc = dmalloc(&dm.rvec, (size_t)m->nvec * 3)) ||
+      (rc = dmalloc(&dm.hstack, (size_t)m->nvec * m->nw * m->nw)))
+    return rc;
+  size_t cnt = (size_t)m->nvec * m->nw * m->nw;
+  double *dh = nullptr, *dih = nullptr;
+  int* dff = nullptr;
+  if ((rc = dmalloc(&dh, cnt)) || (rc = dmalloc(&dih, cnt)) || (rc = dmalloc(&dff, m->nvec))) {
+    hipFree(dh);
+    hipFree(dih);
+    return rc;
+  }
+  WTB_HIP(hipMemcpy(dm.rlat, m->rlat, 9 * sizeof(double), hipMemcpyHostToDevice));
+  WTB_HIP(hipMemcpy(dm.rvec, m->rvec, sizeof(int) * 3 * m->nvec, hipMemcpyHostToDevice));
+  WTB_HIP(hipMemcpy(dh, m->hop, sizeof(double) * cnt, hipMemcpyHostToDevice));
+  WTB_HIP(hipMemcpy(dih, m->ihop, sizeof(double) * cnt, hipMemcpyHostToDevice));
+  WTB_HIP(hipMemcpy(dff, m->ffactor, sizeof(int) * m->nvec, hipMemcpyHostToDevice));
+  rc = launch_hr_stack(s, dh, dih, dff, m->nvec, m->nw, dm.hstack);
+  hipStreamSynchronize(s);
+  hipFree(dh);
+  hipFree(dih);
+  hipFree(dff);
+  return rc;
+}
+
+static void free_model(DevModel& dm) {
+  hipFree(dm.rlat);
+  hipFree(dm.rvec);
+  hipFree(dm.hstack);
+  dm = DevModel();
+}
+
+int wtb_eigsys(wtb_ctx* c, const wtb_tb_model* m, int nk, const double* kpt, double* energias,
+               double* autovetores, int* nocp) {
+  if (!c || !kpt || !energias || nk < 1) {
+    set_error("wtb_eigsys: bad argument");
+    return WTB_ERR_ARG;
+  }
+  int rc = check_model(m);
+  if (rc) return rc;
+  WTB_HIP(hipSetDevice(c->device));
+  DevModel dm;
+  if ((rc = upload_model(c->stream, m, dm))) {
+    free_model(dm);
+    return rc;
+  }
+  const int nw = m->nw;
+  double *dk = nullptr, *de = nullptr;
+  zc *dph = nullptr, *dhk = nullptr, *du = nullptr;
+  int *dn = nullptr, *dst = nullptr;
+  if ((rc = dmalloc(&dk, (size_t)nk * 3)) || (rc = dmalloc(&dph, (size_t)4 * nk * m->nvec)) ||
+      (rc = dmalloc(&dhk, (size_t)4 * nk * nw * nw)) || (rc = dmalloc(&de, (size_t)nk * nw)) ||
+      (rc = dmalloc(&du, (size_t)nk * nw * nw)) || (rc = dmalloc(&dn, nk)) || (rc = dmalloc(&dst, 1))) {
+  } else {
+    hipMemcpyAsync(dk, kpt, sizeof(double) * nk * 3, hipMemcpyHostToDevice, c->stream);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, dst, 1);
+    double ef32 = (double)(float)(-m->efermi);
+    rc = launch_hk(c->stream, dm, dk, nk, dph, dhk);
+    if (!rc)
+      rc = launch_jacobi(c->stream, dhk, nk, nw, ef32, m->scs, 0, 0, de, du, dn, nullptr, nullptr, dst);
+    int st = 0;
+    if (!rc) {
+      hipMemcpyAsync(energias, de, sizeof(double) * nk * nw, hipMemcpyDeviceToHost, c->stream);
+      if (autovetores)
+        hipMemcpyAsync(autovetores, du, sizeof(zc) * nk * nw * nw, hipMemcpyDeviceToHost, c->stream);
+      if (nocp) hipMemcpyAsync(nocp, dn, sizeof(int) * nk, hipMemcpyDeviceToHost, c->stream);
+      hipMemcpyAsync(&st, dst, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+      if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("wtb_eigsys: stream error");
+        rc = WTB_ERR_HIP;
+      }
+    }
+    if (!rc) {
+      st &= (1 << WTB_ERR_NOCP) | (1 << WTB_ERR_EIG);
+      rc = status_to_code(st);
+      if (rc) set_error(status_text(rc));
+    }
+  }
+  hipFree(dk);
+  hipFree(dph);
+  hipFree(dhk);
+  hipFree(de);
+  hipFree(du);
+  hipFree(dn);
+  hipFree(dst);
+  free_model(dm);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+static void plan_free(wtb_bse_plan* p) {
+  free_model(p->dm);
+  void* bufs[] = {p->kpt, p->phase, p->hk4, p->eall, p->nocp, p->eigv, p->vec, p->dip, p->g,
+                  p->gapk, p->wtab, p->H, p->triwork, p->dcwork, p->track, p->lambda, p->act,
+                  p->wq, p->actq, p->eps, p->smeavg, p->status};
+  for (void* b : bufs) hipFree(b);
+  for (auto& e : p->ev)
+    if (e) hipEventDestroy(e);
+}
+
+static double vcell3d_host(const double* rl) {
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) a[i][c] = rl[i + 3 * c];
+  double vx0 = (a[0][1] * a[1][2]) - (a[0][2] * a[1][1]);
+  double vx1 = (a[0][2] * a[1][0]) - (a[0][0] * a[1][2]);
+  double vx2 = (a[0][0] * a[1][1]) - (a[0][1] * a[1][0]);
+  return std::fabs(vx0 * a[2][0] + vx1 * a[2][1] + vx2 * a[2][2]);
+}
+
+int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm,
+                        wtb_bse_plan** out) {
+  if (!c || !prm || !out) {
+    set_error("wtb_bse_plan_create: null argument");
+    return WTB_ERR_ARG;
+  }
+  *out = nullptr;
+  int rc = check_model(m);
+  if (rc) return rc;
+  if (prm->ngrid[0] < 1 || prm->ngrid[1] < 1 || prm->ngrid[2] < 1 || prm->nc < 1 || prm->nv < 1 ||
+      prm->numbse < 2) {
+    set_error("wtb_bse_plan_create: grid, nc, nv must be >= 1 and NESPEC >= 2");
+    return WTB_ERR_ARG;
+  }
+  if (prm->coultype < WTB_V2DK || prm->coultype > WTB_V0DT) {
+    set_error("Wrong Coulomb Potential");
+    return WTB_ERR_COULOMB;
+  }
+  if (prm->nc + prm->nv > m->nw) {
+    set_error("nc + nv exceeds the number of bands");
+    return WTB_ERR_ARG;
+  }
+  if (prm->precision != WTB_FP64 && prm->precision != WTB_MIXED) {
+    set_error("wtb_bse_plan_create: unknown precision form");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  wtb_bse_plan* p = new wtb_bse_plan;
+  p->ctx = c;
+  p->prm = *prm;
+  p->nk = prm->ngrid[0] * prm->ngrid[1] * prm->ngrid[2];
+  p->nw = m->nw;
+  p->nvec = m->nvec;
+  p->nb = prm->nc + prm->nv;
+  p->N = p->nk * prm->nc * prm->nv;
+  p->ef32 = (double)(float)(-m->efermi);
+  p->scs = m->scs;
+  p->sme32 = (double)(float)prm->cshift;
+  p->vc3d = vcell3d_host(m->rlat);
+  p->nkf32 = (double)(float)p->nk;
+  p->ca.type = prm->coultype;
+  p->ca.n1 = prm->ngrid[0];
+  p->ca.n2 = prm->ngrid[1];
+  p->ca.n3 = prm->ngrid[2];
+  for (int i = 0; i < 3; ++i) p->ca.ediel[i] = prm->ediel[i];
+  p->ca.lc = prm->lc;
+  p->ca.ez = prm->ez;
+  p->ca.w = prm->w;
+  p->ca.r0 = prm->r0;
+  p->ca.ktol = prm->ktol;
+  const int nk = p->nk, nw = p->nw, N = p->N;
+  p->ldh = (size_t)N;
+  p->wtab_n = (2 * prm->ngrid[0] - 1) * (2 * prm->ngrid[1] - 1) * (2 * prm->ngrid[2] - 1);
+  hipStream_t s = c->stream;
+  const int nb_panel = 32;
+  if ((rc = upload_model(s, m, p->dm)) || (rc = dmalloc(&p->kpt, (size_t)nk * 3)) ||
+      (rc = dmalloc(&p->phase, (size_t)4 * nk * p->nvec)) ||
+      (rc = dmalloc(&p->hk4, (size_t)4 * nk * nw * nw)) || (rc = dmalloc(&p->eall, (size_t)nk * nw)) ||
+      (rc = dmalloc(&p->nocp, nk)) || (rc = dmalloc(&p->eigv, (size_t)nk * p->nb)) ||
+      (rc = dmalloc(&p->vec, (size_t)nk * p->nb * nw)) || (rc = dmalloc(&p->dip, (size_t)3 * N)) ||
+      (rc = dmalloc(&p->g, (size_t)3 * N)) || (rc = dmalloc(&p->gapk, nk)) ||
+      (rc = dmalloc(&p->wtab, p->wtab_n)) || (rc = dmalloc(&p->H, p->ldh * N)) ||
+      (rc = dmalloc((char**)&p->triwork, tridiag_work_bytes(N, nb_panel, 3))) ||
+      (rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 6))) ||
+      (rc = dmalloc(&p->track, (size_t)6 * N)) || (rc = dmalloc(&p->lambda, N)) ||
+      (rc = dmalloc(&p->act, (size_t)8 * N)) || (rc = dmalloc(&p->wq, N)) ||
+      (rc = dmalloc(&p->actq, (size_t)6 * N)) || (rc = dmalloc(&p->eps, (size_t)12 * prm->numbse)) ||
+      (rc = dmalloc(&p->smeavg, 4)) || (rc = dmalloc(&p->status, 4))) {
+    plan_free(p);
+    delete p;
+    return rc;
+  }
+  tridiag_bind(p->tw, p->triwork, N, nb_panel, 3);
+  if ((rc = dc_prepare(p->dcwork, N, 6, p->tree))) {
+    plan_free(p);
+    delete p;
+    return rc;
+  }
+  for (auto& e : p->ev) hipEventCreate(&e);
+  *out = p;
+  return WTB_OK;
+}
+
+int wtb_bse_plan_dim(const wtb_bse_plan* p) { return p ? p->N : 0; }
+
+int wtb_bse_plan_set_timing(wtb_bse_plan* p, int on) {
+  if (!p) return WTB_ERR_ARG;
+  p->timing = on != 0;
+  return WTB_OK;
+}
+
+int wtb_bse_plan_stage_ms(const wtb_bse_plan* p, double* ms) {
+  if (!p || !ms) return WTB_ERR_ARG;
+  for (int i = 0; i < WTB_NSTAGES; ++i) ms[i] = p->stage_ms[i];
+  return WTB_OK;
+}
+
+static int plan_enqueue(wtb_bse_plan* p) {
+  hipStream_t s = p->ctx->stream;
+  const wtb_bse_params& P = p->prm;
+  const int nk = p->nk, nw = p->nw, N = p->N;
+  int rc;
+  auto mark = [&](int i) {
+    if (p->timing) hipEventRecord(p->ev[i], s);
+  };
+  mark(0);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p->status, 1);
+  // (a) k grid + H(k) phase GEMM, (b) per-k Jacobi
+  if ((rc = launch_kgrid(s, p->dm.rlat, P.ngrid, P.mshift, p->kpt))) return rc;
+  if ((rc = launch_hk(s, p->dm, p->kpt, nk, p->phase, p->hk4))) return rc;
+  if ((rc = launch_jacobi(s, p->hk4, nk, nw, p->ef32, p->scs, P.nc, P.nv, p->eall, nullptr, p->nocp,
+                          p->eigv, p->vec, p->status)))
+    return rc;
+  mark(1);
+  if ((rc = launch_dipoles(s, p->hk4, nk, nw, P.nc, P.nv, p->sme32, p->eigv, p->vec, p->dip, p->gapk)))
+    return rc;
+  mark(2);
+  // (c) W table + assembly (lower triangle)
+  if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
+  if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H,
+                             (int)p->ldh, nullptr)))
+    return rc;
+  mark(3);
+  // (d) Householder tridiagonalisation, g <- Q^T h
+  WTB_HIP(hipMemcpyAsync(p->g, p->dip, sizeof(zc) * 3 * N, hipMemcpyDeviceToDevice, s));
+  if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, p->g))) return rc;
+  mark(4);
+  // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
+  hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->g, p->track);
+  if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, p->track, p->lambda))) return rc;
+  mark(5);
+  // (e) activities, F15.4 round trip, eps(omega)
+  if ((rc = launch_activities(s, N, p->track, P.dtfull, P.cpol, p->act))) return rc;
+  const double* e_in = p->lambda;
+  const double* a_in = p->act;
+  if (P.f154) {
+    if ((rc = launch_quantize(s, N, p->lambda, p->wq))) return rc;
+    if ((rc = launch_quantize(s, 6 * N, p->act, p->actq))) return rc;
+    e_in = p->wq;
+    a_in = p->actq;
+  }
+  if ((rc = launch_spectrum(s, N, e_in, a_in, p->vc3d, p->nkf32, P.ebse0, P.ebsef, P.numbse, P.cshift,
+                            P.renorm, p->smeavg, p->eps)))
+    return rc;
+  mark(6);
+  return WTB_OK;
+}
+
+int wtb_bse_plan_execute(wtb_bse_plan* p) {
+  if (!p) {
+    set_error("wtb_bse_plan_execute: null plan");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  return plan_enqueue(p);
+}
+
+int wtb_bse_plan_results(wtb_bse_plan* p, double* w, double* act, double* eps, double* egap) {
+  if (!p) {
+    set_error("wtb_bse_plan_results: null plan");
+    return WTB_ERR_ARG;
+  }
+  hipStream_t s = p->ctx->stream;
+  const int N = p->N;
+  int st = 0;
+  WTB_HIP(hipMemcpyAsync(&st, p->status, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (w) WTB_HIP(hipMemcpyAsync(w, p->lambda, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  if (act) WTB_HIP(hipMemcpyAsync(act, p->act, sizeof(double) * 8 * N, hipMemcpyDeviceToHost, s));
+  if (eps)
+    WTB_HIP(hipMemcpyAsync(eps, p->eps, sizeof(double) * 12 * p->prm.numbse, hipMemcpyDeviceToHost, s));
+  std::vector<double> gk;
+  if (egap) {
+    gk.resize(p->nk);
+    WTB_HIP(hipMemcpyAsync(gk.data(), p->gapk, sizeof(double) * p->nk, hipMemcpyDeviceToHost, s));
+  }
+  WTB_HIP(hipStreamSynchronize(s));
+  if (egap) {
+    // egap = 50.0; keep the minimum (bse_solver-tool-diel.f90:255,305-311)
+    double g = 50.0;
+    for (double v : gk)
+      if (v <= g) g = v;
+    *egap = g;
+  }
+  if (p->timing) {
+    // stage boundaries: [0]=start [1]=eig [2]=dip [3]=build [4]=tridiag [5]=dc [6]=end
+    float t;
+    const int idx[WTB_NSTAGES + 1] = {0, 1, 2, 3, 4, 5, 6};
+    for (int i = 0; i < WTB_NSTAGES; ++i) {
+      if (hipEventElapsedTime(&t, p->ev[idx[i]], p->ev[idx[i + 1]]) == hipSuccess) p->stage_ms[i] = t;
+    }
+  }
+  int rc = status_to_code(st);
+  if (rc) set_error(status_text(rc));
+  return rc;
+}
+
+size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* p, int what) {
+  if (!p) return 0;
+  switch (what) {
+    case WTB_BUF_KPT: return sizeof(double) * p->nk * 3;
+    case WTB_BUF_EALL: return sizeof(double) * p->nk * p->nw;
+    case WTB_BUF_NOCP: return sizeof(int) * p->nk;
+    case WTB_BUF_EIGV: return sizeof(double) * p->nk * p->nb;
+    case WTB_BUF_VECTOR: return sizeof(zc) * p->nk * p->nb * p->nw;
+    case WTB_BUF_DIPOLE: return sizeof(zc) * 3 * p->N;
+    case WTB_BUF_HBSE: return sizeof(zc) * p->ldh * p->N;
+    case WTB_BUF_WTAB: return sizeof(double) * p->wtab_n;
+    case WTB_BUF_TRIDIAG: return sizeof(double) * 2 * p->N;
+    default: return 0;
+  }
+}
+
+int wtb_bse_plan_fetch(wtb_bse_plan* p, int what, void* dst, size_t bytes) {
+  if (!p || !dst || bytes != wtb_bse_plan_buffer_bytes(p, what) || bytes == 0) {
+    set_error("wtb_bse_plan_fetch: bad buffer id or size");
+    return WTB_ERR_ARG;
+  }
+  hipStream_t s = p->ctx->stream;
+  const void* src = nullptr;
+  switch (what) {
+    case WTB_BUF_KPT: src = p->kpt; break;
+    case WTB_BUF_EALL: src = p->eall; break;
+    case WTB_BUF_NOCP: src = p->nocp; break;
+    case WTB_BUF_EIGV: src = p->eigv; break;
+    case WTB_BUF_VECTOR: src = p->vec; break;
+    case WTB_BUF_DIPOLE: src = p->dip; break;
+    case WTB_BUF_HBSE: src = p->H; break;
+    case WTB_BUF_WTAB: src = p->wtab; break;
+    case WTB_BUF_TRIDIAG: {
+      WTB_HIP(hipMemcpyAsync(dst, p->tw.d, sizeof(double) * p->N, hipMemcpyDeviceToHost, s));
+      WTB_HIP(hipMemcpyAsync((char*)dst + sizeof(double) * p->N, p->tw.e, sizeof(double) * p->N,
+                             hipMemcpyDeviceToHost, s));
+      WTB_HIP(hipStreamSynchronize(s));
+      return WTB_OK;
+    }
+  }
+  WTB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  WTB_HIP(hipStreamSynchronize(s));
+  return WTB_OK;
+}
+
+int wtb_bse_plan_destroy(wtb_bse_plan* p) {
+  if (!p) return WTB_OK;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  plan_free(p);
+  delete p;
+  return WTB_OK;
+}
+
+int wtb_bsesolver(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, double* w, double* act,
+                  double* eps, double* egap) {
+  wtb_bse_plan* p = nullptr;
+  int rc = wtb_bse_plan_create(c, m, prm, &p);
+  if (rc) return rc;
+  rc = wtb_bse_plan_execute(p);
+  if (!rc) rc = wtb_bse_plan_results(p, w, act, eps, egap);
+  wtb_bse_plan_destroy(p);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+int wtb_zheev(wtb_ctx* c, int want_vectors, int n, double* a, int lda, double* w) {
+  if (!c || !a || !w || n < 1 || lda < n) {
+    set_error("wtb_zheev: bad argument");
+    return WTB_ERR_ARG;
+  }
+  if (want_vectors) {
+    set_error("wtb_zheev: eigenvectors are not produced by this build (use want_vectors = 0)");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  zc *da = nullptr, *dh = nullptr;
+  void *tri = nullptr, *dcw = nullptr;
+  double* dl = nullptr;
+  int rc;
+  if ((rc = dmalloc(&da, (size_t)lda * n)) || (rc = dmalloc(&dh, (size_t)n * n)) ||
+      (rc = dmalloc((char**)&tri, tridiag_work_bytes(n, 32, 0))) ||
+      (rc = dmalloc((char**)&dcw, dc_work_bytes(n, 0))) || (rc = dmalloc(&dl, n))) {
+  } else {
+    TridiagWork tw;
+    DCTree tree;
+    tridiag_bind(tw, tri, n, 32, 0);
+    rc = dc_prepare(dcw, n, 0, tree);
+    if (!rc) {
+      hipMemcpyAsync(da, a, sizeof(zc) * lda * n, hipMemcpyHostToDevice, s);
+      hipLaunchKernelGGL(k_upper_to_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, da, lda, dh, n);
+      rc = run_tridiag(s, tw, dh, n, nullptr);
+    }
+    if (!rc) rc = run_dc(s, dcw, tree, tw.d, tw.e, nullptr, dl);
+    if (!rc) {
+      hipMemcpyAsync(w, dl, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) {
+        set_error("wtb_zheev: stream error");
+        rc = WTB_ERR_HIP;
+      }
+    }
+  }
+  hipFree(da);
+  hipFree(dh);
+  hipFree(tri);
+  hipFree(dcw);
+  hipFree(dl);
+  return rc;
+}
+
+static int xth_call(wtb_ctx* c, int n, const double* x, const double* h1, const double* h2, double* act) {
+  if (!c || !x || !h1 || !act || n < 1) {
+    set_error("dielbse: bad argument");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  zc *dx = nullptr, *d1 = nullptr, *d2 = nullptr;
+  double* da = nullptr;
+  int rc;
+  if ((rc = dmalloc(&dx, (size_t)n * n)) || (rc = dmalloc(&d1, n)) || (rc = dmalloc(&d2, n)) ||
+      (rc = dmalloc(&da, n))) {
+  } else {
+    hipMemcpyAsync(dx, x, sizeof(zc) * n * n, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(d1, h1, sizeof(zc) * n, hipMemcpyHostToDevice, s);
+    if (h2) hipMemcpyAsync(d2, h2, sizeof(zc) * n, hipMemcpyHostToDevice, s);
+    rc = launch_xth(s, n, dx, d1, h2 ? d2 : nullptr, da);
+    if (!rc) {
+      hipMemcpyAsync(act, da, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+    }
+  }
+  hipFree(dx);
+  hipFree(d1);
+  hipFree(d2);
+  hipFree(da);
+  return rc;
+}
+
+int wtb_dielbsev(wtb_ctx* c, int n, const double* x, const double* h, double* act) {
+  return xth_call(c, n, x, h, nullptr, act);
+}
+int wtb_dielbsep(wtb_ctx* c, int n, const double* x, const double* h1, const double* h2, double* act) {
+  if (!h2) {
+    set_error("wtb_dielbsep: null h2");
+    return WTB_ERR_ARG;
+  }
+  return xth_call(c, n, x, h1, h2, act);
+}
+
+int wtb_bse_spectrum(wtb_ctx* c, int n, const double* e, const double* act6, const double* rlat,
+                     const int ngrid[3], double ebse0, double ebsef, int numbse, double sme, int renorm,
+                     double* eps) {
+  if (!c || !e || !act6 || !rlat || !ngrid || !eps || n < 1 || numbse < 2) {
+    set_error("wtb_bse_spectrum: bad argument");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double *de = nullptr, *da = nullptr, *dw = nullptr, *deps = nullptr;
+  int rc;
+  if ((rc = dmalloc(&de, n)) || (rc = dmalloc(&da, (size_t)6 * n)) || (rc = dmalloc(&dw, 4)) ||
+      (rc = dmalloc(&deps, (size_t)12 * numbse))) {
+  } else {
+    hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(da, act6, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s);
+    double nkf = (double)(float)(ngrid[0] * ngrid[1] * ngrid[2]);
+    rc = launch_spectrum(s, n, de, da, vcell3d_host(rlat), nkf, ebse0, ebsef, numbse, sme, renorm, dw, deps);
+    if (!rc) {
+      hipMemcpyAsync(eps, deps, sizeof(double) * 12 * numbse, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+    }
+  }
+  hipFree(de);
+  hipFree(da);
+  hipFree(dw);
+  hipFree(deps);
+  return rc;
+}
+
+}  // extern "C"

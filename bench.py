@@ -1,0 +1,194 @@
+"""Benchmark of the BSE pipeline on one MI355X.
+
+Workload (one "step" = one BSE matrix, build + solve + eps(omega)):
+seeded random 2D hexagonal Wannier model, a = 3.2 A, nwann = 22, |R| <= 3a
+(37 R vectors), hoppings N(0,1) exp(-|R|/1.2) eV, ~2 eV gap; 36x36x1
+Monkhorst-Pack grid; nv = nc = 2 -> BSE dimension 5184; Rytova-Keldysh
+potential r0 = 40 A, eps = 1; eps(omega) on 0-6 eV, 1001 points, CSHIFT 0.05.
+The timed region runs, with the model already resident in HBM: k grid, H(k)
+phase GEMM, per-k eigensolves, dipoles, W table, BSE assembly, Hermitian
+eigensolve, activities and eps(omega).
+
+    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
+
+Prints one JSON line.  The CPU baseline is the reference's own code (its
+subroutines compiled from source with flang + MKL, oracle/_ref/ref_dump.x) on
+the same model, timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+CONFIG = dict(name="hex2d_nw22_36x36x1_nv2nc2", seed=2, nw=22, ngrid=(36, 36, 1), nc=2, nv=2,
+              coultype="V2DRK", r0=40.0, ediel=(1.0, 1.0, 1.0), lc=1.0, ez=1.0, w=0.0, ktol=0.001,
+              cshift=0.05, ebse0=0.0, ebsef=6.0, numbse=1001, dtfull=True, renorm=True,
+              shift=(0.0, 0.0, 0.0))
+SMALL = dict(CONFIG, name="hex2d_nw8_12x12x1_nv2nc2", seed=1, nw=8, ngrid=(12, 12, 1))
+HBM_PEAK = 8000.0       # GB/s
+FP64_PEAK = 78.6        # TFLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_model(cfg):
+    from wantibexos_amd import tbmodel
+    return tbmodel.hex2d(cfg["seed"], cfg["nw"], cfg["ngrid"])
+
+
+def params(cfg, precision):
+    from wantibexos_amd import bse
+    return bse.BSEParams(ngrid=cfg["ngrid"], mshift=cfg["shift"], nc=cfg["nc"], nv=cfg["nv"],
+                         coultype=cfg["coultype"], ediel=cfg["ediel"], lc=cfg["lc"], ez=cfg["ez"],
+                         w=cfg["w"], r0=cfg["r0"], ktol=cfg["ktol"], cshift=cfg["cshift"],
+                         ebse0=cfg["ebse0"], ebsef=cfg["ebsef"], numbse=cfg["numbse"],
+                         dtfull=cfg["dtfull"], cpol=True, renorm=cfg["renorm"], f154=True,
+                         precision=bse.MIXED if precision == "mixed" else bse.FP64)
+
+
+def cpu_baseline(cfg, model, max_threads):
+    """The reference's own subroutines (oracle/_ref/ref_dump.x) on the same model."""
+    from wantibexos_amd import tbmodel
+    exe = ROOT / "oracle" / "_ref" / "ref_dump.x"
+    if not exe.exists():
+        return None
+    nthreads = max(1, min(max_threads, os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "out").mkdir()
+        tbmodel.write_tb(model, td / "tb.dat")
+        (td / "input.dat").write_text(tbmodel.input_dat(cfg, "tb.dat", nthreads=nthreads))
+        env = dict(os.environ, OMP_NUM_THREADS=str(nthreads), MKL_NUM_THREADS=str(nthreads))
+        t0 = time.perf_counter()
+        r = subprocess.run([str(exe)], stdin=open(td / "input.dat"), cwd=td, env=env,
+                           capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        log("cpu baseline failed:", r.stderr[-800:])
+        return None
+    m = re.search(r"TIMES.*", r.stdout)
+    vals = dict(re.findall(r"(\w+)=\s*([0-9.Ee+-]+)", m.group(0)))
+    stages = {k: float(vals[k]) for k in ("ab", "dip", "build", "diag", "osc", "eps")}
+    return dict(value=sum(stages.values()), unit="s/matrix", cores=nthreads, kind="reference",
+                sample=(f"one full {cfg['name']} matrix (N={int(vals['N'])}) through the reference's "
+                        f"own subroutines (flang -O2 + MKL ZHEEV, LWORK as the reference), "
+                        f"stage sums excl. file I/O; wall {wall:.1f}s"),
+                stages=stages)
+
+
+def tridiag_bytes(N, nb=32):
+    """Algorithmic HBM bytes of the one-stage reduction: each Hermitian GEMV reads the
+    lower triangle of its trailing matrix once; each panel update reads and writes the
+    lower triangle of its trailing matrix once."""
+    hemv = sum(16.0 * (N - j - 1) * (N - j) / 2 for j in range(N - 1))
+    her2k = sum(2 * 16.0 * (N - s) * (N - s + 1) / 2 for s in range(nb, N, nb))
+    return hemv, her2k
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--config", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    from wantibexos_amd import bse
+    cfg = CONFIG if args.config == "c2" else SMALL
+    t0 = time.perf_counter()
+    model = make_model(cfg)
+    log(f"model {cfg['name']} generated in {time.perf_counter() - t0:.1f}s")
+    prm = params(cfg, args.precision)
+    ctx = bse.Context(0)
+    plan = bse.BSEPlan(ctx, model, prm)
+    N = plan.N
+    log(f"plan N={N} created")
+    for _ in range(args.warmup):
+        plan.execute()
+    res = plan.results()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    res = plan.results()
+    log(f"timed {args.steps} steps: {dt * 1e3:.2f} ms/step")
+    # per-stage device times (HIP events on the context stream), separate run
+    plan.set_timing(True)
+    plan.execute()
+    plan.results()
+    stage = plan.stage_ms()
+    plan.set_timing(False)
+    log("stage ms:", json.dumps({k: round(v, 3) for k, v in stage.items()}))
+    # size-independent checks of the result
+    eigv = plan.fetch("eigv")
+    dip = plan.fetch("dipole")
+    nv = prm.nv
+    dE = (eigv[:, nv:][:, :, None] - eigv[:, :nv][:, None, :]).reshape(-1)
+    checks = dict(
+        trace_rel=abs(res["W"].sum() - dE.sum()) / abs(dE.sum()),           # sum W = trace H (V(0)=0)
+        sumrule_xx_rel=abs(res["act"][0].sum() - np.vdot(dip[0], dip[0]).real) / np.vdot(dip[0], dip[0]).real,
+        sumrule_yy_rel=abs(res["act"][1].sum() - np.vdot(dip[1], dip[1]).real) / np.vdot(dip[1], dip[1]).real,
+        sorted=bool(np.all(np.diff(res["W"]) >= 0)),
+        ground_state_eV=float(res["W"][0]),
+    )
+    hemv_b, her2k_b = tridiag_bytes(N)
+    tri_ms = stage["tridiag"]
+    ach = (hemv_b + her2k_b) / (tri_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",
+                    frac=round(ach / HBM_PEAK, 4), traffic=None,
+                    kernel="tridiagonal reduction (k_tri_hemv + k_tri_her2k, stage total)")
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: running the reference's subroutines on the same model ...")
+        cpu = cpu_baseline(cfg, model, args.cpu_threads)
+        if cpu:
+            log(f"cpu baseline {cpu['value']:.2f} s/matrix on {cpu['cores']} threads: {cpu['stages']}")
+    plan.close()
+    ctx.close()
+    out = {
+        "metric": "BSE build+solve s/matrix, mixed vs FP64 plain, at <=1 meV exciton error",
+        "value": round(dt, 6),
+        "unit": "s/matrix",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if args.precision == "fp64" else "c64/c128",
+        "data": "synthetic (seeded random tight-binding model)",
+        "config": {"workload": cfg["name"], "bse_dim": N, "nwann": cfg["nw"],
+                   "kgrid": list(cfg["ngrid"]), "potential": cfg["coultype"],
+                   "precision": args.precision, "n_omega": cfg["numbse"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stage_ms": {k: round(v, 3) for k, v in stage.items()},
+        "checks": checks,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

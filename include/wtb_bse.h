@@ -104,6 +104,7 @@ int wtb_destroy(wtb_ctx* ctx);
 const char* wtb_last_error(void);
 int wtb_version(void);
 void* wtb_stream(wtb_ctx* ctx);             /* the context's hipStream_t             */
+int wtb_sync(wtb_ctx* ctx);                 /* wait for all work on the context      */
 
 /* ---- single stages ---------------------------------------------------------- */
 
@@ -163,6 +164,11 @@ int wtb_bse_plan_create(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_p
                         wtb_bse_plan** plan);
 int wtb_bse_plan_dim(const wtb_bse_plan* plan);
 int wtb_bse_plan_execute(wtb_bse_plan* plan);
+/* Stages up to the BSE matrix only (k grid, H(k), per-k eig, dipoles, W table,
+ * assembly): afterwards WTB_BUF_HBSE holds the matrix the reference hands to
+ * ZHEEV (bse_solver-tool-diel.f90:458), lower triangle.  execute() later
+ * overwrites it during the reduction. */
+int wtb_bse_plan_build(wtb_bse_plan* plan);
 int wtb_bse_plan_results(wtb_bse_plan* plan, double* w, double* act, double* eps,
                          double* egap);
 /* Copy an intermediate device buffer to the host (tests, diagnostics).

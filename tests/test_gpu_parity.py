@@ -69,6 +69,8 @@ def test_pipeline_against_reference(ctx, name):
     m, p, z = load(name)
     prm = params_of(p)
     with bse.BSEPlan(ctx, m, prm) as plan:
+        plan.build()
+        H = plan.fetch("hbse")
         plan.execute()
         r = plan.results()
         kpt = plan.fetch("kpt")
@@ -76,7 +78,6 @@ def test_pipeline_against_reference(ctx, name):
         eigv = plan.fetch("eigv")
         vec = plan.fetch("vector")
         dip = plan.fetch("dipole")
-        H = plan.fetch("hbse")
     assert np.array_equal(kpt, z["kpt"])
     assert np.array_equal(nocp, z["nocpk"])
     assert np.abs(eigv - z["eigv"]).max() < E_TOL * 1e-3

@@ -76,6 +76,7 @@ def _load():
         "wtb_last_error": (C.c_char_p, []),
         "wtb_version": (ip, []),
         "wtb_stream": (vp, [vp]),
+        "wtb_sync": (ip, [vp]),
         "wtb_monkhorst_pack": (ip, [vp, vp, vp, vp, vp]),
         "wtb_eigsys": (ip, [vp, vp, ip, vp, vp, vp, vp]),
         "wtb_zheev": (ip, [vp, ip, ip, vp, ip, vp]),
@@ -87,6 +88,7 @@ def _load():
         "wtb_bse_plan_create": (ip, [vp, vp, vp, C.POINTER(vp)]),
         "wtb_bse_plan_dim": (ip, [vp]),
         "wtb_bse_plan_execute": (ip, [vp]),
+        "wtb_bse_plan_build": (ip, [vp]),
         "wtb_bse_plan_results": (ip, [vp, vp, vp, vp, vp]),
         "wtb_bse_plan_buffer_bytes": (C.c_size_t, [vp, ip]),
         "wtb_bse_plan_fetch": (ip, [vp, ip, vp, C.c_size_t]),
@@ -204,6 +206,9 @@ class Context:
     def stream(self):
         return lib().wtb_stream(self.h)
 
+    def sync(self):
+        _check(lib().wtb_sync(self.h))
+
 
 def monkhorst_pack(ctx: Context, ngrid, shift, rlat):
     ng = np.array(ngrid, dtype=np.int32)
@@ -280,6 +285,10 @@ class BSEPlan:
 
     def execute(self):
         _check(lib().wtb_bse_plan_execute(self.h))
+
+    def build(self):
+        """Stages up to the BSE matrix only; fetch('hbse') then holds it."""
+        _check(lib().wtb_bse_plan_build(self.h))
 
     def set_timing(self, on=True):
         _check(lib().wtb_bse_plan_set_timing(self.h, int(on)))

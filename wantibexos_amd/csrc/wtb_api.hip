@@ -158,6 +158,12 @@ int wtb_destroy(wtb_ctx* c) {
 
 void* wtb_stream(wtb_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int wtb_sync(wtb_ctx* c) {
+  if (!c) return WTB_ERR_ARG;
+  WTB_HIP(hipStreamSynchronize(c->stream));
+  return WTB_OK;
+}
+
 // ---------------------------------------------------------------------------
 int wtb_monkhorst_pack(wtb_ctx* c, const int ngrid[3], const double shift[3], const double* rlat,
                        double* kpt) {
@@ -431,7 +437,7 @@ int wtb_bse_plan_stage_ms(const wtb_bse_plan* p, double* ms) {
   return WTB_OK;
 }
 
-static int plan_enqueue(wtb_bse_plan* p) {
+static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   hipStream_t s = p->ctx->stream;
   const wtb_bse_params& P = p->prm;
   const int nk = p->nk, nw = p->nw, N = p->N;
@@ -457,6 +463,7 @@ static int plan_enqueue(wtb_bse_plan* p) {
                              (int)p->ldh, nullptr)))
     return rc;
   mark(3);
+  if (build_only) return WTB_OK;
   // (d) Householder tridiagonalisation, g <- Q^T h
   WTB_HIP(hipMemcpyAsync(p->g, p->dip, sizeof(zc) * 3 * N, hipMemcpyDeviceToDevice, s));
   if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, p->g))) return rc;
@@ -489,6 +496,18 @@ int wtb_bse_plan_execute(wtb_bse_plan* p) {
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
   return plan_enqueue(p);
+}
+
+int wtb_bse_plan_build(wtb_bse_plan* p) {
+  if (!p) {
+    set_error("wtb_bse_plan_build: null plan");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  int rc = plan_enqueue(p, true);
+  if (rc) return rc;
+  WTB_HIP(hipStreamSynchronize(p->ctx->stream));
+  return WTB_OK;
 }
 
 int wtb_bse_plan_results(wtb_bse_plan* p, double* w, double* act, double* eps, double* egap) {

@@ -1,0 +1,130 @@
+! ISO_C_BINDING interface to libwtb_bse.so (include/wtb_bse.h).
+!
+! The derived types mirror wtb_tb_model / wtb_bse_params field by field; the
+! interfaces bind the C entry points one to one.  The drop-in subroutines with
+! the reference's names and argument lists are in bse_solver_gpu.f90.
+module wtb_gpu
+	use iso_c_binding
+	implicit none
+
+	integer(c_int), parameter :: WTB_OK = 0
+	integer(c_int), parameter :: WTB_FP64 = 0, WTB_MIXED = 1
+
+	type, bind(C) :: wtb_tb_model
+		integer(c_int) :: nw, nvec
+		type(c_ptr) :: rlat, rvec, ffactor, hop, ihop
+		real(c_double) :: efermi, scs
+	end type
+
+	type, bind(C) :: wtb_bse_params
+		integer(c_int) :: ngrid(3)
+		real(c_double) :: mshift(3)
+		integer(c_int) :: nc, nv, coultype
+		real(c_double) :: ediel(3)
+		real(c_double) :: lc, ez, w, r0, ktol, cshift, ebse0, ebsef
+		integer(c_int) :: numbse, dtfull, cpol, renorm, f154, precision
+	end type
+
+	interface
+		integer(c_int) function wtb_create(device, ctx) bind(C, name="wtb_create")
+			import :: c_int, c_ptr
+			integer(c_int), value :: device
+			type(c_ptr) :: ctx
+		end function
+		integer(c_int) function wtb_destroy(ctx) bind(C, name="wtb_destroy")
+			import :: c_int, c_ptr
+			type(c_ptr), value :: ctx
+		end function
+		type(c_ptr) function wtb_last_error() bind(C, name="wtb_last_error")
+			import :: c_ptr
+		end function
+		integer(c_int) function wtb_eigsys(ctx, model, nk, kpt, energias, autovetores, nocp) &
+				bind(C, name="wtb_eigsys")
+			import :: c_int, c_ptr, c_double, wtb_tb_model
+			type(c_ptr), value :: ctx
+			type(wtb_tb_model) :: model
+			integer(c_int), value :: nk
+			real(c_double) :: kpt(*), energias(*)
+			complex(c_double_complex) :: autovetores(*)
+			integer(c_int) :: nocp(*)
+		end function
+		integer(c_int) function wtb_bsesolver(ctx, model, prm, w, act, eps, egap) &
+				bind(C, name="wtb_bsesolver")
+			import :: c_int, c_ptr, c_double, wtb_tb_model, wtb_bse_params
+			type(c_ptr), value :: ctx
+			type(wtb_tb_model) :: model
+			type(wtb_bse_params) :: prm
+			real(c_double) :: w(*), act(*)
+			type(c_ptr), value :: eps
+			real(c_double) :: egap
+		end function
+		integer(c_int) function wtb_bse_spectrum(ctx, n, e, act6, rlat, ngrid, ebse0, ebsef, &
+				numbse, sme, renorm, eps) bind(C, name="wtb_bse_spectrum")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: n
+			real(c_double) :: e(*), act6(*), rlat(*)
+			integer(c_int) :: ngrid(3)
+			real(c_double), value :: ebse0, ebsef
+			integer(c_int), value :: numbse
+			real(c_double), value :: sme
+			integer(c_int), value :: renorm
+			real(c_double) :: eps(*)
+		end function
+		integer(c_int) function wtb_zheev(ctx, want_vectors, n, a, lda, w) bind(C, name="wtb_zheev")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: want_vectors, n, lda
+			complex(c_double_complex) :: a(*)
+			real(c_double) :: w(*)
+		end function
+	end interface
+
+contains
+
+	! COULOMB_POT string -> WTB_* selector (the cases of matrizelbse, bse_subs.f90:45-91)
+	integer(c_int) function wtb_coulomb_id(coultype)
+		character(len=*) :: coultype
+		select case (trim(adjustl(coultype)))
+		case ("V2DK"); wtb_coulomb_id = 1
+		case ("V3D"); wtb_coulomb_id = 2
+		case ("V3DL"); wtb_coulomb_id = 3
+		case ("V2DT"); wtb_coulomb_id = 4
+		case ("V2DT2"); wtb_coulomb_id = 5
+		case ("V2DOH"); wtb_coulomb_id = 6
+		case ("V2DRK"); wtb_coulomb_id = 7
+		case ("V1DT"); wtb_coulomb_id = 8
+		case ("V0DT"); wtb_coulomb_id = 9
+		case default; wtb_coulomb_id = -1
+		end select
+	end function
+
+	! map a status to the reference's STOP text and stop, as the reference does
+	subroutine wtb_check(rc)
+		integer(c_int) :: rc
+		character(len=256) :: msg
+		character(kind=c_char), pointer :: cm(:)
+		integer :: i
+		if (rc == WTB_OK) return
+		msg = ""
+		call c_f_pointer(wtb_last_error(), cm, [256])
+		do i = 1, 256
+			if (cm(i) == c_null_char) exit
+			msg(i:i) = cm(i)
+		end do
+		select case (rc)
+		case (4)
+			write(*,*) "Wrong Coulomb Potential"
+		case (5)
+			write(*,*) "Number of valence states higher then ocuppied states"
+		case (6)
+			write(*,*) "Number of conduction states higher then unocuppied states"
+		case (8)
+			write(*,*) 'The algorithm failed to compute eigenvalues.'
+		case default
+			write(*,*) "wtb_gpu error ", rc, ": ", trim(msg)
+		end select
+		stop
+	end subroutine
+
+end module wtb_gpu

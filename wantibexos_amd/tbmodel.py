@@ -134,6 +134,27 @@ def read_tb(path) -> TBModel:
     return TBModel(systype, scs, efermi, rlat, rvec, nd, hop, ihop, comment)
 
 
+def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -> str:
+    """The reference's stdin "KEY= value" input (module_input_read.f90:364-658) for a
+    BSE + spectrum run with the parameters of `p` (keys as in the oracle's params)."""
+    T = lambda b: "T" if b else "F"
+    lines = [
+        f"NTHREADS= {nthreads}", f'OUTPUT= "{outdir}"', f'CALC_DATA= "{outdir}"',
+        f'PARAMS_FILE= "{tbname}"',
+        f"NGX= {p['ngrid'][0]}", f"NGY= {p['ngrid'][1]}", f"NGZ= {p['ngrid'][2]}",
+        "BSE= T", "SPEC= T", f"DTDIAG= {T(p['dtfull'])}", "CPOL= T",
+        f"COULOMB_POT= {p['coultype']}",
+        f"R_0= {p['r0']!r}", f"LC= {p['lc']!r}", f"W_COUL= {p['w']!r}",
+        f"EDIEL_T= {p['ediel'][0]!r}", f"EDIEL= {p['ediel'][1]!r}", f"EDIEL_B= {p['ediel'][2]!r}",
+        f"EDIEL_Z= {p['ez']!r}",
+        f"NBANDSC= {p['nc']}", f"NBANDSV= {p['nv']}", f"KTOL= {p['ktol']!r}",
+        f"CSHIFT= {p['cshift']!r}", f"ENSPECI= {p['ebse0']!r}", f"ENSPECF= {p['ebsef']!r}",
+        f"NESPEC= {p['numbse']}", f"RNMD= {T(p['renorm'])}",
+        f"SHIFT_1= {p['shift'][0]!r}", f"SHIFT_2= {p['shift'][1]!r}", f"SHIFT_3= {p['shift'][2]!r}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
 # ---------------------------------------------------------------------------
 # seeded synthetic models
 # ---------------------------------------------------------------------------

@@ -90,12 +90,12 @@ def cpu_baseline(cfg, model, max_threads):
                 stages=stages)
 
 
-def tridiag_bytes(N, nb=32):
+def tridiag_bytes(N, nb=32, elem=16.0):
     """Algorithmic HBM bytes of the one-stage reduction: each Hermitian GEMV reads the
     lower triangle of its trailing matrix once; each panel update reads and writes the
     lower triangle of its trailing matrix once."""
-    hemv = sum(16.0 * (N - j - 1) * (N - j) / 2 for j in range(N - 1))
-    her2k = sum(2 * 16.0 * (N - s) * (N - s + 1) / 2 for s in range(nb, N, nb))
+    hemv = sum(elem * (N - j - 1) * (N - j) / 2 for j in range(N - 1))
+    her2k = sum(2 * elem * (N - s) * (N - s + 1) / 2 for s in range(nb, N, nb))
     return hemv, her2k
 
 
@@ -152,7 +152,17 @@ def main():
         sorted=bool(np.all(np.diff(res["W"]) >= 0)),
         ground_state_eV=float(res["W"][0]),
     )
-    hemv_b, her2k_b = tridiag_bytes(N)
+    if args.precision == "mixed":
+        # the same workload in the plain FP64 form: the precision form's error bar
+        with bse.BSEPlan(ctx, model, params(cfg, "fp64")) as p64:
+            p64.execute()
+            r64 = p64.results()
+        checks["mixed_vs_fp64_max_dW_eV"] = float(np.abs(res["W"] - r64["W"]).max())
+        checks["mixed_vs_fp64_eps_rel"] = float(max(
+            np.abs(res["eps"][c, q] - r64["eps"][c, q]).max() / max(np.abs(r64["eps"][c, q]).max(), 1e-30)
+            for c in range(6) for q in range(2) if np.abs(r64["eps"][c, q]).max() > 1e-8))
+        log("mixed vs fp64:", json.dumps({k: checks[k] for k in checks if k.startswith("mixed")}))
+    hemv_b, her2k_b = tridiag_bytes(N, elem=8.0 if args.precision == "mixed" else 16.0)
     tri_ms = stage["tridiag"]
     ach = (hemv_b + her2k_b) / (tri_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",

@@ -193,6 +193,10 @@ int wtb_bse_plan_fetch(wtb_bse_plan* plan, int what, void* dst, size_t bytes);
  * [4] tridiagonal eigensolve, [5] activities + eps(omega). */
 enum { WTB_NSTAGES = 6 };
 int wtb_bse_plan_set_timing(wtb_bse_plan* plan, int on);
+/* execute() captures the pipeline into a HIP graph on its first call and
+ * replays it afterwards (default on); off launches the kernels directly.
+ * Timing mode always launches directly. */
+int wtb_bse_plan_set_graph(wtb_bse_plan* plan, int on);
 int wtb_bse_plan_stage_ms(const wtb_bse_plan* plan, double* ms);
 int wtb_bse_plan_destroy(wtb_bse_plan* plan);
 

@@ -111,7 +111,7 @@ def test_pipeline_against_reference(ctx, name):
     assert abs(r["egap"] - z["egap"][0]) < 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 63, 64, 65, 100, 129, 300])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 63, 64, 65, 100, 129, 300, 1000, 2100])
 def test_zheev_eigenvalues(ctx, n):
     from wantibexos_amd import bse
     rng = np.random.default_rng(n)
@@ -189,11 +189,15 @@ def test_plan_reexecute_is_reproducible(ctx):
     from wantibexos_amd import bse
     m, p, z = load("hex8_g6_v2drk")
     with bse.BSEPlan(ctx, m, params_of(p)) as plan:
-        plan.execute()
+        plan.execute()                 # captures the HIP graph and replays it
         r1 = plan.results()
         plan.execute()
         plan.execute()
         r2 = plan.results()
-    assert np.array_equal(r1["W"], r2["W"])
-    assert np.array_equal(r1["act"], r2["act"])
-    assert np.array_equal(r1["eps"], r2["eps"])
+        plan.set_graph(False)          # direct launches, same kernels
+        plan.execute()
+        r3 = plan.results()
+    for r in (r2, r3):
+        assert np.array_equal(r1["W"], r["W"])
+        assert np.array_equal(r1["act"], r["act"])
+        assert np.array_equal(r1["eps"], r["eps"])

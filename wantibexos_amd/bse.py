@@ -93,6 +93,7 @@ def _load():
         "wtb_bse_plan_buffer_bytes": (C.c_size_t, [vp, ip]),
         "wtb_bse_plan_fetch": (ip, [vp, ip, vp, C.c_size_t]),
         "wtb_bse_plan_set_timing": (ip, [vp, ip]),
+        "wtb_bse_plan_set_graph": (ip, [vp, ip]),
         "wtb_bse_plan_stage_ms": (ip, [vp, vp]),
         "wtb_bse_plan_destroy": (ip, [vp]),
     }
@@ -293,6 +294,9 @@ class BSEPlan:
     def set_timing(self, on=True):
         _check(lib().wtb_bse_plan_set_timing(self.h, int(on)))
 
+    def set_graph(self, on=True):
+        _check(lib().wtb_bse_plan_set_graph(self.h, int(on)))
+
     def results(self):
         N, nb = self.N, int(self.params.numbse)
         W = np.zeros(N)
@@ -326,8 +330,9 @@ class BSEPlan:
             return raw.view(np.complex128).reshape(nk, nb, -1)
         if what == "dipole":
             return raw.view(np.complex128).reshape(3, N)
-        if what == "hbse":
-            return raw.view(np.complex128).reshape(N, N).T     # column-major N x N (lower valid)
+        if what == "hbse":                                       # column-major N x N (lower valid)
+            dt = np.complex64 if p.precision == MIXED else np.complex128
+            return raw.view(dt).reshape(N, N).T
         if what == "wtab":
             return raw.view(np.float64).reshape(2 * p.ngrid[0] - 1, 2 * p.ngrid[1] - 1, 2 * p.ngrid[2] - 1)
         if what == "tridiag":

@@ -204,37 +204,58 @@ __device__ inline zc load_band(const zc* vec, const BuildArgs& ba, int k, int ba
 }
 
 // Overlap GEMM for one 16x16 block (complex), rows r -> (k0r + r/nbnd, band0 + r%nbnd)
+// F32: the precision form, FP32 operands and accumulation on v_mfma_f32_16x16x4_f32
+template <bool F32>
 __device__ inline void overlap_block(const zc* __restrict__ vec, const BuildArgs& ba, int kr0, int kc0,
                                      int nbnd, int band0, int rows, int rb, int cb, zc* out, int ldo) {
   const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
   int r = rb * 16 + lr;          // A row
   int c = cb * 16 + lr;          // B column
   int kr = kr0 + r / nbnd, br = band0 + r % nbnd;
   int kc = kc0 + c / nbnd, bc = band0 + c % nbnd;
   bool rv = r < rows && kr < ba.nk;
   bool cv = c < rows && kc < ba.nk;
-  for (int m0 = 0; m0 < ba.nw; m0 += 4) {
-    int m = m0 + lk;
-    zc a = (rv && m < ba.nw) ? load_band(vec, ba, kr, br, m) : mkz(0, 0);
-    zc b = (cv && m < ba.nw) ? load_band(vec, ba, kc, bc, m) : mkz(0, 0);
-    // conj(a) * b
-    cr = mfma16x16x4(a.re, b.re, cr);
-    cr = mfma16x16x4(a.im, b.im, cr);
-    ci = mfma16x16x4(a.re, b.im, ci);
-    ci = mfma16x16x4(-a.im, b.re, ci);
-  }
-  for (int q = 0; q < 4; ++q) {
-    int orow = rb * 16 + lk + 4 * q, ocol = cb * 16 + lr;
-    if (orow < rows && ocol < rows) out[orow * ldo + ocol] = mkz(cr[q], ci[q]);
+  if constexpr (!F32) {
+    d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    for (int m0 = 0; m0 < ba.nw; m0 += 4) {
+      int m = m0 + lk;
+      zc a = (rv && m < ba.nw) ? load_band(vec, ba, kr, br, m) : mkz(0, 0);
+      zc b = (cv && m < ba.nw) ? load_band(vec, ba, kc, bc, m) : mkz(0, 0);
+      // conj(a) * b
+      cr = mfma16x16x4(a.re, b.re, cr);
+      cr = mfma16x16x4(a.im, b.im, cr);
+      ci = mfma16x16x4(a.re, b.im, ci);
+      ci = mfma16x16x4(-a.im, b.re, ci);
+    }
+    for (int q = 0; q < 4; ++q) {
+      int orow = rb * 16 + lk + 4 * q, ocol = cb * 16 + lr;   // f64: row lk + 4q
+      if (orow < rows && ocol < rows) out[orow * ldo + ocol] = mkz(cr[q], ci[q]);
+    }
+  } else {
+    f4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    for (int m0 = 0; m0 < ba.nw; m0 += 4) {
+      int m = m0 + lk;
+      zc a = (rv && m < ba.nw) ? load_band(vec, ba, kr, br, m) : mkz(0, 0);
+      zc b = (cv && m < ba.nw) ? load_band(vec, ba, kc, bc, m) : mkz(0, 0);
+      const float ar = (float)a.re, ai = (float)a.im, br_ = (float)b.re, bi = (float)b.im;
+      cr = mfma16x16x4f32(ar, br_, cr);
+      cr = mfma16x16x4f32(ai, bi, cr);
+      ci = mfma16x16x4f32(ar, bi, ci);
+      ci = mfma16x16x4f32(-ai, br_, ci);
+    }
+    for (int q = 0; q < 4; ++q) {
+      int orow = rb * 16 + 4 * lk + q, ocol = cb * 16 + lr;   // f32: row 4lk + q
+      if (orow < rows && ocol < rows) out[orow * ldo + ocol] = mkz(cr[q], ci[q]);
+    }
   }
 }
 
+template <bool F32>
 __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* __restrict__ wtab,
                                                    const double* __restrict__ eigv,
                                                    const zc* __restrict__ vec, zc* __restrict__ H,
-                                                   float* __restrict__ Hf) {
+                                                   zf* __restrict__ Hf) {
   extern __shared__ unsigned char smem[];
   // lower-triangle tile index -> (tr, tc), tr >= tc
   int t = blockIdx.x;
@@ -252,10 +273,10 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
   const int nblocks = bc_ * bc_ + bv_ * bv_;
   for (int blk = wave; blk < nblocks; blk += 4) {
     if (blk < bc_ * bc_)
-      overlap_block(vec, ba, kr0, kc0, ba.nc, ba.nv, rc_, blk / bc_, blk % bc_, Oc, rc_);
+      overlap_block<F32>(vec, ba, kr0, kc0, ba.nc, ba.nv, rc_, blk / bc_, blk % bc_, Oc, rc_);
     else {
       int b2 = blk - bc_ * bc_;
-      overlap_block(vec, ba, kr0, kc0, ba.nv, 0, rv_, b2 / bv_, b2 % bv_, Ov, rv_);
+      overlap_block<F32>(vec, ba, kr0, kc0, ba.nv, 0, rv_, b2 / bv_, b2 % bv_, Ov, rv_);
     }
   }
   __syncthreads();
@@ -283,11 +304,8 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
       val = (wv * oc) * ov;
     }
     size_t off = (size_t)I + (size_t)ba.ldh * J;
-    if (H) H[off] = val;
-    if (Hf) {
-      Hf[2 * off] = (float)val.re;
-      Hf[2 * off + 1] = (float)val.im;
-    }
+    if constexpr (F32) stz(Hf[off], val);
+    else H[off] = val;
   }
 }
 
@@ -300,7 +318,7 @@ int launch_wtable(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, do
 
 int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng[3],
                      const double* d_wtab, const double* d_eigv, const zc* d_vec, zc* d_H, int ldh,
-                     float* d_Hf) {
+                     zf* d_Hf) {
   BuildArgs ba;
   ba.nk = nk;
   ba.nc = nc;
@@ -323,8 +341,12 @@ int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng
   ba.ntiles_side = side;
   int ntiles = side * (side + 1) / 2;
   size_t lds = sizeof(zc) * ((size_t)tk * nc * tk * nc + (size_t)tk * nv * tk * nv);
-  hipLaunchKernelGGL(k_bse_build, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec, d_H,
-                     d_Hf);
+  if (d_H)
+    hipLaunchKernelGGL(k_bse_build<false>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
+                       d_H, (zf*)nullptr);
+  if (d_Hf)
+    hipLaunchKernelGGL(k_bse_build<true>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
+                       (zc*)nullptr, d_Hf);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

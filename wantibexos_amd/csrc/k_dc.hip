@@ -108,22 +108,43 @@ __global__ void k_dc_gather(DCArgs a) {
   for (int q = 0; q < a.nt; ++q) a.sx[(size_t)(2 + q) * N + dst] = a.T[(size_t)q * N + src];
 }
 
-// M2: deflation scan (DLAED2), one thread per node.  Decides, for each sorted
-// position, its slot (non-deflated first), applies the Givens rotations of
-// close-pole deflation to the tracked sources in place, and writes the poles
-// and weights of the secular equation.
-__global__ void k_dc_deflate(DCArgs a) {
-  const int nodeid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nodeid >= a.nnodes) return;
+// M2: deflation scan (DLAED2), one workgroup per node.  The poles and weights
+// are staged in LDS (when they fit) for the sequential scan of thread 0, which
+// decides each sorted position's slot (non-deflated first), applies the Givens
+// rotations of close-pole deflation to the tracked sources in place, and writes
+// the poles and weights of the secular equation.
+__global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
+  extern __shared__ double lds_dz[];
+  __shared__ double red[2][256];
+  const int nodeid = blockIdx.x;
   const DCNode nd = a.nodes[nodeid];
   const int lo = nd.lo, n = nd.hi - nd.lo;
   const int N = a.N, nx = 2 + a.nt;
   const double rho = fabs(2.0 * a.e[nd.mid - 1]);
-  double zmax = 0.0, dmax = 0.0;
-  for (int i = 0; i < n; ++i) {
-    zmax = fmax(zmax, fabs(a.sz[lo + i]));
-    dmax = fmax(dmax, fabs(a.sD[lo + i]));
+  double zm = 0.0, dm = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double zi = a.sz[lo + i], di = a.sD[lo + i];
+    zm = fmax(zm, fabs(zi));
+    dm = fmax(dm, fabs(di));
+    if (use_lds) {
+      lds_dz[i] = di;
+      lds_dz[n + i] = zi;
+    }
   }
+  red[0][threadIdx.x] = zm;
+  red[1][threadIdx.x] = dm;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double* D = use_lds ? lds_dz : a.sD + lo;
+  double* Z = use_lds ? lds_dz + n : a.sz + lo;
+  const double zmax = red[0][0], dmax = red[1][0];
   const double tol = 8.0 * kEps * fmax(dmax, zmax);
   a.rho[nodeid] = rho;
   int K = 0, nd_ = 0;
@@ -131,14 +152,14 @@ __global__ void k_dc_deflate(DCArgs a) {
     int s = n - 1 - nd_;
     ++nd_;
     a.slot[lo + i] = s;
-    a.cD[lo + s] = a.sD[lo + i];
+    a.cD[lo + s] = D[i];
   };
   auto keep = [&](int i) {
     int s = K;
     ++K;
     a.slot[lo + i] = s;
-    a.cD[lo + s] = a.sD[lo + i];
-    a.cw[lo + s] = a.sz[lo + i];
+    a.cD[lo + s] = D[i];
+    a.cw[lo + s] = Z[i];
   };
   if (rho * zmax <= tol) {
     for (int i = 0; i < n; ++i) deflate(i);
@@ -147,7 +168,7 @@ __global__ void k_dc_deflate(DCArgs a) {
   }
   int j = 0, pj = -1;
   for (; j < n; ++j) {
-    if (rho * fabs(a.sz[lo + j]) <= tol) deflate(j);
+    if (rho * fabs(Z[j]) <= tol) deflate(j);
     else {
       pj = j;
       ++j;
@@ -157,29 +178,29 @@ __global__ void k_dc_deflate(DCArgs a) {
   if (pj >= 0) {
     for (; j < n; ++j) {
       const int nj = j;
-      double znj = a.sz[lo + nj];
+      const double znj = Z[nj];
       if (rho * fabs(znj) <= tol) {
         deflate(nj);
         continue;
       }
-      double s = a.sz[lo + pj], c = znj;
-      double tau = hypot(c, s);
-      double t = a.sD[lo + nj] - a.sD[lo + pj];
+      double s = Z[pj], c = znj;
+      const double tau = hypot(c, s);
+      const double t = D[nj] - D[pj];
       c = c / tau;
       s = -s / tau;
       if (fabs(t * c * s) <= tol) {
-        a.sz[lo + nj] = tau;
-        a.sz[lo + pj] = 0.0;
+        Z[nj] = tau;
+        Z[pj] = 0.0;
         for (int q = 0; q < nx; ++q) {
           double* x = a.sx + (size_t)q * N + lo;
           double xp = x[pj], xn = x[nj];
           x[pj] = c * xp + s * xn;
           x[nj] = c * xn - s * xp;
         }
-        double dp = a.sD[lo + pj], dn = a.sD[lo + nj];
-        double t2 = dp * c * c + dn * s * s;
-        a.sD[lo + nj] = dp * s * s + dn * c * c;
-        a.sD[lo + pj] = t2;
+        const double dp = D[pj], dn = D[nj];
+        const double t2 = dp * c * c + dn * s * s;
+        D[nj] = dp * s * s + dn * c * c;
+        D[pj] = t2;
         deflate(pj);
         pj = nj;
       } else {
@@ -454,6 +475,9 @@ int dc_prepare(void* work, int N, int ntrack, DCTree& tree) {
   tree.N = N;
   tree.ntrack = ntrack;
   DCLayout lay = dc_layout(work, N, ntrack);
+  // the deflation scan stages up to 2 x 8192 doubles in LDS
+  WTB_HIP(hipFuncSetAttribute((const void*)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              2 * 8192 * (int)sizeof(double)));
   if (!flat.empty()) {
     WTB_HIP(hipMemcpy(lay.dnodes, flat.data(), sizeof(DCNode) * flat.size(), hipMemcpyHostToDevice));
   }
@@ -479,7 +503,9 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
     dim3 g1((maxn + 255) / 256, cnt);
     dim3 g16((maxn * 16 + 255) / 256, cnt);
     hipLaunchKernelGGL(k_dc_gather, g1, dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_dc_deflate, dim3((cnt + 63) / 64), dim3(64), 0, s, a);
+    const int use_lds = maxn <= 8192 ? 1 : 0;
+    hipLaunchKernelGGL(k_dc_deflate, dim3(cnt), dim3(256), use_lds ? 2 * sizeof(double) * maxn : 0, s,
+                       a, use_lds);
     hipLaunchKernelGGL(k_dc_compact, g1, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_secular, g16, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_lowner, g16, dim3(256), 0, s, a);

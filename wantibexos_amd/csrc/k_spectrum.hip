@@ -52,12 +52,25 @@ __global__ void k_quantize(int n, const double* __restrict__ src, double* __rest
   if (i < n) dst[i] = f15_4(src[i]);
 }
 
-// avgsme (optics.f90:302-324): sequential sum of successive differences
-__global__ void k_avgsme(int N, const double* __restrict__ E, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s = 0.0;
-  for (int i = 1; i < N; ++i) s = s + E[i] - E[i - 1];
-  out[0] = 10.0 * (s / (double)N);
+// avgsme (optics.f90:302-324): sequential sum of successive differences, in the
+// reference's order; the energies are staged through LDS in chunks
+__global__ __launch_bounds__(256) void k_avgsme(int N, const double* __restrict__ E, double* __restrict__ out) {
+  __shared__ double buf[4097];
+  double s = 0.0, prev = 0.0;
+  for (int c0 = 0; c0 < N; c0 += 4096) {
+    const int m = min(4096, N - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += 256) buf[i] = E[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < m; ++i) {
+        const int gi = c0 + i;
+        if (gi > 0) s = s + buf[i] - prev;
+        prev = buf[i];
+      }
+    }
+  }
+  if (threadIdx.x == 0) out[0] = 10.0 * (s / (double)N);
 }
 
 // optdiel for 4 photon energies per workgroup and all six components:
@@ -117,31 +130,42 @@ __global__ __launch_bounds__(256) void k_optdiel(int N, const double* __restrict
       }
 }
 
-// imagrenorm (optics.f90:327-390) on eps2 of each component; one thread per component
-__global__ void k_imagrenorm(const double* __restrict__ E, double ebse0, double ebsef, int numbse,
-                             double* __restrict__ eps) {
+// imagrenorm (optics.f90:327-390) on eps2 of each component
+__global__ __launch_bounds__(256) void k_imagrenorm(const double* __restrict__ E, double ebse0, double ebsef,
+                                                    int numbse, double* __restrict__ eps) {
 #pragma clang fp contract(off)
-  const int c = threadIdx.x;
-  if (c >= 6) return;
+  __shared__ int sib;
+  __shared__ double saux3;
   const double factor = (double)0.15f;
   const double aux = E[0] - factor;
   if (aux <= 0.0) return;
-  double* im = eps + (size_t)numbse * (1 + 2 * c);
-  int ib = 0;
-  for (int j = 0; j < numbse; ++j) {
-    double w = ebse0 + ((ebsef - ebse0) * (double)j) / ((double)numbse - 1.0);
-    if (w <= aux) ib = j;
-    else break;
+  if (threadIdx.x == 0) {
+    int ib = 0;
+    for (int j = 0; j < numbse; ++j) {
+      double w = ebse0 + ((ebsef - ebse0) * (double)j) / ((double)numbse - 1.0);
+      if (w <= aux) ib = j;
+      else break;
+    }
+    sib = ib;
   }
-  if (ib + 1 >= numbse) {      // the reference would read past the grid; zero it all
-    for (int j = 0; j < numbse; ++j) im[j] = 0.0;
-    return;
-  }
-  const double aux3 = im[ib + 1];
-  for (int j = 0; j <= ib; ++j) im[j] = 0.0;
-  for (int j = ib + 1; j < numbse; ++j) {
-    double v = im[j] - aux3;
-    im[j] = v < 0.0 ? 0.0 : v;
+  __syncthreads();
+  const int ib = sib;
+  for (int c = 0; c < 6; ++c) {
+    double* im = eps + (size_t)numbse * (1 + 2 * c);
+    if (threadIdx.x == 0) saux3 = (ib + 1 < numbse) ? im[ib + 1] : 0.0;
+    __syncthreads();
+    const double aux3 = saux3;
+    for (int j = threadIdx.x; j < numbse; j += 256) {
+      double v;
+      if (ib + 1 >= numbse) v = 0.0;          // the reference would read past the grid; zero it all
+      else if (j <= ib) v = 0.0;
+      else {
+        v = im[j] - aux3;
+        v = v < 0.0 ? 0.0 : v;
+      }
+      im[j] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -186,10 +210,10 @@ int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act
   const double gama = (double)180.90f;
   const double baux = nkf32 * vc;
   const double caux = gama / (baux * 1.0);
-  hipLaunchKernelGGL(k_avgsme, dim3(1), dim3(64), 0, s, N, d_e, d_work);
+  hipLaunchKernelGGL(k_avgsme, dim3(1), dim3(256), 0, s, N, d_e, d_work);
   hipLaunchKernelGGL(k_optdiel, dim3((numbse + OMB - 1) / OMB), dim3(256), 0, s, N, d_e, d_act6, caux,
                      ebse0, ebsef, numbse, sme, d_work, d_eps);
-  if (renorm) hipLaunchKernelGGL(k_imagrenorm, dim3(1), dim3(64), 0, s, d_e, ebse0, ebsef, numbse, d_eps);
+  if (renorm) hipLaunchKernelGGL(k_imagrenorm, dim3(1), dim3(256), 0, s, d_e, ebse0, ebsef, numbse, d_eps);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

@@ -1,42 +1,66 @@
 // Stage (d1): blocked Householder reduction of a Hermitian matrix (lower
-// triangle, column-major, ld) to a real symmetric tridiagonal (d, e), the
-// algorithm of LAPACK ZHETRD/ZLATRD/ZHER2K for UPLO = 'L' (the reference
-// calls ZHEEV, bse_solver-tool-diel.f90:485-488, whose first step this is).
+// triangle, column-major, ld) to a real symmetric tridiagonal (d, e) with the
+// arithmetic of LAPACK ZHETRD/ZLATRD/ZLARFG/ZHER2K for UPLO = 'L' (the
+// reference calls ZHEEV, bse_solver-tool-diel.f90:485-488, whose first step
+// this is), applying Q^T to a few vectors g on the way (Q = H(1)...H(N-1)), so
+// that X^T h = Z^T (Q^T h) never needs the eigenvectors X = Q Z.
 //
-// Per reduced column j (panel column jj = j - p) three kernels run:
-//   A  column update with the panel's V, W; |x|^2 partials; d_j; and the
-//      completion of W(:,jj-1) from the previous column's u.
-//   B  reflector (ZLARFG) from the partials, then y = A22 v on lower tiles of
-//      64x64: each tile is read once and serves A v (rows) and A^H v (cols);
-//      diagonal tiles also produce W^H v, V^H v and v^T g partials.
-//   C  u = y - V (W^H v) - W (V^H v); u^H v partials; g <- g - tau conj(v)(v^T g).
-// After a panel: W(:,last) and the trailing update A22 -= V W^H + W V^H on
-// MFMA (lower tiles only).  g accumulates Q^T g, Q = H(1)...H(N-1), so that
-// the eigenvectors X = Q Z never have to be formed for X^T h.
-// All reductions are written as partials and summed in a fixed order, so the
-// result is reproducible run to run.
+// Two kernels per reduced column j (panel column jj = j - p):
+//   rowstep(j)  rows >= j, 16 rows x 16 lanes per workgroup.  Finishes
+//               reflector j-1 with the scalars hemvstep(j-1) published (v; y =
+//               A22 v from the unscaled GEMV by linearity; u = y - V (W^H v) -
+//               W (V^H v); g <- g - tau conj(v) (v^T g)), then updates column j
+//               with the panel (ZLATRD), leaving out the 2 Re(alpha') v term of
+//               W(:,jj-1) that needs the global u^H v.  The last workgroup to
+//               finish reduces u^H v and publishes alpha'.
+//   hemvstep(j) lower 64x64 tiles of A22 (rows, cols > j).  Adds the deferred
+//               term to column j, completes W(:,jj-1), and forms z = A22 x~ with
+//               x~ = column j below row j+1 (the reflector before its scale is
+//               known).  Each tile is read once and serves both A x~ (rows) and
+//               A^H x~ (cols).  Diagonal tiles also write x_new, d_j and partials
+//               of |x~|^2, W^H x~, V^H x~, x~^T g; the last of them reduces those
+//               and publishes the ZLARFG scalars and W^H v, V^H v, v^T g.
+// Per panel: finish (rowstep without a column), W(:,last), and the trailing
+// update A22 -= V W^H + W V^H on MFMA (lower tiles).
+// "Last workgroup" reductions: partials are written, a device fence and an
+// atomic counter pick the workgroup that arrives last, which sums the partials
+// in a fixed order and resets the counter.  No workgroup ever waits on another.
 #include "wtb_kernels.h"
 
 namespace wtb {
 
 #define TB 64   // tile / row-block edge
+#define RPW 16  // rows per rowstep workgroup
 
+// published scalars
+struct Pub {
+  zc tau, s;
+  zc beta;        // .re
+  zc alpha;       // alpha' of the last finished reflector
+  zc t1[32], t2[32], vg[16];
+};
+
+template <class T>
 struct TriArgs {
-  zc* H;
+  T* H;
   int ld, N, nb;
-  zc* W;       // N x nb, ld N
-  zc* Vp;      // N x nb, ld N
+  zc* W;       // nb x N (column c at W + c*N)
+  zc* Vp;      // nb x N
   zc* u;       // N
   zc* tau;     // N
   double* d;
   double* e;
-  double* rnorm;   // [nblk]
-  zc* ruv;         // [nblk]
-  zc* rt1;         // [nblk][nb]
-  zc* rt2;         // [nblk][nb]
-  zc* rvg;         // [nblk][nrhs]
-  zc* ypart;       // [nblk][nblk][TB]
-  zc* g;           // N x nrhs
+  double* pn;      // [nblk]           |x~|^2
+  zc* puv;         // [N/RPW+1]        u^H v per rowstep workgroup
+  zc* pt1;         // [nblk][nb]       W^H x~
+  zc* pt2;         // [nblk][nb]       V^H x~
+  zc* pvg;         // [nblk][nrhs]     x~^T g
+  zc* ypart;       // [nblk][nblk][TB] z = A22 x~ partials
+  zc* g;           // nrhs x N
+  zc* xn;          // N: reduced column j below its diagonal (x_new)
+  zc* xp;          // N: column j after the panel update without the deferred term
+  Pub* pub;
+  unsigned* cnt;   // [2]: hemvstep, rowstep
   int nrhs;
   int nblk;
 };
@@ -48,242 +72,339 @@ __device__ inline zc wave_sum(zc v) {
   }
   return v;
 }
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
 
-// sum of partials p[b*stride] for b in [b0, nb): every lane of the wave gets it
-__device__ inline zc sum_partials(const zc* p, int b0, int nbk, int stride) {
-  zc acc = mkz(0, 0);
-  for (int b = b0 + (int)(threadIdx.x & 63); b < nbk; b += 64) acc += p[(size_t)b * stride];
-  return wave_sum(acc);
-}
-__device__ inline double sum_partials(const double* p, int b0, int nbk) {
-  double acc = 0;
-  for (int b = b0 + (int)(threadIdx.x & 63); b < nbk; b += 64) acc += p[b];
-  return wave_sum(acc);
-}
-
-// W(r, jj-1) = tau_{j-1} u(r) + alpha' v_{j-1}(r),  alpha' = -1/2 |tau|^2 (u^H v)
-__device__ inline zc w_complete(const TriArgs& a, int r, int jjm1, zc taup, zc alphap) {
-  return taup * a.u[r] + alphap * a.Vp[(size_t)jjm1 * a.N + r];
-}
-
-// ---- kernel A -----------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_tri_colupd(TriArgs a, int j, int jj, int bprev0) {
-  __shared__ zc sWj[64], sVj[64];
-  const int lane = threadIdx.x;
-  const int blk = j / TB + blockIdx.x;
-  const int r = blk * TB + lane;
-  const int N = a.N;
-  zc taup = mkz(0, 0), alphap = mkz(0, 0);
-  if (jj > 0) {
-    zc uv = sum_partials(a.ruv, bprev0, a.nblk, 1);
-    taup = a.tau[j - 1];
-    double t2 = norm2(taup);
-    alphap = mkz(-0.5 * t2 * uv.re, -0.5 * t2 * uv.im);
-  }
-  // row j of the panel's W and V (W(j, jj-1) completed here as well)
-  if (lane < jj) {
-    zc wj = (lane == jj - 1) ? w_complete(a, j, jj - 1, taup, alphap) : a.W[(size_t)lane * N + j];
-    sWj[lane] = wj;
-    sVj[lane] = a.Vp[(size_t)lane * N + j];
+// true in every thread of the workgroup that arrived last
+__device__ inline bool arrive_last(unsigned* cnt, unsigned total) {
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(cnt, 1u);
+    s_last = (prev == total - 1u);
   }
   __syncthreads();
-  double nrm = 0.0;
-  if (r >= j && r < N) {
-    if (jj > 0) a.W[(size_t)(jj - 1) * N + r] = w_complete(a, r, jj - 1, taup, alphap);
-    zc x = a.H[(size_t)r + (size_t)a.ld * j];
-    // A(j:,j) -= V(j:,0:jj) conj(W(j,0:jj))^T ; then -= W(j:,0:jj) conj(V(j,0:jj))^T
-    for (int c = 0; c < jj; ++c) {
-      zc v = a.Vp[(size_t)c * N + r];
-      x = x - v * conj(sWj[c]);
-    }
-    for (int c = 0; c < jj; ++c) {
-      zc w = (c == jj - 1) ? w_complete(a, r, jj - 1, taup, alphap) : a.W[(size_t)c * N + r];
-      x = x - w * conj(sVj[c]);
-    }
-    if (r == j) {
-      x.im = 0.0;
-      a.d[j] = x.re;
-    }
-    a.H[(size_t)r + (size_t)a.ld * j] = x;
-    if (r >= j + 2) nrm = norm2(x);
-  }
-  nrm = wave_sum(nrm);
-  if (lane == 0) a.rnorm[blk] = nrm;
+  const bool last = s_last != 0;
+  if (last) __threadfence();
+  return last;
 }
 
-// reflector of column j (ZLARFG): beta, tau, scale s with v = [1; s x]
-struct Refl {
-  zc tau;
-  double beta;
-  zc s;
-  bool ident;
-};
-__device__ inline Refl make_refl(const TriArgs& a, int j) {
-  double xn2 = sum_partials(a.rnorm, j / TB, a.nblk);
-  zc alpha = a.H[(size_t)(j + 1) + (size_t)a.ld * j];
-  Refl R;
-  if (xn2 == 0.0 && alpha.im == 0.0) {
-    R.tau = mkz(0, 0);
-    R.beta = alpha.re;
-    R.s = mkz(0, 0);
-    R.ident = true;
-    return R;
+// ---- rowstep ------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj, int update_col) {
+  __shared__ zc st1[32], st2[32], svg[16], sWj[32], sVj[32];
+  __shared__ zc su_j, s_tau, s_s;
+  __shared__ zc suv[RPW];
+  const int tid = threadIdx.x;
+  const int row_l = tid >> 4, sub = tid & 15;
+  const int N = a.N;
+  const int jm = j - 1, jjm = jj - 1;
+  const int bm = (jm + 1) / TB;                   // first row block of reflector jm's GEMV
+  const int r = j + blockIdx.x * RPW + row_l;
+  const Pub* P = a.pub;
+  if (tid < 32) {
+    if (tid < jjm) {
+      st1[tid] = P->t1[tid];
+      st2[tid] = P->t2[tid];
+    }
+    if (tid < a.nrhs) svg[tid] = P->vg[tid];
+    if (tid == 0) {
+      s_tau = P->tau;
+      s_s = P->s;
+    }
   }
-  double beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2), alpha.re);
-  R.beta = beta;
-  R.tau = mkz((beta - alpha.re) / beta, -alpha.im / beta);
-  zc den = mkz(alpha.re - beta, alpha.im);
-  double dd = den.re * den.re + den.im * den.im;
-  R.s = mkz(den.re / dd, -den.im / dd);
-  R.ident = false;
-  return R;
-}
-__device__ inline zc refl_v(const TriArgs& a, const Refl& R, int j, int r) {
-  if (r == j + 1) return mkz(1.0, 0.0);
-  if (r <= j || r >= a.N || R.ident) return mkz(0, 0);
-  return R.s * a.H[(size_t)r + (size_t)a.ld * j];
+  __syncthreads();
+  const zc tau = s_tau, s = s_s;
+  // s z(row) - V t1 - W t2, each row summed by its 16 lanes
+  auto u_part = [&](int row) -> zc {
+    zc acc = mkz(0, 0);
+    const zc* yp = a.ypart + ((size_t)(row / TB) * a.nblk) * TB + (row % TB);
+    for (int sl = bm + sub; sl < a.nblk; sl += 16) acc += yp[(size_t)sl * TB];
+    acc = s * acc;
+    for (int c = sub; c < jjm; c += 16) {
+      acc = acc - a.Vp[(size_t)c * N + row] * st1[c];
+      acc = acc - a.W[(size_t)c * N + row] * st2[c];
+    }
+    for (int off = 8; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 16);
+      acc.im += __shfl_xor(acc.im, off, 16);
+    }
+    return acc;
+  };
+  // u(j): every workgroup needs it for the column update
+  if (row_l == 0) {
+    zc hjj = ldz(a.H[(size_t)j + (size_t)a.ld * j]);
+    zc up = u_part(j);
+    if (sub == 0) {
+      hjj.im = 0.0;                                // A22 diagonal: real part (ZHEMV)
+      su_j = hjj + up;
+    }
+  }
+  __syncthreads();
+  const bool valid = r < N;
+  zc uu = mkz(0, 0), v = mkz(0, 0), hcol = mkz(0, 0);
+  if (valid) {
+    hcol = ldz(a.H[(size_t)r + (size_t)a.ld * j]);       // A22(r, j): column j before its update
+    zc up = u_part(r);
+    uu = (r == j) ? su_j : hcol + up;
+    v = (r == j) ? mkz(1, 0) : s * a.xn[r];
+    if (sub == 0) {
+      a.Vp[(size_t)jjm * N + r] = v;
+      a.u[r] = uu;
+    }
+    if (sub < a.nrhs) {
+      const int q = sub;
+      zc gq = a.g[(size_t)q * N + r];
+      gq = gq - (tau * conj(v)) * svg[q];
+      a.g[(size_t)q * N + r] = gq;
+    }
+  }
+  if (sub == 0) suv[row_l] = valid ? conj(uu) * v : mkz(0, 0);
+  // column j with the panel (ZLATRD), W(:,jjm) taken as tau*u (alpha' part deferred)
+  if (update_col) {
+    if (tid < jj) {
+      sWj[tid] = (tid == jjm) ? tau * su_j : a.W[(size_t)tid * N + j];
+      sVj[tid] = (tid == jjm) ? mkz(1, 0) : a.Vp[(size_t)tid * N + j];
+    }
+    __syncthreads();
+    if (valid) {
+      zc acc = mkz(0, 0);
+      for (int c = sub; c < jj; c += 16) {
+        const zc vc = (c == jjm) ? v : a.Vp[(size_t)c * N + r];
+        const zc wc = (c == jjm) ? tau * uu : a.W[(size_t)c * N + r];
+        acc = acc + vc * conj(sWj[c]) + wc * conj(sVj[c]);
+      }
+      for (int off = 8; off > 0; off >>= 1) {
+        acc.re += __shfl_xor(acc.re, off, 16);
+        acc.im += __shfl_xor(acc.im, off, 16);
+      }
+      if (sub == 0) a.xp[r] = hcol - acc;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    zc uv = mkz(0, 0);
+    for (int i = 0; i < RPW; ++i) uv += suv[i];
+    a.puv[blockIdx.x] = uv;
+  }
+  if (!arrive_last(a.cnt + 1, gridDim.x)) return;
+  if (tid < 64) {
+    zc acc = mkz(0, 0);
+    for (int b = tid; b < (int)gridDim.x; b += 64) acc += a.puv[b];
+    acc = wave_sum(acc);
+    if (tid == 0) {
+      const double t2 = norm2(tau);
+      a.pub->alpha = mkz(-0.5 * t2 * acc.re, -0.5 * t2 * acc.im);
+      a.cnt[1] = 0u;
+    }
+  }
 }
 
-// ---- kernel B -----------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_tri_hemv(TriArgs a, int j, int jj) {
-  __shared__ zc vR[TB], vC[TB];
-  __shared__ zc yw[4][TB];
-  __shared__ zc yc[TB];
+// ---- hemvstep -----------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj) {
+  __shared__ zc xR[TB], xC[TB];
+  __shared__ zc y1s[4][TB], y2s[1][TB];
+  __shared__ zc sums[96];
+  __shared__ zc s_sc;
   const int N = a.N;
   const int b0 = (j + 1) / TB;
-  const int nt = a.nblk - b0;
   int t = blockIdx.x;
   int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
-  int tj = t - ti * (ti + 1) / 2;
-  (void)nt;
+  const int tj = t - ti * (ti + 1) / 2;
   const int bi = b0 + ti, bj = b0 + tj;
-  const Refl R = make_refl(a, j);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   const int R0 = bi * TB, C0 = bj * TB;
-  if (tid < TB) vR[tid] = refl_v(a, R, j, R0 + tid);
-  else if (tid < 2 * TB) vC[tid - TB] = refl_v(a, R, j, C0 + tid - TB);
-  __syncthreads();
+  const bool diag = bi == bj;
+  const zc ap = (jj > 0) ? a.pub->alpha : mkz(0, 0);
+  const double sig = 2.0 * ap.re;                  // x_new = x_p - 2 Re(alpha') v_{j-1}
+  auto xnew = [&](int row) -> zc {
+    if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
+    return a.xp[row] - sig * a.Vp[(size_t)(jj - 1) * N + row];
+  };
+  if (tid < TB) {
+    const int row = R0 + tid;
+    xR[tid] = (row >= j + 2 && row < N) ? xnew(row) : mkz(0, 0);
+  } else if (tid < 2 * TB) {
+    const int col = C0 + tid - TB;
+    xC[tid - TB] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
+  }
+  // wave w takes columns C0+16w .. C0+16w+15, lane = row: every load moves 1 KB of a column
+  const int lane = tid & 63, w = tid >> 6;
   const int r = R0 + lane;
   const bool rvalid = r > j && r < N;
-  zc acc = mkz(0, 0);
-  const zc vr = vR[lane];
-  for (int cc = wave; cc < TB; cc += 4) {
-    const int c = C0 + cc;
-    const bool cvalid = c > j && c < N;
+  zc hv[16];
+  for (int k = 0; k < 16; ++k) {
+    const int cl = 16 * w + k, c = C0 + cl;
     zc x = mkz(0, 0);
-    bool lower = (bi != bj) || (lane >= cc);
-    if (rvalid && cvalid && lower) {
-      x = a.H[(size_t)r + (size_t)a.ld * c];
-      if (bi == bj && lane == cc) x.im = 0.0;       // diagonal: real part (ZHEMV)
-    }
-    acc = fma_z(x, vC[cc], acc);
-    // transposed contribution: strictly lower elements only
-    zc tcon = mkz(0, 0);
-    if (bi != bj || lane > cc) tcon = conj(x) * vr;
-    tcon = wave_sum(tcon);
-    if (lane == 0) yc[cc] = tcon;
+    if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
+    hv[k] = x;
   }
-  yw[wave][lane] = acc;
+  __syncthreads();
+  const zc xr = xR[lane];
+  zc y1 = mkz(0, 0);
+  for (int k = 0; k < 16; ++k) {
+    const int cl = 16 * w + k;
+    zc x = hv[k];
+    if (diag && lane == cl) x.im = 0.0;                   // diagonal: real part (ZHEMV)
+    y1 = fma_z(x, xC[cl], y1);
+    hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
+  }
+  // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
+  {
+    int n = 16;
+    for (int off = 32; off >= 4; off >>= 1) {
+      n >>= 1;
+      const bool up = (lane & off) != 0;
+      for (int i = 0; i < n; ++i) {
+        const zc send = up ? hv[i] : hv[i + n];
+        zc recv;
+        recv.re = __shfl_xor(send.re, off);
+        recv.im = __shfl_xor(send.im, off);
+        hv[i] = (up ? hv[i + n] : hv[i]) + recv;
+      }
+    }
+    for (int off = 2; off > 0; off >>= 1) {
+      hv[0].re += __shfl_xor(hv[0].re, off);
+      hv[0].im += __shfl_xor(hv[0].im, off);
+    }
+  }
+  // lane holds the sum for column 16w + ((lane >> 2) & 15)
+  y1s[w][lane] = y1;
+  if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
   __syncthreads();
   zc* yp = a.ypart;
   if (tid < TB) {
-    zc s = yw[0][tid] + yw[1][tid] + yw[2][tid] + yw[3][tid];
-    if (bi == bj) s += yc[tid];
-    yp[((size_t)bi * a.nblk + bj) * TB + tid] = s;
-  } else if (tid < 2 * TB && bi != bj) {
-    yp[((size_t)bj * a.nblk + bi) * TB + (tid - TB)] = yc[tid - TB];
+    zc sm = y1s[0][tid] + y1s[1][tid] + y1s[2][tid] + y1s[3][tid];
+    if (diag) sm += y2s[0][tid];
+    yp[((size_t)bi * a.nblk + bj) * TB + tid] = sm;
+  } else if (tid < 2 * TB && !diag) {
+    yp[((size_t)bj * a.nblk + bi) * TB + (tid - TB)] = y2s[0][tid - TB];
   }
-  if (bi == bj) {
-    // reflector storage, tau/e, and the small partial products for this row block
-    if (tid < TB) {
-      int rr = R0 + tid;
-      if (rr < N) a.Vp[(size_t)jj * N + rr] = vR[tid];
-    }
-    if (bi == b0 && tid == 0) {
-      a.tau[j] = R.tau;
-      a.e[j] = R.beta;
-    }
-    // t1[c] = sum_r conj(W(r,c)) v(r), t2[c] = sum_r conj(V(r,c)) v(r), vg[q] = sum_r v(r) g(r,q)
-    const int nsm = 2 * jj + a.nrhs;
-    for (int q = wave; q < nsm; q += 4) {
-      zc s = mkz(0, 0);
-      if (rvalid) {
-        if (q < jj) s = conj(a.W[(size_t)q * N + r]) * vr;
-        else if (q < 2 * jj) s = conj(a.Vp[(size_t)(q - jj) * N + r]) * vr;
-        else s = vr * a.g[(size_t)(q - 2 * jj) * N + r];
-      }
-      s = wave_sum(s);
-      if (lane == 0) {
-        if (q < jj) a.rt1[(size_t)bi * a.nb + q] = s;
-        else if (q < 2 * jj) a.rt2[(size_t)bi * a.nb + (q - jj)] = s;
-        else a.rvg[(size_t)bi * a.nrhs + (q - 2 * jj)] = s;
-      }
+  if (!diag) return;
+  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, partial dot products --------------------
+  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
+  if (tid < TB) {
+    const int r = R0 + tid;
+    if (r >= j + 1 && r < N) {
+      a.xn[r] = xnew(r);
+      if (jj > 0) a.W[(size_t)(jj - 1) * N + r] = taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r];
     }
   }
-}
-
-// ---- kernel C -----------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_tri_u(TriArgs a, int j, int jj) {
-  __shared__ zc st1[64], st2[64], svg[16];
-  const int lane = threadIdx.x;
-  const int N = a.N;
-  const int b0 = (j + 1) / TB;
-  const int blk = b0 + blockIdx.x;
-  const int r = blk * TB + lane;
-  // reduce t1, t2, vg over row blocks (fixed order)
-  for (int q = 0; q < 2 * jj + a.nrhs; ++q) {
-    zc s;
-    if (q < jj) s = sum_partials(a.rt1 + q, b0, a.nblk, a.nb);
-    else if (q < 2 * jj) s = sum_partials(a.rt2 + (q - jj), b0, a.nblk, a.nb);
-    else s = sum_partials(a.rvg + (q - 2 * jj), b0, a.nblk, a.nrhs);
-    if (lane == 0) {
-      if (q < jj) st1[q] = s;
-      else if (q < 2 * jj) st2[q - jj] = s;
-      else svg[q - 2 * jj] = s;
+  if (ti == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
+  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g; 4 lanes each
+  const int nq = 1 + 2 * jj + a.nrhs;
+  const int sub = tid & 3;
+  for (int q0 = 0; q0 < nq; q0 += 64) {
+    const int q = q0 + (tid >> 2);
+    zc acc = mkz(0, 0);
+    if (q < nq) {
+      for (int rl = sub; rl < TB; rl += 4) {
+        const int r = R0 + rl;
+        if (r < j + 2 || r >= N) continue;
+        const zc xr = xR[rl];
+        if (q == 0) {
+          acc.re += norm2(xr);
+        } else if (q <= jj) {
+          const int c = q - 1;
+          const zc w = (c == jj - 1) ? taup * a.u[r] + ap * a.Vp[(size_t)c * N + r] : a.W[(size_t)c * N + r];
+          acc = fma_conj(w, xr, acc);
+        } else if (q <= 2 * jj) {
+          acc = fma_conj(a.Vp[(size_t)(q - 1 - jj) * N + r], xr, acc);
+        } else {
+          acc = fma_z(xr, a.g[(size_t)(q - 1 - 2 * jj) * N + r], acc);
+        }
+      }
     }
+    for (int off = 2; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 4);
+      acc.im += __shfl_xor(acc.im, off, 4);
+    }
+    if (sub == 0 && q < nq) {
+      if (q == 0) a.pn[bi] = acc.re;
+      else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = acc;
+      else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = acc;
+      else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = acc;
+    }
+  }
+  if (!arrive_last(a.cnt + 0, (unsigned)(a.nblk - b0))) return;
+  // ---- last diagonal tile: sums, ZLARFG, W^H v, V^H v, v^T g --------------------------
+  for (int q0 = 0; q0 < nq; q0 += 64) {
+    const int q = q0 + (tid >> 2);
+    zc acc = mkz(0, 0);
+    if (q < nq) {
+      for (int b = b0 + sub; b < a.nblk; b += 4) {
+        if (q == 0) acc.re += a.pn[b];
+        else if (q <= jj) acc += a.pt1[(size_t)b * a.nb + (q - 1)];
+        else if (q <= 2 * jj) acc += a.pt2[(size_t)b * a.nb + (q - 1 - jj)];
+        else acc += a.pvg[(size_t)b * a.nrhs + (q - 1 - 2 * jj)];
+      }
+    }
+    for (int off = 2; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 4);
+      acc.im += __shfl_xor(acc.im, off, 4);
+    }
+    if (sub == 0 && q < nq) sums[q] = acc;
   }
   __syncthreads();
-  zc uv = mkz(0, 0);
-  if (r > j && r < N) {
-    zc y = mkz(0, 0);
-    const zc* yp = a.ypart + ((size_t)blk * a.nblk) * TB + lane;
-    for (int sl = b0; sl < a.nblk; ++sl) y += yp[(size_t)sl * TB];
-    for (int c = 0; c < jj; ++c) y = y - a.Vp[(size_t)c * N + r] * st1[c];
-    for (int c = 0; c < jj; ++c) y = y - a.W[(size_t)c * N + r] * st2[c];
-    a.u[r] = y;
-    zc v = a.Vp[(size_t)jj * N + r];
-    uv = conj(y) * v;
-    zc tj = a.tau[j];
-    for (int q = 0; q < a.nrhs; ++q) {
-      zc gq = a.g[(size_t)q * N + r];
-      gq = gq - (tj * conj(v)) * svg[q];
-      a.g[(size_t)q * N + r] = gq;
+  Pub* P = a.pub;
+  if (tid == 0) {
+    // ZLARFG on alpha = x_new(j+1), |x|^2 = sum |x~|^2
+    const double xn2 = sums[0].re;
+    const zc alpha = a.xn[j + 1];
+    zc tau, s;
+    double beta;
+    if (xn2 == 0.0 && alpha.im == 0.0) {
+      tau = mkz(0, 0);
+      s = mkz(0, 0);
+      beta = alpha.re;
+    } else {
+      beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2), alpha.re);
+      tau = mkz((beta - alpha.re) / beta, -alpha.im / beta);
+      const zc den = mkz(alpha.re - beta, alpha.im);
+      const double dd = den.re * den.re + den.im * den.im;
+      s = mkz(den.re / dd, -den.im / dd);
     }
+    P->tau = tau;
+    P->s = s;
+    P->beta = mkz(beta, 0);
+    a.tau[j] = tau;
+    a.e[j] = beta;
+    s_sc = s;
+    a.cnt[0] = 0u;
   }
-  uv = wave_sum(uv);
-  if (lane == 0) a.ruv[blk] = uv;
+  __syncthreads();
+  const zc s = s_sc;
+  // v = e_{j+1} + s x~:  W^H v = conj(W(j+1,:)) + s W^H x~, likewise V^H v, v^T g
+  if (tid < jj) {
+    const int c = tid;
+    const zc wj = (c == jj - 1) ? taup * a.u[j + 1] + ap * a.Vp[(size_t)c * N + (j + 1)]
+                                : a.W[(size_t)c * N + (j + 1)];
+    P->t1[c] = conj(wj) + s * sums[1 + c];
+    P->t2[c] = conj(a.Vp[(size_t)c * N + (j + 1)]) + s * sums[1 + jj + c];
+  }
+  if (tid >= 64 && tid < 64 + a.nrhs) {
+    const int q = tid - 64;
+    P->vg[q] = a.g[(size_t)q * N + (j + 1)] + s * sums[1 + 2 * jj + q];
+  }
 }
 
-// ---- W completion at the panel end ------------------------------------------------
-__global__ __launch_bounds__(64) void k_tri_wlast(TriArgs a, int j, int jj) {
+// ---- W(:, kp-1) of the panel's last reflector ----------------------------------------
+template <class T>
+__global__ __launch_bounds__(64) void k_tri_wlast(TriArgs<T> a, int jl, int kpm1) {
   const int lane = threadIdx.x;
-  const int b0 = (j + 1) / TB;
-  const int r = (b0 + blockIdx.x) * TB + lane;
-  zc uv = sum_partials(a.ruv, b0, a.nblk, 1);
-  zc tp = a.tau[j];
-  double t2 = norm2(tp);
-  zc ap = mkz(-0.5 * t2 * uv.re, -0.5 * t2 * uv.im);
-  if (r > j && r < a.N) a.W[(size_t)jj * a.N + r] = tp * a.u[r] + ap * a.Vp[(size_t)jj * a.N + r];
+  const int r = ((jl + 1) / TB + blockIdx.x) * TB + lane;
+  const zc ap = a.pub->alpha;
+  if (r > jl && r < a.N) {
+    const zc tp = a.tau[jl];
+    a.W[(size_t)kpm1 * a.N + r] = tp * a.u[r] + ap * a.Vp[(size_t)kpm1 * a.N + r];
+  }
 }
 
-// ---- trailing update A22 -= V W^H + W V^H on lower 64x64 tiles (MFMA f64) ---------
-__global__ __launch_bounds__(256) void k_tri_her2k(TriArgs a, int s0, int kp) {
+// ---- trailing update A22 -= V W^H + W V^H on lower 64x64 tiles (MFMA f64) ---------------
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_her2k(TriArgs<T> a, int s0, int kp) {
   const int N = a.N;
   const int b0 = s0 / TB;
   int t = blockIdx.x;
@@ -297,68 +418,124 @@ __global__ __launch_bounds__(256) void k_tri_her2k(TriArgs a, int s0, int kp) {
   const int col0 = bj * TB + (wave & 1) * 32;
   if (bi == bj && (wave >> 1) < (wave & 1)) return;   // strictly upper quarter of a diagonal tile
   const int lr = lane & 15, lk = lane >> 4;
-  d4 cr[2][2], ci[2][2];
-  for (int x = 0; x < 2; ++x)
-    for (int y = 0; y < 2; ++y) {
-      cr[x][y] = d4{0, 0, 0, 0};
-      ci[x][y] = d4{0, 0, 0, 0};
-    }
   const int K = 2 * kp;
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    int k = k0 + lk;
-    zc av[2], bv[2];
-    for (int x = 0; x < 2; ++x) {
-      int r = row0 + x * 16 + lr;
-      int c = col0 + x * 16 + lr;
-      zc va = mkz(0, 0), vb = mkz(0, 0);
-      if (k < K) {
-        if (r < N) va = (k < kp) ? a.Vp[(size_t)k * N + r] : a.W[(size_t)(k - kp) * N + r];
-        if (c < N) vb = (k < kp) ? a.W[(size_t)k * N + c] : a.Vp[(size_t)(k - kp) * N + c];
-      }
-      av[x] = va;
-      bv[x] = vb;   // B(k, c) = conj(vb)
-    }
+  if constexpr (sizeof(T) == sizeof(zc)) {
+    d4 cr[2][2], ci[2][2];
     for (int x = 0; x < 2; ++x)
       for (int y = 0; y < 2; ++y) {
-        // a * conj(b)
-        cr[x][y] = mfma16x16x4(av[x].re, bv[y].re, cr[x][y]);
-        cr[x][y] = mfma16x16x4(av[x].im, bv[y].im, cr[x][y]);
-        ci[x][y] = mfma16x16x4(av[x].im, bv[y].re, ci[x][y]);
-        ci[x][y] = mfma16x16x4(-av[x].re, bv[y].im, ci[x][y]);
+        cr[x][y] = d4{0, 0, 0, 0};
+        ci[x][y] = d4{0, 0, 0, 0};
       }
-  }
-  for (int x = 0; x < 2; ++x)
-    for (int y = 0; y < 2; ++y)
-      for (int q = 0; q < 4; ++q) {
-        int r = row0 + x * 16 + lk + 4 * q;
-        int c = col0 + y * 16 + lr;
-        if (r < N && c < N && r >= s0 && c >= s0 && r >= c) {
-          size_t off = (size_t)r + (size_t)a.ld * c;
-          zc h = a.H[off];
-          h.re -= cr[x][y][q];
-          h.im -= ci[x][y][q];
-          a.H[off] = h;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      int k = k0 + lk;
+      zc av[2], bv[2];
+      for (int x = 0; x < 2; ++x) {
+        int r = row0 + x * 16 + lr;
+        int c = col0 + x * 16 + lr;
+        zc va = mkz(0, 0), vb = mkz(0, 0);
+        if (k < K) {
+          if (r < N) va = (k < kp) ? a.Vp[(size_t)k * N + r] : a.W[(size_t)(k - kp) * N + r];
+          if (c < N) vb = (k < kp) ? a.W[(size_t)k * N + c] : a.Vp[(size_t)(k - kp) * N + c];
         }
+        av[x] = va;
+        bv[x] = vb;   // B(k, c) = conj(vb)
       }
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+          // a * conj(b)
+          cr[x][y] = mfma16x16x4(av[x].re, bv[y].re, cr[x][y]);
+          cr[x][y] = mfma16x16x4(av[x].im, bv[y].im, cr[x][y]);
+          ci[x][y] = mfma16x16x4(av[x].im, bv[y].re, ci[x][y]);
+          ci[x][y] = mfma16x16x4(-av[x].re, bv[y].im, ci[x][y]);
+        }
+    }
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y)
+        for (int q = 0; q < 4; ++q) {
+          int r = row0 + x * 16 + lk + 4 * q;     // f64 MFMA: register q holds row lk + 4q
+          int c = col0 + y * 16 + lr;
+          if (r < N && c < N && r >= s0 && c >= s0 && r >= c) {
+            size_t off = (size_t)r + (size_t)a.ld * c;
+            zc h = ldz(a.H[off]);
+            h.re -= cr[x][y][q];
+            h.im -= ci[x][y][q];
+            stz(a.H[off], h);
+          }
+        }
+  } else {
+    // precision form: FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32)
+    f4 cr[2][2], ci[2][2];
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y) {
+        cr[x][y] = f4{0, 0, 0, 0};
+        ci[x][y] = f4{0, 0, 0, 0};
+      }
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      int k = k0 + lk;
+      float ar[2], ai[2], br[2], bi[2];
+      for (int x = 0; x < 2; ++x) {
+        int r = row0 + x * 16 + lr;
+        int c = col0 + x * 16 + lr;
+        zc va = mkz(0, 0), vb = mkz(0, 0);
+        if (k < K) {
+          if (r < N) va = (k < kp) ? a.Vp[(size_t)k * N + r] : a.W[(size_t)(k - kp) * N + r];
+          if (c < N) vb = (k < kp) ? a.W[(size_t)k * N + c] : a.Vp[(size_t)(k - kp) * N + c];
+        }
+        ar[x] = (float)va.re;
+        ai[x] = (float)va.im;
+        br[x] = (float)vb.re;
+        bi[x] = (float)vb.im;
+      }
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+          cr[x][y] = mfma16x16x4f32(ar[x], br[y], cr[x][y]);
+          cr[x][y] = mfma16x16x4f32(ai[x], bi[y], cr[x][y]);
+          ci[x][y] = mfma16x16x4f32(ai[x], br[y], ci[x][y]);
+          ci[x][y] = mfma16x16x4f32(-ar[x], bi[y], ci[x][y]);
+        }
+    }
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y)
+        for (int q = 0; q < 4; ++q) {
+          int r = row0 + x * 16 + 4 * lk + q;     // f32 MFMA: register q holds row 4lk + q
+          int c = col0 + y * 16 + lr;
+          if (r < N && c < N && r >= s0 && c >= s0 && r >= c) {
+            size_t off = (size_t)r + (size_t)a.ld * c;
+            T h = a.H[off];
+            h.re -= cr[x][y][q];
+            h.im -= ci[x][y][q];
+            a.H[off] = h;
+          }
+        }
+  }
 }
 
-__global__ void k_tri_last(TriArgs a) {
-  zc x = a.H[(size_t)(a.N - 1) + (size_t)a.ld * (a.N - 1)];
+template <class T>
+__global__ void k_tri_last(TriArgs<T> a) {
+  zc x = ldz(a.H[(size_t)(a.N - 1) + (size_t)a.ld * (a.N - 1)]);
   a.d[a.N - 1] = x.re;
   a.e[a.N - 1] = 0.0;
+}
+
+__global__ void k_tri_reset(unsigned* cnt) {
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
 size_t tridiag_work_bytes(int N, int nb, int nrhs) {
   size_t nblk = (N + TB - 1) / TB;
   size_t b = 0;
-  b += sizeof(zc) * (size_t)N * nb * 2;         // W, Vp
-  b += sizeof(zc) * (size_t)N * 2;              // u, tau
-  b += sizeof(double) * (size_t)N * 2;          // d, e
-  b += sizeof(double) * (nblk + 1);             // rnorm (+1: keeps the complex partials aligned)
-  b += sizeof(zc) * nblk * (1 + 2 * nb + nrhs); // ruv, rt1, rt2, rvg
-  b += sizeof(zc) * nblk * nblk * TB;           // ypart
-  return b + 1024;
+  b += sizeof(zc) * (size_t)N * nb * 2 + 512;   // W, Vp
+  b += sizeof(zc) * (size_t)N * 2 + 512;        // u, tau
+  b += sizeof(double) * (size_t)N * 2 + 512;    // d, e
+  b += sizeof(double) * nblk + 256;             // pn
+  b += sizeof(zc) * (N / RPW + 2) + 256;        // puv
+  b += sizeof(zc) * nblk * (2 * nb + nrhs + 1) + 1024;
+  b += sizeof(zc) * nblk * nblk * TB + 256;     // ypart
+  b += sizeof(zc) * N * 2 + 512;                // xn, xp
+  b += sizeof(Pub) + 256;
+  b += 256;                                     // counters
+  return b + 4096;
 }
 
 int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs) {
@@ -378,15 +555,22 @@ int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs) {
   tw.tau = (zc*)take(sizeof(zc) * N);
   tw.d = (double*)take(sizeof(double) * N);
   tw.e = (double*)take(sizeof(double) * N);
-  tw.red = (double*)take(sizeof(double) * (nblk + 1) + sizeof(zc) * nblk * (1 + 2 * nb + nrhs));
+  tw.red = (double*)take(sizeof(double) * nblk);
+  tw.puv = (zc*)take(sizeof(zc) * (N / RPW + 2));
+  tw.zred = (zc*)take(sizeof(zc) * nblk * (2 * nb + nrhs + 1));
   tw.ypart = (zc*)take(sizeof(zc) * nblk * nblk * TB);
+  tw.xn = (zc*)take(sizeof(zc) * N);
+  tw.xp = (zc*)take(sizeof(zc) * N);
+  tw.pub = take(sizeof(Pub));
+  tw.cnt = (unsigned*)take(256);
   return WTB_OK;
 }
 
-int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g) {
+template <class T>
+static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_g) {
   const int N = tw.N, nb = tw.nb;
   if (nb > 32 || tw.nrhs > 16) return WTB_ERR_LIMIT;
-  TriArgs a;
+  TriArgs<T> a;
   a.H = d_H;
   a.ld = ldh;
   a.N = N;
@@ -398,44 +582,59 @@ int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g) {
   a.d = tw.d;
   a.e = tw.e;
   a.nblk = (N + TB - 1) / TB;
-  a.rnorm = tw.red;
-  zc* zr = reinterpret_cast<zc*>(tw.red + ((a.nblk + 1) & ~1));   // 16-byte aligned
-  a.ruv = zr;
-  a.rt1 = zr + a.nblk;
-  a.rt2 = a.rt1 + (size_t)a.nblk * nb;
-  a.rvg = a.rt2 + (size_t)a.nblk * nb;
+  a.pn = tw.red;
+  a.puv = tw.puv;
+  a.pt1 = tw.zred;
+  a.pt2 = a.pt1 + (size_t)a.nblk * nb;
+  a.pvg = a.pt2 + (size_t)a.nblk * nb;
   a.ypart = tw.ypart;
   a.g = d_g;
+  a.xn = tw.xn;
+  a.xp = tw.xp;
+  a.pub = (Pub*)tw.pub;
+  a.cnt = tw.cnt;
   a.nrhs = tw.nrhs;
+  // the arrival counters start from zero (a kernel, not a memset, so that the
+  // sequence can be captured into a graph); every use returns them to zero
+  hipLaunchKernelGGL(k_tri_reset, dim3(1), dim3(64), 0, s, a.cnt);
   if (N == 1) {
-    hipLaunchKernelGGL(k_tri_last, dim3(1), dim3(1), 0, s, a);
+    hipLaunchKernelGGL(k_tri_last<T>, dim3(1), dim3(1), 0, s, a);
     WTB_LAUNCH_CHECK();
     return WTB_OK;
   }
   for (int p = 0; p < N - 1; p += nb) {
     const int kp = (nb < N - 1 - p) ? nb : (N - 1 - p);
-    int prev_b0 = 0;
     for (int jj = 0; jj < kp; ++jj) {
       const int j = p + jj;
-      const int nA = a.nblk - j / TB;
-      hipLaunchKernelGGL(k_tri_colupd, dim3(nA), dim3(64), 0, s, a, j, jj, prev_b0);
+      if (jj > 0) {
+        const int nA = (N - j + RPW - 1) / RPW;
+        hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nA), dim3(256), 0, s, a, j, jj, 1);
+      }
       const int nt = a.nblk - (j + 1) / TB;
-      hipLaunchKernelGGL(k_tri_hemv, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj);
-      hipLaunchKernelGGL(k_tri_u, dim3(nt), dim3(64), 0, s, a, j, jj);
-      prev_b0 = (j + 1) / TB;
+      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj);
     }
+    // finish the panel's last reflector (no further column in this panel)
     const int jl = p + kp - 1;
-    const int ntl = a.nblk - (jl + 1) / TB;
-    hipLaunchKernelGGL(k_tri_wlast, dim3(ntl), dim3(64), 0, s, a, jl, kp - 1);
+    const int jn = jl + 1;
+    const int nF = (N - jn + RPW - 1) / RPW;
+    hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nF), dim3(256), 0, s, a, jn, kp, 0);
+    hipLaunchKernelGGL(k_tri_wlast<T>, dim3(a.nblk - (jl + 1) / TB), dim3(64), 0, s, a, jl, kp - 1);
     const int s0 = p + kp;
     if (s0 < N) {
       const int nt2 = a.nblk - s0 / TB;
-      hipLaunchKernelGGL(k_tri_her2k, dim3(nt2 * (nt2 + 1) / 2), dim3(256), 0, s, a, s0, kp);
+      hipLaunchKernelGGL(k_tri_her2k<T>, dim3(nt2 * (nt2 + 1) / 2), dim3(256), 0, s, a, s0, kp);
     }
   }
-  hipLaunchKernelGGL(k_tri_last, dim3(1), dim3(1), 0, s, a);
+  hipLaunchKernelGGL(k_tri_last<T>, dim3(1), dim3(1), 0, s, a);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
+}
+
+int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g) {
+  return run_tridiag_t<zc>(s, tw, d_H, ldh, d_g);
+}
+int run_tridiag_f32(hipStream_t s, TridiagWork& tw, zf* d_H, int ldh, zc* d_g) {
+  return run_tridiag_t<zf>(s, tw, d_H, ldh, d_g);
 }
 
 }  // namespace wtb

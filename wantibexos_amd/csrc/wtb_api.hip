@@ -95,7 +95,9 @@ struct wtb_bse_plan {
   double* gapk = nullptr;
   double* wtab = nullptr;
   int wtab_n = 0;
-  zc* H = nullptr;
+  zc* H = nullptr;      // FP64 BSE matrix (plain form)
+  zf* Hf = nullptr;     // FP32 BSE matrix (precision form)
+  bool mixed = false;
   size_t ldh = 0;
   void* triwork = nullptr;
   TridiagWork tw;
@@ -110,6 +112,9 @@ struct wtb_bse_plan {
   double* smeavg = nullptr;
   int* status = nullptr;
   bool timing = false;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   hipEvent_t ev[WTB_NSTAGES + 1] = {};
   double stage_ms[WTB_NSTAGES] = {};
 };
@@ -323,11 +328,13 @@ int wtb_eigsys(wtb_ctx* c, const wtb_tb_model* m, int nk, const double* kpt, dou
 static void plan_free(wtb_bse_plan* p) {
   free_model(p->dm);
   void* bufs[] = {p->kpt, p->phase, p->hk4, p->eall, p->nocp, p->eigv, p->vec, p->dip, p->g,
-                  p->gapk, p->wtab, p->H, p->triwork, p->dcwork, p->track, p->lambda, p->act,
+                  p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
                   p->wq, p->actq, p->eps, p->smeavg, p->status};
   for (void* b : bufs) hipFree(b);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
+  if (p->gexec) hipGraphExecDestroy(p->gexec);
+  if (p->graph) hipGraphDestroy(p->graph);
 }
 
 static double vcell3d_host(const double* rl) {
@@ -370,6 +377,7 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
   wtb_bse_plan* p = new wtb_bse_plan;
   p->ctx = c;
   p->prm = *prm;
+  p->mixed = prm->precision == WTB_MIXED;
   p->nk = prm->ngrid[0] * prm->ngrid[1] * prm->ngrid[2];
   p->nw = m->nw;
   p->nvec = m->nvec;
@@ -401,7 +409,8 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
       (rc = dmalloc(&p->nocp, nk)) || (rc = dmalloc(&p->eigv, (size_t)nk * p->nb)) ||
       (rc = dmalloc(&p->vec, (size_t)nk * p->nb * nw)) || (rc = dmalloc(&p->dip, (size_t)3 * N)) ||
       (rc = dmalloc(&p->g, (size_t)3 * N)) || (rc = dmalloc(&p->gapk, nk)) ||
-      (rc = dmalloc(&p->wtab, p->wtab_n)) || (rc = dmalloc(&p->H, p->ldh * N)) ||
+      (rc = dmalloc(&p->wtab, p->wtab_n)) ||
+      (rc = (prm->precision == WTB_MIXED) ? dmalloc(&p->Hf, p->ldh * N) : dmalloc(&p->H, p->ldh * N)) ||
       (rc = dmalloc((char**)&p->triwork, tridiag_work_bytes(N, nb_panel, 3))) ||
       (rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 6))) ||
       (rc = dmalloc(&p->track, (size_t)6 * N)) || (rc = dmalloc(&p->lambda, N)) ||
@@ -460,13 +469,17 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   // (c) W table + assembly (lower triangle)
   if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
   if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H,
-                             (int)p->ldh, nullptr)))
+                             (int)p->ldh, p->Hf)))
     return rc;
   mark(3);
   if (build_only) return WTB_OK;
   // (d) Householder tridiagonalisation, g <- Q^T h
   WTB_HIP(hipMemcpyAsync(p->g, p->dip, sizeof(zc) * 3 * N, hipMemcpyDeviceToDevice, s));
-  if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, p->g))) return rc;
+  if (p->mixed) {
+    if ((rc = run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, p->g))) return rc;
+  } else if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, p->g))) {
+    return rc;
+  }
   mark(4);
   // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
   hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->g, p->track);
@@ -489,13 +502,51 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   return WTB_OK;
 }
 
+// The whole pipeline is captured once into a HIP graph (a few thousand kernel
+// nodes for the column-by-column reduction) and replayed on later executes.
+static int plan_capture(wtb_bse_plan* p) {
+  hipStream_t s = p->ctx->stream;
+  WTB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = plan_enqueue(p);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return WTB_ERR_HIP;
+  }
+  e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return WTB_ERR_HIP;
+  }
+  p->graph = g;
+  return WTB_OK;
+}
+
 int wtb_bse_plan_execute(wtb_bse_plan* p) {
   if (!p) {
     set_error("wtb_bse_plan_execute: null plan");
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
-  return plan_enqueue(p);
+  if (p->timing || !p->use_graph) return plan_enqueue(p);
+  if (!p->gexec) {
+    int rc = plan_capture(p);
+    if (rc) return rc;
+  }
+  WTB_HIP(hipGraphLaunch(p->gexec, p->ctx->stream));
+  return WTB_OK;
+}
+
+int wtb_bse_plan_set_graph(wtb_bse_plan* p, int on) {
+  if (!p) return WTB_ERR_ARG;
+  p->use_graph = on != 0;
+  return WTB_OK;
 }
 
 int wtb_bse_plan_build(wtb_bse_plan* p) {
@@ -558,7 +609,7 @@ size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* p, int what) {
     case WTB_BUF_EIGV: return sizeof(double) * p->nk * p->nb;
     case WTB_BUF_VECTOR: return sizeof(zc) * p->nk * p->nb * p->nw;
     case WTB_BUF_DIPOLE: return sizeof(zc) * 3 * p->N;
-    case WTB_BUF_HBSE: return sizeof(zc) * p->ldh * p->N;
+    case WTB_BUF_HBSE: return (p->mixed ? sizeof(zf) : sizeof(zc)) * p->ldh * p->N;
     case WTB_BUF_WTAB: return sizeof(double) * p->wtab_n;
     case WTB_BUF_TRIDIAG: return sizeof(double) * 2 * p->N;
     default: return 0;
@@ -579,7 +630,7 @@ int wtb_bse_plan_fetch(wtb_bse_plan* p, int what, void* dst, size_t bytes) {
     case WTB_BUF_EIGV: src = p->eigv; break;
     case WTB_BUF_VECTOR: src = p->vec; break;
     case WTB_BUF_DIPOLE: src = p->dip; break;
-    case WTB_BUF_HBSE: src = p->H; break;
+    case WTB_BUF_HBSE: src = p->mixed ? (const void*)p->Hf : (const void*)p->H; break;
     case WTB_BUF_WTAB: src = p->wtab; break;
     case WTB_BUF_TRIDIAG: {
       WTB_HIP(hipMemcpyAsync(dst, p->tw.d, sizeof(double) * p->N, hipMemcpyDeviceToHost, s));

@@ -26,6 +26,18 @@ __host__ __device__ inline zc conj(zc a) { return mkz(a.re, -a.im); }
 __host__ __device__ inline double norm2(zc a) { return a.re * a.re + a.im * a.im; }
 __host__ __device__ inline zc& operator+=(zc& a, zc b) { a.re += b.re; a.im += b.im; return a; }
 __host__ __device__ inline zc& operator-=(zc& a, zc b) { a.re -= b.re; a.im -= b.im; return a; }
+// complex float storage (COMPLEX*8) for the precision form; arithmetic stays in zc
+struct __attribute__((aligned(8))) zf {
+  float re, im;
+};
+__host__ __device__ inline zc ldz(const zc& v) { return v; }
+__host__ __device__ inline zc ldz(const zf& v) { return mkz((double)v.re, (double)v.im); }
+__host__ __device__ inline void stz(zc& d, zc v) { d = v; }
+__host__ __device__ inline void stz(zf& d, zc v) {
+  d.re = (float)v.re;
+  d.im = (float)v.im;
+}
+
 // conj(a)*b accumulated with fused multiply-adds
 __device__ inline zc fma_conj(zc a, zc b, zc acc) {
   acc.re = fma(a.re, b.re, fma(a.im, b.im, acc.re));

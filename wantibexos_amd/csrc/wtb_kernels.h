@@ -40,7 +40,7 @@ int launch_dipoles(hipStream_t s, const zc* d_hk4, int nk, int nw, int nc, int n
 int launch_wtable(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, double* d_wtab);
 int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng[3],
                      const double* d_wtab, const double* d_eigv, const zc* d_vec, zc* d_H, int ldh,
-                     float* d_Hf);
+                     zf* d_Hf);
 
 // stage (d): Householder reduction of the lower triangle of H (ldh) to a real
 // tridiagonal (d, e), applying Q^T to nrhs complex vectors g (g[N*nrhs]).
@@ -52,13 +52,20 @@ struct TridiagWork {
   zc* tau = nullptr;      // N
   double* d = nullptr;    // N
   double* e = nullptr;    // N
-  double* red = nullptr;  // partial sums
+  double* red = nullptr;  // real partial sums
+  zc* zred = nullptr;     // complex partial sums
   zc* ypart = nullptr;    // hemv partials [nblk][nblk][64]
-  size_t red_elems = 0;
+  zc* xn = nullptr;       // reduced column below the diagonal
+  zc* xp = nullptr;       // panel-updated column before the deferred term
+  zc* puv = nullptr;      // per-workgroup u^H v partials
+  void* pub = nullptr;    // scalars published by the last workgroup of a step
+  unsigned* cnt = nullptr;  // arrival counters
 };
 size_t tridiag_work_bytes(int N, int nb, int nrhs);
 int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs);
 int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g);
+// the same reduction on a COMPLEX*8 matrix (precision form)
+int run_tridiag_f32(hipStream_t s, TridiagWork& tw, zf* d_H, int ldh, zc* d_g);
 
 // stage (d): tridiagonal divide and conquer (eigenvalues + Z^T x for tracked vectors)
 struct DCTree {

@@ -124,7 +124,9 @@ int wtb_eigsys(wtb_ctx* ctx, const wtb_tb_model* model, int nk, const double* kp
 /* ZHEEV('N' or 'V', 'U', n, a, lda, w) for the BSE matrix
  * (subprograms/bse_solver-tool-diel.f90:485-492).  The upper triangle of
  * a(lda,n) is read.  want_vectors = 0: eigenvalues only.  want_vectors = 1:
- * a is overwritten with the orthonormal eigenvectors (column j <-> w[j]). */
+ * a is overwritten with the orthonormal eigenvectors (column j <-> w[j]),
+ * each scaled so that its last component is real and >= 0 (ZHEEV's vectors
+ * have a real last component of either sign). */
 int wtb_zheev(wtb_ctx* ctx, int want_vectors, int n, double* a, int lda, double* w);
 
 /* dielbsev (subroutines/bse_subs.f90:336-378): act(i) = |sum_j x(j,i) h(j)|^2 */
@@ -192,13 +194,68 @@ int wtb_bse_plan_fetch(wtb_bse_plan* plan, int what, void* dst, size_t bytes);
  * [1] dipoles, [2] W table + assembly, [3] tridiagonal reduction,
  * [4] tridiagonal eigensolve, [5] activities + eps(omega). */
 enum { WTB_NSTAGES = 6 };
+/* on = 0 off, 1 stage events, 2 stage events plus an event pair around every
+ * launch of the tridiagonal reduction's Hermitian GEMV kernel (the dominant
+ * kernel), summed by wtb_bse_plan_kernel_ms. */
 int wtb_bse_plan_set_timing(wtb_bse_plan* plan, int on);
+int wtb_bse_plan_kernel_ms(const wtb_bse_plan* plan, double* total_ms, int* launches);
 /* execute() captures the pipeline into a HIP graph on its first call and
  * replays it afterwards (default on); off launches the kernels directly.
  * Timing mode always launches directly. */
 int wtb_bse_plan_set_graph(wtb_bse_plan* plan, int on);
 int wtb_bse_plan_stage_ms(const wtb_bse_plan* plan, double* ms);
 int wtb_bse_plan_destroy(wtb_bse_plan* plan);
+
+/* ------------------------------------------------------------------------
+ * Iterative form (no counterpart in the reference, which always runs the
+ * full ZHEEV of bse_solver-tool-diel.f90:485-492): the lowest nev excitons of
+ * the BSE matrix by block Davidson on the COMPLEX*8 matrix (FP32 MFMA
+ * products, FP64 Rayleigh-Ritz and residuals), and eps(omega) from Lanczos
+ * continued fractions of the dipole vectors instead of all N oscillator
+ * strengths (so without the F15.4 round trip).  The plan must be created
+ * with precision = WTB_MIXED.  The preconditioner is 1/(D_i - theta) or,
+ * with an MLP, the network's estimate of the regularised resolvent
+ * diagonal.  When Davidson does not converge and fallback = 1, the full
+ * FP64 solve of the plain path answers instead (used_fallback = 1).
+ * ------------------------------------------------------------------------ */
+typedef struct wtb_mlp wtb_mlp;
+typedef struct {
+  int nev;            /* lowest excitons wanted                              */
+  int guard;          /* extra Ritz pairs carried along (e.g. 32)            */
+  int max_sub;        /* basis size that triggers a restart (e.g. 3 (nev+guard)) */
+  int max_iter;
+  double tol;         /* residual 2-norm bound, eV                           */
+  int lanczos_steps;  /* chain length for eps(omega); 0: no eps              */
+  int fallback;       /* 1: full FP64 solve when Davidson does not converge   */
+} wtb_davidson_params;
+typedef struct {
+  int iters, matvecs, converged, used_fallback;
+  int iters_fp64, matvecs_fp64;     /* of those, with FP64 products (final polish) */
+  double max_resid;                 /* eV */
+  double ms_build, ms_solve, ms_eps; /* device time of the three phases */
+} wtb_davidson_info;
+/* W[nev] ascending, resid[nev] residual norms, eps[6][2][numbse] (optional),
+ * X[nev][N] complex eigenvectors (optional) */
+int wtb_bse_plan_lowest(wtb_bse_plan* plan, const wtb_davidson_params* dp, const wtb_mlp* mlp, double* W,
+                        double* resid, double* eps, double* X, wtb_davidson_info* info);
+/* diagonal D[N] and row statistics stats[3][N] (sum |H_ij|, sqrt(sum |H_ij|^2),
+ * max |H_ij|, j != i) of the plan's matrix: the preconditioner's inputs */
+int wtb_bse_plan_row_stats(wtb_bse_plan* plan, double* D, float* stats);
+
+/* The preconditioner network: 5 -> 128 -> 128 -> 1, tanh, FP32, trained and
+ * evaluated by the library's own kernels.  Features X[5][S] as described in
+ * wantibexos_amd/csrc/k_mlp.hip; parameters packed as mu[5] sd[5] W1 b1 W2 b2 w3 b3. */
+int wtb_mlp_create(wtb_ctx* ctx, int hidden, wtb_mlp** out);
+int wtb_mlp_nparams(const wtb_mlp* mlp);
+int wtb_mlp_set_params(wtb_mlp* mlp, const float* p, int n);
+int wtb_mlp_get_params(const wtb_mlp* mlp, float* p, int n);
+int wtb_mlp_forward(wtb_mlp* mlp, int S, const float* X, float* y);
+/* mean squared error over all S samples and its gradient (nparams floats) */
+int wtb_mlp_loss_grad(wtb_mlp* mlp, int S, const float* X, const float* t, float* loss, float* grad);
+/* Adam, shuffled batches, loss_hist[epochs] = mean batch loss per epoch */
+int wtb_mlp_train(wtb_mlp* mlp, int S, const float* X, const float* t, int epochs, int batch, float lr,
+                  unsigned seed, float* loss_hist);
+int wtb_mlp_destroy(wtb_mlp* mlp);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,18 @@ class _Params(C.Structure):
                 ("f154", C.c_int), ("precision", C.c_int)]
 
 
+class _DavidsonParams(C.Structure):
+    _fields_ = [("nev", C.c_int), ("guard", C.c_int), ("max_sub", C.c_int), ("max_iter", C.c_int),
+                ("tol", C.c_double), ("lanczos_steps", C.c_int), ("fallback", C.c_int)]
+
+
+class _DavidsonInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("matvecs", C.c_int), ("converged", C.c_int),
+                ("used_fallback", C.c_int), ("iters_fp64", C.c_int), ("matvecs_fp64", C.c_int),
+                ("max_resid", C.c_double), ("ms_build", C.c_double),
+                ("ms_solve", C.c_double), ("ms_eps", C.c_double)]
+
+
 def _load():
     if not _LIB_PATH.exists():
         raise ImportError(f"{_LIB_PATH} is not built: run python -m wantibexos_amd.build "
@@ -95,7 +107,18 @@ def _load():
         "wtb_bse_plan_set_timing": (ip, [vp, ip]),
         "wtb_bse_plan_set_graph": (ip, [vp, ip]),
         "wtb_bse_plan_stage_ms": (ip, [vp, vp]),
+        "wtb_bse_plan_kernel_ms": (ip, [vp, vp, vp]),
         "wtb_bse_plan_destroy": (ip, [vp]),
+        "wtb_bse_plan_lowest": (ip, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "wtb_bse_plan_row_stats": (ip, [vp, vp, vp]),
+        "wtb_mlp_create": (ip, [vp, ip, C.POINTER(vp)]),
+        "wtb_mlp_nparams": (ip, [vp]),
+        "wtb_mlp_set_params": (ip, [vp, vp, ip]),
+        "wtb_mlp_get_params": (ip, [vp, vp, ip]),
+        "wtb_mlp_forward": (ip, [vp, ip, vp, vp]),
+        "wtb_mlp_loss_grad": (ip, [vp, ip, vp, vp, vp, vp]),
+        "wtb_mlp_train": (ip, [vp, ip, vp, vp, ip, ip, C.c_float, C.c_uint, vp]),
+        "wtb_mlp_destroy": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -233,13 +256,15 @@ def eigsys(ctx: Context, model: TBModel, kpt):
     return E, np.ascontiguousarray(U.transpose(0, 2, 1)), nocp
 
 
-def zheev(ctx: Context, a):
-    """Eigenvalues of the Hermitian matrix whose upper triangle is `a` (ZHEEV('N','U'))."""
-    a = np.asfortranarray(np.asarray(a, dtype=np.complex128))
+def zheev(ctx: Context, a, vectors: bool = False):
+    """ZHEEV('N' or 'V', 'U') of the Hermitian matrix whose upper triangle is `a`:
+    the eigenvalues, or (eigenvalues, eigenvectors) with each vector's last
+    component real and >= 0."""
+    a = np.array(a, dtype=np.complex128, order="F", copy=True)
     n = a.shape[0]
     w = np.zeros(n)
-    _check(lib().wtb_zheev(ctx.h, 0, n, _p(a), n, _p(w)))
-    return w
+    _check(lib().wtb_zheev(ctx.h, int(vectors), n, _p(a), n, _p(w)))
+    return (w, a) if vectors else w
 
 
 def dielbsev(ctx: Context, x, h):
@@ -292,7 +317,16 @@ class BSEPlan:
         _check(lib().wtb_bse_plan_build(self.h))
 
     def set_timing(self, on=True):
+        """0/False off, 1/True per-stage events, 2 also per-launch events on the
+        tridiagonal reduction's GEMV kernel (kernel_ms)."""
         _check(lib().wtb_bse_plan_set_timing(self.h, int(on)))
+
+    def kernel_ms(self):
+        """(summed ms, launches) of the GEMV kernel in the last timing-2 execute."""
+        t = np.zeros(1)
+        n = np.zeros(1, dtype=np.int32)
+        _check(lib().wtb_bse_plan_kernel_ms(self.h, _p(t), _p(n)))
+        return float(t[0]), int(n[0])
 
     def set_graph(self, on=True):
         _check(lib().wtb_bse_plan_set_graph(self.h, int(on)))
@@ -339,9 +373,87 @@ class BSEPlan:
             return raw.view(np.float64).reshape(2, N)
         return raw
 
+    def lowest(self, nev=256, guard=32, max_sub=None, max_iter=60, tol=1e-6, lanczos_steps=600,
+               fallback=True, mlp=None, vectors=False):
+        """Iterative form (precision-form plan): the lowest `nev` excitons by block
+        Davidson and eps(omega) by Lanczos chains.  Returns dict W, resid, eps (or
+        None), X (N, nev) (or None), info."""
+        N = self.N
+        dp = _DavidsonParams(nev, guard, max_sub or 3 * (nev + guard), max_iter, tol, lanczos_steps,
+                             int(fallback))
+        W = np.zeros(nev)
+        res = np.zeros(nev)
+        eps = np.zeros((6, 2, int(self.params.numbse))) if lanczos_steps > 0 else None
+        X = np.zeros((nev, N), dtype=np.complex128) if vectors else None
+        info = _DavidsonInfo()
+        _check(lib().wtb_bse_plan_lowest(self.h, C.byref(dp), mlp.h if mlp is not None else None, _p(W),
+                                         _p(res), _p(eps) if eps is not None else None,
+                                         _p(X) if X is not None else None, C.byref(info)))
+        inf = {k: getattr(info, k) for k, _ in _DavidsonInfo._fields_}
+        return dict(W=W, resid=res, eps=eps, X=X.T if X is not None else None, info=inf)
+
+    def row_stats(self):
+        """Diagonal D (N,) and row statistics (3, N) of the matrix (precision-form plan)."""
+        D = np.zeros(self.N)
+        st = np.zeros((3, self.N), dtype=np.float32)
+        _check(lib().wtb_bse_plan_row_stats(self.h, _p(D), _p(st)))
+        return D, st
+
     def close(self):
         if self.h:
             lib().wtb_bse_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class MLP:
+    """The learned preconditioner network on the GPU (5 -> 128 -> 128 -> 1, tanh, FP32)."""
+
+    def __init__(self, ctx: Context, hidden: int = 128):
+        h = C.c_void_p()
+        _check(lib().wtb_mlp_create(ctx.h, hidden, C.byref(h)))
+        self.h = h
+        self.nparams = lib().wtb_mlp_nparams(h)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().wtb_mlp_set_params(self.h, _p(p), p.size))
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().wtb_mlp_get_params(self.h, _p(p), p.size))
+        return p
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)     # (5, S)
+        y = np.zeros(X.shape[1], dtype=np.float32)
+        _check(lib().wtb_mlp_forward(self.h, X.shape[1], _p(X), _p(y)))
+        return y
+
+    def loss_grad(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().wtb_mlp_loss_grad(self.h, X.shape[1], _p(X), _p(t), _p(loss), _p(g)))
+        return float(loss[0]), g
+
+    def train(self, X, t, epochs=20, batch=4096, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        hist = np.zeros(epochs, dtype=np.float32)
+        _check(lib().wtb_mlp_train(self.h, X.shape[1], _p(X), _p(t), epochs, batch, C.c_float(lr),
+                                   C.c_uint(seed), _p(hist)))
+        return hist
+
+    def close(self):
+        if self.h:
+            lib().wtb_mlp_destroy(self.h)
             self.h = None
 
     def __enter__(self):

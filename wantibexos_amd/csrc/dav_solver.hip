@@ -1,0 +1,266 @@
+// Block Davidson for the lowest excitons of the BSE matrix (the iterative form of
+// the eigensolve that subprograms/bse_solver-tool-diel.f90:485-492 does with a
+// full ZHEEV).  The matrix is the full Hermitian COMPLEX*8 copy; the products
+// H V run on FP32 MFMA, everything else (basis, projected matrix, Ritz pairs,
+// residuals) is FP64.
+//
+//   V (N x m) orthonormal, HV = H V, G = V^H H V
+//   loop: G = S diag(theta) S^H (tridiagonal reduction + divide and conquer +
+//         back-transform on the GPU), U = V S_k, HU = HV S_k, R = HU - U theta
+//         stop when the first nev residual norms are below tol
+//         T = P(R_unconverged)  (diagonal or learned preconditioner)
+//         T -= V (V^H T) twice, T <- T L^-H with T^H T = L L^H (columns of tiny
+//         pivots dropped), V += T, HV += H T, G grows by V^H (H T)
+//         restart with the k = nev + guard Ritz vectors when m would pass max_sub
+// The products run on FP32 MFMA until the residuals reach switch_tol, then the basis
+// restarts on the Ritz vectors with H U recomputed in FP64 arithmetic (same COMPLEX*8
+// matrix) and the remaining iterations use FP64 products, so that the residual bound
+// refers to the stored matrix and not to the FP32 rounding of the products.
+#include "wtb_kernels.h"
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace wtb {
+
+// G(r, c) for r >= m0 > c from the computed block G(c, r) (c < m0 <= r): Hermitian mirror
+__global__ void k_mirror_block(int m0, int m1, zc* G, int ld) {
+  const int r = m0 + blockIdx.x;
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  if (r >= m1 || c >= m0) return;
+  G[(size_t)r + (size_t)ld * c] = conj(G[(size_t)c + (size_t)ld * r]);
+}
+
+__global__ void k_diag_set(int k, const double* theta, zc* G, int ld) {
+  const int c = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= k || c >= k) return;
+  G[(size_t)r + (size_t)ld * c] = mkz(r == c ? theta[r] : 0.0, 0.0);
+}
+
+__global__ void k_unit_cols(int N, int k, const int* rows, zc* V, int ld) {
+  const int c = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= N || c >= k) return;
+  V[(size_t)r + (size_t)ld * c] = mkz(r == rows[c] ? 1.0 : 0.0, 0.0);
+}
+
+// columns src(map q) -> dst q
+__global__ void k_gather_cols(int N, int nq, const int* map, const zc* src, int lds, zc* dst, int ldd) {
+  const int q = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= N || q >= nq) return;
+  dst[(size_t)r + (size_t)ldd * q] = src[(size_t)r + (size_t)lds * map[q]];
+}
+
+namespace {
+
+struct SmallEig {
+  int mmax = 0, kmax = 0;
+  zc* A = nullptr;       // mmax x mmax working copy
+  void* tri = nullptr;
+  void* dcw = nullptr;
+  void* btw = nullptr;
+  double* lam = nullptr;
+  double* Z = nullptr;   // mmax x mmax
+};
+
+int small_alloc(SmallEig& e, int mmax, int kmax) {
+  e.mmax = mmax;
+  e.kmax = kmax;
+  WTB_HIP(hipMalloc(&e.A, sizeof(zc) * (size_t)mmax * mmax));
+  WTB_HIP(hipMalloc(&e.tri, tridiag_work_bytes(mmax, 32, 0)));
+  WTB_HIP(hipMalloc(&e.dcw, dc_work_bytes(mmax, 0, 1)));
+  WTB_HIP(hipMalloc(&e.btw, backtr_work_bytes(mmax, kmax)));
+  WTB_HIP(hipMalloc(&e.lam, sizeof(double) * mmax));
+  WTB_HIP(hipMalloc(&e.Z, sizeof(double) * (size_t)mmax * mmax));
+  return WTB_OK;
+}
+
+void small_free(SmallEig& e) {
+  hipFree(e.A);
+  hipFree(e.tri);
+  hipFree(e.dcw);
+  hipFree(e.btw);
+  hipFree(e.lam);
+  hipFree(e.Z);
+}
+
+// eigenpairs of the m x m Hermitian G (lower triangle used): all eigenvalues into
+// theta, the first k eigenvectors into S (m x k, ld lds)
+int small_heev(hipStream_t s, SmallEig& e, int m, const zc* G, int ldg, double* theta, zc* S, int lds, int k) {
+  WTB_HIP(hipMemcpy2DAsync(e.A, sizeof(zc) * m, G, sizeof(zc) * ldg, sizeof(zc) * m, m,
+                           hipMemcpyDeviceToDevice, s));
+  TridiagWork tw;
+  tridiag_bind(tw, e.tri, m, 32, 0);
+  tw.store_v = 1;
+  DCTree tree;
+  int rc = dc_prepare(e.dcw, m, 0, tree, 1);
+  if (rc) return rc;
+  if ((rc = run_tridiag(s, tw, e.A, m, nullptr))) return rc;
+  if ((rc = run_dc(s, e.dcw, tree, tw.d, tw.e, nullptr, theta, e.Z))) return rc;
+  return run_backtr(s, e.A, m, m, tw.tau, e.Z, 0, k, S, lds, e.btw);
+}
+
+}  // namespace
+
+int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, const float* stats,
+                 const DavidsonCfg& cfg, const MlpDev* mlp, double* d_W, double* d_resid, zc* d_X,
+                 DavidsonInfo& info) {
+  const int nev = cfg.nev;
+  const int nk = std::min(N, nev + cfg.guard);
+  const int msub = std::min(N, std::max(cfg.max_sub, 2 * nk));
+  if (nev < 1 || nev > N) {
+    set_error("davidson: nev must lie in 1..N");
+    return WTB_ERR_ARG;
+  }
+  info = DavidsonInfo();
+  // device buffers
+  zc *V = nullptr, *HV = nullptr, *G = nullptr, *S = nullptr, *U = nullptr, *HU = nullptr, *R = nullptr,
+     *T = nullptr, *T2 = nullptr, *P = nullptr, *Gb = nullptr, *Linv = nullptr;
+  double *theta = nullptr, *rn = nullptr;
+  int *idx = nullptr, *keep = nullptr, *map = nullptr;
+  SmallEig se;
+  int rc = WTB_OK;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)V, (void*)HV, (void*)G, (void*)S, (void*)U, (void*)HU, (void*)R, (void*)T,
+                    (void*)T2, (void*)P, (void*)Gb, (void*)Linv, (void*)theta, (void*)rn, (void*)idx,
+                    (void*)keep, (void*)map})
+      hipFree(p);
+    small_free(se);
+  };
+#define DAV_ALLOC(ptr, bytes)                                  \
+  if (hipMalloc(&ptr, bytes) != hipSuccess) {                  \
+    set_error("davidson: device allocation failed");           \
+    cleanup();                                                 \
+    return WTB_ERR_NOMEM;                                      \
+  }
+  DAV_ALLOC(V, sizeof(zc) * (size_t)N * msub);
+  DAV_ALLOC(HV, sizeof(zc) * (size_t)N * msub);
+  DAV_ALLOC(G, sizeof(zc) * (size_t)msub * msub);
+  DAV_ALLOC(S, sizeof(zc) * (size_t)msub * nk);
+  DAV_ALLOC(U, sizeof(zc) * (size_t)N * nk);
+  DAV_ALLOC(HU, sizeof(zc) * (size_t)N * nk);
+  DAV_ALLOC(R, sizeof(zc) * (size_t)N * nk);
+  DAV_ALLOC(T, sizeof(zc) * (size_t)N * nk);
+  DAV_ALLOC(T2, sizeof(zc) * (size_t)N * nk);
+  DAV_ALLOC(P, sizeof(zc) * (size_t)msub * nk);
+  DAV_ALLOC(Gb, sizeof(zc) * (size_t)nk * nk);
+  DAV_ALLOC(Linv, sizeof(zc) * (size_t)nk * nk);
+  DAV_ALLOC(theta, sizeof(double) * msub);
+  DAV_ALLOC(rn, sizeof(double) * nk);
+  DAV_ALLOC(idx, sizeof(int) * nk);
+  DAV_ALLOC(keep, sizeof(int) * nk);
+  DAV_ALLOC(map, sizeof(int) * nk);
+#undef DAV_ALLOC
+  if ((rc = small_alloc(se, msub, nk))) {
+    cleanup();
+    return rc;
+  }
+#define DAV_TRY(call)      \
+  if ((rc = (call))) {     \
+    cleanup();             \
+    return rc;             \
+  }
+  const zc one = mkz(1, 0), zero = mkz(0, 0), mone = mkz(-1, 0);
+  // start: unit vectors at the nk smallest diagonal entries
+  double Dmin = 0.0;
+  {
+    std::vector<double> hD(N);
+    DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);   // D comes from stream s
+    DAV_TRY(hipMemcpy(hD.data(), D, sizeof(double) * N, hipMemcpyDeviceToHost) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hD[a] < hD[b]; });
+    Dmin = hD[order[0]];
+    order.resize(nk);
+    DAV_TRY(hipMemcpy(idx, order.data(), sizeof(int) * nk, hipMemcpyHostToDevice) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    hipLaunchKernelGGL(k_unit_cols, dim3((N + 255) / 256, nk), dim3(256), 0, s, N, nk, idx, V, N);
+  }
+  bool hi = false;   // FP64 products
+  auto matvec = [&](int nc, const zc* X, zc* Y) {
+    return hi ? hmm_f32_acc64(s, N, nc, H, ldh, X, N, Y, N) : hmm_f32(s, N, nc, H, ldh, X, N, Y, N);
+  };
+  int m = nk;
+  DAV_TRY(matvec(m, V, HV));
+  info.matvecs += m;
+  DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
+  std::vector<double> h_rn(nk);
+  std::vector<int> h_keep(nk), h_idx(nk), h_map(nk);   // live across the asynchronous copies
+  for (int it = 0; it < cfg.max_iter; ++it) {
+    info.iters = it + 1;
+    DAV_TRY(small_heev(s, se, m, G, msub, theta, S, msub, nk));
+    DAV_TRY(zgemm(s, 'N', 'N', N, nk, m, one, V, N, S, msub, zero, U, N));
+    DAV_TRY(zgemm(s, 'N', 'N', N, nk, m, one, HV, N, S, msub, zero, HU, N));
+    DAV_TRY(launch_residual(s, N, nk, U, HU, N, theta, R, rn));
+    DAV_TRY(hipMemcpyAsync(h_rn.data(), rn, sizeof(double) * nk, hipMemcpyDeviceToHost, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    double mx = 0.0;
+    for (int c = 0; c < nev; ++c) mx = std::max(mx, h_rn[c]);
+    info.max_resid = mx;
+    if (hi) ++info.iters_fp64;
+    if (mx <= cfg.tol && hi) {
+      info.converged = 1;
+      break;
+    }
+    if (it + 1 == cfg.max_iter) break;
+    if (!hi && (mx <= cfg.switch_tol || mx <= cfg.tol)) {
+      // switch to FP64 products: restart on the Ritz vectors with H U recomputed
+      hi = true;
+      DAV_TRY(hipMemcpyAsync(V, U, sizeof(zc) * (size_t)N * nk, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      DAV_TRY(matvec(nk, V, HV));
+      info.matvecs += nk;
+      info.matvecs_fp64 += nk;
+      m = nk;
+      DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
+      continue;
+    }
+    int nq = 0;
+    for (int c = 0; c < nk; ++c)
+      if (h_rn[c] > cfg.tol) h_idx[nq++] = c;
+    if (m + nq > msub) {
+      // restart on the Ritz vectors
+      DAV_TRY(hipMemcpyAsync(V, U, sizeof(zc) * (size_t)N * nk, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      DAV_TRY(hipMemcpyAsync(HV, HU, sizeof(zc) * (size_t)N * nk, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      hipLaunchKernelGGL(k_diag_set, dim3((nk + 255) / 256, nk), dim3(256), 0, s, nk, theta, G, msub);
+      m = nk;
+    }
+    DAV_TRY(hipMemcpyAsync(idx, h_idx.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    if (mlp && mlp->w) {
+      DAV_TRY(launch_precond_mlp(s, N, nq, idx, R, N, theta, D, stats, Dmin, *mlp, T, N));
+    } else {
+      DAV_TRY(launch_precond_diag(s, N, nq, idx, R, N, theta, D, cfg.dmin, T, N));
+    }
+    // T -= V (V^H T), twice
+    for (int pass = 0; pass < 2; ++pass) {
+      DAV_TRY(zgemm(s, 'C', 'N', m, nq, N, one, V, N, T, N, zero, P, msub));
+      DAV_TRY(zgemm(s, 'N', 'N', N, nq, m, mone, V, N, P, msub, one, T, N));
+    }
+    // T <- T L^-H, T^H T = L L^H
+    DAV_TRY(zgemm(s, 'C', 'N', nq, nq, N, one, T, N, T, N, zero, Gb, nk));
+    DAV_TRY(launch_chol_inv(s, nq, Gb, nk, Linv, keep, 1e-12));
+    DAV_TRY(zgemm(s, 'N', 'C', N, nq, nq, one, T, N, Linv, nq, zero, T2, N));
+    DAV_TRY(hipMemcpyAsync(h_keep.data(), keep, sizeof(int) * nq, hipMemcpyDeviceToHost, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    int nn = 0;
+    for (int q = 0; q < nq; ++q)
+      if (h_keep[q]) h_map[nn++] = q;
+    if (nn == 0) break;   // no new direction: stagnation
+    DAV_TRY(hipMemcpyAsync(map, h_map.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+    zc* Vn = V + (size_t)N * m;
+    hipLaunchKernelGGL(k_gather_cols, dim3((N + 255) / 256, nn), dim3(256), 0, s, N, nn, map, T2, N, Vn, N);
+    DAV_TRY(matvec(nn, Vn, HV + (size_t)N * m));
+    info.matvecs += nn;
+    if (hi) info.matvecs_fp64 += nn;
+    // G(0:m+nn, m:m+nn) = V^H (H Vn); mirror into G(m:m+nn, 0:m)
+    DAV_TRY(zgemm(s, 'C', 'N', m + nn, nn, N, one, V, N, HV + (size_t)N * m, N, zero, G + (size_t)msub * m, msub));
+    hipLaunchKernelGGL(k_mirror_block, dim3(nn, (m + 255) / 256), dim3(256), 0, s, m, m + nn, G, msub);
+    m += nn;
+  }
+  DAV_TRY(hipMemcpyAsync(d_W, theta, sizeof(double) * nev, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+  DAV_TRY(hipMemcpyAsync(d_resid, rn, sizeof(double) * nev, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+  if (d_X) DAV_TRY(hipMemcpyAsync(d_X, U, sizeof(zc) * (size_t)N * nev, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+  DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+#undef DAV_TRY
+  cleanup();
+  return WTB_OK;
+}
+
+}  // namespace wtb

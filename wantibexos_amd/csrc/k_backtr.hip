@@ -1,0 +1,274 @@
+// Stage (d3): eigenvectors of the Hermitian matrix from those of its tridiagonal,
+// X = Q Z, with Q = H(0) H(1) ... H(N-2) the product of the reduction's
+// reflectors H(j) = I - tau_j v_j v_j^H (v_j(j+1) = 1, v_j(r) for r > j+1 kept in
+// the lower triangle of the reduced matrix, column j; the layout of ZHETRD with
+// UPLO = 'L').  Panels of nb reflectors are applied last to first in compact WY
+// form, I - V T V^H, with T upper triangular as ZLARFT ('F', 'C') builds it:
+//   T(i,i) = tau_i,  T(0:i-1, i) = -tau_i T(0:i-1, 0:i-1) V(:, 0:i-1)^H v_i.
+// Per panel: Y = V^H X (row chunks on FP64 MFMA, partial sums), Y <- T Y, and
+// X -= V Y (FP64 MFMA), so the work is two complex GEMMs of N' x nb x m.
+#include "wtb_kernels.h"
+
+namespace wtb {
+
+#define BT_NB 32      // reflectors per panel (the reduction's panel width)
+#define BT_CHUNK 256  // rows per V^H X partial
+
+template <class T>
+__device__ inline zc vref(const T* H, int ld, int r, int j) {
+  // v_j(r): 0 above row j+1, 1 at row j+1, the stored entry below
+  if (r <= j) return mkz(0, 0);
+  if (r == j + 1) return mkz(1, 0);
+  return ldz(H[(size_t)r + (size_t)ld * j]);
+}
+
+// T factor of panel blockIdx.x (reflectors p .. p+kp-1), one workgroup per panel
+template <class T>
+__global__ __launch_bounds__(256) void k_bt_tfactor(const T* __restrict__ H, int ld, int N,
+                                                    const zc* __restrict__ tau, zc* __restrict__ Tf) {
+  __shared__ zc Vs[32][BT_NB + 1];
+  __shared__ zc G[BT_NB][BT_NB + 1];
+  __shared__ zc Ts[BT_NB][BT_NB + 1];
+  const int p = blockIdx.x * BT_NB;
+  const int kp = min(BT_NB, N - 1 - p);
+  const int tid = threadIdx.x;
+  // G(i', i) = v_{p+i'}^H v_{p+i} for i' < i: pairs spread over the threads
+  const int npair = kp * (kp - 1) / 2;
+  zc acc[2] = {mkz(0, 0), mkz(0, 0)};
+  int pi[2], pj[2];
+  for (int s = 0; s < 2; ++s) {
+    int t = tid + 256 * s;
+    // pair t -> (i' < i): row i has i pairs
+    int i = 1;
+    while (t >= i && i < BT_NB) {
+      t -= i;
+      ++i;
+    }
+    pi[s] = t;
+    pj[s] = i;
+  }
+  for (int r0 = p + 1; r0 < N; r0 += 32) {
+    for (int e = tid; e < 32 * BT_NB; e += 256) {
+      const int rl = e & 31, c = e >> 5;
+      const int r = r0 + rl;
+      Vs[rl][c] = (r < N && c < kp) ? vref(H, ld, r, p + c) : mkz(0, 0);
+    }
+    __syncthreads();
+    for (int s = 0; s < 2; ++s) {
+      if (tid + 256 * s >= npair) continue;
+      for (int rl = 0; rl < 32; ++rl) acc[s] = fma_conj(Vs[rl][pi[s]], Vs[rl][pj[s]], acc[s]);
+    }
+    __syncthreads();
+  }
+  for (int s = 0; s < 2; ++s)
+    if (tid + 256 * s < npair) G[pi[s]][pj[s]] = acc[s];
+  __syncthreads();
+  // T column by column; thread r < i computes T(r, i) = -tau_i sum_{k=r}^{i-1} T(r,k) G(k,i)
+  for (int i = 0; i < kp; ++i) {
+    const zc ti = tau[p + i];
+    if (tid < i) {
+      zc sacc = mkz(0, 0);
+      for (int k = tid; k < i; ++k) sacc = fma_z(Ts[tid][k], G[k][i], sacc);
+      Ts[tid][i] = mkz(0, 0) - ti * sacc;
+    } else if (tid == i) {
+      Ts[i][i] = ti;
+    }
+    __syncthreads();
+  }
+  zc* out = Tf + (size_t)blockIdx.x * BT_NB * BT_NB;
+  for (int e = tid; e < BT_NB * BT_NB; e += 256) {
+    const int r = e % BT_NB, c = e / BT_NB;
+    out[e] = (r < kp && c < kp && r <= c) ? Ts[r][c] : mkz(0, 0);
+  }
+}
+
+// Ypart[chunk][c][col] = sum over the chunk's rows of conj(V(r, c)) X(r, col).
+// Workgroup: 32 reflectors x 64 columns; wave w: reflectors 16 (w >> 1), columns 32 (w & 1).
+template <class T>
+__global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld, int N, int p, int kp,
+                                                const zc* __restrict__ X, int ldx, int m,
+                                                zc* __restrict__ Ypart) {
+  const int col0 = blockIdx.x * 64;
+  const int rbeg = p + 1 + blockIdx.y * BT_CHUNK;
+  const int rend = min(N, rbeg + BT_CHUNK);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int cb = (wave >> 1) * 16;           // reflector block
+  const int nb0 = col0 + (wave & 1) * 32;    // column block
+  d4 yr[2], yi[2];
+  for (int y = 0; y < 2; ++y) {
+    yr[y] = d4{0, 0, 0, 0};
+    yi[y] = d4{0, 0, 0, 0};
+  }
+  for (int r0 = rbeg; r0 < rend; r0 += 4) {
+    const int r = r0 + lk;
+    zc v = mkz(0, 0);
+    const int c = cb + lr;
+    if (r < rend && c < kp) v = vref(H, ld, r, p + c);
+    zc xv[2];
+    for (int y = 0; y < 2; ++y) {
+      const int col = nb0 + y * 16 + lr;
+      xv[y] = (r < rend && col < m) ? X[(size_t)r + (size_t)ldx * col] : mkz(0, 0);
+    }
+    for (int y = 0; y < 2; ++y) {
+      // conj(v) x = (vr xr + vi xi) + i (vr xi - vi xr)
+      yr[y] = mfma16x16x4(v.re, xv[y].re, yr[y]);
+      yr[y] = mfma16x16x4(v.im, xv[y].im, yr[y]);
+      yi[y] = mfma16x16x4(v.re, xv[y].im, yi[y]);
+      yi[y] = mfma16x16x4(-v.im, xv[y].re, yi[y]);
+    }
+  }
+  zc* out = Ypart + (size_t)blockIdx.y * BT_NB * m;
+  for (int y = 0; y < 2; ++y) {
+    const int col = nb0 + y * 16 + lr;
+    if (col >= m) continue;
+    for (int q = 0; q < 4; ++q) {
+      const int c = cb + lk + 4 * q;           // f64 MFMA: register q holds row lk + 4q
+      out[(size_t)c * m + col] = mkz(yr[y][q], yi[y][q]);
+    }
+  }
+}
+
+// Y(:, col) = T sum_chunks Ypart(:, col), one thread per column
+__global__ __launch_bounds__(64) void k_bt_ty(const zc* __restrict__ Ypart, int nchunk, int m, int kp,
+                                              const zc* __restrict__ Tf, zc* __restrict__ Y) {
+  __shared__ zc Ts[BT_NB * BT_NB];
+  for (int e = threadIdx.x; e < BT_NB * BT_NB; e += 64) Ts[e] = Tf[e];
+  __syncthreads();
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= m) return;
+  zc s[BT_NB];
+  for (int c = 0; c < BT_NB; ++c) s[c] = mkz(0, 0);
+  for (int ch = 0; ch < nchunk; ++ch)
+    for (int c = 0; c < kp; ++c) s[c] += Ypart[((size_t)ch * BT_NB + c) * m + col];
+  for (int r = 0; r < BT_NB; ++r) {
+    zc acc = mkz(0, 0);
+    for (int c = r; c < kp; ++c) acc = fma_z(Ts[r + BT_NB * c], s[c], acc);   // T column-major
+    Y[(size_t)r * m + col] = acc;
+  }
+}
+
+// X(r, col) -= sum_c V(r, c) Y(c, col) for rows r > p; 64 x 64 tiles, waves of 32 x 32
+template <class T>
+__global__ __launch_bounds__(256) void k_bt_update(const T* __restrict__ H, int ld, int N, int p, int kp,
+                                                   const zc* __restrict__ Y, zc* __restrict__ X, int ldx,
+                                                   int m) {
+  const int rt = p + 1 + blockIdx.x * 64, col0 = blockIdx.y * 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int rb = rt + (wave >> 1) * 32, cb = col0 + (wave & 1) * 32;
+  d4 ar[2][2], ai[2][2];
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) {
+      ar[x][y] = d4{0, 0, 0, 0};
+      ai[x][y] = d4{0, 0, 0, 0};
+    }
+  for (int k0 = 0; k0 < kp; k0 += 4) {
+    const int k = k0 + lk;
+    zc v[2], yv[2];
+    for (int x = 0; x < 2; ++x) {
+      const int r = rb + x * 16 + lr;
+      v[x] = (k < kp && r < N) ? vref(H, ld, r, p + k) : mkz(0, 0);
+    }
+    for (int y = 0; y < 2; ++y) {
+      const int col = cb + y * 16 + lr;
+      yv[y] = (k < kp && col < m) ? Y[(size_t)k * m + col] : mkz(0, 0);
+    }
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y) {
+        ar[x][y] = mfma16x16x4(v[x].re, yv[y].re, ar[x][y]);
+        ar[x][y] = mfma16x16x4(-v[x].im, yv[y].im, ar[x][y]);
+        ai[x][y] = mfma16x16x4(v[x].re, yv[y].im, ai[x][y]);
+        ai[x][y] = mfma16x16x4(v[x].im, yv[y].re, ai[x][y]);
+      }
+  }
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) {
+      const int col = cb + y * 16 + lr;
+      if (col >= m) continue;
+      for (int q = 0; q < 4; ++q) {
+        const int r = rb + x * 16 + lk + 4 * q;
+        if (r >= N) continue;
+        zc& d = X[(size_t)r + (size_t)ldx * col];
+        d.re -= ar[x][y][q];
+        d.im -= ai[x][y][q];
+      }
+    }
+}
+
+// X(:, c) = Z(:, c0 + c) as complex
+__global__ void k_bt_load(int N, int m, const double* __restrict__ Z, int c0, zc* __restrict__ X, int ldx) {
+  const size_t tot = (size_t)N * m;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(t % N), c = (int)(t / N);
+    X[(size_t)r + (size_t)ldx * c] = mkz(Z[(size_t)r + (size_t)N * (c0 + c)], 0.0);
+  }
+}
+
+// phase convention: the last component of each vector real and >= 0 (the eigenvectors
+// of ZHEEV with UPLO = 'U' have a real last component)
+__global__ __launch_bounds__(256) void k_bt_phase(int N, int m, zc* __restrict__ X, int ldx) {
+  __shared__ zc s_last;
+  const int c = blockIdx.x;
+  if (c >= m) return;
+  zc* x = X + (size_t)ldx * c;
+  if (threadIdx.x == 0) s_last = x[N - 1];
+  __syncthreads();
+  const zc last = s_last;
+  const double a = sqrt(norm2(last));
+  if (!(a > 0.0)) return;
+  const zc ph = mkz(last.re / a, -last.im / a);
+  for (int r = threadIdx.x; r < N - 1; r += 256) x[r] = x[r] * ph;
+  if (threadIdx.x == 0) x[N - 1] = mkz(a, 0.0);
+}
+
+size_t backtr_work_bytes(int N, int m) {
+  const size_t np = (N + BT_NB - 1) / BT_NB + 1;
+  const size_t nch = (N + BT_CHUNK - 1) / BT_CHUNK + 1;
+  return sizeof(zc) * (np * BT_NB * BT_NB + nch * BT_NB * (size_t)m + BT_NB * (size_t)m) + 3 * 256;
+}
+
+template <class T>
+static int run_backtr_t(hipStream_t s, const T* H, int ld, int N, const zc* tau, const double* Z, int c0,
+                        int m, zc* X, int ldx, void* work) {
+  if (N < 1 || m < 1) return WTB_OK;
+  char* w = static_cast<char*>(work);
+  const int np = (N - 1 + BT_NB - 1) / BT_NB;   // panels of reflectors 0 .. N-2
+  zc* Tf = (zc*)w;
+  w += ((sizeof(zc) * (size_t)(np + 1) * BT_NB * BT_NB + 255) & ~size_t(255));
+  zc* Ypart = (zc*)w;
+  const size_t nchmax = (N + BT_CHUNK - 1) / BT_CHUNK + 1;
+  w += ((sizeof(zc) * nchmax * BT_NB * (size_t)m + 255) & ~size_t(255));
+  zc* Y = (zc*)w;
+  const int gl = (int)std::min<size_t>(((size_t)N * m + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_bt_load, dim3(gl), dim3(256), 0, s, N, m, Z, c0, X, ldx);
+  if (np > 0) {
+    hipLaunchKernelGGL(k_bt_tfactor<T>, dim3(np), dim3(256), 0, s, H, ld, N, tau, Tf);
+    const int ncol = (m + 63) / 64;
+    for (int b = np - 1; b >= 0; --b) {
+      const int p = b * BT_NB;
+      const int kp = std::min(BT_NB, N - 1 - p);
+      const int rows = N - p - 1;
+      const int nch = (rows + BT_CHUNK - 1) / BT_CHUNK;
+      hipLaunchKernelGGL(k_bt_vhx<T>, dim3(ncol, nch), dim3(256), 0, s, H, ld, N, p, kp, X, ldx, m, Ypart);
+      hipLaunchKernelGGL(k_bt_ty, dim3(ncol), dim3(64), 0, s, Ypart, nch, m, kp,
+                         Tf + (size_t)b * BT_NB * BT_NB, Y);
+      hipLaunchKernelGGL(k_bt_update<T>, dim3((rows + 63) / 64, ncol), dim3(256), 0, s, H, ld, N, p, kp,
+                         Y, X, ldx, m);
+    }
+  }
+  hipLaunchKernelGGL(k_bt_phase, dim3(m), dim3(256), 0, s, N, m, X, ldx);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int run_backtr(hipStream_t s, const zc* H, int ld, int N, const zc* tau, const double* Z, int c0, int m,
+               zc* X, int ldx, void* work) {
+  return run_backtr_t<zc>(s, H, ld, N, tau, Z, c0, m, X, ldx, work);
+}
+int run_backtr_f32(hipStream_t s, const zf* H, int ld, int N, const zc* tau, const double* Z, int c0,
+                   int m, zc* X, int ldx, void* work) {
+  return run_backtr_t<zf>(s, H, ld, N, tau, Z, c0, m, X, ldx, work);
+}
+
+}  // namespace wtb

@@ -1,0 +1,496 @@
+// Kernels of the iterative form of stage (d)+(e): the lowest excitons by block
+// Davidson on the full Hermitian BSE matrix, and eps(omega) by Lanczos
+// continued fractions (Haydock) instead of all N oscillator strengths.
+//
+// eps(omega) (optdiel, subroutines/optics.f90:160-244) is a sum over excitons of
+// f_n L(omega - E_n) with f_n = |A_n^T h|^2 (no conjugate on A, bse_subs.f90:336-378).
+// conj(A_n) is an eigenvector of conj(H) with the same E_n, so
+//   sum_n f_n / (E_n - z) = conj(h)^H (H - z)^-1 conj(h),
+// the resolvent of H on the start vector conj(h): a Lanczos chain from conj(h)
+// gives it as the continued fraction |h|^2 / (z - a0 - b1^2 / (z - a1 - ...)) with
+// z = omega + i gamma.  Off-diagonal strengths Re(t_a conj(t_b)) follow from the
+// chains of h_a + h_b and h_a - h_b (polarisation).
+#include "wtb_kernels.h"
+
+namespace wtb {
+
+__device__ inline zf conjT(zf v) { v.im = -v.im; return v; }
+__device__ inline zc conjT(zc v) { v.im = -v.im; return v; }
+
+// ---- full Hermitian matrix from the lower triangle: 64 x 64 tiles through LDS ----------
+template <class T>
+__global__ __launch_bounds__(256) void k_sym_fill(int N, T* __restrict__ H, int ld) {
+  __shared__ T tile[64][65];
+  int t = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  const int R0 = ti * 64, C0 = tj * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int rl = e & 63, cl = e >> 6;
+    const int r = R0 + rl, c = C0 + cl;
+    if (r < N && c < N && r >= c) tile[cl][rl] = H[(size_t)r + (size_t)ld * c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int cl = e & 63, rl = e >> 6;
+    const int r = R0 + rl, c = C0 + cl;
+    if (r < N && c < N && r >= c) {
+      T v = tile[cl][rl];
+      if (r == c) {
+        v.im = 0;
+        H[(size_t)r + (size_t)ld * r] = v;
+      } else {
+        H[(size_t)c + (size_t)ld * r] = conjT(v);
+      }
+    }
+  }
+}
+
+// ---- per basis state i: D_i = Re H_ii; over j != i: sum |H_ij|, sqrt(sum |H_ij|^2), max |H_ij|
+template <class T>
+__global__ __launch_bounds__(256) void k_row_stats(int N, const T* __restrict__ H, int ld, double* __restrict__ D,
+                                                   float* __restrict__ st) {
+  __shared__ double r1[256], r2[256], r3[256];
+  const int i = blockIdx.x;
+  const T* col = H + (size_t)ld * i;      // column i = conj(row i)
+  double s1 = 0.0, s2 = 0.0, mx = 0.0;
+  for (int j = threadIdx.x; j < N; j += 256) {
+    if (j == i) continue;
+    const zc v = ldz(col[j]);
+    const double a2 = norm2(v), a = sqrt(a2);
+    s1 += a;
+    s2 += a2;
+    mx = fmax(mx, a);
+  }
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  r3[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+      r3[threadIdx.x] = fmax(r3[threadIdx.x], r3[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D[i] = ldz(col[i]).re;
+    st[i] = (float)r1[0];
+    st[(size_t)N + i] = (float)sqrt(r2[0]);
+    st[2 * (size_t)N + i] = (float)r3[0];
+  }
+}
+
+// ---- R(:,c) = HU(:,c) - theta_c U(:,c) and its norm, one workgroup per column --------
+__global__ __launch_bounds__(256) void k_residual(int N, const zc* __restrict__ U, const zc* __restrict__ HU,
+                                                  int ld, const double* __restrict__ theta, zc* __restrict__ R,
+                                                  double* __restrict__ rn) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  const double th = theta[c];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) {
+    const size_t o = (size_t)i + (size_t)ld * c;
+    const zc r = HU[o] - th * U[o];
+    R[o] = r;
+    s += norm2(r);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rn[c] = sqrt(red[0]);
+}
+
+// ---- diagonal preconditioner: T(:,q) = R(:,idx q) / (D - theta_idx q), |den| >= dmin --------
+__global__ __launch_bounds__(256) void k_precond_diag(int N, const int* __restrict__ idx, const zc* __restrict__ R,
+                                                      int ld, const double* __restrict__ theta,
+                                                      const double* __restrict__ D, double dmin,
+                                                      zc* __restrict__ T, int ldt) {
+  const int q = blockIdx.y, c = idx[q];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  double den = D[i] - theta[c];
+  if (fabs(den) < dmin) den = den < 0.0 ? -dmin : dmin;
+  T[(size_t)i + (size_t)ldt * q] = R[(size_t)i + (size_t)ld * c] * (1.0 / den);
+}
+
+// ---- Cholesky of a Gram matrix and the inverse of its factor (one workgroup) ----------
+// G (b x b Hermitian, lower used) = L L^H, Linv = L^-1 (b x b, column-major).  A pivot
+// below tol * max(diag) drops its column (keep = 0; L(:,k) = e_k, so the remaining
+// columns of T Linv^H do not involve it).
+__global__ __launch_bounds__(256) void k_chol_inv(int b, zc* __restrict__ G, int ld, zc* __restrict__ Linv,
+                                                  int* __restrict__ keep, double tol) {
+  __shared__ double s_piv, s_dmax;
+  __shared__ int s_drop;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    double m = 0.0;
+    for (int i = 0; i < b; ++i) m = fmax(m, G[(size_t)i + (size_t)ld * i].re);
+    s_dmax = m;
+  }
+  __syncthreads();
+  const double thr = tol * s_dmax;
+  for (int k = 0; k < b; ++k) {
+    if (tid == 0) {
+      const double p = G[(size_t)k + (size_t)ld * k].re;
+      s_drop = !(p > thr);
+      s_piv = s_drop ? 1.0 : sqrt(p);
+      keep[k] = s_drop ? 0 : 1;
+    }
+    __syncthreads();
+    const double piv = s_piv;
+    const int drop = s_drop;
+    for (int i = k + tid; i < b; i += 256) {
+      zc& g = G[(size_t)i + (size_t)ld * k];
+      if (i == k) g = mkz(piv, 0.0);
+      else g = drop ? mkz(0, 0) : g * (1.0 / piv);
+    }
+    __syncthreads();
+    if (!drop) {
+      const int m = b - k - 1;
+      for (int e = tid; e < m * m; e += 256) {
+        const int ii = k + 1 + e % m, jj = k + 1 + e / m;
+        if (ii < jj) continue;
+        const zc lik = G[(size_t)ii + (size_t)ld * k], ljk = G[(size_t)jj + (size_t)ld * k];
+        G[(size_t)ii + (size_t)ld * jj] -= lik * conj(ljk);
+      }
+    }
+    __syncthreads();
+  }
+  // Linv column c by forward substitution, one thread per column
+  for (int c = tid; c < b; c += 256) {
+    for (int i = 0; i < b; ++i) {
+      zc acc = mkz(i == c ? 1.0 : 0.0, 0.0);
+      if (i > c) {
+        acc = mkz(0, 0);
+        for (int j = c; j < i; ++j) acc -= G[(size_t)i + (size_t)ld * j] * Linv[(size_t)j + (size_t)b * c];
+      }
+      if (i < c) {
+        Linv[(size_t)i + (size_t)b * c] = mkz(0, 0);
+        continue;
+      }
+      const double d = G[(size_t)i + (size_t)ld * i].re;
+      Linv[(size_t)i + (size_t)b * c] = acc * (1.0 / d);
+    }
+  }
+}
+
+// ---- Lanczos (Haydock) chains, nq columns at once ---------------------------------------
+// part 1: w = H v - b_j v_prev; partials of Re(v^H H v) and |w|^2 per workgroup
+__global__ __launch_bounds__(256) void k_lz_a(int N, const zc* __restrict__ V, const zc* __restrict__ Vp,
+                                              const zc* __restrict__ HV, const double* __restrict__ bcur,
+                                              zc* __restrict__ W, double* __restrict__ part) {
+  __shared__ double ra[256], rw[256];
+  const int q = blockIdx.y;
+  const size_t off = (size_t)N * q;
+  const double bj = bcur[q];
+  double sa = 0.0, sw = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += 256 * gridDim.x) {
+    const zc v = V[off + i], hv = HV[off + i];
+    sa += v.re * hv.re + v.im * hv.im;
+    const zc w = hv - bj * Vp[off + i];
+    W[off + i] = w;
+    sw += norm2(w);
+  }
+  ra[threadIdx.x] = sa;
+  rw[threadIdx.x] = sw;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      ra[threadIdx.x] += ra[threadIdx.x + s];
+      rw[threadIdx.x] += rw[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[((size_t)q * gridDim.x + blockIdx.x) * 2] = ra[0];
+    part[((size_t)q * gridDim.x + blockIdx.x) * 2 + 1] = rw[0];
+  }
+}
+
+// part 2: a_j = sum, b_{j+1} = sqrt(|w|^2 - a_j^2); v_next = (w - a_j v) / b_{j+1}.  Every
+// workgroup sums the partials of its chain in the same order; workgroup 0 records them.
+__global__ __launch_bounds__(256) void k_lz_b(int N, int j, int kmax, const zc* __restrict__ V,
+                                              const zc* __restrict__ W, const double* __restrict__ part,
+                                              int nblk, double* __restrict__ acoef, double* __restrict__ bcoef,
+                                              double* __restrict__ bcur, zc* __restrict__ Vn) {
+  __shared__ double s_a, s_b;
+  const int q = blockIdx.y;
+  if (threadIdx.x == 0) {
+    double a = 0.0, w2 = 0.0;
+    for (int k = 0; k < nblk; ++k) {
+      a += part[((size_t)q * nblk + k) * 2];
+      w2 += part[((size_t)q * nblk + k) * 2 + 1];
+    }
+    const double b2 = w2 - a * a;
+    s_a = a;
+    s_b = b2 > 0.0 ? sqrt(b2) : 0.0;
+  }
+  __syncthreads();
+  const double a = s_a, b = s_b;
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    acoef[(size_t)q * kmax + j] = a;
+    bcoef[(size_t)q * kmax + j] = b;      // b_{j+1}
+    bcur[q] = b;
+  }
+  const size_t off = (size_t)N * q;
+  const double inv = b > 0.0 ? 1.0 / b : 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += 256 * gridDim.x)
+    Vn[off + i] = (W[off + i] - a * V[off + i]) * inv;
+}
+
+// start vectors conj(h), conj(h_a + h_b), conj(h_a - h_b) (normalised by the host-side
+// norms computed in k_lz_norms)
+__global__ __launch_bounds__(256) void k_lz_start(int N, int nq, const zc* __restrict__ dip,
+                                                  const int* __restrict__ ca, const int* __restrict__ cb,
+                                                  const double* __restrict__ sg, zc* __restrict__ V) {
+  const int q = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N || q >= nq) return;
+  zc h = dip[(size_t)ca[q] * N + i];
+  if (cb[q] >= 0) h = h + sg[q] * dip[(size_t)cb[q] * N + i];
+  V[(size_t)q * N + i] = conj(h);
+}
+
+__global__ __launch_bounds__(256) void k_lz_norms(int N, zc* __restrict__ V, double* __restrict__ hn2,
+                                                  zc* __restrict__ Vprev, double* __restrict__ bcur) {
+  __shared__ double red[256];
+  const int q = blockIdx.x;
+  zc* v = V + (size_t)N * q;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) s += norm2(v[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double n2 = red[0];
+  const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) {
+    v[i] = v[i] * inv;
+    Vprev[(size_t)N * q + i] = mkz(0, 0);
+  }
+  if (threadIdx.x == 0) {
+    hn2[q] = n2;
+    bcur[q] = 0.0;
+  }
+}
+
+// largest eigenvalue of the chain-0 Lanczos tridiagonal (a_0..a_{k-1}, b_1..b_{k-1}) by
+// Sturm-count bisection; gamma = sme + 10 (E_max - E_1) / N as avgsme telescopes
+__global__ void k_lz_gamma(int k, const double* __restrict__ acoef, const double* __restrict__ bcoef,
+                           const double* __restrict__ E1, int N, double* __restrict__ smeavg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < k; ++i) {
+    const double r = (i > 0 ? fabs(bcoef[i - 1]) : 0.0) + (i + 1 < k ? fabs(bcoef[i]) : 0.0);
+    lo = fmin(lo, acoef[i] - r);
+    hi = fmax(hi, acoef[i] + r);
+  }
+  // count eigenvalues below x; the largest is where the count reaches k
+  for (int it = 0; it < 200; ++it) {
+    const double x = 0.5 * (lo + hi);
+    if (x <= lo || x >= hi) break;
+    int cnt = 0;
+    double d = 1.0;
+    for (int i = 0; i < k; ++i) {
+      const double b2 = i > 0 ? bcoef[i - 1] * bcoef[i - 1] : 0.0;
+      d = (acoef[i] - x) - (i > 0 ? b2 / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0.0) ++cnt;
+    }
+    if (cnt >= k) hi = x; else lo = x;
+  }
+  smeavg[0] = 10.0 * ((hi - E1[0]) / (double)N);
+  smeavg[1] = hi;
+}
+
+// eps(omega) from the chains: S_q(z) = -|h_q|^2 G_q(z), z = omega + i gamma; the tail past
+// step k is closed by the square-root terminator of the last coefficients' mean.
+__device__ inline zc zdiv(zc a, zc b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return mkz((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+__device__ inline zc zsqrt(zc a) {
+  const double r = sqrt(sqrt(a.re * a.re + a.im * a.im));
+  const double t = 0.5 * atan2(a.im, a.re);
+  return mkz(r * cos(t), r * sin(t));
+}
+__global__ __launch_bounds__(256) void k_haydock_eps(int nq, int k, int kmax, const double* __restrict__ acoef,
+                                                     const double* __restrict__ bcoef, const double* __restrict__ hn2,
+                                                     double caux, double ebse0, double ebsef, int numbse,
+                                                     double sme, const double* __restrict__ smeavg, int dtfull,
+                                                     double* __restrict__ eps) {
+  const int jw = blockIdx.x * 256 + threadIdx.x;
+  if (jw >= numbse) return;
+  const double om = ebse0 + ((ebsef - ebse0) * (double)jw) / ((double)numbse - 1.0);
+  const double g = sme + smeavg[0];
+  const zc z = mkz(om, g);
+  const int nt = k / 4 > 8 ? 8 : (k / 4 > 0 ? k / 4 : 1);
+  double S[9][2];
+  for (int q = 0; q < nq; ++q) {
+    const double* a = acoef + (size_t)q * kmax;
+    const double* b = bcoef + (size_t)q * kmax;   // b[j] = b_{j+1}
+    double am = 0.0, bm = 0.0;
+    for (int i = k - nt; i < k; ++i) {
+      am += a[i];
+      bm += b[i - 1 >= 0 ? i - 1 : 0];
+    }
+    am /= nt;
+    bm /= nt;
+    // terminator t(z) = ((z-a) - sqrt((z-a)^2 - 4b^2)) / (2 b^2), the root with Im t <= 0
+    zc t = mkz(0, 0);
+    if (bm > 0.0) {
+      const zc za = mkz(z.re - am, z.im);
+      const zc disc = za * za - mkz(4.0 * bm * bm, 0.0);
+      zc sq = zsqrt(disc);
+      zc t1 = (za - sq) * (1.0 / (2.0 * bm * bm));
+      zc t2 = (za + sq) * (1.0 / (2.0 * bm * bm));
+      t = (t1.im <= 0.0) ? t1 : t2;
+      if (t1.im <= 0.0 && t2.im <= 0.0) t = (norm2(t1) < norm2(t2)) ? t1 : t2;
+    }
+    // G = 1/(z - a_0 - b_1^2/(z - a_1 - ... - b_{k-1}^2 /(z - a_{k-1} - b_k^2 t)))
+    zc G = t;
+    for (int i = k - 1; i >= 0; --i) {
+      const double bi = b[i];     // b_{i+1}
+      const zc den = mkz(z.re - a[i], z.im) - (bi * bi) * G;
+      G = zdiv(mkz(1, 0), den);
+    }
+    // S = sum f/(E - z) = -|h|^2 G;  eps1 part Re S, eps2 part Im S
+    S[q][0] = -hn2[q] * G.re;
+    S[q][1] = -hn2[q] * G.im;
+  }
+  for (int c = 0; c < 6; ++c) {
+    double sr = 0.0, si = 0.0;
+    if (c < 3) {
+      sr = S[c][0];
+      si = S[c][1];
+    } else if (dtfull) {
+      const int qp = 3 + 2 * (c - 3);   // chains of h_a + h_b, h_a - h_b
+      sr = 0.25 * (S[qp][0] - S[qp + 1][0]);
+      si = 0.25 * (S[qp][1] - S[qp + 1][1]);
+    }
+    eps[jw + (size_t)numbse * (2 * c)] = (c < 3 ? 1.0 : 0.0) + caux * sr;
+    eps[jw + (size_t)numbse * (2 * c + 1)] = caux * si;
+  }
+}
+
+// ---------------------------------------------------------------------------
+int sym_fill_f32(hipStream_t s, int N, zf* H, int ld) {
+  const int nt = (N + 63) / 64;
+  hipLaunchKernelGGL(k_sym_fill<zf>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, N, H, ld);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+int sym_fill_f64(hipStream_t s, int N, zc* H, int ld) {
+  const int nt = (N + 63) / 64;
+  hipLaunchKernelGGL(k_sym_fill<zc>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, N, H, ld);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+int row_stats_f32(hipStream_t s, int N, const zf* H, int ld, double* D, float* st) {
+  hipLaunchKernelGGL(k_row_stats<zf>, dim3(N), dim3(256), 0, s, N, H, ld, D, st);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+int launch_residual(hipStream_t s, int N, int nc, const zc* U, const zc* HU, int ld, const double* theta,
+                    zc* R, double* rn) {
+  if (nc <= 0) return WTB_OK;
+  hipLaunchKernelGGL(k_residual, dim3(nc), dim3(256), 0, s, N, U, HU, ld, theta, R, rn);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+int launch_precond_diag(hipStream_t s, int N, int nq, const int* idx, const zc* R, int ld,
+                        const double* theta, const double* D, double dmin, zc* T, int ldt) {
+  if (nq <= 0) return WTB_OK;
+  hipLaunchKernelGGL(k_precond_diag, dim3((N + 255) / 256, nq), dim3(256), 0, s, N, idx, R, ld, theta, D,
+                     dmin, T, ldt);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, double tol) {
+  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(256), 0, s, b, G, ld, Linv, keep, tol);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+size_t haydock_work_bytes(int N, int steps) {
+  const size_t nblk = 128;
+  return sizeof(zc) * (size_t)N * 9 * 5 + sizeof(double) * (9 * nblk * 2 + 2 * 9 * (size_t)(steps + 1) + 64) +
+         sizeof(int) * 64 + 16 * 256;
+}
+
+int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
+                const double* d_E1, double caux, double ebse0, double ebsef, int numbse, double sme,
+                int renorm, void* work, double* d_eps, double* d_smeavg, double* h_coef) {
+  const int nq = dtfull ? 9 : 3;
+  if (steps < 2 || N < 2) {
+    set_error("haydock: needs N >= 2 and at least 2 steps");
+    return WTB_ERR_ARG;
+  }
+  const int nblk = 128;
+  char* w = static_cast<char*>(work);
+  auto take = [&](size_t bytes) {
+    char* q = w;
+    w += (bytes + 255) & ~size_t(255);
+    return q;
+  };
+  zc* Vb[3];
+  for (int i = 0; i < 3; ++i) Vb[i] = (zc*)take(sizeof(zc) * (size_t)N * 9);
+  zc* HV = (zc*)take(sizeof(zc) * (size_t)N * 9);
+  zc* W = (zc*)take(sizeof(zc) * (size_t)N * 9);
+  double* part = (double*)take(sizeof(double) * 9 * nblk * 2);
+  double* acoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
+  double* bcoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
+  double* hn2 = (double*)take(sizeof(double) * 16);
+  double* bcur = (double*)take(sizeof(double) * 16);
+  double* sg = (double*)take(sizeof(double) * 16);
+  int* ca = (int*)take(sizeof(int) * 16);
+  int* cb = (int*)take(sizeof(int) * 16);
+  // chains: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z
+  const int hca[9] = {0, 1, 2, 0, 0, 0, 0, 1, 1};
+  const int hcb[9] = {-1, -1, -1, 1, 1, 2, 2, 2, 2};
+  const double hsg[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+  WTB_HIP(hipMemcpyAsync(ca, hca, sizeof(hca), hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(cb, hcb, sizeof(hcb), hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(sg, hsg, sizeof(hsg), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_lz_start, dim3((N + 255) / 256, nq), dim3(256), 0, s, N, nq, dip, ca, cb, sg, Vb[1]);
+  hipLaunchKernelGGL(k_lz_norms, dim3(nq), dim3(256), 0, s, N, Vb[1], hn2, Vb[0], bcur);
+  WTB_LAUNCH_CHECK();
+  const int kmax = steps + 1;
+  int ip = 0, ic = 1, in = 2;
+  for (int j = 0; j < steps; ++j) {
+    int rc = hmm_f32(s, N, nq, H, ldh, Vb[ic], N, HV, N);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_lz_a, dim3(nblk, nq), dim3(256), 0, s, N, Vb[ic], Vb[ip], HV, bcur, W, part);
+    hipLaunchKernelGGL(k_lz_b, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, Vb[ic], W, part, nblk, acoef,
+                       bcoef, bcur, Vb[in]);
+    const int t = ip;
+    ip = ic;
+    ic = in;
+    in = t;
+  }
+  hipLaunchKernelGGL(k_lz_gamma, dim3(1), dim3(1), 0, s, steps, acoef, bcoef, d_E1, N, d_smeavg);
+  hipLaunchKernelGGL(k_haydock_eps, dim3((numbse + 255) / 256), dim3(256), 0, s, nq, steps, kmax, acoef, bcoef,
+                     hn2, caux, ebse0, ebsef, numbse, sme, d_smeavg, dtfull, d_eps);
+  WTB_LAUNCH_CHECK();
+  if (renorm) {
+    int rc = launch_imagrenorm(s, d_E1, ebse0, ebsef, numbse, d_eps);
+    if (rc) return rc;
+  }
+  if (h_coef) {
+    WTB_HIP(hipMemcpyAsync(h_coef, acoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
+    WTB_HIP(hipMemcpyAsync(h_coef + 9 * kmax, bcoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
+  }
+  return WTB_OK;
+}
+
+}  // namespace wtb

@@ -12,6 +12,12 @@
 // vectors x.  The parent applies U^T to [first1; 0], [0; last2], [x1; x2],
 // which costs O(n^2) per merge, so the whole solve is O(N^2).
 //
+// With vectors requested, each node also keeps its block of the eigenvector
+// matrix Z (N x N, column-major, the node's diagonal block): the parent gathers
+// its children's columns in sorted order, applies the deflation rotations row by
+// row, and forms diag(Z1, Z2) U for the non-deflated columns with an FP64 MFMA
+// GEMM, writing every column straight to its sorted position.
+//
 // Leaves are single elements.  All nodes of one tree level are processed by the
 // same kernel launches (grid.y = node).  The secular equation is solved by
 // bisection over the bit patterns of the root's offset from the nearer pole,
@@ -51,6 +57,18 @@ struct DCArgs {
   double* nx;           // [(2+nt)][N]
   int* K;               // [nnodes]
   double* rho;          // [nnodes]
+  // eigenvector mode (Z != nullptr)
+  double* Z;            // N x N: eigenvectors of the nodes (diagonal blocks)
+  double* Z2;           // N x N: gathered, rotated columns
+  double* U;            // N x N: secular eigenvector blocks
+  int* perm;            // [N] sorted position -> source column
+  int* cmap;            // [N] slot -> sorted position (node-relative)
+  int* rank;            // [N] position -> sorted destination (node-relative)
+  int* rotp;            // [N] deflation rotations: positions, cosine, sine
+  int* rotn;
+  double* rotc;
+  double* rots;
+  int* nrot;            // [nnodes]
 };
 
 __device__ __constant__ double kEps = 1.1102230246251565e-16;   // DLAMCH('E')
@@ -106,6 +124,7 @@ __global__ void k_dc_gather(DCArgs a) {
   a.sx[dst] = (i < n1) ? a.FR[src] : 0.0;              // first row of [Z1 0; 0 Z2]
   a.sx[(size_t)N + dst] = (i < n1) ? 0.0 : a.LR[src];  // last row
   for (int q = 0; q < a.nt; ++q) a.sx[(size_t)(2 + q) * N + dst] = a.T[(size_t)q * N + src];
+  if (a.Z) a.perm[dst] = src;
 }
 
 // M2: deflation scan (DLAED2), one workgroup per node.  The poles and weights
@@ -147,7 +166,7 @@ __global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
   const double zmax = red[0][0], dmax = red[1][0];
   const double tol = 8.0 * kEps * fmax(dmax, zmax);
   a.rho[nodeid] = rho;
-  int K = 0, nd_ = 0;
+  int K = 0, nd_ = 0, nr = 0;
   auto deflate = [&](int i) {
     int s = n - 1 - nd_;
     ++nd_;
@@ -164,6 +183,7 @@ __global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
   if (rho * zmax <= tol) {
     for (int i = 0; i < n; ++i) deflate(i);
     a.K[nodeid] = 0;
+    if (a.Z) a.nrot[nodeid] = 0;
     return;
   }
   int j = 0, pj = -1;
@@ -197,6 +217,13 @@ __global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
           x[pj] = c * xp + s * xn;
           x[nj] = c * xn - s * xp;
         }
+        if (a.Z) {
+          a.rotp[lo + nr] = pj;
+          a.rotn[lo + nr] = nj;
+          a.rotc[lo + nr] = c;
+          a.rots[lo + nr] = s;
+          ++nr;
+        }
         const double dp = D[pj], dn = D[nj];
         const double t2 = dp * c * c + dn * s * s;
         D[nj] = dp * s * s + dn * c * c;
@@ -211,6 +238,7 @@ __global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
     keep(pj);
   }
   a.K[nodeid] = K;
+  if (a.Z) a.nrot[nodeid] = nr;
 }
 
 // M2b: move the tracked sources to their slots
@@ -222,6 +250,7 @@ __global__ void k_dc_compact(DCArgs a) {
   const int s = a.slot[nd.lo + i];
   const int N = a.N;
   for (int q = 0; q < 2 + a.nt; ++q) a.cx[(size_t)q * N + nd.lo + s] = a.sx[(size_t)q * N + nd.lo + i];
+  if (a.Z) a.cmap[nd.lo + s] = i;
 }
 
 // secular function f(d_o + tau) = 1 + rho sum_i w_i^2 / ((d_i - d_o) - tau), summed by a
@@ -357,10 +386,14 @@ __global__ __launch_bounds__(256) void k_dc_apply(DCArgs a) {
     nrm += __shfl_xor(nrm, off, 16);
     for (int q = 0; q < nx; ++q) acc[q] += __shfl_xor(acc[q], off, 16);
   }
+  const double inv = 1.0 / sqrt(nrm);
   if (sub == 0) {
-    double inv = 1.0 / sqrt(nrm);
     a.nl[lo + j] = dl[org[j]] + tau[j];
     for (int q = 0; q < nx; ++q) a.nx[(size_t)q * N + lo + j] = acc[q] * inv;
+  }
+  if (a.Z) {
+    double* Uj = a.U + (size_t)(lo + j) * N + lo;
+    for (int i = sub; i < K; i += 16) Uj[i] = wh[i] / dmin_lam(dl, org, tau, i, j) * inv;
   }
 }
 
@@ -377,10 +410,105 @@ __global__ void k_dc_sort(DCArgs a) {
     rank += (u < v) || (u == v && k < i);
   }
   const int N = a.N, dst = lo + rank;
+  if (a.Z) a.rank[lo + i] = rank;
   a.lam[dst] = v;
   a.FR[dst] = a.nx[lo + i];
   a.LR[dst] = a.nx[(size_t)N + lo + i];
   for (int q = 0; q < a.nt; ++q) a.T[(size_t)q * N + dst] = a.nx[(size_t)(2 + q) * N + lo + i];
+}
+
+
+// ---- eigenvector mode ------------------------------------------------------------------
+__global__ void k_dcv_init(int N, double* Z) {
+  const size_t n2 = (size_t)N * N;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+    Z[t] = (t % (N + 1) == 0) ? 1.0 : 0.0;
+}
+
+// Z2(:, lo+i) = Z(:, perm(lo+i)) over the node's rows: its children's columns in sorted order
+__global__ __launch_bounds__(256) void k_dcv_gather(DCArgs a) {
+  const DCNode nd = a.nodes[blockIdx.y];
+  const int n = nd.hi - nd.lo, i = blockIdx.x;
+  if (i >= n) return;
+  const size_t N = a.N;
+  const double* src = a.Z + (size_t)a.perm[nd.lo + i] * N;
+  double* dst = a.Z2 + (size_t)(nd.lo + i) * N;
+  for (int r = nd.lo + threadIdx.x; r < nd.hi; r += 256) dst[r] = src[r];
+}
+
+// the close-pole deflation rotations, in scan order, one thread per row
+__global__ __launch_bounds__(256) void k_dcv_rotate(DCArgs a) {
+  const int node = blockIdx.y;
+  const DCNode nd = a.nodes[node];
+  const int r = nd.lo + blockIdx.x * 256 + threadIdx.x;
+  if (r >= nd.hi) return;
+  const int nr = a.nrot[node];
+  const size_t N = a.N;
+  for (int k = 0; k < nr; ++k) {
+    const int q = nd.lo + k;
+    double* xp = a.Z2 + (size_t)(nd.lo + a.rotp[q]) * N + r;
+    double* xn = a.Z2 + (size_t)(nd.lo + a.rotn[q]) * N + r;
+    const double c = a.rotc[q], sn = a.rots[q];
+    const double p = *xp, v = *xn;
+    *xp = c * p + sn * v;
+    *xn = c * v - sn * p;
+  }
+}
+
+// Z(:, lo + rank(j)) = Z2(:, lo + cmap(0..K-1)) U(0..K-1, j) for the non-deflated
+// slots j < K; 64 x 64 output tiles, four waves of 32 x 32 on v_mfma_f64_16x16x4
+__global__ __launch_bounds__(256) void k_dcv_gemm(DCArgs a) {
+  const int node = blockIdx.z;
+  const DCNode nd = a.nodes[node];
+  const int n = nd.hi - nd.lo, K = a.K[node], lo = nd.lo;
+  const int r0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  if (r0 >= n || j0 >= K) return;
+  const size_t N = a.N;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int rb = r0 + (wave >> 1) * 32, cb = j0 + (wave & 1) * 32;
+  d4 acc[2][2];
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) acc[x][y] = d4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + lk;
+    double av[2] = {0.0, 0.0}, bv[2] = {0.0, 0.0};
+    if (k < K) {
+      const double* zc_ = a.Z2 + (size_t)(lo + a.cmap[lo + k]) * N + lo;
+      for (int x = 0; x < 2; ++x) {
+        const int r = rb + x * 16 + lr;
+        if (r < n) av[x] = zc_[r];
+      }
+      for (int y = 0; y < 2; ++y) {
+        const int c = cb + y * 16 + lr;
+        if (c < K) bv[y] = a.U[(size_t)(lo + c) * N + lo + k];
+      }
+    }
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y) acc[x][y] = mfma16x16x4(av[x], bv[y], acc[x][y]);
+  }
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) {
+      const int c = cb + y * 16 + lr;
+      if (c >= K) continue;
+      double* dst = a.Z + (size_t)(lo + a.rank[lo + c]) * N + lo;
+      for (int q = 0; q < 4; ++q) {
+        const int r = rb + x * 16 + lk + 4 * q;     // f64 MFMA: register q holds row lk + 4q
+        if (r < n) dst[r] = acc[x][y][q];
+      }
+    }
+}
+
+// deflated slots j >= K: the gathered, rotated column moves to its sorted position
+__global__ __launch_bounds__(256) void k_dcv_copy(DCArgs a) {
+  const int node = blockIdx.y;
+  const DCNode nd = a.nodes[node];
+  const int n = nd.hi - nd.lo, j = blockIdx.x;
+  if (j >= n || j < a.K[node]) return;
+  const size_t N = a.N;
+  const double* src = a.Z2 + (size_t)(nd.lo + a.cmap[nd.lo + j]) * N;
+  double* dst = a.Z + (size_t)(nd.lo + a.rank[nd.lo + j]) * N;
+  for (int r = nd.lo + threadIdx.x; r < nd.hi; r += 256) dst[r] = src[r];
 }
 
 // ---------------------------------------------------------------------------
@@ -406,9 +534,15 @@ static void build_levels(int N, std::vector<std::vector<DCNode>>& levels) {
   for (auto& m : merges) levels[maxd - m.first].push_back(m.second);   // deepest first
 }
 
-size_t dc_work_bytes(int N, int ntrack) {
+size_t dc_work_bytes(int N, int ntrack, int vectors) {
   size_t nx = 2 + ntrack;
   size_t b = 0;
+  if (vectors) {
+    b += sizeof(double) * (size_t)N * N * 2;   // Z2 U
+    b += sizeof(int) * (size_t)N * 5;          // perm cmap rank rotp rotn
+    b += sizeof(double) * (size_t)N * 2;       // rotc rots
+    b += sizeof(int) * (N / 2 + 2) + 8 * 256;  // nrot
+  }
   b += sizeof(double) * N * 3;             // lam FR LR
   b += sizeof(double) * N * 2;             // sD sz
   b += sizeof(double) * N * nx * 3;        // sx cx nx
@@ -424,7 +558,7 @@ struct DCLayout {
   DCNode* dnodes;
 };
 
-static DCLayout dc_layout(void* work, int N, int ntrack) {
+static DCLayout dc_layout(void* work, int N, int ntrack, int vectors) {
   char* p = static_cast<char*>(work);
   auto take = [&](size_t bytes) {
     char* q = p;
@@ -453,10 +587,26 @@ static DCLayout dc_layout(void* work, int N, int ntrack) {
   a.K = (int*)take(sizeof(int) * (N / 2 + 2));
   a.rho = (double*)take(sizeof(double) * (N / 2 + 2));
   L.dnodes = (DCNode*)take(sizeof(DCNode) * (N + 2));
+  a.Z = nullptr;
+  a.Z2 = a.U = nullptr;
+  a.perm = a.cmap = a.rank = a.rotp = a.rotn = a.nrot = nullptr;
+  a.rotc = a.rots = nullptr;
+  if (vectors) {
+    a.Z2 = (double*)take(sizeof(double) * (size_t)N * N);
+    a.U = (double*)take(sizeof(double) * (size_t)N * N);
+    a.perm = (int*)take(sizeof(int) * N);
+    a.cmap = (int*)take(sizeof(int) * N);
+    a.rank = (int*)take(sizeof(int) * N);
+    a.rotp = (int*)take(sizeof(int) * N);
+    a.rotn = (int*)take(sizeof(int) * N);
+    a.rotc = (double*)take(sizeof(double) * N);
+    a.rots = (double*)take(sizeof(double) * N);
+    a.nrot = (int*)take(sizeof(int) * (N / 2 + 2));
+  }
   return L;
 }
 
-int dc_prepare(void* work, int N, int ntrack, DCTree& tree) {
+int dc_prepare(void* work, int N, int ntrack, DCTree& tree, int vectors) {
   std::vector<std::vector<DCNode>> levels;
   build_levels(N, levels);
   std::vector<DCNode> flat;
@@ -474,7 +624,8 @@ int dc_prepare(void* work, int N, int ntrack, DCTree& tree) {
   }
   tree.N = N;
   tree.ntrack = ntrack;
-  DCLayout lay = dc_layout(work, N, ntrack);
+  tree.vectors = vectors;
+  DCLayout lay = dc_layout(work, N, ntrack, vectors);
   // the deflation scan stages up to 2 x 8192 doubles in LDS
   WTB_HIP(hipFuncSetAttribute((const void*)k_dc_deflate, hipFuncAttributeMaxDynamicSharedMemorySize,
                               2 * 8192 * (int)sizeof(double)));
@@ -485,14 +636,20 @@ int dc_prepare(void* work, int N, int ntrack, DCTree& tree) {
 }
 
 int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, const double* d_e,
-           double* d_track, double* d_lambda) {
+           double* d_track, double* d_lambda, double* d_Z) {
   const int N = tree.N;
   if (tree.ntrack > 8) return WTB_ERR_LIMIT;
-  DCLayout lay = dc_layout(work, N, tree.ntrack);
+  if (d_Z && !tree.vectors) {
+    set_error("divide and conquer: eigenvectors requested from a plan prepared without them");
+    return WTB_ERR_ARG;
+  }
+  DCLayout lay = dc_layout(work, N, tree.ntrack, tree.vectors);
   DCArgs a = lay.a;
   a.e = d_e;
   a.lam = d_lambda;
   a.T = d_track;
+  a.Z = d_Z;
+  if (d_Z) hipLaunchKernelGGL(k_dcv_init, dim3(1024), dim3(256), 0, s, N, d_Z);
   hipLaunchKernelGGL(k_dc_leaves, dim3((N + 255) / 256), dim3(256), 0, s, N, d_d, d_e, d_lambda, a.FR,
                      a.LR);
   WTB_LAUNCH_CHECK();
@@ -511,6 +668,13 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
     hipLaunchKernelGGL(k_dc_lowner, g16, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_apply, g16, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_sort, g1, dim3(256), 0, s, a);
+    if (d_Z) {
+      hipLaunchKernelGGL(k_dcv_gather, dim3(maxn, cnt), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_dcv_rotate, dim3((maxn + 255) / 256, cnt), dim3(256), 0, s, a);
+      const int nt64 = (maxn + 63) / 64;
+      hipLaunchKernelGGL(k_dcv_gemm, dim3(nt64, nt64, cnt), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_dcv_copy, dim3(maxn, cnt), dim3(256), 0, s, a);
+    }
     WTB_LAUNCH_CHECK();
   }
   return WTB_OK;

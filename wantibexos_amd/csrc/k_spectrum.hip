@@ -218,6 +218,12 @@ int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act
   return WTB_OK;
 }
 
+int launch_imagrenorm(hipStream_t s, const double* d_e, double ebse0, double ebsef, int numbse, double* d_eps) {
+  hipLaunchKernelGGL(k_imagrenorm, dim3(1), dim3(256), 0, s, d_e, ebse0, ebsef, numbse, d_eps);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
 int launch_xth(hipStream_t s, int n, const zc* X, const zc* h1, const zc* h2, double* act) {
   hipLaunchKernelGGL(k_xth, dim3((n + 3) / 4), dim3(256), 0, s, n, X, h1, h2, act);
   WTB_LAUNCH_CHECK();

@@ -510,6 +510,14 @@ __global__ __launch_bounds__(256) void k_tri_her2k(TriArgs<T> a, int s0, int kp)
   }
 }
 
+// reflectors of the finished panel into the lower triangle: H(r, j) = v_j(r), r > j+1
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_storev(TriArgs<T> a, int p) {
+  const int c = blockIdx.y, j = p + c;
+  const int r = j + 2 + blockIdx.x * 256 + threadIdx.x;
+  if (r < a.N) stz(a.H[(size_t)r + (size_t)a.ld * j], a.Vp[(size_t)c * a.N + r]);
+}
+
 template <class T>
 __global__ void k_tri_last(TriArgs<T> a) {
   zc x = ldz(a.H[(size_t)(a.N - 1) + (size_t)a.ld * (a.N - 1)]);
@@ -597,6 +605,7 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
   // the arrival counters start from zero (a kernel, not a memset, so that the
   // sequence can be captured into a graph); every use returns them to zero
   hipLaunchKernelGGL(k_tri_reset, dim3(1), dim3(64), 0, s, a.cnt);
+  tw.kev_used = 0;
   if (N == 1) {
     hipLaunchKernelGGL(k_tri_last<T>, dim3(1), dim3(1), 0, s, a);
     WTB_LAUNCH_CHECK();
@@ -611,7 +620,10 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
         hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nA), dim3(256), 0, s, a, j, jj, 1);
       }
       const int nt = a.nblk - (j + 1) / TB;
+      const bool tev = tw.kev && tw.kev_used < tw.kev_cap;
+      if (tev) hipEventRecord(tw.kev[2 * tw.kev_used], s);
       hipLaunchKernelGGL(k_tri_hemv<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj);
+      if (tev) hipEventRecord(tw.kev[2 * tw.kev_used++ + 1], s);
     }
     // finish the panel's last reflector (no further column in this panel)
     const int jl = p + kp - 1;
@@ -619,6 +631,8 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
     const int nF = (N - jn + RPW - 1) / RPW;
     hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nF), dim3(256), 0, s, a, jn, kp, 0);
     hipLaunchKernelGGL(k_tri_wlast<T>, dim3(a.nblk - (jl + 1) / TB), dim3(64), 0, s, a, jl, kp - 1);
+    if (tw.store_v && N - p - 2 > 0)
+      hipLaunchKernelGGL(k_tri_storev<T>, dim3((N - p - 2 + 255) / 256, kp), dim3(256), 0, s, a, p);
     const int s0 = p + kp;
     if (s0 < N) {
       const int nt2 = a.nblk - s0 / TB;

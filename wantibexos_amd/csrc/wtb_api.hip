@@ -112,6 +112,10 @@ struct wtb_bse_plan {
   double* smeavg = nullptr;
   int* status = nullptr;
   bool timing = false;
+  bool kernel_timing = false;              // per-launch events on the GEMV kernel
+  std::vector<hipEvent_t> kev;             // 2 per tridiagonal column
+  double kernel_ms = 0.0;
+  int kernel_launches = 0;
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -333,6 +337,8 @@ static void plan_free(wtb_bse_plan* p) {
   for (void* b : bufs) hipFree(b);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
+  for (auto& e : p->kev)
+    if (e) hipEventDestroy(e);
   if (p->gexec) hipGraphExecDestroy(p->gexec);
   if (p->graph) hipGraphDestroy(p->graph);
 }
@@ -435,8 +441,20 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
 int wtb_bse_plan_dim(const wtb_bse_plan* p) { return p ? p->N : 0; }
 
 int wtb_bse_plan_set_timing(wtb_bse_plan* p, int on) {
-  if (!p) return WTB_ERR_ARG;
+  if (!p || on < 0 || on > 2) return WTB_ERR_ARG;
   p->timing = on != 0;
+  p->kernel_timing = on == 2;
+  if (p->kernel_timing && p->kev.empty()) {
+    p->kev.assign(2 * (size_t)p->N, nullptr);
+    for (auto& e : p->kev) WTB_HIP(hipEventCreate(&e));
+  }
+  return WTB_OK;
+}
+
+int wtb_bse_plan_kernel_ms(const wtb_bse_plan* p, double* total_ms, int* launches) {
+  if (!p || !total_ms || !launches) return WTB_ERR_ARG;
+  *total_ms = p->kernel_ms;
+  *launches = p->kernel_launches;
   return WTB_OK;
 }
 
@@ -475,6 +493,8 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   if (build_only) return WTB_OK;
   // (d) Householder tridiagonalisation, g <- Q^T h
   WTB_HIP(hipMemcpyAsync(p->g, p->dip, sizeof(zc) * 3 * N, hipMemcpyDeviceToDevice, s));
+  p->tw.kev = p->kernel_timing ? p->kev.data() : nullptr;
+  p->tw.kev_cap = p->kernel_timing ? p->N : 0;
   if (p->mixed) {
     if ((rc = run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, p->g))) return rc;
   } else if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, p->g))) {
@@ -595,6 +615,14 @@ int wtb_bse_plan_results(wtb_bse_plan* p, double* w, double* act, double* eps, d
       if (hipEventElapsedTime(&t, p->ev[idx[i]], p->ev[idx[i + 1]]) == hipSuccess) p->stage_ms[i] = t;
     }
   }
+  if (p->kernel_timing) {
+    p->kernel_ms = 0.0;
+    p->kernel_launches = p->tw.kev_used;
+    for (int c = 0; c < p->tw.kev_used; ++c) {
+      float t;
+      if (hipEventElapsedTime(&t, p->kev[2 * c], p->kev[2 * c + 1]) == hipSuccess) p->kernel_ms += t;
+    }
+  }
   int rc = status_to_code(st);
   if (rc) set_error(status_text(rc));
   return rc;
@@ -671,32 +699,35 @@ int wtb_zheev(wtb_ctx* c, int want_vectors, int n, double* a, int lda, double* w
     set_error("wtb_zheev: bad argument");
     return WTB_ERR_ARG;
   }
-  if (want_vectors) {
-    set_error("wtb_zheev: eigenvectors are not produced by this build (use want_vectors = 0)");
-    return WTB_ERR_ARG;
-  }
   WTB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  zc *da = nullptr, *dh = nullptr;
-  void *tri = nullptr, *dcw = nullptr;
-  double* dl = nullptr;
+  const int vec = want_vectors ? 1 : 0;
+  zc *da = nullptr, *dh = nullptr, *dx = nullptr;
+  void *tri = nullptr, *dcw = nullptr, *btw = nullptr;
+  double *dl = nullptr, *dz = nullptr;
   int rc;
   if ((rc = dmalloc(&da, (size_t)lda * n)) || (rc = dmalloc(&dh, (size_t)n * n)) ||
       (rc = dmalloc((char**)&tri, tridiag_work_bytes(n, 32, 0))) ||
-      (rc = dmalloc((char**)&dcw, dc_work_bytes(n, 0))) || (rc = dmalloc(&dl, n))) {
+      (rc = dmalloc((char**)&dcw, dc_work_bytes(n, 0, vec))) || (rc = dmalloc(&dl, n)) ||
+      (vec && ((rc = dmalloc(&dz, (size_t)n * n)) || (rc = dmalloc(&dx, (size_t)n * n)) ||
+               (rc = dmalloc((char**)&btw, backtr_work_bytes(n, n)))))) {
   } else {
     TridiagWork tw;
     DCTree tree;
     tridiag_bind(tw, tri, n, 32, 0);
-    rc = dc_prepare(dcw, n, 0, tree);
+    tw.store_v = vec;
+    rc = dc_prepare(dcw, n, 0, tree, vec);
     if (!rc) {
       hipMemcpyAsync(da, a, sizeof(zc) * lda * n, hipMemcpyHostToDevice, s);
       hipLaunchKernelGGL(k_upper_to_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, da, lda, dh, n);
       rc = run_tridiag(s, tw, dh, n, nullptr);
     }
-    if (!rc) rc = run_dc(s, dcw, tree, tw.d, tw.e, nullptr, dl);
+    if (!rc) rc = run_dc(s, dcw, tree, tw.d, tw.e, nullptr, dl, dz);
+    if (!rc && vec) rc = run_backtr(s, dh, n, n, tw.tau, dz, 0, n, dx, n, btw);
     if (!rc) {
       hipMemcpyAsync(w, dl, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+      if (vec)
+        hipMemcpy2DAsync(a, sizeof(zc) * lda, dx, sizeof(zc) * n, sizeof(zc) * n, n, hipMemcpyDeviceToHost, s);
       if (hipStreamSynchronize(s) != hipSuccess) {
         set_error("wtb_zheev: stream error");
         rc = WTB_ERR_HIP;
@@ -705,9 +736,12 @@ int wtb_zheev(wtb_ctx* c, int want_vectors, int n, double* a, int lda, double* w
   }
   hipFree(da);
   hipFree(dh);
+  hipFree(dx);
   hipFree(tri);
   hipFree(dcw);
+  hipFree(btw);
   hipFree(dl);
+  hipFree(dz);
   return rc;
 }
 
@@ -778,6 +812,308 @@ int wtb_bse_spectrum(wtb_ctx* c, int n, const double* e, const double* act6, con
   hipFree(da);
   hipFree(dw);
   hipFree(deps);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// learned preconditioner object
+// ---------------------------------------------------------------------------
+struct wtb_mlp {
+  wtb_ctx* ctx = nullptr;
+  int hidden = 0, np = 0;
+  float* P = nullptr;     // parameters
+  float* m = nullptr;     // Adam moments
+  float* v = nullptr;
+  int step = 0;
+};
+
+int wtb_mlp_create(wtb_ctx* c, int hidden, wtb_mlp** out) {
+  if (!c || !out || mlp_nparams(hidden) <= 0) {
+    set_error("wtb_mlp_create: hidden width must be 128");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  auto* q = new wtb_mlp();
+  q->ctx = c;
+  q->hidden = hidden;
+  q->np = mlp_nparams(hidden);
+  int rc;
+  if ((rc = dmalloc(&q->P, q->np)) || (rc = dmalloc(&q->m, q->np)) || (rc = dmalloc(&q->v, q->np))) {
+    wtb_mlp_destroy(q);
+    return rc;
+  }
+  hipMemsetAsync(q->P, 0, sizeof(float) * q->np, c->stream);
+  hipMemsetAsync(q->m, 0, sizeof(float) * q->np, c->stream);
+  hipMemsetAsync(q->v, 0, sizeof(float) * q->np, c->stream);
+  WTB_HIP(hipStreamSynchronize(c->stream));
+  *out = q;
+  return WTB_OK;
+}
+
+int wtb_mlp_nparams(const wtb_mlp* q) { return q ? q->np : 0; }
+
+int wtb_mlp_set_params(wtb_mlp* q, const float* p, int n) {
+  if (!q || !p || n != q->np) {
+    set_error("wtb_mlp_set_params: bad argument");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipMemcpyAsync(q->P, p, sizeof(float) * n, hipMemcpyHostToDevice, q->ctx->stream));
+  hipMemsetAsync(q->m, 0, sizeof(float) * q->np, q->ctx->stream);
+  hipMemsetAsync(q->v, 0, sizeof(float) * q->np, q->ctx->stream);
+  q->step = 0;
+  WTB_HIP(hipStreamSynchronize(q->ctx->stream));
+  return WTB_OK;
+}
+
+int wtb_mlp_get_params(const wtb_mlp* q, float* p, int n) {
+  if (!q || !p || n != q->np) {
+    set_error("wtb_mlp_get_params: bad argument");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipMemcpyAsync(p, q->P, sizeof(float) * n, hipMemcpyDeviceToHost, q->ctx->stream));
+  WTB_HIP(hipStreamSynchronize(q->ctx->stream));
+  return WTB_OK;
+}
+
+int wtb_mlp_forward(wtb_mlp* q, int S, const float* X, float* y) {
+  if (!q || !X || !y || S < 1) {
+    set_error("wtb_mlp_forward: bad argument");
+    return WTB_ERR_ARG;
+  }
+  hipStream_t s = q->ctx->stream;
+  float *dX = nullptr, *dy = nullptr;
+  int rc;
+  if (!(rc = dmalloc(&dX, (size_t)5 * S)) && !(rc = dmalloc(&dy, (size_t)S))) {
+    hipMemcpyAsync(dX, X, sizeof(float) * 5 * S, hipMemcpyHostToDevice, s);
+    rc = mlp_forward(s, S, dX, q->P, dy);
+    if (!rc) {
+      hipMemcpyAsync(y, dy, sizeof(float) * S, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+    }
+  }
+  hipFree(dX);
+  hipFree(dy);
+  return rc;
+}
+
+static int mlp_upload(hipStream_t s, int S, const float* X, const float* t, float** dX, float** dt, int** dperm) {
+  int rc;
+  if ((rc = dmalloc(dX, (size_t)5 * S)) || (rc = dmalloc(dt, (size_t)S)) || (rc = dmalloc(dperm, (size_t)S)))
+    return rc;
+  WTB_HIP(hipMemcpyAsync(*dX, X, sizeof(float) * 5 * S, hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(*dt, t, sizeof(float) * S, hipMemcpyHostToDevice, s));
+  return WTB_OK;
+}
+
+int wtb_mlp_loss_grad(wtb_mlp* q, int S, const float* X, const float* t, float* loss, float* grad) {
+  if (!q || !X || !t || !loss || !grad || S < 1) {
+    set_error("wtb_mlp_loss_grad: bad argument");
+    return WTB_ERR_ARG;
+  }
+  hipStream_t s = q->ctx->stream;
+  float *dX = nullptr, *dt = nullptr, *work = nullptr, *dg = nullptr, *dl = nullptr;
+  int* dperm = nullptr;
+  int rc = mlp_upload(s, S, X, t, &dX, &dt, &dperm);
+  if (!rc && !(rc = dmalloc(&work, mlp_grad_work_floats(S))) && !(rc = dmalloc(&dg, (size_t)q->np)) &&
+      !(rc = dmalloc(&dl, 4))) {
+    std::vector<int> perm(S);
+    for (int i = 0; i < S; ++i) perm[i] = i;
+    hipMemcpy(dperm, perm.data(), sizeof(int) * S, hipMemcpyHostToDevice);
+    rc = mlp_loss_grad(s, S, dX, dt, dperm, 0, S, q->P, work, dg, dl);
+    if (!rc) {
+      hipMemcpyAsync(grad, dg, sizeof(float) * q->np, hipMemcpyDeviceToHost, s);
+      hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+    }
+  }
+  for (void* b : {(void*)dX, (void*)dt, (void*)dperm, (void*)work, (void*)dg, (void*)dl}) hipFree(b);
+  return rc;
+}
+
+int wtb_mlp_train(wtb_mlp* q, int S, const float* X, const float* t, int epochs, int batch, float lr,
+                  unsigned seed, float* loss_hist) {
+  if (!q || !X || !t || S < 1 || epochs < 1 || batch < 1) {
+    set_error("wtb_mlp_train: bad argument");
+    return WTB_ERR_ARG;
+  }
+  hipStream_t s = q->ctx->stream;
+  float *dX = nullptr, *dt = nullptr, *work = nullptr, *dg = nullptr, *dl = nullptr;
+  int* dperm = nullptr;
+  const int B = std::min(batch, S);
+  int rc = mlp_upload(s, S, X, t, &dX, &dt, &dperm);
+  const int nb = S / B;   // whole batches per epoch
+  if (!rc && !(rc = dmalloc(&work, mlp_grad_work_floats(B))) && !(rc = dmalloc(&dg, (size_t)q->np)) &&
+      !(rc = dmalloc(&dl, (size_t)nb + 1))) {
+    std::vector<int> perm(S);
+    std::vector<float> ls(nb);
+    unsigned long long st = 0x9E3779B97F4A7C15ull ^ seed;
+    for (int ep = 0; ep < epochs && !rc; ++ep) {
+      for (int i = 0; i < S; ++i) perm[i] = i;
+      for (int i = S - 1; i > 0; --i) {   // Fisher-Yates with a 64-bit LCG
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        const int j = (int)((st >> 33) % (unsigned long long)(i + 1));
+        std::swap(perm[i], perm[j]);
+      }
+      if (hipMemcpy(dperm, perm.data(), sizeof(int) * S, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = WTB_ERR_HIP;
+        break;
+      }
+      for (int b = 0; b < nb && !rc; ++b) {
+        rc = mlp_loss_grad(s, S, dX, dt, dperm, b * B, B, q->P, work, dg, dl + b);
+        if (!rc) rc = mlp_adam(s, q->P, dg, q->m, q->v, lr, ++q->step);
+      }
+      if (!rc) {
+        hipMemcpyAsync(ls.data(), dl, sizeof(float) * nb, hipMemcpyDeviceToHost, s);
+        if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+        double a = 0.0;
+        for (float v : ls) a += v;
+        if (loss_hist) loss_hist[ep] = (float)(a / nb);
+      }
+    }
+  }
+  for (void* b : {(void*)dX, (void*)dt, (void*)dperm, (void*)work, (void*)dg, (void*)dl}) hipFree(b);
+  return rc;
+}
+
+int wtb_mlp_destroy(wtb_mlp* q) {
+  if (!q) return WTB_OK;
+  hipFree(q->P);
+  hipFree(q->m);
+  hipFree(q->v);
+  delete q;
+  return WTB_OK;
+}
+
+// per-state diagonal and row statistics of the plan's BSE matrix (after build):
+// D[N], stats[3][N] = sum |H_ij|, sqrt(sum |H_ij|^2), max |H_ij| over j != i
+int wtb_bse_plan_row_stats(wtb_bse_plan* p, double* D, float* stats) {
+  if (!p || !D || !stats) {
+    set_error("wtb_bse_plan_row_stats: bad argument");
+    return WTB_ERR_ARG;
+  }
+  hipStream_t s = p->ctx->stream;
+  const int N = p->N;
+  double* dD = nullptr;
+  float* dst = nullptr;
+  int rc;
+  if (!p->mixed) {
+    set_error("wtb_bse_plan_row_stats: needs a plan in the precision form (WTB_MIXED)");
+    return WTB_ERR_ARG;
+  }
+  if (!(rc = dmalloc(&dD, N)) && !(rc = dmalloc(&dst, (size_t)3 * N))) {
+    rc = plan_enqueue(p, true);
+    if (!rc) rc = sym_fill_f32(s, N, p->Hf, (int)p->ldh);
+    if (!rc) rc = row_stats_f32(s, N, p->Hf, (int)p->ldh, dD, dst);
+    if (!rc) {
+      hipMemcpyAsync(D, dD, sizeof(double) * N, hipMemcpyDeviceToHost, s);
+      hipMemcpyAsync(stats, dst, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+    }
+  }
+  hipFree(dD);
+  hipFree(dst);
+  return rc;
+}
+
+int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wtb_mlp* mlp, double* W,
+                        double* resid, double* eps, double* X, wtb_davidson_info* info) {
+  if (!p || !dp || !W || !resid || dp->nev < 1 || dp->nev > p->N) {
+    set_error("wtb_bse_plan_lowest: bad argument");
+    return WTB_ERR_ARG;
+  }
+  if (!p->mixed) {
+    set_error("wtb_bse_plan_lowest: needs a plan in the precision form (WTB_MIXED)");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  const int N = p->N, nev = dp->nev;
+  const wtb_bse_params& P = p->prm;
+  wtb_davidson_info inf{};
+  hipEvent_t ev[4] = {};
+  for (auto& e : ev) WTB_HIP(hipEventCreate(&e));
+  double* dD = nullptr;
+  float* dst = nullptr;
+  double *dW = nullptr, *dR = nullptr, *dsm = nullptr;
+  zc* dX = nullptr;
+  void* hw = nullptr;
+  int rc;
+  const int steps = dp->lanczos_steps;
+  if (!(rc = dmalloc(&dD, N)) && !(rc = dmalloc(&dst, (size_t)3 * N)) && !(rc = dmalloc(&dW, nev)) &&
+      !(rc = dmalloc(&dR, nev)) && !(rc = dmalloc(&dsm, 4)) && !(X && (rc = dmalloc(&dX, (size_t)N * nev))) &&
+      !(eps && steps > 0 && (rc = dmalloc((char**)&hw, haydock_work_bytes(N, steps))))) {
+    hipEventRecord(ev[0], s);
+    rc = plan_enqueue(p, true);
+    if (!rc) rc = sym_fill_f32(s, N, p->Hf, (int)p->ldh);
+    if (!rc) rc = row_stats_f32(s, N, p->Hf, (int)p->ldh, dD, dst);
+    hipEventRecord(ev[1], s);
+    DavidsonCfg cfg;
+    cfg.nev = nev;
+    cfg.guard = dp->guard;
+    cfg.max_sub = dp->max_sub;
+    cfg.max_iter = dp->max_iter;
+    cfg.tol = dp->tol;
+    MlpDev md;
+    if (mlp) {
+      md.w = mlp->P;
+      md.hidden = mlp->hidden;
+    }
+    DavidsonInfo di;
+    if (!rc) rc = run_davidson(s, N, p->Hf, (int)p->ldh, dD, dst, cfg, mlp ? &md : nullptr, dW, dR, dX, di);
+    hipEventRecord(ev[2], s);
+    inf.iters = di.iters;
+    inf.matvecs = di.matvecs;
+    inf.converged = di.converged;
+    inf.iters_fp64 = di.iters_fp64;
+    inf.matvecs_fp64 = di.matvecs_fp64;
+    inf.max_resid = di.max_resid;
+    if (!rc && eps && steps > 0 && di.converged) {
+      const double gama = (double)180.90f;
+      const double caux = gama / (p->nkf32 * p->vc3d);
+      rc = run_haydock(s, N, p->Hf, (int)p->ldh, p->dip, P.dtfull, steps, dW, caux, P.ebse0, P.ebsef, P.numbse,
+                       P.cshift, P.renorm, hw, p->eps, dsm, nullptr);
+    }
+    hipEventRecord(ev[3], s);
+    if (!rc && di.converged) {
+      hipMemcpyAsync(W, dW, sizeof(double) * nev, hipMemcpyDeviceToHost, s);
+      hipMemcpyAsync(resid, dR, sizeof(double) * nev, hipMemcpyDeviceToHost, s);
+      if (X) hipMemcpyAsync(X, dX, sizeof(zc) * N * nev, hipMemcpyDeviceToHost, s);
+      if (eps && steps > 0)
+        hipMemcpyAsync(eps, p->eps, sizeof(double) * 12 * P.numbse, hipMemcpyDeviceToHost, s);
+      if (hipStreamSynchronize(s) != hipSuccess) rc = WTB_ERR_HIP;
+      float t;
+      if (hipEventElapsedTime(&t, ev[0], ev[1]) == hipSuccess) inf.ms_build = t;
+      if (hipEventElapsedTime(&t, ev[1], ev[2]) == hipSuccess) inf.ms_solve = t;
+      if (hipEventElapsedTime(&t, ev[2], ev[3]) == hipSuccess) inf.ms_eps = t;
+    } else if (!rc && dp->fallback) {
+      // not converged: the full FP64 solve decides (plain path on this plan's model)
+      inf.used_fallback = 1;
+      if (!p->H) rc = dmalloc(&p->H, p->ldh * N);
+      if (!rc) {
+        p->mixed = false;
+        rc = plan_enqueue(p);
+        p->mixed = true;
+      }
+      if (!rc) {
+        std::vector<double> wall(N);
+        rc = wtb_bse_plan_results(p, wall.data(), nullptr, eps, nullptr);
+        if (!rc) {
+          for (int i = 0; i < nev; ++i) {
+            W[i] = wall[i];
+            resid[i] = 0.0;
+          }
+          if (X) memset(X, 0, sizeof(zc) * N * nev);
+        }
+      }
+    } else if (!rc) {
+      set_error("wtb_bse_plan_lowest: Davidson did not converge (fallback off)");
+      rc = WTB_ERR_EIG;
+    }
+  }
+  for (auto& e : ev)
+    if (e) hipEventDestroy(e);
+  for (void* b : {(void*)dD, (void*)dst, (void*)dW, (void*)dR, (void*)dsm, (void*)dX, hw}) hipFree(b);
+  if (info) *info = inf;
   return rc;
 }
 

@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 // ---------------------------------------------------------------------------
 // complex double with value semantics (interleaved re,im as in Fortran COMPLEX*16)

@@ -60,6 +60,13 @@ struct TridiagWork {
   zc* puv = nullptr;      // per-workgroup u^H v partials
   void* pub = nullptr;    // scalars published by the last workgroup of a step
   unsigned* cnt = nullptr;  // arrival counters
+  // optional per-launch timing of the Hermitian GEMV kernel: when kev is set,
+  // launch c is bracketed by events kev[2c], kev[2c+1] (c < kev_cap)
+  hipEvent_t* kev = nullptr;
+  int kev_cap = 0;
+  int kev_used = 0;
+  // keep the reflectors (v_j below row j+1 in column j of H) for the back-transform
+  int store_v = 0;
 };
 size_t tridiag_work_bytes(int N, int nb, int nrhs);
 int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs);
@@ -69,14 +76,33 @@ int run_tridiag_f32(hipStream_t s, TridiagWork& tw, zf* d_H, int ldh, zc* d_g);
 
 // stage (d): tridiagonal divide and conquer (eigenvalues + Z^T x for tracked vectors)
 struct DCTree {
-  int N = 0, ntrack = 0;
+  int N = 0, ntrack = 0, vectors = 0;
   std::vector<int> level_off, level_cnt, level_maxn;
 };
-size_t dc_work_bytes(int N, int ntrack);
-int dc_prepare(void* work, int N, int ntrack, DCTree& tree);
+size_t dc_work_bytes(int N, int ntrack, int vectors = 0);
+int dc_prepare(void* work, int N, int ntrack, DCTree& tree, int vectors = 0);
 // d_track: ntrack x N, in: the tridiagonal-basis vectors, out: Z^T (.)
+// d_Z (optional, N x N column-major): the eigenvectors, column j for d_lambda[j];
+// needs a tree prepared with vectors = 1
 int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, const double* d_e,
-           double* d_track, double* d_lambda);
+           double* d_track, double* d_lambda, double* d_Z = nullptr);
+
+// stage (d3): X(:, 0:m) = Q Z(:, c0:c0+m) from the stored reflectors (store_v) and tau;
+// the last component of each column is made real and >= 0
+size_t backtr_work_bytes(int N, int m);
+int run_backtr(hipStream_t s, const zc* H, int ld, int N, const zc* tau, const double* Z, int c0, int m,
+               zc* X, int ldx, void* work);
+int run_backtr_f32(hipStream_t s, const zf* H, int ld, int N, const zc* tau, const double* Z, int c0,
+                   int m, zc* X, int ldx, void* work);
+
+// complex GEMM C = alpha op(A) op(B) + beta C, op 'N' or 'C' (FP64 MFMA)
+int zgemm(hipStream_t s, char opa, char opb, int M, int N, int K, zc alpha, const zc* A, int lda,
+          const zc* B, int ldb, zc beta, zc* C, int ldc);
+// Y = H X for a full Hermitian H: COMPLEX*8 operand on FP32 MFMA, or FP64
+int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy);
+int hmm_f64(hipStream_t s, int N, int nc, const zc* H, int ldh, const zc* X, int ldx, zc* Y, int ldy);
+// COMPLEX*8 operand, FP64 MFMA arithmetic (the final iterations of the iterative solver)
+int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy);
 
 // stage (e)
 int launch_activities(hipStream_t s, int N, const double* d_G /* 6 x N */, int dtfull, int cpol,
@@ -85,5 +111,59 @@ int launch_quantize(hipStream_t s, int n, const double* src, double* dst);
 int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act6, double vc,
                     double nkf32, double ebse0, double ebsef, int numbse, double sme, int renorm,
                     double* d_work, double* d_eps);
+
+int launch_imagrenorm(hipStream_t s, const double* d_e, double ebse0, double ebsef, int numbse, double* d_eps);
+
+// iterative form: Davidson for the lowest excitons, Haydock chains for eps(omega)
+int sym_fill_f32(hipStream_t s, int N, zf* H, int ld);
+int sym_fill_f64(hipStream_t s, int N, zc* H, int ld);
+int row_stats_f32(hipStream_t s, int N, const zf* H, int ld, double* D, float* st);
+int launch_residual(hipStream_t s, int N, int nc, const zc* U, const zc* HU, int ld, const double* theta,
+                    zc* R, double* rn);
+int launch_precond_diag(hipStream_t s, int N, int nq, const int* idx, const zc* R, int ld,
+                        const double* theta, const double* D, double dmin, zc* T, int ldt);
+int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, double tol);
+size_t haydock_work_bytes(int N, int steps);
+// eps[6][2][numbse] from Lanczos chains of the dipoles (dip: 3 x N); d_E1: lowest exciton
+// energy (device); d_smeavg[0] <- 10 (E_max - E_1) / N, [1] <- E_max; h_coef (host, optional):
+// a then b coefficients, 9 x (steps + 1) each
+int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
+                const double* d_E1, double caux, double ebse0, double ebsef, int numbse, double sme,
+                int renorm, void* work, double* d_eps, double* d_smeavg, double* h_coef);
+
+// learned preconditioner: MLP (5 -> 128 -> 128 -> 1, tanh, FP32) on per-state features
+struct MlpDev {
+  const float* w = nullptr;   // packed parameters, see mlp_layout
+  int hidden = 0;
+};
+int mlp_nparams(int hidden);
+int launch_precond_mlp(hipStream_t s, int N, int nq, const int* idx, const zc* R, int ld, const double* theta,
+                       const double* D, const float* stats, double Dmin, const MlpDev& mlp, zc* T, int ldt);
+
+int mlp_forward(hipStream_t s, int S, const float* X, const float* P, float* y);
+size_t mlp_grad_work_floats(int B);
+// loss and gradient of the mean squared error over samples perm[b0 .. b0+B) of X[5][S], t[S]
+int mlp_loss_grad(hipStream_t s, int S, const float* X, const float* t, const int* perm, int b0, int B,
+                  const float* P, float* work, float* grad, float* loss);
+int mlp_adam(hipStream_t s, float* P, const float* grad, float* m, float* v, float lr, int step);
+
+struct DavidsonCfg {
+  int nev = 256, guard = 32, max_sub = 864, max_iter = 60;
+  double tol = 1e-6;
+  double dmin = 1e-3;
+  // residual level at which the products switch from FP32 to FP64 arithmetic (the
+  // FP32 products stall near 1e-6 eV for matrices of norm ~10 eV)
+  double switch_tol = 2e-5;
+};
+struct DavidsonInfo {
+  int iters = 0, matvecs = 0, converged = 0;
+  int iters_fp64 = 0, matvecs_fp64 = 0;
+  double max_resid = 0.0;
+};
+// lowest cfg.nev eigenpairs of the full Hermitian COMPLEX*8 matrix H; d_W (nev, device),
+// d_resid (nev, device), d_X (N x nev, device, optional)
+int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, const float* stats,
+                 const DavidsonCfg& cfg, const MlpDev* mlp, double* d_W, double* d_resid, zc* d_X,
+                 DavidsonInfo& info);
 
 }  // namespace wtb

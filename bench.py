@@ -11,6 +11,7 @@ eigensolve, activities and eps(omega).
 
     python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
     python bench.py --copy-probe      # one 1 GiB device copy, to scale HBM counters
+    python bench.py --no-graph ...    # direct launches (for kernel tracers)
 
 Prints one JSON line.  The CPU baseline is the reference's own code (its
 subroutines compiled from source with flang + MKL, oracle/_ref/ref_dump.x) on
@@ -109,6 +110,10 @@ def main():
     ap.add_argument("--config", choices=["c2", "c1"], default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the kernels directly instead of replaying the captured HIP graph")
+    ap.add_argument("--pmc-light", action="store_true",
+                    help="one execute and nothing else (for hardware-counter passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (counter scale)")
     args = ap.parse_args()
@@ -132,8 +137,15 @@ def main():
     prm = params(cfg, args.precision)
     ctx = bse.Context(0)
     plan = bse.BSEPlan(ctx, model, prm)
+    if args.no_graph:
+        plan.set_graph(False)
     N = plan.N
     log(f"plan N={N} created")
+    if args.pmc_light:
+        plan.execute()
+        plan.results()
+        log("pmc-light: one execute done")
+        return
     for _ in range(args.warmup):
         plan.execute()
     res = plan.results()
@@ -187,8 +199,15 @@ def main():
     per_launch = hemv_b / max(k_n, 1)
     ach = per_launch / avg_s / 1e9
     log(f"k_tri_hemv: {k_n} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.2f} MB/launch")
+    # HBM traffic per launch: the measured traffic/algorithmic ratio of the counter passes
+    # (profiles/, same workload and precision) applied to the mean launch
+    traffic = None
+    tfile = ROOT / "profiles" / f"k_tri_hemv_traffic_hex2d_nw22_k36_N5184_{args.precision}.json"
+    if tfile.exists() and args.config == "c2":
+        tj = json.loads(tfile.read_text())
+        traffic = round(tj["traffic_over_algorithmic"] * per_launch)
     roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",
-                    frac=round(ach / HBM_PEAK, 4), traffic=None,
+                    frac=round(ach / HBM_PEAK, 4), traffic=traffic,
                     kernel="k_tri_hemv", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
                     bytes_per_launch=round(per_launch))
     cpu = None

@@ -81,11 +81,15 @@ def test_davidson_and_lanczos_against_full_fp64(ctx):
     assert info["converged"] == 1 and info["used_fallback"] == 0, info
     assert r["resid"].max() <= 1e-6
     assert np.abs(r["W"] - full["W"][:64]).max() < 1e-4     # 1 meV bar; FP32 matrix ~1e-6
+    errs = {}
     for c in range(6):
         for part in range(2):
             ref = full["eps"][c, part]
             if np.abs(ref).max() > 1e-8:
-                assert relerr(r["eps"][c, part], ref) < 1e-3, (c, part, relerr(r["eps"][c, part], ref))
+                d = np.abs(r["eps"][c, part] - ref)
+                errs[(c, part)] = (relerr(r["eps"][c, part], ref), int(d.argmax()))
+    print("eps relative errors (component, part): (err, worst omega index)", errs, "info", info)
+    assert max(e for e, _ in errs.values()) < 1e-3
 
 
 def test_learned_preconditioner_against_diagonal(ctx):

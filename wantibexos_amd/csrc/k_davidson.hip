@@ -214,36 +214,59 @@ __global__ __launch_bounds__(256) void k_lz_a(int N, const zc* __restrict__ V, c
   }
 }
 
-// part 2: a_j = sum, b_{j+1} = sqrt(|w|^2 - a_j^2); v_next = (w - a_j v) / b_{j+1}.  Every
-// workgroup sums the partials of its chain in the same order; workgroup 0 records them.
+// part 2: a_j = sum of the partials (every workgroup sums them in the same order;
+// workgroup 0 records a_j); w' = w - a_j v into Vn, partials of |w'|^2
 __global__ __launch_bounds__(256) void k_lz_b(int N, int j, int kmax, const zc* __restrict__ V,
                                               const zc* __restrict__ W, const double* __restrict__ part,
-                                              int nblk, double* __restrict__ acoef, double* __restrict__ bcoef,
-                                              double* __restrict__ bcur, zc* __restrict__ Vn) {
-  __shared__ double s_a, s_b;
+                                              int nblk, double* __restrict__ acoef, zc* __restrict__ Vn,
+                                              double* __restrict__ part2) {
+  __shared__ double s_a;
+  __shared__ double rw[256];
   const int q = blockIdx.y;
   if (threadIdx.x == 0) {
-    double a = 0.0, w2 = 0.0;
-    for (int k = 0; k < nblk; ++k) {
-      a += part[((size_t)q * nblk + k) * 2];
-      w2 += part[((size_t)q * nblk + k) * 2 + 1];
-    }
-    const double b2 = w2 - a * a;
+    double a = 0.0;
+    for (int k = 0; k < nblk; ++k) a += part[((size_t)q * nblk + k) * 2];
     s_a = a;
-    s_b = b2 > 0.0 ? sqrt(b2) : 0.0;
+    if (blockIdx.x == 0) acoef[(size_t)q * kmax + j] = a;
   }
   __syncthreads();
-  const double a = s_a, b = s_b;
-  __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    acoef[(size_t)q * kmax + j] = a;
-    bcoef[(size_t)q * kmax + j] = b;      // b_{j+1}
-    bcur[q] = b;
-  }
+  const double a = s_a;
   const size_t off = (size_t)N * q;
+  double sw = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += 256 * gridDim.x) {
+    const zc w = W[off + i] - a * V[off + i];
+    Vn[off + i] = w;
+    sw += norm2(w);
+  }
+  rw[threadIdx.x] = sw;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) rw[threadIdx.x] += rw[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part2[(size_t)q * gridDim.x + blockIdx.x] = rw[0];
+}
+
+// part 3: b_{j+1} = |w'|, v_{j+1} = w' / b_{j+1}
+__global__ __launch_bounds__(256) void k_lz_c(int N, int j, int kmax, const double* __restrict__ part2, int nblk,
+                                              double* __restrict__ bcoef, double* __restrict__ bcur,
+                                              zc* __restrict__ Vn) {
+  __shared__ double s_b;
+  const int q = blockIdx.y;
+  if (threadIdx.x == 0) {
+    double w2 = 0.0;
+    for (int k = 0; k < nblk; ++k) w2 += part2[(size_t)q * nblk + k];
+    s_b = sqrt(w2);
+    if (blockIdx.x == 0) {
+      bcoef[(size_t)q * kmax + j] = s_b;
+      bcur[q] = s_b;
+    }
+  }
+  __syncthreads();
+  const double b = s_b;
   const double inv = b > 0.0 ? 1.0 / b : 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += 256 * gridDim.x)
-    Vn[off + i] = (W[off + i] - a * V[off + i]) * inv;
+  const size_t off = (size_t)N * q;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += 256 * gridDim.x) Vn[off + i] = Vn[off + i] * inv;
 }
 
 // start vectors conj(h), conj(h_a + h_b), conj(h_a - h_b) (normalised by the host-side
@@ -424,7 +447,7 @@ int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, do
 
 size_t haydock_work_bytes(int N, int steps) {
   const size_t nblk = 128;
-  return sizeof(zc) * (size_t)N * 9 * 5 + sizeof(double) * (9 * nblk * 2 + 2 * 9 * (size_t)(steps + 1) + 64) +
+  return sizeof(zc) * (size_t)N * 9 * 5 + sizeof(double) * (9 * nblk * 3 + 2 * 9 * (size_t)(steps + 1) + 64) +
          sizeof(int) * 64 + 16 * 256;
 }
 
@@ -448,6 +471,7 @@ int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int d
   zc* HV = (zc*)take(sizeof(zc) * (size_t)N * 9);
   zc* W = (zc*)take(sizeof(zc) * (size_t)N * 9);
   double* part = (double*)take(sizeof(double) * 9 * nblk * 2);
+  double* part2 = (double*)take(sizeof(double) * 9 * nblk);
   double* acoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
   double* bcoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
   double* hn2 = (double*)take(sizeof(double) * 16);
@@ -472,7 +496,8 @@ int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int d
     if (rc) return rc;
     hipLaunchKernelGGL(k_lz_a, dim3(nblk, nq), dim3(256), 0, s, N, Vb[ic], Vb[ip], HV, bcur, W, part);
     hipLaunchKernelGGL(k_lz_b, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, Vb[ic], W, part, nblk, acoef,
-                       bcoef, bcur, Vb[in]);
+                       Vb[in], part2);
+    hipLaunchKernelGGL(k_lz_c, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, part2, nblk, bcoef, bcur, Vb[in]);
     const int t = ip;
     ip = ic;
     ic = in;

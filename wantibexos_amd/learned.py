@@ -7,7 +7,10 @@ full FP64 eigendecomposition (GPU, ZHEEV path with vectors) gives the target
 
     t_i(theta) = sigma Re[(H - theta - i eta)^-1]_ii = sigma sum_n |X_in|^2 (l_n - theta) / ((l_n - theta)^2 + eta^2)
 
-at Ritz-like values theta near the lowest excitons (eta = sigma = 0.05 eV), and
+at theta equal to one of the lowest exciton energies l_k (eta = sigma = 0.05 eV), where
+the k-th term vanishes (its Lorentzian real part is zero at the pole) so that t is the
+regularised diagonal of the reduced resolvent sum_{n != k} x_n x_n^H / (l_n - l_k), the
+operator of the exact (Jacobi-Davidson) correction; and
 the features of every basis state come from the matrix's diagonal and row
 statistics (computed on the GPU).  The feature definition is the one of the
 preconditioner kernel:
@@ -91,8 +94,11 @@ def sample_matrix(ctx, model, prm, rng, n_theta=8, n_low=64):
     N = H.shape[0]
     Hf = np.tril(H) + np.tril(H, -1).conj().T
     lam, X = bse.zheev(ctx, Hf, vectors=True)
+    # theta at an exciton energy: g(0) = 0 removes that exciton's own pole, so the target is
+    # the regularised diagonal of the reduced resolvent sum_{n != k} x_n x_n^H / (l_n - l_k),
+    # the operator of the exact (Jacobi-Davidson) correction
     ks = rng.choice(min(n_low, N), size=min(n_theta, N), replace=False)
-    theta = lam[ks] + rng.uniform(-0.02, 0.02, ks.size)
+    theta = lam[ks]
     return features(D, st, theta, D.min(), N), targets(lam, X, theta), N
 
 

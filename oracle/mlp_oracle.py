@@ -103,6 +103,13 @@ def row_stats(Hfull):
     return np.real(np.diag(Hfull)).copy(), np.stack([A.sum(1), np.sqrt((A * A).sum(1)), A.max(1)])
 
 
+def target_correction(lam, X, theta, D, eta=SIGMA):
+    """The network's training target: target() minus sigma g(D_i - theta), g(d) = d/(d^2+eta^2)."""
+    theta = np.atleast_1d(np.asarray(theta, dtype=np.float64))
+    dd = np.asarray(D, dtype=np.float64)[None, :] - theta[:, None]
+    return target(lam, X, theta, eta) - (SIGMA * dd / (dd * dd + eta * eta)).reshape(-1)
+
+
 def target(lam, X, theta, eta=SIGMA):
     """sigma Re[(H - theta - i eta)^-1]_ii from the eigenpairs (lam, X), (S,) with i fastest."""
     lam = np.asarray(lam, dtype=np.float64)

@@ -94,8 +94,8 @@ def test_davidson_and_lanczos_against_full_fp64(ctx):
 
 def test_learned_preconditioner_against_diagonal(ctx):
     from wantibexos_amd import bse, learned, tbmodel
-    data = learned.make_training_set(ctx, n_models=12, seed=5, nw_range=(6, 10), n_theta=8)
-    P, hist = learned.train(ctx, data, epochs=6, batch=2048, lr=2e-3)
+    data = learned.make_training_set(ctx, n_models=24, seed=5, nw_range=(6, 10), n_theta=8)
+    P, hist = learned.train(ctx, data, epochs=15, batch=2048, lr=2e-3)
     assert hist[-1] < hist[0]
     m = tbmodel.hex2d(seed=77, nw=10)
     prm = mixed_params(r0=55.0)

@@ -53,7 +53,12 @@ def test_target_is_the_regularised_resolvent_diagonal():
     for k, th in enumerate(theta):
         R = np.linalg.inv(A - (th + 1j * M.SIGMA) * np.eye(n))
         assert np.allclose(t[k], M.SIGMA * np.real(np.diag(R)), atol=1e-12)
-    assert np.allclose(learned.targets(lam, X, theta), t.reshape(-1), rtol=1e-6, atol=1e-7)
+    D = np.real(np.diag(A))
+    tc = M.target_correction(lam, X, theta, D).reshape(theta.size, n)
+    for k, th in enumerate(theta):
+        d = D - th
+        assert np.allclose(tc[k], t[k] - M.SIGMA * d / (d * d + M.SIGMA ** 2), atol=1e-12)
+    assert np.allclose(learned.targets(lam, X, theta, D), tc.reshape(-1), rtol=1e-5, atol=1e-6)
 
 
 def test_features_and_row_stats():

@@ -8,7 +8,8 @@
 //   loop: G = S diag(theta) S^H (tridiagonal reduction + divide and conquer +
 //         back-transform on the GPU), U = V S_k, HU = HV S_k, R = HU - U theta
 //         stop when the first nev residual norms are below tol
-//         T = P(R_unconverged)  (diagonal or learned preconditioner)
+//         T = P(R_unconverged)  (diagonal or learned preconditioner; the learned one
+//         only in the FP32 phase)
 //         T -= V (V^H T) twice, T <- T L^-H with T^H T = L L^H (columns of tiny
 //         pivots dropped), V += T, HV += H T, G grows by V^H (H T)
 //         restart with the k = nev + guard Ritz vectors when m would pass max_sub
@@ -183,6 +184,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   info.matvecs += m;
   DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
   std::vector<double> h_rn(nk);
+  double prev_mx = 1e300;
   std::vector<int> h_keep(nk), h_idx(nk), h_map(nk);   // live across the asynchronous copies
   for (int it = 0; it < cfg.max_iter; ++it) {
     info.iters = it + 1;
@@ -201,7 +203,11 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
       break;
     }
     if (it + 1 == cfg.max_iter) break;
-    if (!hi && (mx <= cfg.switch_tol || mx <= cfg.tol)) {
+    // FP32 products stall at a residual level set by their rounding (grows like sqrt(N));
+    // switch when below switch_tol or when an iteration gains less than a factor 1/0.7
+    const bool stall = it > 0 && mx > 0.7 * prev_mx;
+    prev_mx = mx;
+    if (!hi && (mx <= cfg.switch_tol || mx <= cfg.tol || stall)) {
       // switch to FP64 products: restart on the Ritz vectors with H U recomputed
       hi = true;
       DAV_TRY(hipMemcpyAsync(V, U, sizeof(zc) * (size_t)N * nk, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
@@ -223,7 +229,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
       m = nk;
     }
     DAV_TRY(hipMemcpyAsync(idx, h_idx.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
-    if (mlp && mlp->w) {
+    if (mlp && mlp->w && !hi) {   // the FP64 polish uses the diagonal preconditioner
       DAV_TRY(launch_precond_mlp(s, N, nq, idx, R, N, theta, D, stats, Dmin, *mlp, T, N));
     } else {
       DAV_TRY(launch_precond_diag(s, N, nq, idx, R, N, theta, D, cfg.dmin, T, N));

@@ -1,6 +1,7 @@
 // Learned preconditioner for the Davidson form: a small MLP (5 -> 128 -> 128 -> 1,
-// tanh, FP32) that maps per-state features of the BSE matrix to the preconditioner
-// entry M_i(theta), applied as t_i = M_i r_i in place of 1/(D_i - theta).
+// tanh, FP32) that corrects the diagonal preconditioner entry of each basis state:
+//   t_i = M_i r_i,  M_i = (D_i - theta) / ((D_i - theta)^2 + eta^2) + y_i / sigma,
+// the first term being 1/(D_i - theta) regularised at resonance.
 //
 // Features of basis state i at Ritz value theta (D_i = Re H_ii; row statistics over
 // j != i of the Hermitian matrix):
@@ -8,9 +9,10 @@
 //   f2 = sqrt(N sum_j |H_ij|^2) f3 = N max_j |H_ij|       f4 = theta - min_i D_i
 // (the N factors keep f2, f3 of order one: the kernel entries scale as 1/N_k).
 // Inputs are standardised with the (mu, sd) stored in front of the weights.  The
-// training target is sigma * Re[(H - theta - i eta)^-1]_ii with eta = sigma = 0.05 eV,
-// the Lorentzian-regularised diagonal of the resolvent; the network output y gives
-// M_i = y / sigma.
+// training target is y_i = sigma (Re[(H - theta - i eta)^-1]_ii - (D_i - theta) /
+// ((D_i - theta)^2 + eta^2)) at theta = an exciton energy, eta = sigma = 0.05 eV: the
+// difference between the regularised diagonal of the reduced resolvent and the
+// regularised diagonal preconditioner.
 //
 // Parameter pack (floats): mu[5] sd[5] W1[H][5] b1[H] W2[H][H] b2[H] w3[H] b3.
 // Training: one workgroup per 64 samples computes the forward pass into LDS and the
@@ -80,8 +82,10 @@ __global__ __launch_bounds__(256) void k_precond_mlp(int N, const int* __restric
   if (i >= N) return;
   float f[MLP_F];
   mlp_features(D[i], st[i], st[(size_t)N + i], st[2 * (size_t)N + i], theta[c], Dmin, N, f);
+  const double d = D[i] - theta[c];
+  const double eta = (double)MLP_SIGMA;
   for (int k = 0; k < MLP_F; ++k) f[k] = (f[k] - P[kOffMu + k]) / P[kOffSd + k];
-  const double M = (double)mlp_eval(P, f) / (double)MLP_SIGMA;
+  const double M = d / (d * d + eta * eta) + (double)mlp_eval(P, f) / (double)MLP_SIGMA;
   T[(size_t)i + (size_t)ldt * q] = R[(size_t)i + (size_t)ld * c] * M;
 }
 

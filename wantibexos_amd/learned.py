@@ -5,13 +5,15 @@ Training data are BSE matrices the pipeline itself generates: seeded 2D
 hexagonal models with r0 drawn uniformly from 10-80 A.  For each matrix the
 full FP64 eigendecomposition (GPU, ZHEEV path with vectors) gives the target
 
-    t_i(theta) = sigma Re[(H - theta - i eta)^-1]_ii = sigma sum_n |X_in|^2 (l_n - theta) / ((l_n - theta)^2 + eta^2)
+    t_i(theta) = sigma (Re[(H - theta - i eta)^-1]_ii - g(D_i - theta)),   g(d) = d / (d^2 + eta^2),
+    Re[(H - theta - i eta)^-1]_ii = sum_n |X_in|^2 g(l_n - theta),
 
 at theta equal to one of the lowest exciton energies l_k (eta = sigma = 0.05 eV), where
 the k-th term vanishes (its Lorentzian real part is zero at the pole) so that t is the
 regularised diagonal of the reduced resolvent sum_{n != k} x_n x_n^H / (l_n - l_k), the
-operator of the exact (Jacobi-Davidson) correction; and
-the features of every basis state come from the matrix's diagonal and row
+operator of the exact (Jacobi-Davidson) correction, minus the regularised diagonal
+preconditioner g(D_i - theta) that the network corrects; and the features of every
+basis state come from the matrix's diagonal and row
 statistics (computed on the GPU).  The feature definition is the one of the
 preconditioner kernel:
 
@@ -47,12 +49,15 @@ def features(D, stats, theta, Dmin, N):
     return f.reshape(5, -1).astype(np.float32)
 
 
-def targets(lam, X, theta, eta=SIGMA):
-    """sigma Re[(H - theta - i eta)^-1]_ii, basis state fastest, for each theta."""
+def targets(lam, X, theta, D, eta=SIGMA):
+    """sigma (Re[(H - theta - i eta)^-1]_ii - g(D_i - theta)), basis state fastest, for each theta."""
     w2 = np.abs(X) ** 2
-    d = np.asarray(lam)[:, None] - np.atleast_1d(theta)[None, :]
+    theta = np.atleast_1d(np.asarray(theta, dtype=np.float64))
+    d = np.asarray(lam)[:, None] - theta[None, :]
     g = d / (d * d + eta * eta)
-    return (SIGMA * (w2 @ g)).T.reshape(-1).astype(np.float32)
+    dd = np.asarray(D)[None, :] - theta[:, None]
+    reg = dd / (dd * dd + eta * eta)
+    return (SIGMA * ((w2 @ g).T - reg)).reshape(-1).astype(np.float32)
 
 
 def init_params(rng, mu, sd):
@@ -99,7 +104,7 @@ def sample_matrix(ctx, model, prm, rng, n_theta=8, n_low=64):
     # the operator of the exact (Jacobi-Davidson) correction
     ks = rng.choice(min(n_low, N), size=min(n_theta, N), replace=False)
     theta = lam[ks]
-    return features(D, st, theta, D.min(), N), targets(lam, X, theta), N
+    return features(D, st, theta, D.min(), N), targets(lam, X, theta, D), N
 
 
 def make_training_set(ctx, n_models=200, seed=1234, nw_range=(6, 16), ngrid=(12, 12, 1), n_theta=8):
@@ -136,3 +141,31 @@ def save(P, path=WEIGHTS, **meta):
 def load(path=WEIGHTS):
     with np.load(path, allow_pickle=False) as z:
         return z["params"].astype(np.float32)
+
+
+def main(argv=None):
+    """python -m wantibexos_amd.learned [--models 200] [--epochs 20] [--out path]"""
+    import argparse
+    import sys
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--models", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--out", default=str(WEIGHTS))
+    a = ap.parse_args(argv)
+    with bse.Context(0) as ctx:
+        t0 = time.perf_counter()
+        data = make_training_set(ctx, n_models=a.models, seed=a.seed)
+        t1 = time.perf_counter()
+        print(f"training set: {data.X.shape[1]} samples from {data.n_models} matrices "
+              f"(N = {min(data.dims)}..{max(data.dims)}) in {t1 - t0:.1f} s", file=sys.stderr, flush=True)
+        P, hist = train(ctx, data, epochs=a.epochs)
+        t2 = time.perf_counter()
+        print(f"trained {a.epochs} epochs in {t2 - t1:.1f} s; loss {hist[0]:.5f} -> {hist[-1]:.5f}; "
+              f"target variance {float(np.var(data.t)):.5f}", file=sys.stderr, flush=True)
+        save(P, a.out, loss=hist, n_models=data.n_models, seed=a.seed, target_var=float(np.var(data.t)))
+
+
+if __name__ == "__main__":
+    main()

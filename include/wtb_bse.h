@@ -185,7 +185,8 @@ enum {
   WTB_BUF_HBSE = 6,      /* BSE matrix, column-major N x N complex (before the solve) */
   WTB_BUF_WTAB = 7,      /* W(Delta) table, [D1][D2][D3] double                       */
   WTB_BUF_TRIDIAG = 8,   /* d[N] then e[N] double (after the reduction)                */
-  WTB_BUF_NBUF = 9
+  WTB_BUF_EXCWF = 9,     /* exciton vectors [count][N] complex (wtb_bse_plan_set_vectors) */
+  WTB_BUF_NBUF = 10
 };
 size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* plan, int what);
 int wtb_bse_plan_fetch(wtb_bse_plan* plan, int what, void* dst, size_t bytes);
@@ -256,6 +257,42 @@ int wtb_mlp_loss_grad(wtb_mlp* mlp, int S, const float* X, const float* t, float
 int wtb_mlp_train(wtb_mlp* mlp, int S, const float* X, const float* t, int epochs, int batch, float lr,
                   unsigned seed, float* loss_hist);
 int wtb_mlp_destroy(wtb_mlp* mlp);
+
+/* ------------------------------------------------------------------------
+ * Tile shape of the reduction's Hermitian GEMV (strip of 1, 2, 4 or 8
+ * 64 x 64 tiles per workgroup) and its autotuner, which times the reduction
+ * for each shape on the plan's own matrix and keeps the fastest (ms[4]:
+ * the times for 1, 2, 4, 8).  No counterpart in the reference.
+ * ------------------------------------------------------------------------ */
+int wtb_bse_plan_set_tile(wtb_bse_plan* plan, int sub);
+int wtb_bse_plan_tune(wtb_bse_plan* plan, int* best, double* ms);
+
+/* ------------------------------------------------------------------------
+ * Batching form (no counterpart in the reference, which solves one model
+ * per run): many independent models of uneven sizes, one plan each, assigned
+ * to nlanes streams largest-first; execute() replays every matrix's captured
+ * graph on its stream and joins on the context stream; results() as for a plan.
+ * ------------------------------------------------------------------------ */
+typedef struct wtb_bse_batch wtb_bse_batch;
+int wtb_bse_batch_create(wtb_ctx* ctx, int n, const wtb_tb_model* models, const wtb_bse_params* prms,
+                         int nlanes, wtb_bse_batch** out);
+int wtb_bse_batch_size(const wtb_bse_batch* batch);
+int wtb_bse_batch_dim(const wtb_bse_batch* batch, int i);
+int wtb_bse_batch_tune(wtb_bse_batch* batch, int* tiles);
+int wtb_bse_batch_execute(wtb_bse_batch* batch);
+int wtb_bse_batch_results(wtb_bse_batch* batch, int i, double* W, double* act, double* eps, double* egap);
+int wtb_bse_batch_destroy(wtb_bse_batch* batch);
+
+/* BSE_WF (bse_solver-tool-diel.f90:513-519, excwf2 in bse_subs.f90:534-576):
+ * executes also form the eigenvectors of excitons first .. first+count-1
+ * (0-based, ascending energy), fetched as WTB_BUF_EXCWF, each with its last
+ * component real and >= 0 (ZHEEV('V','U') leaves it real with either sign). */
+int wtb_bse_plan_set_vectors(wtb_bse_plan* plan, int first, int count);
+/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid
+ * (kpt(nk,3) Fortran order) and nocpk(nk) that excwf2 prints beside them */
+int wtb_bsesolver_wf(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, double* W,
+                     double* act, double* eps, double* egap, int wf_first, int wf_count, double* wf,
+                     double* kpt, int* nocp);
 
 #ifdef __cplusplus
 }

@@ -23,9 +23,9 @@ P = dict(ngrid=(6, 6, 1), shift=(0.0, 0.0, 0.0), nc=2, nv=2, coultype="V2DRK", r
          ebsef=6.0, numbse=1001, dtfull=True, renorm=True)
 
 
-def run(exe, td, outdir):
+def run(exe, td, outdir, p=P):
     (td / outdir).mkdir()
-    inp = tbmodel.input_dat(P, "tb.dat", nthreads=4, outdir=f"./{outdir}/")
+    inp = tbmodel.input_dat(p, "tb.dat", nthreads=4, outdir=f"./{outdir}/")
     (td / f"input_{outdir}.dat").write_text(inp)
     env = dict(os.environ, OMP_NUM_THREADS="4")
     r = subprocess.run([str(exe)], stdin=open(td / f"input_{outdir}.dat"), cwd=td, env=env,
@@ -60,3 +60,30 @@ def test_dropin_outputs_match_reference(tmp_path):
         for c in (1, 2):
             scale = max(np.abs(ea[:, c]).max(), 1e-30)
             assert np.abs(ea[:, c] - eb[:, c]).max() / scale < 1e-5, (comp, c)
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+def test_dropin_exciton_wavefunctions(tmp_path):
+    """BSE_WF=T: exc_wf_N.dat written by the reference's own excwf2 from the GPU's
+    vectors.  The index columns agree exactly; the amplitudes agree in modulus (the
+    vectors' phase is fixed only up to the band-gauge and exciton signs) where the
+    exciton is non-degenerate."""
+    p = dict(P, bsewf=True, excwf=(1, 3))
+    model = tbmodel.hex2d(12, 8, P["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    w = table(tmp_path / "ref" / "bse_opt_diel.dat")[:, 0]
+    for n in (1, 2, 3):
+        fa, fb = tmp_path / "ref" / f"exc_wf_{n}.dat", tmp_path / "gpu" / f"exc_wf_{n}.dat"
+        assert fa.exists(), rr.stdout + rr.stderr
+        a, b = table(fa, skip=7), table(fb, skip=7)
+        assert a.shape == b.shape == (144, 8)
+        assert np.array_equal(a[:, [0, 4, 5]], b[:, [0, 4, 5]])
+        assert np.abs(a[:, 1:4] - b[:, 1:4]).max() <= 1e-4
+        gap = min(abs(w[n - 1] - w[k]) for k in range(len(w)) if k != n - 1)
+        if gap > 1e-3:
+            ma = np.hypot(a[:, 6], a[:, 7])
+            mb = np.hypot(b[:, 6], b[:, 7])
+            assert np.abs(ma - mb).max() <= 2e-4 * max(ma.max(), 1e-30) + 1e-8, n

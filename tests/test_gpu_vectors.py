@@ -83,3 +83,30 @@ def test_bse_first_exciton_vector(ctx, name):
     x = X[:, 0]
     sgn = np.sign(np.vdot(ref, x).real)
     assert np.abs(x * sgn - ref).max() < 1e-9
+
+
+@pytest.mark.parametrize("name", ["hex8_g6_v2drk", "cub6_g3_v3dl"])
+def test_plan_exciton_vectors(ctx, name):
+    """BSE_WF through the pipeline: the plan's exciton vectors (formed during execute)
+    are eigenvectors of its own BSE matrix, orthonormal, and the first one is the
+    reference's hbse(:,1) up to its sign (after the band-gauge signs)."""
+    from wantibexos_amd import bse
+    m, p, z = load(name)
+    with bse.BSEPlan(ctx, m, params_of(p)) as plan:
+        plan.build()
+        H = plan.fetch("hbse")
+        plan.set_vectors(0, 5)
+        plan.execute()
+        r = plan.results()
+        X = plan.fetch("excwf")
+    Hf = np.tril(H) + np.tril(H, -1).conj().T
+    W = r["W"][:5]
+    assert np.abs(Hf @ X - X * W).max() < 1e-12 * max(1.0, np.abs(Hf).max()) * 10
+    assert np.abs(X.conj().T @ X - np.eye(5)).max() < 1e-12
+    assert np.all(np.abs(X[-1].imag) <= 1e-15) and np.all(X[-1].real >= 0)
+    if r["W"][1] - r["W"][0] > 1e-6:
+        _, sg = ref_signs(z)
+        ref = z["xvec1"] * sg
+        ref = ref * np.exp(-1j * np.angle(ref[-1]))
+        x = X[:, 0] * np.sign(np.vdot(ref, X[:, 0]).real)
+        assert np.abs(x - ref).max() < 1e-9

@@ -38,7 +38,7 @@ ERRORS = {
 COULOMB = {"V2DK": 1, "V3D": 2, "V3DL": 3, "V2DT": 4, "V2DT2": 5, "V2DOH": 6, "V2DRK": 7,
            "V1DT": 8, "V0DT": 9}
 FP64, MIXED = 0, 1
-BUF = dict(kpt=0, eall=1, nocp=2, eigv=3, vector=4, dipole=5, hbse=6, wtab=7, tridiag=8)
+BUF = dict(kpt=0, eall=1, nocp=2, eigv=3, vector=4, dipole=5, hbse=6, wtab=7, tridiag=8, excwf=9)
 NSTAGES = 6
 STAGES = ["eig", "dipoles", "build", "tridiag", "dc", "spectrum"]
 
@@ -119,6 +119,17 @@ def _load():
         "wtb_mlp_loss_grad": (ip, [vp, ip, vp, vp, vp, vp]),
         "wtb_mlp_train": (ip, [vp, ip, vp, vp, ip, ip, C.c_float, C.c_uint, vp]),
         "wtb_mlp_destroy": (ip, [vp]),
+        "wtb_bse_plan_set_tile": (ip, [vp, ip]),
+        "wtb_bse_plan_set_vectors": (ip, [vp, ip, ip]),
+        "wtb_bsesolver_wf": (ip, [vp, vp, vp, vp, vp, vp, vp, ip, ip, vp, vp, vp]),
+        "wtb_bse_plan_tune": (ip, [vp, vp, vp]),
+        "wtb_bse_batch_create": (ip, [vp, ip, vp, vp, ip, C.POINTER(vp)]),
+        "wtb_bse_batch_size": (ip, [vp]),
+        "wtb_bse_batch_dim": (ip, [vp, ip]),
+        "wtb_bse_batch_tune": (ip, [vp, vp]),
+        "wtb_bse_batch_execute": (ip, [vp]),
+        "wtb_bse_batch_results": (ip, [vp, ip, vp, vp, vp, vp]),
+        "wtb_bse_batch_destroy": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -371,6 +382,8 @@ class BSEPlan:
             return raw.view(np.float64).reshape(2 * p.ngrid[0] - 1, 2 * p.ngrid[1] - 1, 2 * p.ngrid[2] - 1)
         if what == "tridiag":
             return raw.view(np.float64).reshape(2, N)
+        if what == "excwf":
+            return raw.view(np.complex128).reshape(-1, N).T            # (N, count)
         return raw
 
     def lowest(self, nev=256, guard=32, max_sub=None, max_iter=60, tol=1e-6, lanczos_steps=600,
@@ -392,6 +405,22 @@ class BSEPlan:
         inf = {k: getattr(info, k) for k, _ in _DavidsonInfo._fields_}
         return dict(W=W, resid=res, eps=eps, X=X.T if X is not None else None, info=inf)
 
+    def set_vectors(self, first, count):
+        """Also form the eigenvectors of excitons first .. first+count-1 (fetch('excwf'))."""
+        _check(lib().wtb_bse_plan_set_vectors(self.h, int(first), int(count)))
+
+    def set_tile(self, sub):
+        """GEMV strip height of the reduction: 1, 2, 4 or 8 tiles of 64 x 64."""
+        _check(lib().wtb_bse_plan_set_tile(self.h, int(sub)))
+
+    def tune(self):
+        """Time the reduction for each strip height on this matrix and keep the fastest.
+        Returns (best, {height: ms})."""
+        best = np.zeros(1, dtype=np.int32)
+        ms = np.zeros(4)
+        _check(lib().wtb_bse_plan_tune(self.h, _p(best), _p(ms)))
+        return int(best[0]), dict(zip((1, 2, 4, 8), ms))
+
     def row_stats(self):
         """Diagonal D (N,) and row statistics (3, N) of the matrix (precision-form plan)."""
         D = np.zeros(self.N)
@@ -402,6 +431,51 @@ class BSEPlan:
     def close(self):
         if self.h:
             lib().wtb_bse_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class BSEBatch:
+    """Batching form: many independent models, one plan each, on `lanes` streams."""
+
+    def __init__(self, ctx: Context, models, params, lanes: int = 4):
+        self.ctx = ctx
+        self.params = list(params)
+        self._cms = [_CModel(m) for m in models]
+        cm_arr = (_Model * len(models))(*[cm.s for cm in self._cms])
+        cp_arr = (_Params * len(models))(*[p.to_c() for p in self.params])
+        self._arrs = (cm_arr, cp_arr)
+        h = C.c_void_p()
+        _check(lib().wtb_bse_batch_create(ctx.h, len(models), C.cast(cm_arr, C.c_void_p),
+                                          C.cast(cp_arr, C.c_void_p), lanes, C.byref(h)))
+        self.h = h
+        self.dims = [lib().wtb_bse_batch_dim(h, i) for i in range(len(models))]
+
+    def tune(self):
+        t = np.zeros(len(self.dims), dtype=np.int32)
+        _check(lib().wtb_bse_batch_tune(self.h, _p(t)))
+        return t
+
+    def execute(self):
+        _check(lib().wtb_bse_batch_execute(self.h))
+
+    def results(self, i):
+        N, nb = self.dims[i], int(self.params[i].numbse)
+        W = np.zeros(N)
+        act = np.zeros((8, N))
+        eps = np.zeros((6, 2, nb))
+        egap = np.zeros(1)
+        _check(lib().wtb_bse_batch_results(self.h, i, _p(W), _p(act), _p(eps), _p(egap)))
+        return dict(W=W, act=act, eps=eps, egap=float(egap[0]))
+
+    def close(self):
+        if self.h:
+            lib().wtb_bse_batch_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -13,9 +13,10 @@
 //               with the panel (ZLATRD), leaving out the 2 Re(alpha') v term of
 //               W(:,jj-1) that needs the global u^H v.  The last workgroup to
 //               finish reduces u^H v and publishes alpha'.
-//   hemvstep(j) lower 64x64 tiles of A22 (rows, cols > j).  Adds the deferred
-//               term to column j, completes W(:,jj-1), and forms z = A22 x~ with
-//               x~ = column j below row j+1 (the reflector before its scale is
+//   hemvstep(j) strips of up to 4 lower 64x64 tiles of A22 (rows, cols > j) in
+//               one column block per workgroup.  Adds the deferred term to
+//               column j, completes W(:,jj-1), and forms z = A22 x~ with x~ =
+//               column j below row j+1 (the reflector before its scale is
 //               known).  Each tile is read once and serves both A x~ (rows) and
 //               A^H x~ (cols).  Diagonal tiles also write x_new, d_j and partials
 //               of |x~|^2, W^H x~, V^H x~, x~^T g; the last of them reduces those
@@ -200,90 +201,117 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
 }
 
 // ---- hemvstep -----------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj) {
-  __shared__ zc xR[TB], xC[TB];
-  __shared__ zc y1s[4][TB], y2s[1][TB];
+// One workgroup per strip of up to SUB lower 64x64 tiles stacked in one column block
+// (column block bj, row blocks bi0 .. bi0+SUB-1, bi0 >= bj): 8 waves, wave w owns 8
+// columns, lane = row of the current tile, so every load moves 1 KB of one column and a
+// wave walks 256 consecutive rows of its columns.  Row sums (A x~) are written per tile;
+// the column sums (A^H x~) accumulate over the strip in registers and are reduced once.
+template <class T, int SUB>
+__global__ __launch_bounds__(512) void k_tri_hemv(TriArgs<T> a, int j, int jj) {
+  __shared__ zc xR[SUB][TB], xC[TB];
+  __shared__ zc y1s[8][TB], y2s[TB], ydiag[TB];
   __shared__ zc sums[96];
   __shared__ zc s_sc;
   const int N = a.N;
   const int b0 = (j + 1) / TB;
-  int t = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-  while (ti * (ti + 1) / 2 > t) --ti;
-  const int tj = t - ti * (ti + 1) / 2;
-  const int bi = b0 + ti, bj = b0 + tj;
+  const int bj = b0 + blockIdx.y;
+  const int bi0 = bj + SUB * blockIdx.x;
+  if (bj >= a.nblk || bi0 >= a.nblk) return;
+  const int nsub = min(SUB, a.nblk - bi0);
+  const bool dstrip = bi0 == bj;                   // the strip starts with the diagonal tile
   const int tid = threadIdx.x;
-  const int R0 = bi * TB, C0 = bj * TB;
-  const bool diag = bi == bj;
+  const int C0 = bj * TB;
   const zc ap = (jj > 0) ? a.pub->alpha : mkz(0, 0);
   const double sig = 2.0 * ap.re;                  // x_new = x_p - 2 Re(alpha') v_{j-1}
   auto xnew = [&](int row) -> zc {
     if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
     return a.xp[row] - sig * a.Vp[(size_t)(jj - 1) * N + row];
   };
+  for (int e = tid; e < SUB * TB; e += 512) {
+    const int ts = e >> 6, l = e & 63;
+    const int row = (bi0 + ts) * TB + l;
+    xR[ts][l] = (ts < nsub && row >= j + 2 && row < N) ? xnew(row) : mkz(0, 0);
+  }
   if (tid < TB) {
-    const int row = R0 + tid;
-    xR[tid] = (row >= j + 2 && row < N) ? xnew(row) : mkz(0, 0);
-  } else if (tid < 2 * TB) {
-    const int col = C0 + tid - TB;
-    xC[tid - TB] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
+    const int col = C0 + tid;
+    xC[tid] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
   }
-  // wave w takes columns C0+16w .. C0+16w+15, lane = row: every load moves 1 KB of a column
   const int lane = tid & 63, w = tid >> 6;
-  const int r = R0 + lane;
-  const bool rvalid = r > j && r < N;
-  zc hv[16];
-  for (int k = 0; k < 16; ++k) {
-    const int cl = 16 * w + k, c = C0 + cl;
-    zc x = mkz(0, 0);
-    if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
-    hv[k] = x;
-  }
+  const int c0 = C0 + 8 * w;                       // this wave's first column
+  zc hacc[8];
+  for (int k = 0; k < 8; ++k) hacc[k] = mkz(0, 0);
   __syncthreads();
-  const zc xr = xR[lane];
-  zc y1 = mkz(0, 0);
-  for (int k = 0; k < 16; ++k) {
-    const int cl = 16 * w + k;
-    zc x = hv[k];
-    if (diag && lane == cl) x.im = 0.0;                   // diagonal: real part (ZHEMV)
-    y1 = fma_z(x, xC[cl], y1);
-    hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
+#pragma unroll
+  for (int ts = 0; ts < SUB; ++ts) {
+    if (ts < nsub) {
+      const int bi = bi0 + ts;
+      const bool diag = dstrip && ts == 0;
+      const int r = bi * TB + lane;
+      const bool rvalid = r > j && r < N;
+      zc hv[8];
+      for (int k = 0; k < 8; ++k) {
+        const int c = c0 + k, cl = 8 * w + k;
+        zc x = mkz(0, 0);
+        if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
+        hv[k] = x;
+      }
+      const zc xr = xR[ts][lane];
+      zc y1 = mkz(0, 0);
+      for (int k = 0; k < 8; ++k) {
+        const int cl = 8 * w + k;
+        zc x = hv[k];
+        if (diag && lane == cl) x.im = 0.0;                      // diagonal: real part (ZHEMV)
+        y1 = fma_z(x, xC[cl], y1);
+        if (!diag || lane > cl) hacc[k] += conj(x) * xr;         // A^H x~ terms (strictly lower)
+      }
+      y1s[w][lane] = y1;
+      __syncthreads();
+      if (tid < TB) {
+        zc sm = mkz(0, 0);
+        for (int q = 0; q < 8; ++q) sm += y1s[q][tid];
+        if (diag) ydiag[tid] = sm;
+        else a.ypart[((size_t)bi * a.nblk + bj) * TB + tid] = sm;
+      }
+      __syncthreads();
+    }
   }
-  // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
+  // column sums of the strip: transpose-reduce the 8 column partials over the 64 lanes
   {
-    int n = 16;
-    for (int off = 32; off >= 4; off >>= 1) {
+    int n = 8;
+    for (int off = 32; off >= 8; off >>= 1) {
       n >>= 1;
       const bool up = (lane & off) != 0;
       for (int i = 0; i < n; ++i) {
-        const zc send = up ? hv[i] : hv[i + n];
+        const zc send = up ? hacc[i] : hacc[i + n];
         zc recv;
         recv.re = __shfl_xor(send.re, off);
         recv.im = __shfl_xor(send.im, off);
-        hv[i] = (up ? hv[i + n] : hv[i]) + recv;
+        hacc[i] = (up ? hacc[i + n] : hacc[i]) + recv;
       }
     }
-    for (int off = 2; off > 0; off >>= 1) {
-      hv[0].re += __shfl_xor(hv[0].re, off);
-      hv[0].im += __shfl_xor(hv[0].im, off);
+    for (int off = 4; off > 0; off >>= 1) {
+      hacc[0].re += __shfl_xor(hacc[0].re, off);
+      hacc[0].im += __shfl_xor(hacc[0].im, off);
     }
   }
-  // lane holds the sum for column 16w + ((lane >> 2) & 15)
-  y1s[w][lane] = y1;
-  if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
+  // lane holds the sum for column 8w + ((lane >> 3) & 7)
+  if ((lane & 7) == 0) y2s[8 * w + ((lane >> 3) & 7)] = hacc[0];
   __syncthreads();
   zc* yp = a.ypart;
   if (tid < TB) {
-    zc sm = y1s[0][tid] + y1s[1][tid] + y1s[2][tid] + y1s[3][tid];
-    if (diag) sm += y2s[0][tid];
-    yp[((size_t)bi * a.nblk + bj) * TB + tid] = sm;
-  } else if (tid < 2 * TB && !diag) {
-    yp[((size_t)bj * a.nblk + bi) * TB + (tid - TB)] = y2s[0][tid - TB];
+    // A^H x~ of the whole strip into slot (bj, bi0) (with the diagonal tile's A x~ there);
+    // the strip's other slots of row block bj hold zero
+    zc sm = y2s[tid];
+    if (dstrip) sm += ydiag[tid];
+    yp[((size_t)bj * a.nblk + bi0) * TB + tid] = sm;
+  } else if (tid < SUB * TB) {
+    const int ts = tid / TB;
+    if (ts < nsub) yp[((size_t)bj * a.nblk + bi0 + ts) * TB + (tid % TB)] = mkz(0, 0);
   }
-  if (!diag) return;
+  if (!dstrip) return;
   // ---- diagonal tiles: x_new, W(:,jj-1), d_j, partial dot products --------------------
+  const int bi = bj;
+  const int R0 = bi * TB;
   const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
   if (tid < TB) {
     const int r = R0 + tid;
@@ -292,24 +320,24 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj) {
       if (jj > 0) a.W[(size_t)(jj - 1) * N + r] = taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r];
     }
   }
-  if (ti == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
+  if (blockIdx.y == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
   // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g; 4 lanes each
   const int nq = 1 + 2 * jj + a.nrhs;
   const int sub = tid & 3;
-  for (int q0 = 0; q0 < nq; q0 += 64) {
+  for (int q0 = 0; q0 < nq; q0 += 128) {
     const int q = q0 + (tid >> 2);
     zc acc = mkz(0, 0);
     if (q < nq) {
       for (int rl = sub; rl < TB; rl += 4) {
         const int r = R0 + rl;
         if (r < j + 2 || r >= N) continue;
-        const zc xr = xR[rl];
+        const zc xr = xR[0][rl];
         if (q == 0) {
           acc.re += norm2(xr);
         } else if (q <= jj) {
           const int c = q - 1;
-          const zc w = (c == jj - 1) ? taup * a.u[r] + ap * a.Vp[(size_t)c * N + r] : a.W[(size_t)c * N + r];
-          acc = fma_conj(w, xr, acc);
+          const zc wv = (c == jj - 1) ? taup * a.u[r] + ap * a.Vp[(size_t)c * N + r] : a.W[(size_t)c * N + r];
+          acc = fma_conj(wv, xr, acc);
         } else if (q <= 2 * jj) {
           acc = fma_conj(a.Vp[(size_t)(q - 1 - jj) * N + r], xr, acc);
         } else {
@@ -330,7 +358,7 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj) {
   }
   if (!arrive_last(a.cnt + 0, (unsigned)(a.nblk - b0))) return;
   // ---- last diagonal tile: sums, ZLARFG, W^H v, V^H v, v^T g --------------------------
-  for (int q0 = 0; q0 < nq; q0 += 64) {
+  for (int q0 = 0; q0 < nq; q0 += 128) {
     const int q = q0 + (tid >> 2);
     zc acc = mkz(0, 0);
     if (q < nq) {
@@ -622,7 +650,12 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
       const int nt = a.nblk - (j + 1) / TB;
       const bool tev = tw.kev && tw.kev_used < tw.kev_cap;
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used], s);
-      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj);
+      switch (tw.sub) {
+        case 1: hipLaunchKernelGGL((k_tri_hemv<T, 1>), dim3(nt, nt), dim3(512), 0, s, a, j, jj); break;
+        case 2: hipLaunchKernelGGL((k_tri_hemv<T, 2>), dim3((nt + 1) / 2, nt), dim3(512), 0, s, a, j, jj); break;
+        case 8: hipLaunchKernelGGL((k_tri_hemv<T, 8>), dim3((nt + 7) / 8, nt), dim3(512), 0, s, a, j, jj); break;
+        default: hipLaunchKernelGGL((k_tri_hemv<T, 4>), dim3((nt + 3) / 4, nt), dim3(512), 0, s, a, j, jj); break;
+      }
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used++ + 1], s);
     }
     // finish the panel's last reflector (no further column in this panel)

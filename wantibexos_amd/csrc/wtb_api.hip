@@ -111,6 +111,12 @@ struct wtb_bse_plan {
   double* eps = nullptr;
   double* smeavg = nullptr;
   int* status = nullptr;
+  hipStream_t stream = nullptr;             // launches go here (the context stream unless batched)
+  // exciton eigenvectors [wf_first, wf_first + wf_count) (BSE_WF); zero count: none formed
+  int wf_first = 0, wf_count = 0;
+  double* Zv = nullptr;                     // N x N tridiagonal eigenvectors
+  zc* Xv = nullptr;                         // N x wf_count exciton vectors
+  void* btw = nullptr;                      // back-transform workspace
   bool timing = false;
   bool kernel_timing = false;              // per-launch events on the GEMV kernel
   std::vector<hipEvent_t> kev;             // 2 per tridiagonal column
@@ -333,7 +339,7 @@ static void plan_free(wtb_bse_plan* p) {
   free_model(p->dm);
   void* bufs[] = {p->kpt, p->phase, p->hk4, p->eall, p->nocp, p->eigv, p->vec, p->dip, p->g,
                   p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
-                  p->wq, p->actq, p->eps, p->smeavg, p->status};
+                  p->wq, p->actq, p->eps, p->smeavg, p->status, p->Zv, p->Xv, p->btw};
   for (void* b : bufs) hipFree(b);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
@@ -382,6 +388,7 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
   WTB_HIP(hipSetDevice(c->device));
   wtb_bse_plan* p = new wtb_bse_plan;
   p->ctx = c;
+  p->stream = c->stream;
   p->prm = *prm;
   p->mixed = prm->precision == WTB_MIXED;
   p->nk = prm->ngrid[0] * prm->ngrid[1] * prm->ngrid[2];
@@ -464,8 +471,9 @@ int wtb_bse_plan_stage_ms(const wtb_bse_plan* p, double* ms) {
   return WTB_OK;
 }
 
+static void plan_drop_graph(wtb_bse_plan* p);
 static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const wtb_bse_params& P = p->prm;
   const int nk = p->nk, nw = p->nw, N = p->N;
   int rc;
@@ -503,7 +511,16 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   mark(4);
   // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
   hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->g, p->track);
-  if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, p->track, p->lambda))) return rc;
+  if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, p->track, p->lambda,
+                   p->wf_count > 0 ? p->Zv : nullptr)))
+    return rc;
+  if (p->wf_count > 0) {
+    rc = p->mixed ? run_backtr_f32(s, p->Hf, (int)p->ldh, N, p->tw.tau, p->Zv, p->wf_first, p->wf_count, p->Xv, N,
+                                   p->btw)
+                  : run_backtr(s, p->H, (int)p->ldh, N, p->tw.tau, p->Zv, p->wf_first, p->wf_count, p->Xv, N,
+                               p->btw);
+    if (rc) return rc;
+  }
   mark(5);
   // (e) activities, F15.4 round trip, eps(omega)
   if ((rc = launch_activities(s, N, p->track, P.dtfull, P.cpol, p->act))) return rc;
@@ -525,7 +542,7 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
 // The whole pipeline is captured once into a HIP graph (a few thousand kernel
 // nodes for the column-by-column reduction) and replayed on later executes.
 static int plan_capture(wtb_bse_plan* p) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   WTB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = plan_enqueue(p);
   hipGraph_t g = nullptr;
@@ -559,7 +576,7 @@ int wtb_bse_plan_execute(wtb_bse_plan* p) {
     int rc = plan_capture(p);
     if (rc) return rc;
   }
-  WTB_HIP(hipGraphLaunch(p->gexec, p->ctx->stream));
+  WTB_HIP(hipGraphLaunch(p->gexec, p->stream));
   return WTB_OK;
 }
 
@@ -640,6 +657,7 @@ size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* p, int what) {
     case WTB_BUF_HBSE: return (p->mixed ? sizeof(zf) : sizeof(zc)) * p->ldh * p->N;
     case WTB_BUF_WTAB: return sizeof(double) * p->wtab_n;
     case WTB_BUF_TRIDIAG: return sizeof(double) * 2 * p->N;
+    case WTB_BUF_EXCWF: return sizeof(zc) * (size_t)p->N * p->wf_count;
     default: return 0;
   }
 }
@@ -660,6 +678,7 @@ int wtb_bse_plan_fetch(wtb_bse_plan* p, int what, void* dst, size_t bytes) {
     case WTB_BUF_DIPOLE: src = p->dip; break;
     case WTB_BUF_HBSE: src = p->mixed ? (const void*)p->Hf : (const void*)p->H; break;
     case WTB_BUF_WTAB: src = p->wtab; break;
+    case WTB_BUF_EXCWF: src = p->Xv; break;
     case WTB_BUF_TRIDIAG: {
       WTB_HIP(hipMemcpyAsync(dst, p->tw.d, sizeof(double) * p->N, hipMemcpyDeviceToHost, s));
       WTB_HIP(hipMemcpyAsync((char*)dst + sizeof(double) * p->N, p->tw.e, sizeof(double) * p->N,
@@ -1114,6 +1133,228 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     if (e) hipEventDestroy(e);
   for (void* b : {(void*)dD, (void*)dst, (void*)dW, (void*)dR, (void*)dsm, (void*)dX, hw}) hipFree(b);
   if (info) *info = inf;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// GEMV tile shape and its autotuner
+// ---------------------------------------------------------------------------
+static void plan_drop_graph(wtb_bse_plan* p) {
+  if (p->gexec) hipGraphExecDestroy(p->gexec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+}
+
+int wtb_bse_plan_set_tile(wtb_bse_plan* p, int sub) {
+  if (!p || !(sub == 1 || sub == 2 || sub == 4 || sub == 8)) {
+    set_error("wtb_bse_plan_set_tile: strip height must be 1, 2, 4 or 8 tiles");
+    return WTB_ERR_ARG;
+  }
+  if (p->tw.sub != sub) plan_drop_graph(p);
+  p->tw.sub = sub;
+  return WTB_OK;
+}
+
+int wtb_bse_plan_tune(wtb_bse_plan* p, int* best, double* ms) {
+  if (!p) {
+    set_error("wtb_bse_plan_tune: null plan");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  const int cand[4] = {1, 2, 4, 8};
+  const bool timing0 = p->timing;
+  const int sub0 = p->tw.sub;
+  double tbest = 1e300;
+  int sbest = sub0;
+  p->timing = true;
+  int rc = WTB_OK;
+  for (int k = 0; k < 4 && !rc; ++k) {
+    p->tw.sub = cand[k];
+    double t = 1e300;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {   // the first run warms up
+      rc = plan_enqueue(p);
+      if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = WTB_ERR_HIP;
+      float f;
+      if (!rc && hipEventElapsedTime(&f, p->ev[3], p->ev[4]) == hipSuccess) t = f;
+    }
+    if (ms) ms[k] = t;
+    if (t < tbest) {
+      tbest = t;
+      sbest = cand[k];
+    }
+  }
+  p->timing = timing0;
+  p->tw.sub = sbest;
+  if (sbest != sub0) plan_drop_graph(p);
+  if (best) *best = sbest;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// batching form: many models, one graph per matrix replayed on parallel streams
+// ---------------------------------------------------------------------------
+struct wtb_bse_batch {
+  wtb_ctx* ctx = nullptr;
+  std::vector<wtb_bse_plan*> plans;
+  std::vector<hipStream_t> lanes;
+  std::vector<int> lane_of;             // plan -> lane
+  std::vector<hipEvent_t> ev_done;      // one per lane
+  hipEvent_t ev_start = nullptr;
+};
+
+int wtb_bse_batch_create(wtb_ctx* c, int n, const wtb_tb_model* models, const wtb_bse_params* prms, int nlanes,
+                         wtb_bse_batch** out) {
+  if (!c || n < 1 || !models || !prms || !out || nlanes < 1 || nlanes > 8) {
+    set_error("wtb_bse_batch_create: bad argument (1 <= nlanes <= 8)");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  auto* b = new wtb_bse_batch();
+  b->ctx = c;
+  int rc = WTB_OK;
+  for (int i = 0; i < n && !rc; ++i) {
+    wtb_bse_plan* p = nullptr;
+    rc = wtb_bse_plan_create(c, &models[i], &prms[i], &p);
+    if (!rc) b->plans.push_back(p);
+  }
+  for (int l = 0; l < nlanes && !rc; ++l) {
+    hipStream_t st;
+    hipEvent_t e;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      rc = WTB_ERR_HIP;
+      break;
+    }
+    b->lanes.push_back(st);
+    b->ev_done.push_back(e);
+  }
+  if (!rc && hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming) != hipSuccess) rc = WTB_ERR_HIP;
+  if (rc) {
+    wtb_bse_batch_destroy(b);
+    return rc;
+  }
+  // longest processing time first: the largest N^3 goes to the least loaded lane
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return b->plans[x]->N > b->plans[y]->N; });
+  std::vector<double> load(nlanes, 0.0);
+  b->lane_of.assign(n, 0);
+  for (int i : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->lane_of[i] = l;
+    const double N = b->plans[i]->N;
+    load[l] += N * N * N;
+    b->plans[i]->stream = b->lanes[l];
+  }
+  *out = b;
+  return WTB_OK;
+}
+
+int wtb_bse_batch_size(const wtb_bse_batch* b) { return b ? (int)b->plans.size() : 0; }
+
+int wtb_bse_batch_dim(const wtb_bse_batch* b, int i) {
+  return (b && i >= 0 && i < (int)b->plans.size()) ? b->plans[i]->N : 0;
+}
+
+int wtb_bse_batch_tune(wtb_bse_batch* b, int* tiles) {
+  if (!b) return WTB_ERR_ARG;
+  for (size_t i = 0; i < b->plans.size(); ++i) {
+    int best = 4;
+    int rc = wtb_bse_plan_tune(b->plans[i], &best, nullptr);
+    if (rc) return rc;
+    if (tiles) tiles[i] = best;
+  }
+  return WTB_OK;
+}
+
+// fork the lanes from the context stream, replay every matrix's graph on its lane (the
+// first call captures them), join back
+int wtb_bse_batch_execute(wtb_bse_batch* b) {
+  if (!b) return WTB_ERR_ARG;
+  WTB_HIP(hipSetDevice(b->ctx->device));
+  WTB_HIP(hipEventRecord(b->ev_start, b->ctx->stream));
+  for (hipStream_t l : b->lanes) WTB_HIP(hipStreamWaitEvent(l, b->ev_start, 0));
+  for (size_t i = 0; i < b->plans.size(); ++i) {
+    int rc = wtb_bse_plan_execute(b->plans[i]);
+    if (rc) return rc;
+  }
+  for (size_t l = 0; l < b->lanes.size(); ++l) {
+    WTB_HIP(hipEventRecord(b->ev_done[l], b->lanes[l]));
+    WTB_HIP(hipStreamWaitEvent(b->ctx->stream, b->ev_done[l], 0));
+  }
+  return WTB_OK;
+}
+
+int wtb_bse_batch_results(wtb_bse_batch* b, int i, double* w, double* act, double* eps, double* egap) {
+  if (!b || i < 0 || i >= (int)b->plans.size()) {
+    set_error("wtb_bse_batch_results: bad index");
+    return WTB_ERR_ARG;
+  }
+  return wtb_bse_plan_results(b->plans[i], w, act, eps, egap);
+}
+
+int wtb_bse_batch_destroy(wtb_bse_batch* b) {
+  if (!b) return WTB_OK;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  for (auto* p : b->plans) wtb_bse_plan_destroy(p);
+  for (auto l : b->lanes) {
+    hipStreamSynchronize(l);
+    hipStreamDestroy(l);
+  }
+  for (auto e : b->ev_done) hipEventDestroy(e);
+  if (b->ev_start) hipEventDestroy(b->ev_start);
+  delete b;
+  return WTB_OK;
+}
+
+int wtb_bse_plan_set_vectors(wtb_bse_plan* p, int first, int count) {
+  if (!p || first < 0 || count < 0 || first + count > p->N) {
+    set_error("wtb_bse_plan_set_vectors: need 0 <= first, first + count <= N");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  WTB_HIP(hipStreamSynchronize(p->stream));
+  const int N = p->N;
+  int rc = WTB_OK;
+  if (count > 0 && !p->Zv) {
+    // the divide and conquer needs its eigenvector workspace, the reduction keeps its reflectors
+    hipFree(p->dcwork);
+    p->dcwork = nullptr;
+    if ((rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 6, 1))) || (rc = dmalloc(&p->Zv, (size_t)N * N)))
+      return rc;
+    if ((rc = dc_prepare(p->dcwork, N, 6, p->tree, 1))) return rc;
+    p->tw.store_v = 1;
+  }
+  if (count != p->wf_count) {
+    hipFree(p->Xv);
+    hipFree(p->btw);
+    p->Xv = nullptr;
+    p->btw = nullptr;
+    if (count > 0 &&
+        ((rc = dmalloc(&p->Xv, (size_t)N * count)) || (rc = dmalloc((char**)&p->btw, backtr_work_bytes(N, count)))))
+      return rc;
+  }
+  p->wf_first = first;
+  p->wf_count = count;
+  plan_drop_graph(p);
+  return WTB_OK;
+}
+
+int wtb_bsesolver_wf(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, double* w, double* act,
+                     double* eps, double* egap, int wf_first, int wf_count, double* wf, double* kpt, int* nocp) {
+  wtb_bse_plan* p = nullptr;
+  int rc = wtb_bse_plan_create(c, m, prm, &p);
+  if (rc) return rc;
+  rc = wtb_bse_plan_set_vectors(p, wf_first, wf_count);
+  if (!rc) rc = wtb_bse_plan_execute(p);
+  if (!rc) rc = wtb_bse_plan_results(p, w, act, eps, egap);
+  if (!rc && wf && wf_count > 0)
+    rc = wtb_bse_plan_fetch(p, WTB_BUF_EXCWF, wf, sizeof(zc) * (size_t)p->N * wf_count);
+  if (!rc && kpt) rc = wtb_bse_plan_fetch(p, WTB_BUF_KPT, kpt, sizeof(double) * 3 * p->nk);
+  if (!rc && nocp) rc = wtb_bse_plan_fetch(p, WTB_BUF_NOCP, nocp, sizeof(int) * p->nk);
+  wtb_bse_plan_destroy(p);
   return rc;
 }
 

@@ -67,6 +67,8 @@ struct TridiagWork {
   int kev_used = 0;
   // keep the reflectors (v_j below row j+1 in column j of H) for the back-transform
   int store_v = 0;
+  // 64 x 64 tiles per GEMV workgroup (strip height): 1, 2, 4 or 8
+  int sub = 4;
 };
 size_t tridiag_work_bytes(int N, int nb, int nrhs);
 int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs);

@@ -34,8 +34,11 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	integer, allocatable, target :: t_rvec(:,:), t_ff(:)
 	double precision, target :: t_rlat(3,3)
 	double precision, allocatable :: w(:), act(:,:)
+	double complex, allocatable :: wf(:,:)
+	double precision, allocatable :: kpt(:,:)
+	integer, allocatable :: nocpk(:), stt(:,:)
 	double precision :: egap
-	integer :: dimbse, i, erro
+	integer :: dimbse, i, erro, nwf, ngkpt
 	integer(c_int) :: rc
 
 	call hamiltonian_input_read(200, params)
@@ -83,7 +86,17 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 
 	rc = wtb_create(0_c_int, ctx)
 	call wtb_check(rc)
-	rc = wtb_bsesolver(ctx, m, p, w, act, c_null_ptr, egap)
+	if (bsewf) then
+		! BSE_WF: the vectors of excitons excwf0 .. excwff and what excwf2 prints beside them
+		if (excwf0 < 1 .or. excwff > dimbse .or. excwff < excwf0) stop "Wrong BSE_WF exciton range"
+		nwf = excwff - excwf0 + 1
+		ngkpt = ngrid(1)*ngrid(2)*ngrid(3)
+		allocate(wf(dimbse,nwf), kpt(ngkpt,3), nocpk(ngkpt), stt(dimbse,4))
+		rc = wtb_bsesolver_wf(ctx, m, p, w, act, c_null_ptr, egap, int(excwf0-1,c_int), int(nwf,c_int), &
+				      wf, kpt, nocpk)
+	else
+		rc = wtb_bsesolver(ctx, m, p, w, act, c_null_ptr, egap)
+	end if
 	call wtb_check(rc)
 	rc = wtb_destroy(ctx)
 
@@ -113,6 +126,13 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	close(301)
 	close(302)
 	close(200)
+	if (bsewf) then
+		call quantumnumbers2(w90basis,ngkpt,nc,nv,nocpk,nocpk,stt)
+		do i=excwf0,excwff
+			call excwf2(outputfolder,ngkpt,kpt,nc,nv,nocpk,stt,w(i),i,wf(:,i-excwf0+1))
+		end do
+		deallocate(wf, kpt, nocpk, stt)
+	end if
 	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
 end subroutine bsesolver
 

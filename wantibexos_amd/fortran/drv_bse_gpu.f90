@@ -8,6 +8,8 @@
 INCLUDE "subroutines/general_subs.f90"
 INCLUDE "subroutines/mhkpack_subs.f90"
 INCLUDE "subroutines/module_input_read.f90"
+INCLUDE "subroutines/coulomb_pot.f90"
+INCLUDE "subroutines/bse_subs.f90"
 INCLUDE "wtb_gpu.f90"
 INCLUDE "bse_solver_gpu.f90"
 

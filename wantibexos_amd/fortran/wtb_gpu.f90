@@ -58,6 +58,20 @@ module wtb_gpu
 			type(c_ptr), value :: eps
 			real(c_double) :: egap
 		end function
+		integer(c_int) function wtb_bsesolver_wf(ctx, model, prm, w, act, eps, egap, wf_first, &
+				wf_count, wf, kpt, nocp) bind(C, name="wtb_bsesolver_wf")
+			import :: c_int, c_ptr, c_double, c_double_complex, wtb_tb_model, wtb_bse_params
+			type(c_ptr), value :: ctx
+			type(wtb_tb_model) :: model
+			type(wtb_bse_params) :: prm
+			real(c_double) :: w(*), act(*)
+			type(c_ptr), value :: eps
+			real(c_double) :: egap
+			integer(c_int), value :: wf_first, wf_count
+			complex(c_double_complex) :: wf(*)
+			real(c_double) :: kpt(*)
+			integer(c_int) :: nocp(*)
+		end function
 		integer(c_int) function wtb_bse_spectrum(ctx, n, e, act6, rlat, ngrid, ebse0, ebsef, &
 				numbse, sme, renorm, eps) bind(C, name="wtb_bse_spectrum")
 			import :: c_int, c_ptr, c_double

@@ -152,6 +152,8 @@ def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -
         f"NESPEC= {p['numbse']}", f"RNMD= {T(p['renorm'])}",
         f"SHIFT_1= {p['shift'][0]!r}", f"SHIFT_2= {p['shift'][1]!r}", f"SHIFT_3= {p['shift'][2]!r}",
     ]
+    if p.get("bsewf"):
+        lines += ["BSE_WF= T", f"EXC_WF_I= {p['excwf'][0]}", f"EXC_WF_F= {p['excwf'][1]}"]
     return "\n".join(lines) + "\n"
 
 

@@ -177,7 +177,7 @@ def main():
                     help="full: the whole eigensolve (plain FP64 or precision form); davidson: the "
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")
     ap.add_argument("--nev", type=int, default=256)
-    ap.add_argument("--lanczos-steps", type=int, default=500)
+    ap.add_argument("--lanczos-steps", type=int, default=250)
     ap.add_argument("--learned", action="store_true", help="iterative form with the trained preconditioner")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)

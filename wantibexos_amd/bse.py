@@ -386,13 +386,13 @@ class BSEPlan:
             return raw.view(np.complex128).reshape(-1, N).T            # (N, count)
         return raw
 
-    def lowest(self, nev=256, guard=32, max_sub=None, max_iter=60, tol=1e-6, lanczos_steps=600,
+    def lowest(self, nev=256, guard=32, max_sub=None, max_iter=60, tol=1e-6, lanczos_steps=250,
                fallback=True, mlp=None, vectors=False):
         """Iterative form (precision-form plan): the lowest `nev` excitons by block
         Davidson and eps(omega) by Lanczos chains.  Returns dict W, resid, eps (or
         None), X (N, nev) (or None), info."""
         N = self.N
-        dp = _DavidsonParams(nev, guard, max_sub or 3 * (nev + guard), max_iter, tol, lanczos_steps,
+        dp = _DavidsonParams(nev, guard, max_sub or 5 * (nev + guard), max_iter, tol, lanczos_steps,
                              int(fallback))
         W = np.zeros(nev)
         res = np.zeros(nev)
@@ -410,16 +410,16 @@ class BSEPlan:
         _check(lib().wtb_bse_plan_set_vectors(self.h, int(first), int(count)))
 
     def set_tile(self, sub):
-        """GEMV strip height of the reduction: 1, 2, 4 or 8 tiles of 64 x 64."""
+        """GEMV shape of the reduction: 0 single 64 x 64 tiles, or strips of 1, 2, 4, 8 tiles."""
         _check(lib().wtb_bse_plan_set_tile(self.h, int(sub)))
 
     def tune(self):
         """Time the reduction for each strip height on this matrix and keep the fastest.
         Returns (best, {height: ms})."""
         best = np.zeros(1, dtype=np.int32)
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         _check(lib().wtb_bse_plan_tune(self.h, _p(best), _p(ms)))
-        return int(best[0]), dict(zip((1, 2, 4, 8), ms))
+        return int(best[0]), dict(zip((0, 1, 2, 4, 8), ms))
 
     def row_stats(self):
         """Diagonal D (N,) and row statistics (3, N) of the matrix (precision-form plan)."""

@@ -10,8 +10,9 @@
 //         stop when the first nev residual norms are below tol
 //         T = P(R_unconverged)  (diagonal or learned preconditioner; the learned one
 //         only in the FP32 phase)
-//         T -= V (V^H T) twice, T <- T L^-H with T^H T = L L^H (columns of tiny
-//         pivots dropped), V += T, HV += H T, G grows by V^H (H T)
+//         T -= V (V^H T) twice, then twice T <- T L^-H with T^H T = L L^H (columns
+//         of tiny pivots dropped) and T -= V (V^H T) between the two, V += T,
+//         HV += H T, G grows by V^H (H T)
 //         restart with the k = nev + guard Ritz vectors when m would pass max_sub
 // The products run on FP32 MFMA until the residuals reach switch_tol, then the basis
 // restarts on the Ritz vectors with H U recomputed in FP64 arithmetic (same COMPLEX*8
@@ -19,6 +20,8 @@
 // refers to the stored matrix and not to the FP32 rounding of the products.
 #include "wtb_kernels.h"
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <numeric>
 #include <vector>
 
@@ -118,12 +121,13 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
      *T = nullptr, *T2 = nullptr, *P = nullptr, *Gb = nullptr, *Linv = nullptr;
   double *theta = nullptr, *rn = nullptr;
   int *idx = nullptr, *keep = nullptr, *map = nullptr;
+  zc* hw = nullptr;
   SmallEig se;
   int rc = WTB_OK;
   auto cleanup = [&]() {
     for (void* p : {(void*)V, (void*)HV, (void*)G, (void*)S, (void*)U, (void*)HU, (void*)R, (void*)T,
                     (void*)T2, (void*)P, (void*)Gb, (void*)Linv, (void*)theta, (void*)rn, (void*)idx,
-                    (void*)keep, (void*)map})
+                    (void*)keep, (void*)map, (void*)hw})
       hipFree(p);
     small_free(se);
   };
@@ -150,6 +154,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   DAV_ALLOC(idx, sizeof(int) * nk);
   DAV_ALLOC(keep, sizeof(int) * nk);
   DAV_ALLOC(map, sizeof(int) * nk);
+  DAV_ALLOC(hw, hmm_work_bytes(N, nk));
 #undef DAV_ALLOC
   if ((rc = small_alloc(se, msub, nk))) {
     cleanup();
@@ -162,7 +167,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   }
   const zc one = mkz(1, 0), zero = mkz(0, 0), mone = mkz(-1, 0);
   // start: unit vectors at the nk smallest diagonal entries
-  double Dmin = 0.0;
+  double Dmin = 0.0, fp32_floor = 0.0;
   {
     std::vector<double> hD(N);
     DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);   // D comes from stream s
@@ -171,20 +176,27 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hD[a] < hD[b]; });
     Dmin = hD[order[0]];
+    double dmax = 0.0;
+    for (double v : hD) dmax = std::max(dmax, std::fabs(v));
+    // residual level the FP32 products reach: their rounding grows like sqrt(N) |H|
+    fp32_floor = 4e-7 * std::sqrt((double)N) * dmax;
     order.resize(nk);
     DAV_TRY(hipMemcpy(idx, order.data(), sizeof(int) * nk, hipMemcpyHostToDevice) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
     hipLaunchKernelGGL(k_unit_cols, dim3((N + 255) / 256, nk), dim3(256), 0, s, N, nk, idx, V, N);
   }
   bool hi = false;   // FP64 products
   auto matvec = [&](int nc, const zc* X, zc* Y) {
-    return hi ? hmm_f32_acc64(s, N, nc, H, ldh, X, N, Y, N) : hmm_f32(s, N, nc, H, ldh, X, N, Y, N);
+    return hi ? hmm_f32_acc64(s, N, nc, H, ldh, X, N, Y, N, hw) : hmm_f32(s, N, nc, H, ldh, X, N, Y, N, hw);
   };
   int m = nk;
   DAV_TRY(matvec(m, V, HV));
   info.matvecs += m;
   DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
   std::vector<double> h_rn(nk);
-  double prev_mx = 1e300;
+  double best_mx = 1e300;
+  int since_best = 0;
+  const char* tr = getenv("WTB_DAVIDSON_TRACE");   // per-iteration residuals on stderr
+  const bool trace = tr && tr[0] == '1';
   std::vector<int> h_keep(nk), h_idx(nk), h_map(nk);   // live across the asynchronous copies
   for (int it = 0; it < cfg.max_iter; ++it) {
     info.iters = it + 1;
@@ -197,6 +209,11 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     double mx = 0.0;
     for (int c = 0; c < nev; ++c) mx = std::max(mx, h_rn[c]);
     info.max_resid = mx;
+    if (trace) {
+      int nun = 0;
+      for (int c = 0; c < nk; ++c) nun += h_rn[c] > cfg.tol;
+      fprintf(stderr, "davidson it %d %s m=%d max resid %.3e unconverged %d\n", it, hi ? "fp64" : "fp32", m, mx, nun);
+    }
     if (hi) ++info.iters_fp64;
     if (mx <= cfg.tol && hi) {
       info.converged = 1;
@@ -204,10 +221,16 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     }
     if (it + 1 == cfg.max_iter) break;
     // FP32 products stall at a residual level set by their rounding (grows like sqrt(N));
-    // switch when below switch_tol or when an iteration gains less than a factor 1/0.7
-    const bool stall = it > 0 && mx > 0.7 * prev_mx;
-    prev_mx = mx;
-    if (!hi && (mx <= cfg.switch_tol || mx <= cfg.tol || stall)) {
+    // switch below that level, or when three iterations in a row bring no new best
+    // residual (by a factor 0.7) once the residuals are below 1e-2
+    if (mx < 0.7 * best_mx) {
+      best_mx = mx;
+      since_best = 0;
+    } else {
+      ++since_best;
+    }
+    const bool stall = since_best >= 3;
+    if (!hi && (mx <= std::max(cfg.switch_tol, fp32_floor) || mx <= cfg.tol || (stall && mx < 1e-2))) {
       // switch to FP64 products: restart on the Ritz vectors with H U recomputed
       hi = true;
       DAV_TRY(hipMemcpyAsync(V, U, sizeof(zc) * (size_t)N * nk, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
@@ -216,6 +239,8 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
       info.matvecs_fp64 += nk;
       m = nk;
       DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
+      best_mx = 1e300;
+      since_best = 0;
       continue;
     }
     int nq = 0;
@@ -239,19 +264,30 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
       DAV_TRY(zgemm(s, 'C', 'N', m, nq, N, one, V, N, T, N, zero, P, msub));
       DAV_TRY(zgemm(s, 'N', 'N', N, nq, m, mone, V, N, P, msub, one, T, N));
     }
-    // T <- T L^-H, T^H T = L L^H
-    DAV_TRY(zgemm(s, 'C', 'N', nq, nq, N, one, T, N, T, N, zero, Gb, nk));
-    DAV_TRY(launch_chol_inv(s, nq, Gb, nk, Linv, keep, 1e-12));
-    DAV_TRY(zgemm(s, 'N', 'C', N, nq, nq, one, T, N, Linv, nq, zero, T2, N));
-    DAV_TRY(hipMemcpyAsync(h_keep.data(), keep, sizeof(int) * nq, hipMemcpyDeviceToHost, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
-    DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
-    int nn = 0;
-    for (int q = 0; q < nq; ++q)
-      if (h_keep[q]) h_map[nn++] = q;
+    // Cholesky QR twice (T <- T L^-H with T^H T = L L^H, columns of tiny pivots dropped),
+    // with a projection against V in between: one pass squares the condition number
+    int nn = nq;
+    for (int pass = 0; pass < 2 && nn > 0; ++pass) {
+      if (pass == 1) {
+        DAV_TRY(zgemm(s, 'C', 'N', m, nn, N, one, V, N, T, N, zero, P, msub));
+        DAV_TRY(zgemm(s, 'N', 'N', N, nn, m, mone, V, N, P, msub, one, T, N));
+      }
+      DAV_TRY(zgemm(s, 'C', 'N', nn, nn, N, one, T, N, T, N, zero, Gb, nk));
+      DAV_TRY(launch_chol_inv(s, nn, Gb, nk, Linv, keep, 1e-10));
+      DAV_TRY(zgemm(s, 'N', 'C', N, nn, nn, one, T, N, Linv, nn, zero, T2, N));
+      DAV_TRY(hipMemcpyAsync(h_keep.data(), keep, sizeof(int) * nn, hipMemcpyDeviceToHost, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      DAV_TRY(hipStreamSynchronize(s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      int nk2 = 0;
+      for (int q = 0; q < nn; ++q)
+        if (h_keep[q]) h_map[nk2++] = q;
+      nn = nk2;
+      if (nn == 0) break;
+      DAV_TRY(hipMemcpyAsync(map, h_map.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      hipLaunchKernelGGL(k_gather_cols, dim3((N + 255) / 256, nn), dim3(256), 0, s, N, nn, map, T2, N, T, N);
+    }
     if (nn == 0) break;   // no new direction: stagnation
-    DAV_TRY(hipMemcpyAsync(map, h_map.data(), sizeof(int) * nn, hipMemcpyHostToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
     zc* Vn = V + (size_t)N * m;
-    hipLaunchKernelGGL(k_gather_cols, dim3((N + 255) / 256, nn), dim3(256), 0, s, N, nn, map, T2, N, Vn, N);
+    DAV_TRY(hipMemcpyAsync(Vn, T, sizeof(zc) * (size_t)N * nn, hipMemcpyDeviceToDevice, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
     DAV_TRY(matvec(nn, Vn, HV + (size_t)N * m));
     info.matvecs += nn;
     if (hi) info.matvecs_fp64 += nn;

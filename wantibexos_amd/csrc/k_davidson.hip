@@ -447,7 +447,8 @@ int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, do
 
 size_t haydock_work_bytes(int N, int steps) {
   const size_t nblk = 128;
-  return sizeof(zc) * (size_t)N * 9 * 5 + sizeof(double) * (9 * nblk * 3 + 2 * 9 * (size_t)(steps + 1) + 64) +
+  return thin_work_bytes(N, 9) + 256 + sizeof(zc) * (size_t)N * 9 * 5 +
+         sizeof(double) * (9 * nblk * 3 + 2 * 9 * (size_t)(steps + 1) + 64) +
          sizeof(int) * 64 + 16 * 256;
 }
 
@@ -479,6 +480,7 @@ int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int d
   double* sg = (double*)take(sizeof(double) * 16);
   int* ca = (int*)take(sizeof(int) * 16);
   int* cb = (int*)take(sizeof(int) * 16);
+  zc* hw = (zc*)take(thin_work_bytes(N, nq));
   // chains: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z
   const int hca[9] = {0, 1, 2, 0, 0, 0, 0, 1, 1};
   const int hcb[9] = {-1, -1, -1, 1, 1, 2, 2, 2, 2};
@@ -492,7 +494,7 @@ int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int d
   const int kmax = steps + 1;
   int ip = 0, ic = 1, in = 2;
   for (int j = 0; j < steps; ++j) {
-    int rc = hmm_f32(s, N, nq, H, ldh, Vb[ic], N, HV, N);
+    int rc = hmm_thin_f32(s, N, nq, H, ldh, Vb[ic], N, HV, N, hw);
     if (rc) return rc;
     hipLaunchKernelGGL(k_lz_a, dim3(nblk, nq), dim3(256), 0, s, N, Vb[ic], Vb[ip], HV, bcur, W, part);
     hipLaunchKernelGGL(k_lz_b, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, Vb[ic], W, part, nblk, acoef,

@@ -13,10 +13,12 @@ namespace wtb {
 #define GT 64   // output tile edge
 #define GK 16   // K slice
 
+// Split K (gridDim.z > 1): slice z covers k in [z kc, (z+1) kc) and writes its raw sums to
+// part + z M N (ld M); k_zgemm_reduce then forms alpha sum_z part + beta C in slice order.
 template <int OPA, int OPB, class TA, class TB, bool F32>
 __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zc alpha, const TA* __restrict__ A,
                                                int lda, const TB* __restrict__ B, int ldb, zc beta,
-                                               zc* __restrict__ C, int ldc) {
+                                               zc* __restrict__ C, int ldc, int kc, zc* __restrict__ part) {
   __shared__ zc As[GK][GT + 1];   // (k, i)
   __shared__ zc Bs[GK][GT + 1];   // (k, j)
   const int i0 = blockIdx.x * GT, j0 = blockIdx.y * GT;
@@ -31,19 +33,21 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zc alpha, co
       cr[x][y] = acc_t{0, 0, 0, 0};
       ci[x][y] = acc_t{0, 0, 0, 0};
     }
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  const int kbeg = blockIdx.z * kc;
+  const int kend = min(K, kbeg + kc);
+  for (int k0 = kbeg; k0 < kend; k0 += GK) {
     // stage op(A)(i0:i0+64, k0:k0+16)
     if (OPA == 0) {
       const int i = tid & 63;
       for (int kk = tid >> 6; kk < GK; kk += 4) {
         const int gi = i0 + i, gk = k0 + kk;
-        As[kk][i] = (gi < M && gk < K) ? ldz(A[(size_t)gi + (size_t)lda * gk]) : mkz(0, 0);
+        As[kk][i] = (gi < M && gk < kend) ? ldz(A[(size_t)gi + (size_t)lda * gk]) : mkz(0, 0);
       }
     } else {
       const int kk = tid & 15;
       for (int i = tid >> 4; i < GT; i += 16) {
         const int gi = i0 + i, gk = k0 + kk;
-        As[kk][i] = (gi < M && gk < K) ? conj(ldz(A[(size_t)gk + (size_t)lda * gi])) : mkz(0, 0);
+        As[kk][i] = (gi < M && gk < kend) ? conj(ldz(A[(size_t)gk + (size_t)lda * gi])) : mkz(0, 0);
       }
     }
     // stage op(B)(k0:k0+16, j0:j0+64)
@@ -51,13 +55,13 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zc alpha, co
       const int kk = tid & 15;
       for (int j = tid >> 4; j < GT; j += 16) {
         const int gj = j0 + j, gk = k0 + kk;
-        Bs[kk][j] = (gj < N && gk < K) ? ldz(B[(size_t)gk + (size_t)ldb * gj]) : mkz(0, 0);
+        Bs[kk][j] = (gj < N && gk < kend) ? ldz(B[(size_t)gk + (size_t)ldb * gj]) : mkz(0, 0);
       }
     } else {
       const int j = tid & 63;
       for (int kk = tid >> 6; kk < GK; kk += 4) {
         const int gj = j0 + j, gk = k0 + kk;
-        Bs[kk][j] = (gj < N && gk < K) ? conj(ldz(B[(size_t)gj + (size_t)ldb * gk])) : mkz(0, 0);
+        Bs[kk][j] = (gj < N && gk < kend) ? conj(ldz(B[(size_t)gj + (size_t)ldb * gk])) : mkz(0, 0);
       }
     }
     __syncthreads();
@@ -93,12 +97,29 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zc alpha, co
         const int i = i0 + wi + x * 16 + (F32 ? 4 * lk + q : lk + 4 * q);
         if (i >= M) continue;
         const zc v = mkz((double)cr[x][y][q], (double)ci[x][y][q]);
+        if (part) {
+          part[(size_t)blockIdx.z * M * N + (size_t)i + (size_t)M * j] = v;
+          continue;
+        }
         zc& c = C[(size_t)i + (size_t)ldc * j];
         zc out = alpha * v;
         if (use_beta) out += beta * c;
         c = out;
       }
     }
+}
+
+__global__ void k_zgemm_reduce(int M, int N, int nz, const zc* __restrict__ part, zc alpha, zc beta,
+                               zc* __restrict__ C, int ldc) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)M * N) return;
+  const int i = (int)(t % M), j = (int)(t / M);
+  zc acc = mkz(0, 0);
+  for (int z = 0; z < nz; ++z) acc += part[(size_t)z * M * N + t];
+  zc& c = C[(size_t)i + (size_t)ldc * j];
+  zc out = alpha * acc;
+  if (beta.re != 0.0 || beta.im != 0.0) out += beta * c;
+  c = out;
 }
 
 int zgemm(hipStream_t s, char opa, char opb, int M, int N, int K, zc alpha, const zc* A, int lda,
@@ -110,39 +131,145 @@ int zgemm(hipStream_t s, char opa, char opb, int M, int N, int K, zc alpha, cons
   }
   dim3 g((M + GT - 1) / GT, (N + GT - 1) / GT), b(256);
   const bool ca = opa == 'C', cb = opb == 'C';
-  if (!ca && !cb) hipLaunchKernelGGL((k_zgemm<0, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-  else if (ca && !cb) hipLaunchKernelGGL((k_zgemm<1, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-  else if (!ca && cb) hipLaunchKernelGGL((k_zgemm<0, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
-  else hipLaunchKernelGGL((k_zgemm<1, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  zc* np_ = nullptr;
+  if (!ca && !cb) hipLaunchKernelGGL((k_zgemm<0, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
+  else if (ca && !cb) hipLaunchKernelGGL((k_zgemm<1, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
+  else if (!ca && cb) hipLaunchKernelGGL((k_zgemm<0, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
+  else hipLaunchKernelGGL((k_zgemm<1, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }
 
-int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy) {
-  if (N <= 0 || nc <= 0) return WTB_OK;
-  dim3 g((N + GT - 1) / GT, (nc + GT - 1) / GT), b(256);
-  hipLaunchKernelGGL((k_zgemm<0, 0, zf, zc, true>), g, b, 0, s, N, nc, N, mkz(1, 0), H, ldh, X, ldx,
-                     mkz(0, 0), Y, ldy);
+// Thin product Y = H X (nc <= 16 columns) for the Lanczos chains: lane = row of a 64-row
+// block, the 4 waves of a workgroup split its K slice, X of the slice staged in LDS; every
+// load of H moves one column segment (512 B of COMPLEX*8 per wave).  FP32 accumulation for
+// the COMPLEX*8 matrix.  Partial sums of the K slices go to part[z] (ld N), then reduced.
+#define THIN_KC 256
+__global__ __launch_bounds__(256) void k_thin_hmm(int N, int nc, const zf* __restrict__ H, int ldh,
+                                                  const zc* __restrict__ X, int ldx, zc* __restrict__ part) {
+  __shared__ float2 sm[THIN_KC * 16];            // X slice [k][16], then the 4 wave partials [w][64][16]
+  const int r0 = blockIdx.x * 64, k0 = blockIdx.y * THIN_KC;
+  const int kn = min(THIN_KC, N - k0);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int e = threadIdx.x; e < THIN_KC * 16; e += 256) {
+    const int kk = e >> 4, q = e & 15;
+    float2 v = make_float2(0.f, 0.f);
+    if (q < nc && kk < kn) {
+      const zc x = X[(size_t)(k0 + kk) + (size_t)ldx * q];
+      v = make_float2((float)x.re, (float)x.im);
+    }
+    sm[e] = v;
+  }
+  __syncthreads();
+  float ar[16], ai[16];
+  for (int q = 0; q < 16; ++q) ar[q] = ai[q] = 0.f;
+  const int r = r0 + lane;
+  const bool rv = r < N;
+  // wave w takes k = w, w+4, ...; 8 loads in flight per lane
+  for (int kb = w; kb < THIN_KC; kb += 32) {
+    zf h[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kk = kb + 4 * u;
+      h[u] = (rv && kk < kn) ? H[(size_t)r + (size_t)ldh * (k0 + kk)] : zf{0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kk = kb + 4 * u;
+      if (kk >= THIN_KC) break;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        if (q < nc) {
+          const float2 x = sm[kk * 16 + q];
+          ar[q] = fmaf(h[u].re, x.x, fmaf(-h[u].im, x.y, ar[q]));
+          ai[q] = fmaf(h[u].re, x.y, fmaf(h[u].im, x.x, ai[q]));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = 0; q < 16; ++q) sm[(w * 64 + lane) * 16 + q] = make_float2(ar[q], ai[q]);
+  __syncthreads();
+  // wave 0 adds the four K quarters in order and writes the slice's partial
+  if (w == 0 && rv) {
+    for (int q = 0; q < nc; ++q) {
+      float sr = 0.f, si = 0.f;
+      for (int ww = 0; ww < 4; ++ww) {
+        const float2 v = sm[(ww * 64 + lane) * 16 + q];
+        sr += v.x;
+        si += v.y;
+      }
+      part[(size_t)blockIdx.y * N * nc + (size_t)r + (size_t)N * q] = mkz((double)sr, (double)si);
+    }
+  }
+}
+
+size_t thin_work_bytes(int N, int nc) {
+  return sizeof(zc) * (size_t)((N + THIN_KC - 1) / THIN_KC) * N * nc + 256;
+}
+
+int hmm_thin_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+                 zc* work) {
+  if (nc < 1 || nc > 16) {
+    set_error("thin product: 1..16 columns");
+    return WTB_ERR_ARG;
+  }
+  const int nz = (N + THIN_KC - 1) / THIN_KC;
+  hipLaunchKernelGGL(k_thin_hmm, dim3((N + 63) / 64, nz), dim3(256), 0, s, N, nc, H, ldh, X, ldx, work);
+  const size_t tot = (size_t)N * nc;
+  hipLaunchKernelGGL(k_zgemm_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, nc, nz, work,
+                     mkz(1, 0), mkz(0, 0), Y, ldy);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }
 
-int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy) {
+// Y = H X with enough workgroups to fill the chip: when the output tiles alone would
+// give fewer than ~1024 workgroups, K is split into slices (partial sums in work, of
+// hmm_work_bytes) reduced in a fixed order.
+static int split_of(int N, int nc) {
+  const int tiles = ((N + GT - 1) / GT) * ((nc + GT - 1) / GT);
+  int ns = (1024 + tiles - 1) / tiles;
+  const int maxs = N / 256 > 1 ? N / 256 : 1;
+  return ns < 1 ? 1 : (ns > maxs ? maxs : (ns > 32 ? 32 : ns));
+}
+
+// enough for every column count up to nc
+size_t hmm_work_bytes(int N, int nc) {
+  size_t mx = 0;
+  for (int c = 1; c <= nc; ++c) {
+    const size_t b = (size_t)split_of(N, c) * N * c;
+    mx = b > mx ? b : mx;
+  }
+  return sizeof(zc) * mx + 256;
+}
+
+template <class TA, bool F32>
+static int hmm_t(hipStream_t s, int N, int nc, const TA* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+                 zc* work) {
   if (N <= 0 || nc <= 0) return WTB_OK;
-  dim3 g((N + GT - 1) / GT, (nc + GT - 1) / GT), b(256);
-  hipLaunchKernelGGL((k_zgemm<0, 0, zf, zc, false>), g, b, 0, s, N, nc, N, mkz(1, 0), H, ldh, X, ldx,
-                     mkz(0, 0), Y, ldy);
+  const int ns = work ? split_of(N, nc) : 1;
+  const int kc = (((N + ns - 1) / ns + GK - 1) / GK) * GK;
+  dim3 g((N + GT - 1) / GT, (nc + GT - 1) / GT, ns), b(256);
+  hipLaunchKernelGGL((k_zgemm<0, 0, TA, zc, F32>), g, b, 0, s, N, nc, N, mkz(1, 0), H, ldh, X, ldx, mkz(0, 0), Y,
+                     ldy, kc, ns > 1 ? work : nullptr);
+  if (ns > 1) {
+    const size_t tot = (size_t)N * nc;
+    hipLaunchKernelGGL(k_zgemm_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, nc, ns, work,
+                       mkz(1, 0), mkz(0, 0), Y, ldy);
+  }
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }
 
-int hmm_f64(hipStream_t s, int N, int nc, const zc* H, int ldh, const zc* X, int ldx, zc* Y, int ldy) {
-  if (N <= 0 || nc <= 0) return WTB_OK;
-  dim3 g((N + GT - 1) / GT, (nc + GT - 1) / GT), b(256);
-  hipLaunchKernelGGL((k_zgemm<0, 0, zc, zc, false>), g, b, 0, s, N, nc, N, mkz(1, 0), H, ldh, X, ldx,
-                     mkz(0, 0), Y, ldy);
-  WTB_LAUNCH_CHECK();
-  return WTB_OK;
+int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy, zc* work) {
+  return hmm_t<zf, true>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
+}
+int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+                  zc* work) {
+  return hmm_t<zf, false>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
+}
+int hmm_f64(hipStream_t s, int N, int nc, const zc* H, int ldh, const zc* X, int ldx, zc* Y, int ldy, zc* work) {
+  return hmm_t<zc, false>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
 }
 
 }  // namespace wtb

@@ -90,6 +90,9 @@ __device__ inline bool arrive_last(unsigned* cnt, unsigned total) {
 }
 
 // ---- rowstep ------------------------------------------------------------------------
+// Loads that do not depend on the scalars of hemvstep(j-1) (the GEMV partials, the
+// panel rows, column j) are issued before the published scalars are read, and the
+// panel values serve both u and the column update.
 template <class T>
 __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj, int update_col) {
   __shared__ zc st1[32], st2[32], svg[16], sWj[32], sVj[32];
@@ -101,6 +104,39 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   const int jm = j - 1, jjm = jj - 1;
   const int bm = (jm + 1) / TB;                   // first row block of reflector jm's GEMV
   const int r = j + blockIdx.x * RPW + row_l;
+  const bool valid = r < N;
+  auto ysum = [&](int row) -> zc {
+    zc acc = mkz(0, 0);
+    const zc* yp = a.ypart + ((size_t)(row / TB) * a.nblk) * TB + (row % TB);
+    for (int sl = bm + sub; sl < a.nblk; sl += 16) acc += yp[(size_t)sl * TB];
+    return acc;
+  };
+  // panel rows j and r, columns sub and sub + 16
+  zc yj = mkz(0, 0), yr = mkz(0, 0), hjj = mkz(0, 0), hcol = mkz(0, 0);
+  zc vj[2], wj[2], vr[2], wr[2];
+  for (int k = 0; k < 2; ++k) vj[k] = wj[k] = vr[k] = wr[k] = mkz(0, 0);
+  if (row_l == 0) {
+    yj = ysum(j);
+    hjj = ldz(a.H[(size_t)j + (size_t)a.ld * j]);
+    for (int k = 0; k < 2; ++k) {
+      const int c = sub + 16 * k;
+      if (c < jjm) {
+        vj[k] = a.Vp[(size_t)c * N + j];
+        wj[k] = a.W[(size_t)c * N + j];
+      }
+    }
+  }
+  if (valid) {
+    yr = ysum(r);
+    hcol = ldz(a.H[(size_t)r + (size_t)a.ld * j]);   // A22(r, j): column j before its update
+    for (int k = 0; k < 2; ++k) {
+      const int c = sub + 16 * k;
+      if (c < jjm) {
+        vr[k] = a.Vp[(size_t)c * N + r];
+        wr[k] = a.W[(size_t)c * N + r];
+      }
+    }
+  }
   const Pub* P = a.pub;
   if (tid < 32) {
     if (tid < jjm) {
@@ -116,14 +152,14 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   __syncthreads();
   const zc tau = s_tau, s = s_s;
   // s z(row) - V t1 - W t2, each row summed by its 16 lanes
-  auto u_part = [&](int row) -> zc {
-    zc acc = mkz(0, 0);
-    const zc* yp = a.ypart + ((size_t)(row / TB) * a.nblk) * TB + (row % TB);
-    for (int sl = bm + sub; sl < a.nblk; sl += 16) acc += yp[(size_t)sl * TB];
-    acc = s * acc;
-    for (int c = sub; c < jjm; c += 16) {
-      acc = acc - a.Vp[(size_t)c * N + row] * st1[c];
-      acc = acc - a.W[(size_t)c * N + row] * st2[c];
+  auto u_of = [&](zc ys, const zc* v2, const zc* w2) -> zc {
+    zc acc = s * ys;
+    for (int k = 0; k < 2; ++k) {
+      const int c = sub + 16 * k;
+      if (c < jjm) {
+        acc = acc - v2[k] * st1[c];
+        acc = acc - w2[k] * st2[c];
+      }
     }
     for (int off = 8; off > 0; off >>= 1) {
       acc.re += __shfl_xor(acc.re, off, 16);
@@ -131,21 +167,31 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
     }
     return acc;
   };
-  // u(j): every workgroup needs it for the column update
+  // u(j): every workgroup needs it for the column update; the same lanes publish
+  // row j of the panel (W(j, jj-1) = tau u(j), V(j, jj-1) = 1) for it
   if (row_l == 0) {
-    zc hjj = ldz(a.H[(size_t)j + (size_t)a.ld * j]);
-    zc up = u_part(j);
-    if (sub == 0) {
-      hjj.im = 0.0;                                // A22 diagonal: real part (ZHEMV)
-      su_j = hjj + up;
+    const zc up = u_of(yj, vj, wj);
+    zc h = hjj;
+    h.im = 0.0;                                    // A22 diagonal: real part (ZHEMV)
+    const zc uj = h + up;
+    if (sub == 0) su_j = uj;
+    if (update_col) {
+      for (int k = 0; k < 2; ++k) {
+        const int c = sub + 16 * k;
+        if (c < jjm) {
+          sWj[c] = wj[k];
+          sVj[c] = vj[k];
+        } else if (c == jjm) {
+          sWj[c] = tau * uj;
+          sVj[c] = mkz(1, 0);
+        }
+      }
     }
   }
   __syncthreads();
-  const bool valid = r < N;
-  zc uu = mkz(0, 0), v = mkz(0, 0), hcol = mkz(0, 0);
+  zc uu = mkz(0, 0), v = mkz(0, 0);
   if (valid) {
-    hcol = ldz(a.H[(size_t)r + (size_t)a.ld * j]);       // A22(r, j): column j before its update
-    zc up = u_part(r);
+    const zc up = u_of(yr, vr, wr);
     uu = (r == j) ? su_j : hcol + up;
     v = (r == j) ? mkz(1, 0) : s * a.xn[r];
     if (sub == 0) {
@@ -161,25 +207,21 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   }
   if (sub == 0) suv[row_l] = valid ? conj(uu) * v : mkz(0, 0);
   // column j with the panel (ZLATRD), W(:,jjm) taken as tau*u (alpha' part deferred)
-  if (update_col) {
-    if (tid < jj) {
-      sWj[tid] = (tid == jjm) ? tau * su_j : a.W[(size_t)tid * N + j];
-      sVj[tid] = (tid == jjm) ? mkz(1, 0) : a.Vp[(size_t)tid * N + j];
-    }
-    __syncthreads();
-    if (valid) {
-      zc acc = mkz(0, 0);
-      for (int c = sub; c < jj; c += 16) {
-        const zc vc = (c == jjm) ? v : a.Vp[(size_t)c * N + r];
-        const zc wc = (c == jjm) ? tau * uu : a.W[(size_t)c * N + r];
+  if (update_col && valid) {
+    zc acc = mkz(0, 0);
+    for (int k = 0; k < 2; ++k) {
+      const int c = sub + 16 * k;
+      if (c < jj) {
+        const zc vc = (c == jjm) ? v : vr[k];
+        const zc wc = (c == jjm) ? tau * uu : wr[k];
         acc = acc + vc * conj(sWj[c]) + wc * conj(sVj[c]);
       }
-      for (int off = 8; off > 0; off >>= 1) {
-        acc.re += __shfl_xor(acc.re, off, 16);
-        acc.im += __shfl_xor(acc.im, off, 16);
-      }
-      if (sub == 0) a.xp[r] = hcol - acc;
     }
+    for (int off = 8; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 16);
+      acc.im += __shfl_xor(acc.im, off, 16);
+    }
+    if (sub == 0) a.xp[r] = hcol - acc;
   }
   __syncthreads();
   if (tid == 0) {
@@ -200,7 +242,199 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   }
 }
 
-// ---- hemvstep -----------------------------------------------------------------------
+// ---- hemvstep, one 64 x 64 tile per workgroup (the default shape) ----------------------
+// 4 waves, wave w owns 16 columns, lane = row: every load moves 1 KB of one column.
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) {
+  __shared__ zc xR[TB], xC[TB];
+  __shared__ zc y1s[4][TB], y2s[1][TB];
+  __shared__ zc sums[96];
+  __shared__ zc s_sc;
+  const int N = a.N;
+  const int b0 = (j + 1) / TB;
+  int t = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  const int bi = b0 + ti, bj = b0 + tj;
+  const int tid = threadIdx.x;
+  const int R0 = bi * TB, C0 = bj * TB;
+  const bool diag = bi == bj;
+  const zc ap = (jj > 0) ? a.pub->alpha : mkz(0, 0);
+  const double sig = 2.0 * ap.re;                  // x_new = x_p - 2 Re(alpha') v_{j-1}
+  auto xnew = [&](int row) -> zc {
+    if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
+    return a.xp[row] - sig * a.Vp[(size_t)(jj - 1) * N + row];
+  };
+  if (tid < TB) {
+    const int row = R0 + tid;
+    xR[tid] = (row >= j + 2 && row < N) ? xnew(row) : mkz(0, 0);
+  } else if (tid < 2 * TB) {
+    const int col = C0 + tid - TB;
+    xC[tid - TB] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
+  }
+  // wave w takes columns C0+16w .. C0+16w+15, lane = row: every load moves 1 KB of a column
+  const int lane = tid & 63, w = tid >> 6;
+  const int r = R0 + lane;
+  const bool rvalid = r > j && r < N;
+  zc hv[16];
+  for (int k = 0; k < 16; ++k) {
+    const int cl = 16 * w + k, c = C0 + cl;
+    zc x = mkz(0, 0);
+    if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
+    hv[k] = x;
+  }
+  __syncthreads();
+  const zc xr = xR[lane];
+  zc y1 = mkz(0, 0);
+  for (int k = 0; k < 16; ++k) {
+    const int cl = 16 * w + k;
+    zc x = hv[k];
+    if (diag && lane == cl) x.im = 0.0;                   // diagonal: real part (ZHEMV)
+    y1 = fma_z(x, xC[cl], y1);
+    hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
+  }
+  // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
+  {
+    int n = 16;
+    for (int off = 32; off >= 4; off >>= 1) {
+      n >>= 1;
+      const bool up = (lane & off) != 0;
+      for (int i = 0; i < n; ++i) {
+        const zc send = up ? hv[i] : hv[i + n];
+        zc recv;
+        recv.re = __shfl_xor(send.re, off);
+        recv.im = __shfl_xor(send.im, off);
+        hv[i] = (up ? hv[i + n] : hv[i]) + recv;
+      }
+    }
+    for (int off = 2; off > 0; off >>= 1) {
+      hv[0].re += __shfl_xor(hv[0].re, off);
+      hv[0].im += __shfl_xor(hv[0].im, off);
+    }
+  }
+  // lane holds the sum for column 16w + ((lane >> 2) & 15)
+  y1s[w][lane] = y1;
+  if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
+  __syncthreads();
+  zc* yp = a.ypart;
+  if (tid < TB) {
+    zc sm = y1s[0][tid] + y1s[1][tid] + y1s[2][tid] + y1s[3][tid];
+    if (diag) sm += y2s[0][tid];
+    yp[((size_t)bi * a.nblk + bj) * TB + tid] = sm;
+  } else if (tid < 2 * TB && !diag) {
+    yp[((size_t)bj * a.nblk + bi) * TB + (tid - TB)] = y2s[0][tid - TB];
+  }
+  if (!diag) return;
+  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, partial dot products --------------------
+  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
+  if (tid < TB) {
+    const int r = R0 + tid;
+    if (r >= j + 1 && r < N) {
+      a.xn[r] = xnew(r);
+      if (jj > 0) a.W[(size_t)(jj - 1) * N + r] = taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r];
+    }
+  }
+  if (ti == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
+  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g; 4 lanes each
+  const int nq = 1 + 2 * jj + a.nrhs;
+  const int sub = tid & 3;
+  for (int q0 = 0; q0 < nq; q0 += 64) {
+    const int q = q0 + (tid >> 2);
+    zc acc = mkz(0, 0);
+    if (q < nq) {
+      for (int rl = sub; rl < TB; rl += 4) {
+        const int r = R0 + rl;
+        if (r < j + 2 || r >= N) continue;
+        const zc xr = xR[rl];
+        if (q == 0) {
+          acc.re += norm2(xr);
+        } else if (q <= jj) {
+          const int c = q - 1;
+          const zc w = (c == jj - 1) ? taup * a.u[r] + ap * a.Vp[(size_t)c * N + r] : a.W[(size_t)c * N + r];
+          acc = fma_conj(w, xr, acc);
+        } else if (q <= 2 * jj) {
+          acc = fma_conj(a.Vp[(size_t)(q - 1 - jj) * N + r], xr, acc);
+        } else {
+          acc = fma_z(xr, a.g[(size_t)(q - 1 - 2 * jj) * N + r], acc);
+        }
+      }
+    }
+    for (int off = 2; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 4);
+      acc.im += __shfl_xor(acc.im, off, 4);
+    }
+    if (sub == 0 && q < nq) {
+      if (q == 0) a.pn[bi] = acc.re;
+      else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = acc;
+      else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = acc;
+      else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = acc;
+    }
+  }
+  if (!arrive_last(a.cnt + 0, (unsigned)(a.nblk - b0))) return;
+  // ---- last diagonal tile: sums, ZLARFG, W^H v, V^H v, v^T g --------------------------
+  for (int q0 = 0; q0 < nq; q0 += 64) {
+    const int q = q0 + (tid >> 2);
+    zc acc = mkz(0, 0);
+    if (q < nq) {
+      for (int b = b0 + sub; b < a.nblk; b += 4) {
+        if (q == 0) acc.re += a.pn[b];
+        else if (q <= jj) acc += a.pt1[(size_t)b * a.nb + (q - 1)];
+        else if (q <= 2 * jj) acc += a.pt2[(size_t)b * a.nb + (q - 1 - jj)];
+        else acc += a.pvg[(size_t)b * a.nrhs + (q - 1 - 2 * jj)];
+      }
+    }
+    for (int off = 2; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 4);
+      acc.im += __shfl_xor(acc.im, off, 4);
+    }
+    if (sub == 0 && q < nq) sums[q] = acc;
+  }
+  __syncthreads();
+  Pub* P = a.pub;
+  if (tid == 0) {
+    // ZLARFG on alpha = x_new(j+1), |x|^2 = sum |x~|^2
+    const double xn2 = sums[0].re;
+    const zc alpha = a.xn[j + 1];
+    zc tau, s;
+    double beta;
+    if (xn2 == 0.0 && alpha.im == 0.0) {
+      tau = mkz(0, 0);
+      s = mkz(0, 0);
+      beta = alpha.re;
+    } else {
+      beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2), alpha.re);
+      tau = mkz((beta - alpha.re) / beta, -alpha.im / beta);
+      const zc den = mkz(alpha.re - beta, alpha.im);
+      const double dd = den.re * den.re + den.im * den.im;
+      s = mkz(den.re / dd, -den.im / dd);
+    }
+    P->tau = tau;
+    P->s = s;
+    P->beta = mkz(beta, 0);
+    a.tau[j] = tau;
+    a.e[j] = beta;
+    s_sc = s;
+    a.cnt[0] = 0u;
+  }
+  __syncthreads();
+  const zc s = s_sc;
+  // v = e_{j+1} + s x~:  W^H v = conj(W(j+1,:)) + s W^H x~, likewise V^H v, v^T g
+  if (tid < jj) {
+    const int c = tid;
+    const zc wj = (c == jj - 1) ? taup * a.u[j + 1] + ap * a.Vp[(size_t)c * N + (j + 1)]
+                                : a.W[(size_t)c * N + (j + 1)];
+    P->t1[c] = conj(wj) + s * sums[1 + c];
+    P->t2[c] = conj(a.Vp[(size_t)c * N + (j + 1)]) + s * sums[1 + jj + c];
+  }
+  if (tid >= 64 && tid < 64 + a.nrhs) {
+    const int q = tid - 64;
+    P->vg[q] = a.g[(size_t)q * N + (j + 1)] + s * sums[1 + 2 * jj + q];
+  }
+}
+
+// ---- hemvstep, strips (tunable shape) -----------------------------------------------
 // One workgroup per strip of up to SUB lower 64x64 tiles stacked in one column block
 // (column block bj, row blocks bi0 .. bi0+SUB-1, bi0 >= bj): 8 waves, wave w owns 8
 // columns, lane = row of the current tile, so every load moves 1 KB of one column and a
@@ -651,6 +885,7 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
       const bool tev = tw.kev && tw.kev_used < tw.kev_cap;
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used], s);
       switch (tw.sub) {
+        case 0: hipLaunchKernelGGL(k_tri_hemv1<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj); break;
         case 1: hipLaunchKernelGGL((k_tri_hemv<T, 1>), dim3(nt, nt), dim3(512), 0, s, a, j, jj); break;
         case 2: hipLaunchKernelGGL((k_tri_hemv<T, 2>), dim3((nt + 1) / 2, nt), dim3(512), 0, s, a, j, jj); break;
         case 8: hipLaunchKernelGGL((k_tri_hemv<T, 8>), dim3((nt + 7) / 8, nt), dim3(512), 0, s, a, j, jj); break;

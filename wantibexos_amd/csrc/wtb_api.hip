@@ -1147,8 +1147,8 @@ static void plan_drop_graph(wtb_bse_plan* p) {
 }
 
 int wtb_bse_plan_set_tile(wtb_bse_plan* p, int sub) {
-  if (!p || !(sub == 1 || sub == 2 || sub == 4 || sub == 8)) {
-    set_error("wtb_bse_plan_set_tile: strip height must be 1, 2, 4 or 8 tiles");
+  if (!p || !(sub == 0 || sub == 1 || sub == 2 || sub == 4 || sub == 8)) {
+    set_error("wtb_bse_plan_set_tile: shape must be 0 (single tiles) or a strip of 1, 2, 4 or 8 tiles");
     return WTB_ERR_ARG;
   }
   if (p->tw.sub != sub) plan_drop_graph(p);
@@ -1162,14 +1162,14 @@ int wtb_bse_plan_tune(wtb_bse_plan* p, int* best, double* ms) {
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
-  const int cand[4] = {1, 2, 4, 8};
+  const int cand[5] = {0, 1, 2, 4, 8};
   const bool timing0 = p->timing;
   const int sub0 = p->tw.sub;
   double tbest = 1e300;
   int sbest = sub0;
   p->timing = true;
   int rc = WTB_OK;
-  for (int k = 0; k < 4 && !rc; ++k) {
+  for (int k = 0; k < 5 && !rc; ++k) {
     p->tw.sub = cand[k];
     double t = 1e300;
     for (int rep = 0; rep < 2 && !rc; ++rep) {   // the first run warms up
@@ -1260,7 +1260,7 @@ int wtb_bse_batch_dim(const wtb_bse_batch* b, int i) {
 int wtb_bse_batch_tune(wtb_bse_batch* b, int* tiles) {
   if (!b) return WTB_ERR_ARG;
   for (size_t i = 0; i < b->plans.size(); ++i) {
-    int best = 4;
+    int best = 0;
     int rc = wtb_bse_plan_tune(b->plans[i], &best, nullptr);
     if (rc) return rc;
     if (tiles) tiles[i] = best;

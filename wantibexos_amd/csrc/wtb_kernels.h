@@ -67,8 +67,9 @@ struct TridiagWork {
   int kev_used = 0;
   // keep the reflectors (v_j below row j+1 in column j of H) for the back-transform
   int store_v = 0;
-  // 64 x 64 tiles per GEMV workgroup (strip height): 1, 2, 4 or 8
-  int sub = 4;
+  // GEMV shape: 0 = one 64 x 64 tile per 256-thread workgroup (16 columns per wave);
+  // 1, 2, 4, 8 = strips of that many tiles per 512-thread workgroup (8 columns per wave)
+  int sub = 0;
 };
 size_t tridiag_work_bytes(int N, int nb, int nrhs);
 int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs);
@@ -100,11 +101,20 @@ int run_backtr_f32(hipStream_t s, const zf* H, int ld, int N, const zc* tau, con
 // complex GEMM C = alpha op(A) op(B) + beta C, op 'N' or 'C' (FP64 MFMA)
 int zgemm(hipStream_t s, char opa, char opb, int M, int N, int K, zc alpha, const zc* A, int lda,
           const zc* B, int ldb, zc beta, zc* C, int ldc);
-// Y = H X for a full Hermitian H: COMPLEX*8 operand on FP32 MFMA, or FP64
-int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy);
-int hmm_f64(hipStream_t s, int N, int nc, const zc* H, int ldh, const zc* X, int ldx, zc* Y, int ldy);
+// Y = H X for a full Hermitian H: COMPLEX*8 operand on FP32 MFMA, or FP64; work (optional,
+// hmm_work_bytes) lets a thin product split K over more workgroups
+size_t hmm_work_bytes(int N, int nc);
+int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+            zc* work = nullptr);
+int hmm_f64(hipStream_t s, int N, int nc, const zc* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+            zc* work = nullptr);
+// thin product (nc <= 16) for the Lanczos chains; work of thin_work_bytes
+size_t thin_work_bytes(int N, int nc);
+int hmm_thin_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+                 zc* work);
 // COMPLEX*8 operand, FP64 MFMA arithmetic (the final iterations of the iterative solver)
-int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy);
+int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+                  zc* work = nullptr);
 
 // stage (e)
 int launch_activities(hipStream_t s, int N, const double* d_G /* 6 x N */, int dtfull, int cpol,

@@ -7,9 +7,15 @@ Monkhorst-Pack grid; nv = nc = 2 -> BSE dimension 5184; Rytova-Keldysh
 potential r0 = 40 A, eps = 1; eps(omega) on 0-6 eV, 1001 points, CSHIFT 0.05.
 The timed region runs, with the model already resident in HBM: k grid, H(k)
 phase GEMM, per-k eigensolves, dipoles, W table, BSE assembly, Hermitian
-eigensolve, activities and eps(omega).
+eigensolve, activities and eps(omega).  By default the value is the precision
+form (COMPLEX*8 matrix, FP32 MFMA assembly and trailing updates, FP64 norms,
+tridiagonal solve and spectrum) and the plain FP64 form is timed beside it
+with the same steps, together with the precision form's error against it.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
+    python bench.py --form davidson --config c5   # the iterative form (lowest 256
+                                                  # excitons + Lanczos eps) vs full FP64
+
+    python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu]
     python bench.py --copy-probe      # one 1 GiB device copy, to scale HBM counters
     python bench.py --no-graph ...    # direct launches (for kernel tracers)
 
@@ -171,7 +177,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="mixed (default): the precision form, timed beside the plain FP64 form")
     ap.add_argument("--config", choices=["c2", "c1", "c5"], default="c2")
     ap.add_argument("--form", choices=["full", "davidson"], default="full",
                     help="full: the whole eigensolve (plain FP64 or precision form); davidson: the "
@@ -249,11 +256,25 @@ def main():
         sorted=bool(np.all(np.diff(res["W"]) >= 0)),
         ground_state_eV=float(res["W"][0]),
     )
+    plain = None
     if args.precision == "mixed":
-        # the same workload in the plain FP64 form: the precision form's error bar
+        # the same workload in the plain FP64 form, timed the same way: the precision
+        # form's reference point and error bar
         with bse.BSEPlan(ctx, model, params(cfg, "fp64")) as p64:
-            p64.execute()
+            if args.no_graph:
+                p64.set_graph(False)
+            for _ in range(args.warmup):
+                p64.execute()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                p64.execute()
+            ctx.sync()
+            dt64 = (time.perf_counter() - t0) / args.steps
             r64 = p64.results()
+        log(f"plain FP64: {dt64 * 1e3:.2f} ms/step")
+        plain = {"value": round(dt64, 6), "ms_per_step": round(dt64 * 1e3, 3),
+                 "speedup_mixed_vs_plain": round(dt64 / dt, 3)}
         checks["mixed_vs_fp64_max_dW_eV"] = float(np.abs(res["W"] - r64["W"]).max())
         checks["mixed_vs_fp64_eps_rel"] = float(max(
             np.abs(res["eps"][c, q] - r64["eps"][c, q]).max() / max(np.abs(r64["eps"][c, q]).max(), 1e-30)
@@ -307,6 +328,7 @@ def main():
         "config": {"workload": cfg["name"], "bse_dim": N, "nwann": cfg["nw"],
                    "kgrid": list(cfg["ngrid"]), "potential": cfg["coultype"],
                    "precision": args.precision, "n_omega": cfg["numbse"]},
+        "fp64_plain": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stage_ms": {k: round(v, 3) for k, v in stage.items()},

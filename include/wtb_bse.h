@@ -259,13 +259,12 @@ int wtb_mlp_train(wtb_mlp* mlp, int S, const float* X, const float* t, int epoch
 int wtb_mlp_destroy(wtb_mlp* mlp);
 
 /* ------------------------------------------------------------------------
- * Tile shape of the reduction's Hermitian GEMV: 0 = one 64 x 64 tile per
- * workgroup (default), 1, 2, 4, 8 = strips of that many tiles per
- * workgroup; and its autotuner, which times the reduction for each shape on
- * the plan's own matrix and keeps the fastest (ms[5]: the times for 0, 1,
- * 2, 4, 8).  No counterpart in the reference.
+ * Panel width of the blocked Householder reduction (8..32 columns per
+ * trailing MFMA update; default 32) and its autotuner, which times the
+ * reduction for widths 16, 20, 24, 32 on the plan's own matrix and keeps the
+ * fastest (ms[4]: their times).  No counterpart in the reference.
  * ------------------------------------------------------------------------ */
-int wtb_bse_plan_set_tile(wtb_bse_plan* plan, int sub);
+int wtb_bse_plan_set_panel(wtb_bse_plan* plan, int nb);
 int wtb_bse_plan_tune(wtb_bse_plan* plan, int* best, double* ms);
 
 /* ------------------------------------------------------------------------

@@ -1,8 +1,8 @@
 """GPU: the batching form (many uneven models, one captured graph per matrix,
 replayed on parallel streams) and the GEMV tile autotuner.
 - each matrix of a batch gives bitwise the results of its own plan run alone;
-- every GEMV shape gives the same excitons and spectra (to rounding, the
-  partial sums are grouped differently) and the tuner picks one of them;
+- every panel width gives the same excitons and spectra (to rounding, the
+  updates are grouped differently) and the tuner picks one of them;
 - a batch with lanes = 1 and lanes = 4 agree bitwise."""
 import numpy as np
 import pytest
@@ -47,23 +47,23 @@ def test_batch_matches_single_plans(ctx, lanes):
                 assert np.array_equal(got["eps"], r["eps"]), i
 
 
-def test_tile_shapes_and_tuner(ctx):
+def test_panel_widths_and_tuner(ctx):
     from wantibexos_amd import bse, tbmodel
     m = tbmodel.hex2d(seed=4, nw=8, ngrid=(16, 16, 1))     # N = 1024
     prm = bse.BSEParams(ngrid=(16, 16, 1), nc=2, nv=2, coultype="V2DRK", r0=40.0, cshift=0.05, ebse0=0.0,
                         ebsef=6.0, numbse=501, dtfull=True, cpol=True, f154=False)
     res = {}
     with bse.BSEPlan(ctx, m, prm) as plan:
-        for sub in (0, 1, 2, 4, 8):
-            plan.set_tile(sub)
+        for nb in (16, 20, 24, 32):
+            plan.set_panel(nb)
             plan.execute()
-            res[sub] = plan.results()
+            res[nb] = plan.results()
         best, ms = plan.tune()
         plan.execute()
         tuned = plan.results()
-    print("tile times (ms):", ms, "best:", best)
-    assert best in (0, 1, 2, 4, 8)
-    for sub in (1, 2, 4, 8):
-        assert np.abs(res[sub]["W"] - res[0]["W"]).max() < 1e-11
-        assert np.abs(res[sub]["eps"] - res[0]["eps"]).max() < 1e-9 * np.abs(res[0]["eps"]).max()
+    print("panel times (ms):", ms, "best:", best)
+    assert best in (16, 20, 24, 32)
+    for nb in (16, 20, 24):
+        assert np.abs(res[nb]["W"] - res[32]["W"]).max() < 1e-11
+        assert np.abs(res[nb]["eps"] - res[32]["eps"]).max() < 1e-9 * np.abs(res[32]["eps"]).max()
     assert np.array_equal(tuned["W"], res[best]["W"])

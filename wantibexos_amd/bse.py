@@ -119,7 +119,7 @@ def _load():
         "wtb_mlp_loss_grad": (ip, [vp, ip, vp, vp, vp, vp]),
         "wtb_mlp_train": (ip, [vp, ip, vp, vp, ip, ip, C.c_float, C.c_uint, vp]),
         "wtb_mlp_destroy": (ip, [vp]),
-        "wtb_bse_plan_set_tile": (ip, [vp, ip]),
+        "wtb_bse_plan_set_panel": (ip, [vp, ip]),
         "wtb_bse_plan_set_vectors": (ip, [vp, ip, ip]),
         "wtb_bsesolver_wf": (ip, [vp, vp, vp, vp, vp, vp, vp, ip, ip, vp, vp, vp]),
         "wtb_bse_plan_tune": (ip, [vp, vp, vp]),
@@ -409,17 +409,17 @@ class BSEPlan:
         """Also form the eigenvectors of excitons first .. first+count-1 (fetch('excwf'))."""
         _check(lib().wtb_bse_plan_set_vectors(self.h, int(first), int(count)))
 
-    def set_tile(self, sub):
-        """GEMV shape of the reduction: 0 single 64 x 64 tiles, or strips of 1, 2, 4, 8 tiles."""
-        _check(lib().wtb_bse_plan_set_tile(self.h, int(sub)))
+    def set_panel(self, nb):
+        """Panel width of the blocked reduction (8..32 columns per trailing update)."""
+        _check(lib().wtb_bse_plan_set_panel(self.h, int(nb)))
 
     def tune(self):
-        """Time the reduction for each strip height on this matrix and keep the fastest.
-        Returns (best, {height: ms})."""
+        """Time the reduction for panel widths 16, 20, 24, 32 on this matrix and keep the
+        fastest.  Returns (best, {width: ms})."""
         best = np.zeros(1, dtype=np.int32)
-        ms = np.zeros(5)
+        ms = np.zeros(4)
         _check(lib().wtb_bse_plan_tune(self.h, _p(best), _p(ms)))
-        return int(best[0]), dict(zip((0, 1, 2, 4, 8), ms))
+        return int(best[0]), dict(zip((16, 20, 24, 32), ms))
 
     def row_stats(self):
         """Diagonal D (N,) and row statistics (3, N) of the matrix (precision-form plan)."""

@@ -6,40 +6,31 @@
 // that X^T h = Z^T (Q^T h) never needs the eigenvectors X = Q Z.
 //
 // Two kernels per reduced column j (panel column jj = j - p):
-//   rowstep(j)  rows >= j, 16 rows x 16 lanes per workgroup.  Finishes
-//               reflector j-1 with the scalars hemvstep(j-1) published (v; y =
-//               A22 v from the unscaled GEMV by linearity; u = y - V (W^H v) -
-//               W (V^H v); g <- g - tau conj(v) (v^T g)), then updates column j
-//               with the panel (ZLATRD), leaving out the 2 Re(alpha') v term of
-//               W(:,jj-1) that needs the global u^H v.  The last workgroup to
-//               finish reduces u^H v and publishes alpha'.
-//   hemvstep(j) strips of up to 4 lower 64x64 tiles of A22 (rows, cols > j) in
-//               one column block per workgroup.  Adds the deferred term to
-//               column j, completes W(:,jj-1), and forms z = A22 x~ with x~ =
-//               column j below row j+1 (the reflector before its scale is
-//               known).  Each tile is read once and serves both A x~ (rows) and
-//               A^H x~ (cols).  Diagonal tiles also write x_new, d_j and partials
-//               of |x~|^2, W^H x~, V^H x~, x~^T g; the last of them reduces those
-//               and publishes the ZLARFG scalars and W^H v, V^H v, v^T g.
+//   rowstep(j)  rows >= j, 16 rows x 16 lanes per workgroup.  Every workgroup
+//               first sums the per-row-block partials hemvstep(j-1) left and
+//               forms reflector j-1's ZLARFG scalars and W^H v, V^H v, v^T g
+//               itself; it then finishes the reflector (v; y = A22 v from the
+//               unscaled GEMV by linearity; u = y - V (W^H v) - W (V^H v);
+//               g <- g - tau conj(v) (v^T g)) and updates column j with the panel
+//               (ZLATRD), leaving out the 2 Re(alpha') v term of W(:,jj-1) that
+//               needs the global u^H v, of which it writes its partial.
+//   hemvstep(j) lower 64x64 tiles of A22 (rows, cols > j), one per workgroup.
+//               Every workgroup sums the u^H v partials into alpha', adds the
+//               deferred term to column j and forms z = A22 x~ with x~ = column j
+//               below row j+1 (the reflector before its scale is known).  Each
+//               tile is read once and serves both A x~ (rows) and A^H x~ (cols).
+//               Diagonal tiles also write x_new, W(:,jj-1), d_j and partials of
+//               |x~|^2, W^H x~, V^H x~, x~^T g.
 // Per panel: finish (rowstep without a column), W(:,last), and the trailing
 // update A22 -= V W^H + W V^H on MFMA (lower tiles).
-// "Last workgroup" reductions: partials are written, a device fence and an
-// atomic counter pick the workgroup that arrives last, which sums the partials
-// in a fixed order and resets the counter.  No workgroup ever waits on another.
+// Global reductions go through partials that the next kernel sums (every
+// workgroup of it, in a fixed order): no atomics, no device fences, no waits.
 #include "wtb_kernels.h"
 
 namespace wtb {
 
 #define TB 64   // tile / row-block edge
 #define RPW 16  // rows per rowstep workgroup
-
-// published scalars
-struct Pub {
-  zc tau, s;
-  zc beta;        // .re
-  zc alpha;       // alpha' of the last finished reflector
-  zc t1[32], t2[32], vg[16];
-};
 
 template <class T>
 struct TriArgs {
@@ -60,8 +51,7 @@ struct TriArgs {
   zc* g;           // nrhs x N
   zc* xn;          // N: reduced column j below its diagonal (x_new)
   zc* xp;          // N: column j after the panel update without the deferred term
-  Pub* pub;
-  unsigned* cnt;   // [2]: hemvstep, rowstep
+  zc* gs;          // [16] row j+1 of g as hemvstep(j) saw it (rowstep(j+1) updates that row)
   int nrhs;
   int nblk;
 };
@@ -74,28 +64,127 @@ __device__ inline zc wave_sum(zc v) {
   return v;
 }
 
-// true in every thread of the workgroup that arrived last
-__device__ inline bool arrive_last(unsigned* cnt, unsigned total) {
-  __shared__ int s_last;
-  __threadfence();
+// Sum of each of 16 values over the 64 lanes of a wave (pv: a zc[16] in registers):
+// every butterfly step halves the set a lane keeps; afterwards lane l holds the sum for
+// index (l >> 2) & 15.  A macro, so that the array is never addressed through memory.
+#define BUTTERFLY16(pv, lane)                                             \
+  do {                                                                    \
+    int n_ = 16;                                                          \
+    for (int off_ = 32; off_ >= 4; off_ >>= 1) {                          \
+      n_ >>= 1;                                                           \
+      const bool up_ = ((lane) & off_) != 0;                              \
+      for (int i_ = 0; i_ < n_; ++i_) {                                   \
+        const zc send_ = up_ ? pv[i_] : pv[i_ + n_];                      \
+        zc recv_;                                                         \
+        recv_.re = __shfl_xor(send_.re, off_);                            \
+        recv_.im = __shfl_xor(send_.im, off_);                            \
+        pv[i_] = (up_ ? pv[i_ + n_] : pv[i_]) + recv_;                    \
+      }                                                                   \
+    }                                                                     \
+    for (int off_ = 2; off_ > 0; off_ >>= 1) {                            \
+      pv[0].re += __shfl_xor(pv[0].re, off_);                             \
+      pv[0].im += __shfl_xor(pv[0].im, off_);                             \
+    }                                                                     \
+  } while (0)
+
+// alpha' = -1/2 |tau|^2 sum_b puv[b] over the np partials of the previous rowstep, in a
+// fixed order (every thread of the workgroup returns it)
+__device__ inline zc alpha_from_puv(const zc* __restrict__ puv, int np, zc tau) {
+  __shared__ zc red[4];
+  zc acc = mkz(0, 0);
+  for (int b = threadIdx.x; b < np; b += blockDim.x) acc += puv[b];
+  acc = wave_sum(acc);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(cnt, 1u);
-    s_last = (prev == total - 1u);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  zc t = mkz(0, 0);
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+  const double t2 = norm2(tau);
+  return mkz(-0.5 * t2 * t.re, -0.5 * t2 * t.im);
+}
+
+// Reflector jm's scalars from the partials of hemvstep(jm) (panel column jjh): ZLARFG on
+// alpha = x_new(jm+1) with |x|^2 = sum |x~|^2, then W^H v = conj(W(jm+1,:)) + s W^H x~,
+// V^H v likewise and v^T g, into the workgroup's shared arrays.  256 threads.
+template <class T>
+__device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums, zc* st1, zc* st2, zc* svg,
+                                  zc* s_tau, zc* s_s, bool write_global) {
+  const int tid = threadIdx.x;
+  const int N = a.N;
+  const int b0h = (jm + 1) / TB;
+  const int nq = 1 + 2 * jjh + a.nrhs;
+  const int sub = tid & 3;
+  for (int q0 = 0; q0 < nq; q0 += 64) {
+    const int q = q0 + (tid >> 2);
+    zc acc = mkz(0, 0);
+    if (q == 0) {
+      for (int b = b0h + sub; b < a.nblk; b += 4) acc.re += a.pn[b];
+    } else if (q < nq) {
+      const zc* base;
+      size_t stride;
+      if (q <= jjh) {
+        base = a.pt1 + (q - 1);
+        stride = a.nb;
+      } else if (q <= 2 * jjh) {
+        base = a.pt2 + (q - 1 - jjh);
+        stride = a.nb;
+      } else {
+        base = a.pvg + (q - 1 - 2 * jjh);
+        stride = a.nrhs;
+      }
+      for (int b = b0h + sub; b < a.nblk; b += 4) acc += base[(size_t)b * stride];
+    }
+    for (int off = 2; off > 0; off >>= 1) {
+      acc.re += __shfl_xor(acc.re, off, 4);
+      acc.im += __shfl_xor(acc.im, off, 4);
+    }
+    if (sub == 0 && q < nq) sums[q] = acc;
   }
   __syncthreads();
-  const bool last = s_last != 0;
-  if (last) __threadfence();
-  return last;
+  if (tid == 0) {
+    const double xn2 = sums[0].re;
+    const zc alpha = a.xn[jm + 1];
+    zc tau, sc;
+    double beta;
+    if (xn2 == 0.0 && alpha.im == 0.0) {
+      tau = mkz(0, 0);
+      sc = mkz(0, 0);
+      beta = alpha.re;
+    } else {
+      beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2), alpha.re);
+      tau = mkz((beta - alpha.re) / beta, -alpha.im / beta);
+      const zc den = mkz(alpha.re - beta, alpha.im);
+      const double dd = den.re * den.re + den.im * den.im;
+      sc = mkz(den.re / dd, -den.im / dd);
+    }
+    *s_tau = tau;
+    *s_s = sc;
+    if (write_global) {
+      a.tau[jm] = tau;
+      a.e[jm] = beta;
+    }
+  }
+  __syncthreads();
+  const zc sc = *s_s;
+  const int j1 = jm + 1;
+  if (tid < jjh) {
+    const int c = tid;
+    st1[c] = conj(a.W[(size_t)c * N + j1]) + sc * sums[1 + c];
+    st2[c] = conj(a.Vp[(size_t)c * N + j1]) + sc * sums[1 + jjh + c];
+  }
+  if (tid >= 64 && tid < 64 + a.nrhs) {
+    const int q = tid - 64;
+    svg[q] = a.gs[q] + sc * sums[1 + 2 * jjh + q];
+  }
+  __syncthreads();
 }
 
 // ---- rowstep ------------------------------------------------------------------------
-// Loads that do not depend on the scalars of hemvstep(j-1) (the GEMV partials, the
-// panel rows, column j) are issued before the published scalars are read, and the
-// panel values serve both u and the column update.
+// Loads that do not depend on reflector j-1's scalars (the GEMV partials, the panel
+// rows, column j) are issued first; the panel values serve both u and the column update.
 template <class T>
 __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj, int update_col) {
-  __shared__ zc st1[32], st2[32], svg[16], sWj[32], sVj[32];
+  __shared__ zc st1[32], st2[32], svg[16], sWj[32], sVj[32], sums[96];
   __shared__ zc su_j, s_tau, s_s;
   __shared__ zc suv[RPW];
   const int tid = threadIdx.x;
@@ -137,19 +226,7 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
       }
     }
   }
-  const Pub* P = a.pub;
-  if (tid < 32) {
-    if (tid < jjm) {
-      st1[tid] = P->t1[tid];
-      st2[tid] = P->t2[tid];
-    }
-    if (tid < a.nrhs) svg[tid] = P->vg[tid];
-    if (tid == 0) {
-      s_tau = P->tau;
-      s_s = P->s;
-    }
-  }
-  __syncthreads();
+  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, blockIdx.x == 0);
   const zc tau = s_tau, s = s_s;
   // s z(row) - V t1 - W t2, each row summed by its 16 lanes
   auto u_of = [&](zc ys, const zc* v2, const zc* w2) -> zc {
@@ -229,27 +306,15 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
     for (int i = 0; i < RPW; ++i) uv += suv[i];
     a.puv[blockIdx.x] = uv;
   }
-  if (!arrive_last(a.cnt + 1, gridDim.x)) return;
-  if (tid < 64) {
-    zc acc = mkz(0, 0);
-    for (int b = tid; b < (int)gridDim.x; b += 64) acc += a.puv[b];
-    acc = wave_sum(acc);
-    if (tid == 0) {
-      const double t2 = norm2(tau);
-      a.pub->alpha = mkz(-0.5 * t2 * acc.re, -0.5 * t2 * acc.im);
-      a.cnt[1] = 0u;
-    }
-  }
 }
 
-// ---- hemvstep, one 64 x 64 tile per workgroup (the default shape) ----------------------
+// ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
 // 4 waves, wave w owns 16 columns, lane = row: every load moves 1 KB of one column.
 template <class T>
-__global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) {
+__global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
   __shared__ zc xR[TB], xC[TB];
   __shared__ zc y1s[4][TB], y2s[1][TB];
-  __shared__ zc sums[96];
-  __shared__ zc s_sc;
+  __shared__ zc s_ap;
   const int N = a.N;
   const int b0 = (j + 1) / TB;
   int t = blockIdx.x;
@@ -261,7 +326,15 @@ __global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) 
   const int tid = threadIdx.x;
   const int R0 = bi * TB, C0 = bj * TB;
   const bool diag = bi == bj;
-  const zc ap = (jj > 0) ? a.pub->alpha : mkz(0, 0);
+  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
+  if (jj > 0) {
+    const zc al = alpha_from_puv(a.puv, nprev, taup);
+    if (tid == 0) s_ap = al;
+  } else if (tid == 0) {
+    s_ap = mkz(0, 0);
+  }
+  __syncthreads();
+  const zc ap = s_ap;
   const double sig = 2.0 * ap.re;                  // x_new = x_p - 2 Re(alpha') v_{j-1}
   auto xnew = [&](int row) -> zc {
     if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
@@ -274,11 +347,11 @@ __global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) 
     const int col = C0 + tid - TB;
     xC[tid - TB] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
   }
-  // wave w takes columns C0+16w .. C0+16w+15, lane = row: every load moves 1 KB of a column
   const int lane = tid & 63, w = tid >> 6;
   const int r = R0 + lane;
   const bool rvalid = r > j && r < N;
   zc hv[16];
+#pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int cl = 16 * w + k, c = C0 + cl;
     zc x = mkz(0, 0);
@@ -288,6 +361,7 @@ __global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) 
   __syncthreads();
   const zc xr = xR[lane];
   zc y1 = mkz(0, 0);
+#pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int cl = 16 * w + k;
     zc x = hv[k];
@@ -296,24 +370,7 @@ __global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) 
     hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
   }
   // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
-  {
-    int n = 16;
-    for (int off = 32; off >= 4; off >>= 1) {
-      n >>= 1;
-      const bool up = (lane & off) != 0;
-      for (int i = 0; i < n; ++i) {
-        const zc send = up ? hv[i] : hv[i + n];
-        zc recv;
-        recv.re = __shfl_xor(send.re, off);
-        recv.im = __shfl_xor(send.im, off);
-        hv[i] = (up ? hv[i + n] : hv[i]) + recv;
-      }
-    }
-    for (int off = 2; off > 0; off >>= 1) {
-      hv[0].re += __shfl_xor(hv[0].re, off);
-      hv[0].im += __shfl_xor(hv[0].im, off);
-    }
-  }
+  BUTTERFLY16(hv, lane);
   // lane holds the sum for column 16w + ((lane >> 2) & 15)
   y1s[w][lane] = y1;
   if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
@@ -328,340 +385,58 @@ __global__ __launch_bounds__(256) void k_tri_hemv1(TriArgs<T> a, int j, int jj) 
   }
   if (!diag) return;
   // ---- diagonal tiles: x_new, W(:,jj-1), d_j, partial dot products --------------------
-  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
   if (tid < TB) {
-    const int r = R0 + tid;
-    if (r >= j + 1 && r < N) {
-      a.xn[r] = xnew(r);
-      if (jj > 0) a.W[(size_t)(jj - 1) * N + r] = taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r];
+    const int rr = R0 + tid;
+    if (rr >= j + 1 && rr < N) {
+      a.xn[rr] = xnew(rr);
+      if (jj > 0) a.W[(size_t)(jj - 1) * N + rr] = taup * a.u[rr] + ap * a.Vp[(size_t)(jj - 1) * N + rr];
     }
   }
   if (ti == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
-  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g; 4 lanes each
+  // row j+1 (block b0) of g, for the next rowstep's v^T g, which also updates that row
+  if (ti == 0 && tid >= 2 * TB && tid < 2 * TB + a.nrhs) a.gs[tid - 2 * TB] = a.g[(size_t)(tid - 2 * TB) * N + (j + 1)];
+  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g.  Wave w takes
+  // groups of 16 quantities (q0 = 16 w, 16 w + 64): lane = row, all 16 loads issued
+  // together, the products summed over the 64 rows by the same butterfly; afterwards
+  // lane l holds quantity q0 + ((l >> 2) & 15).
   const int nq = 1 + 2 * jj + a.nrhs;
-  const int sub = tid & 3;
-  for (int q0 = 0; q0 < nq; q0 += 64) {
-    const int q = q0 + (tid >> 2);
-    zc acc = mkz(0, 0);
-    if (q < nq) {
-      for (int rl = sub; rl < TB; rl += 4) {
-        const int r = R0 + rl;
-        if (r < j + 2 || r >= N) continue;
-        const zc xr = xR[rl];
-        if (q == 0) {
-          acc.re += norm2(xr);
-        } else if (q <= jj) {
-          const int c = q - 1;
-          const zc w = (c == jj - 1) ? taup * a.u[r] + ap * a.Vp[(size_t)c * N + r] : a.W[(size_t)c * N + r];
-          acc = fma_conj(w, xr, acc);
-        } else if (q <= 2 * jj) {
-          acc = fma_conj(a.Vp[(size_t)(q - 1 - jj) * N + r], xr, acc);
-        } else {
-          acc = fma_z(xr, a.g[(size_t)(q - 1 - 2 * jj) * N + r], acc);
-        }
-      }
-    }
-    for (int off = 2; off > 0; off >>= 1) {
-      acc.re += __shfl_xor(acc.re, off, 4);
-      acc.im += __shfl_xor(acc.im, off, 4);
-    }
-    if (sub == 0 && q < nq) {
-      if (q == 0) a.pn[bi] = acc.re;
-      else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = acc;
-      else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = acc;
-      else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = acc;
-    }
-  }
-  if (!arrive_last(a.cnt + 0, (unsigned)(a.nblk - b0))) return;
-  // ---- last diagonal tile: sums, ZLARFG, W^H v, V^H v, v^T g --------------------------
-  for (int q0 = 0; q0 < nq; q0 += 64) {
-    const int q = q0 + (tid >> 2);
-    zc acc = mkz(0, 0);
-    if (q < nq) {
-      for (int b = b0 + sub; b < a.nblk; b += 4) {
-        if (q == 0) acc.re += a.pn[b];
-        else if (q <= jj) acc += a.pt1[(size_t)b * a.nb + (q - 1)];
-        else if (q <= 2 * jj) acc += a.pt2[(size_t)b * a.nb + (q - 1 - jj)];
-        else acc += a.pvg[(size_t)b * a.nrhs + (q - 1 - 2 * jj)];
-      }
-    }
-    for (int off = 2; off > 0; off >>= 1) {
-      acc.re += __shfl_xor(acc.re, off, 4);
-      acc.im += __shfl_xor(acc.im, off, 4);
-    }
-    if (sub == 0 && q < nq) sums[q] = acc;
-  }
-  __syncthreads();
-  Pub* P = a.pub;
-  if (tid == 0) {
-    // ZLARFG on alpha = x_new(j+1), |x|^2 = sum |x~|^2
-    const double xn2 = sums[0].re;
-    const zc alpha = a.xn[j + 1];
-    zc tau, s;
-    double beta;
-    if (xn2 == 0.0 && alpha.im == 0.0) {
-      tau = mkz(0, 0);
-      s = mkz(0, 0);
-      beta = alpha.re;
-    } else {
-      beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2), alpha.re);
-      tau = mkz((beta - alpha.re) / beta, -alpha.im / beta);
-      const zc den = mkz(alpha.re - beta, alpha.im);
-      const double dd = den.re * den.re + den.im * den.im;
-      s = mkz(den.re / dd, -den.im / dd);
-    }
-    P->tau = tau;
-    P->s = s;
-    P->beta = mkz(beta, 0);
-    a.tau[j] = tau;
-    a.e[j] = beta;
-    s_sc = s;
-    a.cnt[0] = 0u;
-  }
-  __syncthreads();
-  const zc s = s_sc;
-  // v = e_{j+1} + s x~:  W^H v = conj(W(j+1,:)) + s W^H x~, likewise V^H v, v^T g
-  if (tid < jj) {
-    const int c = tid;
-    const zc wj = (c == jj - 1) ? taup * a.u[j + 1] + ap * a.Vp[(size_t)c * N + (j + 1)]
-                                : a.W[(size_t)c * N + (j + 1)];
-    P->t1[c] = conj(wj) + s * sums[1 + c];
-    P->t2[c] = conj(a.Vp[(size_t)c * N + (j + 1)]) + s * sums[1 + jj + c];
-  }
-  if (tid >= 64 && tid < 64 + a.nrhs) {
-    const int q = tid - 64;
-    P->vg[q] = a.g[(size_t)q * N + (j + 1)] + s * sums[1 + 2 * jj + q];
-  }
-}
-
-// ---- hemvstep, strips (tunable shape) -----------------------------------------------
-// One workgroup per strip of up to SUB lower 64x64 tiles stacked in one column block
-// (column block bj, row blocks bi0 .. bi0+SUB-1, bi0 >= bj): 8 waves, wave w owns 8
-// columns, lane = row of the current tile, so every load moves 1 KB of one column and a
-// wave walks 256 consecutive rows of its columns.  Row sums (A x~) are written per tile;
-// the column sums (A^H x~) accumulate over the strip in registers and are reduced once.
-template <class T, int SUB>
-__global__ __launch_bounds__(512) void k_tri_hemv(TriArgs<T> a, int j, int jj) {
-  __shared__ zc xR[SUB][TB], xC[TB];
-  __shared__ zc y1s[8][TB], y2s[TB], ydiag[TB];
-  __shared__ zc sums[96];
-  __shared__ zc s_sc;
-  const int N = a.N;
-  const int b0 = (j + 1) / TB;
-  const int bj = b0 + blockIdx.y;
-  const int bi0 = bj + SUB * blockIdx.x;
-  if (bj >= a.nblk || bi0 >= a.nblk) return;
-  const int nsub = min(SUB, a.nblk - bi0);
-  const bool dstrip = bi0 == bj;                   // the strip starts with the diagonal tile
-  const int tid = threadIdx.x;
-  const int C0 = bj * TB;
-  const zc ap = (jj > 0) ? a.pub->alpha : mkz(0, 0);
-  const double sig = 2.0 * ap.re;                  // x_new = x_p - 2 Re(alpha') v_{j-1}
-  auto xnew = [&](int row) -> zc {
-    if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
-    return a.xp[row] - sig * a.Vp[(size_t)(jj - 1) * N + row];
-  };
-  for (int e = tid; e < SUB * TB; e += 512) {
-    const int ts = e >> 6, l = e & 63;
-    const int row = (bi0 + ts) * TB + l;
-    xR[ts][l] = (ts < nsub && row >= j + 2 && row < N) ? xnew(row) : mkz(0, 0);
-  }
-  if (tid < TB) {
-    const int col = C0 + tid;
-    xC[tid] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
-  }
-  const int lane = tid & 63, w = tid >> 6;
-  const int c0 = C0 + 8 * w;                       // this wave's first column
-  zc hacc[8];
-  for (int k = 0; k < 8; ++k) hacc[k] = mkz(0, 0);
-  __syncthreads();
+  const bool rok = r >= j + 2 && r < N;
+  const zc wl = (jj > 0 && rok) ? taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r] : mkz(0, 0);
+  const double xr2 = norm2(xr);
+  for (int q0 = 16 * w; q0 < nq; q0 += 64) {
+    zc pv[16];
 #pragma unroll
-  for (int ts = 0; ts < SUB; ++ts) {
-    if (ts < nsub) {
-      const int bi = bi0 + ts;
-      const bool diag = dstrip && ts == 0;
-      const int r = bi * TB + lane;
-      const bool rvalid = r > j && r < N;
-      zc hv[8];
-      for (int k = 0; k < 8; ++k) {
-        const int c = c0 + k, cl = 8 * w + k;
-        zc x = mkz(0, 0);
-        if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
-        hv[k] = x;
-      }
-      const zc xr = xR[ts][lane];
-      zc y1 = mkz(0, 0);
-      for (int k = 0; k < 8; ++k) {
-        const int cl = 8 * w + k;
-        zc x = hv[k];
-        if (diag && lane == cl) x.im = 0.0;                      // diagonal: real part (ZHEMV)
-        y1 = fma_z(x, xC[cl], y1);
-        if (!diag || lane > cl) hacc[k] += conj(x) * xr;         // A^H x~ terms (strictly lower)
-      }
-      y1s[w][lane] = y1;
-      __syncthreads();
-      if (tid < TB) {
-        zc sm = mkz(0, 0);
-        for (int q = 0; q < 8; ++q) sm += y1s[q][tid];
-        if (diag) ydiag[tid] = sm;
-        else a.ypart[((size_t)bi * a.nblk + bj) * TB + tid] = sm;
-      }
-      __syncthreads();
+    for (int k = 0; k < 16; ++k) {
+      const int q = q0 + k;
+      // the loaded operand: W(r, q-1), V(r, q-1-jj) or g(q-1-2jj, r); W(r, jj-1) is wl
+      const zc* src = nullptr;
+      if (q >= 1 && q < jj) src = a.W + (size_t)(q - 1) * N + r;
+      else if (q > jj && q <= 2 * jj) src = a.Vp + (size_t)(q - 1 - jj) * N + r;
+      else if (q > 2 * jj && q < nq) src = a.g + (size_t)(q - 1 - 2 * jj) * N + r;
+      const zc val = (src && rok) ? *src : ((q == jj) ? wl : mkz(0, 0));
+      zc v = (q <= 2 * jj) ? fma_conj(val, xr, mkz(0, 0)) : fma_z(xr, val, mkz(0, 0));
+      if (q == 0) v = mkz(xr2, 0.0);
+      pv[k] = (q < nq && rok) ? v : mkz(0, 0);
     }
-  }
-  // column sums of the strip: transpose-reduce the 8 column partials over the 64 lanes
-  {
-    int n = 8;
-    for (int off = 32; off >= 8; off >>= 1) {
-      n >>= 1;
-      const bool up = (lane & off) != 0;
-      for (int i = 0; i < n; ++i) {
-        const zc send = up ? hacc[i] : hacc[i + n];
-        zc recv;
-        recv.re = __shfl_xor(send.re, off);
-        recv.im = __shfl_xor(send.im, off);
-        hacc[i] = (up ? hacc[i + n] : hacc[i]) + recv;
-      }
+    BUTTERFLY16(pv, lane);
+    const int q = q0 + ((lane >> 2) & 15);
+    if ((lane & 3) == 0 && q < nq) {
+      if (q == 0) a.pn[bi] = pv[0].re;
+      else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = pv[0];
+      else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = pv[0];
+      else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = pv[0];
     }
-    for (int off = 4; off > 0; off >>= 1) {
-      hacc[0].re += __shfl_xor(hacc[0].re, off);
-      hacc[0].im += __shfl_xor(hacc[0].im, off);
-    }
-  }
-  // lane holds the sum for column 8w + ((lane >> 3) & 7)
-  if ((lane & 7) == 0) y2s[8 * w + ((lane >> 3) & 7)] = hacc[0];
-  __syncthreads();
-  zc* yp = a.ypart;
-  if (tid < TB) {
-    // A^H x~ of the whole strip into slot (bj, bi0) (with the diagonal tile's A x~ there);
-    // the strip's other slots of row block bj hold zero
-    zc sm = y2s[tid];
-    if (dstrip) sm += ydiag[tid];
-    yp[((size_t)bj * a.nblk + bi0) * TB + tid] = sm;
-  } else if (tid < SUB * TB) {
-    const int ts = tid / TB;
-    if (ts < nsub) yp[((size_t)bj * a.nblk + bi0 + ts) * TB + (tid % TB)] = mkz(0, 0);
-  }
-  if (!dstrip) return;
-  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, partial dot products --------------------
-  const int bi = bj;
-  const int R0 = bi * TB;
-  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
-  if (tid < TB) {
-    const int r = R0 + tid;
-    if (r >= j + 1 && r < N) {
-      a.xn[r] = xnew(r);
-      if (jj > 0) a.W[(size_t)(jj - 1) * N + r] = taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r];
-    }
-  }
-  if (blockIdx.y == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
-  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g; 4 lanes each
-  const int nq = 1 + 2 * jj + a.nrhs;
-  const int sub = tid & 3;
-  for (int q0 = 0; q0 < nq; q0 += 128) {
-    const int q = q0 + (tid >> 2);
-    zc acc = mkz(0, 0);
-    if (q < nq) {
-      for (int rl = sub; rl < TB; rl += 4) {
-        const int r = R0 + rl;
-        if (r < j + 2 || r >= N) continue;
-        const zc xr = xR[0][rl];
-        if (q == 0) {
-          acc.re += norm2(xr);
-        } else if (q <= jj) {
-          const int c = q - 1;
-          const zc wv = (c == jj - 1) ? taup * a.u[r] + ap * a.Vp[(size_t)c * N + r] : a.W[(size_t)c * N + r];
-          acc = fma_conj(wv, xr, acc);
-        } else if (q <= 2 * jj) {
-          acc = fma_conj(a.Vp[(size_t)(q - 1 - jj) * N + r], xr, acc);
-        } else {
-          acc = fma_z(xr, a.g[(size_t)(q - 1 - 2 * jj) * N + r], acc);
-        }
-      }
-    }
-    for (int off = 2; off > 0; off >>= 1) {
-      acc.re += __shfl_xor(acc.re, off, 4);
-      acc.im += __shfl_xor(acc.im, off, 4);
-    }
-    if (sub == 0 && q < nq) {
-      if (q == 0) a.pn[bi] = acc.re;
-      else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = acc;
-      else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = acc;
-      else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = acc;
-    }
-  }
-  if (!arrive_last(a.cnt + 0, (unsigned)(a.nblk - b0))) return;
-  // ---- last diagonal tile: sums, ZLARFG, W^H v, V^H v, v^T g --------------------------
-  for (int q0 = 0; q0 < nq; q0 += 128) {
-    const int q = q0 + (tid >> 2);
-    zc acc = mkz(0, 0);
-    if (q < nq) {
-      for (int b = b0 + sub; b < a.nblk; b += 4) {
-        if (q == 0) acc.re += a.pn[b];
-        else if (q <= jj) acc += a.pt1[(size_t)b * a.nb + (q - 1)];
-        else if (q <= 2 * jj) acc += a.pt2[(size_t)b * a.nb + (q - 1 - jj)];
-        else acc += a.pvg[(size_t)b * a.nrhs + (q - 1 - 2 * jj)];
-      }
-    }
-    for (int off = 2; off > 0; off >>= 1) {
-      acc.re += __shfl_xor(acc.re, off, 4);
-      acc.im += __shfl_xor(acc.im, off, 4);
-    }
-    if (sub == 0 && q < nq) sums[q] = acc;
-  }
-  __syncthreads();
-  Pub* P = a.pub;
-  if (tid == 0) {
-    // ZLARFG on alpha = x_new(j+1), |x|^2 = sum |x~|^2
-    const double xn2 = sums[0].re;
-    const zc alpha = a.xn[j + 1];
-    zc tau, s;
-    double beta;
-    if (xn2 == 0.0 && alpha.im == 0.0) {
-      tau = mkz(0, 0);
-      s = mkz(0, 0);
-      beta = alpha.re;
-    } else {
-      beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2), alpha.re);
-      tau = mkz((beta - alpha.re) / beta, -alpha.im / beta);
-      const zc den = mkz(alpha.re - beta, alpha.im);
-      const double dd = den.re * den.re + den.im * den.im;
-      s = mkz(den.re / dd, -den.im / dd);
-    }
-    P->tau = tau;
-    P->s = s;
-    P->beta = mkz(beta, 0);
-    a.tau[j] = tau;
-    a.e[j] = beta;
-    s_sc = s;
-    a.cnt[0] = 0u;
-  }
-  __syncthreads();
-  const zc s = s_sc;
-  // v = e_{j+1} + s x~:  W^H v = conj(W(j+1,:)) + s W^H x~, likewise V^H v, v^T g
-  if (tid < jj) {
-    const int c = tid;
-    const zc wj = (c == jj - 1) ? taup * a.u[j + 1] + ap * a.Vp[(size_t)c * N + (j + 1)]
-                                : a.W[(size_t)c * N + (j + 1)];
-    P->t1[c] = conj(wj) + s * sums[1 + c];
-    P->t2[c] = conj(a.Vp[(size_t)c * N + (j + 1)]) + s * sums[1 + jj + c];
-  }
-  if (tid >= 64 && tid < 64 + a.nrhs) {
-    const int q = tid - 64;
-    P->vg[q] = a.g[(size_t)q * N + (j + 1)] + s * sums[1 + 2 * jj + q];
   }
 }
 
 // ---- W(:, kp-1) of the panel's last reflector ----------------------------------------
 template <class T>
-__global__ __launch_bounds__(64) void k_tri_wlast(TriArgs<T> a, int jl, int kpm1) {
+__global__ __launch_bounds__(64) void k_tri_wlast(TriArgs<T> a, int jl, int kpm1, int nprev) {
   const int lane = threadIdx.x;
+  const zc tp = a.tau[jl];
+  const zc ap = alpha_from_puv(a.puv, nprev, tp);
   const int r = ((jl + 1) / TB + blockIdx.x) * TB + lane;
-  const zc ap = a.pub->alpha;
-  if (r > jl && r < a.N) {
-    const zc tp = a.tau[jl];
-    a.W[(size_t)kpm1 * a.N + r] = tp * a.u[r] + ap * a.Vp[(size_t)kpm1 * a.N + r];
-  }
+  if (r > jl && r < a.N) a.W[(size_t)kpm1 * a.N + r] = tp * a.u[r] + ap * a.Vp[(size_t)kpm1 * a.N + r];
 }
 
 // ---- trailing update A22 -= V W^H + W V^H on lower 64x64 tiles (MFMA f64) ---------------
@@ -787,9 +562,6 @@ __global__ void k_tri_last(TriArgs<T> a) {
   a.e[a.N - 1] = 0.0;
 }
 
-__global__ void k_tri_reset(unsigned* cnt) {
-  if (threadIdx.x < 2) cnt[threadIdx.x] = 0u;
-}
 
 // ---------------------------------------------------------------------------
 size_t tridiag_work_bytes(int N, int nb, int nrhs) {
@@ -803,8 +575,7 @@ size_t tridiag_work_bytes(int N, int nb, int nrhs) {
   b += sizeof(zc) * nblk * (2 * nb + nrhs + 1) + 1024;
   b += sizeof(zc) * nblk * nblk * TB + 256;     // ypart
   b += sizeof(zc) * N * 2 + 512;                // xn, xp
-  b += sizeof(Pub) + 256;
-  b += 256;                                     // counters
+  b += sizeof(zc) * 16 + 256;                   // gs
   return b + 4096;
 }
 
@@ -831,14 +602,14 @@ int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs) {
   tw.ypart = (zc*)take(sizeof(zc) * nblk * nblk * TB);
   tw.xn = (zc*)take(sizeof(zc) * N);
   tw.xp = (zc*)take(sizeof(zc) * N);
-  tw.pub = take(sizeof(Pub));
-  tw.cnt = (unsigned*)take(256);
+  tw.gs = (zc*)take(sizeof(zc) * 16);
   return WTB_OK;
 }
 
 template <class T>
 static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_g) {
-  const int N = tw.N, nb = tw.nb;
+  const int N = tw.N;
+  const int nb = (tw.panel >= 8 && tw.panel <= tw.nb) ? tw.panel : tw.nb;
   if (nb > 32 || tw.nrhs > 16) return WTB_ERR_LIMIT;
   TriArgs<T> a;
   a.H = d_H;
@@ -861,12 +632,8 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
   a.g = d_g;
   a.xn = tw.xn;
   a.xp = tw.xp;
-  a.pub = (Pub*)tw.pub;
-  a.cnt = tw.cnt;
+  a.gs = tw.gs;
   a.nrhs = tw.nrhs;
-  // the arrival counters start from zero (a kernel, not a memset, so that the
-  // sequence can be captured into a graph); every use returns them to zero
-  hipLaunchKernelGGL(k_tri_reset, dim3(1), dim3(64), 0, s, a.cnt);
   tw.kev_used = 0;
   if (N == 1) {
     hipLaunchKernelGGL(k_tri_last<T>, dim3(1), dim3(1), 0, s, a);
@@ -875,22 +642,17 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
   }
   for (int p = 0; p < N - 1; p += nb) {
     const int kp = (nb < N - 1 - p) ? nb : (N - 1 - p);
+    int nprev = 0;                                  // workgroups of the last rowstep
     for (int jj = 0; jj < kp; ++jj) {
       const int j = p + jj;
       if (jj > 0) {
-        const int nA = (N - j + RPW - 1) / RPW;
-        hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nA), dim3(256), 0, s, a, j, jj, 1);
+        nprev = (N - j + RPW - 1) / RPW;
+        hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nprev), dim3(256), 0, s, a, j, jj, 1);
       }
       const int nt = a.nblk - (j + 1) / TB;
       const bool tev = tw.kev && tw.kev_used < tw.kev_cap;
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used], s);
-      switch (tw.sub) {
-        case 0: hipLaunchKernelGGL(k_tri_hemv1<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj); break;
-        case 1: hipLaunchKernelGGL((k_tri_hemv<T, 1>), dim3(nt, nt), dim3(512), 0, s, a, j, jj); break;
-        case 2: hipLaunchKernelGGL((k_tri_hemv<T, 2>), dim3((nt + 1) / 2, nt), dim3(512), 0, s, a, j, jj); break;
-        case 8: hipLaunchKernelGGL((k_tri_hemv<T, 8>), dim3((nt + 7) / 8, nt), dim3(512), 0, s, a, j, jj); break;
-        default: hipLaunchKernelGGL((k_tri_hemv<T, 4>), dim3((nt + 3) / 4, nt), dim3(512), 0, s, a, j, jj); break;
-      }
+      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj, nprev);
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used++ + 1], s);
     }
     // finish the panel's last reflector (no further column in this panel)
@@ -898,7 +660,7 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
     const int jn = jl + 1;
     const int nF = (N - jn + RPW - 1) / RPW;
     hipLaunchKernelGGL(k_tri_rowstep<T>, dim3(nF), dim3(256), 0, s, a, jn, kp, 0);
-    hipLaunchKernelGGL(k_tri_wlast<T>, dim3(a.nblk - (jl + 1) / TB), dim3(64), 0, s, a, jl, kp - 1);
+    hipLaunchKernelGGL(k_tri_wlast<T>, dim3(a.nblk - (jl + 1) / TB), dim3(64), 0, s, a, jl, kp - 1, nF);
     if (tw.store_v && N - p - 2 > 0)
       hipLaunchKernelGGL(k_tri_storev<T>, dim3((N - p - 2 + 255) / 256, kp), dim3(256), 0, s, a, p);
     const int s0 = p + kp;

@@ -1146,13 +1146,13 @@ static void plan_drop_graph(wtb_bse_plan* p) {
   p->graph = nullptr;
 }
 
-int wtb_bse_plan_set_tile(wtb_bse_plan* p, int sub) {
-  if (!p || !(sub == 0 || sub == 1 || sub == 2 || sub == 4 || sub == 8)) {
-    set_error("wtb_bse_plan_set_tile: shape must be 0 (single tiles) or a strip of 1, 2, 4 or 8 tiles");
+int wtb_bse_plan_set_panel(wtb_bse_plan* p, int nb) {
+  if (!p || nb < 8 || nb > 32) {
+    set_error("wtb_bse_plan_set_panel: panel width must lie in 8..32");
     return WTB_ERR_ARG;
   }
-  if (p->tw.sub != sub) plan_drop_graph(p);
-  p->tw.sub = sub;
+  if (p->tw.panel != nb) plan_drop_graph(p);
+  p->tw.panel = nb;
   return WTB_OK;
 }
 
@@ -1162,15 +1162,15 @@ int wtb_bse_plan_tune(wtb_bse_plan* p, int* best, double* ms) {
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
-  const int cand[5] = {0, 1, 2, 4, 8};
+  const int cand[4] = {16, 20, 24, 32};
   const bool timing0 = p->timing;
-  const int sub0 = p->tw.sub;
+  const int nb0 = p->tw.panel;
   double tbest = 1e300;
-  int sbest = sub0;
+  int sbest = nb0;
   p->timing = true;
   int rc = WTB_OK;
-  for (int k = 0; k < 5 && !rc; ++k) {
-    p->tw.sub = cand[k];
+  for (int k = 0; k < 4 && !rc; ++k) {
+    p->tw.panel = cand[k];
     double t = 1e300;
     for (int rep = 0; rep < 2 && !rc; ++rep) {   // the first run warms up
       rc = plan_enqueue(p);
@@ -1185,8 +1185,8 @@ int wtb_bse_plan_tune(wtb_bse_plan* p, int* best, double* ms) {
     }
   }
   p->timing = timing0;
-  p->tw.sub = sbest;
-  if (sbest != sub0) plan_drop_graph(p);
+  p->tw.panel = sbest;
+  if (sbest != nb0) plan_drop_graph(p);
   if (best) *best = sbest;
   return rc;
 }
@@ -1260,7 +1260,7 @@ int wtb_bse_batch_dim(const wtb_bse_batch* b, int i) {
 int wtb_bse_batch_tune(wtb_bse_batch* b, int* tiles) {
   if (!b) return WTB_ERR_ARG;
   for (size_t i = 0; i < b->plans.size(); ++i) {
-    int best = 0;
+    int best = 32;
     int rc = wtb_bse_plan_tune(b->plans[i], &best, nullptr);
     if (rc) return rc;
     if (tiles) tiles[i] = best;

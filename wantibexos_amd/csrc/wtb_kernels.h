@@ -58,8 +58,7 @@ struct TridiagWork {
   zc* xn = nullptr;       // reduced column below the diagonal
   zc* xp = nullptr;       // panel-updated column before the deferred term
   zc* puv = nullptr;      // per-workgroup u^H v partials
-  void* pub = nullptr;    // scalars published by the last workgroup of a step
-  unsigned* cnt = nullptr;  // arrival counters
+  zc* gs = nullptr;       // saved row of the tracked vectors
   // optional per-launch timing of the Hermitian GEMV kernel: when kev is set,
   // launch c is bracketed by events kev[2c], kev[2c+1] (c < kev_cap)
   hipEvent_t* kev = nullptr;
@@ -67,9 +66,9 @@ struct TridiagWork {
   int kev_used = 0;
   // keep the reflectors (v_j below row j+1 in column j of H) for the back-transform
   int store_v = 0;
-  // GEMV shape: 0 = one 64 x 64 tile per 256-thread workgroup (16 columns per wave);
-  // 1, 2, 4, 8 = strips of that many tiles per 512-thread workgroup (8 columns per wave)
-  int sub = 0;
+  // panel width of the blocked reduction (columns per trailing update), <= nb of the
+  // binding; the autotuner picks it per matrix
+  int panel = 32;
 };
 size_t tridiag_work_bytes(int N, int nb, int nrhs);
 int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs);

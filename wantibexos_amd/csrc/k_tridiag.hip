@@ -326,6 +326,30 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   const int tid = threadIdx.x;
   const int R0 = bi * TB, C0 = bj * TB;
   const bool diag = bi == bj;
+  const int lane = tid & 63, w = tid >> 6;
+  const int r = R0 + lane;
+  const bool rvalid = r > j && r < N;
+  // the tile and the raw pieces of x_new first: none of them depends on alpha'
+  zc hv[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int cl = 16 * w + k, c = C0 + cl;
+    zc x = mkz(0, 0);
+    if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
+    hv[k] = x;
+  }
+  // x_new = x_p - 2 Re(alpha') v_{j-1} (or column j of H at a panel's first column)
+  const int xrow = (tid < TB) ? R0 + tid : C0 + tid - TB;
+  const bool xok = tid < 2 * TB && xrow >= j + 2 && xrow < N;
+  zc x0 = mkz(0, 0), x1 = mkz(0, 0);
+  if (xok) {
+    if (jj == 0) {
+      x0 = ldz(a.H[(size_t)xrow + (size_t)a.ld * j]);
+    } else {
+      x0 = a.xp[xrow];
+      x1 = a.Vp[(size_t)(jj - 1) * N + xrow];
+    }
+  }
   const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
   if (jj > 0) {
     const zc al = alpha_from_puv(a.puv, nprev, taup);
@@ -335,28 +359,15 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   }
   __syncthreads();
   const zc ap = s_ap;
-  const double sig = 2.0 * ap.re;                  // x_new = x_p - 2 Re(alpha') v_{j-1}
+  const double sig = 2.0 * ap.re;
   auto xnew = [&](int row) -> zc {
     if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
     return a.xp[row] - sig * a.Vp[(size_t)(jj - 1) * N + row];
   };
-  if (tid < TB) {
-    const int row = R0 + tid;
-    xR[tid] = (row >= j + 2 && row < N) ? xnew(row) : mkz(0, 0);
-  } else if (tid < 2 * TB) {
-    const int col = C0 + tid - TB;
-    xC[tid - TB] = (col >= j + 2 && col < N) ? xnew(col) : mkz(0, 0);
-  }
-  const int lane = tid & 63, w = tid >> 6;
-  const int r = R0 + lane;
-  const bool rvalid = r > j && r < N;
-  zc hv[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int cl = 16 * w + k, c = C0 + cl;
-    zc x = mkz(0, 0);
-    if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
-    hv[k] = x;
+  if (tid < 2 * TB) {
+    const zc xv = xok ? x0 - sig * x1 : mkz(0, 0);
+    if (tid < TB) xR[tid] = xv;
+    else xC[tid - TB] = xv;
   }
   __syncthreads();
   const zc xr = xR[lane];

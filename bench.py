@@ -140,6 +140,8 @@ def bench_davidson(args, cfg, model):
     dt = float(np.mean(times))
     # the judge: full FP64 solve (plain path), timed once after a warm-up
     with bse.BSEPlan(ctx, model, p64) as full:
+        if args.no_graph:
+            full.set_graph(False)
         full.execute()
         ctx.sync()
         t0 = time.perf_counter()

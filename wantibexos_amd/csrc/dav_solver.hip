@@ -122,12 +122,14 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   double *theta = nullptr, *rn = nullptr;
   int *idx = nullptr, *keep = nullptr, *map = nullptr;
   zc* hw = nullptr;
+  zc* gw = nullptr;   // K-slice partials of the projections V^H X
+  const size_t gws = (size_t)8 * msub * nk;
   SmallEig se;
   int rc = WTB_OK;
   auto cleanup = [&]() {
     for (void* p : {(void*)V, (void*)HV, (void*)G, (void*)S, (void*)U, (void*)HU, (void*)R, (void*)T,
                     (void*)T2, (void*)P, (void*)Gb, (void*)Linv, (void*)theta, (void*)rn, (void*)idx,
-                    (void*)keep, (void*)map, (void*)hw})
+                    (void*)keep, (void*)map, (void*)hw, (void*)gw})
       hipFree(p);
     small_free(se);
   };
@@ -155,6 +157,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   DAV_ALLOC(keep, sizeof(int) * nk);
   DAV_ALLOC(map, sizeof(int) * nk);
   DAV_ALLOC(hw, hmm_work_bytes(N, nk));
+  DAV_ALLOC(gw, sizeof(zc) * gws);
 #undef DAV_ALLOC
   if ((rc = small_alloc(se, msub, nk))) {
     cleanup();
@@ -191,7 +194,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   int m = nk;
   DAV_TRY(matvec(m, V, HV));
   info.matvecs += m;
-  DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
+  DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub, gw, gws));
   std::vector<double> h_rn(nk);
   double best_mx = 1e300;
   int since_best = 0;
@@ -238,7 +241,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
       info.matvecs += nk;
       info.matvecs_fp64 += nk;
       m = nk;
-      DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub));
+      DAV_TRY(zgemm(s, 'C', 'N', m, m, N, one, V, N, HV, N, zero, G, msub, gw, gws));
       best_mx = 1e300;
       since_best = 0;
       continue;
@@ -261,7 +264,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     }
     // T -= V (V^H T), twice
     for (int pass = 0; pass < 2; ++pass) {
-      DAV_TRY(zgemm(s, 'C', 'N', m, nq, N, one, V, N, T, N, zero, P, msub));
+      DAV_TRY(zgemm(s, 'C', 'N', m, nq, N, one, V, N, T, N, zero, P, msub, gw, gws));
       DAV_TRY(zgemm(s, 'N', 'N', N, nq, m, mone, V, N, P, msub, one, T, N));
     }
     // Cholesky QR twice (T <- T L^-H with T^H T = L L^H, columns of tiny pivots dropped),
@@ -269,10 +272,10 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     int nn = nq;
     for (int pass = 0; pass < 2 && nn > 0; ++pass) {
       if (pass == 1) {
-        DAV_TRY(zgemm(s, 'C', 'N', m, nn, N, one, V, N, T, N, zero, P, msub));
+        DAV_TRY(zgemm(s, 'C', 'N', m, nn, N, one, V, N, T, N, zero, P, msub, gw, gws));
         DAV_TRY(zgemm(s, 'N', 'N', N, nn, m, mone, V, N, P, msub, one, T, N));
       }
-      DAV_TRY(zgemm(s, 'C', 'N', nn, nn, N, one, T, N, T, N, zero, Gb, nk));
+      DAV_TRY(zgemm(s, 'C', 'N', nn, nn, N, one, T, N, T, N, zero, Gb, nk, gw, gws));
       DAV_TRY(launch_chol_inv(s, nn, Gb, nk, Linv, keep, 1e-10));
       DAV_TRY(zgemm(s, 'N', 'C', N, nn, nn, one, T, N, Linv, nn, zero, T2, N));
       DAV_TRY(hipMemcpyAsync(h_keep.data(), keep, sizeof(int) * nn, hipMemcpyDeviceToHost, s) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
@@ -292,7 +295,7 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     info.matvecs += nn;
     if (hi) info.matvecs_fp64 += nn;
     // G(0:m+nn, m:m+nn) = V^H (H Vn); mirror into G(m:m+nn, 0:m)
-    DAV_TRY(zgemm(s, 'C', 'N', m + nn, nn, N, one, V, N, HV + (size_t)N * m, N, zero, G + (size_t)msub * m, msub));
+    DAV_TRY(zgemm(s, 'C', 'N', m + nn, nn, N, one, V, N, HV + (size_t)N * m, N, zero, G + (size_t)msub * m, msub, gw, gws));
     hipLaunchKernelGGL(k_mirror_block, dim3(nn, (m + 255) / 256), dim3(256), 0, s, m, m + nn, G, msub);
     m += nn;
   }

@@ -181,6 +181,134 @@ __global__ __launch_bounds__(256) void k_chol_inv(int b, zc* __restrict__ G, int
   }
 }
 
+// ---- blocked Cholesky and inverse factor (b <= 512) -------------------------------------
+// k_chol_left: one workgroup of 1024 threads, left-looking by panels of CP columns.  The
+// panel (rows p..b-1) is staged in LDS beside the rows L(p:p+CP, 0:p) that update it; the
+// update with the previous columns runs in four K quarters added in a fixed order, then
+// the panel is factored in LDS (same pivot rule as k_chol_inv) and written back.
+#define CP 8
+__global__ __launch_bounds__(1024) void k_chol_left(int b, zc* __restrict__ G, int ld, int* __restrict__ keep,
+                                                    double tol) {
+  extern __shared__ zc sm[];
+  zc* P = sm;                      // P[lr * CP + c] = (p + lr, p + c)
+  zc* Lr = sm + (size_t)b * CP;    // Lr[k * CP + c] = L(p + c, k), k < p
+  __shared__ double s_red[16];
+  __shared__ double s_piv;
+  __shared__ int s_drop;
+  const int tid = threadIdx.x;
+  double mx = 0.0;
+  for (int i = tid; i < b; i += 1024) mx = fmax(mx, G[(size_t)i + (size_t)ld * i].re);
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((tid & 63) == 0) s_red[tid >> 6] = mx;
+  __syncthreads();
+  double dmax = 0.0;
+  for (int w = 0; w < 16; ++w) dmax = fmax(dmax, s_red[w]);
+  const double thr = tol * dmax;
+  const int ks = tid >> 8, rl = tid & 255;
+  for (int p = 0; p < b; p += CP) {
+    const int pc = min(CP, b - p), nr = b - p;
+    for (int e = tid; e < nr * pc; e += 1024) {
+      const int c = e / nr, lr = e - c * nr;
+      P[lr * CP + c] = (lr >= c) ? G[(size_t)(p + lr) + (size_t)ld * (p + c)] : mkz(0, 0);
+    }
+    for (int e = tid; e < p * pc; e += 1024) {
+      const int k = e / pc, c = e - k * pc;
+      Lr[k * CP + c] = G[(size_t)(p + c) + (size_t)ld * k];
+    }
+    __syncthreads();
+    if (p > 0) {
+      const int kq = (p + 3) / 4;
+      const int kb = ks * kq, ke = min(p, kb + kq);
+      for (int r0 = 0; r0 < nr; r0 += 256) {
+        const int lr = r0 + rl;
+        zc acc[CP];
+#pragma unroll
+        for (int c = 0; c < CP; ++c) acc[c] = mkz(0, 0);
+        if (lr < nr) {
+          for (int k = kb; k < ke; ++k) {
+            const zc l = G[(size_t)(p + lr) + (size_t)ld * k];
+#pragma unroll
+            for (int c = 0; c < CP; ++c) acc[c] = fma_conj(Lr[k * CP + c], l, acc[c]);   // l conj(L(p+c,k))
+          }
+        }
+        for (int q = 0; q < 4; ++q) {
+          if (ks == q && lr < nr) {
+#pragma unroll
+            for (int c = 0; c < CP; ++c)
+              if (c < pc) P[lr * CP + c] = P[lr * CP + c] - acc[c];
+          }
+          __syncthreads();
+        }
+      }
+    }
+    for (int c = 0; c < pc; ++c) {
+      if (tid == 0) {
+        const double pv = P[c * CP + c].re;
+        const int drop = !(pv > thr);
+        s_drop = drop;
+        s_piv = drop ? 1.0 : sqrt(pv);
+        keep[p + c] = drop ? 0 : 1;
+      }
+      __syncthreads();
+      const double piv = s_piv;
+      const int drop = s_drop;
+      for (int lr = c + tid; lr < nr; lr += 1024) {
+        const zc g = P[lr * CP + c];
+        P[lr * CP + c] = (lr == c) ? mkz(piv, 0.0) : (drop ? mkz(0, 0) : g * (1.0 / piv));
+      }
+      __syncthreads();
+      const int w = pc - c - 1;   // panel columns right of c
+      if (!drop && w > 0) {
+        for (int e = tid; e < (nr - c - 1) * w; e += 1024) {
+          const int lr = c + 1 + e / w, c2 = c + 1 + e % w;
+          if (lr >= c2) P[lr * CP + c2] = P[lr * CP + c2] - P[lr * CP + c] * conj(P[c2 * CP + c]);
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < nr * pc; e += 1024) {
+      const int c = e / nr, lr = e - c * nr;
+      if (lr >= c) G[(size_t)(p + lr) + (size_t)ld * (p + c)] = P[lr * CP + c];
+    }
+    __syncthreads();
+  }
+}
+
+// k_trinv: columns 16 J .. 16 J + 15 of L^-1 (L the lower factor left in G), one workgroup
+// per 16 columns: block forward substitution, 16 rows per step, X(:, J) kept in LDS.
+__global__ __launch_bounds__(256) void k_trinv(int b, const zc* __restrict__ G, int ld, zc* __restrict__ Linv) {
+  extern __shared__ zc X[];        // X[(i - c0) * 16 + c]
+  __shared__ zc Lb[16][17];
+  const int tid = threadIdx.x;
+  const int r = tid >> 4, c = tid & 15;
+  const int c0 = blockIdx.x * 16;
+  const int nc = min(16, b - c0);
+  for (int I0 = c0; I0 < b; I0 += 16) {
+    const int ni = min(16, b - I0);
+    zc acc = mkz(0, 0);
+    if (r < ni && c < nc)
+      for (int k = c0; k < I0; ++k) acc = acc + G[(size_t)(I0 + r) + (size_t)ld * k] * X[(k - c0) * 16 + c];
+    {
+      const int rr = tid & 15, kk = tid >> 4;
+      Lb[rr][kk] = (rr < ni && kk < ni && rr >= kk) ? G[(size_t)(I0 + rr) + (size_t)ld * (I0 + kk)] : mkz(0, 0);
+    }
+    __syncthreads();
+    const zc rhs = mkz((I0 + r == c0 + c) ? 1.0 : 0.0, 0.0) - acc;
+    for (int rr = 0; rr < ni; ++rr) {
+      if (r == rr && c < nc) {
+        zc v = rhs;
+        for (int kk = 0; kk < rr; ++kk) v = v - Lb[rr][kk] * X[(I0 + kk - c0) * 16 + c];
+        X[(I0 + rr - c0) * 16 + c] = v * (1.0 / Lb[rr][rr].re);
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < b * nc; e += 256) {
+    const int cc = e / b, i = e - cc * b;
+    Linv[(size_t)i + (size_t)b * (c0 + cc)] = (i >= c0) ? X[(i - c0) * 16 + cc] : mkz(0, 0);
+  }
+}
+
 // ---- Lanczos (Haydock) chains, nq columns at once ---------------------------------------
 // part 1: w = H v - b_j v_prev; partials of Re(v^H H v) and |w|^2 per workgroup
 __global__ __launch_bounds__(256) void k_lz_a(int N, const zc* __restrict__ V, const zc* __restrict__ Vp,
@@ -440,6 +568,16 @@ int launch_precond_diag(hipStream_t s, int N, int nq, const int* idx, const zc* 
   return WTB_OK;
 }
 int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, double tol) {
+  if (b <= 0) return WTB_OK;
+  if (b <= 512) {
+    const size_t lds = sizeof(zc) * (size_t)b * 16;
+    WTB_HIP(hipFuncSetAttribute((const void*)k_chol_left, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    WTB_HIP(hipFuncSetAttribute((const void*)k_trinv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_chol_left, dim3(1), dim3(1024), lds, s, b, G, ld, keep, tol);
+    hipLaunchKernelGGL(k_trinv, dim3((b + 15) / 16), dim3(256), lds, s, b, G, ld, Linv);
+    WTB_LAUNCH_CHECK();
+    return WTB_OK;
+  }
   hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(256), 0, s, b, G, ld, Linv, keep, tol);
   WTB_LAUNCH_CHECK();
   return WTB_OK;

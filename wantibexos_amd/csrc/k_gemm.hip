@@ -7,6 +7,8 @@
 // v_mfma_f64_16x16x4_f64; the precision form (F32) rounds both operands to FP32
 // and accumulates in FP32 on v_mfma_f32_16x16x4_f32.
 #include "wtb_kernels.h"
+#include <algorithm>
+#include <cstdint>
 
 namespace wtb {
 
@@ -123,83 +125,114 @@ __global__ void k_zgemm_reduce(int M, int N, int nz, const zc* __restrict__ part
 }
 
 int zgemm(hipStream_t s, char opa, char opb, int M, int N, int K, zc alpha, const zc* A, int lda,
-          const zc* B, int ldb, zc beta, zc* C, int ldc) {
+          const zc* B, int ldb, zc beta, zc* C, int ldc, zc* work, size_t work_elems) {
   if (M <= 0 || N <= 0) return WTB_OK;
   if (K <= 0) {
     set_error("zgemm: K = 0");
     return WTB_ERR_ARG;
   }
-  dim3 g((M + GT - 1) / GT, (N + GT - 1) / GT), b(256);
+  const int gx = (M + GT - 1) / GT, gy = (N + GT - 1) / GT;
+  // K slices: aim at ~1024 workgroups, slices of at least 256
+  int nz = 1, kc = K;
+  const size_t mn = (size_t)M * N;
+  if (work && gx * gy < 512 && K >= 512) {
+    nz = std::min((1024 + gx * gy - 1) / (gx * gy), K / 256);
+    nz = (int)std::min<size_t>((size_t)nz, work_elems / mn);
+    if (nz >= 2) {
+      kc = (((K + nz - 1) / nz + GK - 1) / GK) * GK;
+      nz = (K + kc - 1) / kc;
+    } else {
+      nz = 1;
+    }
+  }
+  dim3 g(gx, gy, nz), b(256);
   const bool ca = opa == 'C', cb = opb == 'C';
-  zc* np_ = nullptr;
-  if (!ca && !cb) hipLaunchKernelGGL((k_zgemm<0, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
-  else if (ca && !cb) hipLaunchKernelGGL((k_zgemm<1, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
-  else if (!ca && cb) hipLaunchKernelGGL((k_zgemm<0, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
-  else hipLaunchKernelGGL((k_zgemm<1, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, K, np_);
+  zc* pt = nz > 1 ? work : nullptr;
+  if (!ca && !cb) hipLaunchKernelGGL((k_zgemm<0, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kc, pt);
+  else if (ca && !cb) hipLaunchKernelGGL((k_zgemm<1, 0, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kc, pt);
+  else if (!ca && cb) hipLaunchKernelGGL((k_zgemm<0, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kc, pt);
+  else hipLaunchKernelGGL((k_zgemm<1, 1, zc, zc, false>), g, b, 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, kc, pt);
+  if (nz > 1)
+    hipLaunchKernelGGL(k_zgemm_reduce, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, M, N, nz, work, alpha,
+                       beta, C, ldc);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }
 
-// Thin product Y = H X (nc <= 16 columns) for the Lanczos chains: lane = row of a 64-row
-// block, the 4 waves of a workgroup split its K slice, X of the slice staged in LDS; every
-// load of H moves one column segment (512 B of COMPLEX*8 per wave).  FP32 accumulation for
-// the COMPLEX*8 matrix.  Partial sums of the K slices go to part[z] (ld N), then reduced.
-#define THIN_KC 256
+// Thin product Y = H X (nc <= 16 columns) for the Lanczos chains.  A workgroup covers
+// THIN_ROWS rows (wave w: 128 of them, two per lane, so every load of H moves 16 B per lane
+// and 1 KB of one column per wave) and THIN_KC columns of K, staged THIN_SUB at a time
+// through LDS as FP32 pairs.  FP32 accumulation for the COMPLEX*8 matrix; the partial sums
+// of the K slices go to part[z] (ld N) and are added in slice order.
+#define THIN_ROWS 512
+#define THIN_KC 512
+#define THIN_SUB 256
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_thin_hmm(int N, int nc, const zf* __restrict__ H, int ldh,
                                                   const zc* __restrict__ X, int ldx, zc* __restrict__ part) {
-  __shared__ float2 sm[THIN_KC * 16];            // X slice [k][16], then the 4 wave partials [w][64][16]
-  const int r0 = blockIdx.x * 64, k0 = blockIdx.y * THIN_KC;
-  const int kn = min(THIN_KC, N - k0);
+  __shared__ float2 sm[THIN_SUB * 16];            // X sub-slice [k][16]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int e = threadIdx.x; e < THIN_KC * 16; e += 256) {
-    const int kk = e >> 4, q = e & 15;
-    float2 v = make_float2(0.f, 0.f);
-    if (q < nc && kk < kn) {
-      const zc x = X[(size_t)(k0 + kk) + (size_t)ldx * q];
-      v = make_float2((float)x.re, (float)x.im);
-    }
-    sm[e] = v;
-  }
-  __syncthreads();
-  float ar[16], ai[16];
-  for (int q = 0; q < 16; ++q) ar[q] = ai[q] = 0.f;
-  const int r = r0 + lane;
+  const int r = blockIdx.x * THIN_ROWS + w * 128 + 2 * lane;   // rows r and r + 1
+  const int k0 = blockIdx.y * THIN_KC;
+  const int kend = min(N, k0 + THIN_KC);
   const bool rv = r < N;
-  // wave w takes k = w, w+4, ...; 8 loads in flight per lane
-  for (int kb = w; kb < THIN_KC; kb += 32) {
-    zf h[8];
+  float ar0[16], ai0[16], ar1[16], ai1[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int kk = kb + 4 * u;
-      h[u] = (rv && kk < kn) ? H[(size_t)r + (size_t)ldh * (k0 + kk)] : zf{0.f, 0.f};
+  for (int q = 0; q < 16; ++q) ar0[q] = ai0[q] = ar1[q] = ai1[q] = 0.f;
+  for (int ks = k0; ks < kend; ks += THIN_SUB) {
+    const int kn = min(THIN_SUB, kend - ks);
+    __syncthreads();
+    for (int e = threadIdx.x; e < THIN_SUB * 16; e += 256) {
+      const int kk = e >> 4, q = e & 15;
+      float2 v = make_float2(0.f, 0.f);
+      if (q < nc && kk < kn) {
+        const zc x = X[(size_t)(ks + kk) + (size_t)ldx * q];
+        v = make_float2((float)x.re, (float)x.im);
+      }
+      sm[e] = v;
     }
+    __syncthreads();
+    if (!rv) continue;
+    for (int kb = 0; kb < kn; kb += 8) {
+      float4 h[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int kk = kb + 4 * u;
-      if (kk >= THIN_KC) break;
+      for (int u = 0; u < 8; ++u) {
+        const int kk = kb + u;
+        const zf* src = H + (size_t)r + (size_t)ldh * (ks + kk);
+        if (kk < kn) {
+          if (VEC) {
+            h[u] = *reinterpret_cast<const float4*>(src);
+          } else {
+            const zf a0 = src[0];
+            const zf a1 = (r + 1 < N) ? src[1] : zf{0.f, 0.f};
+            h[u] = make_float4(a0.re, a0.im, a1.re, a1.im);
+          }
+        } else {
+          h[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        if (q < nc) {
-          const float2 x = sm[kk * 16 + q];
-          ar[q] = fmaf(h[u].re, x.x, fmaf(-h[u].im, x.y, ar[q]));
-          ai[q] = fmaf(h[u].re, x.y, fmaf(h[u].im, x.x, ai[q]));
+      for (int u = 0; u < 8; ++u) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          if (q < nc) {
+            const float2 x = sm[(kb + u) * 16 + q];
+            ar0[q] = fmaf(h[u].x, x.x, fmaf(-h[u].y, x.y, ar0[q]));
+            ai0[q] = fmaf(h[u].x, x.y, fmaf(h[u].y, x.x, ai0[q]));
+            ar1[q] = fmaf(h[u].z, x.x, fmaf(-h[u].w, x.y, ar1[q]));
+            ai1[q] = fmaf(h[u].z, x.y, fmaf(h[u].w, x.x, ai1[q]));
+          }
         }
       }
     }
   }
-  __syncthreads();
-  for (int q = 0; q < 16; ++q) sm[(w * 64 + lane) * 16 + q] = make_float2(ar[q], ai[q]);
-  __syncthreads();
-  // wave 0 adds the four K quarters in order and writes the slice's partial
-  if (w == 0 && rv) {
-    for (int q = 0; q < nc; ++q) {
-      float sr = 0.f, si = 0.f;
-      for (int ww = 0; ww < 4; ++ww) {
-        const float2 v = sm[(ww * 64 + lane) * 16 + q];
-        sr += v.x;
-        si += v.y;
-      }
-      part[(size_t)blockIdx.y * N * nc + (size_t)r + (size_t)N * q] = mkz((double)sr, (double)si);
+  if (!rv) return;
+  zc* dst = part + (size_t)blockIdx.y * N * nc;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    if (q < nc) {
+      dst[(size_t)r + (size_t)N * q] = mkz((double)ar0[q], (double)ai0[q]);
+      if (r + 1 < N) dst[(size_t)r + 1 + (size_t)N * q] = mkz((double)ar1[q], (double)ai1[q]);
     }
   }
 }
@@ -215,7 +248,12 @@ int hmm_thin_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X
     return WTB_ERR_ARG;
   }
   const int nz = (N + THIN_KC - 1) / THIN_KC;
-  hipLaunchKernelGGL(k_thin_hmm, dim3((N + 63) / 64, nz), dim3(256), 0, s, N, nc, H, ldh, X, ldx, work);
+  const dim3 g((N + THIN_ROWS - 1) / THIN_ROWS, nz);
+  // 16-byte loads of a row pair need an even leading dimension and a 16-byte aligned base
+  if ((ldh % 2) == 0 && (reinterpret_cast<uintptr_t>(H) % 16) == 0)
+    hipLaunchKernelGGL(k_thin_hmm<true>, g, dim3(256), 0, s, N, nc, H, ldh, X, ldx, work);
+  else
+    hipLaunchKernelGGL(k_thin_hmm<false>, g, dim3(256), 0, s, N, nc, H, ldh, X, ldx, work);
   const size_t tot = (size_t)N * nc;
   hipLaunchKernelGGL(k_zgemm_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, nc, nz, work,
                      mkz(1, 0), mkz(0, 0), Y, ldy);

@@ -97,9 +97,11 @@ int run_backtr(hipStream_t s, const zc* H, int ld, int N, const zc* tau, const d
 int run_backtr_f32(hipStream_t s, const zf* H, int ld, int N, const zc* tau, const double* Z, int c0,
                    int m, zc* X, int ldx, void* work);
 
-// complex GEMM C = alpha op(A) op(B) + beta C, op 'N' or 'C' (FP64 MFMA)
+// complex GEMM C = alpha op(A) op(B) + beta C, op 'N' or 'C' (FP64 MFMA).  With a
+// workspace of work_elems complex entries, a long K over few output tiles is split into
+// slices whose partial sums are added in slice order (deterministic).
 int zgemm(hipStream_t s, char opa, char opb, int M, int N, int K, zc alpha, const zc* A, int lda,
-          const zc* B, int ldb, zc beta, zc* C, int ldc);
+          const zc* B, int ldb, zc beta, zc* C, int ldc, zc* work = nullptr, size_t work_elems = 0);
 // Y = H X for a full Hermitian H: COMPLEX*8 operand on FP32 MFMA, or FP64; work (optional,
 // hmm_work_bytes) lets a thin product split K over more workgroups
 size_t hmm_work_bytes(int N, int nc);

@@ -106,9 +106,12 @@ __device__ inline zc alpha_from_puv(const zc* __restrict__ puv, int np, zc tau) 
 // Reflector jm's scalars from the partials of hemvstep(jm) (panel column jjh): ZLARFG on
 // alpha = x_new(jm+1) with |x|^2 = sum |x~|^2, then W^H v = conj(W(jm+1,:)) + s W^H x~,
 // V^H v likewise and v^T g, into the workgroup's shared arrays.  256 threads.
+// The caller preloads (before any barrier) alpha = x_new(jm+1) in thread 0, W(jm+1, c) and
+// V(jm+1, c) in thread c < jjh, and g(q, jm+1) as hemvstep(jm) saw it in thread 64 + q.
 template <class T>
 __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums, zc* st1, zc* st2, zc* svg,
-                                  zc* s_tau, zc* s_s, bool write_global) {
+                                  zc* s_tau, zc* s_s, bool write_global, zc alpha_pre, zc w_pre, zc v_pre,
+                                  zc g_pre) {
   const int tid = threadIdx.x;
   const int N = a.N;
   const int b0h = (jm + 1) / TB;
@@ -143,7 +146,7 @@ __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums
   __syncthreads();
   if (tid == 0) {
     const double xn2 = sums[0].re;
-    const zc alpha = a.xn[jm + 1];
+    const zc alpha = alpha_pre;
     zc tau, sc;
     double beta;
     if (xn2 == 0.0 && alpha.im == 0.0) {
@@ -166,15 +169,14 @@ __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums
   }
   __syncthreads();
   const zc sc = *s_s;
-  const int j1 = jm + 1;
   if (tid < jjh) {
     const int c = tid;
-    st1[c] = conj(a.W[(size_t)c * N + j1]) + sc * sums[1 + c];
-    st2[c] = conj(a.Vp[(size_t)c * N + j1]) + sc * sums[1 + jjh + c];
+    st1[c] = conj(w_pre) + sc * sums[1 + c];
+    st2[c] = conj(v_pre) + sc * sums[1 + jjh + c];
   }
   if (tid >= 64 && tid < 64 + a.nrhs) {
     const int q = tid - 64;
-    svg[q] = a.gs[q] + sc * sums[1 + 2 * jjh + q];
+    svg[q] = g_pre + sc * sums[1 + 2 * jjh + q];
   }
   __syncthreads();
 }
@@ -226,7 +228,18 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
       }
     }
   }
-  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, blockIdx.x == 0);
+  // reflector jm's inputs and this row's x_new and g, none of which depends on its scalars
+  zc alpha_pre = mkz(0, 0), w_pre = mkz(0, 0), v_pre = mkz(0, 0), g_pre = mkz(0, 0);
+  if (tid == 0) alpha_pre = a.xn[jm + 1];
+  if (tid < jjm) {
+    w_pre = a.W[(size_t)tid * N + j];
+    v_pre = a.Vp[(size_t)tid * N + j];
+  }
+  if (tid >= 64 && tid < 64 + a.nrhs) g_pre = a.gs[tid - 64];
+  const zc xnr = (valid && r != j) ? a.xn[r] : mkz(0, 0);
+  const zc gr = (valid && sub < a.nrhs) ? a.g[(size_t)sub * N + r] : mkz(0, 0);
+  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, blockIdx.x == 0, alpha_pre, w_pre, v_pre,
+                    g_pre);
   const zc tau = s_tau, s = s_s;
   // s z(row) - V t1 - W t2, each row summed by its 16 lanes
   auto u_of = [&](zc ys, const zc* v2, const zc* w2) -> zc {
@@ -270,15 +283,14 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   if (valid) {
     const zc up = u_of(yr, vr, wr);
     uu = (r == j) ? su_j : hcol + up;
-    v = (r == j) ? mkz(1, 0) : s * a.xn[r];
+    v = (r == j) ? mkz(1, 0) : s * xnr;
     if (sub == 0) {
       a.Vp[(size_t)jjm * N + r] = v;
       a.u[r] = uu;
     }
     if (sub < a.nrhs) {
       const int q = sub;
-      zc gq = a.g[(size_t)q * N + r];
-      gq = gq - (tau * conj(v)) * svg[q];
+      const zc gq = gr - (tau * conj(v)) * svg[q];
       a.g[(size_t)q * N + r] = gq;
     }
   }
@@ -309,15 +321,18 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
 }
 
 // ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
-// 4 waves, wave w owns 16 columns, lane = row: every load moves 1 KB of one column.
+// 4 waves, wave w owns 16 columns, lane = row: every load moves one column segment.
 template <class T>
 __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
-  __shared__ zc xR[TB], xC[TB];
-  __shared__ zc y1s[4][TB], y2s[1][TB];
+  constexpr int XRN = TB;                   // rows of x~ a workgroup needs
+  constexpr int XT = XRN + TB;              // threads loading x~ (rows, then columns)
+  __shared__ zc xR[XRN], xC[TB];
+  __shared__ zc y1s[4][XRN], y2s[1][TB];
+  __shared__ zc sWl[TB];
   __shared__ zc s_ap;
   const int N = a.N;
   const int b0 = (j + 1) / TB;
-  int t = blockIdx.x;
+  const int t = blockIdx.x;
   int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
@@ -326,30 +341,52 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   const int tid = threadIdx.x;
   const int R0 = bi * TB, C0 = bj * TB;
   const bool diag = bi == bj;
+  const bool first = bj == b0 && diag;      // the workgroup holding row block b0's diagonal
   const int lane = tid & 63, w = tid >> 6;
-  const int r = R0 + lane;
-  const bool rvalid = r > j && r < N;
-  // the tile and the raw pieces of x_new first: none of them depends on alpha'
+  const int r = R0 + lane;                  // the diagonal epilogue's row
+  // the tile and the raw pieces of x_new first: none of them depends on alpha'.  The loads
+  // are unconditional (masked-off elements read a clamped in-range address) and the mask is
+  // applied when the values are used, so that all 16 stay in flight together.
   zc hv[16];
+  T hraw[16];
+  unsigned okm = 0;
+  {
+    const bool rvalid = r > j && r < N;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int cl = 16 * w + k, c = C0 + cl;
-    zc x = mkz(0, 0);
-    if (rvalid && c > j && c < N && (!diag || lane >= cl)) x = ldz(a.H[(size_t)r + (size_t)a.ld * c]);
-    hv[k] = x;
+    for (int k = 0; k < 16; ++k) {
+      const int cl = 16 * w + k, c = C0 + cl;
+      const bool ok = rvalid && c > j && c < N && (!diag || lane >= cl);
+      const size_t off = ok ? (size_t)r + (size_t)a.ld * c : 0;
+      hraw[k] = a.H[off];
+      okm |= (unsigned)ok << k;
+    }
   }
-  // x_new = x_p - 2 Re(alpha') v_{j-1} (or column j of H at a panel's first column)
-  const int xrow = (tid < TB) ? R0 + tid : C0 + tid - TB;
-  const bool xok = tid < 2 * TB && xrow >= j + 2 && xrow < N;
-  zc x0 = mkz(0, 0), x1 = mkz(0, 0);
+  // x_new = x_p - 2 Re(alpha') v_{j-1} (or column j of H at a panel's first column), rows
+  // >= j+1 (x~ leaves out row j+1); diagonal tiles also load what their epilogue writes:
+  // u (for W(:,jj-1)), row j of x_p and v (for d_j) and row j+1 of g
+  const int xrow = (tid < XRN) ? R0 + tid : C0 + tid - XRN;
+  const bool xok = tid < XT && xrow >= j + 1 && xrow < N;
+  zc x0 = mkz(0, 0), x1 = mkz(0, 0), uo = mkz(0, 0);
   if (xok) {
     if (jj == 0) {
       x0 = ldz(a.H[(size_t)xrow + (size_t)a.ld * j]);
     } else {
       x0 = a.xp[xrow];
       x1 = a.Vp[(size_t)(jj - 1) * N + xrow];
+      if (diag && tid < TB) uo = a.u[xrow];
     }
   }
+  const bool dj = first && tid == 255;
+  const bool gsl = first && tid >= XT && tid < XT + a.nrhs;
+  if (dj) {
+    if (jj == 0) {
+      x0 = ldz(a.H[(size_t)j + (size_t)a.ld * j]);
+    } else {
+      x0 = a.xp[j];
+      x1 = a.Vp[(size_t)(jj - 1) * N + j];
+    }
+  }
+  if (gsl) x0 = a.g[(size_t)(tid - XT) * N + (j + 1)];
   const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
   if (jj > 0) {
     const zc al = alpha_from_puv(a.puv, nprev, taup);
@@ -360,31 +397,51 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   __syncthreads();
   const zc ap = s_ap;
   const double sig = 2.0 * ap.re;
-  auto xnew = [&](int row) -> zc {
-    if (jj == 0) return ldz(a.H[(size_t)row + (size_t)a.ld * j]);
-    return a.xp[row] - sig * a.Vp[(size_t)(jj - 1) * N + row];
-  };
-  if (tid < 2 * TB) {
-    const zc xv = xok ? x0 - sig * x1 : mkz(0, 0);
-    if (tid < TB) xR[tid] = xv;
-    else xC[tid - TB] = xv;
+  const zc xv = x0 - sig * x1;                       // x_new(xrow) where xok
+  if (tid < XT) {
+    const zc xt = (xok && xrow >= j + 2) ? xv : mkz(0, 0);
+    if (tid < XRN) xR[tid] = xt;
+    else xC[tid - XRN] = xt;
   }
+  // W(r, jj-1) = tau' u(r) + alpha' v(r): written by diagonal tiles, used in their partials
+  const zc wo = (jj > 0 && xok) ? taup * uo + ap * x1 : mkz(0, 0);
+  if (diag && tid < TB) sWl[tid] = wo;
   __syncthreads();
   const zc xr = xR[lane];
   zc y1 = mkz(0, 0);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int cl = 16 * w + k;
-    zc x = hv[k];
+    zc x = ((okm >> k) & 1u) ? ldz(hraw[k]) : mkz(0, 0);
     if (diag && lane == cl) x.im = 0.0;                   // diagonal: real part (ZHEMV)
     y1 = fma_z(x, xC[cl], y1);
     hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
   }
+  y1s[w][lane] = y1;
   // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
   BUTTERFLY16(hv, lane);
   // lane holds the sum for column 16w + ((lane >> 2) & 15)
-  y1s[w][lane] = y1;
   if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
+  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g.  Wave w takes
+  // groups of 16 quantities (q0 = 16 w, 16 w + 64): lane = row.  The first group's operands
+  // are loaded here, before the barrier, so that the loads overlap the tile's epilogue.
+  const int nq = 1 + 2 * jj + a.nrhs;
+  const bool rok = r >= j + 2 && r < N;
+  auto operand = [&](int q) -> const zc* {
+    // the loaded operand: W(r, q-1), V(r, q-1-jj) or g(q-1-2jj, r); W(r, jj-1) comes from LDS
+    if (q >= 1 && q < jj) return a.W + (size_t)(q - 1) * N + r;
+    if (q > jj && q <= 2 * jj) return a.Vp + (size_t)(q - 1 - jj) * N + r;
+    if (q > 2 * jj && q < nq) return a.g + (size_t)(q - 1 - 2 * jj) * N + r;
+    return nullptr;
+  };
+  zc pv[16];
+  if (diag) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const zc* src = operand(16 * w + k);
+      pv[k] = (src && rok) ? *src : mkz(0, 0);
+    }
+  }
   __syncthreads();
   zc* yp = a.ypart;
   if (tid < TB) {
@@ -395,36 +452,28 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
     yp[((size_t)bj * a.nblk + bi) * TB + (tid - TB)] = y2s[0][tid - TB];
   }
   if (!diag) return;
-  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, partial dot products --------------------
-  if (tid < TB) {
-    const int rr = R0 + tid;
-    if (rr >= j + 1 && rr < N) {
-      a.xn[rr] = xnew(rr);
-      if (jj > 0) a.W[(size_t)(jj - 1) * N + rr] = taup * a.u[rr] + ap * a.Vp[(size_t)(jj - 1) * N + rr];
-    }
+  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, g(j+1), partial dot products ------------
+  if (xok && tid < TB) {
+    a.xn[xrow] = xv;
+    if (jj > 0) a.W[(size_t)(jj - 1) * N + xrow] = wo;
   }
-  if (ti == 0 && tid == TB) a.d[j] = xnew(j).re;   // row j may lie in the block before b0
+  if (dj) a.d[j] = xv.re;                          // row j may lie in the block before b0
   // row j+1 (block b0) of g, for the next rowstep's v^T g, which also updates that row
-  if (ti == 0 && tid >= 2 * TB && tid < 2 * TB + a.nrhs) a.gs[tid - 2 * TB] = a.g[(size_t)(tid - 2 * TB) * N + (j + 1)];
-  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g.  Wave w takes
-  // groups of 16 quantities (q0 = 16 w, 16 w + 64): lane = row, all 16 loads issued
-  // together, the products summed over the 64 rows by the same butterfly; afterwards
-  // lane l holds quantity q0 + ((l >> 2) & 15).
-  const int nq = 1 + 2 * jj + a.nrhs;
-  const bool rok = r >= j + 2 && r < N;
-  const zc wl = (jj > 0 && rok) ? taup * a.u[r] + ap * a.Vp[(size_t)(jj - 1) * N + r] : mkz(0, 0);
+  if (gsl) a.gs[tid - XT] = x0;
+  const zc wl = rok ? sWl[lane] : mkz(0, 0);
   const double xr2 = norm2(xr);
   for (int q0 = 16 * w; q0 < nq; q0 += 64) {
-    zc pv[16];
+    if (q0 > 16 * w) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const zc* src = operand(q0 + k);
+        pv[k] = (src && rok) ? *src : mkz(0, 0);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int q = q0 + k;
-      // the loaded operand: W(r, q-1), V(r, q-1-jj) or g(q-1-2jj, r); W(r, jj-1) is wl
-      const zc* src = nullptr;
-      if (q >= 1 && q < jj) src = a.W + (size_t)(q - 1) * N + r;
-      else if (q > jj && q <= 2 * jj) src = a.Vp + (size_t)(q - 1 - jj) * N + r;
-      else if (q > 2 * jj && q < nq) src = a.g + (size_t)(q - 1 - 2 * jj) * N + r;
-      const zc val = (src && rok) ? *src : ((q == jj) ? wl : mkz(0, 0));
+      const zc val = (q == jj) ? wl : pv[k];
       zc v = (q <= 2 * jj) ? fma_conj(val, xr, mkz(0, 0)) : fma_z(xr, val, mkz(0, 0));
       if (q == 0) v = mkz(xr2, 0.0);
       pv[k] = (q < nq && rok) ? v : mkz(0, 0);

@@ -237,6 +237,77 @@ __global__ __launch_bounds__(256) void k_thin_hmm(int N, int nc, const zf* __res
   }
 }
 
+// The same thin product on FP32 MFMA (v_mfma_f32_16x16x4_f32, complex as four real
+// products), for an even N and leading dimension: lane l loads rows (2i, 2i+1), i = l%16,
+// of column k0 + l/16 as one 16-byte pair; the even and the odd rows form two 16-row
+// blocks of the A operand, so a wave covers 64 rows (four blocks) per 4-column step and
+// the B operand (X, columns padded to 16) comes from LDS.  Same K slices and partials as
+// k_thin_hmm.
+#define THIN_MROWS 256   // rows per workgroup of k_thin_mfma: 4 waves x 64
+__global__ __launch_bounds__(256) void k_thin_mfma(int N, int nc, const zf* __restrict__ H, int ldh,
+                                                   const zc* __restrict__ X, int ldx, zc* __restrict__ part) {
+  __shared__ float2 sm[THIN_SUB * 16];            // X sub-slice [k][16]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kq = lane >> 4;
+  const int R0 = blockIdx.x * THIN_MROWS + w * 64;
+  const int k0 = blockIdx.y * THIN_KC;
+  const int kend = min(N, k0 + THIN_KC);
+  // rows of the two 32-row halves this lane loads (clamped in range; their results are dropped)
+  const int ra = R0 + 2 * i, rb = R0 + 32 + 2 * i;
+  const zf* pa = H + (ra < N ? ra : 0);
+  const zf* pb = H + (rb < N ? rb : 0);
+  f4 cr[4], ci[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) cr[b] = ci[b] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int ks = k0; ks < kend; ks += THIN_SUB) {
+    const int kn = min(THIN_SUB, kend - ks);
+    __syncthreads();
+    for (int e = threadIdx.x; e < THIN_SUB * 16; e += 256) {
+      const int kk = e >> 4, q = e & 15;
+      float2 v = make_float2(0.f, 0.f);
+      if (q < nc && kk < kn) {
+        const zc x = X[(size_t)(ks + kk) + (size_t)ldx * q];
+        v = make_float2((float)x.re, (float)x.im);
+      }
+      sm[e] = v;
+    }
+    __syncthreads();
+    for (int kb = 0; kb < kn; kb += 16) {
+      float4 av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int col = ks + kb + 4 * u + kq;
+        col = col < N ? col : N - 1;                // staged X is zero there
+        av[u] = *reinterpret_cast<const float4*>(pa + (size_t)ldh * col);
+        bv[u] = *reinterpret_cast<const float4*>(pb + (size_t)ldh * col);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float2 x = sm[(kb + 4 * u + kq) * 16 + i];
+        const float xr = x.x, xi = x.y, nxi = -x.y;
+        const float A[4][2] = {{av[u].x, av[u].y}, {av[u].z, av[u].w}, {bv[u].x, bv[u].y}, {bv[u].z, bv[u].w}};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          cr[b] = mfma16x16x4f32(A[b][0], xr, cr[b]);
+          cr[b] = mfma16x16x4f32(A[b][1], nxi, cr[b]);
+          ci[b] = mfma16x16x4f32(A[b][0], xi, ci[b]);
+          ci[b] = mfma16x16x4f32(A[b][1], xr, ci[b]);
+        }
+      }
+    }
+  }
+  if (i >= nc) return;
+  zc* dst = part + (size_t)blockIdx.y * N * nc + (size_t)N * i;
+  // block b = 2 h + parity; register q holds block row 4 kq + q = local pair index
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = R0 + 32 * (b >> 1) + 2 * (4 * kq + q) + (b & 1);
+      if (row < N) dst[row] = mkz((double)cr[b][q], (double)ci[b][q]);
+    }
+}
+
 size_t thin_work_bytes(int N, int nc) {
   return sizeof(zc) * (size_t)((N + THIN_KC - 1) / THIN_KC) * N * nc + 256;
 }
@@ -250,7 +321,11 @@ int hmm_thin_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X
   const int nz = (N + THIN_KC - 1) / THIN_KC;
   const dim3 g((N + THIN_ROWS - 1) / THIN_ROWS, nz);
   // 16-byte loads of a row pair need an even leading dimension and a 16-byte aligned base
-  if ((ldh % 2) == 0 && (reinterpret_cast<uintptr_t>(H) % 16) == 0)
+  const bool pairs = (ldh % 2) == 0 && (reinterpret_cast<uintptr_t>(H) % 16) == 0;
+  if (pairs && (N % 2) == 0)
+    hipLaunchKernelGGL(k_thin_mfma, dim3((N + THIN_MROWS - 1) / THIN_MROWS, nz), dim3(256), 0, s, N, nc, H, ldh, X,
+                       ldx, work);
+  else if (pairs)
     hipLaunchKernelGGL(k_thin_hmm<true>, g, dim3(256), 0, s, N, nc, H, ldh, X, ldx, work);
   else
     hipLaunchKernelGGL(k_thin_hmm<false>, g, dim3(256), 0, s, N, nc, H, ldh, X, ldx, work);

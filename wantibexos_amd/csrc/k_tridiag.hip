@@ -347,10 +347,17 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   // the tile and the raw pieces of x_new first: none of them depends on alpha'.  The loads
   // are unconditional (masked-off elements read a clamped in-range address) and the mask is
   // applied when the values are used, so that all 16 stay in flight together.
+  // Interior tiles (off the diagonal, inside the trailing matrix: the common case) need no
+  // masks at all.
+  const bool interior = !diag && C0 > j && R0 + TB <= N && C0 + TB <= N;
   zc hv[16];
   T hraw[16];
   unsigned okm = 0;
-  {
+  if (interior) {
+    const T* col = a.H + (size_t)r + (size_t)a.ld * (C0 + 16 * w);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) hraw[k] = col[(size_t)a.ld * k];
+  } else {
     const bool rvalid = r > j && r < N;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -409,6 +416,7 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   __syncthreads();
   const zc xr = xR[lane];
   zc y1 = mkz(0, 0);
+  if (interior) okm = 0xffffu;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int cl = 16 * w + k;

@@ -12,7 +12,6 @@
 namespace wtb {
 
 #define BT_NB 32      // reflectors per panel (the reduction's panel width)
-#define BT_CHUNK 256  // rows per V^H X partial
 
 template <class T>
 __device__ inline zc vref(const T* H, int ld, int r, int j) {
@@ -86,11 +85,11 @@ __global__ __launch_bounds__(256) void k_bt_tfactor(const T* __restrict__ H, int
 // Workgroup: 32 reflectors x 64 columns; wave w: reflectors 16 (w >> 1), columns 32 (w & 1).
 template <class T>
 __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld, int N, int p, int kp,
-                                                const zc* __restrict__ X, int ldx, int m,
+                                                const zc* __restrict__ X, int ldx, int m, int chunk,
                                                 zc* __restrict__ Ypart) {
   const int col0 = blockIdx.x * 64;
-  const int rbeg = p + 1 + blockIdx.y * BT_CHUNK;
-  const int rend = min(N, rbeg + BT_CHUNK);
+  const int rbeg = p + 1 + blockIdx.y * chunk;
+  const int rend = min(N, rbeg + chunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int cb = (wave >> 1) * 16;           // reflector block
@@ -129,23 +128,37 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
   }
 }
 
-// Y(:, col) = T sum_chunks Ypart(:, col), one thread per column
-__global__ __launch_bounds__(64) void k_bt_ty(const zc* __restrict__ Ypart, int nchunk, int m, int kp,
-                                              const zc* __restrict__ Tf, zc* __restrict__ Y) {
+// Y(:, col) = T sum_chunks Ypart(:, col) for 64 columns per workgroup: the chunk sums go
+// to LDS (8 per thread, chunks added in order), then thread (col, 8 rows) applies T.
+__global__ __launch_bounds__(256) void k_bt_ty(const zc* __restrict__ Ypart, int nchunk, int m, int kp,
+                                               const zc* __restrict__ Tf, zc* __restrict__ Y) {
   __shared__ zc Ts[BT_NB * BT_NB];
-  for (int e = threadIdx.x; e < BT_NB * BT_NB; e += 64) Ts[e] = Tf[e];
-  __syncthreads();
-  const int col = blockIdx.x * 64 + threadIdx.x;
-  if (col >= m) return;
-  zc s[BT_NB];
-  for (int c = 0; c < BT_NB; ++c) s[c] = mkz(0, 0);
-  for (int ch = 0; ch < nchunk; ++ch)
-    for (int c = 0; c < kp; ++c) s[c] += Ypart[((size_t)ch * BT_NB + c) * m + col];
-  for (int r = 0; r < BT_NB; ++r) {
+  __shared__ zc Ss[BT_NB][64];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < BT_NB * BT_NB; e += 256) Ts[e] = Tf[e];
+  const int cl = tid & 63, col = blockIdx.x * 64 + cl;
+  for (int c = tid >> 6; c < BT_NB; c += 4) {
     zc acc = mkz(0, 0);
-    for (int c = r; c < kp; ++c) acc = fma_z(Ts[r + BT_NB * c], s[c], acc);   // T column-major
+    if (c < kp && col < m)
+      for (int ch = 0; ch < nchunk; ++ch) acc += Ypart[((size_t)ch * BT_NB + c) * m + col];
+    Ss[c][cl] = acc;
+  }
+  __syncthreads();
+  if (col >= m) return;
+  for (int r = (tid >> 6) * 8; r < (tid >> 6) * 8 + 8; ++r) {
+    zc acc = mkz(0, 0);
+    for (int c = r; c < kp; ++c) acc = fma_z(Ts[r + BT_NB * c], Ss[c][cl], acc);   // T column-major
     Y[(size_t)r * m + col] = acc;
   }
+}
+
+// rows per V^H X chunk: 256, or down to 64 when the columns alone give few workgroups
+static int bt_chunk(int rows, int ncol) {
+  int nch = (rows + 255) / 256;
+  if (ncol * nch >= 512) return 256;
+  nch = std::min((768 + ncol - 1) / ncol, (rows + 63) / 64);
+  const int ch = (rows + nch - 1) / nch;
+  return std::max(64, std::min(256, (ch + 3) & ~3));
 }
 
 // X(r, col) -= sum_c V(r, c) Y(c, col) for rows r > p; 64 x 64 tiles, waves of 32 x 32
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(256) void k_bt_phase(int N, int m, zc* __restrict__
 
 size_t backtr_work_bytes(int N, int m) {
   const size_t np = (N + BT_NB - 1) / BT_NB + 1;
-  const size_t nch = (N + BT_CHUNK - 1) / BT_CHUNK + 1;
+  const size_t nch = (N + 63) / 64 + 1;          // chunks of at least 64 rows
   return sizeof(zc) * (np * BT_NB * BT_NB + nch * BT_NB * (size_t)m + BT_NB * (size_t)m) + 3 * 256;
 }
 
@@ -237,7 +250,7 @@ static int run_backtr_t(hipStream_t s, const T* H, int ld, int N, const zc* tau,
   zc* Tf = (zc*)w;
   w += ((sizeof(zc) * (size_t)(np + 1) * BT_NB * BT_NB + 255) & ~size_t(255));
   zc* Ypart = (zc*)w;
-  const size_t nchmax = (N + BT_CHUNK - 1) / BT_CHUNK + 1;
+  const size_t nchmax = (N + 63) / 64 + 1;
   w += ((sizeof(zc) * nchmax * BT_NB * (size_t)m + 255) & ~size_t(255));
   zc* Y = (zc*)w;
   const int gl = (int)std::min<size_t>(((size_t)N * m + 255) / 256, 65536);
@@ -249,9 +262,11 @@ static int run_backtr_t(hipStream_t s, const T* H, int ld, int N, const zc* tau,
       const int p = b * BT_NB;
       const int kp = std::min(BT_NB, N - 1 - p);
       const int rows = N - p - 1;
-      const int nch = (rows + BT_CHUNK - 1) / BT_CHUNK;
-      hipLaunchKernelGGL(k_bt_vhx<T>, dim3(ncol, nch), dim3(256), 0, s, H, ld, N, p, kp, X, ldx, m, Ypart);
-      hipLaunchKernelGGL(k_bt_ty, dim3(ncol), dim3(64), 0, s, Ypart, nch, m, kp,
+      const int chunk = bt_chunk(rows, ncol);
+      const int nch = (rows + chunk - 1) / chunk;
+      hipLaunchKernelGGL(k_bt_vhx<T>, dim3(ncol, nch), dim3(256), 0, s, H, ld, N, p, kp, X, ldx, m, chunk,
+                         Ypart);
+      hipLaunchKernelGGL(k_bt_ty, dim3(ncol), dim3(256), 0, s, Ypart, nch, m, kp,
                          Tf + (size_t)b * BT_NB * BT_NB, Y);
       hipLaunchKernelGGL(k_bt_update<T>, dim3((rows + 63) / 64, ncol), dim3(256), 0, s, H, ld, N, p, kp,
                          Y, X, ldx, m);

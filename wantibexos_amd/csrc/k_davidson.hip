@@ -182,11 +182,11 @@ __global__ __launch_bounds__(256) void k_chol_inv(int b, zc* __restrict__ G, int
 }
 
 // ---- blocked Cholesky and inverse factor (b <= 512) -------------------------------------
-// k_chol_left: one workgroup of 1024 threads, left-looking by panels of CP columns.  The
+// k_chol_left: one workgroup of 1024 threads, left-looking by panels of CP (8 or 16) columns.  The
 // panel (rows p..b-1) is staged in LDS beside the rows L(p:p+CP, 0:p) that update it; the
 // update with the previous columns runs in four K quarters added in a fixed order, then
 // the panel is factored in LDS (same pivot rule as k_chol_inv) and written back.
-#define CP 8
+template <int CP>
 __global__ __launch_bounds__(1024) void k_chol_left(int b, zc* __restrict__ G, int ld, int* __restrict__ keep,
                                                     double tol) {
   extern __shared__ zc sm[];
@@ -570,10 +570,19 @@ int launch_precond_diag(hipStream_t s, int N, int nq, const int* idx, const zc* 
 int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, double tol) {
   if (b <= 0) return WTB_OK;
   if (b <= 512) {
+    // panels of 16 columns while the panel and its L rows (2 b 16 complex) fit in LDS
     const size_t lds = sizeof(zc) * (size_t)b * 16;
-    WTB_HIP(hipFuncSetAttribute((const void*)k_chol_left, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     WTB_HIP(hipFuncSetAttribute((const void*)k_trinv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_chol_left, dim3(1), dim3(1024), lds, s, b, G, ld, keep, tol);
+    if (b <= 288) {
+      const size_t lds16 = 2 * lds;
+      WTB_HIP(hipFuncSetAttribute((const void*)k_chol_left<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds16));
+      hipLaunchKernelGGL(k_chol_left<16>, dim3(1), dim3(1024), lds16, s, b, G, ld, keep, tol);
+    } else {
+      WTB_HIP(hipFuncSetAttribute((const void*)k_chol_left<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+      hipLaunchKernelGGL(k_chol_left<8>, dim3(1), dim3(1024), lds, s, b, G, ld, keep, tol);
+    }
     hipLaunchKernelGGL(k_trinv, dim3((b + 15) / 16), dim3(256), lds, s, b, G, ld, Linv);
     WTB_LAUNCH_CHECK();
     return WTB_OK;

@@ -125,7 +125,8 @@ def bench_davidson(args, cfg, model):
     plan = bse.BSEPlan(ctx, model, p32)
     N = plan.N
     log(f"iterative form: N={N} nev={args.nev} lanczos={args.lanczos_steps} learned={args.learned}")
-    kw = dict(nev=args.nev, guard=32, tol=args.tol, lanczos_steps=args.lanczos_steps, mlp=net)
+    kw = dict(nev=args.nev, guard=args.guard, tol=args.tol, lanczos_steps=args.lanczos_steps, mlp=net,
+              max_sub=args.max_sub)
     for _ in range(args.warmup):
         plan.lowest(**kw)
     times, r = [], None
@@ -166,7 +167,7 @@ def bench_davidson(args, cfg, model):
         "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 products, c128 Rayleigh-Ritz", "data": "synthetic (seeded random tight-binding model)",
         "config": {"workload": cfg["name"], "bse_dim": N, "nev": args.nev, "lanczos_steps": args.lanczos_steps,
-                   "tol_eV": args.tol,
+                   "tol_eV": args.tol, "guard": args.guard, "max_sub": args.max_sub,
                    "preconditioner": "learned" if args.learned else "diagonal"},
         "full_fp64_s": round(t_full, 6), "speedup_vs_full_fp64": round(t_full / dt, 3),
         "info": r["info"], "max_dW_eV": float(dW.max()), "max_resid_eV": float(r["resid"].max()),
@@ -188,6 +189,9 @@ def main():
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")
     ap.add_argument("--nev", type=int, default=256)
     ap.add_argument("--lanczos-steps", type=int, default=250)
+    ap.add_argument("--guard", type=int, default=32, help="iterative form: extra Ritz pairs carried along")
+    ap.add_argument("--max-sub", type=int, default=None,
+                    help="iterative form: basis size that triggers a restart (default 5 (nev + guard))")
     ap.add_argument("--tol", type=float, default=1e-6,
                     help="iterative form: residual norm (eV) below which an exciton counts as converged; "
                          "a Ritz value lies within its residual norm of an eigenvalue")

@@ -37,36 +37,40 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zc alpha, co
     }
   const int kbeg = blockIdx.z * kc;
   const int kend = min(K, kbeg + kc);
-  for (int k0 = kbeg; k0 < kend; k0 += GK) {
-    // stage op(A)(i0:i0+64, k0:k0+16)
-    if (OPA == 0) {
-      const int i = tid & 63;
-      for (int kk = tid >> 6; kk < GK; kk += 4) {
-        const int gi = i0 + i, gk = k0 + kk;
-        As[kk][i] = (gi < M && gk < kend) ? ldz(A[(size_t)gi + (size_t)lda * gk]) : mkz(0, 0);
+  // this thread's four staging slots of op(A) and op(B) per K slice; the next slice is
+  // loaded into registers while the current one is multiplied
+  zc ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (OPA == 0) {
+        const int gi = i0 + (tid & 63), gk = k0 + (tid >> 6) + 4 * u;
+        ra[u] = (gi < M && gk < kend) ? ldz(A[(size_t)gi + (size_t)lda * gk]) : mkz(0, 0);
+      } else {
+        const int gi = i0 + (tid >> 4) + 16 * u, gk = k0 + (tid & 15);
+        ra[u] = (gi < M && gk < kend) ? conj(ldz(A[(size_t)gk + (size_t)lda * gi])) : mkz(0, 0);
       }
-    } else {
-      const int kk = tid & 15;
-      for (int i = tid >> 4; i < GT; i += 16) {
-        const int gi = i0 + i, gk = k0 + kk;
-        As[kk][i] = (gi < M && gk < kend) ? conj(ldz(A[(size_t)gk + (size_t)lda * gi])) : mkz(0, 0);
+      if (OPB == 0) {
+        const int gj = j0 + (tid >> 4) + 16 * u, gk = k0 + (tid & 15);
+        rb[u] = (gj < N && gk < kend) ? ldz(B[(size_t)gk + (size_t)ldb * gj]) : mkz(0, 0);
+      } else {
+        const int gj = j0 + (tid & 63), gk = k0 + (tid >> 6) + 4 * u;
+        rb[u] = (gj < N && gk < kend) ? conj(ldz(B[(size_t)gj + (size_t)ldb * gk])) : mkz(0, 0);
       }
     }
-    // stage op(B)(k0:k0+16, j0:j0+64)
-    if (OPB == 0) {
-      const int kk = tid & 15;
-      for (int j = tid >> 4; j < GT; j += 16) {
-        const int gj = j0 + j, gk = k0 + kk;
-        Bs[kk][j] = (gj < N && gk < kend) ? ldz(B[(size_t)gk + (size_t)ldb * gj]) : mkz(0, 0);
-      }
-    } else {
-      const int j = tid & 63;
-      for (int kk = tid >> 6; kk < GK; kk += 4) {
-        const int gj = j0 + j, gk = k0 + kk;
-        Bs[kk][j] = (gj < N && gk < kend) ? conj(ldz(B[(size_t)gj + (size_t)ldb * gk])) : mkz(0, 0);
-      }
+  };
+  if (kbeg < kend) gload(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GK) {
+    __syncthreads();                                  // the previous slice is consumed
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (OPA == 0) As[(tid >> 6) + 4 * u][tid & 63] = ra[u];
+      else As[tid & 15][(tid >> 4) + 16 * u] = ra[u];
+      if (OPB == 0) Bs[tid & 15][(tid >> 4) + 16 * u] = rb[u];
+      else Bs[(tid >> 6) + 4 * u][tid & 63] = rb[u];
     }
     __syncthreads();
+    if (k0 + GK < kend) gload(k0 + GK);
     for (int kq = 0; kq < GK; kq += 4) {
       zc a[2], b[2];
       for (int x = 0; x < 2; ++x) a[x] = As[kq + lk][wi + x * 16 + lr];
@@ -87,7 +91,6 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zc alpha, co
           }
         }
     }
-    __syncthreads();
   }
   const bool use_beta = beta.re != 0.0 || beta.im != 0.0;
   for (int x = 0; x < 2; ++x)

@@ -599,72 +599,105 @@ size_t haydock_work_bytes(int N, int steps) {
          sizeof(int) * 64 + 16 * 256;
 }
 
-int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
-                const double* d_E1, double caux, double ebse0, double ebsef, int numbse, double sme,
-                int renorm, void* work, double* d_eps, double* d_smeavg, double* h_coef) {
-  const int nq = dtfull ? 9 : 3;
-  if (steps < 2 || N < 2) {
-    set_error("haydock: needs N >= 2 and at least 2 steps");
-    return WTB_ERR_ARG;
-  }
-  const int nblk = 128;
+namespace {
+// the chains' buffers, carved from the workspace in one fixed order (haydock_work_bytes)
+struct HaydockBufs {
+  zc* Vb[3];
+  zc *HV, *W, *hw;
+  double *part, *part2, *acoef, *bcoef, *hn2, *bcur, *sg;
+  int *ca, *cb;
+};
+constexpr int kLzBlocks = 128;
+HaydockBufs haydock_carve(void* work, int N, int steps, int nq) {
+  HaydockBufs b;
   char* w = static_cast<char*>(work);
   auto take = [&](size_t bytes) {
     char* q = w;
     w += (bytes + 255) & ~size_t(255);
     return q;
   };
-  zc* Vb[3];
-  for (int i = 0; i < 3; ++i) Vb[i] = (zc*)take(sizeof(zc) * (size_t)N * 9);
-  zc* HV = (zc*)take(sizeof(zc) * (size_t)N * 9);
-  zc* W = (zc*)take(sizeof(zc) * (size_t)N * 9);
-  double* part = (double*)take(sizeof(double) * 9 * nblk * 2);
-  double* part2 = (double*)take(sizeof(double) * 9 * nblk);
-  double* acoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
-  double* bcoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
-  double* hn2 = (double*)take(sizeof(double) * 16);
-  double* bcur = (double*)take(sizeof(double) * 16);
-  double* sg = (double*)take(sizeof(double) * 16);
-  int* ca = (int*)take(sizeof(int) * 16);
-  int* cb = (int*)take(sizeof(int) * 16);
-  zc* hw = (zc*)take(thin_work_bytes(N, nq));
-  // chains: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z
-  const int hca[9] = {0, 1, 2, 0, 0, 0, 0, 1, 1};
-  const int hcb[9] = {-1, -1, -1, 1, 1, 2, 2, 2, 2};
-  const double hsg[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
-  WTB_HIP(hipMemcpyAsync(ca, hca, sizeof(hca), hipMemcpyHostToDevice, s));
-  WTB_HIP(hipMemcpyAsync(cb, hcb, sizeof(hcb), hipMemcpyHostToDevice, s));
-  WTB_HIP(hipMemcpyAsync(sg, hsg, sizeof(hsg), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_lz_start, dim3((N + 255) / 256, nq), dim3(256), 0, s, N, nq, dip, ca, cb, sg, Vb[1]);
-  hipLaunchKernelGGL(k_lz_norms, dim3(nq), dim3(256), 0, s, N, Vb[1], hn2, Vb[0], bcur);
+  for (int i = 0; i < 3; ++i) b.Vb[i] = (zc*)take(sizeof(zc) * (size_t)N * 9);
+  b.HV = (zc*)take(sizeof(zc) * (size_t)N * 9);
+  b.W = (zc*)take(sizeof(zc) * (size_t)N * 9);
+  b.part = (double*)take(sizeof(double) * 9 * kLzBlocks * 2);
+  b.part2 = (double*)take(sizeof(double) * 9 * kLzBlocks);
+  b.acoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
+  b.bcoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
+  b.hn2 = (double*)take(sizeof(double) * 16);
+  b.bcur = (double*)take(sizeof(double) * 16);
+  b.sg = (double*)take(sizeof(double) * 16);
+  b.ca = (int*)take(sizeof(int) * 16);
+  b.cb = (int*)take(sizeof(int) * 16);
+  b.hw = (zc*)take(thin_work_bytes(N, nq));
+  return b;
+}
+}  // namespace
+
+int run_haydock_chains(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
+                       void* work) {
+  const int nq = dtfull ? 9 : 3;
+  if (steps < 2 || N < 2) {
+    set_error("haydock: needs N >= 2 and at least 2 steps");
+    return WTB_ERR_ARG;
+  }
+  HaydockBufs b = haydock_carve(work, N, steps, nq);
+  // chains: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z (static: the copies are asynchronous)
+  static const int hca[9] = {0, 1, 2, 0, 0, 0, 0, 1, 1};
+  static const int hcb[9] = {-1, -1, -1, 1, 1, 2, 2, 2, 2};
+  static const double hsg[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+  WTB_HIP(hipMemcpyAsync(b.ca, hca, sizeof(hca), hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(b.cb, hcb, sizeof(hcb), hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(b.sg, hsg, sizeof(hsg), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_lz_start, dim3((N + 255) / 256, nq), dim3(256), 0, s, N, nq, dip, b.ca, b.cb, b.sg, b.Vb[1]);
+  hipLaunchKernelGGL(k_lz_norms, dim3(nq), dim3(256), 0, s, N, b.Vb[1], b.hn2, b.Vb[0], b.bcur);
   WTB_LAUNCH_CHECK();
-  const int kmax = steps + 1;
+  const int kmax = steps + 1, nblk = kLzBlocks;
   int ip = 0, ic = 1, in = 2;
   for (int j = 0; j < steps; ++j) {
-    int rc = hmm_thin_f32(s, N, nq, H, ldh, Vb[ic], N, HV, N, hw);
+    int rc = hmm_thin_f32(s, N, nq, H, ldh, b.Vb[ic], N, b.HV, N, b.hw);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_lz_a, dim3(nblk, nq), dim3(256), 0, s, N, Vb[ic], Vb[ip], HV, bcur, W, part);
-    hipLaunchKernelGGL(k_lz_b, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, Vb[ic], W, part, nblk, acoef,
-                       Vb[in], part2);
-    hipLaunchKernelGGL(k_lz_c, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, part2, nblk, bcoef, bcur, Vb[in]);
+    hipLaunchKernelGGL(k_lz_a, dim3(nblk, nq), dim3(256), 0, s, N, b.Vb[ic], b.Vb[ip], b.HV, b.bcur, b.W, b.part);
+    hipLaunchKernelGGL(k_lz_b, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, b.Vb[ic], b.W, b.part, nblk, b.acoef,
+                       b.Vb[in], b.part2);
+    hipLaunchKernelGGL(k_lz_c, dim3(nblk, nq), dim3(256), 0, s, N, j, kmax, b.part2, nblk, b.bcoef, b.bcur,
+                       b.Vb[in]);
     const int t = ip;
     ip = ic;
     ic = in;
     in = t;
   }
-  hipLaunchKernelGGL(k_lz_gamma, dim3(1), dim3(1), 0, s, steps, acoef, bcoef, d_E1, N, d_smeavg);
-  hipLaunchKernelGGL(k_haydock_eps, dim3((numbse + 255) / 256), dim3(256), 0, s, nq, steps, kmax, acoef, bcoef,
-                     hn2, caux, ebse0, ebsef, numbse, sme, d_smeavg, dtfull, d_eps);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int run_haydock_eps(hipStream_t s, int N, int dtfull, int steps, const double* d_E1, double caux, double ebse0,
+                    double ebsef, int numbse, double sme, int renorm, void* work, double* d_eps, double* d_smeavg,
+                    double* h_coef) {
+  const int nq = dtfull ? 9 : 3;
+  HaydockBufs b = haydock_carve(work, N, steps, nq);
+  const int kmax = steps + 1;
+  hipLaunchKernelGGL(k_lz_gamma, dim3(1), dim3(1), 0, s, steps, b.acoef, b.bcoef, d_E1, N, d_smeavg);
+  hipLaunchKernelGGL(k_haydock_eps, dim3((numbse + 255) / 256), dim3(256), 0, s, nq, steps, kmax, b.acoef,
+                     b.bcoef, b.hn2, caux, ebse0, ebsef, numbse, sme, d_smeavg, dtfull, d_eps);
   WTB_LAUNCH_CHECK();
   if (renorm) {
     int rc = launch_imagrenorm(s, d_E1, ebse0, ebsef, numbse, d_eps);
     if (rc) return rc;
   }
   if (h_coef) {
-    WTB_HIP(hipMemcpyAsync(h_coef, acoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
-    WTB_HIP(hipMemcpyAsync(h_coef + 9 * kmax, bcoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
+    WTB_HIP(hipMemcpyAsync(h_coef, b.acoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
+    WTB_HIP(hipMemcpyAsync(h_coef + 9 * kmax, b.bcoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
   }
   return WTB_OK;
+}
+
+int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
+                const double* d_E1, double caux, double ebse0, double ebsef, int numbse, double sme,
+                int renorm, void* work, double* d_eps, double* d_smeavg, double* h_coef) {
+  int rc = run_haydock_chains(s, N, H, ldh, dip, dtfull, steps, work);
+  if (rc) return rc;
+  return run_haydock_eps(s, N, dtfull, steps, d_E1, caux, ebse0, ebsef, numbse, sme, renorm, work, d_eps, d_smeavg,
+                         h_coef);
 }
 
 }  // namespace wtb

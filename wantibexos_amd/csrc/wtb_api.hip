@@ -112,6 +112,7 @@ struct wtb_bse_plan {
   double* smeavg = nullptr;
   int* status = nullptr;
   hipStream_t stream = nullptr;             // launches go here (the context stream unless batched)
+  hipStream_t lz_stream = nullptr;          // iterative form: the Lanczos chains, beside the Davidson
   // exciton eigenvectors [wf_first, wf_first + wf_count) (BSE_WF); zero count: none formed
   int wf_first = 0, wf_count = 0;
   double* Zv = nullptr;                     // N x N tridiagonal eigenvectors
@@ -696,6 +697,10 @@ int wtb_bse_plan_destroy(wtb_bse_plan* p) {
   if (!p) return WTB_OK;
   hipSetDevice(p->ctx->device);
   hipStreamSynchronize(p->ctx->stream);
+  if (p->lz_stream) {
+    hipStreamSynchronize(p->lz_stream);
+    hipStreamDestroy(p->lz_stream);
+  }
   plan_free(p);
   delete p;
   return WTB_OK;
@@ -1066,6 +1071,23 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     if (!rc) rc = sym_fill_f32(s, N, p->Hf, (int)p->ldh);
     if (!rc) rc = row_stats_f32(s, N, p->Hf, (int)p->ldh, dD, dst);
     hipEventRecord(ev[1], s);
+    // the Lanczos chains need only the matrix and the dipoles: they run on a stream of
+    // their own beside the Davidson (whose Rayleigh-Ritz steps leave most of the chip
+    // idle); eps is formed from their coefficients once E_1 is known
+    const bool lz = eps && steps > 0;
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_lz = nullptr;
+    if (!rc && lz) {
+      if (!p->lz_stream && hipStreamCreateWithFlags(&p->lz_stream, hipStreamNonBlocking) != hipSuccess) {
+        p->lz_stream = nullptr;
+        rc = WTB_ERR_HIP;
+      }
+      s2 = p->lz_stream;
+      if (!rc && hipEventCreateWithFlags(&ev_lz, hipEventDisableTiming) != hipSuccess) rc = WTB_ERR_HIP;
+      if (!rc && hipStreamWaitEvent(s2, ev[1], 0) != hipSuccess) rc = WTB_ERR_HIP;
+      if (!rc) rc = run_haydock_chains(s2, N, p->Hf, (int)p->ldh, p->dip, P.dtfull, steps, hw);
+      if (!rc && hipEventRecord(ev_lz, s2) != hipSuccess) rc = WTB_ERR_HIP;
+    }
     DavidsonCfg cfg;
     cfg.nev = nev;
     cfg.guard = dp->guard;
@@ -1086,12 +1108,17 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     inf.iters_fp64 = di.iters_fp64;
     inf.matvecs_fp64 = di.matvecs_fp64;
     inf.max_resid = di.max_resid;
-    if (!rc && eps && steps > 0 && di.converged) {
+    if (!rc && lz && di.converged) {
       const double gama = (double)180.90f;
       const double caux = gama / (p->nkf32 * p->vc3d);
-      rc = run_haydock(s, N, p->Hf, (int)p->ldh, p->dip, P.dtfull, steps, dW, caux, P.ebse0, P.ebsef, P.numbse,
-                       P.cshift, P.renorm, hw, p->eps, dsm, nullptr);
+      if (hipStreamWaitEvent(s, ev_lz, 0) != hipSuccess) rc = WTB_ERR_HIP;
+      if (!rc)
+        rc = run_haydock_eps(s, N, P.dtfull, steps, dW, caux, P.ebse0, P.ebsef, P.numbse, P.cshift, P.renorm, hw,
+                             p->eps, dsm, nullptr);
     }
+    // every path: the chains are done before their workspace is freed
+    if (s2 && hipStreamSynchronize(s2) != hipSuccess && !rc) rc = WTB_ERR_HIP;
+    if (ev_lz) hipEventDestroy(ev_lz);
     hipEventRecord(ev[3], s);
     if (!rc && di.converged) {
       hipMemcpyAsync(W, dW, sizeof(double) * nev, hipMemcpyDeviceToHost, s);

@@ -143,6 +143,13 @@ size_t haydock_work_bytes(int N, int steps);
 int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
                 const double* d_E1, double caux, double ebse0, double ebsef, int numbse, double sme,
                 int renorm, void* work, double* d_eps, double* d_smeavg, double* h_coef);
+// the same in two parts: the chains (no dependence on E_1, so they can run beside the
+// eigensolver on a stream of their own), then eps from their coefficients; one workspace
+int run_haydock_chains(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
+                       void* work);
+int run_haydock_eps(hipStream_t s, int N, int dtfull, int steps, const double* d_E1, double caux, double ebse0,
+                    double ebsef, int numbse, double sme, int renorm, void* work, double* d_eps, double* d_smeavg,
+                    double* h_coef);
 
 // learned preconditioner: MLP (5 -> 128 -> 128 -> 1, tanh, FP32) on per-state features
 struct MlpDev {

@@ -176,6 +176,66 @@ def bench_davidson(args, cfg, model):
     print(json.dumps(out), flush=True)
 
 
+def bench_batch(args):
+    """Batching form: a set of uneven models (different Wannier counts and k grids) solved
+    as single plans one after the other, against the same set through the batch API
+    (one captured graph per matrix, replayed on --lanes streams); the results must agree
+    bit for bit."""
+    import dataclasses
+    from wantibexos_amd import bse, tbmodel
+    ctx = bse.Context(0)
+    spec = [(12, 8), (14, 6), (16, 10), (18, 8), (20, 12), (22, 6), (24, 8), (16, 10)]   # (k grid, nw)
+    models, prms = [], []
+    for i, (ng, nw) in enumerate(spec):
+        models.append(tbmodel.hex2d(200 + i, nw, (ng, ng, 1)))
+        cfg = dict(CONFIG, ngrid=(ng, ng, 1))
+        prms.append(params(cfg, args.precision))
+    dims = []
+    plans = [bse.BSEPlan(ctx, m, p) for m, p in zip(models, prms)]
+    dims = [pl.N for pl in plans]
+    log(f"batching form: {len(plans)} matrices, N = {dims}, precision {args.precision}, lanes {args.lanes}")
+
+    def run_single():
+        for pl in plans:
+            pl.execute()
+        ctx.sync()
+
+    for _ in range(args.warmup):
+        run_single()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_single()
+    t_single = (time.perf_counter() - t0) / args.steps
+    ref = [pl.results() for pl in plans]
+    for pl in plans:
+        pl.close()
+    with bse.BSEBatch(ctx, models, prms, lanes=args.lanes) as b:
+        tiles = b.tune().tolist() if args.tune else None
+        for _ in range(args.warmup):
+            b.execute()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.execute()
+        ctx.sync()
+        t_batch = (time.perf_counter() - t0) / args.steps
+        same = all(np.array_equal(b.results(i)["W"], ref[i]["W"]) for i in range(len(ref))) if not args.tune else None
+        dW = max(float(np.abs(b.results(i)["W"] - ref[i]["W"]).max()) for i in range(len(ref)))
+    ctx.close()
+    log(f"single plans {t_single * 1e3:.1f} ms per set, batch {t_batch * 1e3:.1f} ms per set")
+    out = {
+        "metric": "BSE build+solve of a set of uneven matrices, batch API vs single plans",
+        "value": round(t_batch, 6), "unit": "s/set", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128" if args.precision == "fp64" else "c64/c128", "data": "synthetic (seeded random tight-binding models)",
+        "config": {"workload": "hex2d set", "models": [{"kgrid": ng, "nwann": nw, "bse_dim": N} for (ng, nw), N in zip(spec, dims)],
+                   "lanes": args.lanes, "precision": args.precision, "tuned_panels": tiles},
+        "single_plans_s": round(t_single, 6), "speedup_vs_single": round(t_single / t_batch, 3),
+        "matrices_per_s": round(len(dims) / t_batch, 2), "bitwise_equal_to_single": same, "max_dW_eV": dW,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,10 +244,12 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed (default): the precision form, timed beside the plain FP64 form")
     ap.add_argument("--config", choices=["c2", "c1", "c5"], default="c2")
-    ap.add_argument("--form", choices=["full", "davidson"], default="full",
+    ap.add_argument("--form", choices=["full", "davidson", "batch"], default="full",
                     help="full: the whole eigensolve (plain FP64 or precision form); davidson: the "
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")
     ap.add_argument("--nev", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=4, help="batching form: streams")
+    ap.add_argument("--tune", action="store_true", help="batching form: run the panel autotuner first")
     ap.add_argument("--lanczos-steps", type=int, default=250)
     ap.add_argument("--guard", type=int, default=32, help="iterative form: extra Ritz pairs carried along")
     ap.add_argument("--max-sub", type=int, default=None,
@@ -218,6 +280,8 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     from wantibexos_amd import bse
+    if args.form == "batch":
+        return bench_batch(args)
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
     model = make_model(cfg)

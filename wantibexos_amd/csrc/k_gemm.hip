@@ -339,7 +339,8 @@ int hmm_thin_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X
   return WTB_OK;
 }
 
-// Y = H X on FP32 MFMA for the COMPLEX*8 matrix (even N and leading dimension): a
+// Y = H X for the COMPLEX*8 matrix (even N and leading dimension) on FP32 MFMA with FP32
+// accumulation, or (F64) on FP64 MFMA with the matrix widened exactly to FP64: a
 // workgroup computes 128 rows x 64 columns, wave w rows 32w..32w+31 (two 16-row blocks x
 // four 16-column blocks, complex as four real products).  K advances in slices of 16:
 // H as row pairs (one 16-byte load per lane, 1 KB per column per wave) and X converted to
@@ -347,11 +348,14 @@ int hmm_thin_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X
 // current one is multiplied.  K slices of the grid (z) write partial sums.
 #define H32_R 128
 #define H32_C 64
+template <bool F64>
 __global__ __launch_bounds__(256) void k_hmm32(int N, int nc, const zf* __restrict__ H, int ldh,
                                                const zc* __restrict__ X, int ldx, zc* __restrict__ Y, int ldy,
                                                int kc, zc* __restrict__ part) {
+  typedef typename std::conditional<F64, double2, float2>::type bt;
+  typedef typename std::conditional<F64, d4, f4>::type acc_t;
   __shared__ float2 As[16][H32_R];
-  __shared__ float2 Bs[16][H32_C];
+  __shared__ bt Bs[16][H32_C];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int R0 = blockIdx.x * H32_R, C0 = blockIdx.y * H32_C;
@@ -362,11 +366,11 @@ __global__ __launch_bounds__(256) void k_hmm32(int N, int nc, const zf* __restri
   const int arow = R0 + 2 * lrp;
   const zf* abase = H + (arow < N ? arow : 0);     // rows past N: any in-range pair, never written
   const int bk = tid & 15, bcol = tid >> 4;
-  f4 cr[2][4], ci[2][4];
+  acc_t cr[2][4], ci[2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) cr[a][b] = ci[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < 4; ++b) cr[a][b] = ci[a][b] = acc_t{0, 0, 0, 0};
   float4 ra[4];
   zc rb[4];
   auto gload = [&](int k0) {
@@ -388,27 +392,36 @@ __global__ __launch_bounds__(256) void k_hmm32(int N, int nc, const zf* __restri
     for (int u = 0; u < 4; ++u) {
       As[lk + 4 * u][2 * lrp] = make_float2(ra[u].x, ra[u].y);
       As[lk + 4 * u][2 * lrp + 1] = make_float2(ra[u].z, ra[u].w);
-      Bs[bk][bcol + 16 * u] = make_float2((float)rb[u].re, (float)rb[u].im);
+      if constexpr (F64) Bs[bk][bcol + 16 * u] = make_double2(rb[u].re, rb[u].im);
+      else Bs[bk][bcol + 16 * u] = make_float2((float)rb[u].re, (float)rb[u].im);
     }
     __syncthreads();
     if (k0 + 16 < kend) gload(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
       const int k = 4 * kk + kq;
-      float2 av[2], bv[4];
+      float2 av[2];
+      bt bv[4];
 #pragma unroll
       for (int a = 0; a < 2; ++a) av[a] = As[k][32 * w + 16 * a + i16];
 #pragma unroll
       for (int b = 0; b < 4; ++b) bv[b] = Bs[k][16 * b + i16];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const float nbi = -bv[b].y;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-          cr[a][b] = mfma16x16x4f32(av[a].x, bv[b].x, cr[a][b]);
-          cr[a][b] = mfma16x16x4f32(av[a].y, nbi, cr[a][b]);
-          ci[a][b] = mfma16x16x4f32(av[a].x, bv[b].y, ci[a][b]);
-          ci[a][b] = mfma16x16x4f32(av[a].y, bv[b].x, ci[a][b]);
+          if constexpr (F64) {
+            const double ar = av[a].x, ai = av[a].y;
+            cr[a][b] = mfma16x16x4(ar, bv[b].x, cr[a][b]);
+            cr[a][b] = mfma16x16x4(ai, -bv[b].y, cr[a][b]);
+            ci[a][b] = mfma16x16x4(ar, bv[b].y, ci[a][b]);
+            ci[a][b] = mfma16x16x4(ai, bv[b].x, ci[a][b]);
+          } else {
+            cr[a][b] = mfma16x16x4f32(av[a].x, bv[b].x, cr[a][b]);
+            cr[a][b] = mfma16x16x4f32(av[a].y, -bv[b].y, cr[a][b]);
+            ci[a][b] = mfma16x16x4f32(av[a].x, bv[b].y, ci[a][b]);
+            ci[a][b] = mfma16x16x4f32(av[a].y, bv[b].x, ci[a][b]);
+          }
         }
       }
     }
@@ -421,7 +434,8 @@ __global__ __launch_bounds__(256) void k_hmm32(int N, int nc, const zf* __restri
       if (col >= nc) continue;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = R0 + 32 * w + 16 * a + 4 * kq + q;   // f32 MFMA: register q holds row 4 kq + q
+        // f32 MFMA: register q holds row 4 kq + q; f64 MFMA: row kq + 4 q
+        const int row = R0 + 32 * w + 16 * a + (F64 ? kq + 4 * q : 4 * kq + q);
         if (row >= N) continue;
         const zc v = mkz((double)cr[a][b][q], (double)ci[a][b][q]);
         if (part) part[(size_t)blockIdx.z * N * nc + (size_t)row + (size_t)N * col] = v;
@@ -468,10 +482,12 @@ static int hmm_t(hipStream_t s, int N, int nc, const TA* H, int ldh, const zc* X
   return WTB_OK;
 }
 
-int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy, zc* work) {
+template <bool F64>
+static int hmm_h32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
+                   zc* work) {
   if (N <= 0 || nc <= 0) return WTB_OK;
   if ((N % 2) || (ldh % 2) || (reinterpret_cast<uintptr_t>(H) % 16))
-    return hmm_t<zf, true>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
+    return hmm_t<zf, !F64>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
   // K slices so that the grid reaches ~1024 workgroups; the partials fit the workspace
   // (hmm_work_bytes), which is sized for the 64 x 64 tiling's slices (never fewer)
   const int tiles = ((N + H32_R - 1) / H32_R) * ((nc + H32_C - 1) / H32_C);
@@ -480,7 +496,7 @@ int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int
   const int kc = (((N + ns - 1) / ns + 15) / 16) * 16;
   ns = (N + kc - 1) / kc;
   dim3 g((N + H32_R - 1) / H32_R, (nc + H32_C - 1) / H32_C, ns), b(256);
-  hipLaunchKernelGGL(k_hmm32, g, b, 0, s, N, nc, H, ldh, X, ldx, Y, ldy, kc, ns > 1 ? work : nullptr);
+  hipLaunchKernelGGL(k_hmm32<F64>, g, b, 0, s, N, nc, H, ldh, X, ldx, Y, ldy, kc, ns > 1 ? work : nullptr);
   if (ns > 1) {
     const size_t tot = (size_t)N * nc;
     hipLaunchKernelGGL(k_zgemm_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, N, nc, ns, work,
@@ -489,9 +505,13 @@ int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }
+
+int hmm_f32(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy, zc* work) {
+  return hmm_h32<false>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
+}
 int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* X, int ldx, zc* Y, int ldy,
                   zc* work) {
-  return hmm_t<zf, false>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
+  return hmm_h32<true>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);
 }
 int hmm_f64(hipStream_t s, int N, int nc, const zc* H, int ldh, const zc* X, int ldx, zc* Y, int ldy, zc* work) {
   return hmm_t<zc, false>(s, N, nc, H, ldh, X, ldx, Y, ldy, work);

@@ -223,7 +223,7 @@ typedef struct wtb_mlp wtb_mlp;
 typedef struct {
   int nev;            /* lowest excitons wanted                              */
   int guard;          /* extra Ritz pairs carried along (e.g. 32)            */
-  int max_sub;        /* basis size that triggers a restart (e.g. 3 (nev+guard)) */
+  int max_sub;        /* basis size that triggers a restart (e.g. 5 (nev+guard)) */
   int max_iter;
   double tol;         /* residual 2-norm bound, eV                           */
   int lanczos_steps;  /* chain length for eps(omega); 0: no eps              */

@@ -87,6 +87,22 @@ __device__ inline zc wave_sum(zc v) {
     }                                                                     \
   } while (0)
 
+// sum of p[b * stride] over b = b0, b0 + step, ... < b1, added in that order, with the loads
+// issued eight at a time (a plain loop waits for each load before the next)
+template <class V>
+__device__ inline V strided_sum(const V* __restrict__ p, size_t stride, int b0, int b1, int step, V acc) {
+  int b = b0;
+  for (; b + 7 * step < b1; b += 8 * step) {
+    V v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(b + u * step) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; b < b1; b += step) acc += p[(size_t)b * stride];
+  return acc;
+}
+
 // alpha' = -1/2 |tau|^2 sum_b puv[b] over the np partials of the previous rowstep, in a
 // fixed order (every thread of the workgroup returns it)
 __device__ inline zc alpha_from_puv(const zc* __restrict__ puv, int np, zc tau) {
@@ -121,7 +137,7 @@ __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums
     const int q = q0 + (tid >> 2);
     zc acc = mkz(0, 0);
     if (q == 0) {
-      for (int b = b0h + sub; b < a.nblk; b += 4) acc.re += a.pn[b];
+      acc.re = strided_sum(a.pn, 1, b0h + sub, a.nblk, 4, 0.0);
     } else if (q < nq) {
       const zc* base;
       size_t stride;
@@ -135,7 +151,7 @@ __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums
         base = a.pvg + (q - 1 - 2 * jjh);
         stride = a.nrhs;
       }
-      for (int b = b0h + sub; b < a.nblk; b += 4) acc += base[(size_t)b * stride];
+      acc = strided_sum(base, stride, b0h + sub, a.nblk, 4, acc);
     }
     for (int off = 2; off > 0; off >>= 1) {
       acc.re += __shfl_xor(acc.re, off, 4);
@@ -197,10 +213,8 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   const int r = j + blockIdx.x * RPW + row_l;
   const bool valid = r < N;
   auto ysum = [&](int row) -> zc {
-    zc acc = mkz(0, 0);
     const zc* yp = a.ypart + ((size_t)(row / TB) * a.nblk) * TB + (row % TB);
-    for (int sl = bm + sub; sl < a.nblk; sl += 16) acc += yp[(size_t)sl * TB];
-    return acc;
+    return strided_sum(yp, (size_t)TB, bm + sub, a.nblk, 16, mkz(0, 0));
   };
   // panel rows j and r, columns sub and sub + 16
   zc yj = mkz(0, 0), yr = mkz(0, 0), hjj = mkz(0, 0), hcol = mkz(0, 0);

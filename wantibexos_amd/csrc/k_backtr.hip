@@ -139,8 +139,7 @@ __global__ __launch_bounds__(256) void k_bt_ty(const zc* __restrict__ Ypart, int
   const int cl = tid & 63, col = blockIdx.x * 64 + cl;
   for (int c = tid >> 6; c < BT_NB; c += 4) {
     zc acc = mkz(0, 0);
-    if (c < kp && col < m)
-      for (int ch = 0; ch < nchunk; ++ch) acc += Ypart[((size_t)ch * BT_NB + c) * m + col];
+    if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
     Ss[c][cl] = acc;
   }
   __syncthreads();

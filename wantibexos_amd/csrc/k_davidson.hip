@@ -225,10 +225,23 @@ __global__ __launch_bounds__(1024) void k_chol_left(int b, zc* __restrict__ G, i
 #pragma unroll
         for (int c = 0; c < CP; ++c) acc[c] = mkz(0, 0);
         if (lr < nr) {
-          for (int k = kb; k < ke; ++k) {
+          // UL loads in flight (fewer for the wider panel: 1024 threads leave 128 registers
+          // each), then their products in k order
+          constexpr int UL = CP == 16 ? 2 : 4;
+          int k = kb;
+          for (; k + UL - 1 < ke; k += UL) {
+            zc l[UL];
+#pragma unroll
+            for (int u = 0; u < UL; ++u) l[u] = G[(size_t)(p + lr) + (size_t)ld * (k + u)];
+#pragma unroll
+            for (int u = 0; u < UL; ++u)
+#pragma unroll
+              for (int c = 0; c < CP; ++c) acc[c] = fma_conj(Lr[(k + u) * CP + c], l[u], acc[c]);   // l conj(L(p+c,k))
+          }
+          for (; k < ke; ++k) {
             const zc l = G[(size_t)(p + lr) + (size_t)ld * k];
 #pragma unroll
-            for (int c = 0; c < CP; ++c) acc[c] = fma_conj(Lr[k * CP + c], l, acc[c]);   // l conj(L(p+c,k))
+            for (int c = 0; c < CP; ++c) acc[c] = fma_conj(Lr[k * CP + c], l, acc[c]);
           }
         }
         for (int q = 0; q < 4; ++q) {
@@ -286,8 +299,17 @@ __global__ __launch_bounds__(256) void k_trinv(int b, const zc* __restrict__ G, 
   for (int I0 = c0; I0 < b; I0 += 16) {
     const int ni = min(16, b - I0);
     zc acc = mkz(0, 0);
-    if (r < ni && c < nc)
-      for (int k = c0; k < I0; ++k) acc = acc + G[(size_t)(I0 + r) + (size_t)ld * k] * X[(k - c0) * 16 + c];
+    if (r < ni && c < nc) {
+      int k = c0;
+      for (; k + 7 < I0; k += 8) {   // eight loads in flight, products in k order
+        zc g8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g8[u] = G[(size_t)(I0 + r) + (size_t)ld * (k + u)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + g8[u] * X[(k + u - c0) * 16 + c];
+      }
+      for (; k < I0; ++k) acc = acc + G[(size_t)(I0 + r) + (size_t)ld * k] * X[(k - c0) * 16 + c];
+    }
     {
       const int rr = tid & 15, kk = tid >> 4;
       Lb[rr][kk] = (rr < ni && kk < ni && rr >= kk) ? G[(size_t)(I0 + rr) + (size_t)ld * (I0 + kk)] : mkz(0, 0);

@@ -87,22 +87,6 @@ __device__ inline zc wave_sum(zc v) {
     }                                                                     \
   } while (0)
 
-// sum of p[b * stride] over b = b0, b0 + step, ... < b1, added in that order, with the loads
-// issued eight at a time (a plain loop waits for each load before the next)
-template <class V>
-__device__ inline V strided_sum(const V* __restrict__ p, size_t stride, int b0, int b1, int step, V acc) {
-  int b = b0;
-  for (; b + 7 * step < b1; b += 8 * step) {
-    V v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(b + u * step) * stride];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc += v[u];
-  }
-  for (; b < b1; b += step) acc += p[(size_t)b * stride];
-  return acc;
-}
-
 // alpha' = -1/2 |tau|^2 sum_b puv[b] over the np partials of the previous rowstep, in a
 // fixed order (every thread of the workgroup returns it)
 __device__ inline zc alpha_from_puv(const zc* __restrict__ puv, int np, zc tau) {

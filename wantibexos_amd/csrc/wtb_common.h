@@ -51,6 +51,22 @@ __device__ inline zc fma_z(zc a, zc b, zc acc) {
   return acc;
 }
 
+// sum of p[b * stride] over b = b0, b0 + step, ... < b1, added in that order, with the loads
+// issued eight at a time (a plain loop waits for each load before the next)
+template <class V>
+__device__ inline V strided_sum(const V* __restrict__ p, size_t stride, int b0, int b1, int step, V acc) {
+  int b = b0;
+  for (; b + 7 * step < b1; b += 8 * step) {
+    V v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(b + u * step) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; b < b1; b += step) acc += p[(size_t)b * stride];
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // MFMA lane layouts on gfx950 (measured with a lane probe):
 //   v_mfma_f64_16x16x4_f64: lane l holds A[l%16][l/16] and B[l/16][l%16];

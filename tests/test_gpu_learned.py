@@ -92,6 +92,27 @@ def test_davidson_and_lanczos_against_full_fp64(ctx):
     assert max(e for e, _ in errs.values()) < 1e-3
 
 
+def test_davidson_and_lanczos_odd_dimension(ctx):
+    """N odd (nc = nv = 1, 15 x 15 k-points: N = 225): the products take the paths for an
+    odd leading dimension (no 16-byte row pairs); same bars as above."""
+    import dataclasses
+    from wantibexos_amd import bse, tbmodel
+    m = tbmodel.hex2d(seed=3, nw=6, ngrid=(15, 15, 1))
+    prm = dataclasses.replace(mixed_params(ngrid=(15, 15, 1)), nc=1, nv=1)
+    full = bse.bsesolver(ctx, m, dataclasses.replace(prm, precision=bse.FP64))
+    with bse.BSEPlan(ctx, m, prm) as plan:
+        assert plan.N == 225
+        r = plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=300)
+    info = r["info"]
+    assert info["converged"] == 1 and info["used_fallback"] == 0, info
+    assert np.abs(r["W"] - full["W"][:32]).max() < 1e-4
+    for c in range(6):
+        for part in range(2):
+            ref = full["eps"][c, part]
+            if np.abs(ref).max() > 1e-8:
+                assert relerr(r["eps"][c, part], ref) < 1e-3, (c, part)
+
+
 def test_learned_preconditioner_against_diagonal(ctx):
     from wantibexos_amd import bse, learned, tbmodel
     data = learned.make_training_set(ctx, n_models=24, seed=5, nw_range=(6, 10), n_theta=8)

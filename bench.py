@@ -126,7 +126,7 @@ def bench_davidson(args, cfg, model):
     N = plan.N
     log(f"iterative form: N={N} nev={args.nev} lanczos={args.lanczos_steps} learned={args.learned}")
     kw = dict(nev=args.nev, guard=args.guard, tol=args.tol, lanczos_steps=args.lanczos_steps, mlp=net,
-              max_sub=args.max_sub)
+              max_sub=args.max_sub, coarse=args.coarse)
     for _ in range(args.warmup):
         plan.lowest(**kw)
     times, r = [], None
@@ -167,7 +167,7 @@ def bench_davidson(args, cfg, model):
         "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 products, c128 Rayleigh-Ritz", "data": "synthetic (seeded random tight-binding model)",
         "config": {"workload": cfg["name"], "bse_dim": N, "nev": args.nev, "lanczos_steps": args.lanczos_steps,
-                   "tol_eV": args.tol, "guard": args.guard, "max_sub": args.max_sub,
+                   "tol_eV": args.tol, "guard": args.guard, "max_sub": args.max_sub, "coarse": args.coarse,
                    "preconditioner": "learned" if args.learned else "diagonal"},
         "full_fp64_s": round(t_full, 6), "speedup_vs_full_fp64": round(t_full / dt, 3),
         "info": r["info"], "max_dW_eV": float(dW.max()), "max_resid_eV": float(r["resid"].max()),
@@ -252,6 +252,9 @@ def main():
     ap.add_argument("--tune", action="store_true", help="batching form: run the panel autotuner first")
     ap.add_argument("--lanczos-steps", type=int, default=250)
     ap.add_argument("--guard", type=int, default=32, help="iterative form: extra Ritz pairs carried along")
+    ap.add_argument("--coarse", type=int, default=None,
+                    help="iterative form: coarse block of the preconditioner (states; default 6 (nev + guard), "
+                         "at most N/4; 0: diagonal only)")
     ap.add_argument("--max-sub", type=int, default=None,
                     help="iterative form: basis size that triggers a restart (default 5 (nev + guard))")
     ap.add_argument("--tol", type=float, default=1e-6,

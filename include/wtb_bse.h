@@ -216,7 +216,8 @@ int wtb_bse_plan_destroy(wtb_bse_plan* plan);
  * strengths (so without the F15.4 round trip).  The plan must be created
  * with precision = WTB_MIXED.  The preconditioner is 1/(D_i - theta) or,
  * with an MLP, the network's estimate of the regularised resolvent
- * diagonal.  When Davidson does not converge and fallback = 1, the full
+ * diagonal; with coarse > 0 it is replaced on the coarse states by the exact
+ * inverse of their block of H - theta (eigendecomposed once).  When Davidson does not converge and fallback = 1, the full
  * FP64 solve of the plain path answers instead (used_fallback = 1).
  * ------------------------------------------------------------------------ */
 typedef struct wtb_mlp wtb_mlp;
@@ -228,6 +229,9 @@ typedef struct {
   double tol;         /* residual 2-norm bound, eV                           */
   int lanczos_steps;  /* chain length for eps(omega); 0: no eps              */
   int fallback;       /* 1: full FP64 solve when Davidson does not converge   */
+  int coarse;         /* states of lowest diagonal energy whose block of H the
+                         preconditioner inverts exactly (e.g. 6 (nev+guard));
+                         0: diagonal (or learned) preconditioner only          */
 } wtb_davidson_params;
 typedef struct {
   int iters, matvecs, converged, used_fallback;

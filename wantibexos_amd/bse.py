@@ -66,7 +66,7 @@ class _Params(C.Structure):
 
 class _DavidsonParams(C.Structure):
     _fields_ = [("nev", C.c_int), ("guard", C.c_int), ("max_sub", C.c_int), ("max_iter", C.c_int),
-                ("tol", C.c_double), ("lanczos_steps", C.c_int), ("fallback", C.c_int)]
+                ("tol", C.c_double), ("lanczos_steps", C.c_int), ("fallback", C.c_int), ("coarse", C.c_int)]
 
 
 class _DavidsonInfo(C.Structure):
@@ -387,13 +387,17 @@ class BSEPlan:
         return raw
 
     def lowest(self, nev=256, guard=32, max_sub=None, max_iter=60, tol=1e-6, lanczos_steps=250,
-               fallback=True, mlp=None, vectors=False):
+               fallback=True, mlp=None, vectors=False, coarse=None):
         """Iterative form (precision-form plan): the lowest `nev` excitons by block
-        Davidson and eps(omega) by Lanczos chains.  Returns dict W, resid, eps (or
-        None), X (N, nev) (or None), info."""
+        Davidson and eps(omega) by Lanczos chains.  `coarse`: states of lowest diagonal
+        energy whose block the preconditioner inverts exactly (default 6 (nev + guard),
+        at most N/4; 0 for the diagonal preconditioner alone).  Returns dict W, resid,
+        eps (or None), X (N, nev) (or None), info."""
         N = self.N
+        if coarse is None:
+            coarse = min(6 * (nev + guard), N // 4)
         dp = _DavidsonParams(nev, guard, max_sub or 5 * (nev + guard), max_iter, tol, lanczos_steps,
-                             int(fallback))
+                             int(fallback), int(coarse))
         W = np.zeros(nev)
         res = np.zeros(nev)
         eps = np.zeros((6, 2, int(self.params.numbse))) if lanczos_steps > 0 else None

@@ -54,6 +54,35 @@ __global__ void k_gather_cols(int N, int nq, const int* map, const zc* src, int 
   dst[(size_t)r + (size_t)ldd * q] = src[(size_t)r + (size_t)lds * map[q]];
 }
 
+// coarse block of H: G(i, j) = H(rows[i], rows[j]) widened to FP64 (H full Hermitian)
+__global__ void k_gather_block(int M, const int* rows, const zf* H, int ldh, zc* G) {
+  const int j = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M || j >= M) return;
+  G[(size_t)i + (size_t)M * j] = ldz(H[(size_t)rows[i] + (size_t)ldh * rows[j]]);
+}
+// RS(i, q) = R(rows[i], idx[q])
+__global__ void k_gather_rows(int M, int nq, const int* rows, const int* idx, const zc* R, int ldr, zc* RS) {
+  const int q = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M || q >= nq) return;
+  RS[(size_t)i + (size_t)M * q] = R[(size_t)rows[i] + (size_t)ldr * idx[q]];
+}
+// Y(i, q) /= lam_i - theta_idx[q], the difference kept at least dmin away from zero
+__global__ void k_coarse_scale(int M, int nq, const int* idx, const double* lam, const double* theta, double dmin,
+                               zc* Y) {
+  const int q = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M || q >= nq) return;
+  double d = lam[i] - theta[idx[q]];
+  if (fabs(d) < dmin) d = d < 0.0 ? -dmin : dmin;
+  zc& y = Y[(size_t)i + (size_t)M * q];
+  y = y * (1.0 / d);
+}
+// T(rows[i], q) = Z(i, q)
+__global__ void k_scatter_rows(int M, int nq, const int* rows, const zc* Z, zc* T, int ldt) {
+  const int q = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= M || q >= nq) return;
+  T[(size_t)rows[i] + (size_t)ldt * q] = Z[(size_t)i + (size_t)M * q];
+}
+
 namespace {
 
 struct SmallEig {
@@ -123,15 +152,24 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   int *idx = nullptr, *keep = nullptr, *map = nullptr;
   zc* hw = nullptr;
   zc* gw = nullptr;   // K-slice partials of the projections V^H X
+  // coarse block of the preconditioner: rows, eigenvectors QS and eigenvalues lamS of
+  // H(rows, rows), and three M x nk work blocks
+  const int M = std::min(N, std::max(0, cfg.coarse));
+  int* crow = nullptr;
+  zc *QS = nullptr, *CS = nullptr, *CY = nullptr, *CZ = nullptr;
+  double* lamS = nullptr;
+  SmallEig sc;
   const size_t gws = (size_t)8 * msub * nk;
   SmallEig se;
   int rc = WTB_OK;
   auto cleanup = [&]() {
     for (void* p : {(void*)V, (void*)HV, (void*)G, (void*)S, (void*)U, (void*)HU, (void*)R, (void*)T,
                     (void*)T2, (void*)P, (void*)Gb, (void*)Linv, (void*)theta, (void*)rn, (void*)idx,
-                    (void*)keep, (void*)map, (void*)hw, (void*)gw})
+                    (void*)keep, (void*)map, (void*)hw, (void*)gw, (void*)crow, (void*)QS, (void*)CS,
+                    (void*)CY, (void*)CZ, (void*)lamS})
       hipFree(p);
     small_free(se);
+    if (M > 0) small_free(sc);
   };
 #define DAV_ALLOC(ptr, bytes)                                  \
   if (hipMalloc(&ptr, bytes) != hipSuccess) {                  \
@@ -158,8 +196,16 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
   DAV_ALLOC(map, sizeof(int) * nk);
   DAV_ALLOC(hw, hmm_work_bytes(N, nk));
   DAV_ALLOC(gw, sizeof(zc) * gws);
+  if (M > 0) {
+    DAV_ALLOC(crow, sizeof(int) * M);
+    DAV_ALLOC(QS, sizeof(zc) * (size_t)M * M);
+    DAV_ALLOC(CS, sizeof(zc) * (size_t)M * std::max(M, nk));
+    DAV_ALLOC(CY, sizeof(zc) * (size_t)M * nk);
+    DAV_ALLOC(CZ, sizeof(zc) * (size_t)M * nk);
+    DAV_ALLOC(lamS, sizeof(double) * M);
+  }
 #undef DAV_ALLOC
-  if ((rc = small_alloc(se, msub, nk))) {
+  if ((rc = small_alloc(se, msub, nk)) || (M > 0 && (rc = small_alloc(sc, M, M)))) {
     cleanup();
     return rc;
   }
@@ -183,6 +229,13 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     for (double v : hD) dmax = std::max(dmax, std::fabs(v));
     // residual level the FP32 products reach: their rounding grows like sqrt(N) |H|
     fp32_floor = 4e-7 * std::sqrt((double)N) * dmax;
+    if (M > 0) {
+      // the coarse block: the M states of lowest diagonal energy, where the bound excitons
+      // live; H(rows, rows) = QS diag(lamS) QS^H once, on the same small eigensolver
+      DAV_TRY(hipMemcpy(crow, order.data(), sizeof(int) * M, hipMemcpyHostToDevice) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
+      hipLaunchKernelGGL(k_gather_block, dim3((M + 255) / 256, M), dim3(256), 0, s, M, crow, H, ldh, CS);
+      DAV_TRY(small_heev(s, sc, M, CS, M, lamS, QS, M, M));
+    }
     order.resize(nk);
     DAV_TRY(hipMemcpy(idx, order.data(), sizeof(int) * nk, hipMemcpyHostToDevice) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
     hipLaunchKernelGGL(k_unit_cols, dim3((N + 255) / 256, nk), dim3(256), 0, s, N, nk, idx, V, N);
@@ -261,6 +314,15 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
       DAV_TRY(launch_precond_mlp(s, N, nq, idx, R, N, theta, D, stats, Dmin, *mlp, T, N));
     } else {
       DAV_TRY(launch_precond_diag(s, N, nq, idx, R, N, theta, D, cfg.dmin, T, N));
+    }
+    if (M > 0) {
+      // on the coarse rows: T = QS (lamS - theta)^-1 QS^H R, the block's exact inverse
+      const dim3 gq((M + 255) / 256, nq);
+      hipLaunchKernelGGL(k_gather_rows, gq, dim3(256), 0, s, M, nq, crow, idx, R, N, CS);
+      DAV_TRY(zgemm(s, 'C', 'N', M, nq, M, one, QS, M, CS, M, zero, CY, M, gw, gws));
+      hipLaunchKernelGGL(k_coarse_scale, gq, dim3(256), 0, s, M, nq, idx, lamS, theta, cfg.coarse_dmin, CY);
+      DAV_TRY(zgemm(s, 'N', 'N', M, nq, M, one, QS, M, CY, M, zero, CZ, M, gw, gws));
+      hipLaunchKernelGGL(k_scatter_rows, gq, dim3(256), 0, s, M, nq, crow, CZ, T, N);
     }
     // T -= V (V^H T), twice
     for (int pass = 0; pass < 2; ++pass) {

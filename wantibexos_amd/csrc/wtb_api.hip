@@ -1094,6 +1094,7 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     cfg.max_sub = dp->max_sub;
     cfg.max_iter = dp->max_iter;
     cfg.tol = dp->tol;
+    cfg.coarse = dp->coarse;
     MlpDev md;
     if (mlp) {
       md.w = mlp->P;

@@ -174,6 +174,10 @@ struct DavidsonCfg {
   // residual level at which the products switch from FP32 to FP64 arithmetic (the
   // FP32 products stall near 1e-6 eV for matrices of norm ~10 eV)
   double switch_tol = 2e-5;
+  // coarse block of the preconditioner: the `coarse` basis states of lowest diagonal
+  // energy, whose block of H is inverted exactly (0: diagonal preconditioner only)
+  int coarse = 0;
+  double coarse_dmin = 1e-2;
 };
 struct DavidsonInfo {
   int iters = 0, matvecs = 0, converged = 0;

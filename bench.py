@@ -139,6 +139,7 @@ def bench_davidson(args, cfg, model):
         log(f"  step {len(times)}: {times[-1] * 1e3:.1f} ms  {r['info']}")
     plan.close()
     dt = float(np.mean(times))
+    coarse = args.coarse if args.coarse is not None else min(10 * (args.nev + args.guard), N // 4)
     # the judge: full FP64 solve (plain path), timed once after a warm-up
     with bse.BSEPlan(ctx, model, p64) as full:
         if args.no_graph:
@@ -167,8 +168,10 @@ def bench_davidson(args, cfg, model):
         "ms_per_step": round(dt * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 products, c128 Rayleigh-Ritz", "data": "synthetic (seeded random tight-binding model)",
         "config": {"workload": cfg["name"], "bse_dim": N, "nev": args.nev, "lanczos_steps": args.lanczos_steps,
-                   "tol_eV": args.tol, "guard": args.guard, "max_sub": args.max_sub, "coarse": args.coarse,
-                   "preconditioner": "learned" if args.learned else "diagonal"},
+                   "tol_eV": args.tol, "guard": args.guard, "max_sub": args.max_sub or 5 * (args.nev + args.guard),
+                   "coarse": coarse,
+                   "preconditioner": ("learned" if args.learned else "diagonal")
+                   + (f" + coarse block of {coarse} states" if coarse else "")},
         "full_fp64_s": round(t_full, 6), "speedup_vs_full_fp64": round(t_full / dt, 3),
         "info": r["info"], "max_dW_eV": float(dW.max()), "max_resid_eV": float(r["resid"].max()),
         "eps_rel_err": eps_err,
@@ -253,7 +256,7 @@ def main():
     ap.add_argument("--lanczos-steps", type=int, default=250)
     ap.add_argument("--guard", type=int, default=32, help="iterative form: extra Ritz pairs carried along")
     ap.add_argument("--coarse", type=int, default=None,
-                    help="iterative form: coarse block of the preconditioner (states; default 6 (nev + guard), "
+                    help="iterative form: coarse block of the preconditioner (states; default 10 (nev + guard), "
                          "at most N/4; 0: diagonal only)")
     ap.add_argument("--max-sub", type=int, default=None,
                     help="iterative form: basis size that triggers a restart (default 5 (nev + guard))")

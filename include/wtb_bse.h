@@ -230,7 +230,8 @@ typedef struct {
   int lanczos_steps;  /* chain length for eps(omega); 0: no eps              */
   int fallback;       /* 1: full FP64 solve when Davidson does not converge   */
   int coarse;         /* states of lowest diagonal energy whose block of H the
-                         preconditioner inverts exactly (e.g. 6 (nev+guard));
+                         preconditioner inverts exactly (e.g. 10 (nev+guard),
+                         at most N/4);
                          0: diagonal (or learned) preconditioner only          */
 } wtb_davidson_params;
 typedef struct {

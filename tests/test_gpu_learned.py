@@ -113,6 +113,22 @@ def test_davidson_and_lanczos_odd_dimension(ctx):
                 assert relerr(r["eps"][c, part], ref) < 1e-3, (c, part)
 
 
+def test_coarse_block_preconditioner(ctx):
+    """The coarse block (exact inverse of H - theta over the lowest-energy states) changes
+    the path, not the answer: same excitons as the diagonal preconditioner alone, in no
+    more iterations."""
+    from wantibexos_amd import bse, tbmodel
+    m = tbmodel.hex2d(seed=7, nw=10, ngrid=(14, 14, 1))       # N = 784
+    prm = mixed_params(ngrid=(14, 14, 1), r0=15.0)
+    with bse.BSEPlan(ctx, m, prm) as plan:
+        r0 = plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=0, coarse=0)
+        r1 = plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=0, coarse=196)
+    print("diagonal:", r0["info"], "coarse 196:", r1["info"])
+    assert r0["info"]["converged"] and r1["info"]["converged"]
+    assert np.abs(r1["W"] - r0["W"]).max() < 1e-5
+    assert r1["info"]["iters"] <= r0["info"]["iters"]
+
+
 def test_learned_preconditioner_against_diagonal(ctx):
     from wantibexos_amd import bse, learned, tbmodel
     data = learned.make_training_set(ctx, n_models=24, seed=5, nw_range=(6, 10), n_theta=8)

@@ -390,12 +390,12 @@ class BSEPlan:
                fallback=True, mlp=None, vectors=False, coarse=None):
         """Iterative form (precision-form plan): the lowest `nev` excitons by block
         Davidson and eps(omega) by Lanczos chains.  `coarse`: states of lowest diagonal
-        energy whose block the preconditioner inverts exactly (default 6 (nev + guard),
+        energy whose block the preconditioner inverts exactly (default 10 (nev + guard),
         at most N/4; 0 for the diagonal preconditioner alone).  Returns dict W, resid,
         eps (or None), X (N, nev) (or None), info."""
         N = self.N
         if coarse is None:
-            coarse = min(6 * (nev + guard), N // 4)
+            coarse = min(10 * (nev + guard), N // 4)
         dp = _DavidsonParams(nev, guard, max_sub or 5 * (nev + guard), max_iter, tol, lanczos_steps,
                              int(fallback), int(coarse))
         W = np.zeros(nev)

@@ -4,7 +4,8 @@
 // H V run on FP32 MFMA, everything else (basis, projected matrix, Ritz pairs,
 // residuals) is FP64.
 //
-//   V (N x m) orthonormal, HV = H V, G = V^H H V
+//   V (N x m) orthonormal, HV = H V, G = V^H H V; V starts as unit vectors at the nk
+//   smallest D, or as the lowest eigenvectors of the preconditioner's coarse block
 //   loop: G = S diag(theta) S^H (tridiagonal reduction + divide and conquer +
 //         back-transform on the GPU), U = V S_k, HU = HV S_k, R = HU - U theta
 //         stop when the first nev residual norms are below tol
@@ -239,6 +240,10 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     order.resize(nk);
     DAV_TRY(hipMemcpy(idx, order.data(), sizeof(int) * nk, hipMemcpyHostToDevice) == hipSuccess ? WTB_OK : WTB_ERR_HIP);
     hipLaunchKernelGGL(k_unit_cols, dim3((N + 255) / 256, nk), dim3(256), 0, s, N, nk, idx, V, N);
+    // with a coarse block of at least nk states, start from its nk lowest eigenvectors
+    // instead (the unit vectors' rows lie inside the block, so the scatter replaces them)
+    if (M >= nk)
+      hipLaunchKernelGGL(k_scatter_rows, dim3((M + 255) / 256, nk), dim3(256), 0, s, M, nk, crow, QS, V, N);
   }
   bool hi = false;   // FP64 products
   auto matvec = [&](int nc, const zc* X, zc* Y) {

@@ -14,6 +14,9 @@
 !   WTB_DUMP       path of the binary dump (default: no dump)
 !   WTB_DUMP_HBSE  1 = also dump the upper triangle of the BSE matrix
 !   WTB_SKIP_DIAG  1 = stop after the BSE build (timing of stages a..c only)
+! With BSET= T in the input it runs bsesolvertemp's variant instead (subprograms/
+! bse_solver-tool-diel-temp.f90): the gap correction tcor added to the scissor,
+! f_v - f_c per BSE state, matrizelbsetemp and dielbseptemp.
 
 INCLUDE "subroutines/berry_curvature_subs.f90"
 INCLUDE "subroutines/bse_subs.f90"
@@ -85,7 +88,9 @@ program ref_dump
 	double precision, allocatable :: act(:,:), dielf(:,:,:), actq(:,:), wq(:), dielq(:,:,:)
 	double precision :: egap, vc, elux, rr, ii, spinf
 	double precision :: t0, t1, tab, tdip, tbuild, tdiag, tosc, teps
-	double complex :: matrizelbse
+	double complex :: matrizelbse, matrizelbsetemp
+	double precision :: tcor, ecx, evx, fermidisteh, gapcortemp, gapcortemp2
+	double precision, allocatable :: fdeh(:)
 	character(len=256) :: envv
 	character(len=120) :: line
 	logical :: dump_hbse, skip_diag
@@ -127,11 +132,27 @@ program ref_dump
 	allocate(stt(dimbse,4))
 	allocate(hrx(dimbse),hry(dimbse),hrz(dimbse),hrsp(dimbse),hrsm(dimbse))
 
+	! finite temperature: the gap correction (bse_solver-tool-diel-temp.f90:262-283)
+	tcor = 0.0
+	if (bset) then
+		select case (ta)
+		case("FA")
+			tcor = 0.0
+		case("VE")
+			tcor = gapcortemp(st,phavg,temp)
+		case("BE")
+			tcor = gapcortemp2(st,phavg,temp)
+		case default
+			tcor = 0.00
+		end select
+	end if
+	call dump_r8("tcor", 1, (/tcor/))
+
 	! stages (a)+(b): bse_solver-tool-diel.f90:257-313
 	egap = 50.0
 	t0 = omp_get_wtime()
 	do i=1,ngkpt
-		call eigsys(nthreads,scs,exc,nocpk(i),ffactor,kpt(i,1),kpt(i,2),kpt(i,3),nw,nvec,&
+		call eigsys(nthreads,scs+tcor,exc,nocpk(i),ffactor,kpt(i,1),kpt(i,2),kpt(i,3),nw,nvec,&
 			    rlat,rvec,hopmatrices,ihopmatrices,efermi,eaux,vaux)
 		eall(:,i) = eaux
 		do j=1,nc+nv
@@ -170,11 +191,34 @@ program ref_dump
 	call dump_c16("hrz", dimbse, hrz)
 	call dump_c16("hrsp", dimbse, hrsp)
 	call dump_c16("hrsm", dimbse, hrsm)
+	if (bset) then
+		! bse_solver-tool-diel-temp.f90:394-399 (fermidisteh sets temp = 1E-8 when it is 0)
+		allocate(fdeh(dimbse))
+		do i=1,dimbse
+			ecx = eigv(stt(i,4),stt(i,3))
+			evx = eigv(stt(i,4),stt(i,2))
+			fdeh(i) = fermidisteh(ecx,evx,temp)
+		end do
+		call dump_r8("fdeh", dimbse, fdeh)
+	end if
 
 	! stage (c): bse_solver-tool-diel.f90:429-458
 	allocate(hbse(dimbse,dimbse), wbse(dimbse))
 	t0 = omp_get_wtime()
 	hbse = 0.0
+	if (bset) then
+	!$omp parallel do
+	do i=1,dimbse
+		do j=i,dimbse
+  hbse(i,j)= matrizelbsetemp(coultype,ktol,nw,ediel,lc,ez,w,r0,ngrid,rlat,stt(i,:),eigv(stt(i,4)&
+  	    ,stt(i,3)),eigv(stt(i,4),stt(i,2)),vector(stt(i,4)&
+            ,stt(i,3),:) ,vector(stt(i,4),stt(i,2),:),kpt(stt(i,4),:),stt(j,:),eigv(stt(j,4),stt(j,3))&
+  	    ,eigv(stt(j,4),stt(j,2)) &
+            ,vector(stt(j,4),stt(j,3),:),vector(stt(j,4),stt(j,2),:),kpt(stt(j,4),:),temp)
+		end do
+	end do
+	!$omp end parallel do
+	else
 	!$omp parallel do
 	do i=1,dimbse
 		do j=i,dimbse
@@ -186,6 +230,7 @@ program ref_dump
 		end do
 	end do
 	!$omp end parallel do
+	end if
 	t1 = omp_get_wtime(); tbuild = t1-t0
 	if (dump_hbse) then
 		allocate(tri(dimbse*(dimbse+1)/2))
@@ -226,6 +271,17 @@ program ref_dump
 	! oscillator strengths: bse_solver-tool-diel.f90:540-602 (DTDIAG=T and CPOL=T)
 	allocate(act(dimbse,8))
 	t0 = omp_get_wtime()
+	if (bset) then
+	! bse_solver-tool-diel-temp.f90:570-618
+	call dielbseptemp(nthreads,dimbse,hbse,hrx,hrx,fdeh,act(:,1))
+	call dielbseptemp(nthreads,dimbse,hbse,hry,hry,fdeh,act(:,2))
+	call dielbseptemp(nthreads,dimbse,hbse,hrz,hrz,fdeh,act(:,3))
+	call dielbseptemp(nthreads,dimbse,hbse,hrx,hry,fdeh,act(:,4))
+	call dielbseptemp(nthreads,dimbse,hbse,hrx,hrz,fdeh,act(:,5))
+	call dielbseptemp(nthreads,dimbse,hbse,hry,hrz,fdeh,act(:,6))
+	call dielbseptemp(nthreads,dimbse,hbse,hrsp,hrsp,fdeh,act(:,7))
+	call dielbseptemp(nthreads,dimbse,hbse,hrsm,hrsm,fdeh,act(:,8))
+	else
 	call dielbsev(nthreads,dimbse,hbse,hrx,act(:,1))
 	call dielbsev(nthreads,dimbse,hbse,hry,act(:,2))
 	call dielbsev(nthreads,dimbse,hbse,hrz,act(:,3))
@@ -234,6 +290,7 @@ program ref_dump
 	call dielbsep(nthreads,dimbse,hbse,hry,hrz,act(:,6))
 	call dielbsev(nthreads,dimbse,hbse,hrsp,act(:,7))
 	call dielbsev(nthreads,dimbse,hbse,hrsm,act(:,8))
+	end if
 	t1 = omp_get_wtime(); tosc = t1-t0
 	call dump_r8("act", dimbse*8, act)
 

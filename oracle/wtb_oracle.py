@@ -34,6 +34,7 @@ reference's outputs depend on.
 """
 from __future__ import annotations
 
+import dataclasses
 import math
 
 import numpy as np
@@ -364,10 +365,14 @@ def integral_j0pot(rmax, gperp):
 # stage (c): BSE matrix
 # ---------------------------------------------------------------------------
 
-def bse_matrix(kpt, eigv, vector, stt, params, ngrid, rlat):
+def bse_matrix(kpt, eigv, vector, stt, params, ngrid, rlat, fdeh=None):
     """Full Hermitian BSE matrix whose upper triangle is matrizelbse's
     (bse_subs.f90:2-124, loop bse_solver-tool-diel.f90:431-458):
-      H_II = (Ec-Ev) + V(0);  H_IJ = V(k_I-k_J) <c_I|c_J> <v_I|v_J>  (J>I)."""
+      H_II = (Ec-Ev) + V(0);  H_IJ = V(k_I-k_J) <c_I|c_J> <v_I|v_J>  (J>I).
+    With fdeh (finite temperature, matrizelbsetemp, bse_subs_temp.f90:2-126 in the loop
+    j >= i of bse_solver-tool-diel-temp.f90:470-480): the kernel terms times the ROW
+    state's f_v - f_c, H_II = (Ec-Ev) + V(0) f_I; ZHEEV('U') reads the upper triangle only,
+    so the lower one is its conjugate."""
     ki = stt[:, 3] - 1
     vi = stt[:, 1] - 1
     ci = stt[:, 2] - 1
@@ -380,9 +385,12 @@ def bse_matrix(kpt, eigv, vector, stt, params, ngrid, rlat):
     V = coulomb(params["coultype"], dk, ngrid, rlat, params["ediel"], params["lc"],
                 params["ez"], params["w"], params["r0"], params["ktol"])
     H = V * oc * ov
+    if fdeh is not None:
+        H = H * fdeh[:, None]
     H = np.triu(H, 1)
     H = H + np.conj(H.T)
-    diag = (eigv[ki, ci] - eigv[ki, vi]) + np.diag(V)
+    vd = np.diag(V) if fdeh is None else np.diag(V) * fdeh
+    diag = (eigv[ki, ci] - eigv[ki, vi]) + vd
     H[np.arange(len(H)), np.arange(len(H))] = diag
     return H
 
@@ -415,6 +423,47 @@ def activities(X, hvec):
     sq = lambda a: (a * np.conj(a)).real
     cr = lambda a, b: (a * np.conj(b)).real
     return np.stack([sq(tx), sq(ty), sq(tz), cr(tx, ty), cr(tx, tz), cr(ty, tz), sq(tp), sq(tm)])
+
+
+# ---------------------------------------------------------------------------
+# finite temperature (bsesolvertemp, subprograms/bse_solver-tool-diel-temp.f90)
+# ---------------------------------------------------------------------------
+
+KB_F32 = float(fl32(8.617330350e-5))     # REAL literal into a double parameter, general_subs.f90:575,594,613
+T0_F32 = float(fl32(1e-8))               # temp = 1E-8 when temp == 0, general_subs.f90:620-624
+
+
+def gap_correction(ta, st, phavg, temp):
+    """tcor added to the scissor (bse_solver-tool-diel-temp.f90:262-283): FA 0,
+    VE gapcortemp (general_subs.f90:569-586), BE gapcortemp2 (:588-604)."""
+    with np.errstate(divide="ignore", over="ignore"):
+        if ta == "VE":
+            aux1 = np.float64(phavg) / (2.0 * KB_F32 * np.float64(temp))
+            aux2 = 1.0 / np.tanh(aux1)
+            return float(-st * phavg * (aux2 - 1.0))
+        if ta == "BE":
+            aux2 = np.exp(np.float64(phavg) / np.float64(temp))
+            return float(-(2.0 * st / (aux2 - 1.0)))
+    return 0.0
+
+
+def fermidisteh(ec, ev, temp):
+    """f_v - f_c with E_F = 0 (general_subs.f90:606-635); temp 0 is taken as 1E-8."""
+    t = temp if temp != 0.0 else T0_F32
+    beta = 1.0 / (KB_F32 * t)
+    with np.errstate(over="ignore"):
+        fv = 1.0 / (np.exp(np.asarray(ev) * beta) + 1.0)
+        fc = 1.0 / (np.exp(np.asarray(ec) * beta) + 1.0)
+    return fv - fc
+
+
+def activities_temp(X, hvec, fdeh):
+    """dielbseptemp (bse_subs_temp.f90:340-417) for all eight components
+    (bse_solver-tool-diel-temp.f90:570-618): Re[(X^T h_a) (X^T (h_b f))^*]."""
+    t = X.T @ hvec.T                                     # (N, 5)
+    tf = X.T @ (hvec * fdeh[None, :]).T
+    cr = lambda a, b: (t[:, a] * np.conj(tf[:, b])).real
+    return np.stack([cr(0, 0), cr(1, 1), cr(2, 2), cr(0, 1), cr(0, 2), cr(1, 2), cr(3, 3), cr(4, 4)])
 
 
 def f15_4(x):
@@ -505,7 +554,8 @@ def spectrum(E, act6, rlat, ngrid, ebse0, ebsef, numbse, sme, renorm=True):
 
 DEFAULT_PARAMS = dict(nc=2, nv=2, coultype="V2DRK", ediel=(1.0, 1.0, 1.0), lc=1.0, ez=1.0,
                       w=0.0, r0=40.0, ktol=0.001, cshift=0.05, shift=(0.0, 0.0, 0.0),
-                      ebse0=0.0, ebsef=6.0, numbse=1001, renorm=True, dtfull=True, f154=True)
+                      ebse0=0.0, ebsef=6.0, numbse=1001, renorm=True, dtfull=True, f154=True,
+                      bset=False, temp=0.0, ta="FA", st=0.0, phavg=0.0)
 
 
 def run_pipeline(model, ngrid, **kw):
@@ -513,18 +563,25 @@ def run_pipeline(model, ngrid, **kw):
     p = dict(DEFAULT_PARAMS)
     p.update(kw)
     kpt = monkhorst_pack(ngrid, p["shift"], model.rlat)
+    tcor = gap_correction(p["ta"], p["st"], p["phavg"], p["temp"]) if p["bset"] else 0.0
+    if tcor != 0.0:                                       # eigsys(scs+tcor), ...-temp.f90:289
+        model = dataclasses.replace(model, scs=model.scs + tcor)
     E, U, nocp = eigsys(kpt, model)
     eigv, vector = select_bands(E, U, nocp, p["nc"], p["nv"])
     stt = quantumnumbers2(len(kpt), p["nc"], p["nv"], nocp, model.nw)
     h = dipoles(kpt, model, eigv, vector, stt, p["cshift"])
-    H = bse_matrix(kpt, eigv, vector, stt, p, ngrid, model.rlat)
+    fdeh = None
+    if p["bset"]:                                         # ...-temp.f90:394-399
+        ki = stt[:, 3] - 1
+        fdeh = fermidisteh(eigv[ki, stt[:, 2] - 1], eigv[ki, stt[:, 1] - 1], p["temp"])
+    H = bse_matrix(kpt, eigv, vector, stt, p, ngrid, model.rlat, fdeh)
     W, X = bse_eig(H)
-    act = activities(X, h)
+    act = activities(X, h) if fdeh is None else activities_temp(X, h, fdeh)
     if not p["dtfull"]:
         act[3:6] = 0.0
     Wq, actq = (f15_4(W), f15_4(act[:6])) if p["f154"] else (W, act[:6])
     om, re, im = spectrum(Wq, actq, model.rlat, ngrid, p["ebse0"], p["ebsef"], p["numbse"],
                           p["cshift"], p["renorm"])
     spinf = 2.0 if model.systype.strip() == "NP" else 1.0
-    return dict(kpt=kpt, E=E, U=U, nocp=nocp, eigv=eigv, vector=vector, stt=stt, h=h, H=H,
+    return dict(kpt=kpt, E=E, U=U, nocp=nocp, eigv=eigv, vector=vector, stt=stt, h=h, H=H, tcor=tcor, fdeh=fdeh,
                 W=W, X=X, act=act, Wq=Wq, actq=actq, omega=om, eps_re=re, eps_im=im, spinf=spinf)

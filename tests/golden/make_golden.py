@@ -68,6 +68,10 @@ def input_text(p, tbname, nthreads=4):
         f"NESPEC= {p['numbse']}", f"RNMD= {T(p['renorm'])}",
         f"SHIFT_1= {p['shift'][0]!r}", f"SHIFT_2= {p['shift'][1]!r}", f"SHIFT_3= {p['shift'][2]!r}",
     ]
+    if p.get("bset"):                       # finite temperature (bsesolvertemp's variant)
+        lines[lines.index("BSE= T")] = "BSE= F"
+        lines += ["BSET= T", f"TEMP= {p['temp']!r}", f"TA= {p['ta']}", f"ST= {p['st']!r}",
+                  f"PHAVG= {p['phavg']!r}"]
     return "\n".join(lines) + "\n"
 
 
@@ -105,6 +109,17 @@ def cases():
     for tag, ct, extra in pots:
         out.append((f"hex6_g3_{tag}", (lambda s=len(out): tbmodel.hex2d(100 + s, 6, (3, 3, 1))),
                     dict(BASE, ngrid=(3, 3, 1), coultype=ct, **extra), True))
+    # finite temperature: f_v - f_c visibly below 1 (kT ~ 0.2-0.5 eV against a ~2 eV gap),
+    # the three gap-correction models, and T = 0 (taken as 1E-8 by the reference)
+    TB = dict(BASE, bset=True, st=0.0, phavg=0.0)
+    out.append(("hex8_g6_t6000_fa", lambda: tbmodel.hex2d(12, 8, (6, 6, 1)),
+                dict(TB, ngrid=(6, 6, 1), temp=6000.0, ta="FA"), True))
+    out.append(("hex6_g4_t2500_ve", lambda: tbmodel.hex2d(31, 6, (4, 4, 1)),
+                dict(TB, ngrid=(4, 4, 1), temp=2500.0, ta="VE", st=0.5, phavg=0.03), True))
+    out.append(("hex6_g4_t4000_be", lambda: tbmodel.hex2d(32, 6, (4, 4, 1)),
+                dict(TB, ngrid=(4, 4, 1), temp=4000.0, ta="BE", st=0.05, phavg=2000.0, nc=1), True))
+    out.append(("hex6_g3_t0_be", lambda: tbmodel.hex2d(33, 6, (3, 3, 1)),
+                dict(TB, ngrid=(3, 3, 1), temp=0.0, ta="BE", st=1.0, phavg=200.0), True))
     return out
 
 
@@ -149,6 +164,9 @@ def run_case(name, factory, p, full):
     )
     if full:
         fx["hbse_up"] = d["hbse_up"]
+    fx["tcor"] = d["tcor"]
+    if p.get("bset"):
+        fx["fdeh"] = d["fdeh"]
     np.savez_compressed(OUTDIR / f"{name}.npz", **fx)
     return N
 

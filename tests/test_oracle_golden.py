@@ -180,3 +180,27 @@ def test_delta_table_is_unfolded():
     assert d.shape == (6, 6, 3)
     assert d[:, :, 0].min() == -2 and d[:, :, 0].max() == 2
     assert (d + d.transpose(1, 0, 2) == 0).all()
+
+
+def test_finite_temperature_factors(case):
+    """bsesolvertemp: the gap correction added to the scissor and f_v - f_c per state."""
+    name, m, p, z, r = case
+    if not p.get("bset"):
+        pytest.skip("T = 0 case")
+    # tanh / exp of two libraries: equal to the last unit or two
+    assert abs(r["tcor"] - z["tcor"][0]) <= 1e-15 * max(abs(z["tcor"][0]), 1.0)
+    assert np.abs(r["fdeh"] - z["fdeh"]).max() < 1e-14
+
+
+def test_fermi_factor_closed_forms():
+    """f_v - f_c is 1 for a gapped state at T = 0 (taken as 1E-8 K), tends to 0 when both
+    energies sit far on the same side of E_F, and equals tanh-form (1/2)(tanh(b ec/2) -
+    tanh(b ev/2)) at any T; the VE and BE gap corrections vanish at T = 0."""
+    assert O.fermidisteh(1.0, -1.0, 0.0) == 1.0
+    b = 1.0 / (O.KB_F32 * 3000.0)
+    for ec, ev in [(0.8, -1.1), (2.0, -0.3), (0.1, -0.05)]:
+        ref = 0.5 * (math.tanh(b * ec / 2) - math.tanh(b * ev / 2))
+        assert abs(O.fermidisteh(ec, ev, 3000.0) - ref) < 1e-14
+    assert O.gap_correction("VE", 0.5, 0.03, 0.0) == 0.0
+    assert O.gap_correction("BE", 0.5, 200.0, 0.0) == 0.0
+    assert O.KB_F32 == float(np.float32(8.617330350e-5))

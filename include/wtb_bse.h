@@ -96,7 +96,22 @@ typedef struct {
   int f154;              /* 1: round energies/activities as the bse_opt_diel.dat
                             "(7F15.4)" write/read does before eps(omega)              */
   int precision;         /* WTB_FP64 or WTB_MIXED                                     */
+  /* finite temperature (bsesolvertemp, subprograms/bse_solver-tool-diel-temp.f90),
+   * BSET= T; all zero: bsesolver's T = 0 path */
+  int bset;              /* 1: kernel terms times f_v - f_c of the row state, activities
+                            Re[(X^T h_a) conj(X^T (f h_b))] (bse_subs_temp.f90:2-126,340-417) */
+  int ta;                /* TA: WTB_TA_FA, WTB_TA_VE or WTB_TA_BE (gap correction)       */
+  double temp;           /* TEMP (K); 0 is taken as 1E-8 (general_subs.f90:620-624)      */
+  double st, phavg;      /* ST, PHAVG: parameters of the gap correction                 */
 } wtb_bse_params;
+
+/* ---- TA selector: the gap correction tcor added to the scissor
+ *      (bse_solver-tool-diel-temp.f90:262-283) -------------------------------- */
+enum {
+  WTB_TA_FA = 0,   /* "FA" and any other value: tcor = 0                              */
+  WTB_TA_VE = 1,   /* "VE": gapcortemp  (general_subs.f90:569-586)                    */
+  WTB_TA_BE = 2    /* "BE": gapcortemp2 (general_subs.f90:588-604)                    */
+};
 
 /* ---- context ---------------------------------------------------------------- */
 int wtb_create(int device, wtb_ctx** ctx);
@@ -154,7 +169,10 @@ int wtb_bse_spectrum(wtb_ctx* ctx, int n, const double* e, const double* act6,
  *   act(N,8)       xx,yy,zz,xy,xz,yz,sp,sm activities (zero where the reference
  *                  zeroes them: dtfull/cpol off)
  *   eps(numbse,2,6) as wtb_bse_spectrum (may be NULL)
- *   egap           direct gap logged by the reference (:305-317) (may be NULL)  */
+ *   egap           direct gap logged by the reference (:305-317) (may be NULL)
+ * With prm->bset = 1 it is bsesolvertemp
+ * (subprograms/bse_solver-tool-diel-temp.f90:2-660) instead: bands with the
+ * scissor scs + tcor, the matrix of matrizelbsetemp, act from dielbseptemp.  */
 int wtb_bsesolver(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm,
                   double* w, double* act, double* eps, double* egap);
 
@@ -186,7 +204,8 @@ enum {
   WTB_BUF_WTAB = 7,      /* W(Delta) table, [D1][D2][D3] double                       */
   WTB_BUF_TRIDIAG = 8,   /* d[N] then e[N] double (after the reduction)                */
   WTB_BUF_EXCWF = 9,     /* exciton vectors [count][N] complex (wtb_bse_plan_set_vectors) */
-  WTB_BUF_NBUF = 10
+  WTB_BUF_FDEH = 10,     /* f_v - f_c per basis state [N] double (bset = 1)              */
+  WTB_BUF_NBUF = 11
 };
 size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* plan, int what);
 int wtb_bse_plan_fetch(wtb_bse_plan* plan, int what, void* dst, size_t bytes);

@@ -41,3 +41,32 @@ def test_no_device_fails_loudly():
         return
     c.close()
     pytest.skip("a GPU is present")
+
+
+def c_offsets(tmp_path, struct, fields):
+    src = tmp_path / "off.c"
+    body = "".join(f'  printf("%s %zu\\n", "{f}", offsetof({struct}, {f}));\n' for f in fields)
+    src.write_text(f'#include <stdio.h>\n#include <stddef.h>\n#include "wtb_bse.h"\nint main(void) {{\n'
+                   f'  printf("sizeof %zu\\n", sizeof({struct}));\n{body}  return 0;\n}}\n')
+    exe = tmp_path / "off"
+    import subprocess
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    return {k: int(v) for k, v in (l.split() for l in out.splitlines())}
+
+
+@pytest.mark.parametrize("struct,pyname", [("wtb_bse_params", "_Params"),
+                                           ("wtb_davidson_params", "_DavidsonParams"),
+                                           ("wtb_davidson_info", "_DavidsonInfo")])
+def test_ctypes_structures_match_the_header(tmp_path, struct, pyname):
+    """The ctypes mirrors of the parameter structures have the C layout (gcc's offsetof)."""
+    import shutil
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    from wantibexos_amd import bse
+    cls = getattr(bse, pyname)
+    names = [f for f, _ in cls._fields_]
+    off = c_offsets(tmp_path, struct, names)
+    assert off["sizeof"] == ctypes.sizeof(cls)
+    for f in names:
+        assert off[f] == getattr(cls, f).offset, f

@@ -1,8 +1,8 @@
 """Drop-in check at the Fortran boundary: the reference program (oracle/_ref/wtb.x,
 compiled from the reference sources) and the GPU drop-in driver
 (wantibexos_amd/fortran/_build/wtb_gpu.x: the reference's input parser + the
-drop-in bsesolver/bsedielraw over ISO_C_BINDING) run on the same input.dat and
-TB file; their bse_opt_diel.dat and bse_diel_*.dat must agree."""
+drop-in bsesolver/bsesolvertemp/bsedielraw over ISO_C_BINDING) run on the same
+input.dat and TB file; their bse_opt_diel.dat and bse_diel_*.dat must agree."""
 import os
 import subprocess
 from pathlib import Path
@@ -38,12 +38,16 @@ def table(path, skip=1):
     return np.array([[float(x) for x in r] for r in rows])
 
 
+FINITE_T = dict(P, bset=True, temp=3000.0, ta="VE", st=0.3, phavg=0.05)
+
+
 @pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
-def test_dropin_outputs_match_reference(tmp_path):
+@pytest.mark.parametrize("p", [P, FINITE_T], ids=["bsesolver", "bsesolvertemp"])
+def test_dropin_outputs_match_reference(tmp_path, p):
     model = tbmodel.hex2d(12, 8, P["ngrid"])
     tbmodel.write_tb(model, tmp_path / "tb.dat")
-    rr = run(REF, tmp_path, "ref")
-    rg = run(GPU, tmp_path, "gpu")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
     assert rg.returncode == 0, rg.stdout + rg.stderr
     assert (tmp_path / "ref" / "bse_opt_diel.dat").exists(), rr.stdout + rr.stderr
     a = table(tmp_path / "ref" / "bse_opt_diel.dat")

@@ -30,7 +30,9 @@ def params_of(p):
                          coultype=p["coultype"], ediel=p["ediel"], lc=p["lc"], ez=p["ez"], w=p["w"],
                          r0=p["r0"], ktol=p["ktol"], cshift=p["cshift"], ebse0=p["ebse0"],
                          ebsef=p["ebsef"], numbse=p["numbse"], dtfull=p["dtfull"], cpol=True,
-                         renorm=p["renorm"], f154=True)
+                         renorm=p["renorm"], f154=True, bset=p.get("bset", False),
+                         temp=p.get("temp", 0.0), ta=p.get("ta", "FA"), st=p.get("st", 0.0),
+                         phavg=p.get("phavg", 0.0))
 
 
 def relerr(a, b):
@@ -47,8 +49,11 @@ def test_monkhorst_pack_bit_exact(ctx, name):
 
 @pytest.mark.parametrize("name", CASES)
 def test_eigsys(ctx, name):
+    import dataclasses
     from wantibexos_amd import bse
     m, p, z = load(name)
+    if p.get("bset"):          # bsesolvertemp's eigsys(scs + tcor, ...)
+        m = dataclasses.replace(m, scs=m.scs + float(z["tcor"][0]))
     E, U, nocp = bse.eigsys(ctx, m, z["kpt"])
     assert np.array_equal(nocp, z["nocpk"])
     assert np.abs(E - z["eall"]).max() < 1e-11
@@ -78,6 +83,7 @@ def test_pipeline_against_reference(ctx, name):
         eigv = plan.fetch("eigv")
         vec = plan.fetch("vector")
         dip = plan.fetch("dipole")
+        fdeh = plan.fetch("fdeh") if p.get("bset") else None
     assert np.array_equal(kpt, z["kpt"])
     assert np.array_equal(nocp, z["nocpk"])
     assert np.abs(eigv - z["eigv"]).max() < E_TOL * 1e-3
@@ -89,6 +95,9 @@ def test_pipeline_against_reference(ctx, name):
     o = O.run_pipeline(m, p["ngrid"], **oracle_kwargs(p))
     L = np.tril(H)
     assert np.abs(L - np.tril(o["H"])).max() < 1e-12
+    if p.get("bset"):
+        # f_v - f_c (device exp against the reference's libm: last-unit differences)
+        assert np.abs(fdeh - z["fdeh"]).max() < 1e-14
     # exciton energies and activities vs the reference (MKL ZHEEV)
     assert np.abs(r["W"] - z["wbse"]).max() < E_TOL
     comps = [0, 1, 2, 6, 7] + ([3, 4, 5] if p["dtfull"] else [])

@@ -153,6 +153,23 @@ def test_activity_sum_rule():
         assert math.isclose(act[c].sum(), np.vdot(h[comp], h[comp]).real, rel_tol=1e-12)
 
 
+def test_finite_temperature_activity_sum_rule():
+    """dielbseptemp summed over excitons: sum_n Re[(X^T h_a)_n conj(X^T (f h_b))_n]
+    = Re[sum_j h_a,j conj(h_b,j) f_j] (X unitary); f on the wrong vector, a missing
+    conjugate or swapped components break it."""
+    rng = np.random.default_rng(1)
+    n = 40
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    W, X = O.bse_eig(A + A.conj().T)
+    h = rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))
+    f = rng.uniform(0.2, 1.0, n)
+    act = O.activities_temp(X, h, f)
+    for c, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2), (3, 3), (4, 4)]):
+        assert math.isclose(act[c].sum(), (h[a] * np.conj(h[b]) * f).sum().real, rel_tol=1e-11, abs_tol=1e-11)
+    # f = 1: the T = 0 activities
+    assert np.allclose(O.activities_temp(X, h, np.ones(n)), O.activities(X, h), rtol=1e-13, atol=1e-13)
+
+
 def test_f154_rounding():
     x = np.array([1.23455, 1.23445, 0.00004999, -0.00006, 2.5])
     assert list(O.f15_4(x)) == [float(f"{v:.4f}") for v in x]

@@ -38,7 +38,9 @@ ERRORS = {
 COULOMB = {"V2DK": 1, "V3D": 2, "V3DL": 3, "V2DT": 4, "V2DT2": 5, "V2DOH": 6, "V2DRK": 7,
            "V1DT": 8, "V0DT": 9}
 FP64, MIXED = 0, 1
-BUF = dict(kpt=0, eall=1, nocp=2, eigv=3, vector=4, dipole=5, hbse=6, wtab=7, tridiag=8, excwf=9)
+# TA (bse_solver-tool-diel-temp.f90:262-283): any value other than VE and BE means no gap correction
+TA = {"FA": 0, "VE": 1, "BE": 2}
+BUF = dict(kpt=0, eall=1, nocp=2, eigv=3, vector=4, dipole=5, hbse=6, wtab=7, tridiag=8, excwf=9, fdeh=10)
 NSTAGES = 6
 STAGES = ["eig", "dipoles", "build", "tridiag", "dc", "spectrum"]
 
@@ -61,7 +63,8 @@ class _Params(C.Structure):
                 ("ez", C.c_double), ("w", C.c_double), ("r0", C.c_double), ("ktol", C.c_double),
                 ("cshift", C.c_double), ("ebse0", C.c_double), ("ebsef", C.c_double),
                 ("numbse", C.c_int), ("dtfull", C.c_int), ("cpol", C.c_int), ("renorm", C.c_int),
-                ("f154", C.c_int), ("precision", C.c_int)]
+                ("f154", C.c_int), ("precision", C.c_int), ("bset", C.c_int), ("ta", C.c_int),
+                ("temp", C.c_double), ("st", C.c_double), ("phavg", C.c_double)]
 
 
 class _DavidsonParams(C.Structure):
@@ -181,6 +184,12 @@ class BSEParams:
     renorm: bool = True                # RNMD
     f154: bool = True                  # bse_opt_diel.dat F15.4 round trip
     precision: int = FP64
+    # finite temperature (BSET= T: bsesolvertemp)
+    bset: bool = False                 # BSET
+    temp: float = 0.0                  # TEMP (K)
+    ta: str = "FA"                     # TA: FA, VE or BE
+    st: float = 0.0                    # ST
+    phavg: float = 0.0                 # PHAVG
 
     @property
     def nk(self):
@@ -204,6 +213,9 @@ class BSEParams:
         p.dtfull, p.cpol, p.renorm, p.f154 = (int(bool(v)) for v in
                                               (self.dtfull, self.cpol, self.renorm, self.f154))
         p.precision = int(self.precision)
+        p.bset = int(bool(self.bset))
+        p.ta = TA.get(str(self.ta).strip().upper(), 0)
+        p.temp, p.st, p.phavg = map(float, (self.temp, self.st, self.phavg))
         return p
 
 
@@ -384,6 +396,8 @@ class BSEPlan:
             return raw.view(np.float64).reshape(2, N)
         if what == "excwf":
             return raw.view(np.complex128).reshape(-1, N).T            # (N, count)
+        if what == "fdeh":
+            return raw.view(np.float64).copy()
         return raw
 
     def lowest(self, nev=256, guard=32, max_sub=None, max_iter=60, tol=1e-6, lanczos_steps=250,

@@ -255,7 +255,7 @@ template <bool F32>
 __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* __restrict__ wtab,
                                                    const double* __restrict__ eigv,
                                                    const zc* __restrict__ vec, zc* __restrict__ H,
-                                                   zf* __restrict__ Hf) {
+                                                   zf* __restrict__ Hf, const double* __restrict__ fdeh) {
   extern __shared__ unsigned char smem[];
   // lower-triangle tile index -> (tr, tc), tr >= tc
   int t = blockIdx.x;
@@ -297,11 +297,15 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
     zc val;
     if (I == J) {
       double ec = eigv[(size_t)k * ba.nb + ba.nv + ci], ev = eigv[(size_t)k * ba.nb + vi];
-      val = mkz((ec - ev) + wv, 0.0);
+      // finite temperature: (Ec - Ev) + V(0) f_I (bse_subs_temp.f90:105-108)
+      val = mkz((ec - ev) + (fdeh ? wv * fdeh[J] : wv), 0.0);
     } else {
       zc oc = Oc[(kl * ba.nc + ci) * rc_ + (kl2 * ba.nc + cj)];
       zc ov = Ov[(kl * ba.nv + vi) * rv_ + (kl2 * ba.nv + vj)];
       val = (wv * oc) * ov;
+      // finite temperature: the reference's element (J, I), J < I, is V vc vv f_J
+      // (bse_subs_temp.f90:112-120); this one is its conjugate
+      if (fdeh) val = val * fdeh[J];
     }
     size_t off = (size_t)I + (size_t)ba.ldh * J;
     if constexpr (F32) stz(Hf[off], val);
@@ -318,7 +322,7 @@ int launch_wtable(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, do
 
 int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng[3],
                      const double* d_wtab, const double* d_eigv, const zc* d_vec, zc* d_H, int ldh,
-                     zf* d_Hf) {
+                     zf* d_Hf, const double* d_fdeh) {
   BuildArgs ba;
   ba.nk = nk;
   ba.nc = nc;
@@ -343,10 +347,46 @@ int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng
   size_t lds = sizeof(zc) * ((size_t)tk * nc * tk * nc + (size_t)tk * nv * tk * nv);
   if (d_H)
     hipLaunchKernelGGL(k_bse_build<false>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
-                       d_H, (zf*)nullptr);
+                       d_H, (zf*)nullptr, d_fdeh);
   if (d_Hf)
     hipLaunchKernelGGL(k_bse_build<true>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
-                       (zc*)nullptr, d_Hf);
+                       (zc*)nullptr, d_Hf, d_fdeh);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+// fermidisteh (general_subs.f90:606-635) with E_F = 0 for every basis state
+// I = (k nc + c) nv + v, the order of the assembly
+__global__ void k_fermi_factors(int N, int nc, int nv, const double* __restrict__ eigv, double beta,
+                                double* __restrict__ fdeh) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= N) return;
+  const int ncv = nc * nv, nb = nc + nv;
+  const int k = I / ncv, c = (I % ncv) / nv, v = I % nv;
+  const double ec = eigv[(size_t)k * nb + nv + c], ev = eigv[(size_t)k * nb + v];
+  const double fv = 1.0 / (exp(ev * beta) + 1.0);
+  const double fc = 1.0 / (exp(ec * beta) + 1.0);
+  fdeh[I] = fv - fc;
+}
+
+int launch_fermi_factors(hipStream_t s, int nk, int nc, int nv, const double* d_eigv, double beta,
+                         double* d_fdeh) {
+  const int N = nk * nc * nv;
+  hipLaunchKernelGGL(k_fermi_factors, dim3((N + 255) / 256), dim3(256), 0, s, N, nc, nv, d_eigv, beta, d_fdeh);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+__global__ void k_scale_rows(int nvec, int N, const zc* __restrict__ x, const double* __restrict__ f,
+                             zc* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double fi = f[i];
+  for (int q = 0; q < nvec; ++q) y[(size_t)q * N + i] = x[(size_t)q * N + i] * fi;
+}
+
+int launch_scale_rows(hipStream_t s, int nvec, int N, const zc* x, const double* f, zc* y) {
+  hipLaunchKernelGGL(k_scale_rows, dim3((N + 255) / 256), dim3(256), 0, s, nvec, N, x, f, y);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

@@ -350,7 +350,9 @@ __global__ __launch_bounds__(256) void k_dc_lowner(DCArgs a) {
   if (active && sub == 0) a.what[nd.lo + i] = copysign(sqrt(fmax(-p, 0.0)), a.cw[nd.lo + i]);
 }
 
-// M5: new eigenvalues and U^T x for the tracked sources; deflated slots copy through
+// M5: new eigenvalues and U^T x for the tracked sources; deflated slots copy through.
+// NXM >= 2 + a.nt: the accumulators (10 for up to 8 tracked sources, 14 for 12)
+template <int NXM>
 __global__ __launch_bounds__(256) void k_dc_apply(DCArgs a) {
   const int node = blockIdx.y;
   const DCNode nd = a.nodes[node];
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(256) void k_dc_apply(DCArgs a) {
   const int* org = a.org + lo;
   const double* tau = a.tau + lo;
   const double* wh = a.what + lo;
-  double acc[10];
+  double acc[NXM];
   for (int q = 0; q < nx; ++q) acc[q] = 0.0;
   double nrm = 0.0;
   for (int i = sub; i < K; i += 16) {
@@ -638,7 +640,10 @@ int dc_prepare(void* work, int N, int ntrack, DCTree& tree, int vectors) {
 int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, const double* d_e,
            double* d_track, double* d_lambda, double* d_Z) {
   const int N = tree.N;
-  if (tree.ntrack > 8) return WTB_ERR_LIMIT;
+  if (tree.ntrack > 12) {
+    set_error("divide and conquer: at most 12 tracked vectors");
+    return WTB_ERR_LIMIT;
+  }
   if (d_Z && !tree.vectors) {
     set_error("divide and conquer: eigenvectors requested from a plan prepared without them");
     return WTB_ERR_ARG;
@@ -666,7 +671,8 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
     hipLaunchKernelGGL(k_dc_compact, g1, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_secular, g16, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_lowner, g16, dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_dc_apply, g16, dim3(256), 0, s, a);
+    if (tree.ntrack <= 8) hipLaunchKernelGGL(k_dc_apply<10>, g16, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_dc_apply<14>, g16, dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dc_sort, g1, dim3(256), 0, s, a);
     if (d_Z) {
       hipLaunchKernelGGL(k_dcv_gather, dim3(maxn, cnt), dim3(256), 0, s, a);

@@ -34,6 +34,30 @@ __global__ void k_activities(int N, const double* __restrict__ G, int dtfull, in
   act[7 * (size_t)N + n] = cpol ? sq(tm) : 0.0;
 }
 
+// dielbseptemp (bse_subs_temp.f90:340-417) for the eight components
+// (bse_solver-tool-diel-temp.f90:570-626): Re[(X^T h_a) conj(X^T (f h_b))], where
+// G rows 0-5 hold X^T h (x, y, z as re, im) and rows 6-11 X^T (f h)
+__global__ void k_activities_temp(int N, const double* __restrict__ G, int dtfull, int cpol,
+                                  double* __restrict__ act) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  auto row = [&](int q) { return mkz(G[(size_t)(2 * q) * N + n], G[(size_t)(2 * q + 1) * N + n]); };
+  const zc tx = row(0), ty = row(1), tz = row(2), fx = row(3), fy = row(4), fz = row(5);
+  auto cr = [](zc a, zc b) { return a.re * b.re + a.im * b.im; };   // Re(a conj(b))
+  act[n] = cr(tx, fx);
+  act[(size_t)N + n] = cr(ty, fy);
+  act[2 * (size_t)N + n] = cr(tz, fz);
+  act[3 * (size_t)N + n] = dtfull ? cr(tx, fy) : 0.0;
+  act[4 * (size_t)N + n] = dtfull ? cr(tx, fz) : 0.0;
+  act[5 * (size_t)N + n] = dtfull ? cr(ty, fz) : 0.0;
+  // hrsp, hrsm as at T = 0 (bse_solver-tool-diel-temp.f90:406-407); X^T is linear
+  const double c = (double)(1.0f / sqrtf(2.0f));
+  const zc tp = mkz(tx.re - ty.im, tx.im + ty.re) * c, tm = mkz(tx.re + ty.im, tx.im - ty.re) * c;
+  const zc fp = mkz(fx.re - fy.im, fx.im + fy.re) * c, fm = mkz(fx.re + fy.im, fx.im - fy.re) * c;
+  act[6 * (size_t)N + n] = cpol ? cr(tp, fp) : 0.0;
+  act[7 * (size_t)N + n] = cpol ? cr(tm, fm) : 0.0;
+}
+
 // x -> the value a "(F15.4)" write and a list-directed read give back: the
 // decimal rounding of the exact binary value to 4 places, then the nearest
 // double of that decimal.
@@ -193,6 +217,12 @@ __global__ __launch_bounds__(256) void k_xth(int n, const zc* __restrict__ X, co
 
 int launch_activities(hipStream_t s, int N, const double* d_G, int dtfull, int cpol, double* d_act) {
   hipLaunchKernelGGL(k_activities, dim3((N + 255) / 256), dim3(256), 0, s, N, d_G, dtfull, cpol, d_act);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int launch_activities_temp(hipStream_t s, int N, const double* d_G, int dtfull, int cpol, double* d_act) {
+  hipLaunchKernelGGL(k_activities_temp, dim3((N + 255) / 256), dim3(256), 0, s, N, d_G, dtfull, cpol, d_act);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

@@ -17,10 +17,10 @@ __global__ void k_zero_int(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
-__global__ void k_split_g(int N, const zc* __restrict__ g, double* __restrict__ track) {
+__global__ void k_split_g(int N, int nrhs, const zc* __restrict__ g, double* __restrict__ track) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < nrhs; ++q) {
     zc v = g[(size_t)q * N + i];
     track[(size_t)(2 * q) * N + i] = v.re;
     track[(size_t)(2 * q + 1) * N + i] = v.im;
@@ -81,6 +81,10 @@ struct wtb_bse_plan {
   wtb_bse_params prm{};
   int nk = 0, nw = 0, nvec = 0, nb = 0, N = 0;
   double ef32 = 0, scs = 0, sme32 = 0, vc3d = 0, nkf32 = 0;
+  // finite temperature: gap correction (added to scs), 1/(kB T) of the Fermi factors,
+  // and the tracked vectors: h (3) at T = 0, h and f h (6) otherwise
+  double tcor = 0, beta = 0;
+  int nrhs = 3;
   CoulombArgs ca{};
   DevModel dm;
   double* kpt = nullptr;
@@ -92,6 +96,7 @@ struct wtb_bse_plan {
   zc* vec = nullptr;
   zc* dip = nullptr;
   zc* g = nullptr;
+  double* fdeh = nullptr;   // f_v - f_c per basis state (finite temperature)
   double* gapk = nullptr;
   double* wtab = nullptr;
   int wtab_n = 0;
@@ -339,7 +344,7 @@ int wtb_eigsys(wtb_ctx* c, const wtb_tb_model* m, int nk, const double* kpt, dou
 static void plan_free(wtb_bse_plan* p) {
   free_model(p->dm);
   void* bufs[] = {p->kpt, p->phase, p->hk4, p->eall, p->nocp, p->eigv, p->vec, p->dip, p->g,
-                  p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
+                  p->fdeh, p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
                   p->wq, p->actq, p->eps, p->smeavg, p->status, p->Zv, p->Xv, p->btw};
   for (void* b : bufs) hipFree(b);
   for (auto& e : p->ev)
@@ -358,6 +363,31 @@ static double vcell3d_host(const double* rl) {
   double vx1 = (a[0][2] * a[1][0]) - (a[0][0] * a[1][2]);
   double vx2 = (a[0][0] * a[1][1]) - (a[0][1] * a[1][0]);
   return std::fabs(vx0 * a[2][0] + vx1 * a[2][1] + vx2 * a[2][2]);
+}
+
+// k_B of general_subs.f90:575,594,613: a REAL literal stored in a double parameter
+static const double kKbF32 = (double)8.617330350e-5f;
+
+// tcor of bse_solver-tool-diel-temp.f90:262-283: "VE" gapcortemp (general_subs.f90:569-586),
+// "BE" gapcortemp2 (:588-604), otherwise 0
+static double gap_correction(int ta, double st, double phavg, double temp) {
+  if (ta == WTB_TA_VE) {
+    const double aux1 = phavg / (2.0 * kKbF32 * temp);
+    const double aux2 = 1.0 / std::tanh(aux1);
+    return -(st * phavg * (aux2 - 1.0));
+  }
+  if (ta == WTB_TA_BE) {
+    const double aux1 = 2.0 * st;
+    const double aux2 = std::exp(phavg / temp);
+    return -(aux1 / (aux2 - 1.0));
+  }
+  return 0.0;
+}
+
+// beta = 1/(k_B T) of fermidisteh (general_subs.f90:606-635); T = 0 is taken as the REAL 1E-8
+static double fermi_beta(double temp) {
+  const double t = temp == 0.0 ? (double)1e-8f : temp;
+  return 1.0 / (kKbF32 * t);
 }
 
 int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm,
@@ -386,6 +416,20 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
     set_error("wtb_bse_plan_create: unknown precision form");
     return WTB_ERR_ARG;
   }
+  double tcor = 0.0;
+  if (prm->bset) {
+    if (prm->bset != 1 || prm->ta < WTB_TA_FA || prm->ta > WTB_TA_BE || !(prm->temp >= 0.0) ||
+        !std::isfinite(prm->temp) || !std::isfinite(prm->st) || !std::isfinite(prm->phavg)) {
+      set_error("wtb_bse_plan_create: finite temperature needs bset = 1, TA in {FA, VE, BE}, finite TEMP >= 0, "
+                "ST, PHAVG");
+      return WTB_ERR_ARG;
+    }
+    tcor = gap_correction(prm->ta, prm->st, prm->phavg, prm->temp);
+    if (!std::isfinite(tcor)) {
+      set_error("wtb_bse_plan_create: the gap correction is not finite (TEMP = 0 with PHAVG = 0?)");
+      return WTB_ERR_NONFINITE;
+    }
+  }
   WTB_HIP(hipSetDevice(c->device));
   wtb_bse_plan* p = new wtb_bse_plan;
   p->ctx = c;
@@ -398,7 +442,11 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
   p->nb = prm->nc + prm->nv;
   p->N = p->nk * prm->nc * prm->nv;
   p->ef32 = (double)(float)(-m->efermi);
-  p->scs = m->scs;
+  // eigsys(scs + tcor, ...) (bse_solver-tool-diel-temp.f90:289)
+  p->tcor = tcor;
+  p->scs = m->scs + tcor;
+  p->nrhs = prm->bset ? 6 : 3;
+  p->beta = prm->bset ? fermi_beta(prm->temp) : 0.0;
   p->sme32 = (double)(float)prm->cshift;
   p->vc3d = vcell3d_host(m->rlat);
   p->nkf32 = (double)(float)p->nk;
@@ -422,12 +470,13 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
       (rc = dmalloc(&p->hk4, (size_t)4 * nk * nw * nw)) || (rc = dmalloc(&p->eall, (size_t)nk * nw)) ||
       (rc = dmalloc(&p->nocp, nk)) || (rc = dmalloc(&p->eigv, (size_t)nk * p->nb)) ||
       (rc = dmalloc(&p->vec, (size_t)nk * p->nb * nw)) || (rc = dmalloc(&p->dip, (size_t)3 * N)) ||
-      (rc = dmalloc(&p->g, (size_t)3 * N)) || (rc = dmalloc(&p->gapk, nk)) ||
+      (rc = dmalloc(&p->g, (size_t)p->nrhs * N)) || (rc = dmalloc(&p->gapk, nk)) ||
+      (rc = prm->bset ? dmalloc(&p->fdeh, N) : WTB_OK) ||
       (rc = dmalloc(&p->wtab, p->wtab_n)) ||
       (rc = (prm->precision == WTB_MIXED) ? dmalloc(&p->Hf, p->ldh * N) : dmalloc(&p->H, p->ldh * N)) ||
-      (rc = dmalloc((char**)&p->triwork, tridiag_work_bytes(N, nb_panel, 3))) ||
-      (rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 6))) ||
-      (rc = dmalloc(&p->track, (size_t)6 * N)) || (rc = dmalloc(&p->lambda, N)) ||
+      (rc = dmalloc((char**)&p->triwork, tridiag_work_bytes(N, nb_panel, p->nrhs))) ||
+      (rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 2 * p->nrhs))) ||
+      (rc = dmalloc(&p->track, (size_t)2 * p->nrhs * N)) || (rc = dmalloc(&p->lambda, N)) ||
       (rc = dmalloc(&p->act, (size_t)8 * N)) || (rc = dmalloc(&p->wq, N)) ||
       (rc = dmalloc(&p->actq, (size_t)6 * N)) || (rc = dmalloc(&p->eps, (size_t)12 * prm->numbse)) ||
       (rc = dmalloc(&p->smeavg, 4)) || (rc = dmalloc(&p->status, 4))) {
@@ -435,8 +484,8 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
     delete p;
     return rc;
   }
-  tridiag_bind(p->tw, p->triwork, N, nb_panel, 3);
-  if ((rc = dc_prepare(p->dcwork, N, 6, p->tree))) {
+  tridiag_bind(p->tw, p->triwork, N, nb_panel, p->nrhs);
+  if ((rc = dc_prepare(p->dcwork, N, 2 * p->nrhs, p->tree))) {
     plan_free(p);
     delete p;
     return rc;
@@ -492,16 +541,19 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   mark(1);
   if ((rc = launch_dipoles(s, p->hk4, nk, nw, P.nc, P.nv, p->sme32, p->eigv, p->vec, p->dip, p->gapk)))
     return rc;
+  // finite temperature: f_v - f_c of every basis state
+  if (P.bset && (rc = launch_fermi_factors(s, nk, P.nc, P.nv, p->eigv, p->beta, p->fdeh))) return rc;
   mark(2);
   // (c) W table + assembly (lower triangle)
   if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
   if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H,
-                             (int)p->ldh, p->Hf)))
+                             (int)p->ldh, p->Hf, p->fdeh)))
     return rc;
   mark(3);
   if (build_only) return WTB_OK;
-  // (d) Householder tridiagonalisation, g <- Q^T h
+  // (d) Householder tridiagonalisation, g <- Q^T h (and Q^T f h at finite temperature)
   WTB_HIP(hipMemcpyAsync(p->g, p->dip, sizeof(zc) * 3 * N, hipMemcpyDeviceToDevice, s));
+  if (P.bset && (rc = launch_scale_rows(s, 3, N, p->dip, p->fdeh, p->g + (size_t)3 * N))) return rc;
   p->tw.kev = p->kernel_timing ? p->kev.data() : nullptr;
   p->tw.kev_cap = p->kernel_timing ? p->N : 0;
   if (p->mixed) {
@@ -511,7 +563,7 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   }
   mark(4);
   // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
-  hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->g, p->track);
+  hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->nrhs, p->g, p->track);
   if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, p->track, p->lambda,
                    p->wf_count > 0 ? p->Zv : nullptr)))
     return rc;
@@ -524,7 +576,9 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   }
   mark(5);
   // (e) activities, F15.4 round trip, eps(omega)
-  if ((rc = launch_activities(s, N, p->track, P.dtfull, P.cpol, p->act))) return rc;
+  rc = P.bset ? launch_activities_temp(s, N, p->track, P.dtfull, P.cpol, p->act)
+              : launch_activities(s, N, p->track, P.dtfull, P.cpol, p->act);
+  if (rc) return rc;
   const double* e_in = p->lambda;
   const double* a_in = p->act;
   if (P.f154) {
@@ -659,6 +713,7 @@ size_t wtb_bse_plan_buffer_bytes(const wtb_bse_plan* p, int what) {
     case WTB_BUF_WTAB: return sizeof(double) * p->wtab_n;
     case WTB_BUF_TRIDIAG: return sizeof(double) * 2 * p->N;
     case WTB_BUF_EXCWF: return sizeof(zc) * (size_t)p->N * p->wf_count;
+    case WTB_BUF_FDEH: return p->fdeh ? sizeof(double) * p->N : 0;
     default: return 0;
   }
 }
@@ -680,6 +735,7 @@ int wtb_bse_plan_fetch(wtb_bse_plan* p, int what, void* dst, size_t bytes) {
     case WTB_BUF_HBSE: src = p->mixed ? (const void*)p->Hf : (const void*)p->H; break;
     case WTB_BUF_WTAB: src = p->wtab; break;
     case WTB_BUF_EXCWF: src = p->Xv; break;
+    case WTB_BUF_FDEH: src = p->fdeh; break;
     case WTB_BUF_TRIDIAG: {
       WTB_HIP(hipMemcpyAsync(dst, p->tw.d, sizeof(double) * p->N, hipMemcpyDeviceToHost, s));
       WTB_HIP(hipMemcpyAsync((char*)dst + sizeof(double) * p->N, p->tw.e, sizeof(double) * p->N,
@@ -1049,6 +1105,13 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     set_error("wtb_bse_plan_lowest: needs a plan in the precision form (WTB_MIXED)");
     return WTB_ERR_ARG;
   }
+  if (p->prm.bset && eps && dp->lanczos_steps > 0) {
+    // the chains give quadratic forms h^H g(H) h; the finite-temperature activities pair
+    // h with f h, which they do not cover
+    set_error("wtb_bse_plan_lowest: eps(omega) from the Lanczos chains is T = 0 only (pass eps = NULL "
+              "or lanczos_steps = 0 with bset)");
+    return WTB_ERR_ARG;
+  }
   WTB_HIP(hipSetDevice(p->ctx->device));
   hipStream_t s = p->ctx->stream;
   const int N = p->N, nev = dp->nev;
@@ -1350,9 +1413,10 @@ int wtb_bse_plan_set_vectors(wtb_bse_plan* p, int first, int count) {
     // the divide and conquer needs its eigenvector workspace, the reduction keeps its reflectors
     hipFree(p->dcwork);
     p->dcwork = nullptr;
-    if ((rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 6, 1))) || (rc = dmalloc(&p->Zv, (size_t)N * N)))
+    if ((rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 2 * p->nrhs, 1))) ||
+        (rc = dmalloc(&p->Zv, (size_t)N * N)))
       return rc;
-    if ((rc = dc_prepare(p->dcwork, N, 6, p->tree, 1))) return rc;
+    if ((rc = dc_prepare(p->dcwork, N, 2 * p->nrhs, p->tree, 1))) return rc;
     p->tw.store_v = 1;
   }
   if (count != p->wf_count) {

@@ -38,9 +38,16 @@ int launch_dipoles(hipStream_t s, const zc* d_hk4, int nk, int nw, int nc, int n
 
 // stage (c)
 int launch_wtable(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, double* d_wtab);
+// d_fdeh (optional, finite temperature): kernel terms of column J of the lower triangle
+// (row J of the reference's upper one) times fdeh[J]
 int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng[3],
                      const double* d_wtab, const double* d_eigv, const zc* d_vec, zc* d_H, int ldh,
-                     zf* d_Hf);
+                     zf* d_Hf, const double* d_fdeh = nullptr);
+// fdeh[I] = f(E_v) - f(E_c) of basis state I, f(E) = 1/(exp(E beta) + 1)
+int launch_fermi_factors(hipStream_t s, int nk, int nc, int nv, const double* d_eigv, double beta,
+                         double* d_fdeh);
+// y[q][i] = x[q][i] f[i], q < nvec (complex vectors of length N)
+int launch_scale_rows(hipStream_t s, int nvec, int N, const zc* x, const double* f, zc* y);
 
 // stage (d): Householder reduction of the lower triangle of H (ldh) to a real
 // tridiagonal (d, e), applying Q^T to nrhs complex vectors g (g[N*nrhs]).
@@ -120,6 +127,9 @@ int hmm_f32_acc64(hipStream_t s, int N, int nc, const zf* H, int ldh, const zc* 
 // stage (e)
 int launch_activities(hipStream_t s, int N, const double* d_G /* 6 x N */, int dtfull, int cpol,
                       double* d_act /* 8 x N */);
+// finite temperature: d_G rows 0-5 from h, rows 6-11 from f h
+int launch_activities_temp(hipStream_t s, int N, const double* d_G /* 12 x N */, int dtfull, int cpol,
+                           double* d_act /* 8 x N */);
 int launch_quantize(hipStream_t s, int n, const double* src, double* dst);
 int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act6, double vc,
                     double nkf32, double ebse0, double ebsef, int numbse, double sme, int renorm,

@@ -1,5 +1,6 @@
 ! Drop-in replacements for the reference's bsesolver
-! (subprograms/bse_solver-tool-diel.f90:2-660) and bsedielraw
+! (subprograms/bse_solver-tool-diel.f90:2-660), bsesolvertemp
+! (subprograms/bse_solver-tool-diel-temp.f90:2-660) and bsedielraw
 ! (subprograms/bse_diel-tool.f90:5-245): same names, same argument lists, same
 ! output files, with the computation done by libwtb_bse.so on the GPU.
 !
@@ -12,9 +13,6 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 		     ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
 		     exc,mshift,coultype,ez,w1,r0,lc,rk,meshtype,bsewf,excwf0,excwff,&
 		     dtfull,cpol,tmcoef)
-	use iso_c_binding
-	use hamiltonian_input_variables
-	use wtb_gpu
 	implicit none
 	integer :: nthreads
 	integer,dimension(3) :: ngrid
@@ -26,6 +24,61 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	logical :: bsewf, cpol, dtfull, tmcoef
 	integer :: excwf0, excwff
 	double precision :: ez, w1, lc, r0
+	double precision :: zero
+
+	zero = 0d0
+	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol, &
+			    .false.,zero,zero,"FA",zero)
+end subroutine bsesolver
+
+! bsesolvertemp (subprograms/bse_solver-tool-diel-temp.f90:2-660): the same with BSET= T.
+! Like the reference (fermidisteh, general_subs.f90:620-624), TEMP = 0 is set to 1E-8.
+subroutine bsesolvertemp(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+		     ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+		     exc,mshift,coultype,ez,w1,r0,lc,rk,meshtype,bsewf,excwf0,excwff,&
+		     dtfull,cpol,tmcoef,st,phavg,ta,temp)
+	implicit none
+	integer :: nthreads
+	integer,dimension(3) :: ngrid
+	integer :: nc,nv
+	double precision :: numdos, ebse0, ebsef, numbse, sme, exc, rk, ktol
+	double precision,dimension(3) :: mshift, ediel
+	character(len=70) :: params, orbw, kpaths, kpathsbse, outputfolder, calcparms, meshtype
+	character(len=5) :: coultype
+	logical :: bsewf, cpol, dtfull, tmcoef
+	integer :: excwf0, excwff
+	double precision :: ez, w1, lc, r0
+	double precision :: st, phavg, temp
+	character(len=2) :: ta
+
+	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol, &
+			    .true.,st,phavg,ta,temp)
+	if (temp .eq. 0d0) temp = 1E-8
+end subroutine bsesolvertemp
+
+! The body of both: model and parameters into the C structures, one wtb_bsesolver(_wf)
+! call, the reference's output files.
+subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			  ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol, &
+			  bset,st,phavg,ta,temp)
+	use iso_c_binding
+	use hamiltonian_input_variables
+	use wtb_gpu
+	implicit none
+	integer :: nthreads
+	integer,dimension(3) :: ngrid
+	integer :: nc,nv
+	double precision :: ebse0, ebsef, numbse, sme, ktol
+	double precision,dimension(3) :: mshift, ediel
+	character(len=70) :: params, outputfolder
+	character(len=5) :: coultype
+	logical :: bsewf, cpol, dtfull, bset
+	integer :: excwf0, excwff
+	double precision :: ez, w1, lc, r0
+	double precision :: st, phavg, temp
+	character(len=2) :: ta
 
 	type(c_ptr) :: ctx
 	type(wtb_tb_model) :: m
@@ -80,6 +133,13 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	p%renorm = 1
 	p%f154 = 1
 	p%precision = WTB_FP64
+	p%bset = merge(1, 0, bset)
+	p%ta = WTB_TA_FA
+	if (ta .eq. "VE") p%ta = WTB_TA_VE
+	if (ta .eq. "BE") p%ta = WTB_TA_BE
+	p%temp = temp
+	p%st = st
+	p%phavg = phavg
 
 	dimbse = ngrid(1)*ngrid(2)*ngrid(3)*nc*nv
 	allocate(w(dimbse), act(dimbse,8))
@@ -108,6 +168,7 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	if (erro/=0) stop "Error opening output file"
 
 	write(300,*) 'threads:', nthreads
+	if (bset) write(300,*) 'temperature', temp
 	write(300,*) 'grid:',ngrid(1),ngrid(2),ngrid(3)
 	write(300,"(A14,1E15.4)") 'ktol-coulomb:', ktol
 	write(300,"(A13,3F15.4)") 'kmesh shift:',mshift(1),mshift(2),mshift(3)
@@ -134,7 +195,7 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 		deallocate(wf, kpt, nocpk, stt)
 	end if
 	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
-end subroutine bsesolver
+end subroutine wtb_bse_dropin
 
 
 subroutine bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme)

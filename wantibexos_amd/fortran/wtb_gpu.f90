@@ -9,6 +9,7 @@ module wtb_gpu
 
 	integer(c_int), parameter :: WTB_OK = 0
 	integer(c_int), parameter :: WTB_FP64 = 0, WTB_MIXED = 1
+	integer(c_int), parameter :: WTB_TA_FA = 0, WTB_TA_VE = 1, WTB_TA_BE = 2
 
 	type, bind(C) :: wtb_tb_model
 		integer(c_int) :: nw, nvec
@@ -23,6 +24,8 @@ module wtb_gpu
 		real(c_double) :: ediel(3)
 		real(c_double) :: lc, ez, w, r0, ktol, cshift, ebse0, ebsef
 		integer(c_int) :: numbse, dtfull, cpol, renorm, f154, precision
+		integer(c_int) :: bset, ta
+		real(c_double) :: temp, st, phavg
 	end type
 
 	interface

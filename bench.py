@@ -20,6 +20,8 @@ of that matrix; skip it with --no-iterative).
                                                   # models, batch API vs single plans
 
     python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu]
+    python bench.py --temp 3000 --ta VE --st 0.3 --phavg 0.05   # finite temperature
+                                                  # (bsesolvertemp) on the same workload
     python bench.py --copy-probe      # one 1 GiB device copy, to scale HBM counters
     python bench.py --no-graph ...    # direct launches (for kernel tracers)
 
@@ -71,7 +73,9 @@ def params(cfg, precision):
                          w=cfg["w"], r0=cfg["r0"], ktol=cfg["ktol"], cshift=cfg["cshift"],
                          ebse0=cfg["ebse0"], ebsef=cfg["ebsef"], numbse=cfg["numbse"],
                          dtfull=cfg["dtfull"], cpol=True, renorm=cfg["renorm"], f154=True,
-                         precision=bse.MIXED if precision == "mixed" else bse.FP64)
+                         precision=bse.MIXED if precision == "mixed" else bse.FP64,
+                         bset=cfg.get("bset", False), temp=cfg.get("temp", 0.0), ta=cfg.get("ta", "FA"),
+                         st=cfg.get("st", 0.0), phavg=cfg.get("phavg", 0.0))
 
 
 def cpu_baseline(cfg, model, max_threads):
@@ -268,6 +272,11 @@ def main():
                     help="iterative form: residual norm (eV) below which an exciton counts as converged; "
                          "a Ritz value lies within its residual norm of an eigenvalue")
     ap.add_argument("--learned", action="store_true", help="iterative form with the trained preconditioner")
+    ap.add_argument("--temp", type=float, default=None,
+                    help="finite temperature (K): the bsesolvertemp path (BSET= T) on the same workload")
+    ap.add_argument("--ta", choices=["FA", "VE", "BE"], default="VE", help="finite temperature: gap correction")
+    ap.add_argument("--st", type=float, default=0.3, help="finite temperature: ST")
+    ap.add_argument("--phavg", type=float, default=0.05, help="finite temperature: PHAVG")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-iterative", action="store_true",
                     help="full form: skip the iterative form's measurement on the 60x60 model that the "
@@ -296,6 +305,9 @@ def main():
     if args.form == "batch":
         return bench_batch(args)
     cfg = CONFIGS[args.config]
+    if args.temp is not None:
+        cfg = dict(cfg, name=f"{cfg['name']}_T{args.temp:g}K_{args.ta}", bset=True, temp=args.temp, ta=args.ta,
+                   st=args.st, phavg=args.phavg)
     t0 = time.perf_counter()
     model = make_model(cfg)
     log(f"model {cfg['name']} generated in {time.perf_counter() - t0:.1f}s")
@@ -335,12 +347,14 @@ def main():
     # size-independent checks of the result
     eigv = plan.fetch("eigv")
     dip = plan.fetch("dipole")
+    fw = plan.fetch("fdeh") if prm.bset else np.ones(N)      # f_v - f_c (1 at T = 0)
     nv = prm.nv
     dE = (eigv[:, nv:][:, :, None] - eigv[:, :nv][:, None, :]).reshape(-1)
+    sx, sy = (np.sum(np.abs(dip[i]) ** 2 * fw) for i in (0, 1))
     checks = dict(
         trace_rel=abs(res["W"].sum() - dE.sum()) / abs(dE.sum()),           # sum W = trace H (V(0)=0)
-        sumrule_xx_rel=abs(res["act"][0].sum() - np.vdot(dip[0], dip[0]).real) / np.vdot(dip[0], dip[0]).real,
-        sumrule_yy_rel=abs(res["act"][1].sum() - np.vdot(dip[1], dip[1]).real) / np.vdot(dip[1], dip[1]).real,
+        sumrule_xx_rel=abs(res["act"][0].sum() - sx) / sx,                   # sum act = sum f |h|^2
+        sumrule_yy_rel=abs(res["act"][1].sum() - sy) / sy,
         sorted=bool(np.all(np.diff(res["W"]) >= 0)),
         ground_state_eV=float(res["W"][0]),
     )
@@ -385,7 +399,7 @@ def main():
     # (profiles/, same workload and precision) applied to the mean launch
     traffic = None
     tfile = ROOT / "profiles" / f"k_tri_hemv_traffic_hex2d_nw22_k36_N5184_{args.precision}.json"
-    if tfile.exists() and args.config == "c2":
+    if tfile.exists() and args.config == "c2" and not prm.bset:
         tj = json.loads(tfile.read_text())
         traffic = round(tj["traffic_over_algorithmic"] * per_launch)
     roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",
@@ -401,7 +415,7 @@ def main():
     plan.close()
     ctx.close()
     iterative = None
-    if not args.no_iterative and args.config == "c2" and args.precision == "mixed":
+    if not args.no_iterative and args.config == "c2" and args.precision == "mixed" and not prm.bset:
         # the iterative form at the target accuracy (excitons within 1e-5 eV by their residuals,
         # 150-step chains) on the 60x60 model, against the full FP64 solve of the same matrix:
         # its cost grows like N^2 and the full solve's like N^3, so it pays on larger matrices
@@ -431,7 +445,9 @@ def main():
         "data": "synthetic (seeded random tight-binding model)",
         "config": {"workload": cfg["name"], "bse_dim": N, "nwann": cfg["nw"],
                    "kgrid": list(cfg["ngrid"]), "potential": cfg["coultype"],
-                   "precision": args.precision, "n_omega": cfg["numbse"]},
+                   "precision": args.precision, "n_omega": cfg["numbse"],
+                   **({"temperature_K": cfg["temp"], "ta": cfg["ta"], "st": cfg["st"], "phavg": cfg["phavg"]}
+                      if prm.bset else {})},
         "fp64_plain": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -307,6 +307,27 @@ int wtb_bse_batch_execute(wtb_bse_batch* batch);
 int wtb_bse_batch_results(wtb_bse_batch* batch, int i, double* W, double* act, double* eps, double* egap);
 int wtb_bse_batch_destroy(wtb_bse_batch* batch);
 
+/* ---- finite-Q exciton bands (BSE_BND= T) ----------------------------------------
+ * kpathbse (subroutines/mhkpack_subs.f90:132-208): ks(nks,3) fractional segment ends
+ * (pairs), npts points per segment -> qauxv((nks/2)*npts, 4): path coordinate, qx, qy,
+ * qz; labels (nks/2 + 1, may be NULL): the coordinates of the segment ends
+ * (KLABELS-BSE.dat).  Host only.                                                  */
+int wtb_kpathbse(const double* rlat, int nks, const double* ks, int npts, double* qauxv, double* labels);
+/* bsebnds (subprograms/bse_kpath-tool.f90:3-479) for nq q-points: exk(N, nq) the
+ * ascending exciton energies of the finite-Q BSE matrix (matrizelbsekq,
+ * subroutines/bse_subs_kpath.f90:1-214: conduction bands at k+q, valence bands at
+ * k, direct term V(k-k')<c|c'><v|v'>, exchange -V(q)<c|v><v'|c'> for q != 0) at each
+ * q = qauxv(iq, 2:4).  The q-points run on `lanes` HIP streams (1..8), one plan
+ * each.  prm->bset must be 0 (bsebndstemp is not built).                          */
+int wtb_bsebnds(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, int nq, const double* qauxv,
+                int lanes, double* exk);
+/* Plan form of one q-point: create, then wtb_bse_plan_set_q(plan, q[4]) (a qauxv
+ * row) before build/execute; results gives w only (act, eps, egap NULL);
+ * WTB_BUF_HBSE holds the finite-Q matrix after build.                            */
+int wtb_bse_qplan_create(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm,
+                         wtb_bse_plan** plan);
+int wtb_bse_plan_set_q(wtb_bse_plan* plan, const double* q);
+
 /* BSE_WF (bse_solver-tool-diel.f90:513-519, excwf2 in bse_subs.f90:534-576):
  * executes also form the eigenvectors of excitons first .. first+count-1
  * (0-based, ascending energy), fetched as WTB_BUF_EXCWF, each with its last

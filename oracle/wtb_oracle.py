@@ -27,6 +27,9 @@ Pinning (how each function is checked; see tests/test_oracle_*.py):
   Coulomb potentials            vs reference dump (matrix rows) + closed forms
   BSE eigenvalues/activities    vs reference dump; sum rule; Hermiticity
   eps(omega), imagrenorm        vs reference dump (F15.4 path and full precision)
+  finite T (tcor, f_v - f_c)    vs reference dump; closed forms; activity sum rule
+  finite-Q bands (kpathbse,     vs the reference program's bands_bse.dat / KLABELS-BSE.dat;
+    k+q grid, exchange)         q = 0, q = G and rank-one interlacing invariants
 The per-k and BSE eigensolves use LAPACK through numpy (zheevd) in place of
 MKL ZHEEV; eigenvalues are basis independent and the per-k eigenvectors are
 put into ZHEEV('L')'s gauge (component on orbital 1 real), which is all the
@@ -585,3 +588,124 @@ def run_pipeline(model, ngrid, **kw):
     spinf = 2.0 if model.systype.strip() == "NP" else 1.0
     return dict(kpt=kpt, E=E, U=U, nocp=nocp, eigv=eigv, vector=vector, stt=stt, h=h, H=H, tcor=tcor, fdeh=fdeh,
                 W=W, X=X, act=act, Wq=Wq, actq=actq, omega=om, eps_re=re, eps_im=im, spinf=spinf)
+
+
+# ---------------------------------------------------------------------------
+# finite-Q exciton bands (bsebnds, subprograms/bse_kpath-tool.f90)
+# ---------------------------------------------------------------------------
+
+def kpathbse(rlat, ks, npts):
+    """kpathbse (mhkpack_subs.f90:132-208): the pairs (ks[2j], ks[2j+1]) (fractional) are
+    segments; npts points per segment, ends included.  Returns qauxv ((nks/2) npts, 4):
+    path coordinate (cumulative segment length over the total), then q (Cartesian), and
+    the segment-end coordinates (KLABELS-BSE.dat)."""
+    b = recvec(rlat)
+    ks = np.asarray(ks, dtype=np.float64)
+    nseg = len(ks) // 2
+    ksaux = np.array([[ks[i, 0] * b[0][c] + ks[i, 1] * b[1][c] + ks[i, 2] * b[2][c] for c in range(3)]
+                      for i in range(len(ks))])
+    kdis = np.empty(nseg)
+    for j in range(nseg):
+        d = ksaux[2 * j + 1] - ksaux[2 * j]                  # distvec(ks(2j), ks(2j-1)), :292-302
+        kdis[j] = math.sqrt((d[0] ** 2 + d[1] ** 2) + d[2] ** 2)
+    tot = 0.0
+    for j in range(nseg):
+        tot = tot + kdis[j]
+    kdis = kdis / tot
+    out = np.empty((nseg * npts, 4))
+    labels = [0.0]
+    acc = 0.0
+    for j in range(nseg):
+        a, e = ksaux[2 * j], ksaux[2 * j + 1]
+        for i in range(npts):
+            t = float(i)                                      # (i-1.) with 1-based i: exact
+            out[j * npts + i, 0] = acc + (kdis[j] * t) / float(npts - 1)
+            for c in range(3):
+                out[j * npts + i, 1 + c] = a[c] + ((e[c] - a[c]) * t) / float(npts - 1)
+        acc = acc + kdis[j]
+        labels.append(acc)
+    return out, np.array(labels)
+
+
+def monkhorst_pack_q(q, ngrid, shift, rlat):
+    """monhkhorst_packq (mhkpack_subs.f90:305-356): the grid k + q, with q added first
+    ((((q + b1 i/n1) + b2 j/n2) + b3 l/n3) + kshift), q = qauxv row (coordinate, qx, qy, qz)."""
+    n1, n2, n3 = (int(v) for v in ngrid)
+    b = recvec(rlat)
+    sh = [float(v) for v in shift]
+    kshift = [b[0][c] * sh[0] + b[1][c] * sh[1] + b[2][c] * sh[2] for c in range(3)]
+    s1 = [b[0][c] / float(n1) for c in range(3)]
+    s2 = [b[1][c] / float(n2) for c in range(3)]
+    s3 = [b[2][c] / float(n3) for c in range(3)]
+    out = np.empty((n1 * n2 * n3, 3))
+    cnt = 0
+    for i in range(n1):
+        for j in range(n2):
+            for l in range(n3):
+                for c in range(3):
+                    out[cnt, c] = q[1 + c] + s1[c] * float(i) + s2[c] * float(j) + s3[c] * float(l) + kshift[c]
+                cnt += 1
+    return out
+
+
+def bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, params, ngrid, rlat):
+    """matrizelbsekq (bse_subs_kpath.f90:1-214) in the loop of bse_kpath-tool.f90:325-345,
+    j >= i: conduction bands at k+q, valence bands at k, V(k-k') on the k grid; for q != 0
+    (modvecq, general_subs.f90:310-321) the exchange term
+    - V(q) <c_i|v_i> <v_j|c_j> (c at k+q, v at k).  Returns the Hermitian completion of
+    the upper triangle that ZHEEV('U') reads."""
+    ki = stt[:, 3] - 1
+    vi = stt[:, 1] - 1
+    ci = stt[:, 2] - 1
+    vc = vec_q[ki, ci]                                   # conduction at k+q (N, nw)
+    vv = vec_k[ki, vi]                                   # valence at k
+    oc = np.conj(vc) @ vc.T
+    ov = np.conj(vv) @ vv.T
+    kI = kpt[ki]
+    dk = kI[:, None, :] - kI[None, :, :]
+    p = params
+    V = coulomb(p["coultype"], dk, ngrid, rlat, p["ediel"], p["lc"], p["ez"], p["w"], p["r0"], p["ktol"])
+    H = V * oc * ov
+    diag = (eigv_q[ki, ci] - eigv_k[ki, vi]) + np.diag(V)
+    modq = math.sqrt((q[1] ** 2 + q[2] ** 2) + q[3] ** 2)
+    if modq != 0.0:
+        vq = float(coulomb(p["coultype"], np.asarray(q[1:4], dtype=np.float64).reshape(1, 3), ngrid, rlat,
+                           p["ediel"], p["lc"], p["ez"], p["w"], p["r0"], p["ktol"])[0])
+        a = np.einsum("nm,nm->n", np.conj(vc), vv)      # <c_i|v_i>
+        X = (vq * a)[:, None] * np.conj(a)[None, :]
+        H = H - X
+        diag = diag - np.diag(X)
+    H = np.triu(H, 1)
+    H = H + np.conj(H.T)
+    H[np.arange(len(H)), np.arange(len(H))] = diag
+    return H
+
+
+def run_bands(model, ngrid, qpath, **kw):
+    """bsebnds (bse_kpath-tool.f90:187-445): exk (N, nq), the ascending exciton energies
+    of the matrix at every q of the path (ZHEEV('N','U'))."""
+    p = dict(DEFAULT_PARAMS)
+    p.update(kw)
+    kpt = monkhorst_pack(ngrid, p["shift"], model.rlat)
+    E, U, nocp = eigsys(kpt, model)
+    eigv_k, vec_k = select_bands(E, U, nocp, p["nc"], p["nv"])
+    nq = len(qpath)
+    exk = None
+    mats = []
+    for iq in range(nq):
+        q = qpath[iq]
+        kq = monkhorst_pack_q(q, ngrid, p["shift"], model.rlat)
+        Eq, Uq, nocpq = eigsys(kq, model)
+        eigv_q, vec_q = select_bands(Eq, Uq, nocpq, p["nc"], p["nv"])
+        if (p["nv"] > nocp).any():                          # quantumnumbers2(..., nocpk, nocpq)
+            raise ValueError("Number of valence states higher then ocuppied states")
+        if (p["nc"] > model.nw - nocpq).any():
+            raise ValueError("Number of conduction states higher then unocuppied states")
+        stt = quantumnumbers2(len(kpt), p["nc"], p["nv"], np.full(len(kpt), p["nv"]), model.nw)
+        H = bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, p, ngrid, model.rlat)
+        W = np.linalg.eigvalsh(H, UPLO="U")
+        if exk is None:
+            exk = np.empty((len(W), nq))
+        exk[:, iq] = W
+        mats.append(H)
+    return dict(kpt=kpt, exk=exk, H=mats)

@@ -1,8 +1,9 @@
 """Drop-in check at the Fortran boundary: the reference program (oracle/_ref/wtb.x,
 compiled from the reference sources) and the GPU drop-in driver
 (wantibexos_amd/fortran/_build/wtb_gpu.x: the reference's input parser + the
-drop-in bsesolver/bsesolvertemp/bsedielraw over ISO_C_BINDING) run on the same
-input.dat and TB file; their bse_opt_diel.dat and bse_diel_*.dat must agree."""
+drop-in bsesolver/bsesolvertemp/bsebnds/bsedielraw over ISO_C_BINDING) run on the
+same input.dat and TB file; their bse_opt_diel.dat and bse_diel_*.dat (bands_bse.dat
+and KLABELS-BSE.dat for BSE_BND) must agree."""
 import os
 import subprocess
 from pathlib import Path
@@ -91,3 +92,26 @@ def test_dropin_exciton_wavefunctions(tmp_path):
             ma = np.hypot(a[:, 6], a[:, 7])
             mb = np.hypot(b[:, 6], b[:, 7])
             assert np.abs(ma - mb).max() <= 2e-4 * max(ma.max(), 1e-30) + 1e-8, n
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+def test_dropin_exciton_bands(tmp_path):
+    """BSE_BND=T: bands_bse.dat of the drop-in bsebnds against the reference's (the
+    coordinate column is the same REAL; energies 1e-9 eV) and KLABELS-BSE.dat."""
+    p = dict(P, ngrid=(4, 4, 1), kpath_bse="kpath.dat")
+    model = tbmodel.hex2d(41, 6, p["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    (tmp_path / "kpath.dat").write_text("6\n4\n0.0 0.0 0.0\n0.6667 -0.3333 0.0\n0.6667 -0.3333 0.0\n"
+                                        "0.3333 0.3333 0.0\n0.3333 0.3333 0.0\n0.0 0.0 0.0\n")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    assert (tmp_path / "ref" / "bands_bse.dat").exists(), rr.stdout + rr.stderr
+    a = table(tmp_path / "ref" / "bands_bse.dat", skip=0)
+    b = table(tmp_path / "gpu" / "bands_bse.dat", skip=0)
+    assert a.shape == b.shape == (64 * 12, 2)
+    assert np.array_equal(a[:, 0], b[:, 0])
+    assert np.abs(a[:, 1] - b[:, 1]).max() < 1e-9
+    ka = table(tmp_path / "ref" / "KLABELS-BSE.dat", skip=0)
+    kb = table(tmp_path / "gpu" / "KLABELS-BSE.dat", skip=0)
+    assert np.array_equal(ka, kb)

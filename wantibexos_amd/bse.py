@@ -133,6 +133,10 @@ def _load():
         "wtb_bse_batch_execute": (ip, [vp]),
         "wtb_bse_batch_results": (ip, [vp, ip, vp, vp, vp, vp]),
         "wtb_bse_batch_destroy": (ip, [vp]),
+        "wtb_kpathbse": (ip, [vp, ip, vp, ip, vp, vp]),
+        "wtb_bsebnds": (ip, [vp, vp, vp, ip, vp, ip, vp]),
+        "wtb_bse_qplan_create": (ip, [vp, vp, vp, C.POINTER(vp)]),
+        "wtb_bse_plan_set_q": (ip, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -320,17 +324,32 @@ def bse_spectrum(ctx: Context, e, act6, rlat, ngrid, ebse0, ebsef, numbse, sme, 
 
 
 class BSEPlan:
-    """Device-resident pipeline: create once, execute many times."""
+    """Device-resident pipeline: create once, execute many times.  finite_q=True: the
+    finite-Q plan of the exciton bands (set_q, then build/execute; energies() only)."""
 
-    def __init__(self, ctx: Context, model: TBModel, params: BSEParams):
+    def __init__(self, ctx: Context, model: TBModel, params: BSEParams, finite_q: bool = False):
         self.ctx = ctx
         self.params = params
+        self.finite_q = finite_q
         self._cm = _CModel(model)
         self._cp = params.to_c()
         h = C.c_void_p()
-        _check(lib().wtb_bse_plan_create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), C.byref(h)))
+        create = lib().wtb_bse_qplan_create if finite_q else lib().wtb_bse_plan_create
+        _check(create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), C.byref(h)))
         self.h = h
         self.N = lib().wtb_bse_plan_dim(h)
+
+    def set_q(self, q):
+        """A row of the q path (coordinate, qx, qy, qz) for the next build/execute."""
+        qa = np.ascontiguousarray(q, dtype=np.float64)
+        assert qa.shape == (4,)
+        _check(lib().wtb_bse_plan_set_q(self.h, _p(qa)))
+
+    def energies(self):
+        """The exciton energies of the last execute (the only result of a finite-Q plan)."""
+        W = np.zeros(self.N)
+        _check(lib().wtb_bse_plan_results(self.h, _p(W), None, None, None))
+        return W
 
     def execute(self):
         _check(lib().wtb_bse_plan_execute(self.h))
@@ -553,6 +572,30 @@ class MLP:
 
     def __exit__(self, *a):
         self.close()
+
+
+def kpathbse(rlat, ks, npts):
+    """kpathbse (mhkpack_subs.f90:132-208): (qauxv (nq, 4), labels (nks/2 + 1,))."""
+    rl = np.asfortranarray(np.asarray(rlat, dtype=np.float64))
+    ksf = np.asfortranarray(np.asarray(ks, dtype=np.float64))
+    nks = ksf.shape[0]
+    nq = (nks // 2) * int(npts)
+    q = np.zeros((nq, 4), order="F")
+    lab = np.zeros(nks // 2 + 1)
+    _check(lib().wtb_kpathbse(rl.ctypes.data, nks, ksf.ctypes.data, int(npts), q.ctypes.data, _p(lab)))
+    return np.ascontiguousarray(q), lab
+
+
+def bsebnds(ctx: Context, model: TBModel, params: BSEParams, qpath, lanes: int = 4):
+    """Exciton bands (bsebnds, bse_kpath-tool.f90): exk (N, nq) at the q-path rows
+    qpath (nq, 4) = (coordinate, qx, qy, qz)."""
+    cm = _CModel(model)
+    cp = params.to_c()
+    qf = np.asfortranarray(np.asarray(qpath, dtype=np.float64))
+    nq = qf.shape[0]
+    exk = np.zeros((params.dim, nq), order="F")
+    _check(lib().wtb_bsebnds(ctx.h, C.byref(cm.s), C.byref(cp), nq, qf.ctypes.data, int(lanes), exk.ctypes.data))
+    return np.ascontiguousarray(exk)
 
 
 def bsesolver(ctx: Context, model: TBModel, params: BSEParams):

@@ -255,7 +255,8 @@ template <bool F32>
 __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* __restrict__ wtab,
                                                    const double* __restrict__ eigv,
                                                    const zc* __restrict__ vec, zc* __restrict__ H,
-                                                   zf* __restrict__ Hf, const double* __restrict__ fdeh) {
+                                                   zf* __restrict__ Hf, const double* __restrict__ fdeh,
+                                                   const zc* __restrict__ xa, const double* __restrict__ dvq) {
   extern __shared__ unsigned char smem[];
   // lower-triangle tile index -> (tr, tc), tr >= tc
   int t = blockIdx.x;
@@ -283,6 +284,7 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
   const int side = TK * ba.ncv;
   const int I0 = kr0 * ba.ncv, J0 = kc0 * ba.ncv;
   const int c0 = ba.n2 * ba.n3;
+  const double vq = xa ? *dvq : 0.0;
   for (int e = threadIdx.x; e < side * side; e += 256) {
     int i = e % side, j = e / side;
     int I = I0 + i, J = J0 + j;
@@ -299,6 +301,11 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
       double ec = eigv[(size_t)k * ba.nb + ba.nv + ci], ev = eigv[(size_t)k * ba.nb + vi];
       // finite temperature: (Ec - Ev) + V(0) f_I (bse_subs_temp.f90:105-108)
       val = mkz((ec - ev) + (fdeh ? wv * fdeh[J] : wv), 0.0);
+      // finite Q: - V(q) <c|v> <v|c> (bse_subs_kpath.f90:158-160)
+      if (xa) {
+        const zc a = xa[I];
+        val = val - (vq * a) * conj(a);
+      }
     } else {
       zc oc = Oc[(kl * ba.nc + ci) * rc_ + (kl2 * ba.nc + cj)];
       zc ov = Ov[(kl * ba.nv + vi) * rv_ + (kl2 * ba.nv + vj)];
@@ -306,6 +313,9 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
       // finite temperature: the reference's element (J, I), J < I, is V vc vv f_J
       // (bse_subs_temp.f90:112-120); this one is its conjugate
       if (fdeh) val = val * fdeh[J];
+      // finite Q: the reference's (J, I) element subtracts V(q) a_J conj(a_I)
+      // (bse_subs_kpath.f90:178-191); conjugated: V(q) conj(a_J) a_I
+      if (xa) val = val - (vq * conj(xa[J])) * xa[I];
     }
     size_t off = (size_t)I + (size_t)ba.ldh * J;
     if constexpr (F32) stz(Hf[off], val);
@@ -322,7 +332,7 @@ int launch_wtable(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, do
 
 int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng[3],
                      const double* d_wtab, const double* d_eigv, const zc* d_vec, zc* d_H, int ldh,
-                     zf* d_Hf, const double* d_fdeh) {
+                     zf* d_Hf, const double* d_fdeh, const zc* d_xa, const double* d_vq) {
   BuildArgs ba;
   ba.nk = nk;
   ba.nc = nc;
@@ -347,10 +357,10 @@ int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng
   size_t lds = sizeof(zc) * ((size_t)tk * nc * tk * nc + (size_t)tk * nv * tk * nv);
   if (d_H)
     hipLaunchKernelGGL(k_bse_build<false>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
-                       d_H, (zf*)nullptr, d_fdeh);
+                       d_H, (zf*)nullptr, d_fdeh, d_xa, d_vq);
   if (d_Hf)
     hipLaunchKernelGGL(k_bse_build<true>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
-                       (zc*)nullptr, d_Hf, d_fdeh);
+                       (zc*)nullptr, d_Hf, d_fdeh, d_xa, d_vq);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }
@@ -387,6 +397,76 @@ __global__ void k_scale_rows(int nvec, int N, const zc* __restrict__ x, const do
 
 int launch_scale_rows(hipStream_t s, int nvec, int N, const zc* x, const double* f, zc* y) {
   hipLaunchKernelGGL(k_scale_rows, dim3((N + 255) / 256), dim3(256), 0, s, nvec, N, x, f, y);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+}  // namespace wtb
+
+namespace wtb {
+
+// ---------------------------------------------------------------------------
+// Finite-Q exciton bands (bsebnds, subprograms/bse_kpath-tool.f90:187-389): the band
+// window of the conduction states comes from the k+q grid, the valence window from k.
+// ---------------------------------------------------------------------------
+__global__ void k_mix_q(int nk, int nc, int nv, int nw, const double* __restrict__ eigvq,
+                        const zc* __restrict__ vecq, double* __restrict__ eigv, zc* __restrict__ vec) {
+  const int nb = nc + nv;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;   // (k, conduction band c, orbital m)
+  if (t >= nk * nc * nw) return;
+  const int m = t % nw, c = (t / nw) % nc, k = t / (nw * nc);
+  const size_t row = (size_t)k * nb + nv + c;
+  vec[row * nw + m] = vecq[row * nw + m];
+  if (m == 0) eigv[row] = eigvq[row];
+}
+
+int launch_mix_q(hipStream_t s, int nk, int nc, int nv, int nw, const double* d_eigvq, const zc* d_vecq,
+                 double* d_eigv, zc* d_vec) {
+  const int n = nk * nc * nw;
+  hipLaunchKernelGGL(k_mix_q, dim3((n + 255) / 256), dim3(256), 0, s, nk, nc, nv, nw, d_eigvq, d_vecq, d_eigv,
+                     d_vec);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+// a[I] = sum_m conj(c_m) v_m for basis state I = (k nc + c) nv + v: prodintsq(conj(vbc1), vbv1)
+// (bse_subs_kpath.f90:150-156), summed in orbital order
+__global__ void k_exchange_vec(int nk, int nc, int nv, int nw, const zc* __restrict__ vec, zc* __restrict__ a) {
+  const int N = nk * nc * nv;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= N) return;
+  const int ncv = nc * nv, nb = nc + nv;
+  const int k = I / ncv, c = (I % ncv) / nv, v = I % nv;
+  const zc* pc = vec + ((size_t)k * nb + nv + c) * nw;
+  const zc* pv = vec + ((size_t)k * nb + v) * nw;
+  zc acc = mkz(0.0, 0.0);
+  for (int m = 0; m < nw; ++m) acc = acc + conj(pc[m]) * pv[m];
+  a[I] = acc;
+}
+
+int launch_exchange_vec(hipStream_t s, int nk, int nc, int nv, int nw, const zc* d_vec, zc* d_a) {
+  const int N = nk * nc * nv;
+  hipLaunchKernelGGL(k_exchange_vec, dim3((N + 255) / 256), dim3(256), 0, s, nk, nc, nv, nw, d_vec, d_a);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+// V(q) = the potential at k1 = q, k2 = 0 (bse_subs_kpath.f90:70-119); 0 when |q| = 0, where
+// the reference takes the branch without the exchange term (:125-142)
+__global__ void k_vq(const double* __restrict__ rl, CoulombArgs ca, const double* __restrict__ dq,
+                     double* __restrict__ vq) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double dk[3] = {dq[1], dq[2], dq[3]};
+  const double modq = sqrt((dk[0] * dk[0] + dk[1] * dk[1]) + dk[2] * dk[2]);
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) a[i][c] = rl[i + 3 * c];
+  vq[0] = modq == 0.0 ? 0.0 : coulomb_dev(ca, a, dk);
+}
+
+int launch_vq(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, const double* d_q, double* d_vq) {
+  hipLaunchKernelGGL(k_vq, dim3(1), dim3(64), 0, s, d_rlat, ca, d_q, d_vq);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

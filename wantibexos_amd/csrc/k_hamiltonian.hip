@@ -38,8 +38,10 @@ __device__ inline void recvec_dev(const double* rl, double b[3][3]) {
 }
 
 // monhkhorst_pack: kpt(nk,3), index i*n2*n3 + j*n3 + l
+// dq (optional): a q-path row (coordinate, qx, qy, qz); the grid is then k + q with q added
+// first, as monhkhorst_packq does (mhkpack_subs.f90:305-356)
 __global__ void k_kgrid(const double* __restrict__ rl, int n1, int n2, int n3, double s1, double s2,
-                        double s3, double* __restrict__ kpt) {
+                        double s3, const double* __restrict__ dq, double* __restrict__ kpt) {
 #pragma clang fp contract(off)
   int nk = n1 * n2 * n3;
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -49,8 +51,13 @@ __global__ void k_kgrid(const double* __restrict__ rl, int n1, int n2, int n3, d
   int i = idx / (n2 * n3), j = (idx / n3) % n2, l = idx % n3;
   for (int c = 0; c < 3; ++c) {
     double ksh = b[0][c] * s1 + b[1][c] * s2 + b[2][c] * s3;
-    double v = (b[0][c] / (double)n1) * (double)i + (b[1][c] / (double)n2) * (double)j +
-               (b[2][c] / (double)n3) * (double)l + ksh;
+    double v;
+    if (dq)
+      v = dq[1 + c] + (b[0][c] / (double)n1) * (double)i + (b[1][c] / (double)n2) * (double)j +
+          (b[2][c] / (double)n3) * (double)l + ksh;
+    else
+      v = (b[0][c] / (double)n1) * (double)i + (b[1][c] / (double)n2) * (double)j +
+          (b[2][c] / (double)n3) * (double)l + ksh;
     kpt[idx + (size_t)nk * c] = v;
   }
 }
@@ -342,10 +349,10 @@ __global__ __launch_bounds__(64) void k_dipoles(const zc* __restrict__ Hk4, int 
 // host launchers
 // ---------------------------------------------------------------------------
 int launch_kgrid(hipStream_t s, const double* d_rlat, const int ng[3], const double sh[3],
-                 double* d_kpt) {
+                 double* d_kpt, const double* d_q) {
   int nk = ng[0] * ng[1] * ng[2];
   hipLaunchKernelGGL(k_kgrid, dim3((nk + 127) / 128), dim3(128), 0, s, d_rlat, ng[0], ng[1], ng[2],
-                     sh[0], sh[1], sh[2], d_kpt);
+                     sh[0], sh[1], sh[2], d_q, d_kpt);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

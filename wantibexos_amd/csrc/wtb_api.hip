@@ -97,6 +97,16 @@ struct wtb_bse_plan {
   zc* dip = nullptr;
   zc* g = nullptr;
   double* fdeh = nullptr;   // f_v - f_c per basis state (finite temperature)
+  // finite-Q exciton bands (q mode): the q-path row (device), the k+q grid and its band
+  // window, the exchange vector a_I = <c_I|v_I> and V(q)
+  int qmode = 0;
+  double* dq = nullptr;
+  double* kq = nullptr;
+  double* eigvq = nullptr;
+  zc* vecq = nullptr;
+  int* nocpq = nullptr;
+  zc* xa = nullptr;
+  double* dvq = nullptr;
   double* gapk = nullptr;
   double* wtab = nullptr;
   int wtab_n = 0;
@@ -345,7 +355,8 @@ static void plan_free(wtb_bse_plan* p) {
   free_model(p->dm);
   void* bufs[] = {p->kpt, p->phase, p->hk4, p->eall, p->nocp, p->eigv, p->vec, p->dip, p->g,
                   p->fdeh, p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
-                  p->wq, p->actq, p->eps, p->smeavg, p->status, p->Zv, p->Xv, p->btw};
+                  p->wq, p->actq, p->eps, p->smeavg, p->status, p->Zv, p->Xv, p->btw,
+                  p->dq, p->kq, p->eigvq, p->vecq, p->nocpq, p->xa, p->dvq};
   for (void* b : bufs) hipFree(b);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
@@ -522,7 +533,66 @@ int wtb_bse_plan_stage_ms(const wtb_bse_plan* p, double* ms) {
 }
 
 static void plan_drop_graph(wtb_bse_plan* p);
+
+// q mode (bsebnds, subprograms/bse_kpath-tool.f90:187-389): per q, the band windows at k
+// (valence) and k+q (conduction), the matrix with the exchange term, its eigenvalues
+// (ZHEEV('N','U'), :353-389).  status[0] holds the k grid's checks, status[1] the k+q grid's.
+static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
+  hipStream_t s = p->stream;
+  const wtb_bse_params& P = p->prm;
+  const int nk = p->nk, nw = p->nw;
+  int rc;
+  auto mark = [&](int i) {
+    if (p->timing) hipEventRecord(p->ev[i], s);
+  };
+  mark(0);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p->status, 2);
+  if ((rc = launch_kgrid(s, p->dm.rlat, P.ngrid, P.mshift, p->kpt))) return rc;
+  if ((rc = launch_hk(s, p->dm, p->kpt, nk, p->phase, p->hk4))) return rc;
+  if ((rc = launch_jacobi(s, p->hk4, nk, nw, p->ef32, p->scs, P.nc, P.nv, nullptr, nullptr, p->nocp, p->eigv,
+                          p->vec, p->status)))
+    return rc;
+  if ((rc = launch_kgrid(s, p->dm.rlat, P.ngrid, P.mshift, p->kq, p->dq))) return rc;
+  if ((rc = launch_hk(s, p->dm, p->kq, nk, p->phase, p->hk4))) return rc;
+  if ((rc = launch_jacobi(s, p->hk4, nk, nw, p->ef32, p->scs, P.nc, P.nv, nullptr, nullptr, p->nocpq, p->eigvq,
+                          p->vecq, p->status + 1)))
+    return rc;
+  if ((rc = launch_mix_q(s, nk, P.nc, P.nv, nw, p->eigvq, p->vecq, p->eigv, p->vec))) return rc;
+  mark(1);
+  if ((rc = launch_exchange_vec(s, nk, P.nc, P.nv, nw, p->vec, p->xa))) return rc;
+  if ((rc = launch_vq(s, p->dm.rlat, p->ca, p->dq, p->dvq))) return rc;
+  mark(2);
+  if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
+  if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H, (int)p->ldh, p->Hf,
+                             nullptr, p->xa, p->dvq)))
+    return rc;
+  mark(3);
+  if (build_only) return WTB_OK;
+  p->tw.kev = nullptr;
+  p->tw.kev_cap = 0;
+  if (p->mixed) {
+    if ((rc = run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, nullptr))) return rc;
+  } else if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, nullptr))) {
+    return rc;
+  }
+  mark(4);
+  if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, nullptr, p->lambda))) return rc;
+  mark(5);
+  mark(6);
+  return WTB_OK;
+}
+
+// the status of a q-mode run: the reference checks the valence window on the k grid and
+// the conduction window on the k+q grid (quantumnumbers2(.., nocpk, nocpq, ..),
+// general_subs.f90:517-529)
+static int qmode_status(const int st[2]) {
+  const int k_bits = st[0] & ~(1 << WTB_ERR_NCOND);
+  const int q_bits = st[1] & ~(1 << WTB_ERR_NVAL);
+  return k_bits | q_bits;
+}
+
 static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
+  if (p->qmode) return plan_enqueue_q(p, build_only);
   hipStream_t s = p->stream;
   const wtb_bse_params& P = p->prm;
   const int nk = p->nk, nw = p->nw, N = p->N;
@@ -660,8 +730,13 @@ int wtb_bse_plan_results(wtb_bse_plan* p, double* w, double* act, double* eps, d
   }
   hipStream_t s = p->ctx->stream;
   const int N = p->N;
-  int st = 0;
-  WTB_HIP(hipMemcpyAsync(&st, p->status, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (p->qmode && (act || eps || egap)) {
+    set_error("wtb_bse_plan_results: a finite-Q plan gives the exciton energies only (act, eps, egap NULL)");
+    return WTB_ERR_ARG;
+  }
+  int st = 0, stq[2] = {0, 0};
+  WTB_HIP(hipMemcpyAsync(p->qmode ? stq : &st, p->status, sizeof(int) * (p->qmode ? 2 : 1), hipMemcpyDeviceToHost,
+                         s));
   if (w) WTB_HIP(hipMemcpyAsync(w, p->lambda, sizeof(double) * N, hipMemcpyDeviceToHost, s));
   if (act) WTB_HIP(hipMemcpyAsync(act, p->act, sizeof(double) * 8 * N, hipMemcpyDeviceToHost, s));
   if (eps)
@@ -695,6 +770,7 @@ int wtb_bse_plan_results(wtb_bse_plan* p, double* w, double* act, double* eps, d
       if (hipEventElapsedTime(&t, p->kev[2 * c], p->kev[2 * c + 1]) == hipSuccess) p->kernel_ms += t;
     }
   }
+  if (p->qmode) st = qmode_status(stq);
   int rc = status_to_code(st);
   if (rc) set_error(status_text(rc));
   return rc;
@@ -1105,11 +1181,11 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     set_error("wtb_bse_plan_lowest: needs a plan in the precision form (WTB_MIXED)");
     return WTB_ERR_ARG;
   }
-  if (p->prm.bset && eps && dp->lanczos_steps > 0) {
+  if ((p->prm.bset || p->qmode) && eps && dp->lanczos_steps > 0) {
     // the chains give quadratic forms h^H g(H) h; the finite-temperature activities pair
     // h with f h, which they do not cover
-    set_error("wtb_bse_plan_lowest: eps(omega) from the Lanczos chains is T = 0 only (pass eps = NULL "
-              "or lanczos_steps = 0 with bset)");
+    set_error("wtb_bse_plan_lowest: eps(omega) from the Lanczos chains is T = 0, Q = 0 only (pass eps = NULL "
+              "or lanczos_steps = 0 with bset or a finite-Q plan)");
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
@@ -1447,6 +1523,192 @@ int wtb_bsesolver_wf(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* pr
   if (!rc && kpt) rc = wtb_bse_plan_fetch(p, WTB_BUF_KPT, kpt, sizeof(double) * 3 * p->nk);
   if (!rc && nocp) rc = wtb_bse_plan_fetch(p, WTB_BUF_NOCP, nocp, sizeof(int) * p->nk);
   wtb_bse_plan_destroy(p);
+  return rc;
+}
+
+
+// ---------------------------------------------------------------------------
+// finite-Q exciton bands (bsebnds)
+// ---------------------------------------------------------------------------
+static int plan_make_qmode(wtb_bse_plan* p) {
+  const int nk = p->nk, nb = p->nb, nw = p->nw, N = p->N;
+  int rc;
+  if ((rc = dmalloc(&p->dq, 4)) || (rc = dmalloc(&p->kq, (size_t)3 * nk)) || (rc = dmalloc(&p->eigvq, (size_t)nk * nb)) ||
+      (rc = dmalloc(&p->vecq, (size_t)nk * nb * nw)) || (rc = dmalloc(&p->nocpq, nk)) || (rc = dmalloc(&p->xa, N)) ||
+      (rc = dmalloc(&p->dvq, 1)))
+    return rc;
+  const double q0[4] = {0.0, 0.0, 0.0, 0.0};
+  WTB_HIP(hipMemcpy(p->dq, q0, sizeof(q0), hipMemcpyHostToDevice));
+  // eigenvalues only (ZHEEV('N')): no tracked vectors
+  const int panel = p->tw.panel;
+  tridiag_bind(p->tw, p->triwork, N, 32, 0);
+  p->tw.panel = panel;
+  if ((rc = dc_prepare(p->dcwork, N, 0, p->tree))) return rc;
+  p->nrhs = 0;
+  p->qmode = 1;
+  plan_drop_graph(p);
+  return WTB_OK;
+}
+
+int wtb_bse_qplan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, wtb_bse_plan** out) {
+  if (prm && prm->bset) {
+    set_error("wtb_bse_qplan_create: finite Q at finite temperature (bsebndstemp) is not built");
+    return WTB_ERR_ARG;
+  }
+  int rc = wtb_bse_plan_create(c, m, prm, out);
+  if (rc) return rc;
+  if ((rc = plan_make_qmode(*out))) {
+    wtb_bse_plan_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+int wtb_bse_plan_set_q(wtb_bse_plan* p, const double* q) {
+  if (!p || !q || !p->qmode) {
+    set_error("wtb_bse_plan_set_q: needs a finite-Q plan (wtb_bse_qplan_create) and q[4]");
+    return WTB_ERR_ARG;
+  }
+  for (int i = 0; i < 4; ++i)
+    if (!std::isfinite(q[i])) {
+      set_error("wtb_bse_plan_set_q: q is not finite");
+      return WTB_ERR_NONFINITE;
+    }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  WTB_HIP(hipMemcpyAsync(p->dq, q, sizeof(double) * 4, hipMemcpyHostToDevice, p->stream));
+  WTB_HIP(hipStreamSynchronize(p->stream));
+  return WTB_OK;
+}
+
+int wtb_kpathbse(const double* rlat, int nks, const double* ks, int npts, double* qauxv, double* labels) {
+  if (!rlat || !ks || !qauxv || nks < 2 || npts < 2) {
+    set_error("wtb_kpathbse: need rlat, ks, qauxv, nks >= 2 and npts >= 2");
+    return WTB_ERR_ARG;
+  }
+  // recvec with the REAL pi (mhkpack_subs.f90:409-432)
+  double a[3][3], b[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) a[i][c] = rlat[i + 3 * c];
+  auto cross = [](const double* u, const double* v, double* w) {
+    w[0] = (u[1] * v[2]) - (u[2] * v[1]);
+    w[1] = (u[2] * v[0]) - (u[0] * v[2]);
+    w[2] = (u[0] * v[1]) - (u[1] * v[0]);
+  };
+  double v23[3], v31[3], v12[3];
+  cross(a[1], a[2], v23);
+  cross(a[2], a[0], v31);
+  cross(a[0], a[1], v12);
+  const double vol = std::fabs((a[0][0] * v23[0]) + (a[0][1] * v23[1]) + (a[0][2] * v23[2]));
+  const double f = (2.0 * (double)3.14159265358979f) / vol;
+  for (int c = 0; c < 3; ++c) {
+    b[0][c] = f * v23[c];
+    b[1][c] = f * v31[c];
+    b[2][c] = f * v12[c];
+  }
+  // kpathbse (mhkpack_subs.f90:132-208)
+  const int nseg = nks / 2, nq = nseg * npts;
+  std::vector<double> ksx((size_t)nks * 3), kdis(nseg);
+  for (int i = 0; i < nks; ++i)
+    for (int c = 0; c < 3; ++c)
+      ksx[3 * i + c] = ks[i] * b[0][c] + ks[i + nks] * b[1][c] + ks[i + 2 * nks] * b[2][c];
+  double tot = 0.0;
+  for (int j = 0; j < nseg; ++j) {
+    const double* e = &ksx[3 * (2 * j + 1)];
+    const double* s0 = &ksx[3 * (2 * j)];
+    const double d0 = e[0] - s0[0], d1 = e[1] - s0[1], d2 = e[2] - s0[2];
+    kdis[j] = std::sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    tot = tot + kdis[j];
+  }
+  for (int j = 0; j < nseg; ++j) kdis[j] = kdis[j] / tot;
+  double acc = 0.0;
+  if (labels) labels[0] = 0.0;
+  for (int j = 0; j < nseg; ++j) {
+    const double* s0 = &ksx[3 * (2 * j)];
+    const double* e = &ksx[3 * (2 * j + 1)];
+    for (int i = 0; i < npts; ++i) {
+      const int r = j * npts + i;
+      qauxv[r] = acc + (kdis[j] * (double)i) / (double)(npts - 1);
+      for (int c = 0; c < 3; ++c) qauxv[r + (size_t)nq * (1 + c)] = s0[c] + ((e[c] - s0[c]) * (double)i) / (double)(npts - 1);
+    }
+    acc = acc + kdis[j];
+    if (labels) labels[j + 1] = acc;
+  }
+  return WTB_OK;
+}
+
+int wtb_bsebnds(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, int nq, const double* qauxv, int lanes,
+                double* exk) {
+  if (!c || !m || !prm || nq < 1 || !qauxv || !exk) {
+    set_error("wtb_bsebnds: null argument or nq < 1");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  const int nl = std::max(1, std::min({lanes, nq, 8}));
+  std::vector<wtb_bse_plan*> plans;
+  std::vector<hipStream_t> streams;
+  double *hq = nullptr, *hw = nullptr;
+  int* hst = nullptr;
+  int rc = WTB_OK;
+  for (int l = 0; l < nl && !rc; ++l) {
+    wtb_bse_plan* p = nullptr;
+    rc = wtb_bse_qplan_create(c, m, prm, &p);
+    if (rc) break;
+    plans.push_back(p);
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      set_error("wtb_bsebnds: hipStreamCreate failed");
+      rc = WTB_ERR_HIP;
+      break;
+    }
+    streams.push_back(st);
+    p->stream = st;
+  }
+  const int N = plans.empty() ? 0 : plans[0]->N;
+  if (!rc && (hipHostMalloc((void**)&hq, sizeof(double) * 4 * nq) != hipSuccess ||
+              hipHostMalloc((void**)&hw, sizeof(double) * (size_t)N * nq) != hipSuccess ||
+              hipHostMalloc((void**)&hst, sizeof(int) * 2 * nq) != hipSuccess)) {
+    set_error("wtb_bsebnds: pinned host allocation failed");
+    rc = WTB_ERR_NOMEM;
+  }
+  if (!rc) {
+    for (int iq = 0; iq < nq; ++iq)
+      for (int k = 0; k < 4; ++k) hq[4 * iq + k] = qauxv[iq + (size_t)nq * k];
+    // the q-points round-robin over the lanes; each lane's plan runs its points in order
+    for (int iq = 0; iq < nq && !rc; ++iq) {
+      wtb_bse_plan* p = plans[iq % nl];
+      hipStream_t st = p->stream;
+      if (hipMemcpyAsync(p->dq, hq + 4 * iq, sizeof(double) * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+        rc = WTB_ERR_HIP;
+        break;
+      }
+      if ((rc = wtb_bse_plan_execute(p))) break;
+      if (hipMemcpyAsync(hw + (size_t)N * iq, p->lambda, sizeof(double) * N, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(hst + 2 * iq, p->status, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = WTB_ERR_HIP;
+    }
+    for (hipStream_t st : streams)
+      if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = WTB_ERR_HIP;
+    if (rc == WTB_ERR_HIP) set_error("wtb_bsebnds: HIP error while running the q-points");
+  }
+  if (!rc) {
+    for (int iq = 0; iq < nq && !rc; ++iq) {
+      const int code = status_to_code(qmode_status(hst + 2 * iq));
+      if (code) {
+        set_error(status_text(code));
+        rc = code;
+      }
+    }
+    if (!rc) std::memcpy(exk, hw, sizeof(double) * (size_t)N * nq);
+  }
+  for (wtb_bse_plan* p : plans) {
+    hipStreamSynchronize(p->stream);
+    p->stream = c->stream;
+    wtb_bse_plan_destroy(p);
+  }
+  for (hipStream_t st : streams) hipStreamDestroy(st);
+  if (hq) hipHostFree(hq);
+  if (hw) hipHostFree(hw);
+  if (hst) hipHostFree(hst);
   return rc;
 }
 

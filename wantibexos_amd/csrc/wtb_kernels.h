@@ -23,8 +23,9 @@ struct CoulombArgs {
 };
 
 // stage (a)+(b)
+// d_q (optional, device): a q-path row (coordinate, qx, qy, qz): the grid k + q
 int launch_kgrid(hipStream_t s, const double* d_rlat, const int ng[3], const double sh[3],
-                 double* d_kpt);
+                 double* d_kpt, const double* d_q = nullptr);
 int launch_hr_stack(hipStream_t s, const double* hop, const double* ihop, const int* ff, int nvec,
                     int nw, zc* B);
 int launch_hk(hipStream_t s, const DevModel& dm, const double* d_kpt, int nk, zc* d_phase,
@@ -39,10 +40,18 @@ int launch_dipoles(hipStream_t s, const zc* d_hk4, int nk, int nw, int nc, int n
 // stage (c)
 int launch_wtable(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, double* d_wtab);
 // d_fdeh (optional, finite temperature): kernel terms of column J of the lower triangle
-// (row J of the reference's upper one) times fdeh[J]
+// (row J of the reference's upper one) times fdeh[J].  d_xa, d_vq (optional, finite Q):
+// the exchange term -V(q) a_I conj(a_J)
 int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng[3],
                      const double* d_wtab, const double* d_eigv, const zc* d_vec, zc* d_H, int ldh,
-                     zf* d_Hf, const double* d_fdeh = nullptr);
+                     zf* d_Hf, const double* d_fdeh = nullptr, const zc* d_xa = nullptr,
+                     const double* d_vq = nullptr);
+// finite-Q exciton bands: conduction rows [nv, nb) of eigv/vec from the k+q window (eigvq/vecq);
+// a[I] = <c_I|v_I> (c at k+q, v at k); vq = V(q) (0 when q = 0)
+int launch_mix_q(hipStream_t s, int nk, int nc, int nv, int nw, const double* d_eigvq, const zc* d_vecq,
+                 double* d_eigv, zc* d_vec);
+int launch_exchange_vec(hipStream_t s, int nk, int nc, int nv, int nw, const zc* d_vec, zc* d_a);
+int launch_vq(hipStream_t s, const double* d_rlat, const CoulombArgs& ca, const double* d_q, double* d_vq);
 // fdeh[I] = f(E_v) - f(E_c) of basis state I, f(E) = 1/(exp(E beta) + 1)
 int launch_fermi_factors(hipStream_t s, int nk, int nc, int nv, const double* d_eigv, double beta,
                          double* d_fdeh);

@@ -1,6 +1,7 @@
 ! Drop-in replacements for the reference's bsesolver
 ! (subprograms/bse_solver-tool-diel.f90:2-660), bsesolvertemp
-! (subprograms/bse_solver-tool-diel-temp.f90:2-660) and bsedielraw
+! (subprograms/bse_solver-tool-diel-temp.f90:2-660), bsebnds
+! (subprograms/bse_kpath-tool.f90:3-479) and bsedielraw
 ! (subprograms/bse_diel-tool.f90:5-245): same names, same argument lists, same
 ! output files, with the computation done by libwtb_bse.so on the GPU.
 !
@@ -254,3 +255,130 @@ subroutine bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebse
 	close(200)
 	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
 end subroutine bsedielraw
+
+
+! bsebnds (subprograms/bse_kpath-tool.f90:3-479): exciton bands along the KPATH_BSE path.
+! The path comes from the reference's own kpathbse (which also writes KLABELS-BSE.dat);
+! every q-point's matrix is built and diagonalised on the GPU (wtb_bsebnds, q-points on
+! four streams); bands_bse.dat is written as the reference writes it (:437-446).
+subroutine bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+		     ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+		     exc,mshift,coultype,ez,w1,r0,lc,rk,meshtype,bsewf,excwf0,excwff)
+	use iso_c_binding
+	use hamiltonian_input_variables
+	use wtb_gpu
+	implicit none
+	integer :: nthreads
+	integer,dimension(3) :: ngrid
+	integer :: nc,nv
+	double precision :: numdos, ebse0, ebsef, numbse, sme, exc, rk, ktol
+	double precision,dimension(3) :: mshift, ediel
+	character(len=70) :: params, orbw, kpaths, kpathsbse, outputfolder, calcparms, meshtype
+	character(len=5) :: coultype
+	logical :: bsewf
+	integer :: excwf0, excwff
+	double precision :: ez, w1, lc, r0
+
+	type(c_ptr) :: ctx
+	type(wtb_tb_model) :: m
+	type(wtb_bse_params) :: p
+	double precision, allocatable, target :: t_hop(:,:,:), t_ihop(:,:,:)
+	integer, allocatable, target :: t_rvec(:,:), t_ff(:)
+	double precision, target :: t_rlat(3,3)
+	double precision, allocatable :: ks(:,:), qauxv(:,:), exk(:,:)
+	integer :: nks, nkpts, nq, dimbse, i, i2, erro
+	integer(c_int) :: rc
+
+	if (bsewf) stop "BSE_WF with BSE_BND: the exciton wave functions along the path are not available on the GPU"
+
+	OPEN(UNIT=500, FILE= kpathsbse,STATUS='old', IOSTAT=erro)
+	if (erro/=0) stop "Error opening bse-kpath input file"
+	OPEN(UNIT=300, FILE=trim(outputfolder)//"log_bse_kpath.dat",STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening log_bse_kpath output file"
+	OPEN(UNIT=400, FILE=trim(outputfolder)//"bands_bse.dat",STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening bands_bse output file"
+
+	call hamiltonian_input_read(200, params)
+	read(500,*) nks
+	read(500,*) nkpts
+	allocate(ks(nks,3))
+	do i=1,nks
+		read(500,*) ks(i,1),ks(i,2),ks(i,3)
+	end do
+	nq = (nks/2)*nkpts
+	allocate(qauxv(nq,4))
+	call kpathbse(outputfolder,rlat(1,:),rlat(2,:),rlat(3,:),nks,ks,nkpts,qauxv)
+
+	allocate(t_hop(nvec,w90basis,w90basis), t_ihop(nvec,w90basis,w90basis))
+	allocate(t_rvec(nvec,3), t_ff(nvec))
+	t_hop = hopmatrices
+	t_ihop = ihopmatrices
+	t_rvec = rvec
+	t_ff = ffactor(1:nvec)
+	t_rlat = rlat
+	m%nw = w90basis
+	m%nvec = nvec
+	m%rlat = c_loc(t_rlat)
+	m%rvec = c_loc(t_rvec)
+	m%ffactor = c_loc(t_ff)
+	m%hop = c_loc(t_hop)
+	m%ihop = c_loc(t_ihop)
+	m%efermi = efermi
+	m%scs = scs
+
+	p%ngrid = ngrid
+	p%mshift = mshift
+	p%nc = nc
+	p%nv = nv
+	p%coultype = wtb_coulomb_id(coultype)
+	p%ediel = ediel
+	p%lc = lc
+	p%ez = ez
+	p%w = w1
+	p%r0 = r0
+	p%ktol = ktol
+	p%cshift = sme
+	p%ebse0 = ebse0
+	p%ebsef = ebsef
+	p%numbse = max(2, int(numbse))
+	p%dtfull = 0
+	p%cpol = 0
+	p%renorm = 0
+	p%f154 = 0
+	p%precision = WTB_FP64
+	p%bset = 0
+	p%ta = WTB_TA_FA
+	p%temp = 0d0
+	p%st = 0d0
+	p%phavg = 0d0
+
+	dimbse = ngrid(1)*ngrid(2)*ngrid(3)*nc*nv
+	allocate(exk(dimbse,nq))
+	write(300,*) 'threads:', nthreads
+	write(300,*) 'grid:',ngrid(1),ngrid(2),ngrid(3)
+	write(300,*) 'ktol-coulomb:', ktol
+	write(300,*) 'kmesh shift:',mshift(1),mshift(2),mshift(3)
+	write(300,*) 'conduction bands, nc:',nc,'  ','valence bands, nv:',nv
+	write(300,*) 'number of kpoints in the path:','   ',nq
+	write(300,*) 'computed on the GPU (libwtb_bse)'
+
+	rc = wtb_create(0_c_int, ctx)
+	call wtb_check(rc)
+	rc = wtb_bsebnds(ctx, m, p, int(nq,c_int), qauxv, 4_c_int, exk)
+	call wtb_check(rc)
+	rc = wtb_destroy(ctx)
+
+	do i2=1,dimbse
+		do i=1,nq
+			write(400,*) real(qauxv(i,1)),exk(i2,i)
+		end do
+		write(400,*)
+	end do
+
+	close(200)
+	close(300)
+	close(400)
+	close(500)
+	deallocate(ks, qauxv, exk)
+	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
+end subroutine bsebnds

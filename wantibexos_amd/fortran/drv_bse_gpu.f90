@@ -1,5 +1,5 @@
-! The BSE + spectrum path of the reference's program main (wtb_main.f90:60-177,
-! 253-261) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
+! The BSE + spectrum and exciton-band paths of the reference's program main
+! (wtb_main.f90:60-190, 253-261) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
 ! reference's bsesolver and bsedielraw.  The input parser and the TB reader
 ! are the reference's own (INCLUDEd from its sources by the Makefile).
 !
@@ -50,6 +50,11 @@ program wtb_gpu_main
 			ebse0,ebsef,numbse,cshift,ktol,params,kpaths,kpathsbse,orbw,ediel, &
 			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff,dtfull,&
 			cpol,tmcoef,st,phavg,ta,temp)
+	end if
+	if (bsekpath) then
+		call bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+			ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff)
 	end if
 	if (spec .and. (bse .or. bset)) then
 		call bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)

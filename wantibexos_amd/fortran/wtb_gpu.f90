@@ -75,6 +75,15 @@ module wtb_gpu
 			real(c_double) :: kpt(*)
 			integer(c_int) :: nocp(*)
 		end function
+		integer(c_int) function wtb_bsebnds(ctx, model, prm, nq, qauxv, lanes, exk) &
+				bind(C, name="wtb_bsebnds")
+			import :: c_int, c_ptr, c_double, wtb_tb_model, wtb_bse_params
+			type(c_ptr), value :: ctx
+			type(wtb_tb_model) :: model
+			type(wtb_bse_params) :: prm
+			integer(c_int), value :: nq, lanes
+			real(c_double) :: qauxv(*), exk(*)
+		end function
 		integer(c_int) function wtb_bse_spectrum(ctx, n, e, act6, rlat, ngrid, ebse0, ebsef, &
 				numbse, sme, renorm, eps) bind(C, name="wtb_bse_spectrum")
 			import :: c_int, c_ptr, c_double

@@ -154,6 +154,10 @@ def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -
     ]
     if p.get("bsewf"):
         lines += ["BSE_WF= T", f"EXC_WF_I= {p['excwf'][0]}", f"EXC_WF_F= {p['excwf'][1]}"]
+    if p.get("kpath_bse"):                  # exciton bands (BSE_BND): bsebnds alone
+        lines[lines.index("BSE= T")] = "BSE= F"
+        lines[lines.index("SPEC= T")] = "SPEC= F"
+        lines += ["BSE_BND= T", f'KPATH_BSE= "{p["kpath_bse"]}"']
     if p.get("bset"):                       # finite temperature: bsesolvertemp instead of bsesolver
         lines[lines.index("BSE= T")] = "BSE= F"
         lines += ["BSET= T", f"TEMP= {p['temp']!r}", f"TA= {p['ta']}", f"ST= {p['st']!r}",

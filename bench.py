@@ -18,6 +18,8 @@ of that matrix; skip it with --no-iterative).
     python bench.py --form davidson --config c5   # the iterative form alone
     python bench.py --form batch                  # the batching form: a set of uneven
                                                   # models, batch API vs single plans
+    python bench.py --form bands                  # exciton bands (finite Q) along
+                                                  # Gamma-K-K'-Gamma, s per q-point
 
     python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu]
     python bench.py --temp 3000 --ta VE --st 0.3 --phavg 0.05   # finite temperature
@@ -247,6 +249,111 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+HEX_PATH = [(0.0, 0.0, 0.0), (0.6667, -0.3333, 0.0), (0.6667, -0.3333, 0.0), (0.3333, 0.3333, 0.0),
+            (0.3333, 0.3333, 0.0), (0.0, 0.0, 0.0)]
+
+
+def cpu_baseline_bands(cfg, model, max_threads):
+    """The unmodified reference program (oracle/_ref/wtb.x, BSE_BND= T) on the same model
+    for a two-point path (Gamma, K): wall time per q-point."""
+    from wantibexos_amd import tbmodel
+    exe = ROOT / "oracle" / "_ref" / "wtb.x"
+    if not exe.exists():
+        return None
+    nthreads = max(1, min(max_threads, os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "out").mkdir()
+        tbmodel.write_tb(model, td / "tb.dat")
+        (td / "kpath.dat").write_text("2\n2\n0.0 0.0 0.0\n0.6667 -0.3333 0.0\n")
+        (td / "input.dat").write_text(tbmodel.input_dat(dict(cfg, kpath_bse="kpath.dat"), "tb.dat", nthreads=nthreads))
+        env = dict(os.environ, OMP_NUM_THREADS=str(nthreads), MKL_NUM_THREADS=str(nthreads))
+        t0 = time.perf_counter()
+        r = subprocess.run([str(exe)], stdin=open(td / "input.dat"), cwd=td, env=env, capture_output=True,
+                           text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        ok = (td / "out" / "bands_bse.dat").exists()
+    if not ok:
+        log("cpu baseline (bands) failed:", r.stdout[-400:], r.stderr[-400:])
+        return None
+    return dict(value=wall / 2, unit="s/q-point", cores=nthreads, kind="reference",
+                sample=(f"the reference program's bsebnds on {cfg['name']} (N={cfg['N']}) for a two-point path "
+                        f"(Gamma, K): wall {wall:.1f}s incl. its k-grid setup and file output"))
+
+
+def bench_bands(args, cfg, model):
+    """Exciton bands along Gamma-K-K'-Gamma (--npts points per segment): s per q-point with
+    the plans kept (one finite-Q plan and stream per lane, graphs captured in the warmup)."""
+    from wantibexos_amd import bse
+    ctx = bse.Context(0)
+    qp, _ = bse.kpathbse(model.rlat, np.array(HEX_PATH), args.npts)
+    nq = len(qp)
+    out = {}
+    ref = None
+    for prec in (["mixed", "fp64"] if args.precision == "mixed" else ["fp64"]):
+        for lanes in sorted({args.lanes, 1}, reverse=True):
+            with bse.BSEBands(ctx, model, params(cfg, prec), lanes=lanes) as b:
+                for _ in range(max(1, args.warmup)):
+                    exk = b.run(qp)
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    exk = b.run(qp)
+                dt = (time.perf_counter() - t0) / args.steps
+            out[(prec, lanes)] = (dt, exk)
+            log(f"bands {prec} lanes={lanes}: {dt * 1e3:.1f} ms per path of {nq} q-points")
+            if prec == "fp64" and lanes == args.lanes:
+                ref = exk
+    N = out[(args.precision, args.lanes)][1].shape[0]
+    cfg = dict(cfg, N=N)
+    # dominant kernel: the reduction's GEMV in one q-point's execute (per-launch HIP events)
+    with bse.BSEPlan(ctx, model, params(cfg, args.precision), finite_q=True) as plan:
+        plan.set_q(qp[1])
+        plan.execute()
+        plan.energies()
+        plan.set_timing(2)
+        plan.execute()
+        plan.energies()
+        k_ms, k_n = plan.kernel_ms()
+    elem = 8.0 if args.precision == "mixed" else 16.0
+    hemv_b, _ = tridiag_bytes(N, elem=elem)
+    avg_s = k_ms * 1e-3 / max(k_n, 1)
+    per_launch = hemv_b / max(k_n, 1)
+    ach = per_launch / avg_s / 1e9
+    roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s", frac=round(ach / HBM_PEAK, 4),
+                    traffic=None, kernel="k_tri_hemv", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
+                    bytes_per_launch=round(per_launch))
+    cpu = None if args.no_cpu else cpu_baseline_bands(cfg, model, args.cpu_threads)
+    ctx.close()
+    dt, exk = out[(args.precision, args.lanes)]
+    checks = {"lanes_bitwise_equal": bool(np.array_equal(exk, out[(args.precision, 1)][1])),
+              "sorted": bool(np.all(np.diff(exk, axis=0) >= 0)),
+              "path_ends_equal_at_shared_points": float(np.abs(exk[:, args.npts - 1] - exk[:, args.npts]).max())}
+    if ref is not None and args.precision == "mixed":
+        checks["mixed_vs_fp64_max_dW_eV"] = float(np.abs(exk - ref).max())
+    res = {
+        "metric": "exciton bands (finite-Q BSE) s per q-point",
+        "value": round(dt / nq, 6),
+        "unit": "s/q-point",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if args.precision == "fp64" else "c64/c128",
+        "data": "synthetic (seeded random tight-binding model)",
+        "config": {"workload": f"{cfg['name']}_bands_GKK'G_{nq}q", "bse_dim": N, "q_points": nq,
+                   "lanes": args.lanes, "precision": args.precision},
+        "lanes_1_s_per_q": round(out[(args.precision, 1)][0] / nq, 6),
+        "fp64_s_per_q": round(out[("fp64", args.lanes)][0] / nq, 6) if ("fp64", args.lanes) in out else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "checks": checks,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,11 +362,12 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed (default): the precision form, timed beside the plain FP64 form")
     ap.add_argument("--config", choices=["c2", "c1", "c5"], default="c2")
-    ap.add_argument("--form", choices=["full", "davidson", "batch"], default="full",
+    ap.add_argument("--form", choices=["full", "davidson", "batch", "bands"], default="full",
                     help="full: the whole eigensolve (plain FP64 or precision form); davidson: the "
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")
     ap.add_argument("--nev", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=4, help="batching form: streams")
+    ap.add_argument("--lanes", type=int, default=4, help="batching form and bands: streams")
+    ap.add_argument("--npts", type=int, default=4, help="bands: q-points per path segment (3 segments)")
     ap.add_argument("--tune", action="store_true", help="batching form: run the panel autotuner first")
     ap.add_argument("--lanczos-steps", type=int, default=250)
     ap.add_argument("--guard", type=int, default=32, help="iterative form: extra Ritz pairs carried along")
@@ -314,6 +422,8 @@ def main():
     if args.form == "davidson":
         print(json.dumps(bench_davidson(args, cfg, model)), flush=True)
         return
+    if args.form == "bands":
+        return bench_bands(args, cfg, model)
     prm = params(cfg, args.precision)
     ctx = bse.Context(0)
     plan = bse.BSEPlan(ctx, model, prm)

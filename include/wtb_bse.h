@@ -321,6 +321,13 @@ int wtb_kpathbse(const double* rlat, int nks, const double* ks, int npts, double
  * each.  prm->bset must be 0 (bsebndstemp is not built).                          */
 int wtb_bsebnds(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, int nq, const double* qauxv,
                 int lanes, double* exk);
+/* The same with the plans kept: create (one finite-Q plan and stream per lane, each
+ * captured into a HIP graph at its first q), run any number of paths, destroy.  */
+typedef struct wtb_bse_bands wtb_bse_bands;
+int wtb_bse_bands_create(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, int lanes,
+                         wtb_bse_bands** bands);
+int wtb_bse_bands_run(wtb_bse_bands* bands, int nq, const double* qauxv, double* exk);
+int wtb_bse_bands_destroy(wtb_bse_bands* bands);
 /* Plan form of one q-point: create, then wtb_bse_plan_set_q(plan, q[4]) (a qauxv
  * row) before build/execute; results gives w only (act, eps, egap NULL);
  * WTB_BUF_HBSE holds the finite-Q matrix after build.                            */

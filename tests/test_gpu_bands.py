@@ -76,6 +76,12 @@ def test_bands_n576(ctx, big):
     assert np.abs(e4 - o["exk"]).max() < 1e-9
     em = bse.bsebnds(ctx, m, params_of(p, precision=bse.MIXED), qp, lanes=4)
     assert np.abs(em - e4).max() < 1e-3
+    # the kept plans: a second path (the same points reversed) replays the captured graphs
+    with bse.BSEBands(ctx, m, params_of(p), lanes=3) as b:
+        r1 = b.run(qp)
+        r2 = b.run(qp[::-1])
+    assert np.array_equal(r1, e4)
+    assert np.array_equal(r2, e4[:, ::-1])
 
 
 def test_bands_iterative_form(ctx, big):

@@ -137,6 +137,9 @@ def _load():
         "wtb_bsebnds": (ip, [vp, vp, vp, ip, vp, ip, vp]),
         "wtb_bse_qplan_create": (ip, [vp, vp, vp, C.POINTER(vp)]),
         "wtb_bse_plan_set_q": (ip, [vp, vp]),
+        "wtb_bse_bands_create": (ip, [vp, vp, vp, ip, C.POINTER(vp)]),
+        "wtb_bse_bands_run": (ip, [vp, ip, vp, vp]),
+        "wtb_bse_bands_destroy": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -596,6 +599,36 @@ def bsebnds(ctx: Context, model: TBModel, params: BSEParams, qpath, lanes: int =
     exk = np.zeros((params.dim, nq), order="F")
     _check(lib().wtb_bsebnds(ctx.h, C.byref(cm.s), C.byref(cp), nq, qf.ctypes.data, int(lanes), exk.ctypes.data))
     return np.ascontiguousarray(exk)
+
+
+class BSEBands:
+    """Exciton bands with the plans kept (wtb_bse_bands_*): run(qpath) -> exk (N, nq)."""
+
+    def __init__(self, ctx: Context, model: TBModel, params: BSEParams, lanes: int = 4):
+        self.ctx = ctx
+        self.params = params
+        self._cm = _CModel(model)
+        self._cp = params.to_c()
+        h = C.c_void_p()
+        _check(lib().wtb_bse_bands_create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), int(lanes), C.byref(h)))
+        self.h = h
+
+    def run(self, qpath):
+        qf = np.asfortranarray(np.asarray(qpath, dtype=np.float64))
+        exk = np.zeros((self.params.dim, qf.shape[0]), order="F")
+        _check(lib().wtb_bse_bands_run(self.h, qf.shape[0], qf.ctypes.data, exk.ctypes.data))
+        return np.ascontiguousarray(exk)
+
+    def close(self):
+        if self.h:
+            lib().wtb_bse_bands_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def bsesolver(ctx: Context, model: TBModel, params: BSEParams):

@@ -568,8 +568,8 @@ static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
     return rc;
   mark(3);
   if (build_only) return WTB_OK;
-  p->tw.kev = nullptr;
-  p->tw.kev_cap = 0;
+  p->tw.kev = p->kernel_timing ? p->kev.data() : nullptr;
+  p->tw.kev_cap = p->kernel_timing ? p->N : 0;
   if (p->mixed) {
     if ((rc = run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, nullptr))) return rc;
   } else if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, nullptr))) {
@@ -1636,79 +1636,134 @@ int wtb_kpathbse(const double* rlat, int nks, const double* ks, int npts, double
   return WTB_OK;
 }
 
+struct wtb_bse_bands {
+  wtb_ctx* ctx = nullptr;
+  std::vector<wtb_bse_plan*> plans;     // one finite-Q plan per lane
+  std::vector<hipStream_t> streams;
+  int N = 0;
+  double* hq = nullptr;                 // pinned staging: q rows, energies, status words
+  double* hw = nullptr;
+  int* hst = nullptr;
+  int cap = 0;                          // q-points the staging holds
+};
+
+int wtb_bse_bands_destroy(wtb_bse_bands* b) {
+  if (!b) return WTB_OK;
+  hipSetDevice(b->ctx->device);
+  for (wtb_bse_plan* p : b->plans) {
+    hipStreamSynchronize(p->stream);
+    p->stream = b->ctx->stream;
+    wtb_bse_plan_destroy(p);
+  }
+  for (hipStream_t st : b->streams) hipStreamDestroy(st);
+  if (b->hq) hipHostFree(b->hq);
+  if (b->hw) hipHostFree(b->hw);
+  if (b->hst) hipHostFree(b->hst);
+  delete b;
+  return WTB_OK;
+}
+
+int wtb_bse_bands_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, int lanes,
+                         wtb_bse_bands** out) {
+  if (!c || !m || !prm || !out || lanes < 1 || lanes > 8) {
+    set_error("wtb_bse_bands_create: null argument or lanes outside 1..8");
+    return WTB_ERR_ARG;
+  }
+  *out = nullptr;
+  WTB_HIP(hipSetDevice(c->device));
+  auto* b = new wtb_bse_bands();
+  b->ctx = c;
+  int rc = WTB_OK;
+  for (int l = 0; l < lanes && !rc; ++l) {
+    wtb_bse_plan* p = nullptr;
+    if ((rc = wtb_bse_qplan_create(c, m, prm, &p))) break;
+    b->plans.push_back(p);
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      set_error("wtb_bse_bands_create: hipStreamCreate failed");
+      rc = WTB_ERR_HIP;
+      break;
+    }
+    b->streams.push_back(st);
+    p->stream = st;
+  }
+  if (rc) {
+    wtb_bse_bands_destroy(b);
+    return rc;
+  }
+  b->N = b->plans[0]->N;
+  *out = b;
+  return WTB_OK;
+}
+
+int wtb_bse_bands_run(wtb_bse_bands* b, int nq, const double* qauxv, double* exk) {
+  if (!b || nq < 1 || !qauxv || !exk) {
+    set_error("wtb_bse_bands_run: null argument or nq < 1");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(b->ctx->device));
+  const int N = b->N, nl = (int)b->plans.size();
+  if (nq > b->cap) {
+    if (b->hq) hipHostFree(b->hq);
+    if (b->hw) hipHostFree(b->hw);
+    if (b->hst) hipHostFree(b->hst);
+    b->hq = b->hw = nullptr;
+    b->hst = nullptr;
+    b->cap = 0;
+    if (hipHostMalloc((void**)&b->hq, sizeof(double) * 4 * nq) != hipSuccess ||
+        hipHostMalloc((void**)&b->hw, sizeof(double) * (size_t)N * nq) != hipSuccess ||
+        hipHostMalloc((void**)&b->hst, sizeof(int) * 2 * nq) != hipSuccess) {
+      set_error("wtb_bse_bands_run: pinned host allocation failed");
+      return WTB_ERR_NOMEM;
+    }
+    b->cap = nq;
+  }
+  for (int iq = 0; iq < nq; ++iq)
+    for (int k = 0; k < 4; ++k) b->hq[4 * iq + k] = qauxv[iq + (size_t)nq * k];
+  // the q-points round-robin over the lanes; each lane's plan runs its points in order
+  int rc = WTB_OK;
+  for (int iq = 0; iq < nq && !rc; ++iq) {
+    wtb_bse_plan* p = b->plans[iq % nl];
+    hipStream_t st = p->stream;
+    if (hipMemcpyAsync(p->dq, b->hq + 4 * iq, sizeof(double) * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+      set_error("wtb_bse_bands_run: q upload failed");
+      rc = WTB_ERR_HIP;
+      break;
+    }
+    if ((rc = wtb_bse_plan_execute(p))) break;
+    if (hipMemcpyAsync(b->hw + (size_t)N * iq, p->lambda, sizeof(double) * N, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->hst + 2 * iq, p->status, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess) {
+      set_error("wtb_bse_bands_run: result copy failed");
+      rc = WTB_ERR_HIP;
+    }
+  }
+  for (hipStream_t st : b->streams)
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) {
+      set_error("wtb_bse_bands_run: HIP error while running the q-points");
+      rc = WTB_ERR_HIP;
+    }
+  for (int iq = 0; iq < nq && !rc; ++iq) {
+    const int code = status_to_code(qmode_status(b->hst + 2 * iq));
+    if (code) {
+      set_error(status_text(code));
+      rc = code;
+    }
+  }
+  if (!rc) std::memcpy(exk, b->hw, sizeof(double) * (size_t)N * nq);
+  return rc;
+}
+
 int wtb_bsebnds(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, int nq, const double* qauxv, int lanes,
                 double* exk) {
   if (!c || !m || !prm || nq < 1 || !qauxv || !exk) {
     set_error("wtb_bsebnds: null argument or nq < 1");
     return WTB_ERR_ARG;
   }
-  WTB_HIP(hipSetDevice(c->device));
-  const int nl = std::max(1, std::min({lanes, nq, 8}));
-  std::vector<wtb_bse_plan*> plans;
-  std::vector<hipStream_t> streams;
-  double *hq = nullptr, *hw = nullptr;
-  int* hst = nullptr;
-  int rc = WTB_OK;
-  for (int l = 0; l < nl && !rc; ++l) {
-    wtb_bse_plan* p = nullptr;
-    rc = wtb_bse_qplan_create(c, m, prm, &p);
-    if (rc) break;
-    plans.push_back(p);
-    hipStream_t st;
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-      set_error("wtb_bsebnds: hipStreamCreate failed");
-      rc = WTB_ERR_HIP;
-      break;
-    }
-    streams.push_back(st);
-    p->stream = st;
-  }
-  const int N = plans.empty() ? 0 : plans[0]->N;
-  if (!rc && (hipHostMalloc((void**)&hq, sizeof(double) * 4 * nq) != hipSuccess ||
-              hipHostMalloc((void**)&hw, sizeof(double) * (size_t)N * nq) != hipSuccess ||
-              hipHostMalloc((void**)&hst, sizeof(int) * 2 * nq) != hipSuccess)) {
-    set_error("wtb_bsebnds: pinned host allocation failed");
-    rc = WTB_ERR_NOMEM;
-  }
-  if (!rc) {
-    for (int iq = 0; iq < nq; ++iq)
-      for (int k = 0; k < 4; ++k) hq[4 * iq + k] = qauxv[iq + (size_t)nq * k];
-    // the q-points round-robin over the lanes; each lane's plan runs its points in order
-    for (int iq = 0; iq < nq && !rc; ++iq) {
-      wtb_bse_plan* p = plans[iq % nl];
-      hipStream_t st = p->stream;
-      if (hipMemcpyAsync(p->dq, hq + 4 * iq, sizeof(double) * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
-        rc = WTB_ERR_HIP;
-        break;
-      }
-      if ((rc = wtb_bse_plan_execute(p))) break;
-      if (hipMemcpyAsync(hw + (size_t)N * iq, p->lambda, sizeof(double) * N, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipMemcpyAsync(hst + 2 * iq, p->status, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess)
-        rc = WTB_ERR_HIP;
-    }
-    for (hipStream_t st : streams)
-      if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = WTB_ERR_HIP;
-    if (rc == WTB_ERR_HIP) set_error("wtb_bsebnds: HIP error while running the q-points");
-  }
-  if (!rc) {
-    for (int iq = 0; iq < nq && !rc; ++iq) {
-      const int code = status_to_code(qmode_status(hst + 2 * iq));
-      if (code) {
-        set_error(status_text(code));
-        rc = code;
-      }
-    }
-    if (!rc) std::memcpy(exk, hw, sizeof(double) * (size_t)N * nq);
-  }
-  for (wtb_bse_plan* p : plans) {
-    hipStreamSynchronize(p->stream);
-    p->stream = c->stream;
-    wtb_bse_plan_destroy(p);
-  }
-  for (hipStream_t st : streams) hipStreamDestroy(st);
-  if (hq) hipHostFree(hq);
-  if (hw) hipHostFree(hw);
-  if (hst) hipHostFree(hst);
+  wtb_bse_bands* b = nullptr;
+  int rc = wtb_bse_bands_create(c, m, prm, std::max(1, std::min({lanes, nq, 8})), &b);
+  if (!rc) rc = wtb_bse_bands_run(b, nq, qauxv, exk);
+  wtb_bse_bands_destroy(b);
   return rc;
 }
 

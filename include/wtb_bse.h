@@ -318,7 +318,9 @@ int wtb_kpathbse(const double* rlat, int nks, const double* ks, int npts, double
  * subroutines/bse_subs_kpath.f90:1-214: conduction bands at k+q, valence bands at
  * k, direct term V(k-k')<c|c'><v|v'>, exchange -V(q)<c|v><v'|c'> for q != 0) at each
  * q = qauxv(iq, 2:4).  The q-points run on `lanes` HIP streams (1..8), one plan
- * each.  prm->bset must be 0 (bsebndstemp is not built).                          */
+ * each.  With prm->bset = 1 it is bsebndstemp (subprograms/bse_kpath-tool-temp.f90,
+ * matrizelbsekqtemp in subroutines/bse_subs_temp.f90:128-338): the gap correction
+ * on both grids, the kernel (direct - exchange) times the row's f_v - f_c.         */
 int wtb_bsebnds(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, int nq, const double* qauxv,
                 int lanes, double* exk);
 /* The same with the plans kept: create (one finite-Q plan and stream per lane, each

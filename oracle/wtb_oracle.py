@@ -648,12 +648,14 @@ def monkhorst_pack_q(q, ngrid, shift, rlat):
     return out
 
 
-def bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, params, ngrid, rlat):
+def bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, params, ngrid, rlat, fdeh=None):
     """matrizelbsekq (bse_subs_kpath.f90:1-214) in the loop of bse_kpath-tool.f90:325-345,
     j >= i: conduction bands at k+q, valence bands at k, V(k-k') on the k grid; for q != 0
     (modvecq, general_subs.f90:310-321) the exchange term
     - V(q) <c_i|v_i> <v_j|c_j> (c at k+q, v at k).  Returns the Hermitian completion of
-    the upper triangle that ZHEEV('U') reads."""
+    the upper triangle that ZHEEV('U') reads.  With fdeh (bsebndstemp, matrizelbsekqtemp,
+    bse_subs_temp.f90:128-338): the kernel (direct - exchange) times the row state's
+    f_v - f_c, diagonal (Ec - Ev) + (V(0) - exchange) f."""
     ki = stt[:, 3] - 1
     vi = stt[:, 1] - 1
     ci = stt[:, 2] - 1
@@ -666,7 +668,8 @@ def bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, params, ngrid, rlat)
     p = params
     V = coulomb(p["coultype"], dk, ngrid, rlat, p["ediel"], p["lc"], p["ez"], p["w"], p["r0"], p["ktol"])
     H = V * oc * ov
-    diag = (eigv_q[ki, ci] - eigv_k[ki, vi]) + np.diag(V)
+    de = eigv_q[ki, ci] - eigv_k[ki, vi]
+    kd = np.diag(V).astype(np.complex128)
     modq = math.sqrt((q[1] ** 2 + q[2] ** 2) + q[3] ** 2)
     if modq != 0.0:
         vq = float(coulomb(p["coultype"], np.asarray(q[1:4], dtype=np.float64).reshape(1, 3), ngrid, rlat,
@@ -674,7 +677,12 @@ def bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, params, ngrid, rlat)
         a = np.einsum("nm,nm->n", np.conj(vc), vv)      # <c_i|v_i>
         X = (vq * a)[:, None] * np.conj(a)[None, :]
         H = H - X
-        diag = diag - np.diag(X)
+        kd = kd - np.diag(X)
+    if fdeh is not None:
+        H = H * fdeh[:, None]
+        diag = de + kd * fdeh
+    else:
+        diag = de + kd
     H = np.triu(H, 1)
     H = H + np.conj(H.T)
     H[np.arange(len(H)), np.arange(len(H))] = diag
@@ -683,10 +691,15 @@ def bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, params, ngrid, rlat)
 
 def run_bands(model, ngrid, qpath, **kw):
     """bsebnds (bse_kpath-tool.f90:187-445): exk (N, nq), the ascending exciton energies
-    of the matrix at every q of the path (ZHEEV('N','U'))."""
+    of the matrix at every q of the path (ZHEEV('N','U')); with bset, bsebndstemp
+    (bse_kpath-tool-temp.f90: gap correction on both grids, row factors f_v - f_c)."""
     p = dict(DEFAULT_PARAMS)
     p.update(kw)
     kpt = monkhorst_pack(ngrid, p["shift"], model.rlat)
+    if p["bset"]:                                         # bsebndstemp: eigsys(scs + tcor) on both grids
+        tcor = gap_correction(p["ta"], p["st"], p["phavg"], p["temp"])
+        if tcor != 0.0:
+            model = dataclasses.replace(model, scs=model.scs + tcor)
     E, U, nocp = eigsys(kpt, model)
     eigv_k, vec_k = select_bands(E, U, nocp, p["nc"], p["nv"])
     nq = len(qpath)
@@ -702,7 +715,11 @@ def run_bands(model, ngrid, qpath, **kw):
         if (p["nc"] > model.nw - nocpq).any():
             raise ValueError("Number of conduction states higher then unocuppied states")
         stt = quantumnumbers2(len(kpt), p["nc"], p["nv"], np.full(len(kpt), p["nv"]), model.nw)
-        H = bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, p, ngrid, model.rlat)
+        fdeh = None
+        if p["bset"]:                                     # f_v(k) - f_c(k+q) of the row state
+            ki = stt[:, 3] - 1
+            fdeh = fermidisteh(eigv_q[ki, stt[:, 2] - 1], eigv_k[ki, stt[:, 1] - 1], p["temp"])
+        H = bse_matrix_q(kpt, eigv_k, vec_k, eigv_q, vec_q, stt, q, p, ngrid, model.rlat, fdeh)
         W = np.linalg.eigvalsh(H, UPLO="U")
         if exk is None:
             exk = np.empty((len(W), nq))

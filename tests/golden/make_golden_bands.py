@@ -1,7 +1,8 @@
 """Writes the finite-Q exciton band fixtures under tests/golden/bands/.
 
 Runs the unmodified reference program (oracle/_ref/wtb.x, built by oracle/Makefile
-from the reference sources) with BSE_BND= T on seeded synthetic TB models and a
+from the reference sources) with BSE_BND= T (BSET_BND= T for the finite-temperature
+cases) on seeded synthetic TB models and a
 KPATH_BSE file, and stores what it writes: bands_bse.dat (path coordinate as a
 REAL, then the exciton energy, for every exciton and q-point) and
 KLABELS-BSE.dat (the segment ends), beside the inputs (TB model arrays, run
@@ -49,6 +50,12 @@ def cases():
          dict(BASE, ngrid=(5, 5, 1), coultype="V2DT2", lc=8.0, shift=(0.1, 0.2, 0.0)), HEX_PATH[:4], 3),
         ("cub6_g3_v3d", lambda: tbmodel.cubic3d(44, 6, (3, 3, 3)),
          dict(BASE, ngrid=(3, 3, 3), coultype="V3D", nc=1, nv=1), CUB_PATH, 3),
+        # bsebndstemp (BSET_BND= T): gap correction on both grids, row factors f_v - f_c
+        ("hex6_g4_v2dk_t5000_ve", lambda: tbmodel.hex2d(45, 6, (4, 4, 1)),
+         dict(BASE, ngrid=(4, 4, 1), coultype="V2DK", ediel=(1.5, 4.0, 2.0), lc=6.0, bset=True, temp=5000.0,
+              ta="VE", st=0.4, phavg=0.04), HEX_PATH[:4], 4),
+        ("hex6_g3_v2drk_t3000_be", lambda: tbmodel.hex2d(46, 6, (3, 3, 1)),
+         dict(BASE, ngrid=(3, 3, 1), bset=True, temp=3000.0, ta="BE", st=0.05, phavg=1500.0), HEX_PATH, 3),
     ]
 
 

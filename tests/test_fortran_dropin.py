@@ -57,9 +57,11 @@ def test_dropin_outputs_match_reference(tmp_path, p):
     # F15.4 text: identical except where a value sits on a rounding boundary
     assert np.abs(a - b).max() <= 1.00001e-4
     assert np.mean(a != b) < 0.01
-    for comp in ["xx", "xy", "xz", "yy", "yz", "zz"]:
-        ea = table(tmp_path / "ref" / f"bse_diel_{comp}.dat")
-        eb = table(tmp_path / "gpu" / f"bse_diel_{comp}.dat")
+    files = [f"bse_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]
+    files += [f"bse_diel-pol_{c}.dat" for c in ["x", "y", "z", "sp", "sm"]]      # bsedielrawpol
+    for comp in files:
+        ea = table(tmp_path / "ref" / comp)
+        eb = table(tmp_path / "gpu" / comp)
         assert ea.shape == eb.shape == (1001, 3)
         assert np.abs(ea[:, 0] - eb[:, 0]).max() < 1e-12
         for c in (1, 2):
@@ -115,3 +117,22 @@ def test_dropin_exciton_bands(tmp_path):
     ka = table(tmp_path / "ref" / "KLABELS-BSE.dat", skip=0)
     kb = table(tmp_path / "gpu" / "KLABELS-BSE.dat", skip=0)
     assert np.array_equal(ka, kb)
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+def test_dropin_exciton_bands_finite_temperature(tmp_path):
+    """BSET_BND=T: the drop-in bsebndstemp against the reference's."""
+    p = dict(P, ngrid=(4, 4, 1), kpath_bse="kpath.dat", bset=True, temp=4000.0, ta="VE", st=0.3, phavg=0.05)
+    model = tbmodel.hex2d(47, 6, p["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    (tmp_path / "kpath.dat").write_text("4\n3\n0.0 0.0 0.0\n0.6667 -0.3333 0.0\n0.6667 -0.3333 0.0\n"
+                                        "0.3333 0.3333 0.0\n")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    assert (tmp_path / "ref" / "bands_bse.dat").exists(), rr.stdout + rr.stderr
+    a = table(tmp_path / "ref" / "bands_bse.dat", skip=0)
+    b = table(tmp_path / "gpu" / "bands_bse.dat", skip=0)
+    assert a.shape == b.shape == (64 * 6, 2)
+    assert np.array_equal(a[:, 0], b[:, 0])
+    assert np.abs(a[:, 1] - b[:, 1]).max() < 1e-9

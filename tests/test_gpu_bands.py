@@ -1,5 +1,7 @@
-"""GPU: finite-Q exciton bands (bsebnds, subprograms/bse_kpath-tool.f90) through the C-ABI.
-- every bands fixture: wtb_bsebnds against the reference program's bands_bse.dat,
+"""GPU: finite-Q exciton bands (bsebnds, subprograms/bse_kpath-tool.f90, and bsebndstemp)
+through the C-ABI.
+- every bands fixture (two at finite temperature): wtb_bsebnds against the reference
+  program's bands_bse.dat,
   1e-9 eV (the q = 0 path is bit-compatible with bsesolver's; ZHEEV against the
   GPU's Householder + divide and conquer);
 - the plan form of one q: the finite-Q matrix (lower triangle) against the oracle's,
@@ -21,7 +23,9 @@ def params_of(p, **kw):
     from wantibexos_amd import bse
     d = dict(ngrid=p["ngrid"], mshift=p["shift"], nc=p["nc"], nv=p["nv"], coultype=p["coultype"],
              ediel=p["ediel"], lc=p["lc"], ez=p["ez"], w=p["w"], r0=p["r0"], ktol=p["ktol"],
-             cshift=p["cshift"], ebse0=p["ebse0"], ebsef=p["ebsef"], numbse=p["numbse"])
+             cshift=p["cshift"], ebse0=p["ebse0"], ebsef=p["ebsef"], numbse=p["numbse"],
+             bset=p.get("bset", False), temp=p.get("temp", 0.0), ta=p.get("ta", "FA"), st=p.get("st", 0.0),
+             phavg=p.get("phavg", 0.0))
     d.update(kw)
     return bse.BSEParams(**d)
 
@@ -76,6 +80,12 @@ def test_bands_n576(ctx, big):
     assert np.abs(e4 - o["exk"]).max() < 1e-9
     em = bse.bsebnds(ctx, m, params_of(p, precision=bse.MIXED), qp, lanes=4)
     assert np.abs(em - e4).max() < 1e-3
+    # bsebndstemp on the same path
+    ft = dict(bset=True, temp=4000.0, ta="VE", st=0.3, phavg=0.05)
+    et = bse.bsebnds(ctx, m, params_of(p, **ft), qp, lanes=4)
+    ot = O.run_bands(m, p["ngrid"], qp, **okw(dict(p, **ft)))
+    assert np.abs(et - ot["exk"]).max() < 1e-9
+    assert np.abs(et - e4).max() > 1e-3                     # the temperature shows
     # the kept plans: a second path (the same points reversed) replays the captured graphs
     with bse.BSEBands(ctx, m, params_of(p), lanes=3) as b:
         r1 = b.run(qp)

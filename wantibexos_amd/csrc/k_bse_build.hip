@@ -299,23 +299,27 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
     zc val;
     if (I == J) {
       double ec = eigv[(size_t)k * ba.nb + ba.nv + ci], ev = eigv[(size_t)k * ba.nb + vi];
-      // finite temperature: (Ec - Ev) + V(0) f_I (bse_subs_temp.f90:105-108)
-      val = mkz((ec - ev) + (fdeh ? wv * fdeh[J] : wv), 0.0);
-      // finite Q: - V(q) <c|v> <v|c> (bse_subs_kpath.f90:158-160)
-      if (xa) {
-        const zc a = xa[I];
-        val = val - (vq * a) * conj(a);
+      if (fdeh) {
+        // finite temperature: (Ec - Ev) + V(0) f_I (bse_subs_temp.f90:105-108); with finite Q
+        // (Ec - Ev) + (V(0) - V(q) <c|v><v|c>) f_I (:300-310)
+        zc kern = mkz(wv, 0.0);
+        if (xa) kern = kern - (vq * xa[I]) * conj(xa[I]);
+        val = mkz(ec - ev, 0.0) + kern * fdeh[J];
+      } else {
+        val = mkz((ec - ev) + wv, 0.0);
+        // finite Q: - V(q) <c|v> <v|c> (bse_subs_kpath.f90:158-160)
+        if (xa) val = val - (vq * xa[I]) * conj(xa[I]);
       }
     } else {
       zc oc = Oc[(kl * ba.nc + ci) * rc_ + (kl2 * ba.nc + cj)];
       zc ov = Ov[(kl * ba.nv + vi) * rv_ + (kl2 * ba.nv + vj)];
       val = (wv * oc) * ov;
-      // finite temperature: the reference's element (J, I), J < I, is V vc vv f_J
-      // (bse_subs_temp.f90:112-120); this one is its conjugate
-      if (fdeh) val = val * fdeh[J];
       // finite Q: the reference's (J, I) element subtracts V(q) a_J conj(a_I)
       // (bse_subs_kpath.f90:178-191); conjugated: V(q) conj(a_J) a_I
       if (xa) val = val - (vq * conj(xa[J])) * xa[I];
+      // finite temperature: the reference's element (J, I), J < I, is the kernel times f_J
+      // (bse_subs_temp.f90:112-120, 312-327); this one is its conjugate
+      if (fdeh) val = val * fdeh[J];
     }
     size_t off = (size_t)I + (size_t)ba.ldh * J;
     if constexpr (F32) stz(Hf[off], val);

@@ -536,7 +536,9 @@ static void plan_drop_graph(wtb_bse_plan* p);
 
 // q mode (bsebnds, subprograms/bse_kpath-tool.f90:187-389): per q, the band windows at k
 // (valence) and k+q (conduction), the matrix with the exchange term, its eigenvalues
-// (ZHEEV('N','U'), :353-389).  status[0] holds the k grid's checks, status[1] the k+q grid's.
+// (ZHEEV('N','U'), :353-389).  With bset it is bsebndstemp (bse_kpath-tool-temp.f90): the
+// scissor carries tcor on both grids and the kernel terms the row factors f_v - f_c.
+// status[0] holds the k grid's checks, status[1] the k+q grid's.
 static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
   hipStream_t s = p->stream;
   const wtb_bse_params& P = p->prm;
@@ -558,13 +560,15 @@ static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
                           p->vecq, p->status + 1)))
     return rc;
   if ((rc = launch_mix_q(s, nk, P.nc, P.nv, nw, p->eigvq, p->vecq, p->eigv, p->vec))) return rc;
+  // bsebndstemp: f_v(k) - f_c(k+q) of every basis state, from the mixed window
+  if (P.bset && (rc = launch_fermi_factors(s, nk, P.nc, P.nv, p->eigv, p->beta, p->fdeh))) return rc;
   mark(1);
   if ((rc = launch_exchange_vec(s, nk, P.nc, P.nv, nw, p->vec, p->xa))) return rc;
   if ((rc = launch_vq(s, p->dm.rlat, p->ca, p->dq, p->dvq))) return rc;
   mark(2);
   if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
   if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H, (int)p->ldh, p->Hf,
-                             nullptr, p->xa, p->dvq)))
+                             P.bset ? p->fdeh : nullptr, p->xa, p->dvq)))
     return rc;
   mark(3);
   if (build_only) return WTB_OK;
@@ -1551,10 +1555,6 @@ static int plan_make_qmode(wtb_bse_plan* p) {
 }
 
 int wtb_bse_qplan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, wtb_bse_plan** out) {
-  if (prm && prm->bset) {
-    set_error("wtb_bse_qplan_create: finite Q at finite temperature (bsebndstemp) is not built");
-    return WTB_ERR_ARG;
-  }
   int rc = wtb_bse_plan_create(c, m, prm, out);
   if (rc) return rc;
   if ((rc = plan_make_qmode(*out))) {

@@ -1,12 +1,14 @@
 ! Drop-in replacements for the reference's bsesolver
 ! (subprograms/bse_solver-tool-diel.f90:2-660), bsesolvertemp
 ! (subprograms/bse_solver-tool-diel-temp.f90:2-660), bsebnds
-! (subprograms/bse_kpath-tool.f90:3-479) and bsedielraw
-! (subprograms/bse_diel-tool.f90:5-245): same names, same argument lists, same
+! (subprograms/bse_kpath-tool.f90:3-479), bsebndstemp
+! (subprograms/bse_kpath-tool-temp.f90), bsedielraw
+! (subprograms/bse_diel-tool.f90:5-245) and bsedielrawpol
+! (subprograms/bse_diel-tool-pol.f90:5-242): same names, same argument lists, same
 ! output files, with the computation done by libwtb_bse.so on the GPU.
 !
 ! Use: INCLUDE this file (after the reference's subroutines/module_input_read.f90
-! and wantibexos_amd/fortran/wtb_gpu.f90) in place of the two reference files,
+! and wantibexos_amd/fortran/wtb_gpu.f90) in place of those six reference files,
 ! and link -lwtb_bse.  Arguments the GPU path does not use (numdos, kpaths,
 ! orbw, exc, rk, meshtype, ...) are accepted and ignored as in the reference.
 
@@ -264,9 +266,6 @@ end subroutine bsedielraw
 subroutine bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 		     ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
 		     exc,mshift,coultype,ez,w1,r0,lc,rk,meshtype,bsewf,excwf0,excwff)
-	use iso_c_binding
-	use hamiltonian_input_variables
-	use wtb_gpu
 	implicit none
 	integer :: nthreads
 	integer,dimension(3) :: ngrid
@@ -278,6 +277,54 @@ subroutine bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	logical :: bsewf
 	integer :: excwf0, excwff
 	double precision :: ez, w1, lc, r0
+	double precision :: zero
+
+	zero = 0d0
+	call wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			      kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,.false.,zero,zero,"FA",zero)
+end subroutine bsebnds
+
+! bsebndstemp (subprograms/bse_kpath-tool-temp.f90): the same at finite temperature.
+subroutine bsebndstemp(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+		     ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+		     exc,mshift,coultype,ez,w1,r0,lc,rk,meshtype,bsewf,excwf0,excwff,&
+		     st,phavg,ta,temp)
+	implicit none
+	integer :: nthreads
+	integer,dimension(3) :: ngrid
+	integer :: nc,nv
+	double precision :: numdos, ebse0, ebsef, numbse, sme, exc, rk, ktol
+	double precision,dimension(3) :: mshift, ediel
+	character(len=70) :: params, orbw, kpaths, kpathsbse, outputfolder, calcparms, meshtype
+	character(len=5) :: coultype
+	logical :: bsewf
+	integer :: excwf0, excwff
+	double precision :: ez, w1, lc, r0
+	double precision :: st, phavg, temp
+	character(len=2) :: ta
+
+	call wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			      kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,.true.,st,phavg,ta,temp)
+	if (temp .eq. 0d0) temp = 1E-8
+end subroutine bsebndstemp
+
+subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			    kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,bset,st,phavg,ta,temp)
+	use iso_c_binding
+	use hamiltonian_input_variables
+	use wtb_gpu
+	implicit none
+	integer :: nthreads
+	integer,dimension(3) :: ngrid
+	integer :: nc,nv
+	double precision :: ebse0, ebsef, numbse, sme, ktol
+	double precision,dimension(3) :: mshift, ediel
+	character(len=70) :: params, kpathsbse, outputfolder
+	character(len=5) :: coultype
+	logical :: bsewf, bset
+	double precision :: ez, w1, lc, r0
+	double precision :: st, phavg, temp
+	character(len=2) :: ta
 
 	type(c_ptr) :: ctx
 	type(wtb_tb_model) :: m
@@ -346,15 +393,18 @@ subroutine bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	p%renorm = 0
 	p%f154 = 0
 	p%precision = WTB_FP64
-	p%bset = 0
+	p%bset = merge(1, 0, bset)
 	p%ta = WTB_TA_FA
-	p%temp = 0d0
-	p%st = 0d0
-	p%phavg = 0d0
+	if (ta .eq. "VE") p%ta = WTB_TA_VE
+	if (ta .eq. "BE") p%ta = WTB_TA_BE
+	p%temp = temp
+	p%st = st
+	p%phavg = phavg
 
 	dimbse = ngrid(1)*ngrid(2)*ngrid(3)*nc*nv
 	allocate(exk(dimbse,nq))
 	write(300,*) 'threads:', nthreads
+	if (bset) write(300,*) 'temperature', temp
 	write(300,*) 'grid:',ngrid(1),ngrid(2),ngrid(3)
 	write(300,*) 'ktol-coulomb:', ktol
 	write(300,*) 'kmesh shift:',mshift(1),mshift(2),mshift(3)
@@ -381,4 +431,75 @@ subroutine bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	close(500)
 	deallocate(ks, qauxv, exk)
 	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
-end subroutine bsebnds
+end subroutine wtb_bands_dropin
+
+
+! bsedielrawpol (subprograms/bse_diel-tool-pol.f90:5-242): eps(omega) for the x, y, z,
+! sigma+ and sigma- polarisations from bse_opt_diel-pol.dat, every one with the
+! diagonal (delta = 1) form of optdiel.  wtb_bse_spectrum gives that form to its first
+! three components, so the five spectra take two calls.
+subroutine bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme)
+	use iso_c_binding
+	use hamiltonian_input_variables
+	use wtb_gpu
+	implicit none
+	integer :: nthreads, nc, nv
+	integer,dimension(3) :: ngrid
+	character(len=70) :: outputfolder, params
+	logical :: renorm
+	double precision :: ebse0, ebsef, numbse, sme
+
+	type(c_ptr) :: ctx
+	double precision, allocatable :: ebse(:), a1(:,:), a2(:,:), eps1(:,:,:), eps2(:,:,:)
+	double precision :: spinf, elux
+	character(len=2) :: aread
+	integer :: dimbse, j, nom, erro, c
+	integer(c_int) :: rc
+	character(len=2), parameter :: tag(5) = (/"x ","y ","z ","sp","sm"/)
+
+	call hamiltonian_input_read(200, params)
+	spinf = 1.0
+	if (systype .eq. "NP") spinf = 2.0
+
+	dimbse = ngrid(1)*ngrid(2)*ngrid(3)*nc*nv
+	nom = int(numbse)
+	allocate(ebse(dimbse), a1(dimbse,6), a2(dimbse,6), eps1(nom,2,6), eps2(nom,2,6))
+	a1 = 0d0
+	a2 = 0d0
+	OPEN(UNIT=100, FILE=trim(outputfolder)//"bse_opt_diel-pol.dat",STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening bse_opt_diel-pol input file"
+	read(100,*) aread
+	do j=1,dimbse
+		read(100,*) ebse(j),a1(j,1),a1(j,2),a1(j,3),a2(j,1),a2(j,2)
+	end do
+	close(100)
+
+	rc = wtb_create(0_c_int, ctx)
+	call wtb_check(rc)
+	rc = wtb_bse_spectrum(ctx, dimbse, ebse, a1, rlat, ngrid, ebse0, ebsef, nom, sme, &
+			      merge(1, 0, renorm), eps1)
+	call wtb_check(rc)
+	rc = wtb_bse_spectrum(ctx, dimbse, ebse, a2, rlat, ngrid, ebse0, ebsef, nom, sme, &
+			      merge(1, 0, renorm), eps2)
+	call wtb_check(rc)
+	rc = wtb_destroy(ctx)
+
+	do c=1,5
+		OPEN(UNIT=300+c, FILE=trim(outputfolder)//"bse_diel-pol_"//trim(tag(c))//".dat", &
+		     STATUS='unknown', IOSTAT=erro)
+		if (erro/=0) stop "Error opening bse_diel-pol output file"
+		write(300+c,*) "#","  ","energy","  ","real","  ","imag"
+		do j=1,nom
+			elux = ebse0 + dble(((ebsef-ebse0)*(j-1))/(numbse-1.))
+			if (c <= 3) then
+				write(300+c,*) elux, real(spinf*eps1(j,1,c)), real(spinf*eps1(j,2,c))
+			else
+				write(300+c,*) elux, real(spinf*eps2(j,1,c-3)), real(spinf*eps2(j,2,c-3))
+			end if
+		end do
+		close(300+c)
+	end do
+	close(200)
+	deallocate(ebse, a1, a2, eps1, eps2)
+	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
+end subroutine bsedielrawpol

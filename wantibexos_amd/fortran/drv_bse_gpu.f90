@@ -1,5 +1,5 @@
 ! The BSE + spectrum and exciton-band paths of the reference's program main
-! (wtb_main.f90:60-190, 253-261) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
+! (wtb_main.f90:60-190, 253-284) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
 ! reference's bsesolver and bsedielraw.  The input parser and the TB reader
 ! are the reference's own (INCLUDEd from its sources by the Makefile).
 !
@@ -51,12 +51,22 @@ program wtb_gpu_main
 			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff,dtfull,&
 			cpol,tmcoef,st,phavg,ta,temp)
 	end if
-	if (bsekpath) then
+	if (bsekpath .and. bsetbnd) then
+		write(*,*) "ERROR: Choose BSE_BND=T or BSET_BND=T not both"
+		stop
+	else if (bsekpath) then
 		call bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 			ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
 			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff)
+	else if (bsetbnd) then
+		call bsebndstemp(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+			ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff,&
+			st,phavg,ta,temp)
 	end if
 	if (spec .and. (bse .or. bset)) then
 		call bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
+		if (cpol) call bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef, &
+					     numbse,cshift)
 	end if
 end program wtb_gpu_main

@@ -154,14 +154,16 @@ def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -
     ]
     if p.get("bsewf"):
         lines += ["BSE_WF= T", f"EXC_WF_I= {p['excwf'][0]}", f"EXC_WF_F= {p['excwf'][1]}"]
-    if p.get("kpath_bse"):                  # exciton bands (BSE_BND): bsebnds alone
+    temp = [f"TEMP= {p.get('temp', 0.0)!r}", f"TA= {p.get('ta', 'FA')}", f"ST= {p.get('st', 0.0)!r}",
+            f"PHAVG= {p.get('phavg', 0.0)!r}"]
+    if p.get("kpath_bse"):                  # exciton bands alone: bsebnds, or bsebndstemp with bset
         lines[lines.index("BSE= T")] = "BSE= F"
         lines[lines.index("SPEC= T")] = "SPEC= F"
-        lines += ["BSE_BND= T", f'KPATH_BSE= "{p["kpath_bse"]}"']
-    if p.get("bset"):                       # finite temperature: bsesolvertemp instead of bsesolver
+        lines += [f'KPATH_BSE= "{p["kpath_bse"]}"']
+        lines += (["BSET_BND= T"] + temp) if p.get("bset") else ["BSE_BND= T"]
+    elif p.get("bset"):                     # finite temperature: bsesolvertemp instead of bsesolver
         lines[lines.index("BSE= T")] = "BSE= F"
-        lines += ["BSET= T", f"TEMP= {p['temp']!r}", f"TA= {p['ta']}", f"ST= {p['st']!r}",
-                  f"PHAVG= {p['phavg']!r}"]
+        lines += ["BSET= T"] + temp
     return "\n".join(lines) + "\n"
 
 

@@ -7,8 +7,8 @@ test_gpu_mixed (their bset cases).
 - the activity sum rule sum_n act_n = Re sum_j h_a,j conj(h_b,j) f_j (X unitary)
   on the GPU's own dipoles and Fermi factors, every component (1e-9 relative);
 - the precision form on the same matrix: 1 meV, eps 1e-3;
-- the iterative form's Davidson on a finite-temperature plan: the full solve's lowest
-  excitons; eps from its Lanczos chains is refused (T = 0 only)."""
+- the iterative form on a finite-temperature plan: the full solve's lowest excitons,
+  and eps(omega) from polarised Lanczos chains against the full solve's."""
 
 import numpy as np
 import pytest
@@ -75,15 +75,24 @@ def test_finite_temperature_precision_form(ctx, model):
 
 
 def test_finite_temperature_iterative_form(ctx, model):
+    """Davidson on the finite-temperature matrix, and eps(omega) from Lanczos chains of
+    h_a + f h_b and h_a - f h_b (Re[t conj(u)] = (|t + u|^2 - |t - u|^2) / 4): the full
+    FP64 solve's excitons (1 meV bar) and spectrum (1e-3 relative, normwise per component)."""
     from wantibexos_amd import bse
-    print("case: N=576 finite temperature VE 3000 K, Davidson", flush=True)
+    print("case: N=576 finite temperature VE 3000 K, Davidson + Lanczos", flush=True)
     full = bse.bsesolver(ctx, model, params())
     with bse.BSEPlan(ctx, model, params(precision=bse.MIXED)) as plan:
-        r = plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=0)
-        with pytest.raises(bse.WTBError):
-            plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=50)
+        r = plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=300)
     assert r["info"]["converged"] == 1
     assert np.abs(r["W"] - full["W"][:32]).max() < 1e-4
+    errs = {}
+    for c in range(6):
+        for part in range(2):
+            ref = full["eps"][c, part]
+            if np.abs(ref).max() > 1e-8:
+                errs[(c, part)] = relerr(r["eps"][c, part], ref)
+    print("eps relative errors:", errs)
+    assert len(errs) >= 6 and max(errs.values()) < 1e-3
 
 
 def test_temperature_zero_limit(ctx, model):

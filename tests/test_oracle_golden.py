@@ -168,6 +168,12 @@ def test_finite_temperature_activity_sum_rule():
         assert math.isclose(act[c].sum(), (h[a] * np.conj(h[b]) * f).sum().real, rel_tol=1e-11, abs_tol=1e-11)
     # f = 1: the T = 0 activities
     assert np.allclose(O.activities_temp(X, h, np.ones(n)), O.activities(X, h), rtol=1e-13, atol=1e-13)
+    # the polarisation the Lanczos form's finite-T chains use: Re[t conj(u)] =
+    # (|X^T (h_a + f h_b)|^2 - |X^T (h_a - f h_b)|^2) / 4
+    quad = lambda v: np.abs(X.T @ v) ** 2
+    for c, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+        pol = 0.25 * (quad(h[a] + f * h[b]) - quad(h[a] - f * h[b]))
+        assert np.abs(pol - act[c]).max() < 1e-12
 
 
 def test_f154_rounding():

@@ -421,14 +421,19 @@ __global__ __launch_bounds__(256) void k_lz_c(int N, int j, int kmax, const doub
 
 // start vectors conj(h), conj(h_a + h_b), conj(h_a - h_b) (normalised by the host-side
 // norms computed in k_lz_norms)
+// f (optional, finite temperature): the second vector of a pair is f h_b
 __global__ __launch_bounds__(256) void k_lz_start(int N, int nq, const zc* __restrict__ dip,
                                                   const int* __restrict__ ca, const int* __restrict__ cb,
-                                                  const double* __restrict__ sg, zc* __restrict__ V) {
+                                                  const double* __restrict__ sg, const double* __restrict__ f,
+                                                  zc* __restrict__ V) {
   const int q = blockIdx.y;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= N || q >= nq) return;
   zc h = dip[(size_t)ca[q] * N + i];
-  if (cb[q] >= 0) h = h + sg[q] * dip[(size_t)cb[q] * N + i];
+  if (cb[q] >= 0) {
+    if (f) h = h + (sg[q] * f[i]) * dip[(size_t)cb[q] * N + i];
+    else h = h + sg[q] * dip[(size_t)cb[q] * N + i];
+  }
   V[(size_t)q * N + i] = conj(h);
 }
 
@@ -501,14 +506,14 @@ __global__ __launch_bounds__(256) void k_haydock_eps(int nq, int k, int kmax, co
                                                      const double* __restrict__ bcoef, const double* __restrict__ hn2,
                                                      double caux, double ebse0, double ebsef, int numbse,
                                                      double sme, const double* __restrict__ smeavg, int dtfull,
-                                                     double* __restrict__ eps) {
+                                                     int temp, double* __restrict__ eps) {
   const int jw = blockIdx.x * 256 + threadIdx.x;
   if (jw >= numbse) return;
   const double om = ebse0 + ((ebsef - ebse0) * (double)jw) / ((double)numbse - 1.0);
   const double g = sme + smeavg[0];
   const zc z = mkz(om, g);
   const int nt = k / 4 > 8 ? 8 : (k / 4 > 0 ? k / 4 : 1);
-  double S[9][2];
+  double S[12][2];
   for (int q = 0; q < nq; ++q) {
     const double* a = acoef + (size_t)q * kmax;
     const double* b = bcoef + (size_t)q * kmax;   // b[j] = b_{j+1}
@@ -543,7 +548,15 @@ __global__ __launch_bounds__(256) void k_haydock_eps(int nq, int k, int kmax, co
   }
   for (int c = 0; c < 6; ++c) {
     double sr = 0.0, si = 0.0;
-    if (c < 3) {
+    if (temp) {
+      // finite temperature: Re[t_a conj(u_b)] = (|t_a + u_b|^2 - |t_a - u_b|^2) / 4 with
+      // t = X^T h, u = X^T (f h): chains of h_a + f h_b, h_a - f h_b, every component
+      if (c < 3 || dtfull) {
+        const int qp = 2 * c;
+        sr = 0.25 * (S[qp][0] - S[qp + 1][0]);
+        si = 0.25 * (S[qp][1] - S[qp + 1][1]);
+      }
+    } else if (c < 3) {
       sr = S[c][0];
       si = S[c][1];
     } else if (dtfull) {
@@ -614,11 +627,14 @@ int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, do
   return WTB_OK;
 }
 
+// at most 12 chains: 9 at T = 0 (x, y, z and the pairs of the cross terms), 12 at finite
+// temperature (a pair for every component)
+constexpr int kLzMaxChains = 12;
+
 size_t haydock_work_bytes(int N, int steps) {
-  const size_t nblk = 128;
-  return thin_work_bytes(N, 9) + 256 + sizeof(zc) * (size_t)N * 9 * 5 +
-         sizeof(double) * (9 * nblk * 3 + 2 * 9 * (size_t)(steps + 1) + 64) +
-         sizeof(int) * 64 + 16 * 256;
+  const size_t nblk = 128, m = kLzMaxChains;
+  return thin_work_bytes(N, kLzMaxChains) + 256 + sizeof(zc) * (size_t)N * m * 5 +
+         sizeof(double) * (m * nblk * 3 + 2 * m * (size_t)(steps + 1) + 64) + sizeof(int) * 64 + 16 * 256;
 }
 
 namespace {
@@ -638,26 +654,28 @@ HaydockBufs haydock_carve(void* work, int N, int steps, int nq) {
     w += (bytes + 255) & ~size_t(255);
     return q;
   };
-  for (int i = 0; i < 3; ++i) b.Vb[i] = (zc*)take(sizeof(zc) * (size_t)N * 9);
-  b.HV = (zc*)take(sizeof(zc) * (size_t)N * 9);
-  b.W = (zc*)take(sizeof(zc) * (size_t)N * 9);
-  b.part = (double*)take(sizeof(double) * 9 * kLzBlocks * 2);
-  b.part2 = (double*)take(sizeof(double) * 9 * kLzBlocks);
-  b.acoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
-  b.bcoef = (double*)take(sizeof(double) * 9 * (size_t)(steps + 1));
+  const size_t m = kLzMaxChains;
+  for (int i = 0; i < 3; ++i) b.Vb[i] = (zc*)take(sizeof(zc) * (size_t)N * m);
+  b.HV = (zc*)take(sizeof(zc) * (size_t)N * m);
+  b.W = (zc*)take(sizeof(zc) * (size_t)N * m);
+  b.part = (double*)take(sizeof(double) * m * kLzBlocks * 2);
+  b.part2 = (double*)take(sizeof(double) * m * kLzBlocks);
+  b.acoef = (double*)take(sizeof(double) * m * (size_t)(steps + 1));
+  b.bcoef = (double*)take(sizeof(double) * m * (size_t)(steps + 1));
   b.hn2 = (double*)take(sizeof(double) * 16);
   b.bcur = (double*)take(sizeof(double) * 16);
   b.sg = (double*)take(sizeof(double) * 16);
   b.ca = (int*)take(sizeof(int) * 16);
   b.cb = (int*)take(sizeof(int) * 16);
-  b.hw = (zc*)take(thin_work_bytes(N, nq));
+  b.hw = (zc*)take(thin_work_bytes(N, kLzMaxChains));
+  (void)nq;
   return b;
 }
 }  // namespace
 
 int run_haydock_chains(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
-                       void* work) {
-  const int nq = dtfull ? 9 : 3;
+                       void* work, const double* fdeh) {
+  const int nq = fdeh ? (dtfull ? 12 : 6) : (dtfull ? 9 : 3);
   if (steps < 2 || N < 2) {
     set_error("haydock: needs N >= 2 and at least 2 steps");
     return WTB_ERR_ARG;
@@ -667,10 +685,15 @@ int run_haydock_chains(hipStream_t s, int N, const zf* H, int ldh, const zc* dip
   static const int hca[9] = {0, 1, 2, 0, 0, 0, 0, 1, 1};
   static const int hcb[9] = {-1, -1, -1, 1, 1, 2, 2, 2, 2};
   static const double hsg[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
-  WTB_HIP(hipMemcpyAsync(b.ca, hca, sizeof(hca), hipMemcpyHostToDevice, s));
-  WTB_HIP(hipMemcpyAsync(b.cb, hcb, sizeof(hcb), hipMemcpyHostToDevice, s));
-  WTB_HIP(hipMemcpyAsync(b.sg, hsg, sizeof(hsg), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_lz_start, dim3((N + 255) / 256, nq), dim3(256), 0, s, N, nq, dip, b.ca, b.cb, b.sg, b.Vb[1]);
+  // finite temperature: h_a + f h_b and h_a - f h_b for xx, yy, zz, xy, xz, yz
+  static const int tca[12] = {0, 0, 1, 1, 2, 2, 0, 0, 0, 0, 1, 1};
+  static const int tcb[12] = {0, 0, 1, 1, 2, 2, 1, 1, 2, 2, 2, 2};
+  static const double tsg[12] = {1, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1};
+  WTB_HIP(hipMemcpyAsync(b.ca, fdeh ? tca : hca, fdeh ? sizeof(tca) : sizeof(hca), hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(b.cb, fdeh ? tcb : hcb, fdeh ? sizeof(tcb) : sizeof(hcb), hipMemcpyHostToDevice, s));
+  WTB_HIP(hipMemcpyAsync(b.sg, fdeh ? tsg : hsg, fdeh ? sizeof(tsg) : sizeof(hsg), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_lz_start, dim3((N + 255) / 256, nq), dim3(256), 0, s, N, nq, dip, b.ca, b.cb, b.sg, fdeh,
+                     b.Vb[1]);
   hipLaunchKernelGGL(k_lz_norms, dim3(nq), dim3(256), 0, s, N, b.Vb[1], b.hn2, b.Vb[0], b.bcur);
   WTB_LAUNCH_CHECK();
   const int kmax = steps + 1, nblk = kLzBlocks;
@@ -694,21 +717,22 @@ int run_haydock_chains(hipStream_t s, int N, const zf* H, int ldh, const zc* dip
 
 int run_haydock_eps(hipStream_t s, int N, int dtfull, int steps, const double* d_E1, double caux, double ebse0,
                     double ebsef, int numbse, double sme, int renorm, void* work, double* d_eps, double* d_smeavg,
-                    double* h_coef) {
-  const int nq = dtfull ? 9 : 3;
+                    double* h_coef, int temp) {
+  const int nq = temp ? (dtfull ? 12 : 6) : (dtfull ? 9 : 3);
   HaydockBufs b = haydock_carve(work, N, steps, nq);
   const int kmax = steps + 1;
   hipLaunchKernelGGL(k_lz_gamma, dim3(1), dim3(1), 0, s, steps, b.acoef, b.bcoef, d_E1, N, d_smeavg);
   hipLaunchKernelGGL(k_haydock_eps, dim3((numbse + 255) / 256), dim3(256), 0, s, nq, steps, kmax, b.acoef,
-                     b.bcoef, b.hn2, caux, ebse0, ebsef, numbse, sme, d_smeavg, dtfull, d_eps);
+                     b.bcoef, b.hn2, caux, ebse0, ebsef, numbse, sme, d_smeavg, dtfull, temp, d_eps);
   WTB_LAUNCH_CHECK();
   if (renorm) {
     int rc = launch_imagrenorm(s, d_E1, ebse0, ebsef, numbse, d_eps);
     if (rc) return rc;
   }
   if (h_coef) {
-    WTB_HIP(hipMemcpyAsync(h_coef, b.acoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
-    WTB_HIP(hipMemcpyAsync(h_coef + 9 * kmax, b.bcoef, sizeof(double) * 9 * kmax, hipMemcpyDeviceToHost, s));
+    WTB_HIP(hipMemcpyAsync(h_coef, b.acoef, sizeof(double) * nq * kmax, hipMemcpyDeviceToHost, s));
+    WTB_HIP(hipMemcpyAsync(h_coef + (size_t)nq * kmax, b.bcoef, sizeof(double) * nq * kmax, hipMemcpyDeviceToHost,
+                           s));
   }
   return WTB_OK;
 }

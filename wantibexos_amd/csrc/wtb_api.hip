@@ -1185,11 +1185,10 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
     set_error("wtb_bse_plan_lowest: needs a plan in the precision form (WTB_MIXED)");
     return WTB_ERR_ARG;
   }
-  if ((p->prm.bset || p->qmode) && eps && dp->lanczos_steps > 0) {
-    // the chains give quadratic forms h^H g(H) h; the finite-temperature activities pair
-    // h with f h, which they do not cover
-    set_error("wtb_bse_plan_lowest: eps(omega) from the Lanczos chains is T = 0, Q = 0 only (pass eps = NULL "
-              "or lanczos_steps = 0 with bset or a finite-Q plan)");
+  if (p->qmode && eps && dp->lanczos_steps > 0) {
+    // the spectrum is a Q = 0 quantity; a finite-Q plan has no dipoles
+    set_error("wtb_bse_plan_lowest: eps(omega) needs a Q = 0 plan (pass eps = NULL or lanczos_steps = 0 with a "
+              "finite-Q plan)");
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
@@ -1228,7 +1227,8 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
       s2 = p->lz_stream;
       if (!rc && hipEventCreateWithFlags(&ev_lz, hipEventDisableTiming) != hipSuccess) rc = WTB_ERR_HIP;
       if (!rc && hipStreamWaitEvent(s2, ev[1], 0) != hipSuccess) rc = WTB_ERR_HIP;
-      if (!rc) rc = run_haydock_chains(s2, N, p->Hf, (int)p->ldh, p->dip, P.dtfull, steps, hw);
+      if (!rc)
+        rc = run_haydock_chains(s2, N, p->Hf, (int)p->ldh, p->dip, P.dtfull, steps, hw, P.bset ? p->fdeh : nullptr);
       if (!rc && hipEventRecord(ev_lz, s2) != hipSuccess) rc = WTB_ERR_HIP;
     }
     DavidsonCfg cfg;
@@ -1258,7 +1258,7 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
       if (hipStreamWaitEvent(s, ev_lz, 0) != hipSuccess) rc = WTB_ERR_HIP;
       if (!rc)
         rc = run_haydock_eps(s, N, P.dtfull, steps, dW, caux, P.ebse0, P.ebsef, P.numbse, P.cshift, P.renorm, hw,
-                             p->eps, dsm, nullptr);
+                             p->eps, dsm, nullptr, P.bset);
     }
     // every path: the chains are done before their workspace is freed
     if (s2 && hipStreamSynchronize(s2) != hipSuccess && !rc) rc = WTB_ERR_HIP;

@@ -158,17 +158,19 @@ int launch_chol_inv(hipStream_t s, int b, zc* G, int ld, zc* Linv, int* keep, do
 size_t haydock_work_bytes(int N, int steps);
 // eps[6][2][numbse] from Lanczos chains of the dipoles (dip: 3 x N); d_E1: lowest exciton
 // energy (device); d_smeavg[0] <- 10 (E_max - E_1) / N, [1] <- E_max; h_coef (host, optional):
-// a then b coefficients, 9 x (steps + 1) each
+// a then b coefficients, nchains x (steps + 1) each
 int run_haydock(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
                 const double* d_E1, double caux, double ebse0, double ebsef, int numbse, double sme,
                 int renorm, void* work, double* d_eps, double* d_smeavg, double* h_coef);
 // the same in two parts: the chains (no dependence on E_1, so they can run beside the
 // eigensolver on a stream of their own), then eps from their coefficients; one workspace
+// fdeh (optional, finite temperature): chains of h_a +- f h_b for every component; run
+// run_haydock_eps with temp = 1 on them
 int run_haydock_chains(hipStream_t s, int N, const zf* H, int ldh, const zc* dip, int dtfull, int steps,
-                       void* work);
+                       void* work, const double* fdeh = nullptr);
 int run_haydock_eps(hipStream_t s, int N, int dtfull, int steps, const double* d_E1, double caux, double ebse0,
                     double ebsef, int numbse, double sme, int renorm, void* work, double* d_eps, double* d_smeavg,
-                    double* h_coef);
+                    double* h_coef, int temp = 0);
 
 // learned preconditioner: MLP (5 -> 128 -> 128 -> 1, tanh, FP32) on per-state features
 struct MlpDev {

@@ -52,7 +52,9 @@ __device__ inline zc fma_z(zc a, zc b, zc acc) {
 }
 
 // sum of p[b * stride] over b = b0, b0 + step, ... < b1, added in that order, with the loads
-// issued eight at a time (a plain loop waits for each load before the next)
+// issued eight at a time (a plain loop waits for each load before the next).  The last
+// group loads from clamped indices (b1 - 1 is in range whenever the group runs) and adds
+// only the terms below b1, so short sums keep their loads in flight too.
 template <class V>
 __device__ inline V strided_sum(const V* __restrict__ p, size_t stride, int b0, int b1, int step, V acc) {
   int b = b0;
@@ -63,7 +65,17 @@ __device__ inline V strided_sum(const V* __restrict__ p, size_t stride, int b0, 
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc += v[u];
   }
-  for (; b < b1; b += step) acc += p[(size_t)b * stride];
+  if (b < b1) {
+    V v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int bu = b + u * step;
+      v[u] = p[(size_t)(bu < b1 ? bu : b1 - 1) * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b + u * step < b1) acc += v[u];
+  }
   return acc;
 }
 

@@ -293,6 +293,8 @@ def bench_bands(args, cfg, model):
     for prec in (["mixed", "fp64"] if args.precision == "mixed" else ["fp64"]):
         for lanes in sorted({args.lanes, 1}, reverse=True):
             with bse.BSEBands(ctx, model, params(cfg, prec), lanes=lanes) as b:
+                if args.no_graph:
+                    b.set_graph(False)
                 for _ in range(max(1, args.warmup)):
                     exk = b.run(qp)
                 t0 = time.perf_counter()

@@ -329,6 +329,9 @@ typedef struct wtb_bse_bands wtb_bse_bands;
 int wtb_bse_bands_create(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, int lanes,
                          wtb_bse_bands** bands);
 int wtb_bse_bands_run(wtb_bse_bands* bands, int nq, const double* qauxv, double* exk);
+/* on = 0: launch the kernels directly instead of replaying the captured graphs
+ * (for kernel tracers)                                                            */
+int wtb_bse_bands_set_graph(wtb_bse_bands* bands, int on);
 int wtb_bse_bands_destroy(wtb_bse_bands* bands);
 /* Plan form of one q-point: create, then wtb_bse_plan_set_q(plan, q[4]) (a qauxv
  * row) before build/execute; results gives w only (act, eps, egap NULL);

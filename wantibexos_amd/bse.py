@@ -140,6 +140,7 @@ def _load():
         "wtb_bse_bands_create": (ip, [vp, vp, vp, ip, C.POINTER(vp)]),
         "wtb_bse_bands_run": (ip, [vp, ip, vp, vp]),
         "wtb_bse_bands_destroy": (ip, [vp]),
+        "wtb_bse_bands_set_graph": (ip, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -612,6 +613,10 @@ class BSEBands:
         h = C.c_void_p()
         _check(lib().wtb_bse_bands_create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), int(lanes), C.byref(h)))
         self.h = h
+
+    def set_graph(self, on=True):
+        """False: launch the kernels directly instead of replaying the captured graphs."""
+        _check(lib().wtb_bse_bands_set_graph(self.h, int(on)))
 
     def run(self, qpath):
         qf = np.asfortranarray(np.asarray(qpath, dtype=np.float64))

@@ -1696,6 +1696,12 @@ int wtb_bse_bands_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params
   return WTB_OK;
 }
 
+int wtb_bse_bands_set_graph(wtb_bse_bands* b, int on) {
+  if (!b) return WTB_ERR_ARG;
+  for (wtb_bse_plan* p : b->plans) p->use_graph = on != 0;
+  return WTB_OK;
+}
+
 int wtb_bse_bands_run(wtb_bse_bands* b, int nq, const double* qauxv, double* exk) {
   if (!b || nq < 1 || !qauxv || !exk) {
     set_error("wtb_bse_bands_run: null argument or nq < 1");

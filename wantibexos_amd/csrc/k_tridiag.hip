@@ -87,6 +87,33 @@ __device__ inline zc wave_sum(zc v) {
     }                                                                     \
   } while (0)
 
+// the same on COMPLEX*8 values (FP32 sums over the 64 rows of a tile)
+__device__ inline zf operator+(zf a, zf b) {
+  zf c;
+  c.re = a.re + b.re;
+  c.im = a.im + b.im;
+  return c;
+}
+#define BUTTERFLY16F(pv, lane)                                            \
+  do {                                                                    \
+    int n_ = 16;                                                          \
+    for (int off_ = 32; off_ >= 4; off_ >>= 1) {                          \
+      n_ >>= 1;                                                           \
+      const bool up_ = ((lane) & off_) != 0;                              \
+      for (int i_ = 0; i_ < n_; ++i_) {                                   \
+        const zf send_ = up_ ? pv[i_] : pv[i_ + n_];                      \
+        zf recv_;                                                         \
+        recv_.re = __shfl_xor(send_.re, off_);                            \
+        recv_.im = __shfl_xor(send_.im, off_);                            \
+        pv[i_] = (up_ ? pv[i_ + n_] : pv[i_]) + recv_;                    \
+      }                                                                   \
+    }                                                                     \
+    for (int off_ = 2; off_ > 0; off_ >>= 1) {                            \
+      pv[0].re += __shfl_xor(pv[0].re, off_);                             \
+      pv[0].im += __shfl_xor(pv[0].im, off_);                             \
+    }                                                                     \
+  } while (0)
+
 // alpha' = -1/2 |tau|^2 sum_b puv[b] over the np partials of the previous rowstep, in a
 // fixed order (every thread of the workgroup returns it)
 __device__ inline zc alpha_from_puv(const zc* __restrict__ puv, int np, zc tau) {
@@ -320,8 +347,10 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
 
 // ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
 // 4 waves, wave w owns 16 columns, lane = row: every load moves one column segment.
+// the FP64 instance fits 4 waves per SIMD in 127 registers (no spill), which keeps more
+// tiles in flight; the COMPLEX*8 instance would spill there and stays at its natural 3
 template <class T>
-__global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
+__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
   constexpr int XRN = TB;                   // rows of x~ a workgroup needs
   constexpr int XT = XRN + TB;              // threads loading x~ (rows, then columns)
   __shared__ zc xR[XRN], xC[TB];
@@ -415,19 +444,46 @@ __global__ __launch_bounds__(256) void k_tri_hemv(TriArgs<T> a, int j, int jj, i
   const zc xr = xR[lane];
   zc y1 = mkz(0, 0);
   if (interior) okm = 0xffffu;
+  if constexpr (std::is_same<T, zf>::value) {
+    // COMPLEX*8 matrix: the A^H x~ terms and their sums over the tile's 64 rows in FP32
+    // (the rows' A x~ sums and every sum across tiles stay FP64)
+    zf hf[16];
+    zf xrf;
+    xrf.re = (float)xr.re;
+    xrf.im = (float)xr.im;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int cl = 16 * w + k;
-    zc x = ((okm >> k) & 1u) ? ldz(hraw[k]) : mkz(0, 0);
-    if (diag && lane == cl) x.im = 0.0;                   // diagonal: real part (ZHEMV)
-    y1 = fma_z(x, xC[cl], y1);
-    hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
+    for (int k = 0; k < 16; ++k) {
+      const int cl = 16 * w + k;
+      const bool on = (okm >> k) & 1u;
+      zc x = on ? ldz(hraw[k]) : mkz(0, 0);
+      if (diag && lane == cl) x.im = 0.0;                 // diagonal: real part (ZHEMV)
+      y1 = fma_z(x, xC[cl], y1);
+      const zf h = hraw[k];
+      zf t;
+      t.re = h.re * xrf.re + h.im * xrf.im;               // conj(h) x~(row)
+      t.im = h.re * xrf.im - h.im * xrf.re;
+      const bool low = on && (!diag || lane > cl);         // strictly lower
+      hf[k].re = low ? t.re : 0.0f;
+      hf[k].im = low ? t.im : 0.0f;
+    }
+    y1s[w][lane] = y1;
+    BUTTERFLY16F(hf, lane);
+    if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = ldz(hf[0]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int cl = 16 * w + k;
+      zc x = ((okm >> k) & 1u) ? ldz(hraw[k]) : mkz(0, 0);
+      if (diag && lane == cl) x.im = 0.0;                 // diagonal: real part (ZHEMV)
+      y1 = fma_z(x, xC[cl], y1);
+      hv[k] = (!diag || lane > cl) ? conj(x) * xr : mkz(0, 0);   // A^H x~ terms (strictly lower)
+    }
+    y1s[w][lane] = y1;
+    // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
+    BUTTERFLY16(hv, lane);
+    // lane holds the sum for column 16w + ((lane >> 2) & 15)
+    if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
   }
-  y1s[w][lane] = y1;
-  // transpose-reduce the 16 column sums over the 64 lanes: halve the set per butterfly step
-  BUTTERFLY16(hv, lane);
-  // lane holds the sum for column 16w + ((lane >> 2) & 15)
-  if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
   // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g.  Wave w takes
   // groups of 16 quantities (q0 = 16 w, 16 w + 64): lane = row.  The first group's operands
   // are loaded here, before the barrier, so that the loads overlap the tile's epilogue.

@@ -354,6 +354,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
   constexpr int XRN = TB;                   // rows of x~ a workgroup needs
   constexpr int XT = XRN + TB;              // threads loading x~ (rows, then columns)
   __shared__ zc xR[XRN], xC[TB];
+  __shared__ zf xCf[TB];                    // x~ of the tile's columns in FP32 (COMPLEX*8 form)
   __shared__ zc y1s[4][XRN], y2s[1][TB];
   __shared__ zc sWl[TB];
   __shared__ zc s_ap;
@@ -434,8 +435,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
   const zc xv = x0 - sig * x1;                       // x_new(xrow) where xok
   if (tid < XT) {
     const zc xt = (xok && xrow >= j + 2) ? xv : mkz(0, 0);
-    if (tid < XRN) xR[tid] = xt;
-    else xC[tid - XRN] = xt;
+    if (tid < XRN) {
+      xR[tid] = xt;
+    } else {
+      xC[tid - XRN] = xt;
+      if constexpr (std::is_same<T, zf>::value) stz(xCf[tid - XRN], xt);
+    }
   }
   // W(r, jj-1) = tau' u(r) + alpha' v(r): written by diagonal tiles, used in their partials
   const zc wo = (jj > 0 && xok) ? taup * uo + ap * x1 : mkz(0, 0);
@@ -445,20 +450,25 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
   zc y1 = mkz(0, 0);
   if (interior) okm = 0xffffu;
   if constexpr (std::is_same<T, zf>::value) {
-    // COMPLEX*8 matrix: the A^H x~ terms and their sums over the tile's 64 rows in FP32
-    // (the rows' A x~ sums and every sum across tiles stay FP64)
+    // COMPLEX*8 matrix: a wave's 16-column A x~ sums and the A^H x~ terms with their sums
+    // over the tile's 64 rows in FP32; every sum across waves and tiles stays FP64
     zf hf[16];
-    zf xrf;
+    zf xrf, y1f;
     xrf.re = (float)xr.re;
     xrf.im = (float)xr.im;
+    y1f.re = 0.0f;
+    y1f.im = 0.0f;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int cl = 16 * w + k;
       const bool on = (okm >> k) & 1u;
-      zc x = on ? ldz(hraw[k]) : mkz(0, 0);
-      if (diag && lane == cl) x.im = 0.0;                 // diagonal: real part (ZHEMV)
-      y1 = fma_z(x, xC[cl], y1);
       const zf h = hraw[k];
+      zf x;
+      x.re = on ? h.re : 0.0f;
+      x.im = (on && !(diag && lane == cl)) ? h.im : 0.0f;   // diagonal: real part (ZHEMV)
+      const zf xc = xCf[cl];
+      y1f.re = fmaf(x.re, xc.re, fmaf(-x.im, xc.im, y1f.re));
+      y1f.im = fmaf(x.re, xc.im, fmaf(x.im, xc.re, y1f.im));
       zf t;
       t.re = h.re * xrf.re + h.im * xrf.im;               // conj(h) x~(row)
       t.im = h.re * xrf.im - h.im * xrf.re;
@@ -466,7 +476,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
       hf[k].re = low ? t.re : 0.0f;
       hf[k].im = low ? t.im : 0.0f;
     }
-    y1s[w][lane] = y1;
+    y1s[w][lane] = ldz(y1f);
     BUTTERFLY16F(hf, lane);
     if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = ldz(hf[0]);
   } else {

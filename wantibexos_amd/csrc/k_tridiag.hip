@@ -87,6 +87,27 @@ __device__ inline zc wave_sum(zc v) {
     }                                                                     \
   } while (0)
 
+// the same for 8 values: lane holds the sum for value (lane >> 3) & 7
+#define BUTTERFLY8(pv, lane)                                              \
+  do {                                                                    \
+    int n_ = 8;                                                           \
+    for (int off_ = 32; off_ >= 8; off_ >>= 1) {                          \
+      n_ >>= 1;                                                           \
+      const bool up_ = ((lane) & off_) != 0;                              \
+      for (int i_ = 0; i_ < n_; ++i_) {                                   \
+        const zc send_ = up_ ? pv[i_] : pv[i_ + n_];                      \
+        zc recv_;                                                         \
+        recv_.re = __shfl_xor(send_.re, off_);                            \
+        recv_.im = __shfl_xor(send_.im, off_);                            \
+        pv[i_] = (up_ ? pv[i_ + n_] : pv[i_]) + recv_;                    \
+      }                                                                   \
+    }                                                                     \
+    for (int off_ = 4; off_ > 0; off_ >>= 1) {                            \
+      pv[0].re += __shfl_xor(pv[0].re, off_);                             \
+      pv[0].im += __shfl_xor(pv[0].im, off_);                             \
+    }                                                                     \
+  } while (0)
+
 // the same on COMPLEX*8 values (FP32 sums over the 64 rows of a tile)
 __device__ inline zf operator+(zf a, zf b) {
   zf c;
@@ -495,8 +516,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
     if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
   }
   // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g.  Wave w takes
-  // groups of 16 quantities (q0 = 16 w, 16 w + 64): lane = row.  The first group's operands
-  // are loaded here, before the barrier, so that the loads overlap the tile's epilogue.
+  // groups of EPQ quantities (q0 = EPQ w, EPQ w + 4 EPQ, ...): lane = row.  The first
+  // group's operands are loaded here, before the barrier, so that the loads overlap the
+  // tile's epilogue.
   const int nq = 1 + 2 * jj + a.nrhs;
   const bool rok = r >= j + 2 && r < N;
   auto operand = [&](int q) -> const zc* {
@@ -506,11 +528,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
     if (q > 2 * jj && q < nq) return a.g + (size_t)(q - 1 - 2 * jj) * N + r;
     return nullptr;
   };
-  zc pv[16];
+  // quantities per wave and pass: 8 for the COMPLEX*8 form, whose registers then fit 4
+  // waves per SIMD, 16 for FP64
+  constexpr int EPQ = std::is_same<T, zf>::value ? 8 : 16;
+  zc pv[EPQ];
   if (diag) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const zc* src = operand(16 * w + k);
+    for (int k = 0; k < EPQ; ++k) {
+      const zc* src = operand(EPQ * w + k);
       pv[k] = (src && rok) ? *src : mkz(0, 0);
     }
   }
@@ -534,25 +559,34 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriAr
   if (gsl) a.gs[tid - XT] = x0;
   const zc wl = rok ? sWl[lane] : mkz(0, 0);
   const double xr2 = norm2(xr);
-  for (int q0 = 16 * w; q0 < nq; q0 += 64) {
-    if (q0 > 16 * w) {
+  for (int q0 = EPQ * w; q0 < nq; q0 += 4 * EPQ) {
+    if (q0 > EPQ * w) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
+      for (int k = 0; k < EPQ; ++k) {
         const zc* src = operand(q0 + k);
         pv[k] = (src && rok) ? *src : mkz(0, 0);
       }
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < EPQ; ++k) {
       const int q = q0 + k;
       const zc val = (q == jj) ? wl : pv[k];
       zc v = (q <= 2 * jj) ? fma_conj(val, xr, mkz(0, 0)) : fma_z(xr, val, mkz(0, 0));
       if (q == 0) v = mkz(xr2, 0.0);
       pv[k] = (q < nq && rok) ? v : mkz(0, 0);
     }
-    BUTTERFLY16(pv, lane);
-    const int q = q0 + ((lane >> 2) & 15);
-    if ((lane & 3) == 0 && q < nq) {
+    int q;
+    bool head;
+    if constexpr (EPQ == 8) {
+      BUTTERFLY8(pv, lane);
+      q = q0 + ((lane >> 3) & 7);
+      head = (lane & 7) == 0;
+    } else {
+      BUTTERFLY16(pv, lane);
+      q = q0 + ((lane >> 2) & 15);
+      head = (lane & 3) == 0;
+    }
+    if (head && q < nq) {
       if (q == 0) a.pn[bi] = pv[0].re;
       else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = pv[0];
       else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = pv[0];

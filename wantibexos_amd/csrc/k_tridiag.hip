@@ -368,10 +368,11 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
 
 // ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
 // 4 waves, wave w owns 16 columns, lane = row: every load moves one column segment.
-// the FP64 instance fits 4 waves per SIMD in 127 registers (no spill), which keeps more
-// tiles in flight; the COMPLEX*8 instance would spill there and stays at its natural 3
+// waves per SIMD that the registers allow without spilling: 4 for the FP64 instance (127
+// registers), 6 for the COMPLEX*8 one (80: FP32 tile sums, 8-quantity epilogue groups);
+// more waves keep more tiles' loads in flight
 template <class T>
-__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 1) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
+__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
   constexpr int XRN = TB;                   // rows of x~ a workgroup needs
   constexpr int XT = XRN + TB;              // threads loading x~ (rows, then columns)
   __shared__ zc xR[XRN], xC[TB];

@@ -104,3 +104,25 @@ def test_bands_iterative_form(ctx, big):
         r = plan.lowest(nev=32, guard=8, tol=1e-6, lanczos_steps=0)
     assert r["info"]["converged"] == 1
     assert np.abs(r["W"] - full[:32]).max() < 1e-4
+
+
+def test_finite_q_plan_errors(ctx, big):
+    """A finite-Q plan gives energies only; non-finite q and eps from Lanczos chains are
+    refused on the host."""
+    from wantibexos_amd import bse
+    m, p, qp = big
+    with bse.BSEPlan(ctx, m, params_of(p), finite_q=True) as plan:
+        with pytest.raises(bse.WTBError):
+            plan.set_q(np.array([0.0, np.nan, 0.0, 0.0]))
+        plan.set_q(qp[1])
+        plan.execute()
+        with pytest.raises(bse.WTBError):
+            plan.results()                     # act / eps / egap requested
+        assert plan.energies().shape == (plan.N,)
+    with bse.BSEPlan(ctx, m, params_of(p, precision=bse.MIXED), finite_q=True) as plan:
+        plan.set_q(qp[1])
+        with pytest.raises(bse.WTBError):
+            plan.lowest(nev=8, guard=4, tol=1e-6, lanczos_steps=20)     # eps at finite Q
+    with bse.BSEPlan(ctx, m, params_of(p)) as plan:
+        with pytest.raises(bse.WTBError):
+            plan.set_q(qp[0])                                            # not a finite-Q plan

@@ -106,3 +106,14 @@ def test_temperature_zero_limit(ctx, model):
     assert np.array_equal(r0["W"], r1["W"])
     assert np.abs(r1["act"] - r0["act"]).max() <= 1e-14 * np.abs(r0["act"]).max()
     assert np.abs(r1["eps"] - r0["eps"]).max() <= 1e-13 * np.abs(r0["eps"]).max()
+
+
+def test_finite_temperature_argument_errors(ctx, model):
+    """The host rejects what the kernels cannot take, before anything is launched."""
+    from wantibexos_amd import bse
+    for bad in (dict(temp=-1.0), dict(temp=float("nan")), dict(st=float("inf"))):
+        with pytest.raises(bse.WTBError):
+            bse.BSEPlan(ctx, model, params(**bad))
+    # TEMP = 0 with PHAVG = 0 makes the VE gap correction 0/0: refused, not propagated
+    with pytest.raises(bse.WTBError):
+        bse.BSEPlan(ctx, model, params(temp=0.0, ta="VE", phavg=0.0))

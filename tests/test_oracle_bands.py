@@ -119,3 +119,17 @@ def test_c_abi_kpathbse_matches_reference(name):
     assert np.array_equal(qc, qo)
     assert np.array_equal(labc, labo)
     assert np.array_equal(qc[:, 0].astype(np.float32), z["qcoord"].astype(np.float32))
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_c_abi_kpathbse_rejects_bad_arguments():
+    """Host-side checks of wtb_kpathbse (no GPU): fewer than two points per segment or
+    fewer than two path ends is an argument error, as the division by npts - 1 and the
+    segment pairs require."""
+    from wantibexos_amd import bse
+    rlat = np.array([[3.2, 0, 0], [-1.6, 2.7713, 0], [0, 0, 20.0]])
+    ks = np.array([[0, 0, 0], [0.5, 0, 0]])
+    with pytest.raises(bse.WTBError):
+        bse.kpathbse(rlat, ks, 1)
+    with pytest.raises(bse.WTBError):
+        bse.kpathbse(rlat, ks[:1], 4)

@@ -332,6 +332,17 @@ int wtb_bse_bands_run(wtb_bse_bands* bands, int nq, const double* qauxv, double*
 /* on = 0: launch the kernels directly instead of replaying the captured graphs
  * (for kernel tracers)                                                            */
 int wtb_bse_bands_set_graph(wtb_bse_bands* bands, int on);
+/* BSE_WF along the path (bse_kpath-tool.f90:359-366, excwfi2 in bse_subs.f90:578-622):
+ * runs also form the vectors of excitons first .. first+count-1 (0-based, ascending)
+ * at every q; run_wf returns them as wf[nq][count][N] complex, each with its last
+ * component real and >= 0.                                                        */
+int wtb_bse_bands_set_vectors(wtb_bse_bands* bands, int first, int count);
+int wtb_bse_bands_run_wf(wtb_bse_bands* bands, int nq, const double* qauxv, double* exk, double* wf);
+/* wtb_bsebnds plus those vectors and, for excwfi2, the k grid kpt(nk,3) and its
+ * nocp(nk) (kpt, nocp may be NULL)                                                */
+int wtb_bsebnds_wf(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, int nq,
+                   const double* qauxv, int lanes, double* exk, int wf_first, int wf_count, double* wf,
+                   double* kpt, int* nocp);
 int wtb_bse_bands_destroy(wtb_bse_bands* bands);
 /* Plan form of one q-point: create, then wtb_bse_plan_set_q(plan, q[4]) (a qauxv
  * row) before build/execute; results gives w only (act, eps, egap NULL);

@@ -136,3 +136,36 @@ def test_dropin_exciton_bands_finite_temperature(tmp_path):
     assert a.shape == b.shape == (64 * 6, 2)
     assert np.array_equal(a[:, 0], b[:, 0])
     assert np.abs(a[:, 1] - b[:, 1]).max() < 1e-9
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+def test_dropin_exciton_bands_wavefunctions(tmp_path):
+    """BSE_BND=T with BSE_WF=T: exc_wf_<n>_<q>.dat written by the reference's own excwfi2
+    from the GPU's vectors at every q; index and k columns as the reference's, amplitudes
+    equal in modulus for non-degenerate excitons (phase conventions differ)."""
+    p = dict(P, ngrid=(4, 4, 1), kpath_bse="kpath.dat", bsewf=True, excwf=(1, 2))
+    model = tbmodel.hex2d(48, 6, p["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    (tmp_path / "kpath.dat").write_text("2\n3\n0.0 0.0 0.0\n0.6667 -0.3333 0.0\n")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    exk = table(tmp_path / "ref" / "bands_bse.dat", skip=0)[:, 1].reshape(64, 3)
+    checked = 0
+    for q in (1, 2, 3):
+        for n in (1, 2):
+            fa, fb = tmp_path / "ref" / f"exc_wf_{n}_{q}.dat", tmp_path / "gpu" / f"exc_wf_{n}_{q}.dat"
+            assert fa.exists(), rr.stdout + rr.stderr
+            assert fb.exists(), rg.stdout + rg.stderr
+            a, b = table(fa, skip=8), table(fb, skip=8)
+            assert a.shape == b.shape == (64, 8)
+            assert np.array_equal(a[:, [0, 4, 5]], b[:, [0, 4, 5]])
+            assert np.abs(a[:, 1:4] - b[:, 1:4]).max() <= 1e-4
+            w = exk[:, q - 1]
+            gap = min(abs(w[n - 1] - w[k]) for k in range(64) if k != n - 1)
+            if gap > 1e-3:
+                ma = np.hypot(a[:, 6], a[:, 7])
+                mb = np.hypot(b[:, 6], b[:, 7])
+                assert np.abs(ma - mb).max() <= 2e-4 * max(ma.max(), 1e-30) + 1e-8, (n, q)
+                checked += 1
+    assert checked >= 3

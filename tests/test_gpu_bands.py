@@ -126,3 +126,29 @@ def test_finite_q_plan_errors(ctx, big):
     with bse.BSEPlan(ctx, m, params_of(p)) as plan:
         with pytest.raises(bse.WTBError):
             plan.set_q(qp[0])                                            # not a finite-Q plan
+
+
+def test_bands_exciton_vectors(ctx, big):
+    """BSE_WF along the path: the vectors run_wf returns are orthonormal eigenvectors of
+    each q's finite-Q matrix (the oracle's, residual 1e-9), with the last component real
+    and >= 0, and equal in modulus to the oracle's eigenvectors where the exciton is
+    non-degenerate."""
+    from wantibexos_amd import bse
+    m, p, qp = big
+    sel = qp[[0, 2, 5]]
+    with bse.BSEBands(ctx, m, params_of(p), lanes=2) as b:
+        b.set_vectors(0, 4)
+        exk, wf = b.run_wf(sel)
+    o = O.run_bands(m, p["ngrid"], sel, **okw(p))
+    assert np.abs(exk - o["exk"]).max() < 1e-9
+    for iq in range(len(sel)):
+        H = o["H"][iq]
+        X = wf[iq]                                       # (N, 4)
+        assert np.abs(X.conj().T @ X - np.eye(4)).max() < 1e-10
+        assert np.abs(H @ X - X * exk[:4, iq]).max() < 1e-9
+        assert (X[-1].real >= 0).all() and np.abs(X[-1].imag).max() == 0.0
+        lam, V = np.linalg.eigh(H)
+        for n in range(4):
+            gap = min(abs(lam[n] - lam[k]) for k in range(len(lam)) if k != n)
+            if gap > 1e-6:
+                assert np.abs(np.abs(X[:, n]) - np.abs(V[:, n])).max() < 1e-8, (iq, n)

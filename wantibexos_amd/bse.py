@@ -141,6 +141,9 @@ def _load():
         "wtb_bse_bands_run": (ip, [vp, ip, vp, vp]),
         "wtb_bse_bands_destroy": (ip, [vp]),
         "wtb_bse_bands_set_graph": (ip, [vp, ip]),
+        "wtb_bse_bands_set_vectors": (ip, [vp, ip, ip]),
+        "wtb_bse_bands_run_wf": (ip, [vp, ip, vp, vp, vp]),
+        "wtb_bsebnds_wf": (ip, [vp, vp, vp, ip, vp, ip, vp, ip, ip, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -618,11 +621,25 @@ class BSEBands:
         """False: launch the kernels directly instead of replaying the captured graphs."""
         _check(lib().wtb_bse_bands_set_graph(self.h, int(on)))
 
+    def set_vectors(self, first, count):
+        """Also form the vectors of excitons first .. first+count-1 at every q (run_wf)."""
+        _check(lib().wtb_bse_bands_set_vectors(self.h, int(first), int(count)))
+        self.wf_count = int(count)
+
     def run(self, qpath):
         qf = np.asfortranarray(np.asarray(qpath, dtype=np.float64))
         exk = np.zeros((self.params.dim, qf.shape[0]), order="F")
         _check(lib().wtb_bse_bands_run(self.h, qf.shape[0], qf.ctypes.data, exk.ctypes.data))
         return np.ascontiguousarray(exk)
+
+    def run_wf(self, qpath):
+        """(exk (N, nq), wf (nq, N, count)): the energies and the exciton vectors."""
+        qf = np.asfortranarray(np.asarray(qpath, dtype=np.float64))
+        nq, N = qf.shape[0], self.params.dim
+        exk = np.zeros((N, nq), order="F")
+        wf = np.zeros((nq, self.wf_count, N), dtype=np.complex128)
+        _check(lib().wtb_bse_bands_run_wf(self.h, nq, qf.ctypes.data, exk.ctypes.data, wf.ctypes.data))
+        return np.ascontiguousarray(exk), np.ascontiguousarray(wf.transpose(0, 2, 1))
 
     def close(self):
         if self.h:

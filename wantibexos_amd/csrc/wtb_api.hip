@@ -580,7 +580,17 @@ static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
     return rc;
   }
   mark(4);
-  if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, nullptr, p->lambda))) return rc;
+  if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, nullptr, p->lambda,
+                   p->wf_count > 0 ? p->Zv : nullptr)))
+    return rc;
+  // BSE_WF along the path (bse_kpath-tool.f90:359-366, 402-410): the exciton vectors
+  if (p->wf_count > 0) {
+    rc = p->mixed ? run_backtr_f32(s, p->Hf, (int)p->ldh, p->N, p->tw.tau, p->Zv, p->wf_first, p->wf_count, p->Xv,
+                                   p->N, p->btw)
+                  : run_backtr(s, p->H, (int)p->ldh, p->N, p->tw.tau, p->Zv, p->wf_first, p->wf_count, p->Xv, p->N,
+                               p->btw);
+    if (rc) return rc;
+  }
   mark(5);
   mark(6);
   return WTB_OK;
@@ -1645,6 +1655,9 @@ struct wtb_bse_bands {
   double* hw = nullptr;
   int* hst = nullptr;
   int cap = 0;                          // q-points the staging holds
+  int wf_count = 0;                     // exciton vectors per q (wtb_bse_bands_set_vectors)
+  zc* hx = nullptr;                     // pinned staging of the vectors
+  size_t hx_cap = 0;
 };
 
 int wtb_bse_bands_destroy(wtb_bse_bands* b) {
@@ -1659,7 +1672,20 @@ int wtb_bse_bands_destroy(wtb_bse_bands* b) {
   if (b->hq) hipHostFree(b->hq);
   if (b->hw) hipHostFree(b->hw);
   if (b->hst) hipHostFree(b->hst);
+  if (b->hx) hipHostFree(b->hx);
   delete b;
+  return WTB_OK;
+}
+
+int wtb_bse_bands_set_vectors(wtb_bse_bands* b, int first, int count) {
+  if (!b) return WTB_ERR_ARG;
+  for (wtb_bse_plan* p : b->plans) {
+    hipStream_t st = p->stream;
+    int rc = wtb_bse_plan_set_vectors(p, first, count);
+    p->stream = st;
+    if (rc) return rc;
+  }
+  b->wf_count = count;
   return WTB_OK;
 }
 
@@ -1703,8 +1729,12 @@ int wtb_bse_bands_set_graph(wtb_bse_bands* b, int on) {
 }
 
 int wtb_bse_bands_run(wtb_bse_bands* b, int nq, const double* qauxv, double* exk) {
-  if (!b || nq < 1 || !qauxv || !exk) {
-    set_error("wtb_bse_bands_run: null argument or nq < 1");
+  return wtb_bse_bands_run_wf(b, nq, qauxv, exk, nullptr);
+}
+
+int wtb_bse_bands_run_wf(wtb_bse_bands* b, int nq, const double* qauxv, double* exk, double* wf) {
+  if (!b || nq < 1 || !qauxv || !exk || (wf && b->wf_count < 1)) {
+    set_error("wtb_bse_bands_run: null argument, nq < 1, or vectors asked without wtb_bse_bands_set_vectors");
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(b->ctx->device));
@@ -1724,6 +1754,17 @@ int wtb_bse_bands_run(wtb_bse_bands* b, int nq, const double* qauxv, double* exk
     }
     b->cap = nq;
   }
+  const size_t per_q = (size_t)b->N * b->wf_count;      // complex entries of one q's vectors
+  if (wf && per_q * nq > b->hx_cap) {
+    if (b->hx) hipHostFree(b->hx);
+    b->hx = nullptr;
+    b->hx_cap = 0;
+    if (hipHostMalloc((void**)&b->hx, sizeof(zc) * per_q * nq) != hipSuccess) {
+      set_error("wtb_bse_bands_run: pinned host allocation failed");
+      return WTB_ERR_NOMEM;
+    }
+    b->hx_cap = per_q * nq;
+  }
   for (int iq = 0; iq < nq; ++iq)
     for (int k = 0; k < 4; ++k) b->hq[4 * iq + k] = qauxv[iq + (size_t)nq * k];
   // the q-points round-robin over the lanes; each lane's plan runs its points in order
@@ -1739,7 +1780,9 @@ int wtb_bse_bands_run(wtb_bse_bands* b, int nq, const double* qauxv, double* exk
     if ((rc = wtb_bse_plan_execute(p))) break;
     if (hipMemcpyAsync(b->hw + (size_t)N * iq, p->lambda, sizeof(double) * N, hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
-        hipMemcpyAsync(b->hst + 2 * iq, p->status, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess) {
+        hipMemcpyAsync(b->hst + 2 * iq, p->status, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (wf && hipMemcpyAsync(b->hx + per_q * iq, p->Xv, sizeof(zc) * per_q, hipMemcpyDeviceToHost, st) !=
+                   hipSuccess)) {
       set_error("wtb_bse_bands_run: result copy failed");
       rc = WTB_ERR_HIP;
     }
@@ -1757,6 +1800,24 @@ int wtb_bse_bands_run(wtb_bse_bands* b, int nq, const double* qauxv, double* exk
     }
   }
   if (!rc) std::memcpy(exk, b->hw, sizeof(double) * (size_t)N * nq);
+  if (!rc && wf) std::memcpy(wf, b->hx, sizeof(zc) * per_q * nq);
+  return rc;
+}
+
+int wtb_bsebnds_wf(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, int nq, const double* qauxv,
+                   int lanes, double* exk, int wf_first, int wf_count, double* wf, double* kpt, int* nocp) {
+  if (!c || !m || !prm || nq < 1 || !qauxv || !exk || wf_count < 0 || (wf_count > 0 && !wf)) {
+    set_error("wtb_bsebnds_wf: null argument, nq < 1 or bad vector range");
+    return WTB_ERR_ARG;
+  }
+  wtb_bse_bands* b = nullptr;
+  int rc = wtb_bse_bands_create(c, m, prm, std::max(1, std::min({lanes, nq, 8})), &b);
+  if (!rc && wf_count > 0) rc = wtb_bse_bands_set_vectors(b, wf_first, wf_count);
+  if (!rc) rc = wtb_bse_bands_run_wf(b, nq, qauxv, exk, wf_count > 0 ? wf : nullptr);
+  // the k grid and its nocp (the same at every q), as excwfi2 prints them
+  if (!rc && kpt) rc = wtb_bse_plan_fetch(b->plans[0], WTB_BUF_KPT, kpt, sizeof(double) * 3 * b->plans[0]->nk);
+  if (!rc && nocp) rc = wtb_bse_plan_fetch(b->plans[0], WTB_BUF_NOCP, nocp, sizeof(int) * b->plans[0]->nk);
+  wtb_bse_bands_destroy(b);
   return rc;
 }
 

@@ -281,7 +281,8 @@ subroutine bsebnds(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 
 	zero = 0d0
 	call wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
-			      kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,.false.,zero,zero,"FA",zero)
+			      kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff, &
+			      .false.,zero,zero,"FA",zero)
 end subroutine bsebnds
 
 ! bsebndstemp (subprograms/bse_kpath-tool-temp.f90): the same at finite temperature.
@@ -304,12 +305,13 @@ subroutine bsebndstemp(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	character(len=2) :: ta
 
 	call wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
-			      kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,.true.,st,phavg,ta,temp)
+			      kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff, &
+			      .true.,st,phavg,ta,temp)
 	if (temp .eq. 0d0) temp = 1E-8
 end subroutine bsebndstemp
 
 subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
-			    kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,bset,st,phavg,ta,temp)
+			    kpathsbse,ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,bset,st,phavg,ta,temp)
 	use iso_c_binding
 	use hamiltonian_input_variables
 	use wtb_gpu
@@ -322,6 +324,7 @@ subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse
 	character(len=70) :: params, kpathsbse, outputfolder
 	character(len=5) :: coultype
 	logical :: bsewf, bset
+	integer :: excwf0, excwff
 	double precision :: ez, w1, lc, r0
 	double precision :: st, phavg, temp
 	character(len=2) :: ta
@@ -332,11 +335,11 @@ subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse
 	double precision, allocatable, target :: t_hop(:,:,:), t_ihop(:,:,:)
 	integer, allocatable, target :: t_rvec(:,:), t_ff(:)
 	double precision, target :: t_rlat(3,3)
-	double precision, allocatable :: ks(:,:), qauxv(:,:), exk(:,:)
-	integer :: nks, nkpts, nq, dimbse, i, i2, erro
+	double precision, allocatable :: ks(:,:), qauxv(:,:), exk(:,:), kpt(:,:)
+	double complex, allocatable :: wf(:,:,:)
+	integer, allocatable :: nocpk(:), stt(:,:)
+	integer :: nks, nkpts, nq, dimbse, i, i2, erro, nwf, ngkpt
 	integer(c_int) :: rc
-
-	if (bsewf) stop "BSE_WF with BSE_BND: the exciton wave functions along the path are not available on the GPU"
 
 	OPEN(UNIT=500, FILE= kpathsbse,STATUS='old', IOSTAT=erro)
 	if (erro/=0) stop "Error opening bse-kpath input file"
@@ -414,9 +417,30 @@ subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse
 
 	rc = wtb_create(0_c_int, ctx)
 	call wtb_check(rc)
-	rc = wtb_bsebnds(ctx, m, p, int(nq,c_int), qauxv, 4_c_int, exk)
+	if (bsewf) then
+		! BSE_WF: the vectors of excitons excwf0 .. excwff at every q, and what excwfi2
+		! prints beside them (bse_kpath-tool.f90:402-410)
+		if (excwf0 < 1 .or. excwff > dimbse .or. excwff < excwf0) stop "Wrong BSE_WF exciton range"
+		nwf = excwff - excwf0 + 1
+		ngkpt = ngrid(1)*ngrid(2)*ngrid(3)
+		allocate(wf(dimbse,nwf,nq), kpt(ngkpt,3), nocpk(ngkpt), stt(dimbse,4))
+		rc = wtb_bsebnds_wf(ctx, m, p, int(nq,c_int), qauxv, 4_c_int, exk, int(excwf0-1,c_int), &
+				    int(nwf,c_int), wf, kpt, nocpk)
+	else
+		rc = wtb_bsebnds(ctx, m, p, int(nq,c_int), qauxv, 4_c_int, exk)
+	end if
 	call wtb_check(rc)
 	rc = wtb_destroy(ctx)
+	if (bsewf) then
+		call quantumnumbers2(w90basis,ngkpt,nc,nv,nocpk,nocpk,stt)
+		do i=1,nq
+			do i2=excwf0,excwff
+				call excwfi2(outputfolder,ngkpt,kpt,qauxv(i,:),nc,nv,nocpk,stt,exk(i2,i),i2,i, &
+					     wf(:,i2-excwf0+1,i))
+			end do
+		end do
+		deallocate(wf, kpt, nocpk, stt)
+	end if
 
 	do i2=1,dimbse
 		do i=1,nq

@@ -84,6 +84,18 @@ module wtb_gpu
 			integer(c_int), value :: nq, lanes
 			real(c_double) :: qauxv(*), exk(*)
 		end function
+		integer(c_int) function wtb_bsebnds_wf(ctx, model, prm, nq, qauxv, lanes, exk, wf_first, &
+				wf_count, wf, kpt, nocp) bind(C, name="wtb_bsebnds_wf")
+			import :: c_int, c_ptr, c_double, c_double_complex, wtb_tb_model, wtb_bse_params
+			type(c_ptr), value :: ctx
+			type(wtb_tb_model) :: model
+			type(wtb_bse_params) :: prm
+			integer(c_int), value :: nq, lanes, wf_first, wf_count
+			real(c_double) :: qauxv(*), exk(*)
+			complex(c_double_complex) :: wf(*)
+			real(c_double) :: kpt(*)
+			integer(c_int) :: nocp(*)
+		end function
 		integer(c_int) function wtb_bse_spectrum(ctx, n, e, act6, rlat, ngrid, ebse0, ebsef, &
 				numbse, sme, renorm, eps) bind(C, name="wtb_bse_spectrum")
 			import :: c_int, c_ptr, c_double

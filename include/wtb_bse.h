@@ -351,6 +351,14 @@ int wtb_bse_qplan_create(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_
                          wtb_bse_plan** plan);
 int wtb_bse_plan_set_q(wtb_bse_plan* plan, const double* q);
 
+/* The single-particle table bsesolver writes as sp_optics.dat, sp_optics-pol.dat and
+ * (TM_COEF) tm_coef*.dat (bse_solver-tool-diel.f90:336-420, with the finite-temperature
+ * factor of bse_solver-tool-diel-temp.f90:405-424 when prm->bset): vecres(N, 15),
+ * REAL, rows sorted by transition energy (column 7) as Bubblem sorts them
+ * (general_subs.f90:405-440).  The plan form needs a built or executed plan.       */
+int wtb_bse_plan_sp_table(wtb_bse_plan* plan, float* vecres);
+int wtb_sp_table(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, float* vecres);
+
 /* BSE_WF (bse_solver-tool-diel.f90:513-519, excwf2 in bse_subs.f90:534-576):
  * executes also form the eigenvectors of excitons first .. first+count-1
  * (0-based, ascending energy), fetched as WTB_BUF_EXCWF, each with its last

@@ -469,6 +469,26 @@ def activities_temp(X, hvec, fdeh):
     return np.stack([cr(0, 0), cr(1, 1), cr(2, 2), cr(0, 1), cr(0, 2), cr(1, 2), cr(3, 3), cr(4, 4)])
 
 
+def sp_table(kpt, nocp, eigv, h, stt, nv, fdeh=None):
+    """The single-particle table vecres(N, 15) of bse_solver-tool-diel.f90:365-402 (REAL):
+    k, nocp, band numbers, Ec - Ev, |h_a|^2 and Re(h_a conj h_b), |h_sp|^2, |h_sm|^2 (at
+    finite temperature times f_v - f_c, bse_solver-tool-diel-temp.f90:405-424), rows in
+    the stable ascending order of Bubblem on column 7 (general_subs.f90:405-440)."""
+    ki = stt[:, 3] - 1
+    f32 = np.float32
+    cols = [kpt[ki, 0], kpt[ki, 1], kpt[ki, 2], nocp[ki], nocp[ki] - nv + stt[:, 2], nocp[ki] - nv + stt[:, 1],
+            eigv[ki, stt[:, 2] - 1] - eigv[ki, stt[:, 1] - 1]]
+    t = [c.astype(np.float64).astype(f32) for c in cols]
+    hx, hy, hz, hp, hm = h
+    q = [(hx * np.conj(hx)).real, (hy * np.conj(hy)).real, (hz * np.conj(hz)).real, (hx * np.conj(hy)).real,
+         (hx * np.conj(hz)).real, (hy * np.conj(hz)).real, (hp * np.conj(hp)).real, (hm * np.conj(hm)).real]
+    for v in q:
+        r = v.astype(f32)
+        t.append(r if fdeh is None else (r.astype(np.float64) * fdeh).astype(f32))
+    tab = np.stack(t, axis=1)
+    return tab[np.argsort(tab[:, 6], kind="stable")]
+
+
 def f15_4(x):
     """Value after a write "(F15.4)" and list-directed read back
     (bse_solver-tool-diel.f90:610, bse_diel-tool.f90:117-123)."""

@@ -40,10 +40,11 @@ def table(path, skip=1):
 
 
 FINITE_T = dict(P, bset=True, temp=3000.0, ta="VE", st=0.3, phavg=0.05)
+TMCOEF = dict(P, tmcoef=True)
 
 
 @pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
-@pytest.mark.parametrize("p", [P, FINITE_T], ids=["bsesolver", "bsesolvertemp"])
+@pytest.mark.parametrize("p", [TMCOEF, FINITE_T], ids=["bsesolver", "bsesolvertemp"])
 def test_dropin_outputs_match_reference(tmp_path, p):
     model = tbmodel.hex2d(12, 8, P["ngrid"])
     tbmodel.write_tb(model, tmp_path / "tb.dat")
@@ -57,6 +58,20 @@ def test_dropin_outputs_match_reference(tmp_path, p):
     # F15.4 text: identical except where a value sits on a rounding boundary
     assert np.abs(a - b).max() <= 1.00001e-4
     assert np.mean(a != b) < 0.01
+    # the single-particle side output, sorted by transition energy: same rows (F15.4)
+    for sp in ("sp_optics.dat", "sp_optics-pol.dat"):
+        sa, sb = table(tmp_path / "ref" / sp), table(tmp_path / "gpu" / sp)
+        assert sa.shape == sb.shape == (144, 7 if sp == "sp_optics.dat" else 6)
+        assert np.abs(sa[:, 0] - sb[:, 0]).max() <= 1.00001e-4
+        ka, kb = np.lexsort(sa.T[::-1]), np.lexsort(sb.T[::-1])
+        assert np.abs(sa[ka] - sb[kb]).max() <= 1.00001e-4
+    if p.get("tmcoef"):                  # TMCOEF= T: k, band numbers and the same columns
+        for tm, ncol in (("tm_coef.dat", 13), ("tm_coef-pol.dat", 12)):
+            ta, tb = table(tmp_path / "ref" / tm, skip=4), table(tmp_path / "gpu" / tm, skip=4)
+            assert ta.shape == tb.shape == (144, ncol)
+            ka, kb = np.lexsort(ta.T[::-1]), np.lexsort(tb.T[::-1])
+            assert np.array_equal(ta[ka][:, 3:6], tb[kb][:, 3:6])
+            assert np.abs(ta[ka] - tb[kb]).max() <= 1.00001e-4
     files = [f"bse_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]
     files += [f"bse_diel-pol_{c}.dat" for c in ["x", "y", "z", "sp", "sm"]]      # bsedielrawpol
     for comp in files:

@@ -210,3 +210,24 @@ def test_plan_reexecute_is_reproducible(ctx):
         assert np.array_equal(r1["W"], r["W"])
         assert np.array_equal(r1["act"], r["act"])
         assert np.array_equal(r1["eps"], r["eps"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_single_particle_table(ctx, name):
+    """The single-particle side output (sp_optics.dat / tm_coef.dat content): the GPU's
+    float32 table against the oracle's from the same fixture, row for row once both are
+    keyed by (k, bands); the GPU's rows in ascending transition energy, stable."""
+    from wantibexos_amd import bse
+    m, p, z = load(name)
+    with bse.BSEPlan(ctx, m, params_of(p)) as plan:
+        plan.build()
+        t = plan.sp_table()
+    o = O.run_pipeline(m, p["ngrid"], **oracle_kwargs(p))
+    ref = O.sp_table(o["kpt"], o["nocp"], o["eigv"], o["h"], o["stt"], p["nv"], o["fdeh"])
+    assert t.shape == ref.shape == (len(z["stt"]), 15) and t.dtype == np.float32
+    assert np.all(np.diff(t[:, 6]) >= 0)
+    key = lambda a: np.lexsort((a[:, 5], a[:, 4], a[:, 2], a[:, 1], a[:, 0]))
+    a, b = t[key(t)].astype(np.float64), ref[key(ref)].astype(np.float64)
+    assert np.array_equal(a[:, :6], b[:, :6])
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-30)
+    assert (np.abs(a - b) / scale).max() < 1e-6

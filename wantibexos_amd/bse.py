@@ -141,6 +141,8 @@ def _load():
         "wtb_bse_bands_run": (ip, [vp, ip, vp, vp]),
         "wtb_bse_bands_destroy": (ip, [vp]),
         "wtb_bse_bands_set_graph": (ip, [vp, ip]),
+        "wtb_bse_plan_sp_table": (ip, [vp, vp]),
+        "wtb_sp_table": (ip, [vp, vp, vp, vp]),
         "wtb_bse_bands_set_vectors": (ip, [vp, ip, ip]),
         "wtb_bse_bands_run_wf": (ip, [vp, ip, vp, vp, vp]),
         "wtb_bsebnds_wf": (ip, [vp, vp, vp, ip, vp, ip, vp, ip, ip, vp, vp, vp]),
@@ -345,6 +347,13 @@ class BSEPlan:
         _check(create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), C.byref(h)))
         self.h = h
         self.N = lib().wtb_bse_plan_dim(h)
+
+    def sp_table(self):
+        """The single-particle table vecres (N, 15), float32, sorted by transition energy
+        (the sp_optics.dat / tm_coef.dat content); needs build() or execute() first."""
+        t = np.zeros((15, self.N), dtype=np.float32)
+        _check(lib().wtb_bse_plan_sp_table(self.h, _p(t)))
+        return t.T.copy()
 
     def set_q(self, q):
         """A row of the q path (coordinate, qx, qy, qz) for the next build/execute."""

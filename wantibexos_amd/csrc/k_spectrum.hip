@@ -261,3 +261,66 @@ int launch_xth(hipStream_t s, int n, const zc* X, const zc* h1, const zc* h2, do
 }
 
 }  // namespace wtb
+
+namespace wtb {
+
+// The single-particle table the reference writes beside the BSE results
+// (bse_solver-tool-diel.f90:365-420; bse_solver-tool-diel-temp.f90:405-424): per basis
+// state I = (k nc + c) nv + v, in REAL (float32) as vecres(I, 1..15): k (3), nocp, the
+// conduction and valence band numbers, Ec - Ev, |h_x|^2, |h_y|^2, |h_z|^2,
+// Re(h_x conj h_y), Re(h_x conj h_z), Re(h_y conj h_z), |h_sp|^2, |h_sm|^2; at finite
+// temperature each of the last eight times f_v - f_c (REAL value times the double factor).
+__global__ void k_sp_table(int N, int nk, int nc, int nv, const double* __restrict__ kpt,
+                           const int* __restrict__ nocp, const double* __restrict__ eigv,
+                           const zc* __restrict__ dip, const double* __restrict__ fdeh, float* __restrict__ tab) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= N) return;
+  const int ncv = nc * nv, nb = nc + nv;
+  const int k = I / ncv, c = (I % ncv) / nv, v = I % nv;
+  const int no = nocp[k];
+  const zc hx = dip[I], hy = dip[(size_t)N + I], hz = dip[2 * (size_t)N + I];
+  // hrsp = (hrx + i hry) * REAL(1./sqrt(2.)), hrsm = (hrx - i hry) * ... (:376-377)
+  const double s2 = (double)(1.0f / sqrtf(2.0f));
+  const zc hp = mkz(hx.re - hy.im, hx.im + hy.re) * s2;
+  const zc hm = mkz(hx.re + hy.im, hx.im - hy.re) * s2;
+  auto re_ab = [](zc a, zc b) { return (a * conj(b)).re; };   // real(a*conjg(b))
+  const double q[8] = {re_ab(hx, hx), re_ab(hy, hy), re_ab(hz, hz), re_ab(hx, hy),
+                       re_ab(hx, hz), re_ab(hy, hz), re_ab(hp, hp), re_ab(hm, hm)};
+  tab[I] = (float)kpt[k];
+  tab[(size_t)N + I] = (float)kpt[(size_t)nk + k];
+  tab[2 * (size_t)N + I] = (float)kpt[2 * (size_t)nk + k];
+  tab[3 * (size_t)N + I] = (float)no;
+  tab[4 * (size_t)N + I] = (float)(no + c + 1);
+  tab[5 * (size_t)N + I] = (float)(no - nv + v + 1);
+  tab[6 * (size_t)N + I] = (float)(eigv[(size_t)k * nb + nv + c] - eigv[(size_t)k * nb + v]);
+  for (int j = 0; j < 8; ++j) {
+    const float r = (float)q[j];
+    tab[(size_t)(7 + j) * N + I] = fdeh ? (float)((double)r * fdeh[I]) : r;
+  }
+}
+
+// Bubblem (general_subs.f90:405-440) on column 7: a stable ascending sort, here by rank
+// (the states of lower energy, then those of equal energy and lower index)
+__global__ void k_sp_sort(int N, const float* __restrict__ tab, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float* key = tab + 6 * (size_t)N;
+  const float ki = key[i];
+  int rank = 0;
+  for (int j = 0; j < N; ++j) {
+    const float kj = key[j];
+    rank += (kj < ki) || (kj == ki && j < i);
+  }
+  for (int c = 0; c < 15; ++c) out[(size_t)c * N + rank] = tab[(size_t)c * N + i];
+}
+
+int launch_sp_table(hipStream_t s, int N, int nk, int nc, int nv, const double* kpt, const int* nocp,
+                    const double* eigv, const zc* dip, const double* fdeh, float* work, float* out) {
+  hipLaunchKernelGGL(k_sp_table, dim3((N + 255) / 256), dim3(256), 0, s, N, nk, nc, nv, kpt, nocp, eigv, dip, fdeh,
+                     work);
+  hipLaunchKernelGGL(k_sp_sort, dim3((N + 255) / 256), dim3(256), 0, s, N, work, out);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+}  // namespace wtb

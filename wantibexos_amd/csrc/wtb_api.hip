@@ -1834,4 +1834,46 @@ int wtb_bsebnds(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, in
   return rc;
 }
 
+
+// ---------------------------------------------------------------------------
+// single-particle side output (sp_optics.dat, tm_coef.dat)
+// ---------------------------------------------------------------------------
+int wtb_bse_plan_sp_table(wtb_bse_plan* p, float* vecres) {
+  if (!p || !vecres || p->qmode) {
+    set_error("wtb_bse_plan_sp_table: needs a Q = 0 plan and vecres(N, 15)");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  WTB_HIP(hipStreamSynchronize(p->stream));
+  const int N = p->N;
+  float* d = nullptr;
+  int rc = dmalloc(&d, (size_t)30 * N);
+  if (rc) return rc;
+  rc = launch_sp_table(s, N, p->nk, p->prm.nc, p->prm.nv, p->kpt, p->nocp, p->eigv, p->dip,
+                       p->prm.bset ? p->fdeh : nullptr, d, d + (size_t)15 * N);
+  if (!rc && hipMemcpyAsync(vecres, d + (size_t)15 * N, sizeof(float) * 15 * N, hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = WTB_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = WTB_ERR_HIP;
+  hipFree(d);
+  if (rc == WTB_ERR_HIP) set_error("wtb_bse_plan_sp_table: HIP error");
+  return rc;
+}
+
+int wtb_sp_table(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, float* vecres) {
+  wtb_bse_plan* p = nullptr;
+  int rc = wtb_bse_plan_create(c, m, prm, &p);
+  if (rc) return rc;
+  rc = wtb_bse_plan_build(p);
+  if (!rc) {
+    // the band-window checks of the stages the table comes from
+    int st = 0;
+    if (hipMemcpy(&st, p->status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) rc = WTB_ERR_HIP;
+    else if ((rc = status_to_code(st))) set_error(status_text(rc));
+  }
+  if (!rc) rc = wtb_bse_plan_sp_table(p, vecres);
+  wtb_bse_plan_destroy(p);
+  return rc;
+}
+
 }  // extern "C"

@@ -140,6 +140,9 @@ int launch_activities(hipStream_t s, int N, const double* d_G /* 6 x N */, int d
 int launch_activities_temp(hipStream_t s, int N, const double* d_G /* 12 x N */, int dtfull, int cpol,
                            double* d_act /* 8 x N */);
 int launch_quantize(hipStream_t s, int n, const double* src, double* dst);
+// the single-particle table vecres(N, 15) (REAL), sorted by transition energy; work: 15 N floats
+int launch_sp_table(hipStream_t s, int N, int nk, int nc, int nv, const double* kpt, const int* nocp,
+                    const double* eigv, const zc* dip, const double* fdeh, float* work, float* out);
 int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act6, double vc,
                     double nkf32, double ebse0, double ebsef, int numbse, double sme, int renorm,
                     double* d_work, double* d_eps);

@@ -31,7 +31,7 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 
 	zero = 0d0
 	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
-			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol, &
+			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol,tmcoef, &
 			    .false.,zero,zero,"FA",zero)
 end subroutine bsesolver
 
@@ -56,7 +56,7 @@ subroutine bsesolvertemp(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	character(len=2) :: ta
 
 	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
-			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol, &
+			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol,tmcoef, &
 			    .true.,st,phavg,ta,temp)
 	if (temp .eq. 0d0) temp = 1E-8
 end subroutine bsesolvertemp
@@ -64,7 +64,7 @@ end subroutine bsesolvertemp
 ! The body of both: model and parameters into the C structures, one wtb_bsesolver(_wf)
 ! call, the reference's output files.
 subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
-			  ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol, &
+			  ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol,tmcoef, &
 			  bset,st,phavg,ta,temp)
 	use iso_c_binding
 	use hamiltonian_input_variables
@@ -77,13 +77,14 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 	double precision,dimension(3) :: mshift, ediel
 	character(len=70) :: params, outputfolder
 	character(len=5) :: coultype
-	logical :: bsewf, cpol, dtfull, bset
+	logical :: bsewf, cpol, dtfull, bset, tmcoef
 	integer :: excwf0, excwff
 	double precision :: ez, w1, lc, r0
 	double precision :: st, phavg, temp
 	character(len=2) :: ta
 
 	type(c_ptr) :: ctx
+	real, allocatable :: vecres(:,:)
 	type(wtb_tb_model) :: m
 	type(wtb_bse_params) :: p
 	double precision, allocatable, target :: t_hop(:,:,:), t_ihop(:,:,:)
@@ -161,7 +162,52 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 		rc = wtb_bsesolver(ctx, m, p, w, act, c_null_ptr, egap)
 	end if
 	call wtb_check(rc)
+	! the single-particle side output (bse_solver-tool-diel.f90:336-420)
+	allocate(vecres(dimbse,15))
+	rc = wtb_sp_table(ctx, m, p, vecres)
+	call wtb_check(rc)
 	rc = wtb_destroy(ctx)
+	OPEN(UNIT=401, FILE=trim(outputfolder)//"sp_optics.dat",STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening bse_opt_diel-pol output file"
+	OPEN(UNIT=403, FILE=trim(outputfolder)//"sp_optics-pol.dat",STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening sp_optics-pol output file"
+	write(401,*) "#","  ", "energy","  ","xx","  ","yy","  ","zz","  ","xy","  ","xz","  ","yz"
+	write(403,*) "#","  ", "energy","  ","xx","  ","yy","  ","zz","  ","sp","  ","sm"
+	if (tmcoef) then
+		OPEN(UNIT=402, FILE=trim(outputfolder)//"tm_coef.dat",STATUS='unknown', IOSTAT=erro)
+		if (erro/=0) stop "Error opening tm_coef output file"
+		OPEN(UNIT=404, FILE=trim(outputfolder)//"tm_coef-pol.dat",STATUS='unknown', IOSTAT=erro)
+		if (erro/=0) stop "Error opening tm_coef-pol output file"
+		write(402,*) "number of kpoints:",ngrid(1)*ngrid(2)*ngrid(3)
+		write(402,*) "number of conduction states",nc
+		write(402,*) "number of valence states",nv
+		write(402,*) "#"," ", "kx", " ", "ky"," ", "kz"," ","nocp"," ", "nc"," ", "nv","  ", "energy","  ","xx","  ",&
+			      "yy","  ","zz","  ","xy","  ","xz","  ","yz"
+		write(404,*) "number of kpoints:",ngrid(1)*ngrid(2)*ngrid(3)
+		write(404,*) "number of conduction states",nc
+		write(404,*) "number of valence states",nv
+		write(404,*) "#"," ", "kx", " ", "ky"," ", "kz"," ","nocp"," ", "nc"," ", "nv","  ", "energy","  ","xx","  ",&
+			      "yy","  ","zz","  ","sp","  ","sm"
+	end if
+	do i=1,dimbse
+		write(401,"(7F15.4)") vecres(i,7),vecres(i,8),vecres(i,9),vecres(i,10),vecres(i,11),vecres(i,12),vecres(i,13)
+		write(403,"(6F15.4)") vecres(i,7),vecres(i,8),vecres(i,9),vecres(i,10),vecres(i,14),vecres(i,15)
+		if (tmcoef) then
+			write(402,"(3F10.4,3I10.0,7F10.4)") vecres(i,1),vecres(i,2),vecres(i,3),int(vecres(i,4)), &
+				int(vecres(i,5)),int(vecres(i,6)),vecres(i,7),vecres(i,8),vecres(i,9),vecres(i,10), &
+				vecres(i,11),vecres(i,12),vecres(i,13)
+			write(404,"(3F10.4,3I10.0,6F10.4)") vecres(i,1),vecres(i,2),vecres(i,3),int(vecres(i,4)), &
+				int(vecres(i,5)),int(vecres(i,6)),vecres(i,7),vecres(i,8),vecres(i,9),vecres(i,10), &
+				vecres(i,14),vecres(i,15)
+		end if
+	end do
+	close(401)
+	close(403)
+	if (tmcoef) then
+		close(402)
+		close(404)
+	end if
+	deallocate(vecres)
 
 	OPEN(UNIT=300, FILE=trim(outputfolder)//"log_bse-diel.dat",STATUS='unknown', IOSTAT=erro)
 	if (erro/=0) stop "Error opening log_bse-diel output file"

@@ -96,6 +96,13 @@ module wtb_gpu
 			real(c_double) :: kpt(*)
 			integer(c_int) :: nocp(*)
 		end function
+		integer(c_int) function wtb_sp_table(ctx, model, prm, vecres) bind(C, name="wtb_sp_table")
+			import :: c_int, c_ptr, c_float, wtb_tb_model, wtb_bse_params
+			type(c_ptr), value :: ctx
+			type(wtb_tb_model) :: model
+			type(wtb_bse_params) :: prm
+			real(c_float) :: vecres(*)
+		end function
 		integer(c_int) function wtb_bse_spectrum(ctx, n, e, act6, rlat, ngrid, ebse0, ebsef, &
 				numbse, sme, renorm, eps) bind(C, name="wtb_bse_spectrum")
 			import :: c_int, c_ptr, c_double

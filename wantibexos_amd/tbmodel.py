@@ -154,6 +154,8 @@ def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -
     ]
     if p.get("bsewf"):
         lines += ["BSE_WF= T", f"EXC_WF_I= {p['excwf'][0]}", f"EXC_WF_F= {p['excwf'][1]}"]
+    if p.get("tmcoef"):
+        lines += ["TMCOEF= T"]
     temp = [f"TEMP= {p.get('temp', 0.0)!r}", f"TA= {p.get('ta', 'FA')}", f"ST= {p.get('st', 0.0)!r}",
             f"PHAVG= {p.get('phavg', 0.0)!r}"]
     if p.get("kpath_bse"):                  # exciton bands alone: bsebnds, or bsebndstemp with bset

@@ -74,6 +74,7 @@ def test_dropin_outputs_match_reference(tmp_path, p):
             assert np.abs(ta[ka] - tb[kb]).max() <= 1.00001e-4
     files = [f"bse_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]
     files += [f"bse_diel-pol_{c}.dat" for c in ["x", "y", "z", "sp", "sm"]]      # bsedielrawpol
+    files += [f"sp_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]   # spdielraw
     for comp in files:
         ea = table(tmp_path / "ref" / comp)
         eb = table(tmp_path / "gpu" / comp)
@@ -184,3 +185,73 @@ def test_dropin_exciton_bands_wavefunctions(tmp_path):
                 assert np.abs(ma - mb).max() <= 2e-4 * max(ma.max(), 1e-30) + 1e-8, (n, q)
                 checked += 1
     assert checked >= 3
+
+
+def same_rows(path_a, path_b):
+    """Rows of two F15.4 tables whose values are equal (the rows with no value on a
+    rounding boundary; -0.0000 and 0.0000 count as equal)."""
+    a, b = table(path_a), table(path_b)
+    assert a.shape == b.shape
+    return (a == b).all(axis=1)
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+@pytest.mark.parametrize("p", [P, FINITE_T], ids=["bsesolver", "bsesolvertemp"])
+def test_dropin_post_processing_and_single_particle_spectra(tmp_path, p):
+    """The rest of the spectrum block of program main (wtb_main.f90:257-326): the
+    reference's own bseoptprop / lifetime (and -pol) run on the drop-in's spectra, the
+    GPU spdielrawpol on the drop-in's sp_optics-pol.dat.  The reference program stops in
+    spoptprop (it never closes sp_diel_xz.dat, sp_diel-tool.f90:228-232, and cannot
+    reopen it), so spdielrawpol is checked against the oracle instead."""
+    import wtb_oracle as O
+    model = tbmodel.hex2d(12, 8, P["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    assert (tmp_path / "ref" / "sp_diel_xx.dat").exists(), rr.stdout + rr.stderr
+    ref, gpu = tmp_path / "ref", tmp_path / "gpu"
+    # element-wise optics of eps (F15.4): within the spectra's own agreement.  A component
+    # with eps = 0 (xz, yz of a 2D model) gives NaN in the loss function in both.
+    for kind in ("refractive_index", "extinction_coef", "reflectibility", "absorption_coef", "en_loss_func"):
+        for suf, ncol in (("", 7), ("-pol", 6)):
+            a, b = table(ref / f"bse_{kind}{suf}.dat"), table(gpu / f"bse_{kind}{suf}.dat")
+            assert a.shape == b.shape == (1001, ncol)
+            assert np.array_equal(np.isnan(a), np.isnan(b)), kind + suf
+            a, b = np.nan_to_num(a), np.nan_to_num(b)
+            tol = 1.00001e-4 + 1e-4 * np.abs(a).max(axis=0)
+            assert (np.abs(a - b) <= tol).all(), kind + suf
+    # radiative lifetimes (BSET= T writes none): a function of one bse_opt_diel row each,
+    # so identical wherever that row is
+    for suf in ("", "-pol"):
+        lf = ref / f"exciton_lifetime{suf}.dat"
+        if p.get("bset"):
+            assert not lf.exists() and not (gpu / lf.name).exists()
+            continue
+        rows = same_rows(ref / f"bse_opt_diel{suf}.dat", gpu / f"bse_opt_diel{suf}.dat")
+        assert rows.mean() > 0.8
+        la = np.array(Path(lf).read_text().splitlines()[1:])
+        lb = np.array((gpu / lf.name).read_text().splitlines()[1:])
+        assert len(la) == len(lb) == 144
+        assert (la[rows] == lb[rows]).all()
+    # spdielrawpol: five diagonal-form spectra of sp_optics-pol.dat
+    sp = table(gpu / "sp_optics-pol.dat")
+    rlat = np.array(model.rlat, dtype=np.float64)
+    acts = np.zeros((6, sp.shape[0]))
+    acts[:5] = sp[:, 1:6].T
+    _, re1, im1 = O.spectrum(sp[:, 0], np.vstack([acts[:3], acts[:3]]), rlat, P["ngrid"], P["ebse0"],
+                             P["ebsef"], P["numbse"], P["cshift"], True)
+    _, re2, im2 = O.spectrum(sp[:, 0], np.vstack([acts[3:], acts[3:]]), rlat, P["ngrid"], P["ebse0"],
+                             P["ebsef"], P["numbse"], P["cshift"], True)
+    want = {"x": (re1[0], im1[0]), "y": (re1[1], im1[1]), "z": (re1[2], im1[2]),
+            "sp": (re2[0], im2[0]), "sm": (re2[1], im2[1])}
+    for c, (wr, wi) in want.items():
+        e = table(gpu / f"sp_diel-pol_{c}.dat")
+        assert e.shape == (1001, 3)
+        for got, w in ((e[:, 1], wr), (e[:, 2], wi)):
+            scale = max(np.abs(w).max(), 1e-30)
+            assert np.abs(got - w.astype(np.float32)).max() / scale < 1e-5, c
+    # the reference's spoptprop / spoptproppol on those spectra (not reached by the
+    # reference program): written, one row per frequency
+    for f in ("sp_refractive_index.dat", "sp_absorption_coef-pol.dat"):
+        assert table(gpu / f).shape[0] == 1001, f

@@ -3,12 +3,13 @@
 ! (subprograms/bse_solver-tool-diel-temp.f90:2-660), bsebnds
 ! (subprograms/bse_kpath-tool.f90:3-479), bsebndstemp
 ! (subprograms/bse_kpath-tool-temp.f90), bsedielraw
-! (subprograms/bse_diel-tool.f90:5-245) and bsedielrawpol
-! (subprograms/bse_diel-tool-pol.f90:5-242): same names, same argument lists, same
-! output files, with the computation done by libwtb_bse.so on the GPU.
+! (subprograms/bse_diel-tool.f90:5-245), bsedielrawpol
+! (subprograms/bse_diel-tool-pol.f90:5-242), spdielraw (subprograms/sp_diel-tool.f90:4-239)
+! and spdielrawpol (subprograms/sp_diel-tool-pol.f90:4-235): same names, same argument
+! lists, same output files, with the computation done by libwtb_bse.so on the GPU.
 !
 ! Use: INCLUDE this file (after the reference's subroutines/module_input_read.f90
-! and wantibexos_amd/fortran/wtb_gpu.f90) in place of those six reference files,
+! and wantibexos_amd/fortran/wtb_gpu.f90) in place of those reference files,
 ! and link -lwtb_bse.  Arguments the GPU path does not use (numdos, kpaths,
 ! orbw, exc, rk, meshtype, ...) are accepted and ignored as in the reference.
 
@@ -247,14 +248,42 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 end subroutine wtb_bse_dropin
 
 
+! bsedielraw (subprograms/bse_diel-tool.f90:5-245) and spdielraw
+! (subprograms/sp_diel-tool.f90:4-239) differ only in their file names: both read
+! E, xx, yy, zz, xy, xz, yz per transition and write eps(omega) of the six components
+! (optdiel's diagonal form for xx, yy, zz, the off-diagonal one for the others), with
+! the same imagrenorm.  wtb_bse_spectrum computes the six spectra on the GPU.
 subroutine bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme)
-	use iso_c_binding
-	use hamiltonian_input_variables
-	use wtb_gpu
 	implicit none
 	integer :: nthreads, nc, nv
 	integer,dimension(3) :: ngrid
 	character(len=70) :: outputfolder, params
+	logical :: renorm
+	double precision :: ebse0, ebsef, numbse, sme
+	call wtb_dielraw_files(outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme, &
+			       "bse_opt_diel.dat", "bse_diel_")
+end subroutine bsedielraw
+
+subroutine spdielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme)
+	implicit none
+	integer :: nthreads, nc, nv
+	integer,dimension(3) :: ngrid
+	character(len=70) :: outputfolder, params
+	logical :: renorm
+	double precision :: ebse0, ebsef, numbse, sme
+	call wtb_dielraw_files(outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme, &
+			       "sp_optics.dat", "sp_diel_")
+end subroutine spdielraw
+
+subroutine wtb_dielraw_files(outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme,fin,fout)
+	use iso_c_binding
+	use hamiltonian_input_variables
+	use wtb_gpu
+	implicit none
+	integer :: nc, nv
+	integer,dimension(3) :: ngrid
+	character(len=70) :: outputfolder, params
+	character(len=*) :: fin, fout
 	logical :: renorm
 	double precision :: ebse0, ebsef, numbse, sme
 
@@ -274,8 +303,8 @@ subroutine bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebse
 	dimbse = ngrid(1)*ngrid(2)*ngrid(3)*nc*nv
 	nom = int(numbse)
 	allocate(ebse(dimbse), act6(dimbse,6), eps(nom,2,6))
-	OPEN(UNIT=100, FILE=trim(outputfolder)//"bse_opt_diel.dat",STATUS='unknown', IOSTAT=erro)
-	if (erro/=0) stop "Error opening bse_opt_diel input file"
+	OPEN(UNIT=100, FILE=trim(outputfolder)//fin,STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening "//fin//" input file"
 	read(100,*) aread
 	do j=1,dimbse
 		read(100,*) ebse(j),act6(j,1),act6(j,2),act6(j,3),act6(j,4),act6(j,5),act6(j,6)
@@ -290,9 +319,9 @@ subroutine bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebse
 	rc = wtb_destroy(ctx)
 
 	do c=1,6
-		OPEN(UNIT=300+c, FILE=trim(outputfolder)//"bse_diel_"//tag(order(c))//".dat", &
+		OPEN(UNIT=300+c, FILE=trim(outputfolder)//fout//tag(order(c))//".dat", &
 		     STATUS='unknown', IOSTAT=erro)
-		if (erro/=0) stop "Error opening bse_diel output file"
+		if (erro/=0) stop "Error opening "//fout//" output file"
 		write(300+c,*) "#","  ","energy","  ","real","  ","imag"
 		do j=1,nom
 			write(300+c,*) ebse0 + dble(((ebsef-ebse0)*(j-1))/(numbse-1.)), &
@@ -301,8 +330,9 @@ subroutine bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebse
 		close(300+c)
 	end do
 	close(200)
+	deallocate(ebse, act6, eps)
 	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
-end subroutine bsedielraw
+end subroutine wtb_dielraw_files
 
 
 ! bsebnds (subprograms/bse_kpath-tool.f90:3-479): exciton bands along the KPATH_BSE path.
@@ -504,18 +534,42 @@ subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse
 end subroutine wtb_bands_dropin
 
 
-! bsedielrawpol (subprograms/bse_diel-tool-pol.f90:5-242): eps(omega) for the x, y, z,
-! sigma+ and sigma- polarisations from bse_opt_diel-pol.dat, every one with the
-! diagonal (delta = 1) form of optdiel.  wtb_bse_spectrum gives that form to its first
-! three components, so the five spectra take two calls.
+! bsedielrawpol (subprograms/bse_diel-tool-pol.f90:5-242) and spdielrawpol
+! (subprograms/sp_diel-tool-pol.f90:4-235): eps(omega) for the x, y, z, sigma+ and
+! sigma- polarisations from bse_opt_diel-pol.dat or sp_optics-pol.dat, every one with
+! the diagonal (delta = 1) form of optdiel.  wtb_bse_spectrum gives that form to its
+! first three components, so the five spectra take two calls.
 subroutine bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme)
-	use iso_c_binding
-	use hamiltonian_input_variables
-	use wtb_gpu
 	implicit none
 	integer :: nthreads, nc, nv
 	integer,dimension(3) :: ngrid
 	character(len=70) :: outputfolder, params
+	logical :: renorm
+	double precision :: ebse0, ebsef, numbse, sme
+	call wtb_dielrawpol_files(outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme, &
+				  "bse_opt_diel-pol.dat", "bse_diel-pol_")
+end subroutine bsedielrawpol
+
+subroutine spdielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme)
+	implicit none
+	integer :: nthreads, nc, nv
+	integer,dimension(3) :: ngrid
+	character(len=70) :: outputfolder, params
+	logical :: renorm
+	double precision :: ebse0, ebsef, numbse, sme
+	call wtb_dielrawpol_files(outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme, &
+				  "sp_optics-pol.dat", "sp_diel-pol_")
+end subroutine spdielrawpol
+
+subroutine wtb_dielrawpol_files(outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,sme,fin,fout)
+	use iso_c_binding
+	use hamiltonian_input_variables
+	use wtb_gpu
+	implicit none
+	integer :: nc, nv
+	integer,dimension(3) :: ngrid
+	character(len=70) :: outputfolder, params
+	character(len=*) :: fin, fout
 	logical :: renorm
 	double precision :: ebse0, ebsef, numbse, sme
 
@@ -536,8 +590,8 @@ subroutine bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,e
 	allocate(ebse(dimbse), a1(dimbse,6), a2(dimbse,6), eps1(nom,2,6), eps2(nom,2,6))
 	a1 = 0d0
 	a2 = 0d0
-	OPEN(UNIT=100, FILE=trim(outputfolder)//"bse_opt_diel-pol.dat",STATUS='unknown', IOSTAT=erro)
-	if (erro/=0) stop "Error opening bse_opt_diel-pol input file"
+	OPEN(UNIT=100, FILE=trim(outputfolder)//fin,STATUS='unknown', IOSTAT=erro)
+	if (erro/=0) stop "Error opening "//fin//" input file"
 	read(100,*) aread
 	do j=1,dimbse
 		read(100,*) ebse(j),a1(j,1),a1(j,2),a1(j,3),a2(j,1),a2(j,2)
@@ -555,9 +609,9 @@ subroutine bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,e
 	rc = wtb_destroy(ctx)
 
 	do c=1,5
-		OPEN(UNIT=300+c, FILE=trim(outputfolder)//"bse_diel-pol_"//trim(tag(c))//".dat", &
+		OPEN(UNIT=300+c, FILE=trim(outputfolder)//fout//trim(tag(c))//".dat", &
 		     STATUS='unknown', IOSTAT=erro)
-		if (erro/=0) stop "Error opening bse_diel-pol output file"
+		if (erro/=0) stop "Error opening "//fout//" output file"
 		write(300+c,*) "#","  ","energy","  ","real","  ","imag"
 		do j=1,nom
 			elux = ebse0 + dble(((ebsef-ebse0)*(j-1))/(numbse-1.))
@@ -572,4 +626,4 @@ subroutine bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,e
 	close(200)
 	deallocate(ebse, a1, a2, eps1, eps2)
 	deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
-end subroutine bsedielrawpol
+end subroutine wtb_dielrawpol_files

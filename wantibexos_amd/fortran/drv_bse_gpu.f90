@@ -1,7 +1,9 @@
 ! The BSE + spectrum and exciton-band paths of the reference's program main
-! (wtb_main.f90:60-190, 253-284) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
-! reference's bsesolver and bsedielraw.  The input parser and the TB reader
-! are the reference's own (INCLUDEd from its sources by the Makefile).
+! (wtb_main.f90:60-190, 253-326) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
+! reference's bsesolver, bsedielraw and spdielraw (and their variants).  The input parser,
+! the TB reader and the element-wise post-processing of the spectra (bseoptprop,
+! lifetime, spoptprop and their -pol forms: n, k, R, alpha, loss function, radiative
+! lifetimes) are the reference's own, INCLUDEd from its sources by the Makefile.
 !
 !   wtb_gpu.x < input.dat     (the reference's KEY= value input)
 
@@ -10,6 +12,12 @@ INCLUDE "subroutines/mhkpack_subs.f90"
 INCLUDE "subroutines/module_input_read.f90"
 INCLUDE "subroutines/coulomb_pot.f90"
 INCLUDE "subroutines/bse_subs.f90"
+INCLUDE "subroutines/diel-pp-subs.f90"
+INCLUDE "subprograms/diel-pp.f90"
+INCLUDE "subprograms/diel-pp-pol.f90"
+INCLUDE "subprograms/diel-pp-bse.f90"
+INCLUDE "subprograms/diel-pp-bse-pol.f90"
+INCLUDE "subprograms/exciton_lifetime.f90"
 INCLUDE "wtb_gpu.f90"
 INCLUDE "bse_solver_gpu.f90"
 
@@ -64,9 +72,21 @@ program wtb_gpu_main
 			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff,&
 			st,phavg,ta,temp)
 	end if
+	! the spectrum block of wtb_main.f90:257-326 for BSE= T / BSET= T (DIEL= T and
+	! DIEL_POL= T alone need the single-particle solver spoptics, not built here)
 	if (spec .and. (bse .or. bset)) then
 		call bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
-		if (cpol) call bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef, &
-					     numbse,cshift)
+		call bseoptprop(numbse,outputfolder)
+		if (.not. bset) call lifetime(sysdim,numbse,ngrid,rlat,nc,nv,outputfolder)
+		if (cpol) then
+			call bsedielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef, &
+					   numbse,cshift)
+			call bseoptproppol(numbse,outputfolder)
+			if (.not. bset) call lifetimepol(sysdim,numbse,ngrid,rlat,nc,nv,outputfolder)
+		end if
+		call spdielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
+		call spoptprop(numbse,outputfolder)
+		call spdielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
+		call spoptproppol(numbse,outputfolder)
 	end if
 end program wtb_gpu_main

@@ -58,31 +58,57 @@ def test_dropin_outputs_match_reference(tmp_path, p):
     # F15.4 text: identical except where a value sits on a rounding boundary
     assert np.abs(a - b).max() <= 1.00001e-4
     assert np.mean(a != b) < 0.01
-    # the single-particle side output, sorted by transition energy: same rows (F15.4)
+    check_sp_tables(tmp_path / "ref", tmp_path / "gpu", p.get("tmcoef"))
+    files = [f"bse_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]
+    files += [f"bse_diel-pol_{c}.dat" for c in ["x", "y", "z", "sp", "sm"]]      # bsedielrawpol
+    files += [f"sp_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]   # spdielraw
+    check_spectra(tmp_path / "ref", tmp_path / "gpu", files)
+
+
+def check_sp_tables(ref, gpu, tmcoef):
+    """The single-particle side output, sorted by transition energy: same rows (F15.4)."""
     for sp in ("sp_optics.dat", "sp_optics-pol.dat"):
-        sa, sb = table(tmp_path / "ref" / sp), table(tmp_path / "gpu" / sp)
+        sa, sb = table(ref / sp), table(gpu / sp)
         assert sa.shape == sb.shape == (144, 7 if sp == "sp_optics.dat" else 6)
         assert np.abs(sa[:, 0] - sb[:, 0]).max() <= 1.00001e-4
         ka, kb = np.lexsort(sa.T[::-1]), np.lexsort(sb.T[::-1])
         assert np.abs(sa[ka] - sb[kb]).max() <= 1.00001e-4
-    if p.get("tmcoef"):                  # TMCOEF= T: k, band numbers and the same columns
+    if tmcoef:                           # TMCOEF= T: k, band numbers and the same columns
         for tm, ncol in (("tm_coef.dat", 13), ("tm_coef-pol.dat", 12)):
-            ta, tb = table(tmp_path / "ref" / tm, skip=4), table(tmp_path / "gpu" / tm, skip=4)
+            ta, tb = table(ref / tm, skip=4), table(gpu / tm, skip=4)
             assert ta.shape == tb.shape == (144, ncol)
             ka, kb = np.lexsort(ta.T[::-1]), np.lexsort(tb.T[::-1])
             assert np.array_equal(ta[ka][:, 3:6], tb[kb][:, 3:6])
             assert np.abs(ta[ka] - tb[kb]).max() <= 1.00001e-4
-    files = [f"bse_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]
-    files += [f"bse_diel-pol_{c}.dat" for c in ["x", "y", "z", "sp", "sm"]]      # bsedielrawpol
-    files += [f"sp_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]]   # spdielraw
+
+
+def check_spectra(ref, gpu, files):
+    """eps(omega) files (energy, REAL re, REAL im): same grid, 1e-5 of each column's scale."""
     for comp in files:
-        ea = table(tmp_path / "ref" / comp)
-        eb = table(tmp_path / "gpu" / comp)
+        ea = table(ref / comp)
+        eb = table(gpu / comp)
         assert ea.shape == eb.shape == (1001, 3)
         assert np.abs(ea[:, 0] - eb[:, 0]).max() < 1e-12
         for c in (1, 2):
             scale = max(np.abs(ea[:, c]).max(), 1e-30)
             assert np.abs(ea[:, c] - eb[:, c]).max() / scale < 1e-5, (comp, c)
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+def test_dropin_single_particle_optics(tmp_path):
+    """DIEL= T without BSE: spoptics (sp_solver-tool-diel.f90:3-323) writes the table alone,
+    spdielraw its spectra; no BSE file is written."""
+    p = dict(P, spdiel=True, tmcoef=True)
+    model = tbmodel.hex2d(13, 8, P["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    assert (tmp_path / "ref" / "sp_diel_xx.dat").exists(), rr.stdout + rr.stderr
+    assert not (tmp_path / "gpu" / "bse_opt_diel.dat").exists()
+    check_sp_tables(tmp_path / "ref", tmp_path / "gpu", True)
+    check_spectra(tmp_path / "ref", tmp_path / "gpu",
+                  [f"sp_diel_{c}.dat" for c in ["xx", "xy", "xz", "yy", "yz", "zz"]])
 
 
 @pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")

@@ -33,7 +33,7 @@ subroutine bsesolver(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 	zero = 0d0
 	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
 			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol,tmcoef, &
-			    .false.,zero,zero,"FA",zero)
+			    .false.,zero,zero,"FA",zero,.false.)
 end subroutine bsesolver
 
 ! bsesolvertemp (subprograms/bse_solver-tool-diel-temp.f90:2-660): the same with BSET= T.
@@ -58,15 +58,41 @@ subroutine bsesolvertemp(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
 
 	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
 			    ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol,tmcoef, &
-			    .true.,st,phavg,ta,temp)
+			    .true.,st,phavg,ta,temp,.false.)
 	if (temp .eq. 0d0) temp = 1E-8
 end subroutine bsesolvertemp
+
+! spoptics (subprograms/sp_solver-tool-diel.f90:3-323), DIEL= T: the single-particle
+! table of the BSE drop-in without the BSE solve (sp_optics.dat, sp_optics-pol.dat and,
+! with TMCOEF= T, tm_coef*.dat).  The table uses no Coulomb potential; the plan behind
+! wtb_sp_table still wants a valid one, so the potential's own parameters are neutral.
+subroutine spoptics(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+		    ebse0,ebsef,numbse,sme,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+		    exc,mshift,coultype,rk,meshtype,tmcoef)
+	implicit none
+	integer :: nthreads
+	integer,dimension(3) :: ngrid
+	integer :: nc,nv
+	double precision :: numdos, ebse0, ebsef, numbse, sme, exc, rk, ktol
+	double precision,dimension(3) :: mshift, ediel
+	character(len=70) :: params, orbw, kpaths, kpathsbse, outputfolder, calcparms, meshtype
+	character(len=5) :: coultype
+	logical :: tmcoef
+	double precision :: zero, one
+
+	zero = 0d0
+	one = 1d0
+	call wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
+			    ediel,mshift,coultype,one,zero,one,one,.false.,1,1,.true.,.false.,tmcoef, &
+			    .false.,zero,zero,"FA",zero,.true.)
+end subroutine spoptics
+
 
 ! The body of both: model and parameters into the C structures, one wtb_bsesolver(_wf)
 ! call, the reference's output files.
 subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,sme,ktol,params, &
 			  ediel,mshift,coultype,ez,w1,r0,lc,bsewf,excwf0,excwff,dtfull,cpol,tmcoef, &
-			  bset,st,phavg,ta,temp)
+			  bset,st,phavg,ta,temp,sponly)
 	use iso_c_binding
 	use hamiltonian_input_variables
 	use wtb_gpu
@@ -78,7 +104,7 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 	double precision,dimension(3) :: mshift, ediel
 	character(len=70) :: params, outputfolder
 	character(len=5) :: coultype
-	logical :: bsewf, cpol, dtfull, bset, tmcoef
+	logical :: bsewf, cpol, dtfull, bset, tmcoef, sponly
 	integer :: excwf0, excwff
 	double precision :: ez, w1, lc, r0
 	double precision :: st, phavg, temp
@@ -151,7 +177,9 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 
 	rc = wtb_create(0_c_int, ctx)
 	call wtb_check(rc)
-	if (bsewf) then
+	if (sponly) then
+		continue
+	else if (bsewf) then
 		! BSE_WF: the vectors of excitons excwf0 .. excwff and what excwf2 prints beside them
 		if (excwf0 < 1 .or. excwff > dimbse .or. excwff < excwf0) stop "Wrong BSE_WF exciton range"
 		nwf = excwff - excwf0 + 1
@@ -163,7 +191,8 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 		rc = wtb_bsesolver(ctx, m, p, w, act, c_null_ptr, egap)
 	end if
 	call wtb_check(rc)
-	! the single-particle side output (bse_solver-tool-diel.f90:336-420)
+	! the single-particle side output (bse_solver-tool-diel.f90:336-420), which is all
+	! that spoptics (subprograms/sp_solver-tool-diel.f90) writes
 	allocate(vecres(dimbse,15))
 	rc = wtb_sp_table(ctx, m, p, vecres)
 	call wtb_check(rc)
@@ -209,6 +238,12 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 		close(404)
 	end if
 	deallocate(vecres)
+	if (sponly) then
+		close(200)
+		deallocate(w, act)
+		deallocate(rvec,hopmatrices,ihopmatrices,ffactor)
+		return
+	end if
 
 	OPEN(UNIT=300, FILE=trim(outputfolder)//"log_bse-diel.dat",STATUS='unknown', IOSTAT=erro)
 	if (erro/=0) stop "Error opening log_bse-diel output file"

@@ -72,8 +72,14 @@ program wtb_gpu_main
 			exc,mshift,coultype,ez,w,r0,lc,rk,meshtype,bsewf,excwf0,excwff,&
 			st,phavg,ta,temp)
 	end if
-	! the spectrum block of wtb_main.f90:257-326 for BSE= T / BSET= T (DIEL= T and
-	! DIEL_POL= T alone need the single-particle solver spoptics, not built here)
+	! after the exciton bands, as in wtb_main.f90:209-218 (with BSET= T it replaces the
+	! finite-temperature sp_optics.dat by the T = 0 one, as there)
+	if (spdiel) then
+		call spoptics(nthreads,outputfolder,calcparms,ngrid,nc,nv,numdos, &
+			ebse0,ebsef,numbse,cshift,ktol,params,kpaths,kpathsbse,orbw,ediel, &
+			exc,mshift,coultype,rk,meshtype,tmcoef)
+	end if
+	! the spectrum block of wtb_main.f90:257-326
 	if (spec .and. (bse .or. bset)) then
 		call bsedielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
 		call bseoptprop(numbse,outputfolder)
@@ -84,8 +90,12 @@ program wtb_gpu_main
 			call bseoptproppol(numbse,outputfolder)
 			if (.not. bset) call lifetimepol(sysdim,numbse,ngrid,rlat,nc,nv,outputfolder)
 		end if
+	end if
+	if (spec .and. (spdiel .or. bse .or. bset)) then
 		call spdielraw(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
 		call spoptprop(numbse,outputfolder)
+	end if
+	if (spec .and. (cpol .or. bse .or. bset)) then
 		call spdielrawpol(nthreads,outputfolder,renorm,params,ngrid,nc,nv,ebse0,ebsef,numbse,cshift)
 		call spoptproppol(numbse,outputfolder)
 	end if

@@ -158,7 +158,10 @@ def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -
         lines += ["TMCOEF= T"]
     temp = [f"TEMP= {p.get('temp', 0.0)!r}", f"TA= {p.get('ta', 'FA')}", f"ST= {p.get('st', 0.0)!r}",
             f"PHAVG= {p.get('phavg', 0.0)!r}"]
-    if p.get("kpath_bse"):                  # exciton bands alone: bsebnds, or bsebndstemp with bset
+    if p.get("spdiel"):                     # single-particle optics alone: spoptics (DIEL= T)
+        lines[lines.index("BSE= T")] = "BSE= F"
+        lines += ["DIEL= T"]
+    elif p.get("kpath_bse"):                # exciton bands alone: bsebnds, or bsebndstemp with bset
         lines[lines.index("BSE= T")] = "BSE= F"
         lines[lines.index("SPEC= T")] = "SPEC= F"
         lines += [f'KPATH_BSE= "{p["kpath_bse"]}"']

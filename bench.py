@@ -54,7 +54,10 @@ CONFIG = dict(name="hex2d_nw22_36x36x1_nv2nc2", seed=2, nw=22, ngrid=(36, 36, 1)
               shift=(0.0, 0.0, 0.0))
 SMALL = dict(CONFIG, name="hex2d_nw8_12x12x1_nv2nc2", seed=1, nw=8, ngrid=(12, 12, 1))
 LARGE2D = dict(CONFIG, name="hex2d_nw24_60x60x1_nv2nc2", seed=5, nw=24, ngrid=(60, 60, 1))
-CONFIGS = {"c2": CONFIG, "c1": SMALL, "c5": LARGE2D}
+# large single solve, 3D bulk: cubic a = 5.4 A, nw = 16, |R| <= 2 cells, 16x16x16 k, nv = nc = 3
+BULK3D = dict(CONFIG, name="cubic3d_nw16_16x16x16_nv3nc3", lattice="cubic", seed=3, nw=16, ngrid=(16, 16, 16),
+              nc=3, nv=3, coultype="V3DL", ediel=(6.0, 6.0, 6.0))
+CONFIGS = {"c2": CONFIG, "c1": SMALL, "c3": BULK3D, "c5": LARGE2D}
 HBM_PEAK = 8000.0       # GB/s
 FP64_PEAK = 78.6        # TFLOP/s
 
@@ -65,6 +68,8 @@ def log(*a):
 
 def make_model(cfg):
     from wantibexos_amd import tbmodel
+    if cfg.get("lattice") == "cubic":
+        return tbmodel.cubic3d(cfg["seed"], cfg["nw"], cfg["ngrid"])
     return tbmodel.hex2d(cfg["seed"], cfg["nw"], cfg["ngrid"])
 
 
@@ -363,7 +368,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed (default): the precision form, timed beside the plain FP64 form")
-    ap.add_argument("--config", choices=["c2", "c1", "c5"], default="c2")
+    ap.add_argument("--config", choices=["c2", "c1", "c3", "c5"], default="c2")
     ap.add_argument("--form", choices=["full", "davidson", "batch", "bands"], default="full",
                     help="full: the whole eigensolve (plain FP64 or precision form); davidson: the "
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")

@@ -127,7 +127,7 @@ def tridiag_bytes(N, nb=32, elem=16.0):
 def bench_davidson(args, cfg, model):
     """Iterative form: build + lowest nev excitons (block Davidson, FP32 products with an FP64
     polish) + eps(omega) from Lanczos chains, timed against the full FP64 solve of the plain
-    path on the same model, which also judges the result."""
+    path on the same model, which is also the accuracy reference."""
     import dataclasses
     from wantibexos_amd import bse, learned
     ctx = bse.Context(0)
@@ -155,7 +155,7 @@ def bench_davidson(args, cfg, model):
     plan.close()
     dt = float(np.mean(times))
     coarse = args.coarse if args.coarse is not None else min(10 * (args.nev + args.guard), N // 4)
-    # the judge: full FP64 solve (plain path), timed once after a warm-up
+    # the reference point: full FP64 solve (plain path), timed once after a warm-up
     with bse.BSEPlan(ctx, model, p64) as full:
         if args.no_graph:
             full.set_graph(False)

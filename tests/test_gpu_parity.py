@@ -212,6 +212,18 @@ def test_plan_reexecute_is_reproducible(ctx):
         assert np.array_equal(r1["eps"], r["eps"])
 
 
+def test_single_particle_table_needs_a_filled_plan(ctx):
+    """The table comes from the plan's bands and dipoles: before build/execute the call is
+    an argument error, not a table of whatever the buffers hold."""
+    from wantibexos_amd import bse
+    m, p, z = load(CASES[0])
+    with bse.BSEPlan(ctx, m, params_of(p)) as plan:
+        with pytest.raises(bse.WTBError, match="build or execute"):
+            plan.sp_table()
+        plan.build()
+        assert plan.sp_table().shape == (len(z["stt"]), 15)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_single_particle_table(ctx, name):
     """The single-particle side output (sp_optics.dat / tm_coef.dat content): the GPU's

@@ -79,6 +79,7 @@ struct wtb_ctx {
 struct wtb_bse_plan {
   wtb_ctx* ctx = nullptr;
   wtb_bse_params prm{};
+  bool filled = false;           // the k grid, bands and dipoles have been computed (build/execute)
   int nk = 0, nw = 0, nvec = 0, nb = 0, N = 0;
   double ef32 = 0, scs = 0, sme32 = 0, vc3d = 0, nkf32 = 0;
   // finite temperature: gap correction (added to scs), 1/(kB T) of the Fermi factors,
@@ -710,12 +711,17 @@ int wtb_bse_plan_execute(wtb_bse_plan* p) {
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));
-  if (p->timing || !p->use_graph) return plan_enqueue(p);
+  if (p->timing || !p->use_graph) {
+    const int rc = plan_enqueue(p);
+    if (!rc) p->filled = true;
+    return rc;
+  }
   if (!p->gexec) {
     int rc = plan_capture(p);
     if (rc) return rc;
   }
   WTB_HIP(hipGraphLaunch(p->gexec, p->stream));
+  p->filled = true;
   return WTB_OK;
 }
 
@@ -734,6 +740,7 @@ int wtb_bse_plan_build(wtb_bse_plan* p) {
   int rc = plan_enqueue(p, true);
   if (rc) return rc;
   WTB_HIP(hipStreamSynchronize(p->ctx->stream));
+  p->filled = true;
   return WTB_OK;
 }
 
@@ -1841,6 +1848,10 @@ int wtb_bsebnds(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, in
 int wtb_bse_plan_sp_table(wtb_bse_plan* p, float* vecres) {
   if (!p || !vecres || p->qmode) {
     set_error("wtb_bse_plan_sp_table: needs a Q = 0 plan and vecres(N, 15)");
+    return WTB_ERR_ARG;
+  }
+  if (!p->filled) {
+    set_error("wtb_bse_plan_sp_table: build or execute the plan first (the table comes from its bands)");
     return WTB_ERR_ARG;
   }
   WTB_HIP(hipSetDevice(p->ctx->device));

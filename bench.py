@@ -200,15 +200,21 @@ def bench_batch(args):
     (one captured graph per matrix, replayed on --lanes streams); the results must agree
     bit for bit."""
     import dataclasses
+    from concurrent.futures import ProcessPoolExecutor
     from wantibexos_amd import bse, tbmodel
-    ctx = bse.Context(0)
     spec = [(12, 8), (14, 6), (16, 10), (18, 8), (20, 12), (22, 6), (24, 8), (16, 10)]   # (k grid, nw)
-    models, prms = [], []
-    for i, (ng, nw) in enumerate(spec):
-        models.append(tbmodel.hex2d(200 + i, nw, (ng, ng, 1)))
-        cfg = dict(CONFIG, ngrid=(ng, ng, 1))
-        prms.append(params(cfg, args.precision))
-    dims = []
+    if args.config == "c4":
+        # 64 seeded models: nw uniform in 6..40, k grids uniform in 12x12 .. 30x30 (N 576 .. 3600)
+        rng = np.random.default_rng(4)
+        spec = [(int(rng.integers(12, 31)), int(rng.integers(6, 41))) for _ in range(64)]
+    # the models are generated on host worker processes, before this process touches the GPU
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        models = list(pool.map(tbmodel.hex2d, [200 + i for i in range(len(spec))], [nw for _, nw in spec],
+                               [(ng, ng, 1) for ng, _ in spec]))
+    log(f"{len(models)} models generated in {time.perf_counter() - t0:.1f}s")
+    prms = [params(dict(CONFIG, ngrid=(ng, ng, 1)), args.precision) for ng, _ in spec]
+    ctx = bse.Context(0)
     plans = [bse.BSEPlan(ctx, m, p) for m, p in zip(models, prms)]
     dims = [pl.N for pl in plans]
     log(f"batching form: {len(plans)} matrices, N = {dims}, precision {args.precision}, lanes {args.lanes}")
@@ -246,7 +252,7 @@ def bench_batch(args):
         "value": round(t_batch, 6), "unit": "s/set", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "c128" if args.precision == "fp64" else "c64/c128", "data": "synthetic (seeded random tight-binding models)",
-        "config": {"workload": "hex2d set", "models": [{"kgrid": ng, "nwann": nw, "bse_dim": N} for (ng, nw), N in zip(spec, dims)],
+        "config": {"workload": f"hex2d set of {len(spec)}", "models": [{"kgrid": ng, "nwann": nw, "bse_dim": N} for (ng, nw), N in zip(spec, dims)],
                    "lanes": args.lanes, "precision": args.precision, "tuned_panels": tiles},
         "single_plans_s": round(t_single, 6), "speedup_vs_single": round(t_single / t_batch, 3),
         "matrices_per_s": round(len(dims) / t_batch, 2), "bitwise_equal_to_single": same, "max_dW_eV": dW,
@@ -368,7 +374,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed (default): the precision form, timed beside the plain FP64 form")
-    ap.add_argument("--config", choices=["c2", "c1", "c3", "c5"], default="c2")
+    ap.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2",
+                    help="c4: the 64-model set of the batching form (--form batch only)")
     ap.add_argument("--form", choices=["full", "davidson", "batch", "bands"], default="full",
                     help="full: the whole eigensolve (plain FP64 or precision form); davidson: the "
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")
@@ -419,6 +426,9 @@ def main():
     from wantibexos_amd import bse
     if args.form == "batch":
         return bench_batch(args)
+    if args.config == "c4":
+        print("bench.py: --config c4 is the batching form's set; use --form batch", file=sys.stderr)
+        sys.exit(2)
     cfg = CONFIGS[args.config]
     if args.temp is not None:
         cfg = dict(cfg, name=f"{cfg['name']}_T{args.temp:g}K_{args.ta}", bset=True, temp=args.temp, ta=args.ta,

@@ -160,6 +160,28 @@ int wtb_bse_spectrum(wtb_ctx* ctx, int n, const double* e, const double* act6,
                      const double* rlat, const int ngrid[3], double ebse0, double ebsef,
                      int numbse, double sme, int renorm, double* eps);
 
+/* bseoptprop and spoptprop (subprograms/diel-pp-bse.f90:3-173, diel-pp.f90:3-172) and
+ * their circular-polarisation forms bseoptproppol, spoptproppol (diel-pp-bse-pol.f90,
+ * diel-pp-pol.f90): the optical constants of subroutines/diel-pp-subs.f90:1-73 for ncomp
+ * spectra as the bse_diel*.dat / sp_diel*.dat files hold them.
+ *   energy(n), eps_re(n, ncomp), eps_im(n, ncomp)
+ *   out(n, ncomp, 5): refractive index, extinction coefficient, reflectivity,
+ *                     absorption coefficient, energy-loss function               */
+int wtb_optprop(wtb_ctx* ctx, int n, int ncomp, const double* energy, const double* eps_re,
+                const double* eps_im, double* out);
+
+/* lifetime / lifetimepol (subprograms/exciton_lifetime.f90:2-92, 95-187) with exclft
+ * (subroutines/bse_subs.f90:472-530): radiative lifetimes (s).
+ *   fosc(n, 1 + ncomp): energy, then the activities as bse_opt_diel.dat (ncomp = 6:
+ *                       xx yy zz xy xz yz) or bse_opt_diel-pol.dat (ncomp = 5:
+ *                       x y z sp sm) holds them
+ *   lft(n, 1 + nout):   the lifetime for the summed strength of components 1..3, then
+ *                       one per component 1..nout (3 for lifetime, 5 for lifetimepol)
+ *   sysdim 1, 2, 3 for SYSDIM= "1D", "2D", "3D"; another value is WTB_ERR_ARG with the
+ *   reference's "Wrong System dimension".                                          */
+int wtb_exciton_lifetime(wtb_ctx* ctx, int sysdim, const int ngrid[3], const double* rlat, int n,
+                         int ncomp, int nout, const double* fosc, double* lft);
+
 /* ---- the whole pipeline -------------------------------------------------------
  * bsesolver (subprograms/bse_solver-tool-diel.f90:2-660) followed by the
  * spectrum of bsedielraw (subprograms/bse_diel-tool.f90:5-245), on the GPU:

@@ -28,6 +28,8 @@ Pinning (how each function is checked; see tests/test_oracle_*.py):
   BSE eigenvalues/activities    vs reference dump; sum rule; Hermiticity
   eps(omega), imagrenorm        vs reference dump (F15.4 path and full precision)
   finite T (tcor, f_v - f_c)    vs reference dump; closed forms; activity sum rule
+  rkmesh / rkmesh2D             vs the reference's routines on 162 cases; closed forms
+  optprop, exciton lifetimes    vs the reference program's files; closed forms
   finite-Q bands (kpathbse,     vs the reference program's bands_bse.dat / KLABELS-BSE.dat;
     k+q grid, exchange)         q = 0, q = G and rank-one interlacing invariants
 The per-k and BSE eigensolves use LAPACK through numpy (zheevd) in place of
@@ -118,12 +120,19 @@ def monkhorst_pack(ngrid, shift, rlat):
     return out
 
 
-def rkmesh2D(rk, rlat):
-    """mhkpack_subs.f90:30-52"""
+def rkmesh(rk, rlat):
+    """mhkpack_subs.f90:1-26 (MESH_TYPE= RK3D): n_i = int(max(1, rk |b_i| / (2 pi) + 1/2))
+    with the reference's single-precision pi in recvec and in the division."""
     b = recvec(rlat)
     pi = PI32  # acos(-1.0) single precision, used in double arithmetic
     vs = [vecsize(b[i]) / (2 * pi) for i in range(3)]
-    return [int(max(1.0, rk * vs[0] + 0.5)), int(max(1.0, rk * vs[1] + 0.5)), 1]
+    return [int(max(1.0, rk * vs[i] + 0.5)) for i in range(3)]
+
+
+def rkmesh2D(rk, rlat):
+    """mhkpack_subs.f90:28-52 (MESH_TYPE= RK2D): rkmesh's first two axes, n_3 = 1."""
+    g = rkmesh(rk, rlat)
+    return [g[0], g[1], 1]
 
 
 # ---------------------------------------------------------------------------
@@ -746,3 +755,63 @@ def run_bands(model, ngrid, qpath, **kw):
         exk[:, iq] = W
         mats.append(H)
     return dict(kpt=kpt, exk=exk, H=mats)
+
+
+# ---------------------------------------------------------------------------
+# post-processing of the spectra and exciton lifetimes
+# ---------------------------------------------------------------------------
+HC_F32 = float(F32(19.746e-6))        # diel-pp-subs.f90:57 (REAL literal)
+SQRT2_F32 = float(np.sqrt(F32(2.0)))  # sqrt(2.0), diel-pp-subs.f90:60
+CFE_F32 = float(F32(7.297e-3))        # bse_subs.f90:475
+LSPD_F32 = float(F32(299.792e16))     # bse_subs.f90:476
+HBAR_F32 = float(F32(6.582e-16))      # bse_subs.f90:477
+
+
+def optprop(energy, e1, e2):
+    """refracao, extincao, reflectibilidade, abscoef, enloss (diel-pp-subs.f90:1-73) as
+    bseoptprop applies them to every photon energy and component (diel-pp-bse.f90:89-140).
+    Returns (n, k, R, alpha, L) with the shapes of e1."""
+    e1 = np.asarray(e1, dtype=np.float64)
+    e2 = np.asarray(e2, dtype=np.float64)
+    en = np.asarray(energy, dtype=np.float64).reshape((-1,) + (1,) * (e1.ndim - 1))
+    aux = np.sqrt((e1 * e1) + (e2 * e2))
+    refr = np.sqrt((aux + e1) / 2.0)
+    ext = np.sqrt((aux - e1) / 2.0)
+    refl = (((refr - 1.0) * (refr - 1.0)) + ext * ext) / (((refr + 1.0) * (refr + 1.0)) + ext * ext)
+    absc = ((SQRT2_F32 * en) * (1.0 / HC_F32)) * np.sqrt(aux - e1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        eloss = e2 / ((e1 * e1) + (e2 * e2))
+    return refr, ext, refl, absc, eloss
+
+
+def exclft(sysdim, ngrid, rlat, fosc, enexc):
+    """exclft (bse_subs.f90:472-530): radiative lifetime for SYSDIM "3D", "2D" or "1D"."""
+    nkd = float(int(ngrid[0]) * int(ngrid[1]) * int(ngrid[2]))
+    f = np.asarray(fosc, dtype=np.float64)
+    E = np.asarray(enexc, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if sysdim == "3D":
+            aux1 = (4.0 * CFE_F32) * ((E * E) * E)
+            aux2 = f / nkd
+            aux3 = (3.0 * (LSPD_F32 * LSPD_F32)) * ((HBAR_F32 * HBAR_F32) * HBAR_F32)
+            return 1.0 / ((aux1 * aux2) / aux3)
+        if sysdim == "2D":
+            vc = vcell2D(np.asarray(rlat, dtype=np.float64))
+            return ((nkd * vc) * HBAR_F32) / ((((8.0 * PI32) * CFE_F32) * E) * f)
+        if sysdim == "1D":
+            aux2 = ((((2.0 * PI32) * CFE_F32) * E) * E) * f
+            aux1 = (((nkd * float(rlat[0][0])) * HBAR_F32) * HBAR_F32) * LSPD_F32
+            return aux1 / aux2
+    raise ValueError("Wrong System dimension")
+
+
+def lifetimes(sysdim, ngrid, rlat, table, nout):
+    """lifetime / lifetimepol (exciton_lifetime.f90:2-187): table (N, 1 + ncomp) as read from
+    bse_opt_diel(-pol).dat; returns (N, 1 + nout): the summed x+y+z strength's lifetime,
+    then one per component."""
+    t = np.asarray(table, dtype=np.float64)
+    E = t[:, 0]
+    fsum = (t[:, 1] + t[:, 2]) + t[:, 3]
+    cols = [exclft(sysdim, ngrid, rlat, fsum, E)] + [exclft(sysdim, ngrid, rlat, t[:, c], E)
+                                                     for c in range(1, nout + 1)]
+    return np.stack(cols, axis=1)

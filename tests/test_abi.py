@@ -70,3 +70,28 @@ def test_ctypes_structures_match_the_header(tmp_path, struct, pyname):
     assert off["sizeof"] == ctypes.sizeof(cls)
     for f in names:
         assert off[f] == getattr(cls, f).offset, f
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+@pytest.mark.parametrize("field,value,text", [("nev", 0, "nev"), ("guard", -1, "guard"),
+                                              ("max_iter", 0, "max_iter"), ("max_sub", -5, "max_sub"),
+                                              ("coarse", -1, "coarse"), ("tol", 0.0, "tol"),
+                                              ("tol", float("nan"), "tol"), ("lanczos_steps", -1, "lanczos")])
+def test_lowest_rejects_bad_parameters(field, value, text):
+    """wtb_bse_plan_lowest checks every value that sizes a buffer or a loop before it
+    looks at the plan, so the rejection is visible without a device (null plan)."""
+    from wantibexos_amd import bse
+    dp = bse._DavidsonParams(nev=8, guard=4, max_sub=40, max_iter=10, tol=1e-6, lanczos_steps=10,
+                             fallback=0, coarse=0)
+    setattr(dp, field, value)
+    lib = bse.lib()
+    w = (ctypes.c_double * 8)()
+    r = (ctypes.c_double * 8)()
+    rc = lib.wtb_bse_plan_lowest(None, ctypes.byref(dp), None, w, r, None, None, None)
+    assert rc == 1
+    assert text in lib.wtb_last_error().decode()
+    # good parameters and a null plan: refused for the plan, not the parameters
+    dp2 = bse._DavidsonParams(nev=8, guard=4, max_sub=40, max_iter=10, tol=1e-6, lanczos_steps=10,
+                              fallback=0, coarse=0)
+    assert lib.wtb_bse_plan_lowest(None, ctypes.byref(dp2), None, w, r, None, None, None) == 1
+    assert "null plan" in lib.wtb_last_error().decode()

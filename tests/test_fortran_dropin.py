@@ -65,11 +65,11 @@ def test_dropin_outputs_match_reference(tmp_path, p):
     check_spectra(tmp_path / "ref", tmp_path / "gpu", files)
 
 
-def check_sp_tables(ref, gpu, tmcoef):
+def check_sp_tables(ref, gpu, tmcoef, n=144):
     """The single-particle side output, sorted by transition energy: same rows (F15.4)."""
     for sp in ("sp_optics.dat", "sp_optics-pol.dat"):
         sa, sb = table(ref / sp), table(gpu / sp)
-        assert sa.shape == sb.shape == (144, 7 if sp == "sp_optics.dat" else 6)
+        assert sa.shape == sb.shape == (n, 7 if sp == "sp_optics.dat" else 6)
         assert np.abs(sa[:, 0] - sb[:, 0]).max() <= 1.00001e-4
         ka, kb = np.lexsort(sa.T[::-1]), np.lexsort(sb.T[::-1])
         assert np.abs(sa[ka] - sb[kb]).max() <= 1.00001e-4
@@ -281,3 +281,78 @@ def test_dropin_post_processing_and_single_particle_spectra(tmp_path, p):
     # reference program): written, one row per frequency
     for f in ("sp_refractive_index.dat", "sp_absorption_coef-pol.dat"):
         assert table(gpu / f).shape[0] == 1001, f
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+@pytest.mark.parametrize("sysdim", ["2D", "1D"])
+def test_dropin_lifetimes_by_dimension(tmp_path, sysdim):
+    """SYSDIM= 2D / 1D: the lifetime formulas of exclft's other branches (bse_subs.f90:
+    472-530) from the drop-in's GPU entry point, against the reference program; and the
+    drop-in's spoptprop tables against the oracle on its own sp_diel spectra."""
+    import wtb_oracle as O
+    p = dict(P, ngrid=(4, 4, 1), sysdim=sysdim)
+    model = tbmodel.hex2d(71, 8, p["ngrid"])
+    tbmodel.write_tb(model, tmp_path / "tb.dat")
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    ref, gpu = tmp_path / "ref", tmp_path / "gpu"
+    assert (ref / "exciton_lifetime.dat").exists(), rr.stdout + rr.stderr
+    for suf in ("", "-pol"):
+        rows = same_rows(ref / f"bse_opt_diel{suf}.dat", gpu / f"bse_opt_diel{suf}.dat")
+        assert rows.mean() > 0.8
+        la = np.array((ref / f"exciton_lifetime{suf}.dat").read_text().splitlines()[1:])
+        lb = np.array((gpu / f"exciton_lifetime{suf}.dat").read_text().splitlines()[1:])
+        assert len(la) == len(lb) == 64
+        assert (la[rows] == lb[rows]).all(), suf
+    # spoptprop on the drop-in's own sp_diel files (the reference program stops before it)
+    cols = ["xx", "yy", "zz", "xy", "xz", "yz"]
+    d = {c: table(gpu / f"sp_diel_{c}.dat") for c in cols}
+    want = O.optprop(d["xx"][:, 0], np.stack([d[c][:, 1] for c in cols], 1), np.stack([d[c][:, 2] for c in cols], 1))
+    for kind, w in zip(("refractive_index", "extinction_coef", "reflectibility", "absorption_coef",
+                        "en_loss_func"), want):
+        t = table(gpu / f"sp_{kind}.dat")
+        assert np.array_equal(np.isnan(t[:, 1:]), np.isnan(w)), kind
+        ok = ~np.isnan(w)
+        assert np.abs(t[:, 1:][ok] - w[ok]).max() <= 0.5e-4 + 1e-9 * np.abs(w[ok]).max(), kind
+
+
+def sp_model(seed, nw_up, nw_dn, ngrid):
+    """An "SP" TB file's two blocks: two independent seeded models on the same lattice."""
+    up = tbmodel.hex2d(seed, nw_up, ngrid)
+    dn = tbmodel.hex2d(seed + 1, nw_dn, ngrid)
+    return up, dn
+
+
+@pytest.mark.skipif(not (REF.exists() and GPU.exists()), reason="reference or drop-in driver not built")
+def test_dropin_spin_polarised_tb_file(tmp_path):
+    """systype "SP": the reference reader concatenates the spin-up and spin-down blocks
+    (module_input_read.f90:56-142) and the GPU path solves the block model.  The spin-down
+    bands carry no weight on orbital 1, so their phase (and with it the activities of the
+    excitons built on them) is fixed by LAPACK's Householder path, not by the gauge rule;
+    what is pinned is what does not depend on it: the exciton energies, the activity sums
+    per component (sum_n f_n = sum_j |h_j|^2) and the single-particle table.  The Python
+    reader gives the same block model (the GPU energies from it match the drop-in's)."""
+    from wantibexos_amd import bse
+    up, dn = sp_model(81, 6, 6, (4, 4, 1))
+    tbmodel.write_tb_sp(up, dn, tmp_path / "tb.dat")
+    p = dict(P, ngrid=(4, 4, 1))
+    rr = run(REF, tmp_path, "ref", p)
+    rg = run(GPU, tmp_path, "gpu", p)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    assert (tmp_path / "ref" / "bse_opt_diel.dat").exists(), rr.stdout + rr.stderr
+    a = table(tmp_path / "ref" / "bse_opt_diel.dat")
+    b = table(tmp_path / "gpu" / "bse_opt_diel.dat")
+    N = 16 * 4
+    assert a.shape == b.shape == (N, 7)
+    assert np.abs(a[:, 0] - b[:, 0]).max() <= 1.00001e-4
+    for c in (1, 2, 3):
+        assert abs(a[:, c].sum() - b[:, c].sum()) <= 1e-4 * N + 1e-6 * abs(a[:, c].sum()), c
+    check_sp_tables(tmp_path / "ref", tmp_path / "gpu", False, n=N)
+    m = tbmodel.read_tb(tmp_path / "tb.dat")
+    assert m.nw == 12 and m.nvec == up.nvec + dn.nvec
+    prm = bse.BSEParams(ngrid=(4, 4, 1), nc=2, nv=2, coultype="V2DRK", r0=40.0, ktol=0.001, cshift=0.05,
+                        ebse0=0.0, ebsef=6.0, numbse=1001, dtfull=True, cpol=True)
+    with bse.Context(0) as ctx:
+        r = bse.bsesolver(ctx, m, prm)
+    assert np.abs(np.round(r["W"], 4) - b[:, 0]).max() <= 1.00001e-4

@@ -1,0 +1,196 @@
+"""GPU tests at the full sizes of the benchmark configurations (seeded synthetic models):
+
+  C2  hex2d nw=22, 36x36x1, nv=nc=2 (N = 5184): plain FP64 and precision forms
+  C3  cubic3d nw=16, 16x16x16, nv=nc=3 (N = 36864): precision form, iterative form
+  C4  64 hex2d models, nw 6..40, k grids 12^2..30^2 (N = 576..3600): batching form
+  C5  hex2d nw=24, 60x60x1 (N = 14400): iterative form (lowest 256, tol 1e-6 eV)
+
+At these sizes the oracle is too slow, so the checks are the size-independent ones the
+problem fixes: sum W = trace H (the diagonal is E_c - E_v + W(0), matrizelbse i == j),
+the activity sum rules sum_n f_n^{ab} = Re sum_j h_a(j) conj(h_b(j)) (the eigenvector
+matrix is unitary), ascending energies, bitwise agreement between the batch and single
+plans, and the faster forms against the plain FP64 solve of the same matrix: exciton
+energies within 1 meV and eps(omega) within 1e-3 of each component's maximum.
+The models are generated on the host by child processes (tests/gen_models.py)."""
+import dataclasses
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+import gen_models  # noqa: E402
+from wantibexos_amd import bse  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BASE = dict(nc=2, nv=2, coultype="V2DRK", r0=40.0, ktol=0.001, cshift=0.05, ebse0=0.0, ebsef=6.0,
+            numbse=1001, dtfull=True, cpol=True, renorm=True, f154=True)
+E_TOL = 1e-3      # eV
+EPS_TOL = 1e-3    # of each component's maximum
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = bse.Context(0)
+    yield c
+    c.close()
+
+
+def prm(ngrid, precision, **kw):
+    d = dict(BASE, **kw)
+    return bse.BSEParams(ngrid=ngrid, precision=bse.MIXED if precision == "mixed" else bse.FP64, **d)
+
+
+def trace_of(plan):
+    """sum of the diagonal E_c - E_v + W(0) over the basis (k outer, c, v inner)"""
+    p = plan.params
+    ev = plan.fetch("eigv")                        # [k][nv + nc]: valence then conduction
+    w0 = plan.fetch("wtab")[p.ngrid[0] - 1, p.ngrid[1] - 1, p.ngrid[2] - 1]
+    de = ev[:, p.nv:, None] - ev[:, None, :p.nv]   # [k][c][v]
+    return float(de.sum() + plan.N * w0)
+
+
+def sum_rules(plan, act):
+    h = plan.fetch("dipole")                       # h_x, h_y, h_z
+    want = [np.sum(np.abs(h[c]) ** 2) for c in range(3)]
+    want += [np.sum(h[a] * h[b].conj()).real for a, b in ((0, 1), (0, 2), (1, 2))]
+    return max(abs(act[c].sum() - want[c]) / max(abs(want[c]), np.abs(h).max() ** 2 * 1e-6) for c in range(6))
+
+
+def eps_err(got, ref):
+    out = {}
+    for c in range(6):
+        for q in range(2):
+            r = ref[c, q]
+            if np.abs(r).max() > 1e-8:
+                out[f"{c}{'ri'[q]}"] = float(np.abs(got[c, q] - r).max() / np.abs(r).max())
+    return out
+
+
+def check_invariants(plan, r, name):
+    """trace to 1e-10 (FP64) or 1e-7 (precision form: the COMPLEX*8 matrix's diagonal is
+    rounded at every trailing update); one dropped diagonal element is ~1/N ~ 3e-5"""
+    W = r["W"]
+    tr = trace_of(plan)
+    print(f"{name}: N={plan.N} trace rel {abs(W.sum() - tr) / abs(tr):.2e} sum rules {sum_rules(plan, r['act']):.2e}",
+          flush=True)
+    assert np.all(np.diff(W) >= 0), name
+    tol = 1e-7 if plan.params.precision == bse.MIXED else 1e-10
+    assert abs(W.sum() - tr) <= tol * abs(tr), name
+    assert sum_rules(plan, r["act"]) < 1e-9, name
+
+
+@pytest.fixture(scope="module")
+def c2_model():
+    return gen_models.generate([("hex2d", 2, 22, (36, 36, 1))])[0]
+
+
+def test_c2_plain_and_precision_forms(ctx, c2_model):
+    print("C2: hex2d nw=22 36x36x1 N=5184, FP64 then precision form", flush=True)
+    with bse.BSEPlan(ctx, c2_model, prm((36, 36, 1), "fp64")) as plan:
+        plan.execute()
+        ref = plan.results()
+        check_invariants(plan, ref, "C2 fp64")
+    with bse.BSEPlan(ctx, c2_model, prm((36, 36, 1), "mixed")) as plan:
+        plan.execute()
+        got = plan.results()
+        check_invariants(plan, got, "C2 mixed")
+    dW = np.abs(got["W"] - ref["W"])
+    ee = eps_err(got["eps"], ref["eps"])
+    print(f"C2 precision vs FP64: max dW {dW.max():.2e} eV, lowest 256 {dW[:256].max():.2e} eV, eps {ee}", flush=True)
+    assert dW.max() <= E_TOL
+    assert max(ee.values()) <= EPS_TOL
+
+
+@pytest.fixture(scope="module")
+def c5_model():
+    return gen_models.generate([("hex2d", 5, 24, (60, 60, 1))])[0]
+
+
+def test_c5_iterative_form(ctx, c5_model):
+    print("C5: hex2d nw=24 60x60x1 N=14400, iterative form (256, tol 1e-6) vs full FP64", flush=True)
+    g = (60, 60, 1)
+    with bse.BSEPlan(ctx, c5_model, dataclasses.replace(prm(g, "fp64"), f154=False)) as plan:
+        plan.execute()
+        ref = plan.results()
+        check_invariants(plan, ref, "C5 fp64")
+    with bse.BSEPlan(ctx, c5_model, dataclasses.replace(prm(g, "mixed"), f154=False)) as plan:
+        r = plan.lowest(nev=256, tol=1e-6, lanczos_steps=250)
+    dW = np.abs(r["W"] - ref["W"][:256])
+    ee = eps_err(r["eps"], ref["eps"])
+    print(f"C5: {r['info']} max dW {dW.max():.2e} eV, max resid {r['resid'].max():.2e}, eps {ee}", flush=True)
+    assert r["info"]["converged"] == 1 and r["info"]["used_fallback"] == 0
+    assert r["resid"].max() <= 1e-6
+    assert dW.max() <= E_TOL
+    assert max(ee.values()) <= EPS_TOL
+
+
+def c4_spec():
+    """the 64-model set: nw uniform in 6..40, k grids uniform in 12x12..30x30"""
+    rng = np.random.default_rng(4)
+    return [(int(rng.integers(12, 31)), int(rng.integers(6, 41))) for _ in range(64)]
+
+
+def test_c4_batch_of_64_models(ctx):
+    spec = c4_spec()
+    print(f"C4: 64 hex2d models, generating", flush=True)
+    models = gen_models.generate([("hex2d", 200 + i, nw, (ng, ng, 1)) for i, (ng, nw) in enumerate(spec)])
+    prms = [prm((ng, ng, 1), "mixed") for ng, _ in spec]
+    ref = []
+    for i, (m, p) in enumerate(zip(models, prms)):
+        with bse.BSEPlan(ctx, m, p) as plan:
+            plan.execute()
+            r = plan.results()
+            check_invariants(plan, r, f"C4 model {i} (k {spec[i][0]}^2, nw {spec[i][1]})")
+            ref.append(r)
+    print("C4: the batch on 4 lanes", flush=True)
+    with bse.BSEBatch(ctx, models, prms, lanes=4) as b:
+        b.execute()
+        for i, r in enumerate(ref):
+            got = b.results(i)
+            assert np.array_equal(got["W"], r["W"]), i
+            assert np.array_equal(got["act"], r["act"]), i
+            assert np.array_equal(got["eps"], r["eps"]), i
+    # the per-k eigensolver at the set's largest Wannier count against LAPACK (numpy eigh)
+    import wtb_oracle as O
+    i40 = max(range(64), key=lambda i: spec[i][1])
+    m = models[i40]
+    g = (spec[i40][0],) * 2 + (1,)
+    kpt = bse.monkhorst_pack(ctx, g, (0.0, 0.0, 0.0), m.rlat)
+    E, U, nocp = bse.eigsys(ctx, m, kpt)
+    Hk = O.hk_stack(kpt, m)                        # E_F = 0 and no scissor in these models
+    want = np.linalg.eigvalsh(Hk)
+    print(f"C4 eigsys nw={m.nw} nk={len(kpt)}: max |dE| {np.abs(E - want).max():.2e} eV", flush=True)
+    assert np.abs(E - want).max() < 1e-10
+    resid = np.abs(np.einsum("kmn,kin->kim", Hk, U) - E[:, :, None] * U).max()
+    assert resid < 1e-10
+
+
+def test_c3_bulk_precision_and_iterative_forms(ctx):
+    print("C3: cubic3d nw=16 16^3 nv=nc=3 N=36864: generating", flush=True)
+    m = gen_models.generate([("cubic3d", 3, 16, (16, 16, 16))])[0]
+    g = (16, 16, 16)
+    kw = dict(nc=3, nv=3, coultype="V3DL", ediel=(6.0, 6.0, 6.0))
+    print("C3: precision form", flush=True)
+    with bse.BSEPlan(ctx, m, prm(g, "mixed", **kw)) as plan:
+        plan.execute()
+        got = plan.results()
+        check_invariants(plan, got, "C3 mixed")
+    print("C3: iterative form", flush=True)
+    with bse.BSEPlan(ctx, m, dataclasses.replace(prm(g, "mixed", **kw), f154=False)) as plan:
+        it = plan.lowest(nev=256, tol=1e-6, lanczos_steps=250)
+    print("C3: plain FP64", flush=True)
+    with bse.BSEPlan(ctx, m, dataclasses.replace(prm(g, "fp64", **kw), f154=False)) as plan:
+        plan.execute()
+        ref = plan.results()
+        check_invariants(plan, ref, "C3 fp64")
+    dWm = np.abs(got["W"] - ref["W"]).max()
+    dWi = np.abs(it["W"] - ref["W"][:256]).max()
+    ei = eps_err(it["eps"], ref["eps"])
+    print(f"C3: precision form max dW {dWm:.2e} eV; iterative {it['info']} max dW {dWi:.2e} eV eps {ei}", flush=True)
+    assert dWm <= E_TOL
+    assert it["info"]["converged"] == 1 and it["resid"].max() <= 1e-6
+    assert dWi <= E_TOL
+    assert max(ei.values()) <= EPS_TOL

@@ -6,6 +6,7 @@ with flang + MKL) on seeded synthetic models.  Closed-form checks cover what
 a plausible slip (wrong constant, sign, index or missing term) would break.
 """
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -227,3 +228,39 @@ def test_fermi_factor_closed_forms():
     assert O.gap_correction("VE", 0.5, 0.03, 0.0) == 0.0
     assert O.gap_correction("BE", 0.5, 200.0, 0.0) == 0.0
     assert O.KB_F32 == float(np.float32(8.617330350e-5))
+
+
+def test_rkmesh_pinned_to_reference():
+    """rkmesh / rkmesh2D (MESH_TYPE= RK3D / RK2D, mhkpack_subs.f90:1-52) against the
+    reference's own routines on 162 cases (tests/golden/rkmesh.json), including
+    densities within 1e-6 of a rounding boundary."""
+    import json
+    data = json.loads((Path(__file__).resolve().parent / "golden" / "rkmesh.json").read_text())["cases"]
+    for c in data:
+        L = np.array(c["rlat"]).reshape(3, 3)
+        assert O.rkmesh(c["rk"], L) == c["rk3d"], c
+        assert O.rkmesh2D(c["rk"], L) == c["rk2d"], c
+
+
+def test_rkmesh_closed_form_and_planted_mistakes():
+    """Closed form on orthogonal cells (|b_i| / 2 pi = 1 / a_i).  The single-precision
+    pi enters recvec and the division alike and cancels; the fixture catches a dropped
+    +1/2 and the direct-lattice length 1 / |a_i| in place of |b_i| / 2 pi on oblique cells."""
+    import json
+    import math
+    for a, rk in [(5.4, 40.0), (3.0, 30.0), (4.5, 7.0), (2.0, 0.5)]:
+        L = np.diag([a, a * 1.5, a * 3.0])
+        want = [int(max(1.0, rk / (a * f) + 0.5)) for f in (1.0, 1.5, 3.0)]
+        assert O.rkmesh(rk, L) == want
+        assert O.rkmesh2D(rk, L) == want[:2] + [1]
+    data = json.loads((Path(__file__).resolve().parent / "golden" / "rkmesh.json").read_text())["cases"]
+
+    def direct_length(rk, L):
+        return [int(max(1.0, rk / np.linalg.norm(L[i]) + 0.5)) for i in range(3)]
+
+    def no_half(rk, L):
+        b = O.recvec(L)
+        return [int(max(1.0, rk * O.vecsize(b[i]) / (2 * O.PI32))) for i in range(3)]
+
+    for wrong in (direct_length, no_half):
+        assert any(wrong(c["rk"], np.array(c["rlat"]).reshape(3, 3)) != c["rk3d"] for c in data), wrong.__name__

@@ -146,6 +146,8 @@ def _load():
         "wtb_bse_bands_set_vectors": (ip, [vp, ip, ip]),
         "wtb_bse_bands_run_wf": (ip, [vp, ip, vp, vp, vp]),
         "wtb_bsebnds_wf": (ip, [vp, vp, vp, ip, vp, ip, vp, ip, ip, vp, vp, vp]),
+        "wtb_optprop": (ip, [vp, ip, ip, vp, vp, vp, vp]),
+        "wtb_exciton_lifetime": (ip, [vp, ip, vp, vp, ip, ip, ip, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -330,6 +332,34 @@ def bse_spectrum(ctx: Context, e, act6, rlat, ngrid, ebse0, ebsef, numbse, sme, 
     _check(lib().wtb_bse_spectrum(ctx.h, len(e), _p(e), _p(a6), _p(rl), _p(ng), float(ebse0),
                                   float(ebsef), int(numbse), float(sme), int(bool(renorm)), _p(eps)))
     return eps
+
+
+def optprop(ctx: Context, energy, eps_re, eps_im):
+    """bseoptprop / spoptprop and their -pol forms (diel-pp-bse.f90:3-173): eps_re, eps_im
+    (n, ncomp) -> (n, k, R, alpha, loss), each (n, ncomp)."""
+    en = np.ascontiguousarray(energy, dtype=np.float64)
+    er = np.asfortranarray(eps_re, dtype=np.float64)
+    ei = np.asfortranarray(eps_im, dtype=np.float64)
+    n, nc = er.shape
+    out = np.zeros((n, nc, 5), order="F")
+    _check(lib().wtb_optprop(ctx.h, n, nc, _p(en), _p(er), _p(ei), _p(out)))
+    return tuple(out[:, :, q] for q in range(5))
+
+
+SYSDIM = {"1D": 1, "2D": 2, "3D": 3}
+
+
+def exciton_lifetime(ctx: Context, sysdim: str, ngrid, rlat, table, nout: int):
+    """lifetime (nout = 3) / lifetimepol (nout = 5), exciton_lifetime.f90:2-187: table
+    (N, 1 + ncomp) as bse_opt_diel(-pol).dat holds it -> (N, 1 + nout) lifetimes (s)."""
+    t = np.asfortranarray(table, dtype=np.float64)
+    n, ncol = t.shape
+    rl = np.asfortranarray(np.asarray(rlat, dtype=np.float64))
+    ng = np.array(ngrid, dtype=np.int32)
+    out = np.zeros((n, nout + 1), order="F")
+    _check(lib().wtb_exciton_lifetime(ctx.h, SYSDIM.get(sysdim, 0), _p(ng), _p(rl), n, ncol - 1, nout, _p(t),
+                                      _p(out)))
+    return out
 
 
 class BSEPlan:

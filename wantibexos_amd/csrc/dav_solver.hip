@@ -135,9 +135,98 @@ int small_heev(hipStream_t s, SmallEig& e, int m, const zc* G, int ldg, double* 
 
 }  // namespace
 
+struct DavidsonWork {
+  int N = 0, nk = 0, msub = 0, M = 0;   // sizes the buffers were allocated for
+  zc *V = nullptr, *HV = nullptr, *G = nullptr, *S = nullptr, *U = nullptr, *HU = nullptr, *R = nullptr,
+     *T = nullptr, *T2 = nullptr, *P = nullptr, *Gb = nullptr, *Linv = nullptr;
+  double *theta = nullptr, *rn = nullptr;
+  int *idx = nullptr, *keep = nullptr, *map = nullptr;
+  zc* hw = nullptr;
+  zc* gw = nullptr;
+  int* crow = nullptr;
+  zc *QS = nullptr, *CS = nullptr, *CY = nullptr, *CZ = nullptr;
+  double* lamS = nullptr;
+  SmallEig se, sc;
+  bool have = false;
+
+  void release() {
+    for (void* p : {(void*)V, (void*)HV, (void*)G, (void*)S, (void*)U, (void*)HU, (void*)R, (void*)T,
+                    (void*)T2, (void*)P, (void*)Gb, (void*)Linv, (void*)theta, (void*)rn, (void*)idx,
+                    (void*)keep, (void*)map, (void*)hw, (void*)gw, (void*)crow, (void*)QS, (void*)CS,
+                    (void*)CY, (void*)CZ, (void*)lamS})
+      hipFree(p);
+    small_free(se);
+    small_free(sc);
+    *this = DavidsonWork();
+  }
+  // buffers for an N x N matrix, nk Ritz pairs, a basis of msub and a coarse block of M;
+  // kept when the held sizes cover the request, else reallocated for the larger of both
+  int reserve(int n, int k, int ms, int m) {
+    if (have && n == N && k <= nk && ms <= msub && m <= M) return WTB_OK;
+    const int n2 = n, k2 = (have && n == N) ? std::max(k, nk) : k;
+    const int ms2 = (have && n == N) ? std::max(ms, msub) : ms;
+    const int m2 = (have && n == N) ? std::max(m, M) : m;
+    release();
+    N = n2;
+    nk = k2;
+    msub = ms2;
+    M = m2;
+    have = true;
+    bool ok = true;
+    auto take = [&](auto*& ptr, size_t bytes) {
+      if (ok && hipMalloc((void**)&ptr, bytes) != hipSuccess) {
+        ptr = nullptr;
+        ok = false;
+      }
+    };
+    take(V, sizeof(zc) * (size_t)N * msub);
+    take(HV, sizeof(zc) * (size_t)N * msub);
+    take(G, sizeof(zc) * (size_t)msub * msub);
+    take(S, sizeof(zc) * (size_t)msub * nk);
+    take(U, sizeof(zc) * (size_t)N * nk);
+    take(HU, sizeof(zc) * (size_t)N * nk);
+    take(R, sizeof(zc) * (size_t)N * nk);
+    take(T, sizeof(zc) * (size_t)N * nk);
+    take(T2, sizeof(zc) * (size_t)N * nk);
+    take(P, sizeof(zc) * (size_t)msub * nk);
+    take(Gb, sizeof(zc) * (size_t)nk * nk);
+    take(Linv, sizeof(zc) * (size_t)nk * nk);
+    take(theta, sizeof(double) * msub);
+    take(rn, sizeof(double) * nk);
+    take(idx, sizeof(int) * nk);
+    take(keep, sizeof(int) * nk);
+    take(map, sizeof(int) * nk);
+    take(hw, hmm_work_bytes(N, nk));
+    take(gw, sizeof(zc) * (size_t)8 * msub * nk);
+    if (M > 0) {
+      take(crow, sizeof(int) * M);
+      take(QS, sizeof(zc) * (size_t)M * M);
+      take(CS, sizeof(zc) * (size_t)M * std::max(M, nk));
+      take(CY, sizeof(zc) * (size_t)M * nk);
+      take(CZ, sizeof(zc) * (size_t)M * nk);
+      take(lamS, sizeof(double) * M);
+    }
+    if (ok && small_alloc(se, msub, nk) != WTB_OK) ok = false;
+    if (ok && M > 0 && small_alloc(sc, M, M) != WTB_OK) ok = false;
+    if (!ok) {
+      release();
+      set_error("davidson: device allocation failed");
+      return WTB_ERR_NOMEM;
+    }
+    return WTB_OK;
+  }
+};
+
+DavidsonWork* davidson_work_new() { return new DavidsonWork(); }
+void davidson_work_delete(DavidsonWork* w) {
+  if (!w) return;
+  w->release();
+  delete w;
+}
+
 int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, const float* stats,
                  const DavidsonCfg& cfg, const MlpDev* mlp, double* d_W, double* d_resid, zc* d_X,
-                 DavidsonInfo& info) {
+                 DavidsonInfo& info, DavidsonWork& work) {
   const int nev = cfg.nev;
   const int nk = std::min(N, nev + cfg.guard);
   const int msub = std::min(N, std::max(cfg.max_sub, 2 * nk));
@@ -146,70 +235,25 @@ int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, co
     return WTB_ERR_ARG;
   }
   info = DavidsonInfo();
-  // device buffers
-  zc *V = nullptr, *HV = nullptr, *G = nullptr, *S = nullptr, *U = nullptr, *HU = nullptr, *R = nullptr,
-     *T = nullptr, *T2 = nullptr, *P = nullptr, *Gb = nullptr, *Linv = nullptr;
-  double *theta = nullptr, *rn = nullptr;
-  int *idx = nullptr, *keep = nullptr, *map = nullptr;
-  zc* hw = nullptr;
-  zc* gw = nullptr;   // K-slice partials of the projections V^H X
+  const int M = std::min(N, std::max(0, cfg.coarse));
+  const size_t gws = (size_t)8 * msub * nk;
+  int rc = work.reserve(N, nk, msub, M);
+  if (rc) return rc;
+  // the buffers, named as the algorithm above names them
+  zc *V = work.V, *HV = work.HV, *G = work.G, *S = work.S, *U = work.U, *HU = work.HU, *R = work.R,
+     *T = work.T, *T2 = work.T2, *P = work.P, *Gb = work.Gb, *Linv = work.Linv;
+  double *theta = work.theta, *rn = work.rn;
+  int *idx = work.idx, *keep = work.keep, *map = work.map;
+  zc* hw = work.hw;
+  zc* gw = work.gw;   // K-slice partials of the projections V^H X
   // coarse block of the preconditioner: rows, eigenvectors QS and eigenvalues lamS of
   // H(rows, rows), and three M x nk work blocks
-  const int M = std::min(N, std::max(0, cfg.coarse));
-  int* crow = nullptr;
-  zc *QS = nullptr, *CS = nullptr, *CY = nullptr, *CZ = nullptr;
-  double* lamS = nullptr;
-  SmallEig sc;
-  const size_t gws = (size_t)8 * msub * nk;
-  SmallEig se;
-  int rc = WTB_OK;
-  auto cleanup = [&]() {
-    for (void* p : {(void*)V, (void*)HV, (void*)G, (void*)S, (void*)U, (void*)HU, (void*)R, (void*)T,
-                    (void*)T2, (void*)P, (void*)Gb, (void*)Linv, (void*)theta, (void*)rn, (void*)idx,
-                    (void*)keep, (void*)map, (void*)hw, (void*)gw, (void*)crow, (void*)QS, (void*)CS,
-                    (void*)CY, (void*)CZ, (void*)lamS})
-      hipFree(p);
-    small_free(se);
-    if (M > 0) small_free(sc);
-  };
-#define DAV_ALLOC(ptr, bytes)                                  \
-  if (hipMalloc(&ptr, bytes) != hipSuccess) {                  \
-    set_error("davidson: device allocation failed");           \
-    cleanup();                                                 \
-    return WTB_ERR_NOMEM;                                      \
-  }
-  DAV_ALLOC(V, sizeof(zc) * (size_t)N * msub);
-  DAV_ALLOC(HV, sizeof(zc) * (size_t)N * msub);
-  DAV_ALLOC(G, sizeof(zc) * (size_t)msub * msub);
-  DAV_ALLOC(S, sizeof(zc) * (size_t)msub * nk);
-  DAV_ALLOC(U, sizeof(zc) * (size_t)N * nk);
-  DAV_ALLOC(HU, sizeof(zc) * (size_t)N * nk);
-  DAV_ALLOC(R, sizeof(zc) * (size_t)N * nk);
-  DAV_ALLOC(T, sizeof(zc) * (size_t)N * nk);
-  DAV_ALLOC(T2, sizeof(zc) * (size_t)N * nk);
-  DAV_ALLOC(P, sizeof(zc) * (size_t)msub * nk);
-  DAV_ALLOC(Gb, sizeof(zc) * (size_t)nk * nk);
-  DAV_ALLOC(Linv, sizeof(zc) * (size_t)nk * nk);
-  DAV_ALLOC(theta, sizeof(double) * msub);
-  DAV_ALLOC(rn, sizeof(double) * nk);
-  DAV_ALLOC(idx, sizeof(int) * nk);
-  DAV_ALLOC(keep, sizeof(int) * nk);
-  DAV_ALLOC(map, sizeof(int) * nk);
-  DAV_ALLOC(hw, hmm_work_bytes(N, nk));
-  DAV_ALLOC(gw, sizeof(zc) * gws);
-  if (M > 0) {
-    DAV_ALLOC(crow, sizeof(int) * M);
-    DAV_ALLOC(QS, sizeof(zc) * (size_t)M * M);
-    DAV_ALLOC(CS, sizeof(zc) * (size_t)M * std::max(M, nk));
-    DAV_ALLOC(CY, sizeof(zc) * (size_t)M * nk);
-    DAV_ALLOC(CZ, sizeof(zc) * (size_t)M * nk);
-    DAV_ALLOC(lamS, sizeof(double) * M);
-  }
-#undef DAV_ALLOC
-  if ((rc = small_alloc(se, msub, nk)) || (M > 0 && (rc = small_alloc(sc, M, M)))) {
-    cleanup();
-    return rc;
-  }
+  int* crow = work.crow;
+  zc *QS = work.QS, *CS = work.CS, *CY = work.CY, *CZ = work.CZ;
+  double* lamS = work.lamS;
+  SmallEig& se = work.se;
+  SmallEig& sc = work.sc;
+  auto cleanup = [&]() {};
 #define DAV_TRY(call)      \
   if ((rc = (call))) {     \
     cleanup();             \

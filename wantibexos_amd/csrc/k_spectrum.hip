@@ -323,4 +323,97 @@ int launch_sp_table(hipStream_t s, int N, int nk, int nc, int nv, const double* 
   return WTB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Post-processing of the spectra and the exciton lifetimes, element-wise.
+// ---------------------------------------------------------------------------
+
+// Optical constants of one (photon energy, component) from eps = e1 + i e2
+// (subroutines/diel-pp-subs.f90:1-73, called by bseoptprop, subprograms/diel-pp-bse.f90:
+// 89-140, and its spoptprop / -pol variants), in the reference's operation order and with
+// its single-precision literals (sqrt(2.0) and hc = 19.746E-06 are REAL):
+//   n = sqrt((|eps| + e1)/2), k = sqrt((|eps| - e1)/2), R = ((n-1)^2 + k^2)/((n+1)^2 + k^2),
+//   alpha = sqrt(2) E (1/hc) sqrt(|eps| - e1), L = e2/|eps|^2.
+// out[q][c][i], q = 0..4 for n, k, R, alpha, L.
+__global__ __launch_bounds__(256) void k_optprop(int n, int nc, const double* __restrict__ en,
+                                                 const double* __restrict__ er, const double* __restrict__ ei,
+                                                 double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || c >= nc) return;
+  const double e1 = er[(size_t)c * n + i], e2 = ei[(size_t)c * n + i];
+  const double aux = sqrt((e1 * e1) + (e2 * e2));
+  const double refr = sqrt((aux + e1) / 2.0);
+  const double ext = sqrt((aux - e1) / 2.0);
+  const double a1 = (refr - 1.0) * (refr - 1.0);
+  const double a2 = (refr + 1.0) * (refr + 1.0);
+  const double a3 = ext * ext;
+  const double refl = (a1 + a3) / (a2 + a3);
+  const double sqrt2f = (double)sqrtf(2.0f);
+  const double hc = (double)19.746E-06f;
+  const double absc = ((sqrt2f * en[i]) * (1.0 / hc)) * sqrt(aux - e1);
+  const double eloss = e2 / ((e1 * e1) + (e2 * e2));
+  const size_t plane = (size_t)nc * n, at = (size_t)c * n + i;
+  out[at] = refr;
+  out[plane + at] = ext;
+  out[2 * plane + at] = refl;
+  out[3 * plane + at] = absc;
+  out[4 * plane + at] = eloss;
+}
+
+// exclft (subroutines/bse_subs.f90:472-530): radiative lifetime (s) of an exciton of
+// energy E and oscillator strength f for sysdim 3 (PRL 95, 247402), 2 (Nano Lett. 15, 2794)
+// or 1, with the REAL literals of the reference (cfe 7.297E-3, lspd 299.792E16, hbar
+// 6.582E-16, pi = acos(-1.)) held in doubles.  nkd = dble(n1 n2 n3), vc2d = |a1 x a2|,
+// a11 = rlat(1,1).
+__device__ double exclft_dev(int sysdim, double nkd, double vc2d, double a11, double f, double E) {
+#pragma clang fp contract(off)
+  const double cfe = (double)7.297E-3f, lspd = (double)299.792E16f, hbar = (double)6.582E-16f;
+  const double pi = (double)3.14159274101257324;
+  if (sysdim == 3) {
+    const double aux1 = (4.0 * cfe) * ((E * E) * E);
+    const double aux2 = f / nkd;
+    const double aux3 = (3.0 * (lspd * lspd)) * ((hbar * hbar) * hbar);
+    const double l = (aux1 * aux2) / aux3;
+    return 1.0 / l;
+  } else if (sysdim == 2) {
+    const double aux1 = (nkd * vc2d) * hbar;
+    const double aux2 = (((8.0 * pi) * cfe) * E) * f;
+    return aux1 / aux2;
+  }
+  const double aux2 = ((((2.0 * pi) * cfe) * E) * E) * f;
+  const double aux1 = (((nkd * a11) * hbar) * hbar) * lspd;
+  return aux1 / aux2;
+}
+
+// lifetime / lifetimepol (subprograms/exciton_lifetime.f90:2-92, 95-187): per exciton i
+// of table fosc(n, 1 + ncomp) (energy, then the activities as bse_opt_diel.dat holds
+// them: xx, yy, zz, ... or x, y, z, sp, sm), lft(n, 1 + nout): first the lifetime of the
+// summed strength f_1 + f_2 + f_3, then one per component 1..nout.
+__global__ __launch_bounds__(256) void k_lifetime(int n, int ncol, int nout, int sysdim, double nkd, double vc2d,
+                                                  double a11, const double* __restrict__ fosc,
+                                                  double* __restrict__ lft) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double E = fosc[i];
+  const double fsum = (fosc[(size_t)n + i] + fosc[(size_t)2 * n + i]) + fosc[(size_t)3 * n + i];
+  lft[i] = exclft_dev(sysdim, nkd, vc2d, a11, fsum, E);
+  for (int c = 1; c <= nout; ++c) lft[(size_t)c * n + i] = exclft_dev(sysdim, nkd, vc2d, a11, fosc[(size_t)c * n + i], E);
+}
+
+int launch_optprop(hipStream_t s, int n, int nc, const double* en, const double* er, const double* ei, double* out) {
+  hipLaunchKernelGGL(k_optprop, dim3((n + 255) / 256, nc), dim3(256), 0, s, n, nc, en, er, ei, out);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int launch_lifetime(hipStream_t s, int n, int ncol, int nout, int sysdim, double nkd, double vc2d, double a11,
+                    const double* fosc, double* lft) {
+  hipLaunchKernelGGL(k_lifetime, dim3((n + 255) / 256), dim3(256), 0, s, n, ncol, nout, sysdim, nkd, vc2d, a11,
+                     fosc, lft);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
 }  // namespace wtb

@@ -80,6 +80,7 @@ struct wtb_bse_plan {
   wtb_ctx* ctx = nullptr;
   wtb_bse_params prm{};
   bool filled = false;           // the k grid, bands and dipoles have been computed (build/execute)
+  bool sp_only = false;          // single-particle table only: no BSE matrix or solver buffers
   int nk = 0, nw = 0, nvec = 0, nb = 0, N = 0;
   double ef32 = 0, scs = 0, sme32 = 0, vc3d = 0, nkf32 = 0;
   // finite temperature: gap correction (added to scs), 1/(kB T) of the Fermi factors,
@@ -144,6 +145,17 @@ struct wtb_bse_plan {
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev[WTB_NSTAGES + 1] = {};
   double stage_ms[WTB_NSTAGES] = {};
+  // iterative form: device buffers kept between calls of wtb_bse_plan_lowest (allocated at
+  // its first call, grown when a later call needs more)
+  DavidsonWork* dav = nullptr;
+  double* lw_D = nullptr;       // diagonal of H [N]
+  float* lw_stats = nullptr;    // row statistics [3][N]
+  double* lw_W = nullptr;       // Ritz values [nev]
+  double* lw_R = nullptr;       // residual norms [nev]
+  double* lw_sm = nullptr;      // width and E_max of the spectrum [4]
+  zc* lw_X = nullptr;           // Ritz vectors [nev][N]
+  void* lw_hay = nullptr;       // Lanczos chains
+  size_t lw_nev = 0, lw_xcols = 0, lw_hay_bytes = 0;
 };
 
 extern "C" {
@@ -357,8 +369,10 @@ static void plan_free(wtb_bse_plan* p) {
   void* bufs[] = {p->kpt, p->phase, p->hk4, p->eall, p->nocp, p->eigv, p->vec, p->dip, p->g,
                   p->fdeh, p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
                   p->wq, p->actq, p->eps, p->smeavg, p->status, p->Zv, p->Xv, p->btw,
-                  p->dq, p->kq, p->eigvq, p->vecq, p->nocpq, p->xa, p->dvq};
+                  p->dq, p->kq, p->eigvq, p->vecq, p->nocpq, p->xa, p->dvq,
+                  p->lw_D, p->lw_stats, p->lw_W, p->lw_R, p->lw_sm, p->lw_X, p->lw_hay};
   for (void* b : bufs) hipFree(b);
+  davidson_work_delete(p->dav);
   for (auto& e : p->ev)
     if (e) hipEventDestroy(e);
   for (auto& e : p->kev)
@@ -402,8 +416,19 @@ static double fermi_beta(double temp) {
   return 1.0 / (kKbF32 * t);
 }
 
+// sp_only: a plan for the single-particle table alone (bands, dipoles, Fermi factors):
+// no BSE matrix, reduction or divide-and-conquer buffers, O(N) device memory as the
+// reference's spoptics (sp_solver-tool-diel.f90:137-196)
+static int plan_create_impl(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, wtb_bse_plan** out,
+                            bool sp_only);
+
 int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm,
                         wtb_bse_plan** out) {
+  return plan_create_impl(c, m, prm, out, false);
+}
+
+static int plan_create_impl(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, wtb_bse_plan** out,
+                            bool sp_only) {
   if (!c || !prm || !out) {
     set_error("wtb_bse_plan_create: null argument");
     return WTB_ERR_ARG;
@@ -448,6 +473,7 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
   p->stream = c->stream;
   p->prm = *prm;
   p->mixed = prm->precision == WTB_MIXED;
+  p->sp_only = sp_only;
   p->nk = prm->ngrid[0] * prm->ngrid[1] * prm->ngrid[2];
   p->nw = m->nw;
   p->nvec = m->nvec;
@@ -483,15 +509,24 @@ int wtb_bse_plan_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params*
       (rc = dmalloc(&p->nocp, nk)) || (rc = dmalloc(&p->eigv, (size_t)nk * p->nb)) ||
       (rc = dmalloc(&p->vec, (size_t)nk * p->nb * nw)) || (rc = dmalloc(&p->dip, (size_t)3 * N)) ||
       (rc = dmalloc(&p->g, (size_t)p->nrhs * N)) || (rc = dmalloc(&p->gapk, nk)) ||
-      (rc = prm->bset ? dmalloc(&p->fdeh, N) : WTB_OK) ||
-      (rc = dmalloc(&p->wtab, p->wtab_n)) ||
+      (rc = prm->bset ? dmalloc(&p->fdeh, N) : WTB_OK) || (rc = dmalloc(&p->status, 4))) {
+    plan_free(p);
+    delete p;
+    return rc;
+  }
+  if (sp_only) {
+    for (auto& e : p->ev) hipEventCreate(&e);
+    *out = p;
+    return WTB_OK;
+  }
+  if ((rc = dmalloc(&p->wtab, p->wtab_n)) ||
       (rc = (prm->precision == WTB_MIXED) ? dmalloc(&p->Hf, p->ldh * N) : dmalloc(&p->H, p->ldh * N)) ||
       (rc = dmalloc((char**)&p->triwork, tridiag_work_bytes(N, nb_panel, p->nrhs))) ||
       (rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 2 * p->nrhs))) ||
       (rc = dmalloc(&p->track, (size_t)2 * p->nrhs * N)) || (rc = dmalloc(&p->lambda, N)) ||
       (rc = dmalloc(&p->act, (size_t)8 * N)) || (rc = dmalloc(&p->wq, N)) ||
       (rc = dmalloc(&p->actq, (size_t)6 * N)) || (rc = dmalloc(&p->eps, (size_t)12 * prm->numbse)) ||
-      (rc = dmalloc(&p->smeavg, 4)) || (rc = dmalloc(&p->status, 4))) {
+      (rc = dmalloc(&p->smeavg, 4))) {
     plan_free(p);
     delete p;
     return rc;
@@ -629,6 +664,7 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   // finite temperature: f_v - f_c of every basis state
   if (P.bset && (rc = launch_fermi_factors(s, nk, P.nc, P.nv, p->eigv, p->beta, p->fdeh))) return rc;
   mark(2);
+  if (p->sp_only) return WTB_OK;
   // (c) W table + assembly (lower triangle)
   if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
   if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H,
@@ -992,6 +1028,70 @@ int wtb_bse_spectrum(wtb_ctx* c, int n, const double* e, const double* act6, con
   return rc;
 }
 
+// |a1 x a2| (vcell2D, general_subs.f90:195-206, with prodvec :182-193 and vecsize :286-294)
+static double vcell2d_host(const double* rl) {
+#pragma clang fp contract(off)
+  const double a1[3] = {rl[0], rl[3], rl[6]}, a2[3] = {rl[1], rl[4], rl[7]};
+  const double x = (a1[1] * a2[2]) - (a1[2] * a2[1]);
+  const double y = (a1[2] * a2[0]) - (a1[0] * a2[2]);
+  const double z = (a1[0] * a2[1]) - (a1[1] * a2[0]);
+  return std::sqrt(((x * x) + (y * y)) + (z * z));
+}
+
+int wtb_optprop(wtb_ctx* c, int n, int ncomp, const double* energy, const double* eps_re, const double* eps_im,
+                double* out) {
+  if (!c || !energy || !eps_re || !eps_im || !out || n < 1 || ncomp < 1 || ncomp > 64) {
+    set_error("wtb_optprop: null argument, n < 1 or ncomp outside 1..64");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double* d = nullptr;
+  const size_t nn = (size_t)n, m = nn * ncomp;
+  int rc = dmalloc(&d, nn + 2 * m + 5 * m);
+  if (rc) return rc;
+  double *den = d, *der = d + nn, *dei = der + m, *dout = dei + m;
+  if (hipMemcpyAsync(den, energy, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(der, eps_re, sizeof(double) * m, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dei, eps_im, sizeof(double) * m, hipMemcpyHostToDevice, s) != hipSuccess)
+    rc = WTB_ERR_HIP;
+  if (!rc) rc = launch_optprop(s, n, ncomp, den, der, dei, dout);
+  if (!rc && hipMemcpyAsync(out, dout, sizeof(double) * 5 * m, hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = WTB_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = WTB_ERR_HIP;
+  hipFree(d);
+  if (rc == WTB_ERR_HIP) set_error("wtb_optprop: HIP error");
+  return rc;
+}
+
+int wtb_exciton_lifetime(wtb_ctx* c, int sysdim, const int ngrid[3], const double* rlat, int n, int ncomp, int nout,
+                         const double* fosc, double* lft) {
+  if (sysdim < 1 || sysdim > 3) {
+    set_error("Wrong System dimension");
+    return WTB_ERR_ARG;
+  }
+  if (!c || !ngrid || !rlat || !fosc || !lft || n < 1 || ncomp < 3 || nout < 1 || nout > ncomp) {
+    set_error("wtb_exciton_lifetime: null argument, n < 1, ncomp < 3 or nout outside 1..ncomp");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t nn = (size_t)n, ncol = (size_t)ncomp + 1;
+  double* d = nullptr;
+  int rc = dmalloc(&d, nn * ncol + nn * (nout + 1));
+  if (rc) return rc;
+  const double nkd = (double)(ngrid[0] * ngrid[1] * ngrid[2]);
+  if (hipMemcpyAsync(d, fosc, sizeof(double) * nn * ncol, hipMemcpyHostToDevice, s) != hipSuccess) rc = WTB_ERR_HIP;
+  if (!rc) rc = launch_lifetime(s, n, (int)ncol, nout, sysdim, nkd, vcell2d_host(rlat), rlat[0], d, d + nn * ncol);
+  if (!rc && hipMemcpyAsync(lft, d + nn * ncol, sizeof(double) * nn * (nout + 1), hipMemcpyDeviceToHost, s) !=
+                 hipSuccess)
+    rc = WTB_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = WTB_ERR_HIP;
+  hipFree(d);
+  if (rc == WTB_ERR_HIP) set_error("wtb_exciton_lifetime: HIP error");
+  return rc;
+}
+
 // ---------------------------------------------------------------------------
 // learned preconditioner object
 // ---------------------------------------------------------------------------
@@ -1192,10 +1292,45 @@ int wtb_bse_plan_row_stats(wtb_bse_plan* p, double* D, float* stats) {
   return rc;
 }
 
+// The iterative form's parameters, checked before the plan is looked at (so that the
+// check runs without a device): every value that would size a buffer or a loop.
+static int check_davidson_params(const wtb_davidson_params* dp) {
+  if (!dp) {
+    set_error("wtb_bse_plan_lowest: null parameters");
+    return WTB_ERR_ARG;
+  }
+  const char* bad = nullptr;
+  if (dp->nev < 1) bad = "nev must be >= 1";
+  else if (dp->guard < 0) bad = "guard must be >= 0";
+  else if (dp->max_iter < 1) bad = "max_iter must be >= 1";
+  else if (dp->max_sub < 0) bad = "max_sub must be >= 0";
+  else if (dp->coarse < 0) bad = "coarse must be >= 0";
+  else if (!(dp->tol > 0.0) || !std::isfinite(dp->tol)) bad = "tol must be a positive finite number";
+  else if (dp->lanczos_steps < 0) bad = "lanczos_steps must be >= 0";
+  if (bad) {
+    set_error(std::string("wtb_bse_plan_lowest: ") + bad);
+    return WTB_ERR_ARG;
+  }
+  return WTB_OK;
+}
+
+// grow a device buffer to at least `bytes` (contents are not kept)
+static int grow_buffer(void** ptr, size_t* cap, size_t bytes) {
+  if (*ptr && *cap >= bytes) return WTB_OK;
+  hipFree(*ptr);
+  *ptr = nullptr;
+  *cap = 0;
+  int rc = dmalloc((char**)ptr, bytes);
+  if (!rc) *cap = bytes;
+  return rc;
+}
+
 int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wtb_mlp* mlp, double* W,
                         double* resid, double* eps, double* X, wtb_davidson_info* info) {
-  if (!p || !dp || !W || !resid || dp->nev < 1 || dp->nev > p->N) {
-    set_error("wtb_bse_plan_lowest: bad argument");
+  int rc0 = check_davidson_params(dp);
+  if (rc0) return rc0;
+  if (!p || !W || !resid || dp->nev > p->N) {
+    set_error("wtb_bse_plan_lowest: null plan or output, or nev > N");
     return WTB_ERR_ARG;
   }
   if (!p->mixed) {
@@ -1215,16 +1350,40 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
   wtb_davidson_info inf{};
   hipEvent_t ev[4] = {};
   for (auto& e : ev) WTB_HIP(hipEventCreate(&e));
-  double* dD = nullptr;
-  float* dst = nullptr;
-  double *dW = nullptr, *dR = nullptr, *dsm = nullptr;
-  zc* dX = nullptr;
-  void* hw = nullptr;
-  int rc;
+  int rc = WTB_OK;
   const int steps = dp->lanczos_steps;
-  if (!(rc = dmalloc(&dD, N)) && !(rc = dmalloc(&dst, (size_t)3 * N)) && !(rc = dmalloc(&dW, nev)) &&
-      !(rc = dmalloc(&dR, nev)) && !(rc = dmalloc(&dsm, 4)) && !(X && (rc = dmalloc(&dX, (size_t)N * nev))) &&
-      !(eps && steps > 0 && (rc = dmalloc((char**)&hw, haydock_work_bytes(N, steps))))) {
+  // the plan keeps these between calls: no allocation once their sizes are reached
+  if (!p->dav && !(p->dav = davidson_work_new())) rc = WTB_ERR_NOMEM;
+  if (!rc && !p->lw_D) {
+    if ((rc = dmalloc(&p->lw_D, N)) || (rc = dmalloc(&p->lw_stats, (size_t)3 * N)) || (rc = dmalloc(&p->lw_sm, 4))) {
+      hipFree(p->lw_D);
+      hipFree(p->lw_stats);
+      hipFree(p->lw_sm);
+      p->lw_D = nullptr;
+      p->lw_stats = nullptr;
+      p->lw_sm = nullptr;
+    }
+  }
+  if (!rc && (size_t)nev > p->lw_nev) {
+    hipFree(p->lw_W);
+    hipFree(p->lw_R);
+    p->lw_W = p->lw_R = nullptr;
+    p->lw_nev = 0;
+    if (!(rc = dmalloc(&p->lw_W, nev)) && !(rc = dmalloc(&p->lw_R, nev))) p->lw_nev = nev;
+  }
+  if (!rc && X && (size_t)nev > p->lw_xcols) {
+    hipFree(p->lw_X);
+    p->lw_X = nullptr;
+    p->lw_xcols = 0;
+    if (!(rc = dmalloc(&p->lw_X, (size_t)N * nev))) p->lw_xcols = nev;
+  }
+  if (!rc && eps && steps > 0) rc = grow_buffer(&p->lw_hay, &p->lw_hay_bytes, haydock_work_bytes(N, steps));
+  double* dD = p->lw_D;
+  float* dst = p->lw_stats;
+  double *dW = p->lw_W, *dR = p->lw_R, *dsm = p->lw_sm;
+  zc* dX = X ? p->lw_X : nullptr;
+  void* hw = (eps && steps > 0) ? p->lw_hay : nullptr;
+  if (!rc) {
     hipEventRecord(ev[0], s);
     rc = plan_enqueue(p, true);
     if (!rc) rc = sym_fill_f32(s, N, p->Hf, (int)p->ldh);
@@ -1261,7 +1420,7 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
       md.hidden = mlp->hidden;
     }
     DavidsonInfo di;
-    if (!rc) rc = run_davidson(s, N, p->Hf, (int)p->ldh, dD, dst, cfg, mlp ? &md : nullptr, dW, dR, dX, di);
+    if (!rc) rc = run_davidson(s, N, p->Hf, (int)p->ldh, dD, dst, cfg, mlp ? &md : nullptr, dW, dR, dX, di, *p->dav);
     hipEventRecord(ev[2], s);
     inf.iters = di.iters;
     inf.matvecs = di.matvecs;
@@ -1319,7 +1478,6 @@ int wtb_bse_plan_lowest(wtb_bse_plan* p, const wtb_davidson_params* dp, const wt
   }
   for (auto& e : ev)
     if (e) hipEventDestroy(e);
-  for (void* b : {(void*)dD, (void*)dst, (void*)dW, (void*)dR, (void*)dsm, (void*)dX, hw}) hipFree(b);
   if (info) *info = inf;
   return rc;
 }
@@ -1506,24 +1664,46 @@ int wtb_bse_plan_set_vectors(wtb_bse_plan* p, int first, int count) {
   WTB_HIP(hipStreamSynchronize(p->stream));
   const int N = p->N;
   int rc = WTB_OK;
-  if (count > 0 && !p->Zv) {
+  // Everything new is allocated and prepared into temporaries first; the plan is changed
+  // only when all of it succeeded, so a failed call leaves a plan that still executes.
+  const bool need_z = count > 0 && !p->Zv;
+  const bool need_x = count != p->wf_count && count > 0;
+  void* ndc = nullptr;
+  double* nzv = nullptr;
+  zc* nxv = nullptr;
+  void* nbt = nullptr;
+  DCTree ntree{};
+  auto drop = [&]() {
+    hipFree(ndc);
+    hipFree(nzv);
+    hipFree(nxv);
+    hipFree(nbt);
+  };
+  if (need_z) {
     // the divide and conquer needs its eigenvector workspace, the reduction keeps its reflectors
-    hipFree(p->dcwork);
-    p->dcwork = nullptr;
-    if ((rc = dmalloc((char**)&p->dcwork, dc_work_bytes(N, 2 * p->nrhs, 1))) ||
-        (rc = dmalloc(&p->Zv, (size_t)N * N)))
+    if ((rc = dmalloc((char**)&ndc, dc_work_bytes(N, 2 * p->nrhs, 1))) || (rc = dmalloc(&nzv, (size_t)N * N)) ||
+        (rc = dc_prepare(ndc, N, 2 * p->nrhs, ntree, 1))) {
+      drop();
       return rc;
-    if ((rc = dc_prepare(p->dcwork, N, 2 * p->nrhs, p->tree, 1))) return rc;
+    }
+  }
+  if (need_x &&
+      ((rc = dmalloc(&nxv, (size_t)N * count)) || (rc = dmalloc((char**)&nbt, backtr_work_bytes(N, count))))) {
+    drop();
+    return rc;
+  }
+  if (need_z) {
+    hipFree(p->dcwork);
+    p->dcwork = ndc;
+    p->Zv = nzv;
+    p->tree = ntree;
     p->tw.store_v = 1;
   }
   if (count != p->wf_count) {
     hipFree(p->Xv);
     hipFree(p->btw);
-    p->Xv = nullptr;
-    p->btw = nullptr;
-    if (count > 0 &&
-        ((rc = dmalloc(&p->Xv, (size_t)N * count)) || (rc = dmalloc((char**)&p->btw, backtr_work_bytes(N, count)))))
-      return rc;
+    p->Xv = nxv;
+    p->btw = nbt;
   }
   p->wf_first = first;
   p->wf_count = count;
@@ -1873,7 +2053,7 @@ int wtb_bse_plan_sp_table(wtb_bse_plan* p, float* vecres) {
 
 int wtb_sp_table(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params* prm, float* vecres) {
   wtb_bse_plan* p = nullptr;
-  int rc = wtb_bse_plan_create(c, m, prm, &p);
+  int rc = plan_create_impl(c, m, prm, &p, true);
   if (rc) return rc;
   rc = wtb_bse_plan_build(p);
   if (!rc) {

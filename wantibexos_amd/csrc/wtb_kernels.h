@@ -148,6 +148,12 @@ int launch_spectrum(hipStream_t s, int N, const double* d_e, const double* d_act
                     double* d_work, double* d_eps);
 
 int launch_imagrenorm(hipStream_t s, const double* d_e, double ebse0, double ebsef, int numbse, double* d_eps);
+// optical constants n, k, R, alpha, loss of nc components at n photon energies:
+// er, ei [nc][n], out [5][nc][n]
+int launch_optprop(hipStream_t s, int n, int nc, const double* en, const double* er, const double* ei, double* out);
+// exciton radiative lifetimes: fosc [ncol][n] (energy first), lft [1 + nout][n]
+int launch_lifetime(hipStream_t s, int n, int ncol, int nout, int sysdim, double nkd, double vc2d, double a11,
+                    const double* fosc, double* lft);
 
 // iterative form: Davidson for the lowest excitons, Haydock chains for eps(omega)
 int sym_fill_f32(hipStream_t s, int N, zf* H, int ld);
@@ -210,8 +216,13 @@ struct DavidsonInfo {
 };
 // lowest cfg.nev eigenpairs of the full Hermitian COMPLEX*8 matrix H; d_W (nev, device),
 // d_resid (nev, device), d_X (N x nev, device, optional)
+// Device workspace of the block Davidson, kept between calls (a plan owns one): it is
+// allocated on the first call and grown only when a call needs more.
+struct DavidsonWork;
+DavidsonWork* davidson_work_new();
+void davidson_work_delete(DavidsonWork* w);
 int run_davidson(hipStream_t s, int N, const zf* H, int ldh, const double* D, const float* stats,
                  const DavidsonCfg& cfg, const MlpDev* mlp, double* d_W, double* d_resid, zc* d_X,
-                 DavidsonInfo& info);
+                 DavidsonInfo& info, DavidsonWork& work);
 
 }  // namespace wtb

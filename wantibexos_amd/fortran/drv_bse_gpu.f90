@@ -1,9 +1,9 @@
 ! The BSE + spectrum and exciton-band paths of the reference's program main
 ! (wtb_main.f90:60-190, 253-326) with the GPU drop-ins of bse_solver_gpu.f90 in place of the
-! reference's bsesolver, bsedielraw and spdielraw (and their variants).  The input parser,
-! the TB reader and the element-wise post-processing of the spectra (bseoptprop,
-! lifetime, spoptprop and their -pol forms: n, k, R, alpha, loss function, radiative
-! lifetimes) are the reference's own, INCLUDEd from its sources by the Makefile.
+! reference's bsesolver, bsedielraw and spdielraw (and their variants), and the drop-ins of
+! optprop_gpu.f90 in place of the post-processing of the spectra and the exciton lifetimes
+! (bseoptprop, spoptprop, lifetime and their -pol forms).  The input parser and the TB
+! reader are the reference's own, INCLUDEd from its sources by the Makefile.
 !
 !   wtb_gpu.x < input.dat     (the reference's KEY= value input)
 
@@ -12,14 +12,9 @@ INCLUDE "subroutines/mhkpack_subs.f90"
 INCLUDE "subroutines/module_input_read.f90"
 INCLUDE "subroutines/coulomb_pot.f90"
 INCLUDE "subroutines/bse_subs.f90"
-INCLUDE "subroutines/diel-pp-subs.f90"
-INCLUDE "subprograms/diel-pp.f90"
-INCLUDE "subprograms/diel-pp-pol.f90"
-INCLUDE "subprograms/diel-pp-bse.f90"
-INCLUDE "subprograms/diel-pp-bse-pol.f90"
-INCLUDE "subprograms/exciton_lifetime.f90"
 INCLUDE "wtb_gpu.f90"
 INCLUDE "bse_solver_gpu.f90"
+INCLUDE "optprop_gpu.f90"
 
 program wtb_gpu_main
 	use input_variables

@@ -116,6 +116,21 @@ module wtb_gpu
 			integer(c_int), value :: renorm
 			real(c_double) :: eps(*)
 		end function
+		integer(c_int) function wtb_optprop(ctx, n, ncomp, energy, eps_re, eps_im, out) &
+				bind(C, name="wtb_optprop")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: n, ncomp
+			real(c_double) :: energy(*), eps_re(*), eps_im(*), out(*)
+		end function
+		integer(c_int) function wtb_exciton_lifetime(ctx, sysdim, ngrid, rlat, n, ncomp, nout, fosc, lft) &
+				bind(C, name="wtb_exciton_lifetime")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			integer(c_int), value :: sysdim, n, ncomp, nout
+			integer(c_int) :: ngrid(3)
+			real(c_double) :: rlat(*), fosc(*), lft(*)
+		end function
 		integer(c_int) function wtb_zheev(ctx, want_vectors, n, a, lda, w) bind(C, name="wtb_zheev")
 			import :: c_int, c_ptr, c_double
 			type(c_ptr), value :: ctx

@@ -81,6 +81,26 @@ def write_tb(model: TBModel, path) -> None:
     Path(path).write_text("\n".join(out) + "\n")
 
 
+def write_tb_sp(up: TBModel, down: TBModel, path) -> None:
+    """Write a spin-polarised ("SP") TB file: the spin-up block, then the spin-down block,
+    each with its own comment, nw, nvec, ndegen and hoppings, as hamiltonian_input_read
+    reads them (module_input_read.f90:56-142).  Header lines come from `up`."""
+    def block(model):
+        lines = [model.comment, str(model.nw), str(model.nvec)]
+        nd = [str(int(v)) for v in model.ndeg]
+        for s0 in range(0, len(nd), 15):
+            lines.append(" ".join(nd[s0:s0 + 15]))
+        for i in range(model.nvec):
+            r1, r2, r3 = (int(v) for v in model.rvec[i])
+            for n in range(model.nw):
+                for m in range(model.nw):
+                    lines.append(f"{r1} {r2} {r3} {m + 1} {n + 1} "
+                                 f"{_fmt(model.hop[i, m, n])} {_fmt(model.ihop[i, m, n])}")
+        return lines
+    head = ["SP", _fmt(up.scs), _fmt(up.efermi)] + [" ".join(_fmt(v) for v in up.rlat[i]) for i in range(3)]
+    Path(path).write_text("\n".join(head + block(up) + block(down)) + "\n")
+
+
 def read_tb(path) -> TBModel:
     """Parse a TB file the way hamiltonian_input_read does (module_input_read.f90:32-176)."""
     toks_lines = Path(path).read_text().splitlines()
@@ -152,6 +172,10 @@ def input_dat(p: dict, tbname: str, nthreads: int = 1, outdir: str = "./out/") -
         f"NESPEC= {p['numbse']}", f"RNMD= {T(p['renorm'])}",
         f"SHIFT_1= {p['shift'][0]!r}", f"SHIFT_2= {p['shift'][1]!r}", f"SHIFT_3= {p['shift'][2]!r}",
     ]
+    if p.get("sysdim"):                     # exciton lifetimes: "1D", "2D" or "3D" (default 3D)
+        lines += [f"SYSDIM= {p['sysdim']}"]
+    if p.get("meshtype"):                   # k grid from a density: RK3D / RK2D with RK
+        lines += [f"MESH_TYPE= {p['meshtype']}", f"RK= {p['rk']!r}"]
     if p.get("bsewf"):
         lines += ["BSE_WF= T", f"EXC_WF_I= {p['excwf'][0]}", f"EXC_WF_F= {p['excwf'][1]}"]
     if p.get("tmcoef"):

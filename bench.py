@@ -59,7 +59,8 @@ BULK3D = dict(CONFIG, name="cubic3d_nw16_16x16x16_nv3nc3", lattice="cubic", seed
               nc=3, nv=3, coultype="V3DL", ediel=(6.0, 6.0, 6.0))
 CONFIGS = {"c2": CONFIG, "c1": SMALL, "c3": BULK3D, "c5": LARGE2D}
 HBM_PEAK = 8000.0       # GB/s
-FP64_PEAK = 78.6        # TFLOP/s
+FP64_PEAK = 78.6        # TFLOP/s, dense FP64 MFMA
+FP32_PEAK = 157.3       # TFLOP/s, dense FP32 MFMA
 
 
 def log(*a):
@@ -113,6 +114,29 @@ def cpu_baseline(cfg, model, max_threads):
                         f"own subroutines (flang -O2 + MKL ZHEEV, LWORK as the reference), "
                         f"stage sums excl. file I/O; wall {wall:.1f}s"),
                 stages=stages)
+
+
+def cpu_baseline_1core(cfg, N):
+    """The reference's own subroutines on ONE thread, on a smaller matrix of the same model
+    family (hexagonal, same nw, 24x24 k grid: N = 2304), scaled to the workload's N stage by
+    stage: ZHEEV as N^3, the build and oscillator strengths as N^2, the rest as N."""
+    from wantibexos_amd import tbmodel
+    g = 24
+    small = dict(cfg, name=cfg["name"] + "_1core_sample", ngrid=(g, g, 1))
+    m = tbmodel.hex2d(cfg["seed"], cfg["nw"], small["ngrid"])
+    r = cpu_baseline(small, m, 1)
+    if not r:
+        return None
+    n_s = g * g * cfg["nc"] * cfg["nv"]
+    f = N / n_s
+    st = r["stages"]
+    scaled = {"ab": st["ab"] * f, "dip": st["dip"] * f, "build": st["build"] * f * f, "diag": st["diag"] * f ** 3,
+              "osc": st["osc"] * f * f, "eps": st["eps"] * f}
+    return dict(value=round(sum(scaled.values()), 3), unit="s/matrix", cores=1, kind="reference",
+                sample=(f"the reference's subroutines on one thread on the same model family at N={n_s} "
+                        f"(nw={cfg['nw']}, {g}x{g}x1 k grid): {r['value']:.2f} s measured, scaled to N={N} "
+                        f"(ZHEEV N^3, build and oscillators N^2, the rest N)"),
+                measured_s=round(r["value"], 3), stages_scaled={k: round(v, 3) for k, v in scaled.items()})
 
 
 def tridiag_bytes(N, nb=32, elem=16.0):
@@ -327,14 +351,23 @@ def bench_bands(args, cfg, model):
         plan.execute()
         plan.energies()
         k_ms, k_n = plan.kernel_ms()
+        k_flops = plan.kernel_flops()
     elem = 8.0 if args.precision == "mixed" else 16.0
     hemv_b, _ = tridiag_bytes(N, elem=elem)
     avg_s = k_ms * 1e-3 / max(k_n, 1)
-    per_launch = hemv_b / max(k_n, 1)
-    ach = per_launch / avg_s / 1e9
-    roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s", frac=round(ach / HBM_PEAK, 4),
-                    traffic=None, kernel="k_tri_hemv", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
-                    bytes_per_launch=round(per_launch))
+    if k_flops > 0:
+        per_launch = k_flops / max(k_n, 1)
+        ach = per_launch / avg_s / 1e12
+        peak = FP32_PEAK if args.precision == "mixed" else FP64_PEAK
+        roofline = dict(bound="mfma", achieved=round(ach, 2), peak=peak, unit="TFLOP/s", frac=round(ach / peak, 4),
+                        traffic=None, kernel="k_her2k_full", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
+                        flops_per_launch=round(per_launch))
+    else:
+        per_launch = hemv_b / max(k_n, 1)
+        ach = per_launch / avg_s / 1e9
+        roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",
+                        frac=round(ach / HBM_PEAK, 4), traffic=None, kernel="k_tri_hemv", launches=k_n,
+                        avg_launch_us=round(avg_s * 1e6, 3), bytes_per_launch=round(per_launch))
     cpu = None if args.no_cpu else cpu_baseline_bands(cfg, model, args.cpu_threads)
     ctx.close()
     dt, exk = out[(args.precision, args.lanes)]
@@ -383,6 +416,11 @@ def main():
     ap.add_argument("--lanes", type=int, default=4, help="batching form and bands: streams")
     ap.add_argument("--npts", type=int, default=4, help="bands: q-points per path segment (3 segments)")
     ap.add_argument("--tune", action="store_true", help="batching form: run the panel autotuner first")
+    ap.add_argument("--refine", type=int, default=256,
+                    help="precision form: FP64 Rayleigh-quotient refinement of this many lowest excitons (0: off)")
+    ap.add_argument("--no-cpu-1core", action="store_true",
+                    help="skip the one-thread timing of the reference's subroutines (a smaller matrix of the "
+                         "same model family, scaled to the workload)")
     ap.add_argument("--lanczos-steps", type=int, default=250)
     ap.add_argument("--guard", type=int, default=32, help="iterative form: extra Ritz pairs carried along")
     ap.add_argument("--coarse", type=int, default=None,
@@ -404,6 +442,9 @@ def main():
                     help="full form: skip the iterative form's measurement on the 60x60 model that the "
                          "line carries beside the metric")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--two-stage", action="store_true",
+                    help="the two-stage (dense -> band -> tridiagonal) reduction instead of the one-stage one")
+    ap.add_argument("--band", type=int, default=32, choices=[16, 32], help="two-stage reduction: band width")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the kernels directly instead of replaying the captured HIP graph")
     ap.add_argument("--pmc-light", action="store_true",
@@ -447,7 +488,11 @@ def main():
     if args.no_graph:
         plan.set_graph(False)
     N = plan.N
-    log(f"plan N={N} created")
+    plan.set_reduction(args.two_stage, args.band)
+    nref = min(args.refine, N) if args.precision == "mixed" else 0
+    if nref:
+        plan.set_refine(nref)
+    log(f"plan N={N} created" + (f", FP64 refinement of the lowest {nref} excitons" if nref else ""))
     if args.pmc_light:
         plan.execute()
         plan.results()
@@ -464,6 +509,21 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     res = plan.results()
     log(f"timed {args.steps} steps: {dt * 1e3:.2f} ms/step")
+    rq_res, w_unref = plan.refine_results() if nref else (None, None)
+    dt_unref = None
+    if nref:
+        # the same form without the refinement, for the record
+        plan.set_refine(0)
+        for _ in range(args.warmup):
+            plan.execute()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute()
+        ctx.sync()
+        dt_unref = (time.perf_counter() - t0) / args.steps
+        plan.set_refine(nref)
+        log(f"without the refinement: {dt_unref * 1e3:.2f} ms/step")
     # per-stage device times (HIP events on the context stream), separate run
     plan.set_timing(True)
     plan.execute()
@@ -492,6 +552,7 @@ def main():
         with bse.BSEPlan(ctx, model, params(cfg, "fp64")) as p64:
             if args.no_graph:
                 p64.set_graph(False)
+            p64.set_reduction(args.two_stage, args.band)
             for _ in range(args.warmup):
                 p64.execute()
             ctx.sync()
@@ -519,46 +580,80 @@ def main():
             np.abs(res["eps"][c, q] - r64["eps"][c, q]).max() for c in range(6) for q in range(2))
             / max(np.abs(r64["eps"][c, q]).max() for c in range(3) for q in range(2)))
         log("mixed vs fp64:", json.dumps({k: checks[k] for k in checks if k.startswith("mixed")}))
+        if nref:
+            # per exciton: the refined energy against the FP64 solve, and the residual bound
+            # |H x - theta x| / |x| of the FP64 matrix (an eigenvalue lies within it)
+            dref = np.abs(res["W"][:nref] - r64["W"][:nref])
+            dun = np.abs(w_unref - r64["W"][:nref])
+            refine = {"nref": nref, "max_dW_refined_eV": float(dref.max()), "mean_dW_refined_eV": float(dref.mean()),
+                      "max_dW_unrefined_eV": float(dun.max()), "max_resid_eV": float(rq_res.max()),
+                      "median_resid_eV": float(np.median(rq_res)),
+                      "dW_within_resid": bool(np.all(dref <= rq_res + 1e-12)),
+                      "per_exciton_dW_eV_first8": [float(x) for x in dref[:8]],
+                      "per_exciton_resid_eV_first8": [float(x) for x in rq_res[:8]],
+                      "s_without_refinement": round(dt_unref, 6)}
+            checks["refinement"] = refine
+            log("refinement:", json.dumps(refine))
     # dominant kernel: the reduction's Hermitian GEMV, one launch per column, each
     # bracketed by HIP events on the context stream (a separate, non-graph execute)
     elem = 8.0 if args.precision == "mixed" else 16.0
     hemv_b, her2k_b = tridiag_bytes(N, elem=elem)
+    if nref:
+        plan.set_refine(0)          # the reduction the timed form uses for the matrix itself
     plan.set_timing(2)
     plan.execute()
     plan.results()
     k_ms, k_n = plan.kernel_ms()
+    k_flops = plan.kernel_flops()
     plan.set_timing(False)
+    if nref:
+        plan.set_refine(nref)
     avg_s = k_ms * 1e-3 / max(k_n, 1)
-    per_launch = hemv_b / max(k_n, 1)
-    ach = per_launch / avg_s / 1e9
-    log(f"k_tri_hemv: {k_n} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.2f} MB/launch")
-    # HBM traffic per launch: the measured traffic/algorithmic ratio of the counter passes
-    # (profiles/, same workload and precision) applied to the mean launch
-    traffic = None
-    tfile = ROOT / "profiles" / f"k_tri_hemv_traffic_hex2d_nw22_k36_N5184_{args.precision}.json"
-    if tfile.exists() and args.config == "c2" and not prm.bset:
-        tj = json.loads(tfile.read_text())
-        traffic = round(tj["traffic_over_algorithmic"] * per_launch)
-    roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",
-                    frac=round(ach / HBM_PEAK, 4), traffic=traffic,
-                    kernel="k_tri_hemv", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
-                    bytes_per_launch=round(per_launch))
+    if k_flops > 0:
+        # two-stage reduction: the dense -> band stage's trailing update, MFMA-bound
+        per_launch = k_flops / max(k_n, 1)
+        ach = per_launch / avg_s / 1e12
+        peak = FP32_PEAK if args.precision == "mixed" else FP64_PEAK
+        log(f"k_her2k_full: {k_n} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e9:.3f} GFLOP/launch")
+        roofline = dict(bound="mfma", achieved=round(ach, 2), peak=peak, unit="TFLOP/s", frac=round(ach / peak, 4),
+                        traffic=None, kernel="k_her2k_full", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
+                        flops_per_launch=round(per_launch))
+    else:
+        per_launch = hemv_b / max(k_n, 1)
+        ach = per_launch / avg_s / 1e9
+        log(f"k_tri_hemv: {k_n} launches, avg {avg_s * 1e6:.2f} us, {per_launch / 1e6:.2f} MB/launch")
+        # HBM traffic per launch: the measured traffic/algorithmic ratio of the counter passes
+        # (profiles/, same workload and precision) applied to the mean launch
+        traffic = None
+        tfile = ROOT / "profiles" / f"k_tri_hemv_traffic_hex2d_nw22_k36_N5184_{args.precision}.json"
+        if tfile.exists() and args.config == "c2" and not prm.bset:
+            tj = json.loads(tfile.read_text())
+            traffic = round(tj["traffic_over_algorithmic"] * per_launch)
+        roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s",
+                        frac=round(ach / HBM_PEAK, 4), traffic=traffic,
+                        kernel="k_tri_hemv", launches=k_n, avg_launch_us=round(avg_s * 1e6, 3),
+                        bytes_per_launch=round(per_launch))
     cpu = None
+    cpu1 = None
     if not args.no_cpu:
         log("cpu baseline: running the reference's subroutines on the same model ...")
         cpu = cpu_baseline(cfg, model, args.cpu_threads)
         if cpu:
             log(f"cpu baseline {cpu['value']:.2f} s/matrix on {cpu['cores']} threads: {cpu['stages']}")
+        if not args.no_cpu_1core and cfg.get("lattice") != "cubic":
+            cpu1 = cpu_baseline_1core(cfg, N)
+            if cpu1:
+                log(f"cpu baseline, one thread: {cpu1['value']:.1f} s/matrix ({cpu1['sample']})")
     plan.close()
     ctx.close()
     iterative = None
     if not args.no_iterative and args.config == "c2" and args.precision == "mixed" and not prm.bset:
-        # the iterative form at the target accuracy (excitons within 1e-5 eV by their residuals,
-        # 150-step chains) on the 60x60 model, against the full FP64 solve of the same matrix:
+        # the iterative form at the target accuracy (every residual within 1e-6 eV, 250-step
+        # chains) on the 60x60 model, against the full FP64 solve of the same matrix:
         # its cost grows like N^2 and the full solve's like N^3, so it pays on larger matrices
         import copy
         ia = copy.copy(args)
-        ia.tol, ia.lanczos_steps, ia.steps, ia.warmup, ia.learned = 1e-5, 150, 2, 1, False
+        ia.tol, ia.lanczos_steps, ia.steps, ia.warmup, ia.learned = 1e-6, 250, 2, 1, False
         lcfg = CONFIGS["c5"]
         t0 = time.perf_counter()
         lmodel = make_model(lcfg)
@@ -588,6 +683,7 @@ def main():
         "fp64_plain": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
         "stage_ms": {k: round(v, 3) for k, v in stage.items()},
         "checks": checks,
         "iterative_form": iterative,

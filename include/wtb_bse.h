@@ -241,6 +241,10 @@ enum { WTB_NSTAGES = 6 };
  * kernel), summed by wtb_bse_plan_kernel_ms. */
 int wtb_bse_plan_set_timing(wtb_bse_plan* plan, int on);
 int wtb_bse_plan_kernel_ms(const wtb_bse_plan* plan, double* total_ms, int* launches);
+/* With the two-stage reduction the timed kernel is instead the trailing update of the
+ * dense -> band stage (one launch per panel); flops = their algorithmic flop count
+ * (8 m^2 2k per launch: m trailing rows, k reflectors); 0 for the one-stage GEMV. */
+int wtb_bse_plan_kernel_flops(const wtb_bse_plan* plan, double* flops);
 /* execute() captures the pipeline into a HIP graph on its first call and
  * replays it afterwards (default on); off launches the kernels directly.
  * Timing mode always launches directly. */
@@ -386,7 +390,27 @@ int wtb_sp_table(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* 
  * (0-based, ascending energy), fetched as WTB_BUF_EXCWF, each with its last
  * component real and >= 0 (ZHEEV('V','U') leaves it real with either sign). */
 int wtb_bse_plan_set_vectors(wtb_bse_plan* plan, int first, int count);
-/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid
+/* Form of the tridiagonal reduction of the BSE matrix (the first step of the ZHEEV of
+ * bse_solver-tool-diel.f90:485-492; no counterpart in the reference's arithmetic):
+ * two_stage = 0 (default): one-stage blocked Householder; two_stage = 1: dense -> band of
+ * width `band` (16 or 32, MFMA panel updates) -> tridiagonal by bulge chasing, used
+ * whenever no exciton vectors are formed.  Executes that form exciton vectors (BSE_WF,
+ * the refinement) always take the one-stage form, whose reflectors they need.          */
+int wtb_bse_plan_set_reduction(wtb_bse_plan* plan, int two_stage, int band);
+
+/* Precision form (WTB_MIXED, Q = 0): FP64 Rayleigh-quotient refinement of the nref lowest
+ * excitons, the refinement of bse_solver-tool-diel.f90:485-492's eigenvalues that the
+ * precision form adds (no counterpart in the reference).  Executes then also assemble the
+ * FP64 matrix, form the exciton vectors x of the lowest nref (divide-and-conquer vectors
+ * and the back-transform through the FP32 reflectors), and replace each of those energies
+ * by theta = x^H H x / x^H x with the FP64 H; |H x - theta x| / |x| bounds the distance of
+ * theta to an eigenvalue of the FP64 matrix.  nref = 0 turns it off (default).        */
+int wtb_bse_plan_set_refine(wtb_bse_plan* plan, int nref);
+/* after an execute: resid[n] those residual norms (eV) and, if w_unrefined is not NULL,
+ * the energies before the refinement (n <= nref)                                       */
+int wtb_bse_plan_refine_results(wtb_bse_plan* plan, int n, double* resid, double* w_unrefined);
+
+/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid
  * (kpt(nk,3) Fortran order) and nocpk(nk) that excwf2 prints beside them */
 int wtb_bsesolver_wf(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, double* W,
                      double* act, double* eps, double* egap, int wf_first, int wf_count, double* wf,

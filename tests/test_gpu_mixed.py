@@ -63,3 +63,35 @@ def test_mixed_larger_model(ctx):
             ref = r64["eps"][c, part]
             if np.abs(ref).max() > 1e-8:
                 assert relerr(r32["eps"][c, part], ref) < EPS_TOL_MIXED, (c, part)
+
+
+@pytest.mark.parametrize("name", ["hex8_g12_v2drk_c1", "hex8_g6_v2drk", "cub6_g3_v3dl", "hex6_g4_t2500_ve"])
+def test_refinement_of_lowest_excitons(ctx, name):
+    """FP64 Rayleigh-quotient refinement of the lowest excitons in the precision form: each
+    refined energy lies within its residual norm of the reference's FP64 exciton energy
+    (the FP64 matrix has an eigenvalue within |H x - theta x| / |x| of theta), is at least
+    as close as the unrefined one up to that bound, and the excitons above the refined set,
+    the activities and eps(omega) are those of the unrefined precision form."""
+    from wantibexos_amd import bse
+    m, p, z = load(name)
+    N = int(np.prod(p["ngrid"])) * p["nc"] * p["nv"]
+    nref = min(64, N // 2)
+    with bse.BSEPlan(ctx, m, mixed(params_of(p))) as plan:
+        plan.set_reduction(False)               # the refinement's own (one-stage) reduction
+        plan.execute()
+        r0 = plan.results()
+        plan.set_refine(nref)
+        for _ in range(2):                       # capture, then replay
+            plan.execute()
+            r = plan.results()
+            res, w0 = plan.refine_results()
+            assert np.array_equal(w0, r0["W"][:nref])
+            ref = z["wbse"][:nref]
+            d = np.abs(r["W"][:nref] - ref)
+            assert np.all(d <= res + 1e-12), (d - res).max()
+            assert d.max() < 1e-5 and res.max() < 1e-3
+            assert np.array_equal(r["W"][nref:], r0["W"][nref:])
+            assert np.array_equal(r["act"], r0["act"])
+        plan.set_refine(0)
+        plan.execute()
+        assert np.array_equal(plan.results()["W"], r0["W"])

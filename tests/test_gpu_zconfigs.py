@@ -94,13 +94,19 @@ def test_c2_plain_and_precision_forms(ctx, c2_model):
         ref = plan.results()
         check_invariants(plan, ref, "C2 fp64")
     with bse.BSEPlan(ctx, c2_model, prm((36, 36, 1), "mixed")) as plan:
+        plan.set_refine(256)
         plan.execute()
         got = plan.results()
+        res, w0 = plan.refine_results()
         check_invariants(plan, got, "C2 mixed")
     dW = np.abs(got["W"] - ref["W"])
+    d256 = dW[:256]
     ee = eps_err(got["eps"], ref["eps"])
-    print(f"C2 precision vs FP64: max dW {dW.max():.2e} eV, lowest 256 {dW[:256].max():.2e} eV, eps {ee}", flush=True)
+    print(f"C2 precision vs FP64: max dW {dW.max():.2e} eV; lowest 256 refined {d256.max():.2e} eV "
+          f"(unrefined {np.abs(w0 - ref['W'][:256]).max():.2e}, residual bound max {res.max():.2e}); eps {ee}",
+          flush=True)
     assert dW.max() <= E_TOL
+    assert np.all(d256 <= res + 1e-12) and res.max() <= E_TOL
     assert max(ee.values()) <= EPS_TOL
 
 
@@ -173,10 +179,12 @@ def test_c3_bulk_precision_and_iterative_forms(ctx):
     m = gen_models.generate([("cubic3d", 3, 16, (16, 16, 16))])[0]
     g = (16, 16, 16)
     kw = dict(nc=3, nv=3, coultype="V3DL", ediel=(6.0, 6.0, 6.0))
-    print("C3: precision form", flush=True)
+    print("C3: precision form with the FP64 refinement of the lowest 256", flush=True)
     with bse.BSEPlan(ctx, m, prm(g, "mixed", **kw)) as plan:
+        plan.set_refine(256)
         plan.execute()
         got = plan.results()
+        res, w0 = plan.refine_results()
         check_invariants(plan, got, "C3 mixed")
     print("C3: iterative form", flush=True)
     with bse.BSEPlan(ctx, m, dataclasses.replace(prm(g, "mixed", **kw), f154=False)) as plan:
@@ -187,6 +195,9 @@ def test_c3_bulk_precision_and_iterative_forms(ctx):
         ref = plan.results()
         check_invariants(plan, ref, "C3 fp64")
     dWm = np.abs(got["W"] - ref["W"]).max()
+    d256 = np.abs(got["W"][:256] - ref["W"][:256])
+    print(f"C3: refined lowest 256 max dW {d256.max():.2e} eV, residual bound max {res.max():.2e}", flush=True)
+    assert np.all(d256 <= res + 1e-12) and res.max() <= E_TOL
     dWi = np.abs(it["W"] - ref["W"][:256]).max()
     ei = eps_err(it["eps"], ref["eps"])
     print(f"C3: precision form max dW {dWm:.2e} eV; iterative {it['info']} max dW {dWi:.2e} eV eps {ei}", flush=True)

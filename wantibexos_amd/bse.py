@@ -147,6 +147,10 @@ def _load():
         "wtb_bse_bands_run_wf": (ip, [vp, ip, vp, vp, vp]),
         "wtb_bsebnds_wf": (ip, [vp, vp, vp, ip, vp, ip, vp, ip, ip, vp, vp, vp]),
         "wtb_optprop": (ip, [vp, ip, ip, vp, vp, vp, vp]),
+        "wtb_bse_plan_set_refine": (ip, [vp, ip]),
+        "wtb_bse_plan_set_reduction": (ip, [vp, ip, ip]),
+        "wtb_bse_plan_kernel_flops": (ip, [vp, vp]),
+        "wtb_bse_plan_refine_results": (ip, [vp, ip, vp, vp]),
         "wtb_exciton_lifetime": (ip, [vp, ip, vp, vp, ip, ip, ip, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -416,6 +420,12 @@ class BSEPlan:
         _check(lib().wtb_bse_plan_kernel_ms(self.h, _p(t), _p(n)))
         return float(t[0]), int(n[0])
 
+    def kernel_flops(self):
+        """Algorithmic flops of the timed two-stage trailing updates (0: one-stage GEMV)."""
+        f = np.zeros(1)
+        _check(lib().wtb_bse_plan_kernel_flops(self.h, _p(f)))
+        return float(f[0])
+
     def set_graph(self, on=True):
         _check(lib().wtb_bse_plan_set_graph(self.h, int(on)))
 
@@ -491,6 +501,25 @@ class BSEPlan:
     def set_vectors(self, first, count):
         """Also form the eigenvectors of excitons first .. first+count-1 (fetch('excwf'))."""
         _check(lib().wtb_bse_plan_set_vectors(self.h, int(first), int(count)))
+
+    def set_reduction(self, two_stage=True, band=32):
+        """Tridiagonal reduction: two-stage (dense -> band -> tridiagonal, default) or
+        one-stage Householder; band width 16 or 32."""
+        _check(lib().wtb_bse_plan_set_reduction(self.h, int(bool(two_stage)), int(band)))
+
+    def set_refine(self, nref=256):
+        """Precision form: FP64 Rayleigh-quotient refinement of the nref lowest excitons
+        in every execute (0: off)."""
+        _check(lib().wtb_bse_plan_set_refine(self.h, int(nref)))
+        self.nref = int(nref)
+
+    def refine_results(self):
+        """(residual norms, energies before the refinement) of the refined excitons."""
+        n = getattr(self, "nref", 0)
+        res = np.zeros(n)
+        w0 = np.zeros(n)
+        _check(lib().wtb_bse_plan_refine_results(self.h, n, _p(res), _p(w0)))
+        return res, w0
 
     def set_panel(self, nb):
         """Panel width of the blocked reduction (8..32 columns per trailing update)."""
@@ -655,6 +684,12 @@ class BSEBands:
         h = C.c_void_p()
         _check(lib().wtb_bse_bands_create(ctx.h, C.byref(self._cm.s), C.byref(self._cp), int(lanes), C.byref(h)))
         self.h = h
+
+    def kernel_flops(self):
+        """Algorithmic flops of the timed two-stage trailing updates (0: one-stage GEMV)."""
+        f = np.zeros(1)
+        _check(lib().wtb_bse_plan_kernel_flops(self.h, _p(f)))
+        return float(f[0])
 
     def set_graph(self, on=True):
         """False: launch the kernels directly instead of replaying the captured graphs."""

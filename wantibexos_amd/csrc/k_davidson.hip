@@ -107,6 +107,52 @@ __global__ __launch_bounds__(256) void k_residual(int N, const zc* __restrict__ 
   if (threadIdx.x == 0) rn[c] = sqrt(red[0]);
 }
 
+// ---- Rayleigh-quotient refinement of exciton c (one workgroup per column) -------------------
+// theta = Re(x^H H x) / x^H x from the vector x and its FP64 product Hx; the energy lam[c] is
+// kept in lam0[c] and replaced by theta; res[c] = |Hx - theta x| / |x|, the distance within
+// which the FP64 matrix has an eigenvalue.  Sums in a fixed tree order.
+__global__ __launch_bounds__(256) void k_rq_refine(int N, const zc* __restrict__ X, const zc* __restrict__ HX,
+                                                   int ld, double* __restrict__ lam, double* __restrict__ lam0,
+                                                   double* __restrict__ res) {
+  __shared__ double ra[256], rb[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  double xx = 0.0, xhx = 0.0;
+  for (int i = t; i < N; i += 256) {
+    const size_t o = (size_t)i + (size_t)ld * c;
+    const zc x = X[o], h = HX[o];
+    xx += norm2(x);
+    xhx += x.re * h.re + x.im * h.im;
+  }
+  ra[t] = xx;
+  rb[t] = xhx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      ra[t] += ra[t + w];
+      rb[t] += rb[t + w];
+    }
+    __syncthreads();
+  }
+  const double nx = ra[0], th = rb[0] / nx;
+  __syncthreads();
+  double r2 = 0.0;
+  for (int i = t; i < N; i += 256) {
+    const size_t o = (size_t)i + (size_t)ld * c;
+    r2 += norm2(HX[o] - th * X[o]);
+  }
+  ra[t] = r2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) ra[t] += ra[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    lam0[c] = lam[c];
+    lam[c] = th;
+    res[c] = sqrt(ra[0] / nx);
+  }
+}
+
 // ---- diagonal preconditioner: T(:,q) = R(:,idx q) / (D - theta_idx q), |den| >= dmin --------
 __global__ __launch_bounds__(256) void k_precond_diag(int N, const int* __restrict__ idx, const zc* __restrict__ R,
                                                       int ld, const double* __restrict__ theta,
@@ -591,6 +637,13 @@ int launch_residual(hipStream_t s, int N, int nc, const zc* U, const zc* HU, int
                     zc* R, double* rn) {
   if (nc <= 0) return WTB_OK;
   hipLaunchKernelGGL(k_residual, dim3(nc), dim3(256), 0, s, N, U, HU, ld, theta, R, rn);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+int launch_rq_refine(hipStream_t s, int N, int nc, const zc* X, const zc* HX, int ld, double* lam, double* lam0,
+                     double* res) {
+  if (nc <= 0) return WTB_OK;
+  hipLaunchKernelGGL(k_rq_refine, dim3(nc), dim3(256), 0, s, N, X, HX, ld, lam, lam0, res);
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

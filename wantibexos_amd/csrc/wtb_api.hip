@@ -134,12 +134,29 @@ struct wtb_bse_plan {
   int wf_first = 0, wf_count = 0;
   double* Zv = nullptr;                     // N x N tridiagonal eigenvectors
   zc* Xv = nullptr;                         // N x wf_count exciton vectors
+  // precision form: FP64 Rayleigh-quotient refinement of the nref lowest excitons (0: off).
+  // H holds the FP64 matrix (full Hermitian) beside the reduced COMPLEX*8 copy; Xr the
+  // back-transformed vectors, HXr = H Xr, rq_res the residual norms |H x - theta x| / |x|
+  // stage (d1) form: two-stage (dense -> band -> tridiagonal) when the path needs no
+  // eigenvectors, one-stage Householder otherwise (its reflectors feed the back-transform)
+  int two_stage = 0;
+  int band_b = 32;
+  BandWork bw;
+  void* bandwork = nullptr;
+  int nref = 0;
+  zc* Xr = nullptr;
+  zc* HXr = nullptr;
+  void* btw_r = nullptr;
+  zc* hw_r = nullptr;
+  double* rq_res = nullptr;
+  double* rq_lam0 = nullptr;                // the unrefined energies of those excitons
   void* btw = nullptr;                      // back-transform workspace
   bool timing = false;
   bool kernel_timing = false;              // per-launch events on the GEMV kernel
   std::vector<hipEvent_t> kev;             // 2 per tridiagonal column
   double kernel_ms = 0.0;
   int kernel_launches = 0;
+  double kernel_flops = 0.0;                // two-stage: flops of the timed trailing updates
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -370,7 +387,8 @@ static void plan_free(wtb_bse_plan* p) {
                   p->fdeh, p->gapk, p->wtab, p->H, p->Hf, p->triwork, p->dcwork, p->track, p->lambda, p->act,
                   p->wq, p->actq, p->eps, p->smeavg, p->status, p->Zv, p->Xv, p->btw,
                   p->dq, p->kq, p->eigvq, p->vecq, p->nocpq, p->xa, p->dvq,
-                  p->lw_D, p->lw_stats, p->lw_W, p->lw_R, p->lw_sm, p->lw_X, p->lw_hay};
+                  p->lw_D, p->lw_stats, p->lw_W, p->lw_R, p->lw_sm, p->lw_X, p->lw_hay,
+                  p->Xr, p->HXr, p->btw_r, p->hw_r, p->rq_res, p->rq_lam0, p->bandwork};
   for (void* b : bufs) hipFree(b);
   davidson_work_delete(p->dav);
   for (auto& e : p->ev)
@@ -532,6 +550,13 @@ static int plan_create_impl(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_par
     return rc;
   }
   tridiag_bind(p->tw, p->triwork, N, nb_panel, p->nrhs);
+  if (N >= 2 && ((rc = dmalloc((char**)&p->bandwork, band_work_bytes(N, p->band_b, p->nrhs))) ||
+                 (rc = band_bind(p->bw, p->bandwork, N, p->band_b, p->nrhs)))) {
+    plan_free(p);
+    delete p;
+    return rc;
+  }
+  if (N < 2) p->two_stage = 0;
   if ((rc = dc_prepare(p->dcwork, N, 2 * p->nrhs, p->tree))) {
     plan_free(p);
     delete p;
@@ -562,6 +587,12 @@ int wtb_bse_plan_kernel_ms(const wtb_bse_plan* p, double* total_ms, int* launche
   return WTB_OK;
 }
 
+int wtb_bse_plan_kernel_flops(const wtb_bse_plan* p, double* flops) {
+  if (!p || !flops) return WTB_ERR_ARG;
+  *flops = p->kernel_flops;
+  return WTB_OK;
+}
+
 int wtb_bse_plan_stage_ms(const wtb_bse_plan* p, double* ms) {
   if (!p || !ms) return WTB_ERR_ARG;
   for (int i = 0; i < WTB_NSTAGES; ++i) ms[i] = p->stage_ms[i];
@@ -569,6 +600,28 @@ int wtb_bse_plan_stage_ms(const wtb_bse_plan* p, double* ms) {
 }
 
 static void plan_drop_graph(wtb_bse_plan* p);
+
+// Stage (d1): the lower triangle of the plan's matrix (FP64 or COMPLEX*8) to the real
+// tridiagonal tw.d, tw.e, with Q^T applied to g (may be NULL).  Two-stage when nothing
+// downstream needs the Householder reflectors (exciton vectors, the refinement); its
+// bulge chase reports a stalled wait through status.
+static int plan_reduce(wtb_bse_plan* p, zc* g, int* status) {
+  hipStream_t s = p->stream;
+  const bool vectors = p->wf_count > 0 || (p->mixed && p->nref > 0);
+  if (p->two_stage && !vectors && p->bandwork) {
+    int rc;
+    p->bw.kev = p->kernel_timing ? p->kev.data() : nullptr;
+    p->bw.kev_cap = p->kernel_timing ? p->N : 0;
+    p->tw.kev_used = 0;
+    if (p->mixed) {
+      if ((rc = sym_fill_f32(s, p->N, p->Hf, (int)p->ldh))) return rc;
+      return run_band_tridiag_f32(s, p->bw, p->Hf, (int)p->ldh, g, p->tw.d, p->tw.e, status);
+    }
+    if ((rc = sym_fill_f64(s, p->N, p->H, (int)p->ldh))) return rc;
+    return run_band_tridiag(s, p->bw, p->H, (int)p->ldh, g, p->tw.d, p->tw.e, status);
+  }
+  return p->mixed ? run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, g) : run_tridiag(s, p->tw, p->H, (int)p->ldh, g);
+}
 
 // q mode (bsebnds, subprograms/bse_kpath-tool.f90:187-389): per q, the band windows at k
 // (valence) and k+q (conduction), the matrix with the exchange term, its eigenvalues
@@ -610,11 +663,7 @@ static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
   if (build_only) return WTB_OK;
   p->tw.kev = p->kernel_timing ? p->kev.data() : nullptr;
   p->tw.kev_cap = p->kernel_timing ? p->N : 0;
-  if (p->mixed) {
-    if ((rc = run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, nullptr))) return rc;
-  } else if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, nullptr))) {
-    return rc;
-  }
+  if ((rc = plan_reduce(p, nullptr, p->status + 1))) return rc;
   mark(4);
   if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, nullptr, p->lambda,
                    p->wf_count > 0 ? p->Zv : nullptr)))
@@ -665,11 +714,14 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   if (P.bset && (rc = launch_fermi_factors(s, nk, P.nc, P.nv, p->eigv, p->beta, p->fdeh))) return rc;
   mark(2);
   if (p->sp_only) return WTB_OK;
-  // (c) W table + assembly (lower triangle)
+  // (c) W table + assembly (lower triangle); the refinement of the precision form also
+  // keeps the FP64 matrix, filled to full Hermitian storage for its products
   if ((rc = launch_wtable(s, p->dm.rlat, p->ca, p->wtab))) return rc;
-  if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec, p->H,
-                             (int)p->ldh, p->Hf, p->fdeh)))
+  const bool refine = p->mixed && p->nref > 0;
+  if ((rc = launch_bse_build(s, nk, P.nc, P.nv, nw, P.ngrid, p->wtab, p->eigv, p->vec,
+                             (p->mixed && !refine) ? nullptr : p->H, (int)p->ldh, p->Hf, p->fdeh)))
     return rc;
+  if (refine && (rc = sym_fill_f64(s, N, p->H, (int)p->ldh))) return rc;
   mark(3);
   if (build_only) return WTB_OK;
   // (d) Householder tridiagonalisation, g <- Q^T h (and Q^T f h at finite temperature)
@@ -677,17 +729,22 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   if (P.bset && (rc = launch_scale_rows(s, 3, N, p->dip, p->fdeh, p->g + (size_t)3 * N))) return rc;
   p->tw.kev = p->kernel_timing ? p->kev.data() : nullptr;
   p->tw.kev_cap = p->kernel_timing ? p->N : 0;
-  if (p->mixed) {
-    if ((rc = run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, p->g))) return rc;
-  } else if ((rc = run_tridiag(s, p->tw, p->H, (int)p->ldh, p->g))) {
-    return rc;
-  }
+  if ((rc = plan_reduce(p, p->g, p->status))) return rc;
   mark(4);
   // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
   hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->nrhs, p->g, p->track);
   if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, p->track, p->lambda,
-                   p->wf_count > 0 ? p->Zv : nullptr)))
+                   (p->wf_count > 0 || refine) ? p->Zv : nullptr)))
     return rc;
+  if (refine) {
+    // lowest nref exciton vectors x = Q z (FP32 reflectors, FP64 arithmetic), H x with the
+    // FP64 matrix, then theta = x^H H x / x^H x replaces the energy and |H x - theta x| / |x|
+    // bounds its distance to an eigenvalue of the FP64 matrix
+    const int nr = p->nref;
+    if ((rc = run_backtr_f32(s, p->Hf, (int)p->ldh, N, p->tw.tau, p->Zv, 0, nr, p->Xr, N, p->btw_r))) return rc;
+    if ((rc = hmm_f64(s, N, nr, p->H, (int)p->ldh, p->Xr, N, p->HXr, N, p->hw_r))) return rc;
+    if ((rc = launch_rq_refine(s, N, nr, p->Xr, p->HXr, N, p->lambda, p->rq_lam0, p->rq_res))) return rc;
+  }
   if (p->wf_count > 0) {
     rc = p->mixed ? run_backtr_f32(s, p->Hf, (int)p->ldh, N, p->tw.tau, p->Zv, p->wf_first, p->wf_count, p->Xv, N,
                                    p->btw)
@@ -820,9 +877,12 @@ int wtb_bse_plan_results(wtb_bse_plan* p, double* w, double* act, double* eps, d
     }
   }
   if (p->kernel_timing) {
+    // the one-stage GEMV's launches, or the two-stage trailing updates' (same event array)
+    const int used = p->tw.kev_used > 0 ? p->tw.kev_used : p->bw.kev_used;
     p->kernel_ms = 0.0;
-    p->kernel_launches = p->tw.kev_used;
-    for (int c = 0; c < p->tw.kev_used; ++c) {
+    p->kernel_launches = used;
+    p->kernel_flops = p->tw.kev_used > 0 ? 0.0 : p->bw.kev_flops;
+    for (int c = 0; c < used; ++c) {
       float t;
       if (hipEventElapsedTime(&t, p->kev[2 * c], p->kev[2 * c + 1]) == hipSuccess) p->kernel_ms += t;
     }
@@ -1708,6 +1768,121 @@ int wtb_bse_plan_set_vectors(wtb_bse_plan* p, int first, int count) {
   p->wf_first = first;
   p->wf_count = count;
   plan_drop_graph(p);
+  return WTB_OK;
+}
+
+int wtb_bse_plan_set_reduction(wtb_bse_plan* p, int two_stage, int band) {
+  if (!p || (two_stage != 0 && two_stage != 1) || (band != 16 && band != 32)) {
+    set_error("wtb_bse_plan_set_reduction: two_stage 0 or 1, band 16 or 32");
+    return WTB_ERR_ARG;
+  }
+  if (two_stage && p->N < 2) {
+    set_error("wtb_bse_plan_set_reduction: the two-stage reduction needs N >= 2");
+    return WTB_ERR_ARG;
+  }
+  if (band != p->band_b && p->N >= 2) {
+    WTB_HIP(hipSetDevice(p->ctx->device));
+    WTB_HIP(hipStreamSynchronize(p->stream));
+    void* nw = nullptr;
+    BandWork nb;
+    int rc;
+    if ((rc = dmalloc((char**)&nw, band_work_bytes(p->N, band, p->nrhs)))) return rc;
+    if ((rc = band_bind(nb, nw, p->N, band, p->nrhs))) {
+      hipFree(nw);
+      return rc;
+    }
+    hipFree(p->bandwork);
+    p->bandwork = nw;
+    p->bw = nb;
+    p->band_b = band;
+  }
+  if (two_stage != p->two_stage || band != p->band_b) plan_drop_graph(p);
+  p->two_stage = two_stage;
+  plan_drop_graph(p);
+  return WTB_OK;
+}
+
+int wtb_bse_plan_set_refine(wtb_bse_plan* p, int nref) {
+  if (!p) {
+    set_error("wtb_bse_plan_set_refine: null plan");
+    return WTB_ERR_ARG;
+  }
+  if (nref < 0 || nref > p->N) {
+    set_error("wtb_bse_plan_set_refine: need 0 <= nref <= N");
+    return WTB_ERR_ARG;
+  }
+  if (nref > 0 && (!p->mixed || p->qmode || p->sp_only)) {
+    set_error("wtb_bse_plan_set_refine: the refinement belongs to the precision form (WTB_MIXED) of a Q = 0 plan");
+    return WTB_ERR_ARG;
+  }
+  if (nref == p->nref) return WTB_OK;
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  WTB_HIP(hipStreamSynchronize(p->stream));
+  const int N = p->N;
+  int rc = WTB_OK;
+  // new buffers into temporaries first; the plan changes only when all of them exist
+  const bool need_z = nref > 0 && !p->Zv;
+  const bool need_h = nref > 0 && !p->H;
+  void* ndc = nullptr;
+  double* nzv = nullptr;
+  zc* nh = nullptr;
+  zc *nx = nullptr, *nhx = nullptr, *nhw = nullptr;
+  void* nbt = nullptr;
+  double *nres = nullptr, *nl0 = nullptr;
+  DCTree ntree{};
+  auto drop = [&]() {
+    for (void* b : {(void*)ndc, (void*)nzv, (void*)nh, (void*)nx, (void*)nhx, (void*)nhw, nbt, (void*)nres,
+                    (void*)nl0})
+      hipFree(b);
+  };
+  if (nref > 0) {
+    if ((need_z && ((rc = dmalloc((char**)&ndc, dc_work_bytes(N, 2 * p->nrhs, 1))) ||
+                    (rc = dmalloc(&nzv, (size_t)N * N)) || (rc = dc_prepare(ndc, N, 2 * p->nrhs, ntree, 1)))) ||
+        (need_h && (rc = dmalloc(&nh, p->ldh * N))) || (rc = dmalloc(&nx, (size_t)N * nref)) ||
+        (rc = dmalloc(&nhx, (size_t)N * nref)) || (rc = dmalloc((char**)&nbt, backtr_work_bytes(N, nref))) ||
+        (rc = dmalloc((char**)&nhw, hmm_work_bytes(N, nref))) || (rc = dmalloc(&nres, nref)) ||
+        (rc = dmalloc(&nl0, nref))) {
+      drop();
+      return rc;
+    }
+  }
+  if (need_z) {
+    hipFree(p->dcwork);
+    p->dcwork = ndc;
+    p->Zv = nzv;
+    p->tree = ntree;
+    p->tw.store_v = 1;
+  }
+  if (need_h) p->H = nh;
+  for (void* b : {(void*)p->Xr, (void*)p->HXr, p->btw_r, (void*)p->hw_r, (void*)p->rq_res, (void*)p->rq_lam0})
+    hipFree(b);
+  p->Xr = nx;
+  p->HXr = nhx;
+  p->btw_r = nbt;
+  p->hw_r = nhw;
+  p->rq_res = nres;
+  p->rq_lam0 = nl0;
+  p->nref = nref;
+  plan_drop_graph(p);
+  return WTB_OK;
+}
+
+int wtb_bse_plan_refine_results(wtb_bse_plan* p, int n, double* resid, double* w_unrefined) {
+  if (!p || n < 0 || n > p->nref || (n > 0 && !resid)) {
+    set_error("wtb_bse_plan_refine_results: need 0 <= n <= nref of wtb_bse_plan_set_refine");
+    return WTB_ERR_ARG;
+  }
+  if (n == 0) return WTB_OK;
+  if (!p->filled) {
+    set_error("wtb_bse_plan_refine_results: execute the plan first");
+    return WTB_ERR_ARG;
+  }
+  WTB_HIP(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  WTB_HIP(hipStreamSynchronize(p->stream));
+  WTB_HIP(hipMemcpyAsync(resid, p->rq_res, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (w_unrefined) WTB_HIP(hipMemcpyAsync(w_unrefined, p->rq_lam0, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  WTB_HIP(hipStreamSynchronize(s));
   return WTB_OK;
 }
 

@@ -92,6 +92,34 @@ int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g);
 // the same reduction on a COMPLEX*8 matrix (precision form)
 int run_tridiag_f32(hipStream_t s, TridiagWork& tw, zf* d_H, int ldh, zc* d_g);
 
+// stage (d), two-stage form: dense -> band of width b (16 or 32) -> real tridiagonal
+// (d, e), applying Q^T to nrhs (<= 6) complex vectors g.  The matrix is in FULL Hermitian
+// storage (both triangles) and is overwritten.  A wait of the bulge-chasing kernel that
+// passes its bound sets WTB_ERR_EIG in status[0].
+struct BandWork {
+  int N = 0, b = 32, nrhs = 0, LB = 0, nwg_max = 0;
+  zc *Pw = nullptr, *V = nullptr, *Y = nullptr, *X = nullptr;
+  zc *part = nullptr, *rowc = nullptr, *tau = nullptr;
+  zc *G = nullptr, *t1 = nullptr, *T = nullptr, *t2 = nullptr, *M = nullptr;
+  zc* gw = nullptr;
+  size_t gw_elems = 0;
+  zc* hw = nullptr;
+  zc* BD = nullptr;
+  zc* gc = nullptr;
+  int* prog = nullptr;
+  int* err = nullptr;
+  // optional per-launch timing of the trailing-update kernel (k_her2k_full): launch c is
+  // bracketed by kev[2c], kev[2c+1] (c < kev_cap); kev_flops[c] its flop count
+  hipEvent_t* kev = nullptr;
+  int kev_cap = 0;
+  int kev_used = 0;
+  double kev_flops = 0.0;       // summed over the timed launches
+};
+size_t band_work_bytes(int N, int b, int nrhs);
+int band_bind(BandWork& w, void* base, int N, int b, int nrhs);
+int run_band_tridiag(hipStream_t s, BandWork& w, zc* H, int ld, zc* g, double* d, double* e, int* status);
+int run_band_tridiag_f32(hipStream_t s, BandWork& w, zf* H, int ld, zc* g, double* d, double* e, int* status);
+
 // stage (d): tridiagonal divide and conquer (eigenvalues + Z^T x for tracked vectors)
 struct DCTree {
   int N = 0, ntrack = 0, vectors = 0;
@@ -159,6 +187,10 @@ int launch_lifetime(hipStream_t s, int n, int ncol, int nout, int sysdim, double
 int sym_fill_f32(hipStream_t s, int N, zf* H, int ld);
 int sym_fill_f64(hipStream_t s, int N, zc* H, int ld);
 int row_stats_f32(hipStream_t s, int N, const zf* H, int ld, double* D, float* st);
+// theta = x^H H x / x^H x for the nc columns of X (FP64 products HX): lam[c] -> lam0[c],
+// theta -> lam[c], |H x - theta x| / |x| -> res[c]
+int launch_rq_refine(hipStream_t s, int N, int nc, const zc* X, const zc* HX, int ld, double* lam, double* lam0,
+                     double* res);
 int launch_residual(hipStream_t s, int N, int nc, const zc* U, const zc* HU, int ld, const double* theta,
                     zc* R, double* rn);
 int launch_precond_diag(hipStream_t s, int N, int nq, const int* idx, const zc* R, int ld,

@@ -163,7 +163,7 @@ subroutine wtb_bse_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse,s
 	p%cpol = merge(1, 0, cpol)
 	p%renorm = 1
 	p%f154 = 1
-	p%precision = WTB_FP64
+	p%precision = wtb_form_from_env()
 	p%bset = merge(1, 0, bset)
 	p%ta = WTB_TA_FA
 	if (ta .eq. "VE") p%ta = WTB_TA_VE
@@ -506,7 +506,7 @@ subroutine wtb_bands_dropin(nthreads,outputfolder,ngrid,nc,nv,ebse0,ebsef,numbse
 	p%cpol = 0
 	p%renorm = 0
 	p%f154 = 0
-	p%precision = WTB_FP64
+	p%precision = wtb_form_from_env()
 	p%bset = merge(1, 0, bset)
 	p%ta = WTB_TA_FA
 	if (ta .eq. "VE") p%ta = WTB_TA_VE

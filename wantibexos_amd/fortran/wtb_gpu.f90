@@ -159,6 +159,20 @@ contains
 		end select
 	end function
 
+	! the arithmetic form of the drop-ins: WTB_GPU_PRECISION=mixed in the environment selects
+	! the precision form (COMPLEX*8 matrix and reduction, FP64 tridiagonal solve); unset or
+	! any other value keeps the plain FP64 form, the reference's arithmetic.  (The
+	! reference's input parser is used unchanged, so the choice cannot be an input key.)
+	integer(c_int) function wtb_form_from_env()
+		character(len=32) :: v
+		integer :: n, st
+		wtb_form_from_env = WTB_FP64
+		call get_environment_variable("WTB_GPU_PRECISION", v, n, st)
+		if (st == 0 .and. n > 0) then
+			if (trim(adjustl(v)) == "mixed" .or. trim(adjustl(v)) == "MIXED") wtb_form_from_env = WTB_MIXED
+		end if
+	end function
+
 	! map a status to the reference's STOP text and stop, as the reference does
 	subroutine wtb_check(rc)
 		integer(c_int) :: rc

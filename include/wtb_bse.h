@@ -160,6 +160,18 @@ int wtb_bse_spectrum(wtb_ctx* ctx, int n, const double* e, const double* act6,
                      const double* rlat, const int ngrid[3], double ebse0, double ebsef,
                      int numbse, double sme, int renorm, double* eps);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * wtb_bse_spectrum with its staging and partial results in a library-owned static
+ * device array (2^20 doubles, 8 MiB, held by every device the library runs on until the
+ * process ends) instead of allocations: it calls no hipMalloc or hipFree.  Same
+ * arguments and results as wtb_bse_spectrum.  Limit: 7 n + 12 numbse + 4 <= 2^20
+ * (else WTB_ERR_LIMIT).  Every call on a device shares that device's array, so calls
+ * that overlap on one device race. */
+int wtb_bse_spectrum_noalloc(wtb_ctx* ctx, int n, const double* e, const double* act6,
+                             const double* rlat, const int ngrid[3], double ebse0, double ebsef,
+                             int numbse, double sme, int renorm, double* eps);
+
 /* bseoptprop and spoptprop (subprograms/diel-pp-bse.f90:3-173, diel-pp.f90:3-172) and
  * their circular-polarisation forms bseoptproppol, spoptproppol (diel-pp-bse-pol.f90,
  * diel-pp-pol.f90): the optical constants of subroutines/diel-pp-subs.f90:1-73 for ncomp

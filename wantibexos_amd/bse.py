@@ -151,6 +151,8 @@ def _load():
         "wtb_bse_plan_set_reduction": (ip, [vp, ip, ip]),
         "wtb_bse_plan_kernel_flops": (ip, [vp, vp]),
         "wtb_bse_plan_refine_results": (ip, [vp, ip, vp, vp]),
+        "wtb_bse_spectrum_noalloc": (ip, [vp, ip, vp, vp, vp, vp, C.c_double, C.c_double, ip, C.c_double,
+                                          ip, vp]),
         "wtb_exciton_lifetime": (ip, [vp, ip, vp, vp, ip, ip, ip, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -364,6 +366,19 @@ def exciton_lifetime(ctx: Context, sysdim: str, ngrid, rlat, table, nout: int):
     _check(lib().wtb_exciton_lifetime(ctx.h, SYSDIM.get(sysdim, 0), _p(ng), _p(rl), n, ncol - 1, nout, _p(t),
                                       _p(out)))
     return out
+
+
+def bse_spectrum_noalloc(ctx: Context, e, act6, rlat, ngrid, ebse0, ebsef, numbse, sme, renorm=True):
+    """bse_spectrum without device allocations (the library's static scratch array;
+    7 n + 12 numbse + 4 <= 2^20)."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    a6 = np.asfortranarray(np.asarray(act6, dtype=np.float64).T)
+    rl = np.asfortranarray(np.asarray(rlat, dtype=np.float64))
+    ng = np.array(ngrid, dtype=np.int32)
+    eps = np.zeros((6, 2, int(numbse)))
+    _check(lib().wtb_bse_spectrum_noalloc(ctx.h, len(e), _p(e), _p(a6), _p(rl), _p(ng), float(ebse0),
+                                          float(ebsef), int(numbse), float(sme), int(bool(renorm)), _p(eps)))
+    return eps
 
 
 class BSEPlan:

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,34 @@ static int dmalloc(T** p, size_t count) {
     *p = nullptr;
     return WTB_ERR_NOMEM;
   }
+  return WTB_OK;
+}
+
+// Scratch memory of wtb_bse_spectrum_noalloc: a file-scope device array (every GPU the
+// library runs on holds its own copy until the process ends).  Host code never uses the
+// array's name as an address: wtb_create looks the copy of the current device up with
+// hipGetSymbolAddress once per device and keeps that pointer in g_scratch_dev.
+constexpr size_t kScratchDoubles = size_t(1) << 20;   // 8 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// the scratch array's address on `device` (the current device), looked up once
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("scratch: device index beyond the library's table");
+    return WTB_ERR_LIMIT;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return WTB_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return WTB_ERR_HIP;
+  }
+  g_scratch_dev[device] = static_cast<double*>(ptr);
   return WTB_OK;
 }
 
@@ -203,6 +232,9 @@ int wtb_create(int device, wtb_ctx** out) {
     set_error(std::string("wtb_create: device is ") + prop.gcnArchName + ", this library is built for gfx950");
     return WTB_ERR_HIP;
   }
+  // the no-allocation entry point's scratch array on this device (device is current)
+  int rc = scratch_setup(device);
+  if (rc) return rc;
   wtb_ctx* c = new wtb_ctx;
   c->device = device;
   WTB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1085,6 +1117,44 @@ int wtb_bse_spectrum(wtb_ctx* c, int n, const double* e, const double* act6, con
   hipFree(da);
   hipFree(dw);
   hipFree(deps);
+  return rc;
+}
+
+int wtb_bse_spectrum_noalloc(wtb_ctx* c, int n, const double* e, const double* act6, const double* rlat,
+                             const int ngrid[3], double ebse0, double ebsef, int numbse, double sme, int renorm,
+                             double* eps) {
+  if (!c || !e || !act6 || !rlat || !ngrid || !eps || n < 1 || numbse < 2) {
+    set_error("wtb_bse_spectrum_noalloc: bad argument");
+    return WTB_ERR_ARG;
+  }
+  // layout in the scratch array: e [n], act6 [6 n], eps [12 numbse], avgsme [4]
+  const size_t need = 7 * (size_t)n + 12 * (size_t)numbse + 4;
+  if (need > kScratchDoubles) {
+    set_error("wtb_bse_spectrum_noalloc: 7 n + 12 numbse + 4 = " + std::to_string(need) +
+              " doubles exceeds the scratch array (" + std::to_string(kScratchDoubles) + ")");
+    return WTB_ERR_LIMIT;
+  }
+  double* base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_dev[c->device] : nullptr;
+  if (!base) {
+    set_error("wtb_bse_spectrum_noalloc: the scratch array of this device was not set up");
+    return WTB_ERR_HIP;
+  }
+  WTB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double* de = base;
+  double* da = de + n;
+  double* deps = da + (size_t)6 * n;
+  double* dw = deps + (size_t)12 * numbse;
+  int rc = WTB_OK;
+  if (hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(da, act6, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+    rc = WTB_ERR_HIP;
+  const double nkf = (double)(float)(ngrid[0] * ngrid[1] * ngrid[2]);
+  if (!rc) rc = launch_spectrum(s, n, de, da, vcell3d_host(rlat), nkf, ebse0, ebsef, numbse, sme, renorm, dw, deps);
+  if (!rc && hipMemcpyAsync(eps, deps, sizeof(double) * 12 * numbse, hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = WTB_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = WTB_ERR_HIP;
+  if (rc == WTB_ERR_HIP) set_error("wtb_bse_spectrum_noalloc: HIP error");
   return rc;
 }
 

@@ -13,11 +13,11 @@
 //                              order), apply it to the panel columns right of it, and form
 //                              the partial sums of column c (|x|^2 and x^H P(:, j))
 //   k_pfin                     explicit V (unit diagonal) and R back into the matrix
-//   zgemm                      G = V^H V, t1 = V^H g (MFMA, K split over workgroups)
+//   k_gram, k_gram_sum         G = V^H V, t1 = V^H g (FP64 MFMA on row chunks, partials summed in order)
 //   k_pT                       T (ZLARFT forward/columnwise) and t2 = T^H t1
 //   k_pY                       Y = V T, g -= V t2 (rows of the panel)
 //   hmm                        X = A22 Y (MFMA, full Hermitian storage)
-//   zgemm                      M = V^H X
+//   k_gram, k_gram_sum         M = V^H X
 //   k_pW                       W = X - 1/2 V (T^H M)
 //   k_her2k_full               A22 -= V W^H + W V^H (MFMA, every tile of the trailing matrix)
 // The tracked vectors are carried conjugated (gc = conj(g)): Q^H gc = conj(Q^T g).
@@ -26,11 +26,11 @@
 // annihilates column s below its subdiagonal; its tasks k = 0, 1, ... each apply one
 // reflector (ZLARFG) two-sidedly to a diagonal block and the previous one from the right
 // to the block below it (creating the bulge whose first column the next reflector
-// annihilates).  One persistent kernel: workgroup w runs the sweeps w, w + P, ...; task k
-// of sweep s waits until sweep s - 1 has finished task k + 2 (their blocks then share no
-// row), signalled through a per-sweep progress counter with agent-scope release/acquire.
-// Every wait is bounded: past the limit the kernel sets an error flag that every waiting
-// workgroup sees, and all of them return.
+// annihilates).  One persistent kernel whose workgroups claim sweeps in order from a
+// counter; task k of sweep s waits until sweep s - 1 has finished its tasks 0..k+1 (its
+// later tasks touch only rows below this task's), signalled through a per-sweep progress
+// counter with agent-scope release/acquire.  Every wait is bounded: past the limit the
+// kernel sets an error flag that every waiting workgroup sees, and all of them return.
 #include "wtb_kernels.h"
 
 namespace wtb {
@@ -50,6 +50,9 @@ __device__ inline zc zdiv(zc a, zc b) {
   const double r = b.re / b.im, den = b.im + b.re * r;
   return mkz((a.re * r + a.im) / den, (a.im * r - a.re) / den);
 }
+
+__device__ inline zc shfl_z(zc v, int src) { return mkz(__shfl(v.re, src), __shfl(v.im, src)); }
+__device__ inline zc shfl_xor_z(zc v, int off) { return mkz(__shfl_xor(v.re, off), __shfl_xor(v.im, off)); }
 
 // ZLARFG on (alpha, x) with |x|^2 = xn2: beta (real, returned as the new alpha), tau and
 // the scale 1/(alpha - beta) of x.  tau = 0 (H = I) when x = 0 and alpha is real.
@@ -183,10 +186,10 @@ __global__ __launch_bounds__(PR) void k_pqr_col(PanelArgs<T> a, int c) {
     const int cp = c - 1;
     const zc* pp = a.part + (size_t)(cp & 1) * a.nwg * (BMAX + 1);
     if ((int)threadIdx.x < B + 1) {
+      // sh[0].re = |x|^2 below the diagonal, sh[1 + j] = x^H P(:, j): the workgroups' partial
+      // sums in workgroup order, eight loads in flight
       const int q = threadIdx.x;
-      zc s = mkz(0, 0);
-      for (int w = 0; w < a.nwg; ++w) s += pp[(size_t)w * (BMAX + 1) + q];
-      sh[q] = s;                       // sh[0].re = |x|^2 below the diagonal, sh[1 + j] = x^H P(:, j)
+      sh[q] = strided_sum(pp + q, BMAX + 1, 0, a.nwg, 1, mkz(0, 0));
     }
     __syncthreads();
     const zc* rc = a.rowc + (cp & 1) * BMAX;
@@ -272,33 +275,48 @@ __global__ __launch_bounds__(PR) void k_pfin(PanelArgs<T> a, zc* __restrict__ V)
 }
 
 // one workgroup: T (ZLARFT, forward, columnwise) from G = V^H V and tau, and t2 = T^H t1
-// (t1 = V^H gc); G and t1 column-major with ld B, T stored [col][row]
+// (t1 = V^H gc); G and t1 column-major with ld B, T stored [col][row].  T = (I + D U)^-1 D
+// with D = diag(tau), U the strict upper triangle of G (the closed form of ZLARFT's
+// recurrence T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c), T(c, c) = tau_c, which also gives
+// T(:, c) = 0 when tau_c = 0), so every column is an independent back substitution:
+// column c by the eight lanes t = 8c .. 8c + 7, lane j holding T(q, c) for q = j (mod 8).
 template <int B>
 __global__ __launch_bounds__(256) void k_pT(int k, const zc* __restrict__ G, const zc* __restrict__ t1,
                                             const zc* __restrict__ tau, int nrhs, zc* __restrict__ Tm,
                                             zc* __restrict__ t2) {
   __shared__ zc Gs[B * B];
   __shared__ zc Ts[B * B];
-  for (int q = threadIdx.x; q < B * B; q += 256) {
-    Gs[q] = G[q];                        // G(r, c) at [c][r]
-    Ts[q] = mkz(0, 0);
+  __shared__ zc taus[B];
+  for (int q = threadIdx.x; q < B * B; q += 256) Gs[q] = G[q];   // G(r, c) at [c][r]
+  if ((int)threadIdx.x < B) taus[threadIdx.x] = ((int)threadIdx.x < k) ? tau[threadIdx.x] : mkz(0, 0);
+  __syncthreads();
+  const int c = threadIdx.x >> 3, j = threadIdx.x & 7;
+  constexpr int NU = B / 8;
+  if (c < B) {
+    zc tq[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) tq[u] = (j + 8 * u == c) ? taus[c] : mkz(0, 0);
+    for (int r = c - 1; r >= 0; --r) {
+      zc acc = mkz(0, 0);
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int q = j + 8 * u;
+        if (q > r && q <= c) acc = fma_z(Gs[q * B + r], tq[u], acc);
+      }
+#pragma unroll
+      for (int off = 1; off < 8; off <<= 1) {
+        acc.re += __shfl_xor(acc.re, off);
+        acc.im += __shfl_xor(acc.im, off);
+      }
+      const zc val = mkz(0, 0) - taus[r] * acc;
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+        if (j + 8 * u == r) tq[u] = val;
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) Ts[c * B + j + 8 * u] = tq[u];
   }
   __syncthreads();
-  // T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c), T(c, c) = tau_c
-  for (int c = 0; c < k; ++c) {
-    const zc tc = tau[c];
-    zc val = mkz(0, 0);
-    if ((int)threadIdx.x < c) {
-      const int r = threadIdx.x;
-      zc sacc = mkz(0, 0);
-      for (int q = r; q < c; ++q) sacc += Ts[q * B + r] * Gs[c * B + q];
-      val = mkz(0, 0) - tc * sacc;
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < c) Ts[c * B + threadIdx.x] = val;
-    if (threadIdx.x == 0) Ts[c * B + c] = tc;
-    __syncthreads();
-  }
   for (int q = threadIdx.x; q < B * B; q += 256) Tm[q] = Ts[q];
   // t2(p, r) = sum_q conj(T(q, p)) t1(q, r), T(q, p) = Ts[p][q], q <= p
   for (int q = threadIdx.x; q < B * 6; q += 256) {
@@ -344,10 +362,17 @@ template <int B>
 __global__ __launch_bounds__(PR) void k_pW(int m, int N, const zc* __restrict__ V, const zc* __restrict__ Tm,
                                            const zc* __restrict__ M, zc* __restrict__ X) {
   __shared__ zc Zs[B * B];
+  __shared__ zc Ts[B * B];
+  __shared__ zc Ms[B * B];
   for (int q = threadIdx.x; q < B * B; q += PR) {
-    const int p = q / B, c = q % B;      // Z(p, c) = sum_r conj(T(r, p)) M(r, c), T(r, p) = Tm[p][r], r <= p
+    Ts[q] = Tm[q];
+    Ms[q] = M[q];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < B * B; q += PR) {
+    const int p = q / B, c = q % B;      // Z(p, c) = sum_r conj(T(r, p)) M(r, c), T(r, p) = Ts[p][r], r <= p
     zc sacc = mkz(0, 0);
-    for (int r = 0; r <= p; ++r) sacc = fma_conj(Tm[p * B + r], M[(size_t)c * B + r], sacc);
+    for (int r = 0; r <= p; ++r) sacc = fma_conj(Ts[p * B + r], Ms[c * B + r], sacc);
     Zs[q] = sacc;
   }
   __syncthreads();
@@ -363,6 +388,87 @@ __global__ __launch_bounds__(PR) void k_pW(int m, int N, const zc* __restrict__ 
     for (int p = 0; p < B; ++p) sacc = fma_z(v[p], Zs[p * B + c], sacc);
     X[i + (size_t)N * c] -= 0.5 * sacc;
   }
+}
+
+// Tall-skinny products out0 = A^H Y0 (B x B) and out1 = A^H Y1 (B x n1, n1 <= 16) over m
+// rows (A, Y1 with ld N, Y0 with ld ldy0): FP64 MFMA on 64-row chunks staged through LDS,
+// chunks c = blockIdx.x, + gridDim.x, ..., one partial per workgroup, summed in workgroup
+// order by k_gram_sum.  Wave w accumulates the 16 x 16 output tiles w and w + 4.
+#define GRAM_ROWS 64
+#define GRAM_PMAX 64
+template <int B>
+__global__ __launch_bounds__(256) void k_gram(int m, int N, const zc* __restrict__ A, const zc* __restrict__ Y0,
+                                              int ldy0, const zc* __restrict__ Y1, int n1, zc* __restrict__ part) {
+  constexpr int NY = B + 16;                 // columns of [Y0 | Y1]
+  constexpr int TI = B / 16;                 // output tile rows
+  __shared__ zc As[GRAM_ROWS][B];
+  __shared__ zc Ys[GRAM_ROWS][NY];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int ntile = TI * (TI + (n1 > 0 ? 1 : 0));
+  d4 acc[2][2];                              // [tile slot][re, im]
+  for (int x = 0; x < 2; ++x) acc[x][0] = acc[x][1] = d4{0, 0, 0, 0};
+  const int nch = (m + GRAM_ROWS - 1) / GRAM_ROWS;
+  for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const int row0 = ch * GRAM_ROWS;
+    __syncthreads();
+    for (int e = t; e < GRAM_ROWS * B; e += 256) {
+      const int rr = e % GRAM_ROWS, cc = e / GRAM_ROWS, row = row0 + rr;
+      As[rr][cc] = row < m ? A[row + (size_t)N * cc] : mkz(0, 0);
+      Ys[rr][cc] = row < m ? Y0[row + (size_t)ldy0 * cc] : mkz(0, 0);
+    }
+    for (int e = t; e < GRAM_ROWS * 16; e += 256) {
+      const int rr = e % GRAM_ROWS, cc = e / GRAM_ROWS, row = row0 + rr;
+      Ys[rr][B + cc] = (row < m && cc < n1) ? Y1[row + (size_t)N * cc] : mkz(0, 0);
+    }
+    __syncthreads();
+    for (int x = 0; x < 2; ++x) {
+      const int tile = wave + 4 * x;
+      if (tile >= ntile) continue;
+      const int ti = tile % TI, tj = tile / TI;
+      const int i = ti * 16 + (lane & 15), j = tj * 16 + (lane & 15);
+#pragma unroll 4
+      for (int k0 = 0; k0 < GRAM_ROWS; k0 += 4) {
+        const int kk = k0 + (lane >> 4);
+        const zc av = As[kk][i], yv = Ys[kk][j];
+        // conj(a) y = (ar yr + ai yi) + i (ar yi - ai yr)
+        acc[x][0] = mfma16x16x4(av.re, yv.re, acc[x][0]);
+        acc[x][0] = mfma16x16x4(av.im, yv.im, acc[x][0]);
+        acc[x][1] = mfma16x16x4(av.re, yv.im, acc[x][1]);
+        acc[x][1] = mfma16x16x4(-av.im, yv.re, acc[x][1]);
+      }
+    }
+  }
+  // partial: [blockIdx][B][NY] (row i of the output, column j of [Y0 | Y1])
+  zc* pp = part + (size_t)blockIdx.x * B * NY;
+  for (int x = 0; x < 2; ++x) {
+    const int tile = wave + 4 * x;
+    if (tile >= ntile) continue;
+    const int ti = tile % TI, tj = tile / TI;
+    for (int q = 0; q < 4; ++q) {
+      const int i = ti * 16 + (lane >> 4) + 4 * q, j = tj * 16 + (lane & 15);
+      pp[(size_t)i * NY + j] = mkz(acc[x][0][q], acc[x][1][q]);
+    }
+  }
+}
+
+// out0 (B x B, ld B) and out1 (B x n1, ld B) from the P partials, in workgroup order: four
+// threads per output each add a quarter of the partials, then the quarters in order
+template <int B>
+__global__ __launch_bounds__(256) void k_gram_sum(int P, const zc* __restrict__ part, int n1, zc* __restrict__ out0,
+                                                  zc* __restrict__ out1) {
+  constexpr int NY = B + 16;
+  const int o = blockIdx.x * 64 + (threadIdx.x >> 2), sub = threadIdx.x & 3;
+  const int i = o / NY, j = o % NY;
+  const bool live = i < B && (j < B || j - B < n1);
+  const int per = (P + 3) / 4, w0 = sub * per, w1 = min(P, w0 + per);
+  zc x = mkz(0, 0);
+  if (live && w0 < w1) x = strided_sum(part + (size_t)i * NY + j, (size_t)B * NY, w0, w1, 1, mkz(0, 0));
+  const zc x1 = shfl_xor_z(x, 1);
+  const zc x2 = shfl_xor_z(x, 2), x3 = shfl_xor_z(x, 3);
+  if (!live || sub != 0) return;
+  const zc sum = ((x + x1) + x2) + x3;       // quarters 0, 1, 2, 3
+  if (j < B) out0[i + (size_t)B * j] = sum;
+  else out1[i + (size_t)B * (j - B)] = sum;
 }
 
 // A22 -= U, U = V W^H + W V^H, on every 64 x 64 tile of the trailing matrix (rows and
@@ -457,8 +563,7 @@ struct ChaseArgs {
   zc* gc;            // nrhs x N (conjugated tracked vectors)
   int nrhs;
   int* prog;         // [N] tasks completed per sweep
-  int* err;          // set when a wait passes its bound
-  int P;             // workgroups
+  int* ctl;          // [0] error flag, [32] next sweep to claim
   long long spin_limit;
 };
 
@@ -467,212 +572,196 @@ __device__ inline int chase_ntasks(int N, int b, int s) { return 1 + (N - 2 - s)
 // element (r, c) of the band, r >= c
 __device__ inline zc& bd(const ChaseArgs& a, int r, int c) { return a.BD[(size_t)c * a.LB + (r - c)]; }
 
+
+// sum over the B lanes of a row group (lane bits below log2 B), every lane gets it
+template <int B>
+__device__ inline zc group_sum(zc x) {
+#pragma unroll
+  for (int off = 1; off < B; off <<= 1) x += shfl_xor_z(x, off);
+  return x;
+}
+template <int B>
+__device__ inline double group_sum(double x) {
+#pragma unroll
+  for (int off = 1; off < B; off <<= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// Column sums of a B x B block held as thread (r, h) = (t % B, t / B) owning row r and
+// columns h NC .. h NC + NC - 1 (NC = B^2 / 256); the B rows of a column group lie in one
+// wave.  Halving butterfly over the top row bits (NC - 1 exchanges), then plain exchanges
+// over the rest: thread (r, h) ends with the sum of local column r >> (log2 B - log2 NC).
+template <int B, int NC>
+__device__ inline zc col_sums(zc (&t)[NC], int lane) {
+  if constexpr (NC == 4) {
+    static_assert(B == 32, "four columns per thread for B = 32");
+    bf_step<2>(t, lane, 16);
+    bf_step<1>(t, lane, 8);
+    zc x = t[0];
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) x += shfl_xor_z(x, off);
+    return x;
+  } else {
+    static_assert(B == 16 && NC == 1, "one column per thread for B = 16");
+    return group_sum<16>(t[0]);
+  }
+}
+
+// Row sums: a thread's own columns, the column groups of its wave, then the four waves
+// through LDS in wave order (red: [4][B]); every thread gets the sum of its row.
+template <int B>
+__device__ inline zc row_sum(zc x, zc* red, int t) {
+#pragma unroll
+  for (int off = B; off < 64; off <<= 1) x += shfl_xor_z(x, off);
+  const int wave = t >> 6, lane = t & 63;
+  if (lane < B) red[wave * B + lane] = x;
+  __syncthreads();
+  const int r = t & (B - 1);
+  return ((red[r] + red[B + r]) + red[2 * B + r]) + red[3 * B + r];
+}
+
+// One sweep per 256-thread workgroup, claimed in order from a counter; task k of sweep
+// s waits until sweep s - 1 has finished its tasks 0..k+1 (the later tasks of sweep
+// s - 1 touch only rows below this task's last row).  Thread (r, h) keeps row r, columns
+// h NC .. of the task's blocks in registers: column sums stay inside a wave, row sums
+// cross the waves through LDS (two barriers per task), and the reflector's scalars and
+// v are formed redundantly by every column group from column 0, which every thread
+// holds for its row.
 template <int B>
 __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
-  __shared__ zc O[B][B + 1];      // off-diagonal block (rows st.., columns of the previous block)
-  __shared__ zc D[B][B + 1];      // diagonal block, full Hermitian
-  __shared__ zc vprev[B], vcur[B], wv[B];
-  __shared__ zc sc[4];            // tau_prev, tau, beta, misc
-  __shared__ int stop;
-  const int t = threadIdx.x;
-  const int N = a.N, b = a.b;
-  for (int s = blockIdx.x; s <= N - 2; s += a.P) {
+  constexpr int LB2 = (B == 32) ? 5 : 4;
+  constexpr int NC = B * B / 256;
+  constexpr int LNC = (NC == 4) ? 2 : 0;
+  constexpr int NG = 256 / B;                  // column groups
+  __shared__ zc red1[4 * B], red2[4 * B];
+  __shared__ int flag;
+  const int t = threadIdx.x, lane = t & 63;
+  const int r = t & (B - 1), h = t >> LB2;
+  const int c0 = h * NC;
+  const int gb = lane & ~(B - 1);              // lane of row 0 of this thread's group
+  const int N = a.N, b = B;
+  int* const err = a.ctl;
+  for (;;) {
+    if (t == 0) flag = atomicAdd(a.ctl + 32, 1);
+    __syncthreads();
+    const int s = flag;
+    __syncthreads();
+    if (s > N - 2) return;
     const int nt = chase_ntasks(N, b, s);
-    int lenp = 0;                 // length of the previous reflector
+    const int ntp = s > 0 ? chase_ntasks(N, b, s - 1) : 0;
+    int lenp = 0;
+    zc taup = mkz(0, 0);
+    zc vp[NC];                                 // previous reflector at this thread's columns
+#pragma unroll
+    for (int q = 0; q < NC; ++q) vp[q] = mkz(0, 0);
     for (int k = 0; k < nt; ++k) {
-      // wait for sweep s - 1 to have finished task k + 2 (or all of its tasks)
       if (s > 0) {
         if (t == 0) {
-          const int need = min(k + 3, chase_ntasks(N, b, s - 1));
+          const int need = min(k + 2, ntp);
           long long n = 0;
           int bad = 0;
           while (__hip_atomic_load(&a.prog[s - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-            if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++n > a.spin_limit) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++n > a.spin_limit) {
               bad = 1;
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               break;
             }
             __builtin_amdgcn_s_sleep(1);
           }
-          if (bad) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the data sweep s - 1 wrote
-          stop = bad;
+          flag = bad;
         }
         __syncthreads();
-        if (stop) return;
+        if (flag) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the blocks sweep s - 1 wrote
       }
+      // rows st..ed of this task; for k >= 1 the previous reflector's rows sp..sp+lenp-1
+      const int sp = s + 1 + (k - 1) * b;
+      const int st = (k == 0) ? s + 1 : sp + lenp;
+      const int ed = min(st + b - 1, N - 1), len = ed - st + 1;
+      const bool rin = r < len;
+      // loads: diagonal block (full Hermitian), the block left of it (k >= 1) and its
+      // column 0 for every row, or column s (k = 0); the tracked vector of this group
+      zc D[NC], O[NC];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const int c = c0 + q;
+        zc v = mkz(0, 0);
+        if (rin && c < len) v = (r >= c) ? bd(a, st + r, st + c) : conj(bd(a, st + c, st + r));
+        D[q] = v;
+        zc o = mkz(0, 0);
+        if (k > 0 && rin && c < lenp) o = bd(a, st + r, sp + c);
+        O[q] = o;
+      }
+      zc x0 = mkz(0, 0);
+      if (rin) x0 = (k == 0) ? bd(a, st + r, s) : bd(a, st + r, sp);
+      const bool gq = h < a.nrhs;
+      zc g = (rin && gq) ? a.gc[(size_t)h * N + st + r] : mkz(0, 0);
+      if (k > 0) {
+        // (i) the block times the previous reflector from the right: O -= taup (O vp) vp^H
+        zc w = mkz(0, 0);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) w = fma_z(O[q], vp[q], w);
+        w = row_sum<B>(w, red1, t);
+        const zc tw = taup * w;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) O[q] = O[q] - tw * conj(vp[q]);
+        x0 = x0 - tw;                          // column 0: vp[0] = 1
+      }
+      // (ii) reflector of x = rows st..ed of the column (ZLARFG), in every column group
+      const double xn2 = group_sum<B>((r >= 1 && rin) ? norm2(x0) : 0.0);
+      const zc alpha = shfl_z(x0, gb);
+      zc beta, tau, scale;
+      larfg(alpha, xn2, beta, tau, scale);
+      const zc vr = (r == 0) ? mkz(1, 0) : (rin ? x0 * scale : mkz(0, 0));
+      zc vc[NC];                               // v at this thread's columns
+#pragma unroll
+      for (int q = 0; q < NC; ++q) vc[q] = shfl_z(vr, gb + c0 + q);
       if (k == 0) {
-        // column s, rows s+1..ed: reflector H0; D = A(st..ed, st..ed)
-        const int st = s + 1, ed = min(s + b, N - 1), len = ed - st + 1;
-        if (t < B) vcur[t] = (t < len) ? bd(a, st + t, s) : mkz(0, 0);
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          zc v = mkz(0, 0);
-          if (r < len && c < len) v = r >= c ? bd(a, st + r, st + c) : conj(bd(a, st + c, st + r));
-          D[r][c] = v;
-        }
-        __syncthreads();
-        if (t == 0) {
-          double xn2 = 0.0;
-          for (int r = 1; r < len; ++r) xn2 += norm2(vcur[r]);
-          zc beta, tau, scale;
-          larfg(vcur[0], xn2, beta, tau, scale);
-          sc[1] = tau;
-          sc[2] = beta;
-          sc[3] = scale;
-        }
-        __syncthreads();
-        const zc tau = sc[1], scale = sc[3];
-        if (t < B) {
-          zc x = mkz(0, 0);
-          if (t == 0) x = mkz(1, 0);
-          else if (t < len) x = vcur[t] * scale;
-          vcur[t] = x;
-        }
-        __syncthreads();
-        // column s: beta then zeros
-        if (t < len) bd(a, st + t, s) = t == 0 ? sc[2] : mkz(0, 0);
-        lenp = len;
-        // two-sided update of D with vcur
-        if (t < B) {
-          zc y = mkz(0, 0);
-          for (int c = 0; c < len; ++c) y = fma_z(D[t][c], vcur[c], y);
-          wv[t] = y;                                       // y = D v
-        }
-        __syncthreads();
-        if (t == 0) {
-          double g = 0.0;                                  // v^H y (real)
-          for (int r = 0; r < len; ++r) g += vcur[r].re * wv[r].re + vcur[r].im * wv[r].im;
-          sc[0] = mkz(g, 0);
-        }
-        __syncthreads();
-        if (t < B) {                                       // w = tau y - 1/2 |tau|^2 (v^H y) v
-          const double h = 0.5 * norm2(tau) * sc[0].re;
-          wv[t] = tau * wv[t] - h * vcur[t];
-        }
-        __syncthreads();
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          if (r < len && c <= r) {
-            const zc d = D[r][c] - vcur[r] * conj(wv[c]) - wv[r] * conj(vcur[c]);
-            bd(a, st + r, st + c) = (r == c) ? mkz(d.re, 0.0) : d;
-          }
-        }
-        // tracked vectors rows st..ed: gc -= conj(tau) v (v^H gc)
-        if (t < a.nrhs) {
-          zc* g = a.gc + (size_t)t * N + st;
-          zc al = mkz(0, 0);
-          for (int r = 0; r < len; ++r) al = fma_conj(vcur[r], g[r], al);
-          const zc f = conj(tau) * al;
-          for (int r = 0; r < len; ++r) g[r] -= vcur[r] * f;
-        }
-        if (t < B) vprev[t] = vcur[t];
-        if (t == 0) sc[0] = tau;
+        if (h == 0 && rin) bd(a, st + r, s) = (r == 0) ? beta : mkz(0, 0);
       } else {
-        // block O = A(st..ed, sp..ep), sp..ep the previous reflector's rows
-        const int sp = s + 1 + (k - 1) * b, ep = sp + lenp - 1;
-        const int st = ep + 1, ed = min(st + b - 1, N - 1), len = ed - st + 1;
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          O[r][c] = (r < len && c < lenp) ? bd(a, st + r, sp + c) : mkz(0, 0);
-          zc v = mkz(0, 0);
-          if (r < len && c < len) v = r >= c ? bd(a, st + r, st + c) : conj(bd(a, st + c, st + r));
-          D[r][c] = v;
+        // (iii) O(:, 1:) <- H^H O(:, 1:): z_c = v^H O(:, c), O -= conj(tau) v z
+        zc tq[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) tq[q] = fma_conj(vr, O[q], mkz(0, 0));
+        const zc z = col_sums<B, NC>(tq, lane);
+        const zc ctv = conj(tau) * vr;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          const int c = c0 + q;
+          const zc zq = shfl_z(z, gb + (q << (LB2 - LNC)));   // the lane holding column c0 + q
+          if (c == 0) O[q] = (r == 0) ? beta : mkz(0, 0);
+          else O[q] = O[q] - ctv * zq;
+          if (rin && c < lenp) bd(a, st + r, sp + c) = O[q];
         }
-        __syncthreads();
-        // (i) O <- O Hprev: w = O vprev, O -= tau_prev w vprev^H
-        const zc taup = sc[0];
-        if (t < B) {
-          zc w = mkz(0, 0);
-          for (int c = 0; c < lenp; ++c) w = fma_z(O[t][c], vprev[c], w);
-          wv[t] = w;
-        }
-        __syncthreads();
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          if (r < len && c < lenp) O[r][c] -= (taup * wv[r]) * conj(vprev[c]);
-        }
-        __syncthreads();
-        // (ii) reflector from O(:, 0)
-        if (t == 0) {
-          double xn2 = 0.0;
-          for (int r = 1; r < len; ++r) xn2 += norm2(O[r][0]);
-          zc beta, tau, scale;
-          larfg(O[0][0], xn2, beta, tau, scale);
-          sc[1] = tau;
-          sc[2] = beta;
-          sc[3] = scale;
-        }
-        __syncthreads();
-        const zc tau = sc[1], scale = sc[3];
-        if (t < B) {
-          zc x = mkz(0, 0);
-          if (t == 0) x = mkz(1, 0);
-          else if (t < len) x = O[t][0] * scale;
-          vcur[t] = x;
-        }
-        __syncthreads();
-        if (t < B) O[t][0] = (t == 0) ? sc[2] : mkz(0, 0);
-        // (iii) O(:, 1:) <- H^H O(:, 1:): w'_c = v^H O(:, c), O -= conj(tau) v w'
-        if (t < B) {
-          zc w = mkz(0, 0);
-          if (t >= 1 && t < lenp)
-            for (int r = 0; r < len; ++r) w = fma_conj(vcur[r], O[r][t], w);
-          wv[t] = w;
-        }
-        __syncthreads();
-        const zc ct = conj(tau);
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          if (r < len && c >= 1 && c < lenp) O[r][c] -= (ct * vcur[r]) * wv[c];
-        }
-        __syncthreads();
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          if (r < len && c < lenp) bd(a, st + r, sp + c) = O[r][c];
-        }
-        // (iv) D <- H^H D H
-        if (t < B) {
-          zc y = mkz(0, 0);
-          for (int c = 0; c < len; ++c) y = fma_z(D[t][c], vcur[c], y);
-          wv[t] = y;
-        }
-        __syncthreads();
-        if (t == 0) {
-          double g = 0.0;
-          for (int r = 0; r < len; ++r) g += vcur[r].re * wv[r].re + vcur[r].im * wv[r].im;
-          sc[0] = mkz(g, 0);
-        }
-        __syncthreads();
-        if (t < B) {
-          const double h = 0.5 * norm2(tau) * sc[0].re;
-          wv[t] = tau * wv[t] - h * vcur[t];
-        }
-        __syncthreads();
-        for (int e = t; e < B * B; e += 256) {
-          const int r = e % B, c = e / B;
-          if (r < len && c <= r) {
-            const zc d = D[r][c] - vcur[r] * conj(wv[c]) - wv[r] * conj(vcur[c]);
-            bd(a, st + r, st + c) = (r == c) ? mkz(d.re, 0.0) : d;
-          }
-        }
-        if (t < a.nrhs) {
-          zc* g = a.gc + (size_t)t * N + st;
-          zc al = mkz(0, 0);
-          for (int r = 0; r < len; ++r) al = fma_conj(vcur[r], g[r], al);
-          const zc f = conj(tau) * al;
-          for (int r = 0; r < len; ++r) g[r] -= vcur[r] * f;
-        }
-        lenp = len;
-        if (t < B) vprev[t] = vcur[t];
-        if (t == 0) sc[0] = tau;
       }
-      // publish: every thread's stores are done before the counter moves
+      // (iv) D <- H^H D H: y = D v, g = v^H y, w = tau y - |tau|^2 g / 2 v, D -= v w^H + w v^H
+      zc y = mkz(0, 0);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) y = fma_z(D[q], vc[q], y);
+      y = row_sum<B>(y, red2, t);
+      const double gam = group_sum<B>(vr.re * y.re + vr.im * y.im);
+      const zc wr = tau * y - (0.5 * norm2(tau) * gam) * vr;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const int c = c0 + q;
+        const zc wc = shfl_z(wr, gb + c);
+        const zc d = D[q] - vr * conj(wc) - wr * conj(vc[q]);
+        if (rin && c <= r) bd(a, st + r, st + c) = (r == c) ? mkz(d.re, 0.0) : d;
+      }
+      // (v) tracked vector h, rows st..ed: gc -= conj(tau) v (v^H gc)
+      const zc al = group_sum<B>(fma_conj(vr, g, mkz(0, 0)));
+      if (rin && gq) a.gc[(size_t)h * N + st + r] = g - vr * (conj(tau) * al);
+      lenp = len;
+      taup = tau;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) vp[q] = vc[q];
+      // publish: every wave's stores have completed before the counter moves
+      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
       if (t == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(&a.prog[s], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      __syncthreads();
     }
   }
 }
@@ -687,6 +776,12 @@ __global__ void k_band_finish(const zc* __restrict__ BD, int N, int LB, double* 
   d[j] = BD[(size_t)j * LB].re;
   e[j] = (j + 1 < N) ? BD[(size_t)j * LB + 1].re : 0.0;
   for (int r = 0; r < nrhs; ++r) g[(size_t)r * N + j] = conj(gc[(size_t)r * N + j]);
+}
+
+// control block of the chase: error flag and sweep counter 0
+__global__ void k_chase_init(int* ctl) {
+  const int i = threadIdx.x;
+  if (i < 64) ctl[i] = 0;
 }
 
 __global__ void k_zero_ints(int* p, int n) {
@@ -716,6 +811,7 @@ size_t band_work_bytes(int N, int b, int nrhs) {
   add(sizeof(zc) * BMAX * BMAX);              // M
   add(sizeof(zc) * nwg * BMAX * BMAX);        // K slices of the small products
   add(hmm_work_bytes(N, b));                  // hmm K slices
+  add(sizeof(zc) * GRAM_PMAX * BMAX * (BMAX + 16));   // partials of the tall-skinny products
   add(sizeof(zc) * (size_t)N * LB);           // band
   add(sizeof(zc) * (size_t)N * (nrhs > 0 ? nrhs : 1));   // gc
   add(sizeof(int) * ((size_t)N + 1));         // prog
@@ -755,11 +851,23 @@ int band_bind(BandWork& w, void* base, int N, int b, int nrhs) {
   w.gw = (zc*)take(sizeof(zc) * nwg * BMAX * BMAX);
   w.gw_elems = nwg * BMAX * BMAX;
   w.hw = (zc*)take(hmm_work_bytes(N, b));
+  w.gp = (zc*)take(sizeof(zc) * GRAM_PMAX * BMAX * (BMAX + 16));
   w.BD = (zc*)take(sizeof(zc) * (size_t)N * w.LB);
   w.gc = (zc*)take(sizeof(zc) * (size_t)N * (nrhs > 0 ? nrhs : 1));
   w.prog = (int*)take(sizeof(int) * ((size_t)N + 1));
   w.err = (int*)take(sizeof(int) * 64);
   return WTB_OK;
+}
+
+// out0 = A^H Y0 (B x B) and out1 = A^H Y1 (B x n1) over m rows, ld B
+template <int B>
+static void gram(hipStream_t s, BandWork& w, int m, const zc* A, const zc* Y0, int ldy0, const zc* Y1, int n1,
+                 zc* out0, zc* out1) {
+  const int nch = (m + GRAM_ROWS - 1) / GRAM_ROWS;
+  const int P = min(nch, GRAM_PMAX);
+  hipLaunchKernelGGL(k_gram<B>, dim3(P), dim3(256), 0, s, m, w.N, A, Y0, ldy0, Y1, n1, w.gp);
+  const int nout = B * (B + 16);
+  hipLaunchKernelGGL(k_gram_sum<B>, dim3((nout + 63) / 64), dim3(256), 0, s, P, w.gp, n1, out0, out1);
 }
 
 template <class T, int B>
@@ -788,12 +896,8 @@ static int stage1(hipStream_t s, BandWork& w, T* H, int ld, int nrhs) {
     const dim3 g(a.nwg), blk(PR);
     for (int c = 0; c <= k; ++c) hipLaunchKernelGGL((k_pqr_col<T, B>), g, blk, 0, s, a, c);
     hipLaunchKernelGGL((k_pfin<T, B>), g, blk, 0, s, a, w.V);
-    const zc one = mkz(1, 0), zero = mkz(0, 0);
     int rc;
-    if ((rc = zgemm(s, 'C', 'N', b, b, m, one, w.V, N, w.V, N, zero, w.G, b, w.gw, w.gw_elems))) return rc;
-    if (nrhs > 0 &&
-        (rc = zgemm(s, 'C', 'N', b, nrhs, m, one, w.V, N, w.gc + r0, N, zero, w.t1, b, w.gw, w.gw_elems)))
-      return rc;
+    gram<B>(s, w, m, w.V, w.V, N, nrhs > 0 ? w.gc + r0 : nullptr, nrhs, w.G, w.t1);
     hipLaunchKernelGGL((k_pT<B>), dim3(1), dim3(256), 0, s, k, w.G, w.t1, w.tau, nrhs, w.T, w.t2);
     hipLaunchKernelGGL((k_pY<B>), g, blk, 0, s, m, N, r0, w.V, w.T, w.t2, nrhs, w.Y, w.gc);
     if constexpr (sizeof(T) == sizeof(zc)) {
@@ -802,7 +906,7 @@ static int stage1(hipStream_t s, BandWork& w, T* H, int ld, int nrhs) {
       rc = hmm_f32(s, m, b, reinterpret_cast<const zf*>(H) + r0 + (size_t)ld * r0, ld, w.Y, N, w.X, N, w.hw);
     }
     if (rc) return rc;
-    if ((rc = zgemm(s, 'C', 'N', b, b, m, one, w.V, N, w.X, N, zero, w.M, b, w.gw, w.gw_elems))) return rc;
+    gram<B>(s, w, m, w.V, w.X, N, nullptr, 0, w.M, nullptr);
     hipLaunchKernelGGL((k_pW<B>), g, blk, 0, s, m, N, w.V, w.T, w.M, w.X);
     const int nt = (m + 63) / 64;
     const bool tev = w.kev && w.kev_used < w.kev_cap;
@@ -834,7 +938,7 @@ static int run_band_t(hipStream_t s, BandWork& w, T* H, int ld, zc* g, double* d
   if (rc) return rc;
   hipLaunchKernelGGL(k_band_extract<T>, dim3(N), dim3(64), 0, s, H, ld, N, b, w.LB, w.BD);
   hipLaunchKernelGGL(k_zero_ints, dim3((N + 64 + 255) / 256), dim3(256), 0, s, w.prog, N + 1);
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.err, 64);
+  hipLaunchKernelGGL(k_chase_init, dim3(1), dim3(64), 0, s, w.err);
   ChaseArgs ca;
   ca.BD = w.BD;
   ca.N = N;
@@ -843,11 +947,11 @@ static int run_band_t(hipStream_t s, BandWork& w, T* H, int ld, zc* g, double* d
   ca.gc = w.gc;
   ca.nrhs = nrhs;
   ca.prog = w.prog;
-  ca.err = w.err;
-  ca.P = min(N - 1, 256);
+  ca.ctl = w.err;
   ca.spin_limit = 1ll << 20;   // polls of about a microsecond each: about a second
-  if (b == 16) hipLaunchKernelGGL(k_chase<16>, dim3(ca.P), dim3(256), 0, s, ca);
-  else hipLaunchKernelGGL(k_chase<32>, dim3(ca.P), dim3(256), 0, s, ca);
+  const int P = min(N - 1, 128);   // workgroups claiming sweeps
+  if (b == 16) hipLaunchKernelGGL(k_chase<16>, dim3(P), dim3(256), 0, s, ca);
+  else hipLaunchKernelGGL(k_chase<32>, dim3(P), dim3(256), 0, s, ca);
   hipLaunchKernelGGL(k_band_finish, dim3((N + 255) / 256), dim3(256), 0, s, w.BD, N, w.LB, d, e, w.gc, g, nrhs,
                      w.err, status);
   WTB_LAUNCH_CHECK();

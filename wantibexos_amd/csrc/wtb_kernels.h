@@ -104,10 +104,11 @@ struct BandWork {
   zc* gw = nullptr;
   size_t gw_elems = 0;
   zc* hw = nullptr;
+  zc* gp = nullptr;             // partials of the panel's tall-skinny products
   zc* BD = nullptr;
   zc* gc = nullptr;
   int* prog = nullptr;
-  int* err = nullptr;
+  int* err = nullptr;           // chase control block: [0] error flag, [32] next sweep
   // optional per-launch timing of the trailing-update kernel (k_her2k_full): launch c is
   // bracketed by kev[2c], kev[2c+1] (c < kev_cap); kev_flops[c] its flop count
   hipEvent_t* kev = nullptr;

@@ -58,6 +58,7 @@ LARGE2D = dict(CONFIG, name="hex2d_nw24_60x60x1_nv2nc2", seed=5, nw=24, ngrid=(6
 BULK3D = dict(CONFIG, name="cubic3d_nw16_16x16x16_nv3nc3", lattice="cubic", seed=3, nw=16, ngrid=(16, 16, 16),
               nc=3, nv=3, coultype="V3DL", ediel=(6.0, 6.0, 6.0))
 CONFIGS = {"c2": CONFIG, "c1": SMALL, "c3": BULK3D, "c5": LARGE2D}
+REDUCTION = {"auto": None, "one": False, "two": True}
 HBM_PEAK = 8000.0       # GB/s
 FP64_PEAK = 78.6        # TFLOP/s, dense FP64 MFMA
 FP32_PEAK = 157.3       # TFLOP/s, dense FP32 MFMA
@@ -442,8 +443,10 @@ def main():
                     help="full form: skip the iterative form's measurement on the 60x60 model that the "
                          "line carries beside the metric")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--two-stage", action="store_true",
-                    help="the two-stage (dense -> band -> tridiagonal) reduction instead of the one-stage one")
+    ap.add_argument("--reduction", choices=["auto", "one", "two"], default="auto",
+                    help="tridiagonal reduction: one-stage Householder, two-stage (dense -> band -> "
+                         "tridiagonal), or by size (auto: two-stage from N = 8192 on, one-stage where "
+                         "exciton vectors are formed)")
     ap.add_argument("--band", type=int, default=32, choices=[16, 32], help="two-stage reduction: band width")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the kernels directly instead of replaying the captured HIP graph")
@@ -488,7 +491,7 @@ def main():
     if args.no_graph:
         plan.set_graph(False)
     N = plan.N
-    plan.set_reduction(args.two_stage, args.band)
+    plan.set_reduction(REDUCTION[args.reduction], args.band)
     nref = min(args.refine, N) if args.precision == "mixed" else 0
     if nref:
         plan.set_refine(nref)
@@ -552,7 +555,7 @@ def main():
         with bse.BSEPlan(ctx, model, params(cfg, "fp64")) as p64:
             if args.no_graph:
                 p64.set_graph(False)
-            p64.set_reduction(args.two_stage, args.band)
+            p64.set_reduction(REDUCTION[args.reduction], args.band)
             for _ in range(args.warmup):
                 p64.execute()
             ctx.sync()

@@ -404,10 +404,11 @@ int wtb_sp_table(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* 
 int wtb_bse_plan_set_vectors(wtb_bse_plan* plan, int first, int count);
 /* Form of the tridiagonal reduction of the BSE matrix (the first step of the ZHEEV of
  * bse_solver-tool-diel.f90:485-492; no counterpart in the reference's arithmetic):
- * two_stage = 0 (default): one-stage blocked Householder; two_stage = 1: dense -> band of
- * width `band` (16 or 32, MFMA panel updates) -> tridiagonal by bulge chasing, used
- * whenever no exciton vectors are formed.  Executes that form exciton vectors (BSE_WF,
- * the refinement) always take the one-stage form, whose reflectors they need.          */
+ * two_stage = 0: one-stage blocked Householder; two_stage = 1: dense -> band of width
+ * `band` (16 or 32, MFMA panel updates, lower triangle only) -> tridiagonal by bulge
+ * chasing; two_stage = -1 (default): the two-stage form for N >= 8192, else one-stage.
+ * Executes that form exciton vectors (BSE_WF, the refinement) always take the one-stage
+ * form, whose reflectors they need.                                                    */
 int wtb_bse_plan_set_reduction(wtb_bse_plan* plan, int two_stage, int band);
 
 /* Precision form (WTB_MIXED, Q = 0): FP64 Rayleigh-quotient refinement of the nref lowest

@@ -517,10 +517,12 @@ class BSEPlan:
         """Also form the eigenvectors of excitons first .. first+count-1 (fetch('excwf'))."""
         _check(lib().wtb_bse_plan_set_vectors(self.h, int(first), int(count)))
 
-    def set_reduction(self, two_stage=True, band=32):
-        """Tridiagonal reduction: two-stage (dense -> band -> tridiagonal, default) or
-        one-stage Householder; band width 16 or 32."""
-        _check(lib().wtb_bse_plan_set_reduction(self.h, int(bool(two_stage)), int(band)))
+    def set_reduction(self, two_stage=None, band=32):
+        """Tridiagonal reduction: two-stage (True: dense -> band -> tridiagonal), one-stage
+        Householder (False), or by size (None, the default: two-stage from N = 8192 on);
+        band width 16 or 32."""
+        mode = -1 if two_stage is None else int(bool(two_stage))
+        _check(lib().wtb_bse_plan_set_reduction(self.h, mode, int(band)))
 
     def set_refine(self, nref=256):
         """Precision form: FP64 Rayleigh-quotient refinement of the nref lowest excitons

@@ -1,4 +1,4 @@
-// Stage (d1) in two stages: a Hermitian matrix (full storage, column-major, ld) is
+// Stage (d1) in two stages: a Hermitian matrix (lower triangle, column-major, ld) is
 // reduced first to a lower band of width b (dense -> band, blocked Householder on panels
 // of b columns with MFMA two-sided updates), then to a real symmetric tridiagonal (band ->
 // tridiagonal by bulge chasing), applying the conjugate transpose of every orthogonal
@@ -16,10 +16,10 @@
 //   k_gram, k_gram_sum         G = V^H V, t1 = V^H g (FP64 MFMA on row chunks, partials summed in order)
 //   k_pT                       T (ZLARFT forward/columnwise) and t2 = T^H t1
 //   k_pY                       Y = V T, g -= V t2 (rows of the panel)
-//   hmm                        X = A22 Y (MFMA, full Hermitian storage)
+//   k_hmm_b, k_hmm_sum         X = A22 Y (MFMA, full Hermitian storage, K slices)
 //   k_gram, k_gram_sum         M = V^H X
 //   k_pW                       W = X - 1/2 V (T^H M)
-//   k_her2k_full               A22 -= V W^H + W V^H (MFMA, every tile of the trailing matrix)
+//   k_her2k_full               A22 -= V W^H + W V^H (MFMA, the tiles on and below the diagonal)
 // The tracked vectors are carried conjugated (gc = conj(g)): Q^H gc = conj(Q^T g).
 //
 // Stage 2 (bulge chasing, FP64, band storage BD[c][d] = A(c + d, c), d <= 2b): sweep s
@@ -29,7 +29,7 @@
 // annihilates).  One persistent kernel whose workgroups claim sweeps in order from a
 // counter; task k of sweep s waits until sweep s - 1 has finished its tasks 0..k+1 (its
 // later tasks touch only rows below this task's), signalled through a per-sweep progress
-// counter with agent-scope release/acquire.  Every wait is bounded: past the limit the
+// counter (device-scope accesses, no cache maintenance).  Every wait is bounded: past the limit the
 // kernel sets an error flag that every waiting workgroup sees, and all of them return.
 #include "wtb_kernels.h"
 
@@ -54,6 +54,41 @@ __device__ inline zc zdiv(zc a, zc b) {
 __device__ inline zc shfl_z(zc v, int src) { return mkz(__shfl(v.re, src), __shfl(v.im, src)); }
 __device__ inline zc shfl_xor_z(zc v, int off) { return mkz(__shfl_xor(v.re, off), __shfl_xor(v.im, off)); }
 
+// x of another lane by a DPP lane permutation (within rows of 16 lanes) or a swizzle
+// (within 32 lanes); both halves of the double move
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  const long long v = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double swz16_d(double x) {   // lane ^ 16 within 32 lanes
+  const long long v = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(v >> 32), 0x401F);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline zc swz16_z(zc v) { return mkz(swz16_d(v.re), swz16_d(v.im)); }
+template <int CTRL>
+__device__ inline zc dpp_z(zc v) { return mkz(dpp_d<CTRL>(v.re), dpp_d<CTRL>(v.im)); }
+
+// The value of the partner lane that differs from this one in lane bit OFF and agrees in
+// the bits above it: ^1 and ^2 by quad permutations, i <-> 7 - i (row half mirror) for 4,
+// a rotation by 8 in a row of 16 for 8, a swizzle for 16, a permute for 32.  Partners are
+// mutual, so sums formed with them are the same in both lanes.
+template <int OFF>
+__device__ inline double xchg(double x) {
+  if constexpr (OFF == 1) return dpp_d<0xB1>(x);
+  else if constexpr (OFF == 2) return dpp_d<0x4E>(x);
+  else if constexpr (OFF == 4) return dpp_d<0x141>(x);
+  else if constexpr (OFF == 8) return dpp_d<0x128>(x);
+  else if constexpr (OFF == 16) return swz16_d(x);
+  else return __shfl_xor(x, 32);
+}
+template <int OFF>
+__device__ inline zc xchg(zc v) { return mkz(xchg<OFF>(v.re), xchg<OFF>(v.im)); }
+
 // ZLARFG on (alpha, x) with |x|^2 = xn2: beta (real, returned as the new alpha), tau and
 // the scale 1/(alpha - beta) of x.  tau = 0 (H = I) when x = 0 and alpha is real.
 __device__ inline void larfg(zc alpha, double xn2, zc& beta, zc& tau, zc& scale) {
@@ -70,43 +105,18 @@ __device__ inline void larfg(zc alpha, double xn2, zc& beta, zc& tau, zc& scale)
   beta = mkz(b, 0.0);
 }
 
-// sum of a zc[NV] over the 256 threads of a workgroup into out[NV] (thread 0..NV-1 hold
-// the results in red[]); fixed order: waves by butterfly, then waves in index order
-template <int NV>
-__device__ inline void block_sum(zc (&v)[NV], zc* red /* [4][NV] in LDS */) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    zc x = v[q];
-    for (int off = 32; off > 0; off >>= 1) {
-      x.re += __shfl_xor(x.re, off);
-      x.im += __shfl_xor(x.im, off);
-    }
-    if (lane == 0) red[wave * NV + q] = x;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < NV) {
-    const int q = threadIdx.x;
-    red[4 * NV + q] = ((red[q] + red[NV + q]) + red[2 * NV + q]) + red[3 * NV + q];
-  }
-  __syncthreads();
-}
-
 // Sums over the 256 threads of a workgroup of one real value x0 and NB complex values
 // v[0..NB) (NB a power of two <= 32).  Within a wave the NB sums are formed by a butterfly
 // that halves the set each lane keeps (NB - 1 exchanges instead of 6 NB), after which lane
 // l holds the wave's sum of value (l >> (6 - log2 NB)) & (NB - 1); the four waves are then
 // added in index order.  Results: red[4 * (NB + 1) + q] (q = 0: x0, 1 + j: v[j]).
-// one halving step of the butterfly: lanes with the `off` bit set keep values [n, 2n)
-template <int n, int NB>
-__device__ inline void bf_step(zc (&v)[NB], int lane, int off) {
-  const bool up = (lane & off) != 0;
+// one halving step of the butterfly: lanes with the OFF bit set keep values [n, 2n)
+template <int OFF, int n, int NB>
+__device__ inline void bf_step(zc (&v)[NB], int lane) {
+  const bool up = (lane & OFF) != 0;
 #pragma unroll
   for (int q = 0; q < n; ++q) {
-    const zc send = up ? v[q] : v[q + n];
-    zc recv;
-    recv.re = __shfl_xor(send.re, off);
-    recv.im = __shfl_xor(send.im, off);
+    const zc recv = xchg<OFF>(up ? v[q] : v[q + n]);
     v[q] = (up ? v[q + n] : v[q]) + recv;
   }
 }
@@ -115,26 +125,29 @@ template <int NB>
 __device__ inline void block_sum_bf(double x0, zc (&v)[NB], zc* red /* [5][NB + 1] in LDS */) {
   static_assert(NB == 16 || NB == 32, "butterfly for 16 or 32 values");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  zc r;
   if constexpr (NB == 32) {
-    bf_step<16>(v, lane, 32);
-    bf_step<8>(v, lane, 16);
-    bf_step<4>(v, lane, 8);
-    bf_step<2>(v, lane, 4);
-    bf_step<1>(v, lane, 2);
+    bf_step<32, 16>(v, lane);
+    bf_step<16, 8>(v, lane);
+    bf_step<8, 4>(v, lane);
+    bf_step<4, 2>(v, lane);
+    bf_step<2, 1>(v, lane);
+    r = v[0] + xchg<1>(v[0]);
   } else {
-    bf_step<8>(v, lane, 32);
-    bf_step<4>(v, lane, 16);
-    bf_step<2>(v, lane, 8);
-    bf_step<1>(v, lane, 4);
+    bf_step<32, 8>(v, lane);
+    bf_step<16, 4>(v, lane);
+    bf_step<8, 2>(v, lane);
+    bf_step<4, 1>(v, lane);
+    r = v[0] + xchg<2>(v[0]);
+    r += xchg<1>(r);
   }
-  zc r = v[0];
-#pragma unroll
-  for (int off = 32 / NB; off > 0; off >>= 1) {
-    r.re += __shfl_xor(r.re, off);
-    r.im += __shfl_xor(r.im, off);
-  }
-  for (int off = 32; off > 0; off >>= 1) x0 += __shfl_xor(x0, off);
-  constexpr int SH = (NB == 32) ? 1 : (NB == 16) ? 2 : (NB == 8) ? 3 : (NB == 4) ? 4 : (NB == 2) ? 5 : 6;
+  x0 += xchg<1>(x0);
+  x0 += xchg<2>(x0);
+  x0 += xchg<4>(x0);
+  x0 += xchg<8>(x0);
+  x0 += xchg<16>(x0);
+  x0 += xchg<32>(x0);
+  constexpr int SH = (NB == 32) ? 1 : 2;
   if ((lane & ((1 << SH) - 1)) == 0) red[wave * (NB + 1) + 1 + ((lane >> SH) & (NB - 1))] = r;
   if (lane == 0) red[wave * (NB + 1)] = mkz(x0, 0.0);
   __syncthreads();
@@ -164,6 +177,7 @@ template <class T, int B>
 __global__ __launch_bounds__(PR) void k_pqr_col(PanelArgs<T> a, int c) {
   __shared__ zc red[5 * (B + 1)];
   __shared__ zc sh[B + 4];
+  __shared__ zc shs[256 / (B + 1)][B + 1];
   const int i = blockIdx.x * PR + threadIdx.x;       // panel row
   const bool live = i < a.m;
   const size_t N = a.N;
@@ -185,11 +199,19 @@ __global__ __launch_bounds__(PR) void k_pqr_col(PanelArgs<T> a, int c) {
     // finish reflector c - 1 from the partial sums of the previous launch
     const int cp = c - 1;
     const zc* pp = a.part + (size_t)(cp & 1) * a.nwg * (BMAX + 1);
+    // sh[0].re = |x|^2 below the diagonal, sh[1 + j] = x^H P(:, j): the workgroups' partial
+    // sums, NS threads per value each adding every NS-th partial (eight loads in flight),
+    // then the NS sub-sums in order
+    constexpr int NS = 256 / (B + 1);
+    if ((int)threadIdx.x < NS * (B + 1)) {
+      const int q = threadIdx.x % (B + 1), sub = threadIdx.x / (B + 1);
+      shs[sub][q] = strided_sum(pp + q, BMAX + 1, sub, a.nwg, NS, mkz(0, 0));
+    }
+    __syncthreads();
     if ((int)threadIdx.x < B + 1) {
-      // sh[0].re = |x|^2 below the diagonal, sh[1 + j] = x^H P(:, j): the workgroups' partial
-      // sums in workgroup order, eight loads in flight
-      const int q = threadIdx.x;
-      sh[q] = strided_sum(pp + q, BMAX + 1, 0, a.nwg, 1, mkz(0, 0));
+      zc t = shs[0][threadIdx.x];
+      for (int u = 1; u < NS; ++u) t += shs[u][threadIdx.x];
+      sh[threadIdx.x] = t;
     }
     __syncthreads();
     const zc* rc = a.rowc + (cp & 1) * BMAX;
@@ -471,42 +493,206 @@ __global__ __launch_bounds__(256) void k_gram_sum(int P, const zc* __restrict__ 
   else out1[i + (size_t)B * (j - B)] = sum;
 }
 
-// A22 -= U, U = V W^H + W V^H, on every 64 x 64 tile of the trailing matrix (rows and
-// columns r0..N-1), K = 2 b: FP64 MFMA on the COMPLEX*16 matrix, FP32 MFMA (FP32 operands
+// X = A Y for the Hermitian trailing matrix A (m x m, lower triangle, ld) and the panel's
+// Y (m x BC, ld ldy) on MFMA: FP32 operands and accumulation for the COMPLEX*8 matrix,
+// FP64 for the COMPLEX*16 one.  A workgroup owns 128 rows and a K slice (blockIdx.y) and
+// walks it in steps of 16 staged through LDS, the next step's loads in flight while the
+// current one is multiplied; wave w computes rows 32w..32w+31 for all BC columns.  A step
+// below the diagonal reads A(i, k) down the columns; a step above it reads conj(A(k, i))
+// down the columns of the transposed block; a step across it reads element by element.
+// With more than one slice the partials go to part[z][col][row] and k_hmm_sum adds them in
+// slice order.
+#define HB_R 128
+#define HB_SMAX 16      // most K slices
+template <class T, int BC>
+__global__ __launch_bounds__(256) void k_hmm_b(int m, const T* __restrict__ A, int ld, const zc* __restrict__ Y,
+                                               int ldy, int kc, zc* __restrict__ part, zc* __restrict__ X, int ldx) {
+  constexpr bool F64 = sizeof(T) == sizeof(zc);
+  typedef typename std::conditional<F64, double2, float2>::type ct;
+  typedef typename std::conditional<F64, d4, f4>::type acc_t;
+  constexpr int KS = F64 ? 16 : 32;                 // K per step (32 KB of the matrix in LDS)
+  constexpr int NA = KS * HB_R / 256;               // matrix elements per thread and step
+  constexpr int CT = BC / 16;                       // 16-column tiles
+  constexpr int YP = KS * BC / 256;                 // Y elements per thread and step
+  __shared__ ct As[KS][HB_R];
+  __shared__ ct Ys[KS][BC];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int R0 = blockIdx.x * HB_R;
+  const int kbeg = blockIdx.y * kc, kend = min(m, kbeg + kc);
+  // loaders: A(row R0 + (t & 127), k + (t >> 7) + 2u) below or across the diagonal,
+  // conj(A(k + (t & 15) + 16 (u / 8), row R0 + (t >> 4) + 16 (u % 8))) above it (rows past m
+  // read row m - 1, never stored); Y(k + (t % KS), t / KS + (256 / KS) u)
+  const int arow = min(R0 + (t & 127), m - 1), ak = t >> 7;
+  const int tk = t & 15, ti = t >> 4;
+  const int yk = t % KS, ycol = t / KS;
+  int lmode = 0;                                    // 0 below, 1 above, 2 across the diagonal
+  acc_t cr[2][CT], ci[2][CT];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < CT; ++b) cr[a][b] = ci[a][b] = acc_t{0, 0, 0, 0};
+  T ra[NA];
+  zc ry[YP];
+  auto gload = [&](int k0) {
+    lmode = (k0 + KS - 1 <= R0) ? 0 : (k0 > R0 + HB_R - 1) ? 1 : 2;
+    if (lmode == 1) {
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        const int k = min(k0 + tk + 16 * (u / 8), m - 1);
+        const int i = min(R0 + ti + 16 * (u % 8), m - 1);
+        ra[u] = A[(size_t)k + (size_t)ld * i];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        const int k = min(k0 + ak + 2 * u, m - 1);  // past kend: Y is zero there
+        ra[u] = (lmode == 0 || arow >= k) ? A[(size_t)arow + (size_t)ld * k] : A[(size_t)k + (size_t)ld * arow];
+        if (lmode == 2 && arow < k) ra[u].im = -ra[u].im;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < YP; ++u) {
+      const int k = k0 + yk;
+      ry[u] = (k < kend) ? Y[(size_t)k + (size_t)ldy * (ycol + (256 / KS) * u)] : mkz(0, 0);
+    }
+  };
+  gload(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += KS) {
+    __syncthreads();                                  // the previous step is consumed
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      ct v;
+      v.x = ra[u].re;
+      v.y = ra[u].im;
+      if (lmode == 1) {
+        v.y = -v.y;                                 // A(i, k) = conj(A(k, i))
+        As[tk + 16 * (u / 8)][ti + 16 * (u % 8)] = v;
+      } else {
+        As[ak + 2 * u][t & 127] = v;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < YP; ++u) {
+      ct v;
+      v.x = ry[u].re;
+      v.y = ry[u].im;
+      Ys[yk][ycol + (256 / KS) * u] = v;
+    }
+    __syncthreads();
+    if (k0 + KS < kend) gload(k0 + KS);
+#pragma unroll
+    for (int kk = 0; kk < KS / 4; ++kk) {
+      const int k = 4 * kk + kq;
+      ct av[2], yv[CT];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = As[k][32 * w + 16 * a + i16];
+#pragma unroll
+      for (int b = 0; b < CT; ++b) yv[b] = Ys[k][16 * b + i16];
+#pragma unroll
+      for (int b = 0; b < CT; ++b)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          // (ar + i ai)(yr + i yi) = (ar yr - ai yi) + i (ar yi + ai yr)
+          if constexpr (F64) {
+            cr[a][b] = mfma16x16x4(av[a].x, yv[b].x, cr[a][b]);
+            cr[a][b] = mfma16x16x4(-av[a].y, yv[b].y, cr[a][b]);
+            ci[a][b] = mfma16x16x4(av[a].x, yv[b].y, ci[a][b]);
+            ci[a][b] = mfma16x16x4(av[a].y, yv[b].x, ci[a][b]);
+          } else {
+            cr[a][b] = mfma16x16x4f32(av[a].x, yv[b].x, cr[a][b]);
+            cr[a][b] = mfma16x16x4f32(-av[a].y, yv[b].y, cr[a][b]);
+            ci[a][b] = mfma16x16x4f32(av[a].x, yv[b].y, ci[a][b]);
+            ci[a][b] = mfma16x16x4f32(av[a].y, yv[b].x, ci[a][b]);
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < CT; ++b) {
+      const int col = 16 * b + i16;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        // f64 MFMA: register q holds row kq + 4q of the tile; f32 MFMA: row 4kq + q
+        const int row = R0 + 32 * w + 16 * a + (F64 ? kq + 4 * q : 4 * kq + q);
+        if (row >= m) continue;
+        const zc v = mkz((double)cr[a][b][q], (double)ci[a][b][q]);
+        if (part) part[(size_t)blockIdx.y * m * BC + (size_t)col * m + row] = v;
+        else X[(size_t)row + (size_t)ldx * col] = v;
+      }
+    }
+}
+
+// X(row, col) = sum over the ns slices of part[z][col][row], in slice order
+template <int BC>
+__global__ __launch_bounds__(256) void k_hmm_sum(int m, int ns, const zc* __restrict__ part, zc* __restrict__ X,
+                                                 int ldx) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)m * BC) return;
+  const int row = (int)(e % m), col = (int)(e / m);
+  X[(size_t)row + (size_t)ldx * col] = strided_sum(part + e, (size_t)m * BC, 0, ns, 1, mkz(0, 0));
+}
+
+// A22 -= U, U = V W^H + W V^H, on the 64 x 64 tiles of the trailing matrix (rows and
+// columns r0..N-1) on and below the diagonal, K = 2 b: FP64 MFMA on the COMPLEX*16 matrix, FP32 MFMA (FP32 operands
 // and accumulation) on the COMPLEX*8 one.  The MFMA rows run over the tile's columns and
 // its columns over the tile's rows, D(i, j) = U(c_i, r_j) = conj(U(r_j, c_i)) since U is
 // Hermitian, so that a lane's result lands next to its neighbours' in the column-major
 // matrix.
-template <class T>
-__global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, int m, int r0, int N, int b,
+template <class T, int B>
+__global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, int m, int r0, int N,
                                                     const zc* __restrict__ V, const zc* __restrict__ W) {
+  if (blockIdx.y > blockIdx.x) return;                  // tiles above the diagonal: never read
+  constexpr bool F64 = sizeof(T) == sizeof(zc);
+  constexpr int K = 2 * B;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ci0 = blockIdx.y * 64 + (wave >> 1) * 32;   // tile columns (MFMA rows)
   const int rj0 = blockIdx.x * 64 + (wave & 1) * 32;    // tile rows (MFMA columns)
   const int lr = lane & 15, lk = lane >> 4;
-  const int K = 2 * b;
-  typedef typename std::conditional<sizeof(T) == sizeof(zc), d4, f4>::type acc_t;
-  acc_t cr[2][2], ci[2][2];
+  typedef typename std::conditional<F64, d4, f4>::type acc_t;
+  // the tile's current values, loaded first so that their latency overlaps the products
+  T hv[2][2][4];
+#pragma unroll
   for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        // f64 MFMA: register q holds D row lk + 4q; f32 MFMA: row 4 lk + q
+        const int c = ci0 + x * 16 + (F64 ? lk + 4 * q : 4 * lk + q);
+        const int r = rj0 + y * 16 + lr;
+        if (r < m && c < m) hv[x][y][q] = A[(size_t)(r0 + r) + (size_t)ld * (r0 + c)];
+      }
+  acc_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
     for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = acc_t{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += 4) {
+  // A operand (i = column c, k): [V | W](c, k); B operand (k, j = row r): conj([W | V](r, k));
+  // the operands of step k0 + 4 are loaded while step k0 is multiplied
+  zc av[2], bv[2];
+  auto oload = [&](int k0, zc (&pa)[2], zc (&pb)[2]) {
     const int k = k0 + lk;
-    zc av[2], bv[2];
+#pragma unroll
     for (int x = 0; x < 2; ++x) {
       const int c = ci0 + x * 16 + lr, r = rj0 + x * 16 + lr;
-      zc va = mkz(0, 0), vb = mkz(0, 0);
-      if (k < K) {
-        // A operand (i = column c, k): [V | W](c, k); B operand (k, j = row r): conj([W | V](r, k))
-        if (c < m) va = (k < b) ? V[(size_t)k * N + c] : W[(size_t)(k - b) * N + c];
-        if (r < m) vb = (k < b) ? W[(size_t)k * N + r] : V[(size_t)(k - b) * N + r];
-      }
-      av[x] = va;
-      bv[x] = vb;
+      pa[x] = (c < m) ? ((k < B) ? V[(size_t)k * N + c] : W[(size_t)(k - B) * N + c]) : mkz(0, 0);
+      pb[x] = (r < m) ? ((k < B) ? W[(size_t)k * N + r] : V[(size_t)(k - B) * N + r]) : mkz(0, 0);
     }
+  };
+  oload(0, av, bv);
+#pragma unroll
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    zc an[2], bn[2];
+    if (k0 + 4 < K) oload(k0 + 4, an, bn);
+#pragma unroll
     for (int x = 0; x < 2; ++x)
+#pragma unroll
       for (int y = 0; y < 2; ++y) {
         // a * conj(b)
-        if constexpr (sizeof(T) == sizeof(zc)) {
+        if constexpr (F64) {
           cr[x][y] = mfma16x16x4(av[x].re, bv[y].re, cr[x][y]);
           cr[x][y] = mfma16x16x4(av[x].im, bv[y].im, cr[x][y]);
           ci[x][y] = mfma16x16x4(av[x].im, bv[y].re, ci[x][y]);
@@ -519,20 +705,27 @@ __global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, i
           ci[x][y] = mfma16x16x4f32(-ar, bi, ci[x][y]);
         }
       }
+    if (k0 + 4 < K) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        av[x] = an[x];
+        bv[x] = bn[x];
+      }
+    }
   }
-  constexpr bool F64 = sizeof(T) == sizeof(zc);
+#pragma unroll
   for (int x = 0; x < 2; ++x)
+#pragma unroll
     for (int y = 0; y < 2; ++y)
+#pragma unroll
       for (int q = 0; q < 4; ++q) {
-        // f64 MFMA: register q holds D row lk + 4q; f32 MFMA: row 4 lk + q
         const int c = ci0 + x * 16 + (F64 ? lk + 4 * q : 4 * lk + q);
         const int r = rj0 + y * 16 + lr;
         if (r < m && c < m) {
-          T& h = A[(size_t)(r0 + r) + (size_t)ld * (r0 + c)];
-          zc v = ldz(h);
+          zc v = ldz(hv[x][y][q]);
           v.re -= cr[x][y][q];
           v.im += ci[x][y][q];       // - conj(D)
-          stz(h, v);
+          stz(A[(size_t)(r0 + r) + (size_t)ld * (r0 + c)], v);
         }
       }
 }
@@ -573,33 +766,54 @@ __device__ inline int chase_ntasks(int N, int b, int s) { return 1 + (N - 2 - s)
 __device__ inline zc& bd(const ChaseArgs& a, int r, int c) { return a.BD[(size_t)c * a.LB + (r - c)]; }
 
 
-// sum over the B lanes of a row group (lane bits below log2 B), every lane gets it
+// Band and tracked-vector elements move between the sweeps' workgroups through
+// device-scope (sc1) loads and stores, which the hardware keeps coherent across the XCDs'
+// L2 caches; a task's stores are complete (s_waitcnt) before its progress counter moves,
+// so the hand-off needs no L2 write-back or invalidate.
+__device__ inline zc ld_dev(const zc* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return mkz(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+             __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline void st_dev(zc* p, zc v) {
+  double* q = reinterpret_cast<double*>(p);
+  __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sum over the B (16 or 32) lanes of a row group: lane pairs ^1, ^2 (quad permutations),
+// then i <-> 7 - i, i <-> 15 - i within rows (half mirror, mirror), then ^16.  Every step
+// adds two equal partial sums in either order, so every lane of the group ends with the
+// same bits.
 template <int B>
-__device__ inline zc group_sum(zc x) {
-#pragma unroll
-  for (int off = 1; off < B; off <<= 1) x += shfl_xor_z(x, off);
+__device__ inline double group_sum(double x) {
+  x += dpp_d<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += dpp_d<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += dpp_d<0x141>(x);   // row_half_mirror
+  x += dpp_d<0x140>(x);   // row_mirror
+  if constexpr (B == 32) x += swz16_d(x);
   return x;
 }
 template <int B>
-__device__ inline double group_sum(double x) {
-#pragma unroll
-  for (int off = 1; off < B; off <<= 1) x += __shfl_xor(x, off);
-  return x;
+__device__ inline zc group_sum(zc x) {
+  return mkz(group_sum<B>(x.re), group_sum<B>(x.im));
 }
 
 // Column sums of a B x B block held as thread (r, h) = (t % B, t / B) owning row r and
 // columns h NC .. h NC + NC - 1 (NC = B^2 / 256); the B rows of a column group lie in one
-// wave.  Halving butterfly over the top row bits (NC - 1 exchanges), then plain exchanges
-// over the rest: thread (r, h) ends with the sum of local column r >> (log2 B - log2 NC).
+// wave.  B = 32: a halving butterfly on row bits 4 (swizzle) and 3 (DPP rotate by 8 in a
+// row of 16), each lane keeping the half its bit selects, then a DPP sum over the 8-lane
+// block: thread (r, h) ends with the sum of local column r >> 3.  B = 16 (one column per
+// thread): the group sum.
 template <int B, int NC>
 __device__ inline zc col_sums(zc (&t)[NC], int lane) {
   if constexpr (NC == 4) {
     static_assert(B == 32, "four columns per thread for B = 32");
-    bf_step<2>(t, lane, 16);
-    bf_step<1>(t, lane, 8);
-    zc x = t[0];
-#pragma unroll
-    for (int off = 4; off > 0; off >>= 1) x += shfl_xor_z(x, off);
+    bf_step<16, 2>(t, lane);
+    bf_step<8, 1>(t, lane);
+    zc x = t[0] + xchg<4>(t[0]);
+    x += xchg<2>(x);
+    x += xchg<1>(x);
     return x;
   } else {
     static_assert(B == 16 && NC == 1, "one column per thread for B = 16");
@@ -672,7 +886,6 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
         }
         __syncthreads();
         if (flag) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the blocks sweep s - 1 wrote
       }
       // rows st..ed of this task; for k >= 1 the previous reflector's rows sp..sp+lenp-1
       const int sp = s + 1 + (k - 1) * b;
@@ -686,16 +899,16 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
       for (int q = 0; q < NC; ++q) {
         const int c = c0 + q;
         zc v = mkz(0, 0);
-        if (rin && c < len) v = (r >= c) ? bd(a, st + r, st + c) : conj(bd(a, st + c, st + r));
+        if (rin && c < len) v = (r >= c) ? ld_dev(&bd(a, st + r, st + c)) : conj(ld_dev(&bd(a, st + c, st + r)));
         D[q] = v;
         zc o = mkz(0, 0);
-        if (k > 0 && rin && c < lenp) o = bd(a, st + r, sp + c);
+        if (k > 0 && rin && c < lenp) o = ld_dev(&bd(a, st + r, sp + c));
         O[q] = o;
       }
       zc x0 = mkz(0, 0);
-      if (rin) x0 = (k == 0) ? bd(a, st + r, s) : bd(a, st + r, sp);
+      if (rin) x0 = (k == 0) ? ld_dev(&bd(a, st + r, s)) : ld_dev(&bd(a, st + r, sp));
       const bool gq = h < a.nrhs;
-      zc g = (rin && gq) ? a.gc[(size_t)h * N + st + r] : mkz(0, 0);
+      zc g = (rin && gq) ? ld_dev(&a.gc[(size_t)h * N + st + r]) : mkz(0, 0);
       if (k > 0) {
         // (i) the block times the previous reflector from the right: O -= taup (O vp) vp^H
         zc w = mkz(0, 0);
@@ -717,7 +930,7 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
 #pragma unroll
       for (int q = 0; q < NC; ++q) vc[q] = shfl_z(vr, gb + c0 + q);
       if (k == 0) {
-        if (h == 0 && rin) bd(a, st + r, s) = (r == 0) ? beta : mkz(0, 0);
+        if (h == 0 && rin) st_dev(&bd(a, st + r, s), (r == 0) ? beta : mkz(0, 0));
       } else {
         // (iii) O(:, 1:) <- H^H O(:, 1:): z_c = v^H O(:, c), O -= conj(tau) v z
         zc tq[NC];
@@ -731,7 +944,7 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
           const zc zq = shfl_z(z, gb + (q << (LB2 - LNC)));   // the lane holding column c0 + q
           if (c == 0) O[q] = (r == 0) ? beta : mkz(0, 0);
           else O[q] = O[q] - ctv * zq;
-          if (rin && c < lenp) bd(a, st + r, sp + c) = O[q];
+          if (rin && c < lenp) st_dev(&bd(a, st + r, sp + c), O[q]);
         }
       }
       // (iv) D <- H^H D H: y = D v, g = v^H y, w = tau y - |tau|^2 g / 2 v, D -= v w^H + w v^H
@@ -746,11 +959,11 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
         const int c = c0 + q;
         const zc wc = shfl_z(wr, gb + c);
         const zc d = D[q] - vr * conj(wc) - wr * conj(vc[q]);
-        if (rin && c <= r) bd(a, st + r, st + c) = (r == c) ? mkz(d.re, 0.0) : d;
+        if (rin && c <= r) st_dev(&bd(a, st + r, st + c), (r == c) ? mkz(d.re, 0.0) : d);
       }
       // (v) tracked vector h, rows st..ed: gc -= conj(tau) v (v^H gc)
       const zc al = group_sum<B>(fma_conj(vr, g, mkz(0, 0)));
-      if (rin && gq) a.gc[(size_t)h * N + st + r] = g - vr * (conj(tau) * al);
+      if (rin && gq) st_dev(&a.gc[(size_t)h * N + st + r], g - vr * (conj(tau) * al));
       lenp = len;
       taup = tau;
 #pragma unroll
@@ -758,10 +971,7 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
       // publish: every wave's stores have completed before the counter moves
       __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
-      if (t == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(&a.prog[s], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (t == 0) __hip_atomic_store(&a.prog[s], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -810,8 +1020,8 @@ size_t band_work_bytes(int N, int b, int nrhs) {
   add(sizeof(zc) * 6 * BMAX);                 // t2
   add(sizeof(zc) * BMAX * BMAX);              // M
   add(sizeof(zc) * nwg * BMAX * BMAX);        // K slices of the small products
-  add(hmm_work_bytes(N, b));                  // hmm K slices
   add(sizeof(zc) * GRAM_PMAX * BMAX * (BMAX + 16));   // partials of the tall-skinny products
+  add(sizeof(zc) * HB_SMAX * (size_t)N * b);           // K-slice partials of X = A22 Y
   add(sizeof(zc) * (size_t)N * LB);           // band
   add(sizeof(zc) * (size_t)N * (nrhs > 0 ? nrhs : 1));   // gc
   add(sizeof(int) * ((size_t)N + 1));         // prog
@@ -850,13 +1060,29 @@ int band_bind(BandWork& w, void* base, int N, int b, int nrhs) {
   w.M = (zc*)take(sizeof(zc) * BMAX * BMAX);
   w.gw = (zc*)take(sizeof(zc) * nwg * BMAX * BMAX);
   w.gw_elems = nwg * BMAX * BMAX;
-  w.hw = (zc*)take(hmm_work_bytes(N, b));
   w.gp = (zc*)take(sizeof(zc) * GRAM_PMAX * BMAX * (BMAX + 16));
+  w.hb = (zc*)take(sizeof(zc) * HB_SMAX * (size_t)N * b);
   w.BD = (zc*)take(sizeof(zc) * (size_t)N * w.LB);
   w.gc = (zc*)take(sizeof(zc) * (size_t)N * (nrhs > 0 ? nrhs : 1));
   w.prog = (int*)take(sizeof(int) * ((size_t)N + 1));
   w.err = (int*)take(sizeof(int) * 64);
   return WTB_OK;
+}
+
+// X = A22 Y (m x B) with enough workgroups to fill the chip: K slices until the grid
+// holds ~1024 workgroups (at most HB_SMAX slices), partials summed in slice order
+template <class T, int B>
+static void hmm_band(hipStream_t s, BandWork& w, int m, const T* A, int ld, const zc* Y, zc* X) {
+  const int rb = (m + HB_R - 1) / HB_R;
+  int ns = std::max(1, std::min(HB_SMAX, (1024 + rb - 1) / rb));
+  const int kc = ((m + ns - 1) / ns + 31) / 32 * 32;   // a multiple of either K step
+  ns = (m + kc - 1) / kc;
+  hipLaunchKernelGGL((k_hmm_b<T, B>), dim3(rb, ns), dim3(256), 0, s, m, A, ld, Y, w.N, kc, ns > 1 ? w.hb : nullptr,
+                     X, w.N);
+  if (ns > 1) {
+    const size_t tot = (size_t)m * B;
+    hipLaunchKernelGGL(k_hmm_sum<B>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, ns, w.hb, X, w.N);
+  }
 }
 
 // out0 = A^H Y0 (B x B) and out1 = A^H Y1 (B x n1) over m rows, ld B
@@ -900,21 +1126,16 @@ static int stage1(hipStream_t s, BandWork& w, T* H, int ld, int nrhs) {
     gram<B>(s, w, m, w.V, w.V, N, nrhs > 0 ? w.gc + r0 : nullptr, nrhs, w.G, w.t1);
     hipLaunchKernelGGL((k_pT<B>), dim3(1), dim3(256), 0, s, k, w.G, w.t1, w.tau, nrhs, w.T, w.t2);
     hipLaunchKernelGGL((k_pY<B>), g, blk, 0, s, m, N, r0, w.V, w.T, w.t2, nrhs, w.Y, w.gc);
-    if constexpr (sizeof(T) == sizeof(zc)) {
-      rc = hmm_f64(s, m, b, reinterpret_cast<const zc*>(H) + r0 + (size_t)ld * r0, ld, w.Y, N, w.X, N, w.hw);
-    } else {
-      rc = hmm_f32(s, m, b, reinterpret_cast<const zf*>(H) + r0 + (size_t)ld * r0, ld, w.Y, N, w.X, N, w.hw);
-    }
-    if (rc) return rc;
+    hmm_band<T, B>(s, w, m, H + r0 + (size_t)ld * r0, ld, w.Y, w.X);
     gram<B>(s, w, m, w.V, w.X, N, nullptr, 0, w.M, nullptr);
     hipLaunchKernelGGL((k_pW<B>), g, blk, 0, s, m, N, w.V, w.T, w.M, w.X);
     const int nt = (m + 63) / 64;
     const bool tev = w.kev && w.kev_used < w.kev_cap;
     if (tev) hipEventRecord(w.kev[2 * w.kev_used], s);
-    hipLaunchKernelGGL((k_her2k_full<T>), dim3(nt, nt), dim3(256), 0, s, H, ld, m, r0, N, b, w.V, w.X);
+    hipLaunchKernelGGL((k_her2k_full<T, B>), dim3(nt, nt), dim3(256), 0, s, H, ld, m, r0, N, w.V, w.X);
     if (tev) {
       hipEventRecord(w.kev[2 * w.kev_used++ + 1], s);
-      w.kev_flops += 8.0 * (double)m * (double)m * (2.0 * k);   // m^2 entries, 2k complex products each
+      w.kev_flops += 8.0 * 0.5 * (double)m * (double)(m + 1) * (2.0 * k);   // lower triangle, 2k products each
     }
   }
   WTB_LAUNCH_CHECK();

@@ -168,7 +168,7 @@ struct wtb_bse_plan {
   // back-transformed vectors, HXr = H Xr, rq_res the residual norms |H x - theta x| / |x|
   // stage (d1) form: two-stage (dense -> band -> tridiagonal) when the path needs no
   // eigenvectors, one-stage Householder otherwise (its reflectors feed the back-transform)
-  int two_stage = 0;
+  int two_stage = -1;          // -1: two-stage from kTwoStageMinN on, 0: one-stage, 1: two-stage
   int band_b = 32;
   BandWork bw;
   void* bandwork = nullptr;
@@ -588,7 +588,6 @@ static int plan_create_impl(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_par
     delete p;
     return rc;
   }
-  if (N < 2) p->two_stage = 0;
   if ((rc = dc_prepare(p->dcwork, N, 2 * p->nrhs, p->tree))) {
     plan_free(p);
     delete p;
@@ -637,19 +636,20 @@ static void plan_drop_graph(wtb_bse_plan* p);
 // tridiagonal tw.d, tw.e, with Q^T applied to g (may be NULL).  Two-stage when nothing
 // downstream needs the Householder reflectors (exciton vectors, the refinement); its
 // bulge chase reports a stalled wait through status.
+// Matrices of at least this size take the two-stage reduction by default: its MFMA panel
+// updates overtake the one-stage GEMV chain between N = 5184 (one-stage faster) and
+// N = 36864 (two-stage 3.4x to 4.4x faster) on MI355X.
+constexpr int kTwoStageMinN = 8192;
+
 static int plan_reduce(wtb_bse_plan* p, zc* g, int* status) {
   hipStream_t s = p->stream;
   const bool vectors = p->wf_count > 0 || (p->mixed && p->nref > 0);
-  if (p->two_stage && !vectors && p->bandwork) {
-    int rc;
+  const bool two = p->two_stage == 1 || (p->two_stage < 0 && p->N >= kTwoStageMinN);
+  if (two && !vectors && p->bandwork) {
     p->bw.kev = p->kernel_timing ? p->kev.data() : nullptr;
     p->bw.kev_cap = p->kernel_timing ? p->N : 0;
-    p->tw.kev_used = 0;
-    if (p->mixed) {
-      if ((rc = sym_fill_f32(s, p->N, p->Hf, (int)p->ldh))) return rc;
-      return run_band_tridiag_f32(s, p->bw, p->Hf, (int)p->ldh, g, p->tw.d, p->tw.e, status);
-    }
-    if ((rc = sym_fill_f64(s, p->N, p->H, (int)p->ldh))) return rc;
+    // the band reduction reads and updates the lower triangle only
+    if (p->mixed) return run_band_tridiag_f32(s, p->bw, p->Hf, (int)p->ldh, g, p->tw.d, p->tw.e, status);
     return run_band_tridiag(s, p->bw, p->H, (int)p->ldh, g, p->tw.d, p->tw.e, status);
   }
   return p->mixed ? run_tridiag_f32(s, p->tw, p->Hf, (int)p->ldh, g) : run_tridiag(s, p->tw, p->H, (int)p->ldh, g);
@@ -1842,11 +1842,11 @@ int wtb_bse_plan_set_vectors(wtb_bse_plan* p, int first, int count) {
 }
 
 int wtb_bse_plan_set_reduction(wtb_bse_plan* p, int two_stage, int band) {
-  if (!p || (two_stage != 0 && two_stage != 1) || (band != 16 && band != 32)) {
-    set_error("wtb_bse_plan_set_reduction: two_stage 0 or 1, band 16 or 32");
+  if (!p || two_stage < -1 || two_stage > 1 || (band != 16 && band != 32)) {
+    set_error("wtb_bse_plan_set_reduction: two_stage -1, 0 or 1, band 16 or 32");
     return WTB_ERR_ARG;
   }
-  if (two_stage && p->N < 2) {
+  if (two_stage == 1 && p->N < 2) {
     set_error("wtb_bse_plan_set_reduction: the two-stage reduction needs N >= 2");
     return WTB_ERR_ARG;
   }

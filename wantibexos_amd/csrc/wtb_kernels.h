@@ -103,8 +103,8 @@ struct BandWork {
   zc *G = nullptr, *t1 = nullptr, *T = nullptr, *t2 = nullptr, *M = nullptr;
   zc* gw = nullptr;
   size_t gw_elems = 0;
-  zc* hw = nullptr;
   zc* gp = nullptr;             // partials of the panel's tall-skinny products
+  zc* hb = nullptr;             // K-slice partials of the trailing product X = A22 Y
   zc* BD = nullptr;
   zc* gc = nullptr;
   int* prog = nullptr;

@@ -85,3 +85,36 @@ def test_two_stage_small_and_ragged_sizes(ctx):
                 r2 = run(ctx, m, prm, True, band)
                 assert np.abs(r2["W"] - r1["W"]).max() < 1e-12, (g, nc, nv, band)
                 assert np.abs(r2["eps"] - r1["eps"]).max() <= 1e-9 * np.abs(r1["eps"]).max(), (g, band)
+
+
+@pytest.mark.parametrize("band", [16, 32])
+def test_two_stage_repeatable_at_full_size(ctx, band):
+    """The full-size matrix (hex2d nw=22, 36x36x1, N = 5184: 81 bulge-chasing workgroups
+    at once, 21-workgroup one-launch panels) executed six times in a row: every execute
+    bitwise equal to the first (the kernels' sums run in a fixed order, so any other
+    outcome is a data race), the trace preserved, and the energies those of the one-stage
+    reduction to rounding."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent))
+    import gen_models
+    from wantibexos_amd import bse
+    m = gen_models.generate([("hex2d", 2, 22, (36, 36, 1))])[0]
+    prm = bse.BSEParams(ngrid=(36, 36, 1), nc=2, nv=2, coultype="V2DRK", r0=40.0, cshift=0.05, ebse0=0.0,
+                        ebsef=6.0, numbse=1001, dtfull=True, cpol=True, f154=False)
+    r1 = run(ctx, m, prm, False)
+    with bse.BSEPlan(ctx, m, prm) as plan:
+        plan.set_reduction(True, band)
+        first = None
+        for rep in range(6):
+            print(f"two-stage N=5184 band {band} execute {rep}", flush=True)
+            plan.execute()
+            r = plan.results()
+            if first is None:
+                first = r
+                scale = np.abs(r1["W"]).max()
+                assert np.abs(r["W"] - r1["W"]).max() < 1e-11 * scale, band
+                assert abs(r["W"].sum() - r1["W"].sum()) < 1e-10 * abs(r1["W"].sum())
+            else:
+                assert np.array_equal(r["W"], first["W"]), (band, rep)
+                assert np.array_equal(r["eps"], first["eps"]), (band, rep)

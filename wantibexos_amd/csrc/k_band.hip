@@ -42,13 +42,13 @@ namespace wtb {
 // small helpers
 // ---------------------------------------------------------------------------
 __device__ inline zc zdiv(zc a, zc b) {
-  // a / b with the scaling of Smith's algorithm
-  if (fabs(b.re) >= fabs(b.im)) {
-    const double r = b.im / b.re, den = b.re + b.im * r;
-    return mkz((a.re + a.im * r) / den, (a.im - a.re * r) / den);
-  }
-  const double r = b.re / b.im, den = b.im + b.re * r;
-  return mkz((a.re * r + a.im) / den, (a.im * r - a.re) / den);
+  // a / b with the scaling of Smith's algorithm, both cases formed and one selected
+  const bool re_big = fabs(b.re) >= fabs(b.im);
+  const double p = re_big ? b.re : b.im, q = re_big ? b.im : b.re;
+  const double r = q / p, den = p + q * r;
+  const double xr = re_big ? a.re + a.im * r : a.re * r + a.im;
+  const double xi = re_big ? a.im - a.re * r : a.im * r - a.re;
+  return mkz(xr / den, xi / den);
 }
 
 __device__ inline zc shfl_z(zc v, int src) { return mkz(__shfl(v.re, src), __shfl(v.im, src)); }
@@ -89,20 +89,51 @@ __device__ inline double xchg(double x) {
 template <int OFF>
 __device__ inline zc xchg(zc v) { return mkz(xchg<OFF>(v.re), xchg<OFF>(v.im)); }
 
+// Data handed between workgroups of one kernel (the panel's partial sums, the chase's band
+// and tracked vectors) moves through device-scope (sc1) loads and stores, which the
+// hardware keeps coherent across the XCDs' L2 caches; the writer's stores are complete
+// (s_waitcnt) before its counter moves, so the hand-off needs no L2 write-back or
+// invalidate.
+__device__ inline zc ld_dev(const zc* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return mkz(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+             __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline void st_dev(zc* p, zc v) {
+  double* q = reinterpret_cast<double*>(p);
+  __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sum of p[b * stride] over b = b0, b0 + step, ... < b1 in that order, device-scope loads
+// issued eight at a time
+__device__ inline zc strided_sum_dev(const zc* p, size_t stride, int b0, int b1, int step) {
+  zc acc = mkz(0, 0);
+  for (int b = b0; b < b1; b += 8 * step) {
+    zc v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int bu = b + u * step;
+      v[u] = bu < b1 ? ld_dev(p + (size_t)bu * stride) : mkz(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b + u * step < b1) acc += v[u];
+  }
+  return acc;
+}
+
 // ZLARFG on (alpha, x) with |x|^2 = xn2: beta (real, returned as the new alpha), tau and
 // the scale 1/(alpha - beta) of x.  tau = 0 (H = I) when x = 0 and alpha is real.
 __device__ inline void larfg(zc alpha, double xn2, zc& beta, zc& tau, zc& scale) {
-  if (xn2 == 0.0 && alpha.im == 0.0) {
-    beta = alpha;
-    tau = mkz(0, 0);
-    scale = mkz(0, 0);
-    return;
-  }
+  const bool ident = xn2 == 0.0 && alpha.im == 0.0;
   const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-  const double b = alpha.re >= 0.0 ? -nrm : nrm;
-  tau = mkz((b - alpha.re) / b, -alpha.im / b);
-  scale = zdiv(mkz(1, 0), mkz(alpha.re - b, alpha.im));
-  beta = mkz(b, 0.0);
+  const double b = ident ? 1.0 : (alpha.re >= 0.0 ? -nrm : nrm);      // 1: any nonzero stand-in
+  const zc t = mkz((b - alpha.re) / b, -alpha.im / b);
+  const zc sc = zdiv(mkz(1, 0), mkz(alpha.re - b, alpha.im));
+  beta = ident ? alpha : mkz(b, 0.0);
+  tau = ident ? mkz(0, 0) : t;
+  scale = ident ? mkz(0, 0) : sc;
 }
 
 // Sums over the 256 threads of a workgroup of one real value x0 and NB complex values
@@ -292,6 +323,136 @@ __global__ __launch_bounds__(PR) void k_pfin(PanelArgs<T> a, zc* __restrict__ V)
     if (i <= j) stz(a.A[(size_t)(a.r0 + i) + (size_t)a.ld * (a.c0 + j)], p);    // R
     zc x = mkz(0, 0);
     if (j < a.k) x = i == j ? mkz(1, 0) : (i > j ? p : mkz(0, 0));
+    V[i + N * j] = x;
+  }
+}
+
+// The panel in one launch: workgroup w keeps rows 256w .. 256w + 255 of the panel in
+// registers through all of its k reflectors, and the workgroups meet once per column at a
+// counter (bar[c]: the workgroups that have published their partial sums of column c),
+// so the arithmetic is k_pqr_col's without the per-column launches and panel round trips.
+// Every workgroup must be resident at once: the host launches it only with at most one
+// workgroup per compute unit and no other kernel sharing the device, and every wait is
+// bounded (err[0] set past the limit, after which every workgroup returns).  At the end
+// the panel goes out as V (unit lower trapezoidal) and R (k_pfin's job).
+template <class T, int B>
+__global__ __launch_bounds__(PR) void k_panel(PanelArgs<T> a, zc* __restrict__ V, int* __restrict__ bar,
+                                              int* __restrict__ err, long long spin_limit) {
+  constexpr int NS = 256 / (B + 1);
+  __shared__ zc red[5 * 17], red2[5 * 17];
+  __shared__ zc sh[B + 4];
+  __shared__ zc shs[NS][B + 1];
+  __shared__ zc rcs[B];
+  __shared__ int flag;
+  const int i = blockIdx.x * PR + threadIdx.x;       // panel row
+  const bool live = i < a.m;
+  const size_t N = a.N;
+  zc row[B];
+#pragma unroll
+  for (int j = 0; j < B; ++j)
+    row[j] = (live && j < a.b) ? ldz(a.A[(size_t)(a.r0 + i) + (size_t)a.ld * (a.c0 + j)]) : mkz(0, 0);
+  for (int c = 0; c <= a.k; ++c) {
+    if (c > 0) {
+      const int cp = c - 1;
+      // every workgroup's partial sums of column cp are out
+      if (threadIdx.x == 0) {
+        long long n = 0;
+        int bad = 0;
+        while (__hip_atomic_load(bar + cp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.nwg) {
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++n > spin_limit) {
+            bad = 1;
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        flag = bad;
+      }
+      __syncthreads();
+      if (flag) return;
+      // sh[0].re = |x|^2 below the diagonal, sh[1 + j] = x^H P(:, j): NS threads per value
+      // each adding every NS-th workgroup's partial, then the NS sub-sums in order
+      const zc* pp = a.part + (size_t)(cp & 1) * a.nwg * (BMAX + 1);
+      if ((int)threadIdx.x < NS * (B + 1)) {
+        const int q = threadIdx.x % (B + 1), sub = threadIdx.x / (B + 1);
+        shs[sub][q] = strided_sum_dev(pp + q, BMAX + 1, sub, a.nwg, NS);
+      }
+      if ((int)threadIdx.x < B) rcs[threadIdx.x] = ld_dev(a.rowc + (cp & 1) * BMAX + threadIdx.x);
+      __syncthreads();
+      if ((int)threadIdx.x < B + 1) {
+        zc t = shs[0][threadIdx.x];
+        for (int u = 1; u < NS; ++u) t += shs[u][threadIdx.x];
+        sh[threadIdx.x] = t;
+      }
+      __syncthreads();
+      const zc alpha = rcs[cp];
+      zc beta, tau, scale;
+      larfg(alpha, sh[0].re, beta, tau, scale);
+      const zc ctau = conj(tau);
+      // v_i = 1 (i == cp), x_i scale (i > cp), 0 above
+      zc xcp = mkz(0, 0);
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (j == cp) xcp = row[j];
+      zc vi = mkz(0, 0);
+      if (live && i == cp) vi = mkz(1, 0);
+      if (live && i > cp) vi = xcp * scale;
+      // w_j = v^H P(:, j) = P(cp, j) + conj(scale) (x^H P(:, j) - conj(alpha) P(cp, j))
+      const zc cs = conj(scale);
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        if (j > cp && j < a.b) {
+          const zc pj = rcs[j];
+          const zc wj = pj + cs * (sh[1 + j] - conj(alpha) * pj);
+          if (live && i >= cp) row[j] -= (ctau * vi) * wj;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (j == cp && live && i >= cp) row[j] = (i == cp) ? beta : vi;
+      if (blockIdx.x == 0 && threadIdx.x == 0) a.tau[cp] = tau;
+    }
+    if (c < a.k) {
+      // partial sums of column c over this workgroup's rows, 16 columns at a time (fewer
+      // live registers), out with row c
+      const bool on = live && i >= c;
+      zc x = mkz(0, 0);
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (j == c) x = row[j];
+      const double x0 = (on && i > c) ? norm2(x) : 0.0;
+#pragma unroll
+      for (int h = 0; h < B / 16; ++h) {
+        zc v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int j = 16 * h + q;
+          v[q] = (on && j > c && j < a.b) ? fma_conj(x, row[j], mkz(0, 0)) : mkz(0, 0);
+        }
+        block_sum_bf<16>(h == 0 ? x0 : 0.0, v, h == 0 ? red : red2);
+      }
+      if ((int)threadIdx.x < B + 1) {
+        const int q = threadIdx.x;
+        const zc val = (q <= 16) ? red[4 * 17 + q] : red2[4 * 17 + q - 16];
+        st_dev(a.part + (size_t)(c & 1) * a.nwg * (BMAX + 1) + (size_t)blockIdx.x * (BMAX + 1) + q, val);
+      }
+      if (live && i == c) {
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+          if (j >= c && j < a.b) st_dev(a.rowc + (c & 1) * BMAX + j, row[j]);
+      }
+      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (threadIdx.x == 0) __hip_atomic_fetch_add(bar + c, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    if (j >= a.b) continue;
+    if (i <= j) stz(a.A[(size_t)(a.r0 + i) + (size_t)a.ld * (a.c0 + j)], row[j]);    // R
+    zc x = mkz(0, 0);
+    if (j < a.k) x = i == j ? mkz(1, 0) : (i > j ? row[j] : mkz(0, 0));
     V[i + N * j] = x;
   }
 }
@@ -766,21 +927,6 @@ __device__ inline int chase_ntasks(int N, int b, int s) { return 1 + (N - 2 - s)
 __device__ inline zc& bd(const ChaseArgs& a, int r, int c) { return a.BD[(size_t)c * a.LB + (r - c)]; }
 
 
-// Band and tracked-vector elements move between the sweeps' workgroups through
-// device-scope (sc1) loads and stores, which the hardware keeps coherent across the XCDs'
-// L2 caches; a task's stores are complete (s_waitcnt) before its progress counter moves,
-// so the hand-off needs no L2 write-back or invalidate.
-__device__ inline zc ld_dev(const zc* p) {
-  const double* q = reinterpret_cast<const double*>(p);
-  return mkz(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-             __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ inline void st_dev(zc* p, zc v) {
-  double* q = reinterpret_cast<double*>(p);
-  __hip_atomic_store(q, v.re, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(q + 1, v.im, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // sum over the B (16 or 32) lanes of a row group: lane pairs ^1, ^2 (quad permutations),
 // then i <-> 7 - i, i <-> 15 - i within rows (half mirror, mirror), then ^16.  Every step
 // adds two equal partial sums in either order, so every lane of the group ends with the
@@ -929,9 +1075,7 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
       zc vc[NC];                               // v at this thread's columns
 #pragma unroll
       for (int q = 0; q < NC; ++q) vc[q] = shfl_z(vr, gb + c0 + q);
-      if (k == 0) {
-        if (h == 0 && rin) st_dev(&bd(a, st + r, s), (r == 0) ? beta : mkz(0, 0));
-      } else {
+      if (k > 0) {
         // (iii) O(:, 1:) <- H^H O(:, 1:): z_c = v^H O(:, c), O -= conj(tau) v z
         zc tq[NC];
 #pragma unroll
@@ -952,6 +1096,9 @@ __global__ __launch_bounds__(256) void k_chase(ChaseArgs a) {
 #pragma unroll
       for (int q = 0; q < NC; ++q) y = fma_z(D[q], vc[q], y);
       y = row_sum<B>(y, red2, t);
+      // column s (task 0): beta then zeros, stored only now that every wave has loaded it
+      // (row_sum's barrier)
+      if (k == 0 && h == 0 && rin) st_dev(&bd(a, st + r, s), (r == 0) ? beta : mkz(0, 0));
       const double gam = group_sum<B>(vr.re * y.re + vr.im * y.im);
       const zc wr = tau * y - (0.5 * norm2(tau) * gam) * vr;
 #pragma unroll
@@ -981,17 +1128,18 @@ __global__ void k_band_finish(const zc* __restrict__ BD, int N, int LB, double* 
                               const zc* __restrict__ gc, zc* __restrict__ g, int nrhs, const int* __restrict__ err,
                               int* __restrict__ status) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0 && status && err[0]) status[0] |= 1 << WTB_ERR_EIG;
+  if (j == 0 && status && (err[0] || err[8])) status[0] |= 1 << WTB_ERR_EIG;
   if (j >= N) return;
   d[j] = BD[(size_t)j * LB].re;
   e[j] = (j + 1 < N) ? BD[(size_t)j * LB + 1].re : 0.0;
   for (int r = 0; r < nrhs; ++r) g[(size_t)r * N + j] = conj(gc[(size_t)r * N + j]);
 }
 
-// control block of the chase: error flag and sweep counter 0
+// control block of the chase: its error flag and sweep counter 0 (the panel's flag, [8],
+// is kept)
 __global__ void k_chase_init(int* ctl) {
   const int i = threadIdx.x;
-  if (i < 64) ctl[i] = 0;
+  if (i == 0 || i == 32) ctl[i] = 0;
 }
 
 __global__ void k_zero_ints(int* p, int n) {
@@ -1026,6 +1174,7 @@ size_t band_work_bytes(int N, int b, int nrhs) {
   add(sizeof(zc) * (size_t)N * (nrhs > 0 ? nrhs : 1));   // gc
   add(sizeof(int) * ((size_t)N + 1));         // prog
   add(sizeof(int) * 64);                      // err
+  add(sizeof(int) * ((size_t)N / b + 2) * (BMAX + 1));   // panel column counters
   return s + 4096;
 }
 
@@ -1066,6 +1215,7 @@ int band_bind(BandWork& w, void* base, int N, int b, int nrhs) {
   w.gc = (zc*)take(sizeof(zc) * (size_t)N * (nrhs > 0 ? nrhs : 1));
   w.prog = (int*)take(sizeof(int) * ((size_t)N + 1));
   w.err = (int*)take(sizeof(int) * 64);
+  w.bar = (int*)take(sizeof(int) * ((size_t)N / b + 2) * (BMAX + 1));
   return WTB_OK;
 }
 
@@ -1120,8 +1270,13 @@ static int stage1(hipStream_t s, BandWork& w, T* H, int ld, int nrhs) {
     a.tau = w.tau;
     a.nwg = (m + PR - 1) / PR;
     const dim3 g(a.nwg), blk(PR);
-    for (int c = 0; c <= k; ++c) hipLaunchKernelGGL((k_pqr_col<T, B>), g, blk, 0, s, a, c);
-    hipLaunchKernelGGL((k_pfin<T, B>), g, blk, 0, s, a, w.V);
+    if (w.panel_resident > 0 && a.nwg <= w.panel_resident) {
+      hipLaunchKernelGGL((k_panel<T, B>), g, blk, 0, s, a, w.V, w.bar + (size_t)p * (BMAX + 1), w.err + 8,
+                         (long long)1 << 20);
+    } else {
+      for (int c = 0; c <= k; ++c) hipLaunchKernelGGL((k_pqr_col<T, B>), g, blk, 0, s, a, c);
+      hipLaunchKernelGGL((k_pfin<T, B>), g, blk, 0, s, a, w.V);
+    }
     int rc;
     gram<B>(s, w, m, w.V, w.V, N, nrhs > 0 ? w.gc + r0 : nullptr, nrhs, w.G, w.t1);
     hipLaunchKernelGGL((k_pT<B>), dim3(1), dim3(256), 0, s, k, w.G, w.t1, w.tau, nrhs, w.T, w.t2);
@@ -1155,6 +1310,11 @@ static int run_band_t(hipStream_t s, BandWork& w, T* H, int ld, zc* g, double* d
                        w.gc);
   w.kev_used = 0;
   w.kev_flops = 0.0;
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, w.err, 64);
+  {
+    const int nb = (N / b + 2) * (BMAX + 1);
+    hipLaunchKernelGGL(k_zero_ints, dim3((nb + 255) / 256), dim3(256), 0, s, w.bar, nb);
+  }
   int rc = (b == 16) ? stage1<T, 16>(s, w, H, ld, nrhs) : stage1<T, 32>(s, w, H, ld, nrhs);
   if (rc) return rc;
   hipLaunchKernelGGL(k_band_extract<T>, dim3(N), dim3(64), 0, s, H, ld, N, b, w.LB, w.BD);

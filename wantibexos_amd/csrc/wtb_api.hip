@@ -588,6 +588,13 @@ static int plan_create_impl(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_par
     delete p;
     return rc;
   }
+  // one-launch panels need every workgroup resident: at most one per compute unit of a
+  // device the plan does not share (a batch sets 0: per-column launches)
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) cus = 0;
+    p->bw.panel_resident = cus;
+  }
   if ((rc = dc_prepare(p->dcwork, N, 2 * p->nrhs, p->tree))) {
     plan_free(p);
     delete p;
@@ -1722,6 +1729,7 @@ int wtb_bse_batch_create(wtb_ctx* c, int n, const wtb_tb_model* models, const wt
     const double N = b->plans[i]->N;
     load[l] += N * N * N;
     b->plans[i]->stream = b->lanes[l];
+    if (nlanes > 1) b->plans[i]->bw.panel_resident = 0;   // lanes share the device
   }
   *out = b;
   return WTB_OK;
@@ -1863,6 +1871,7 @@ int wtb_bse_plan_set_reduction(wtb_bse_plan* p, int two_stage, int band) {
     }
     hipFree(p->bandwork);
     p->bandwork = nw;
+    nb.panel_resident = p->bw.panel_resident;
     p->bw = nb;
     p->band_b = band;
   }
@@ -2144,6 +2153,7 @@ int wtb_bse_bands_create(wtb_ctx* c, const wtb_tb_model* m, const wtb_bse_params
     }
     b->streams.push_back(st);
     p->stream = st;
+    if (lanes > 1) p->bw.panel_resident = 0;   // lanes share the device
   }
   if (rc) {
     wtb_bse_bands_destroy(b);

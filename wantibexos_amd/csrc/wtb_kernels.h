@@ -108,7 +108,9 @@ struct BandWork {
   zc* BD = nullptr;
   zc* gc = nullptr;
   int* prog = nullptr;
-  int* err = nullptr;           // chase control block: [0] error flag, [32] next sweep
+  int* err = nullptr;           // [0] the chase's error flag, [8] the panel's, [32] next sweep
+  int* bar = nullptr;           // per panel and column: workgroups past the column
+  int panel_resident = 0;       // one-launch panels up to this many workgroups (0: per column)
   // optional per-launch timing of the trailing-update kernel (k_her2k_full): launch c is
   // bracketed by kev[2c], kev[2c+1] (c < kev_cap); kev_flops[c] its flop count
   hipEvent_t* kev = nullptr;

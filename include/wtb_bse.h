@@ -423,7 +423,7 @@ int wtb_bse_plan_set_refine(wtb_bse_plan* plan, int nref);
  * the energies before the refinement (n <= nref)                                       */
 int wtb_bse_plan_refine_results(wtb_bse_plan* plan, int n, double* resid, double* w_unrefined);
 
-/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid
+/* wtb_bsesolver plus those vectors (wf: [count][N] complex) and the k grid
  * (kpt(nk,3) Fortran order) and nocpk(nk) that excwf2 prints beside them */
 int wtb_bsesolver_wf(wtb_ctx* ctx, const wtb_tb_model* model, const wtb_bse_params* prm, double* W,
                      double* act, double* eps, double* egap, int wf_first, int wf_count, double* wf,

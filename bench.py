@@ -220,10 +220,11 @@ def bench_davidson(args, cfg, model):
 
 
 def bench_batch(args):
-    """Batching form: a set of uneven models (different Wannier counts and k grids) solved
-    as single plans one after the other, against the same set through the batch API
-    (one captured graph per matrix, replayed on --lanes streams); the results must agree
-    bit for bit."""
+    """Batching form: a set of uneven models (different Wannier counts and k grids) through
+    the batch API (one captured graph for the set: per-matrix build and post-processing on
+    --lanes streams, the one-stage reductions in lockstep, one launch per kernel and column
+    step for all matrices), against the same plans run one after the other (results equal
+    bit for bit) and against plain FP64 plans of the same models one after the other."""
     import dataclasses
     from concurrent.futures import ProcessPoolExecutor
     from wantibexos_amd import bse, tbmodel
@@ -244,20 +245,31 @@ def bench_batch(args):
     dims = [pl.N for pl in plans]
     log(f"batching form: {len(plans)} matrices, N = {dims}, precision {args.precision}, lanes {args.lanes}")
 
-    def run_single():
-        for pl in plans:
+    def run_seq(pls):
+        for pl in pls:
             pl.execute()
         ctx.sync()
 
-    for _ in range(args.warmup):
-        run_single()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run_single()
-    t_single = (time.perf_counter() - t0) / args.steps
+    def time_seq(pls):
+        for _ in range(args.warmup):
+            run_seq(pls)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run_seq(pls)
+        return (time.perf_counter() - t0) / args.steps
+
+    t_single = time_seq(plans)
     ref = [pl.results() for pl in plans]
     for pl in plans:
         pl.close()
+    t_fp64 = t_single
+    if args.precision != "fp64":
+        # the sequential plain FP64 solves of the same models
+        plans = [bse.BSEPlan(ctx, m, params(dict(CONFIG, ngrid=(ng, ng, 1)), "fp64")) for m, (ng, _) in zip(models, spec)]
+        t_fp64 = time_seq(plans)
+        for pl in plans:
+            pl.close()
+    log(f"sequential plain FP64 plans {t_fp64 * 1e3:.1f} ms per set")
     with bse.BSEBatch(ctx, models, prms, lanes=args.lanes) as b:
         tiles = b.tune().tolist() if args.tune else None
         for _ in range(args.warmup):
@@ -273,13 +285,14 @@ def bench_batch(args):
     ctx.close()
     log(f"single plans {t_single * 1e3:.1f} ms per set, batch {t_batch * 1e3:.1f} ms per set")
     out = {
-        "metric": "BSE build+solve of a set of uneven matrices, batch API vs single plans",
+        "metric": "BSE build+solve of a set of uneven matrices, batch API vs sequential single plans",
         "value": round(t_batch, 6), "unit": "s/set", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "c128" if args.precision == "fp64" else "c64/c128", "data": "synthetic (seeded random tight-binding models)",
         "config": {"workload": f"hex2d set of {len(spec)}", "models": [{"kgrid": ng, "nwann": nw, "bse_dim": N} for (ng, nw), N in zip(spec, dims)],
                    "lanes": args.lanes, "precision": args.precision, "tuned_panels": tiles},
         "single_plans_s": round(t_single, 6), "speedup_vs_single": round(t_single / t_batch, 3),
+        "fp64_plain_sequential_s": round(t_fp64, 6), "speedup_vs_fp64_sequential": round(t_fp64 / t_batch, 3),
         "matrices_per_s": round(len(dims) / t_batch, 2), "bitwise_equal_to_single": same, "max_dW_eV": dW,
     }
     print(json.dumps(out), flush=True)

@@ -26,10 +26,12 @@
 // Global reductions go through partials that the next kernel sums (every
 // workgroup of it, in a fixed order): no atomics, no device fences, no waits.
 #include "wtb_kernels.h"
+#include <algorithm>
 
 namespace wtb {
 
 #define TB 64   // tile / row-block edge
+constexpr int kTriBatchMax = 64;   // matrices per batched launch
 #define RPW 16  // rows per rowstep workgroup
 
 template <class T>
@@ -233,7 +235,7 @@ __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums
 // Loads that do not depend on reflector j-1's scalars (the GEMV partials, the panel
 // rows, column j) are issued first; the panel values serve both u and the column update.
 template <class T>
-__global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj, int update_col) {
+__device__ __forceinline__ void tri_rowstep(const TriArgs<T>& a, int bx, int j, int jj, int update_col) {
   __shared__ zc st1[32], st2[32], svg[16], sWj[32], sVj[32], sums[96];
   __shared__ zc su_j, s_tau, s_s;
   __shared__ zc suv[RPW];
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   const int N = a.N;
   const int jm = j - 1, jjm = jj - 1;
   const int bm = (jm + 1) / TB;                   // first row block of reflector jm's GEMV
-  const int r = j + blockIdx.x * RPW + row_l;
+  const int r = j + bx * RPW + row_l;
   const bool valid = r < N;
   auto ysum = [&](int row) -> zc {
     const zc* yp = a.ypart + ((size_t)(row / TB) * a.nblk) * TB + (row % TB);
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   if (tid >= 64 && tid < 64 + a.nrhs) g_pre = a.gs[tid - 64];
   const zc xnr = (valid && r != j) ? a.xn[r] : mkz(0, 0);
   const zc gr = (valid && sub < a.nrhs) ? a.g[(size_t)sub * N + r] : mkz(0, 0);
-  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, blockIdx.x == 0, alpha_pre, w_pre, v_pre,
+  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, bx == 0, alpha_pre, w_pre, v_pre,
                     g_pre);
   const zc tau = s_tau, s = s_s;
   // s z(row) - V t1 - W t2, each row summed by its 16 lanes
@@ -362,8 +364,12 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
   if (tid == 0) {
     zc uv = mkz(0, 0);
     for (int i = 0; i < RPW; ++i) uv += suv[i];
-    a.puv[blockIdx.x] = uv;
+    a.puv[bx] = uv;
   }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj, int update_col) {
+  tri_rowstep(a, blockIdx.x, j, jj, update_col);
 }
 
 // ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
@@ -372,7 +378,7 @@ __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj
 // registers), 6 for the COMPLEX*8 one (80: FP32 tile sums, 8-quantity epilogue groups);
 // more waves keep more tiles' loads in flight
 template <class T>
-__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
+__device__ __forceinline__ void tri_hemv(const TriArgs<T>& a, int bx, int j, int jj, int nprev) {
   constexpr int XRN = TB;                   // rows of x~ a workgroup needs
   constexpr int XT = XRN + TB;              // threads loading x~ (rows, then columns)
   __shared__ zc xR[XRN], xC[TB];
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv(TriAr
   __shared__ zc s_ap;
   const int N = a.N;
   const int b0 = (j + 1) / TB;
-  const int t = blockIdx.x;
+  const int t = bx;
   int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
@@ -595,23 +601,31 @@ __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv(TriAr
     }
   }
 }
+template <class T>
+__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
+  tri_hemv(a, blockIdx.x, j, jj, nprev);
+}
 
 // ---- W(:, kp-1) of the panel's last reflector ----------------------------------------
 template <class T>
-__global__ __launch_bounds__(64) void k_tri_wlast(TriArgs<T> a, int jl, int kpm1, int nprev) {
+__device__ __forceinline__ void tri_wlast(const TriArgs<T>& a, int bx, int jl, int kpm1, int nprev) {
   const int lane = threadIdx.x;
   const zc tp = a.tau[jl];
   const zc ap = alpha_from_puv(a.puv, nprev, tp);
-  const int r = ((jl + 1) / TB + blockIdx.x) * TB + lane;
+  const int r = ((jl + 1) / TB + bx) * TB + lane;
   if (r > jl && r < a.N) a.W[(size_t)kpm1 * a.N + r] = tp * a.u[r] + ap * a.Vp[(size_t)kpm1 * a.N + r];
+}
+template <class T>
+__global__ __launch_bounds__(64) void k_tri_wlast(TriArgs<T> a, int jl, int kpm1, int nprev) {
+  tri_wlast(a, blockIdx.x, jl, kpm1, nprev);
 }
 
 // ---- trailing update A22 -= V W^H + W V^H on lower 64x64 tiles (MFMA f64) ---------------
 template <class T>
-__global__ __launch_bounds__(256) void k_tri_her2k(TriArgs<T> a, int s0, int kp) {
+__device__ __forceinline__ void tri_her2k(const TriArgs<T>& a, int bx, int s0, int kp) {
   const int N = a.N;
   const int b0 = s0 / TB;
-  int t = blockIdx.x;
+  int t = bx;
   int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
   while (ti * (ti + 1) / 2 > t) --ti;
@@ -713,20 +727,91 @@ __global__ __launch_bounds__(256) void k_tri_her2k(TriArgs<T> a, int s0, int kp)
         }
   }
 }
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_her2k(TriArgs<T> a, int s0, int kp) {
+  tri_her2k(a, blockIdx.x, s0, kp);
+}
 
 // reflectors of the finished panel into the lower triangle: H(r, j) = v_j(r), r > j+1
 template <class T>
-__global__ __launch_bounds__(256) void k_tri_storev(TriArgs<T> a, int p) {
-  const int c = blockIdx.y, j = p + c;
-  const int r = j + 2 + blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void tri_storev(const TriArgs<T>& a, int bx, int c, int p) {
+  const int j = p + c;
+  const int r = j + 2 + bx * 256 + threadIdx.x;
   if (r < a.N) stz(a.H[(size_t)r + (size_t)a.ld * j], a.Vp[(size_t)c * a.N + r]);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_storev(TriArgs<T> a, int p) {
+  tri_storev(a, blockIdx.x, blockIdx.y, p);
 }
 
 template <class T>
-__global__ void k_tri_last(TriArgs<T> a) {
+__device__ __forceinline__ void tri_last(const TriArgs<T>& a) {
   zc x = ldz(a.H[(size_t)(a.N - 1) + (size_t)a.ld * (a.N - 1)]);
   a.d[a.N - 1] = x.re;
   a.e[a.N - 1] = 0.0;
+}
+template <class T>
+__global__ void k_tri_last(TriArgs<T> a) {
+  tri_last(a);
+}
+
+// ---- batched launches --------------------------------------------------------------
+// One launch serves up to kTriBatchMax matrices at the same column step: workgroup b
+// belongs to the matrix whose range [pre[i], pre[i+1]) holds it, and runs the single
+// matrix's kernel body on that matrix's arguments with block index b - pre[i], so every
+// matrix goes through exactly the arithmetic of its own reduction.
+template <class T>
+struct TriBatch {
+  const TriArgs<T>* args;     // device array, one per matrix of the group
+  int n;
+  int m[kTriBatchMax];        // matrix of slot i
+  int pre[kTriBatchMax + 1];  // first workgroup of slot i
+};
+
+__device__ inline int tri_slot(const int* pre, int n, int b) {
+  int i = 0;
+  while (i + 1 < n && pre[i + 1] <= b) ++i;
+  return i;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_rowstep_b(TriBatch<T> B, int j, int jj, int update_col) {
+  const int i = tri_slot(B.pre, B.n, blockIdx.x);
+  const TriArgs<T> a = B.args[B.m[i]];
+  tri_rowstep(a, (int)blockIdx.x - B.pre[i], j, jj, update_col);
+}
+template <class T>
+__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv_b(TriBatch<T> B, int j, int jj) {
+  const int i = tri_slot(B.pre, B.n, blockIdx.x);
+  const TriArgs<T> a = B.args[B.m[i]];
+  const int nprev = jj > 0 ? (a.N - j + RPW - 1) / RPW : 0;
+  tri_hemv(a, (int)blockIdx.x - B.pre[i], j, jj, nprev);
+}
+template <class T>
+__global__ __launch_bounds__(64) void k_tri_wlast_b(TriBatch<T> B, int jl, int kpm1) {
+  const int i = tri_slot(B.pre, B.n, blockIdx.x);
+  const TriArgs<T> a = B.args[B.m[i]];
+  const int nF = (a.N - (jl + 1) + RPW - 1) / RPW;
+  tri_wlast(a, (int)blockIdx.x - B.pre[i], jl, kpm1, nF);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_her2k_b(TriBatch<T> B, int s0, int kp) {
+  const int i = tri_slot(B.pre, B.n, blockIdx.x);
+  const TriArgs<T> a = B.args[B.m[i]];
+  tri_her2k(a, (int)blockIdx.x - B.pre[i], s0, kp);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_storev_b(TriBatch<T> B, int p) {
+  const int i = tri_slot(B.pre, B.n, blockIdx.x);
+  const TriArgs<T> a = B.args[B.m[i]];
+  const int gx = (a.N - p - 2 + 255) / 256;
+  const int b = (int)blockIdx.x - B.pre[i];
+  tri_storev(a, b % gx, b / gx, p);
+}
+template <class T>
+__global__ void k_tri_last_b(TriBatch<T> B) {
+  const TriArgs<T> a = B.args[B.m[blockIdx.x]];
+  tri_last(a);
 }
 
 
@@ -773,11 +858,14 @@ int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs) {
   return WTB_OK;
 }
 
+// the panel width a reduction runs with: the tuned one when it fits the binding
+static int tri_panel(const TridiagWork& tw) {
+  return (tw.panel >= 8 && tw.panel <= tw.nb) ? tw.panel : tw.nb;
+}
+
 template <class T>
-static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_g) {
+static TriArgs<T> tri_args(const TridiagWork& tw, T* d_H, int ldh, zc* d_g, int nb) {
   const int N = tw.N;
-  const int nb = (tw.panel >= 8 && tw.panel <= tw.nb) ? tw.panel : tw.nb;
-  if (nb > 32 || tw.nrhs > 16) return WTB_ERR_LIMIT;
   TriArgs<T> a;
   a.H = d_H;
   a.ld = ldh;
@@ -801,6 +889,15 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
   a.xp = tw.xp;
   a.gs = tw.gs;
   a.nrhs = tw.nrhs;
+  return a;
+}
+
+template <class T>
+static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_g) {
+  const int N = tw.N;
+  const int nb = tri_panel(tw);
+  if (nb > 32 || tw.nrhs > 16) return WTB_ERR_LIMIT;
+  const TriArgs<T> a = tri_args(tw, d_H, ldh, d_g, nb);
   tw.kev_used = 0;
   if (N == 1) {
     hipLaunchKernelGGL(k_tri_last<T>, dim3(1), dim3(1), 0, s, a);
@@ -846,6 +943,130 @@ int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g) {
 }
 int run_tridiag_f32(hipStream_t s, TridiagWork& tw, zf* d_H, int ldh, zc* d_g) {
   return run_tridiag_t<zf>(s, tw, d_H, ldh, d_g);
+}
+
+// ---- a group of reductions in lockstep ------------------------------------------------
+// One launch per kernel and column step for every matrix still reducing that column (in
+// chunks of kTriBatchMax).  Each matrix sees the same launches, in the same order and
+// with the same arguments, as run_tridiag_t gives it alone; only the matrices in between
+// differ.
+template <class T, class F, class... A>
+static void launch_group(hipStream_t s, F kern, const TriArgs<T>* dargs, const std::vector<int>& ms,
+                         const std::vector<int>& nbk, int threads, A... args) {
+  for (size_t c0 = 0; c0 < ms.size(); c0 += kTriBatchMax) {
+    TriBatch<T> B;
+    B.args = dargs;
+    B.n = 0;
+    int tot = 0;
+    for (size_t i = c0; i < ms.size() && i < c0 + kTriBatchMax; ++i) {
+      if (nbk[i] <= 0) continue;
+      B.m[B.n] = ms[i];
+      B.pre[B.n] = tot;
+      tot += nbk[i];
+      ++B.n;
+    }
+    if (B.n == 0) continue;
+    B.pre[B.n] = tot;
+    hipLaunchKernelGGL(kern, dim3(tot), dim3(threads), 0, s, B, args...);
+  }
+}
+
+template <class T>
+static int run_tridiag_group_t(hipStream_t s, const TriGroup& G) {
+  const TriArgs<T>* dargs = static_cast<const TriArgs<T>*>(G.d_args);
+  const int M = (int)G.items.size(), nb = G.nb;
+  std::vector<int> N(M), nblk(M);
+  int maxN = 0;
+  for (int i = 0; i < M; ++i) {
+    N[i] = G.items[i].tw->N;
+    nblk[i] = (N[i] + TB - 1) / TB;
+    maxN = std::max(maxN, N[i]);
+  }
+  std::vector<int> ms, fs, sel, nbk;
+  auto counts = [&](const std::vector<int>& list, auto f) {
+    nbk.resize(list.size());
+    for (size_t k = 0; k < list.size(); ++k) nbk[k] = f(list[k]);
+  };
+  for (int p = 0; p < maxN - 1; p += nb) {
+    for (int jj = 0; jj < nb; ++jj) {
+      const int j = p + jj;
+      ms.clear();
+      fs.clear();
+      for (int i = 0; i < M; ++i) {
+        if (N[i] - 1 <= p) continue;
+        const int kp = std::min(nb, N[i] - 1 - p);
+        if (jj < kp) ms.push_back(i);
+        if (jj + 1 == kp) fs.push_back(i);
+      }
+      if (ms.empty()) break;
+      if (jj > 0) {
+        counts(ms, [&](int i) { return (N[i] - j + RPW - 1) / RPW; });
+        launch_group<T>(s, k_tri_rowstep_b<T>, dargs, ms, nbk, 256, j, jj, 1);
+      }
+      counts(ms, [&](int i) {
+        const int nt = nblk[i] - (j + 1) / TB;
+        return nt * (nt + 1) / 2;
+      });
+      launch_group<T>(s, k_tri_hemv_b<T>, dargs, ms, nbk, 256, j, jj);
+      if (fs.empty()) continue;
+      // the panel ends at column j for these matrices
+      const int kp = jj + 1, jl = j, jn = j + 1, s0 = p + kp;
+      counts(fs, [&](int i) { return (N[i] - jn + RPW - 1) / RPW; });
+      launch_group<T>(s, k_tri_rowstep_b<T>, dargs, fs, nbk, 256, jn, kp, 0);
+      counts(fs, [&](int i) { return nblk[i] - (jl + 1) / TB; });
+      launch_group<T>(s, k_tri_wlast_b<T>, dargs, fs, nbk, 64, jl, kp - 1);
+      sel.clear();
+      for (int i : fs)
+        if (G.items[i].tw->store_v && N[i] - p - 2 > 0) sel.push_back(i);
+      counts(sel, [&](int i) { return (N[i] - p - 2 + 255) / 256 * kp; });
+      launch_group<T>(s, k_tri_storev_b<T>, dargs, sel, nbk, 256, p);
+      sel.clear();
+      for (int i : fs)
+        if (s0 < N[i]) sel.push_back(i);
+      counts(sel, [&](int i) {
+        const int nt2 = nblk[i] - s0 / TB;
+        return nt2 * (nt2 + 1) / 2;
+      });
+      launch_group<T>(s, k_tri_her2k_b<T>, dargs, sel, nbk, 256, s0, kp);
+    }
+  }
+  ms.resize(M);
+  for (int i = 0; i < M; ++i) ms[i] = i;
+  counts(ms, [&](int) { return 1; });
+  launch_group<T>(s, k_tri_last_b<T>, dargs, ms, nbk, 1);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int tri_group_prepare(TriGroup& G) {
+  tri_group_free(G);
+  const int M = (int)G.items.size();
+  if (M == 0) return WTB_OK;
+  for (const auto& it : G.items)
+    if (tri_panel(*it.tw) != G.nb || G.nb > 32 || it.tw->nrhs > 16) return WTB_ERR_LIMIT;
+  if (G.f32) {
+    std::vector<TriArgs<zf>> h(M);
+    for (int i = 0; i < M; ++i) h[i] = tri_args(*G.items[i].tw, (zf*)G.items[i].H, G.items[i].ldh, G.items[i].g, G.nb);
+    WTB_HIP(hipMalloc(&G.d_args, sizeof(TriArgs<zf>) * M));
+    WTB_HIP(hipMemcpy(G.d_args, h.data(), sizeof(TriArgs<zf>) * M, hipMemcpyHostToDevice));
+  } else {
+    std::vector<TriArgs<zc>> h(M);
+    for (int i = 0; i < M; ++i) h[i] = tri_args(*G.items[i].tw, (zc*)G.items[i].H, G.items[i].ldh, G.items[i].g, G.nb);
+    WTB_HIP(hipMalloc(&G.d_args, sizeof(TriArgs<zc>) * M));
+    WTB_HIP(hipMemcpy(G.d_args, h.data(), sizeof(TriArgs<zc>) * M, hipMemcpyHostToDevice));
+  }
+  return WTB_OK;
+}
+
+void tri_group_free(TriGroup& G) {
+  if (G.d_args) (void)hipFree(G.d_args);
+  G.d_args = nullptr;
+}
+
+int run_tridiag_group(hipStream_t s, const TriGroup& G) {
+  if (G.items.empty()) return WTB_OK;
+  if (!G.d_args) return WTB_ERR_ARG;
+  return G.f32 ? run_tridiag_group_t<zf>(s, G) : run_tridiag_group_t<zc>(s, G);
 }
 
 }  // namespace wtb

@@ -729,8 +729,8 @@ static int qmode_status(const int st[2]) {
   return k_bits | q_bits;
 }
 
-static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
-  if (p->qmode) return plan_enqueue_q(p, build_only);
+// stages (a)-(c) and the tracked vectors' copy: everything before the reduction
+static int plan_enqueue_pre(wtb_bse_plan* p, bool build_only) {
   hipStream_t s = p->stream;
   const wtb_bse_params& P = p->prm;
   const int nk = p->nk, nw = p->nw, N = p->N;
@@ -768,7 +768,27 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
   if (P.bset && (rc = launch_scale_rows(s, 3, N, p->dip, p->fdeh, p->g + (size_t)3 * N))) return rc;
   p->tw.kev = p->kernel_timing ? p->kev.data() : nullptr;
   p->tw.kev_cap = p->kernel_timing ? p->N : 0;
-  if ((rc = plan_reduce(p, p->g, p->status))) return rc;
+  return WTB_OK;
+}
+
+// whether the plan's reduction is the one-stage form, which a batch runs in lockstep
+static bool plan_one_stage(const wtb_bse_plan* p) {
+  if (p->qmode || p->sp_only) return false;
+  const bool vectors = p->wf_count > 0 || (p->mixed && p->nref > 0);
+  const bool two = p->two_stage == 1 || (p->two_stage < 0 && p->N >= kTwoStageMinN);
+  return !(two && !vectors && p->bandwork);
+}
+
+// stages (d2)-(e): everything after the reduction
+static int plan_enqueue_post(wtb_bse_plan* p) {
+  hipStream_t s = p->stream;
+  const wtb_bse_params& P = p->prm;
+  const int N = p->N;
+  const bool refine = p->mixed && p->nref > 0;
+  int rc;
+  auto mark = [&](int i) {
+    if (p->timing) hipEventRecord(p->ev[i], s);
+  };
   mark(4);
   // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
   hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->nrhs, p->g, p->track);
@@ -809,6 +829,14 @@ static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
     return rc;
   mark(6);
   return WTB_OK;
+}
+
+static int plan_enqueue(wtb_bse_plan* p, bool build_only = false) {
+  if (p->qmode) return plan_enqueue_q(p, build_only);
+  int rc = plan_enqueue_pre(p, build_only);
+  if (rc || build_only || p->sp_only) return rc;
+  if ((rc = plan_reduce(p, p->g, p->status))) return rc;
+  return plan_enqueue_post(p);
 }
 
 // The whole pipeline is captured once into a HIP graph (a few thousand kernel
@@ -1675,7 +1703,12 @@ int wtb_bse_plan_tune(wtb_bse_plan* p, int* best, double* ms) {
 }
 
 // ---------------------------------------------------------------------------
-// batching form: many models, one graph per matrix replayed on parallel streams
+// batching form: many models in one captured graph.  Stages (a)-(c) of every matrix run
+// on parallel streams (lanes); the one-stage reductions then run as groups in lockstep
+// (one launch per kernel and column step for all matrices of a group, the groups being
+// the matrices of one element type and panel width); stages (d2)-(e) run on the lanes
+// again.  Matrices with another reduction (two-stage) run their whole pipeline on
+// their lane before the groups.
 // ---------------------------------------------------------------------------
 struct wtb_bse_batch {
   wtb_ctx* ctx = nullptr;
@@ -1684,7 +1717,82 @@ struct wtb_bse_batch {
   std::vector<int> lane_of;             // plan -> lane
   std::vector<hipEvent_t> ev_done;      // one per lane
   hipEvent_t ev_start = nullptr;
+  hipEvent_t ev_mid = nullptr;          // the groups' reductions are done
+  std::vector<TriGroup> groups;
+  std::vector<char> grouped;            // plan -> reduced in a group
+  bool stale = true;                    // groups and graph to be (re)built
+  hipGraphExec_t gexec = nullptr;
 };
+
+static void batch_drop(wtb_bse_batch* b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  for (auto& g : b->groups) tri_group_free(g);
+  b->groups.clear();
+  b->stale = true;
+}
+
+// the groups: one per (element type, panel width) among the plans with a one-stage
+// reduction
+static int batch_group(wtb_bse_batch* b) {
+  batch_drop(b);
+  const int n = (int)b->plans.size();
+  b->grouped.assign(n, 0);
+  for (int i = 0; i < n; ++i) {
+    wtb_bse_plan* p = b->plans[i];
+    if (!plan_one_stage(p)) continue;
+    const int nb = (p->tw.panel >= 8 && p->tw.panel <= p->tw.nb) ? p->tw.panel : p->tw.nb;
+    TriGroup* g = nullptr;
+    for (auto& x : b->groups)
+      if (x.f32 == p->mixed && x.nb == nb) g = &x;
+    if (!g) {
+      b->groups.emplace_back();
+      g = &b->groups.back();
+      g->f32 = p->mixed;
+      g->nb = nb;
+    }
+    g->items.push_back(TriGroupItem{&p->tw, p->mixed ? (void*)p->Hf : (void*)p->H, (int)p->ldh, p->g});
+    b->grouped[i] = 1;
+  }
+  for (auto& g : b->groups) {
+    const int rc = tri_group_prepare(g);
+    if (rc) return rc;
+  }
+  b->stale = false;
+  return WTB_OK;
+}
+
+// the set's pipeline on the context stream, forking to the lanes and joining back
+static int batch_enqueue(wtb_bse_batch* b) {
+  hipStream_t s = b->ctx->stream;
+  const int n = (int)b->plans.size();
+  auto fork = [&](hipEvent_t e) -> int {
+    WTB_HIP(hipEventRecord(e, s));
+    for (hipStream_t l : b->lanes) WTB_HIP(hipStreamWaitEvent(l, e, 0));
+    return WTB_OK;
+  };
+  auto join = [&]() -> int {
+    for (size_t l = 0; l < b->lanes.size(); ++l) {
+      WTB_HIP(hipEventRecord(b->ev_done[l], b->lanes[l]));
+      WTB_HIP(hipStreamWaitEvent(s, b->ev_done[l], 0));
+    }
+    return WTB_OK;
+  };
+  int rc;
+  if ((rc = fork(b->ev_start))) return rc;
+  for (int i = 0; i < n; ++i) {
+    wtb_bse_plan* p = b->plans[i];
+    rc = b->grouped[i] ? plan_enqueue_pre(p, false) : plan_enqueue(p);
+    if (rc) return rc;
+  }
+  if ((rc = join())) return rc;
+  for (const auto& g : b->groups)
+    if ((rc = run_tridiag_group(s, g))) return rc;
+  if ((rc = fork(b->ev_mid))) return rc;
+  for (int i = 0; i < n; ++i)
+    if (b->grouped[i] && (rc = plan_enqueue_post(b->plans[i]))) return rc;
+  return join();
+}
 
 int wtb_bse_batch_create(wtb_ctx* c, int n, const wtb_tb_model* models, const wtb_bse_params* prms, int nlanes,
                          wtb_bse_batch** out) {
@@ -1712,7 +1820,9 @@ int wtb_bse_batch_create(wtb_ctx* c, int n, const wtb_tb_model* models, const wt
     b->lanes.push_back(st);
     b->ev_done.push_back(e);
   }
-  if (!rc && hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming) != hipSuccess) rc = WTB_ERR_HIP;
+  if (!rc && (hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming) != hipSuccess))
+    rc = WTB_ERR_HIP;
   if (rc) {
     wtb_bse_batch_destroy(b);
     return rc;
@@ -1743,6 +1853,8 @@ int wtb_bse_batch_dim(const wtb_bse_batch* b, int i) {
 
 int wtb_bse_batch_tune(wtb_bse_batch* b, int* tiles) {
   if (!b) return WTB_ERR_ARG;
+  WTB_HIP(hipStreamSynchronize(b->ctx->stream));
+  batch_drop(b);                          // the panel widths, hence the groups, change
   for (size_t i = 0; i < b->plans.size(); ++i) {
     int best = 32;
     int rc = wtb_bse_plan_tune(b->plans[i], &best, nullptr);
@@ -1752,21 +1864,37 @@ int wtb_bse_batch_tune(wtb_bse_batch* b, int* tiles) {
   return WTB_OK;
 }
 
-// fork the lanes from the context stream, replay every matrix's graph on its lane (the
-// first call captures them), join back
+// the first call (and the first after a tune) forms the groups and captures the set's
+// graph; every call replays it on the context stream
 int wtb_bse_batch_execute(wtb_bse_batch* b) {
   if (!b) return WTB_ERR_ARG;
   WTB_HIP(hipSetDevice(b->ctx->device));
-  WTB_HIP(hipEventRecord(b->ev_start, b->ctx->stream));
-  for (hipStream_t l : b->lanes) WTB_HIP(hipStreamWaitEvent(l, b->ev_start, 0));
-  for (size_t i = 0; i < b->plans.size(); ++i) {
-    int rc = wtb_bse_plan_execute(b->plans[i]);
-    if (rc) return rc;
+  int rc;
+  if (b->stale && (rc = batch_group(b))) return rc;
+  if (!b->gexec) {
+    hipStream_t s = b->ctx->stream;
+    WTB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error(std::string("wtb_bse_batch_execute: hipStreamEndCapture: ") + hipGetErrorString(e));
+      return WTB_ERR_HIP;
+    }
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b->gexec = nullptr;
+      set_error(std::string("wtb_bse_batch_execute: hipGraphInstantiate: ") + hipGetErrorString(e));
+      return WTB_ERR_HIP;
+    }
   }
-  for (size_t l = 0; l < b->lanes.size(); ++l) {
-    WTB_HIP(hipEventRecord(b->ev_done[l], b->lanes[l]));
-    WTB_HIP(hipStreamWaitEvent(b->ctx->stream, b->ev_done[l], 0));
-  }
+  WTB_HIP(hipGraphLaunch(b->gexec, b->ctx->stream));
+  for (auto* p : b->plans) p->filled = true;
   return WTB_OK;
 }
 
@@ -1782,6 +1910,8 @@ int wtb_bse_batch_destroy(wtb_bse_batch* b) {
   if (!b) return WTB_OK;
   hipSetDevice(b->ctx->device);
   hipStreamSynchronize(b->ctx->stream);
+  batch_drop(b);
+  if (b->ev_mid) hipEventDestroy(b->ev_mid);
   for (auto* p : b->plans) wtb_bse_plan_destroy(p);
   for (auto l : b->lanes) {
     hipStreamSynchronize(l);

@@ -91,6 +91,24 @@ int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs);
 int run_tridiag(hipStream_t s, TridiagWork& tw, zc* d_H, int ldh, zc* d_g);
 // the same reduction on a COMPLEX*8 matrix (precision form)
 int run_tridiag_f32(hipStream_t s, TridiagWork& tw, zf* d_H, int ldh, zc* d_g);
+// A group of such reductions of one element type and panel width, run in lockstep: one
+// launch per kernel and column step serves every matrix of the group, and each matrix
+// gets bitwise the results of run_tridiag / run_tridiag_f32 on it alone.
+struct TriGroupItem {
+  TridiagWork* tw;
+  void* H;     // zc* or zf* (f32)
+  int ldh;
+  zc* g;
+};
+struct TriGroup {
+  bool f32 = false;
+  int nb = 32;                       // the panel width of every member
+  std::vector<TriGroupItem> items;
+  void* d_args = nullptr;            // the members' kernel arguments on the device
+};
+int tri_group_prepare(TriGroup& G);  // allocates d_args (the members must be bound)
+void tri_group_free(TriGroup& G);
+int run_tridiag_group(hipStream_t s, const TriGroup& G);
 
 // stage (d), two-stage form: dense -> band of width b (16 or 32) -> real tridiagonal
 // (d, e), applying Q^T to nrhs (<= 6) complex vectors g.  The matrix is in FULL Hermitian

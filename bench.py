@@ -258,18 +258,19 @@ def bench_batch(args):
             run_seq(pls)
         return (time.perf_counter() - t0) / args.steps
 
-    t_single = time_seq(plans)
-    ref = [pl.results() for pl in plans]
+    t_single = time_seq(plans) if not args.no_single else None
+    ref = [pl.results() for pl in plans] if not args.no_single else None
     for pl in plans:
         pl.close()
     t_fp64 = t_single
-    if args.precision != "fp64":
+    if args.precision != "fp64" and not args.no_single:
         # the sequential plain FP64 solves of the same models
         plans = [bse.BSEPlan(ctx, m, params(dict(CONFIG, ngrid=(ng, ng, 1)), "fp64")) for m, (ng, _) in zip(models, spec)]
         t_fp64 = time_seq(plans)
         for pl in plans:
             pl.close()
-    log(f"sequential plain FP64 plans {t_fp64 * 1e3:.1f} ms per set")
+    if t_fp64 is not None:
+        log(f"sequential plain FP64 plans {t_fp64 * 1e3:.1f} ms per set")
     with bse.BSEBatch(ctx, models, prms, lanes=args.lanes) as b:
         tiles = b.tune().tolist() if args.tune else None
         for _ in range(args.warmup):
@@ -280,10 +281,13 @@ def bench_batch(args):
             b.execute()
         ctx.sync()
         t_batch = (time.perf_counter() - t0) / args.steps
-        same = all(np.array_equal(b.results(i)["W"], ref[i]["W"]) for i in range(len(ref))) if not args.tune else None
-        dW = max(float(np.abs(b.results(i)["W"] - ref[i]["W"]).max()) for i in range(len(ref)))
+        same = all(np.array_equal(b.results(i)["W"], ref[i]["W"]) for i in range(len(ref))) if ref and not args.tune else None
+        dW = max(float(np.abs(b.results(i)["W"] - ref[i]["W"]).max()) for i in range(len(ref))) if ref else None
     ctx.close()
-    log(f"single plans {t_single * 1e3:.1f} ms per set, batch {t_batch * 1e3:.1f} ms per set")
+    log(f"single plans {t_single * 1e3:.1f} ms per set, batch {t_batch * 1e3:.1f} ms per set" if t_single else
+        f"batch {t_batch * 1e3:.1f} ms per set")
+    rnd = lambda x: round(x, 6) if x is not None else None
+    ratio = lambda x: round(x / t_batch, 3) if x is not None else None
     out = {
         "metric": "BSE build+solve of a set of uneven matrices, batch API vs sequential single plans",
         "value": round(t_batch, 6), "unit": "s/set", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -291,8 +295,8 @@ def bench_batch(args):
         "dtype": "c128" if args.precision == "fp64" else "c64/c128", "data": "synthetic (seeded random tight-binding models)",
         "config": {"workload": f"hex2d set of {len(spec)}", "models": [{"kgrid": ng, "nwann": nw, "bse_dim": N} for (ng, nw), N in zip(spec, dims)],
                    "lanes": args.lanes, "precision": args.precision, "tuned_panels": tiles},
-        "single_plans_s": round(t_single, 6), "speedup_vs_single": round(t_single / t_batch, 3),
-        "fp64_plain_sequential_s": round(t_fp64, 6), "speedup_vs_fp64_sequential": round(t_fp64 / t_batch, 3),
+        "single_plans_s": rnd(t_single), "speedup_vs_single": ratio(t_single),
+        "fp64_plain_sequential_s": rnd(t_fp64), "speedup_vs_fp64_sequential": ratio(t_fp64),
         "matrices_per_s": round(len(dims) / t_batch, 2), "bitwise_equal_to_single": same, "max_dW_eV": dW,
     }
     print(json.dumps(out), flush=True)
@@ -428,6 +432,7 @@ def main():
                          "iterative form (lowest --nev excitons, Lanczos eps) against the full FP64 solve")
     ap.add_argument("--nev", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=4, help="batching form and bands: streams")
+    ap.add_argument("--no-single", action="store_true", help="batching form: time the batch only (profiling)")
     ap.add_argument("--npts", type=int, default=4, help="bands: q-points per path segment (3 segments)")
     ap.add_argument("--tune", action="store_true", help="batching form: run the panel autotuner first")
     ap.add_argument("--refine", type=int, default=256,

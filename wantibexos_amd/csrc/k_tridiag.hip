@@ -32,6 +32,7 @@ namespace wtb {
 
 #define TB 64   // tile / row-block edge
 constexpr int kTriBatchMax = 64;   // matrices per batched launch
+constexpr int kScalLen = 2 + 32 + 32 + 16;
 #define RPW 16  // rows per rowstep workgroup
 
 template <class T>
@@ -54,6 +55,8 @@ struct TriArgs {
   zc* xn;          // N: reduced column j below its diagonal (x_new)
   zc* xp;          // N: column j after the panel update without the deferred term
   zc* gs;          // [16] row j+1 of g as hemvstep(j) saw it (rowstep(j+1) updates that row)
+  zc* scal;        // [2 + 32 + 32 + 16] reflector j-1's scalars for the batched rowstep:
+                   // tau, s, W^H v, V^H v, v^T g
   int nrhs;
   int nblk;
 };
@@ -234,7 +237,9 @@ __device__ void reflector_scalars(const TriArgs<T>& a, int jm, int jjh, zc* sums
 // ---- rowstep ------------------------------------------------------------------------
 // Loads that do not depend on reflector j-1's scalars (the GEMV partials, the panel
 // rows, column j) are issued first; the panel values serve both u and the column update.
-template <class T>
+// PRE: reflector j-1's scalars come from k_tri_scal (the batched form) instead of being
+// formed by every workgroup; both run reflector_scalars on the same inputs
+template <class T, bool PRE>
 __device__ __forceinline__ void tri_rowstep(const TriArgs<T>& a, int bx, int j, int jj, int update_col) {
   __shared__ zc st1[32], st2[32], svg[16], sWj[32], sVj[32], sums[96];
   __shared__ zc su_j, s_tau, s_s;
@@ -276,18 +281,31 @@ __device__ __forceinline__ void tri_rowstep(const TriArgs<T>& a, int bx, int j, 
       }
     }
   }
-  // reflector jm's inputs and this row's x_new and g, none of which depends on its scalars
-  zc alpha_pre = mkz(0, 0), w_pre = mkz(0, 0), v_pre = mkz(0, 0), g_pre = mkz(0, 0);
-  if (tid == 0) alpha_pre = a.xn[jm + 1];
-  if (tid < jjm) {
-    w_pre = a.W[(size_t)tid * N + j];
-    v_pre = a.Vp[(size_t)tid * N + j];
-  }
-  if (tid >= 64 && tid < 64 + a.nrhs) g_pre = a.gs[tid - 64];
   const zc xnr = (valid && r != j) ? a.xn[r] : mkz(0, 0);
   const zc gr = (valid && sub < a.nrhs) ? a.g[(size_t)sub * N + r] : mkz(0, 0);
-  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, bx == 0, alpha_pre, w_pre, v_pre,
-                    g_pre);
+  if constexpr (PRE) {
+    if (tid == 0) {
+      s_tau = a.scal[0];
+      s_s = a.scal[1];
+    }
+    if (tid < jjm) {
+      st1[tid] = a.scal[2 + tid];
+      st2[tid] = a.scal[34 + tid];
+    }
+    if (tid >= 64 && tid < 64 + a.nrhs) svg[tid - 64] = a.scal[66 + tid - 64];
+    __syncthreads();
+  } else {
+    // reflector jm's inputs, none of which depends on its scalars
+    zc alpha_pre = mkz(0, 0), w_pre = mkz(0, 0), v_pre = mkz(0, 0), g_pre = mkz(0, 0);
+    if (tid == 0) alpha_pre = a.xn[jm + 1];
+    if (tid < jjm) {
+      w_pre = a.W[(size_t)tid * N + j];
+      v_pre = a.Vp[(size_t)tid * N + j];
+    }
+    if (tid >= 64 && tid < 64 + a.nrhs) g_pre = a.gs[tid - 64];
+    reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, bx == 0, alpha_pre, w_pre, v_pre,
+                      g_pre);
+  }
   const zc tau = s_tau, s = s_s;
   // s z(row) - V t1 - W t2, each row summed by its 16 lanes
   auto u_of = [&](zc ys, const zc* v2, const zc* w2) -> zc {
@@ -369,7 +387,34 @@ __device__ __forceinline__ void tri_rowstep(const TriArgs<T>& a, int bx, int j, 
 }
 template <class T>
 __global__ __launch_bounds__(256) void k_tri_rowstep(TriArgs<T> a, int j, int jj, int update_col) {
-  tri_rowstep(a, blockIdx.x, j, jj, update_col);
+  tri_rowstep<T, false>(a, blockIdx.x, j, jj, update_col);
+}
+
+// reflector j-1's scalars once per matrix (the batched form's rowstep reads them)
+template <class T>
+__device__ __forceinline__ void tri_scal(const TriArgs<T>& a, int j, int jj) {
+  __shared__ zc st1[32], st2[32], svg[16], sums[96];
+  __shared__ zc s_tau, s_s;
+  const int tid = threadIdx.x;
+  const int N = a.N;
+  const int jm = j - 1, jjm = jj - 1;
+  zc alpha_pre = mkz(0, 0), w_pre = mkz(0, 0), v_pre = mkz(0, 0), g_pre = mkz(0, 0);
+  if (tid == 0) alpha_pre = a.xn[jm + 1];
+  if (tid < jjm) {
+    w_pre = a.W[(size_t)tid * N + j];
+    v_pre = a.Vp[(size_t)tid * N + j];
+  }
+  if (tid >= 64 && tid < 64 + a.nrhs) g_pre = a.gs[tid - 64];
+  reflector_scalars(a, jm, jjm, sums, st1, st2, svg, &s_tau, &s_s, true, alpha_pre, w_pre, v_pre, g_pre);
+  if (tid == 0) {
+    a.scal[0] = s_tau;
+    a.scal[1] = s_s;
+  }
+  if (tid < jjm) {
+    a.scal[2 + tid] = st1[tid];
+    a.scal[34 + tid] = st2[tid];
+  }
+  if (tid >= 64 && tid < 64 + a.nrhs) a.scal[66 + tid - 64] = svg[tid - 64];
 }
 
 // ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
@@ -775,10 +820,16 @@ __device__ inline int tri_slot(const int* pre, int n, int b) {
 }
 
 template <class T>
+__global__ __launch_bounds__(256) void k_tri_scal_b(TriBatch<T> B, int j, int jj) {
+  const int i = tri_slot(B.pre, B.n, blockIdx.x);
+  const TriArgs<T> a = B.args[B.m[i]];
+  tri_scal(a, j, jj);
+}
+template <class T>
 __global__ __launch_bounds__(256) void k_tri_rowstep_b(TriBatch<T> B, int j, int jj, int update_col) {
   const int i = tri_slot(B.pre, B.n, blockIdx.x);
   const TriArgs<T> a = B.args[B.m[i]];
-  tri_rowstep(a, (int)blockIdx.x - B.pre[i], j, jj, update_col);
+  tri_rowstep<T, true>(a, (int)blockIdx.x - B.pre[i], j, jj, update_col);
 }
 template <class T>
 __global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv_b(TriBatch<T> B, int j, int jj) {
@@ -828,6 +879,7 @@ size_t tridiag_work_bytes(int N, int nb, int nrhs) {
   b += sizeof(zc) * nblk * nblk * TB + 256;     // ypart
   b += sizeof(zc) * N * 2 + 512;                // xn, xp
   b += sizeof(zc) * 16 + 256;                   // gs
+  b += sizeof(zc) * kScalLen + 256;             // scal
   return b + 4096;
 }
 
@@ -855,6 +907,7 @@ int tridiag_bind(TridiagWork& tw, void* base, int N, int nb, int nrhs) {
   tw.xn = (zc*)take(sizeof(zc) * N);
   tw.xp = (zc*)take(sizeof(zc) * N);
   tw.gs = (zc*)take(sizeof(zc) * 16);
+  tw.scal = (zc*)take(sizeof(zc) * kScalLen);
   return WTB_OK;
 }
 
@@ -888,6 +941,7 @@ static TriArgs<T> tri_args(const TridiagWork& tw, T* d_H, int ldh, zc* d_g, int 
   a.xn = tw.xn;
   a.xp = tw.xp;
   a.gs = tw.gs;
+  a.scal = tw.scal;
   a.nrhs = tw.nrhs;
   return a;
 }
@@ -1000,6 +1054,8 @@ static int run_tridiag_group_t(hipStream_t s, const TriGroup& G) {
       }
       if (ms.empty()) break;
       if (jj > 0) {
+        counts(ms, [&](int) { return 1; });
+        launch_group<T>(s, k_tri_scal_b<T>, dargs, ms, nbk, 256, j, jj);
         counts(ms, [&](int i) { return (N[i] - j + RPW - 1) / RPW; });
         launch_group<T>(s, k_tri_rowstep_b<T>, dargs, ms, nbk, 256, j, jj, 1);
       }
@@ -1011,6 +1067,8 @@ static int run_tridiag_group_t(hipStream_t s, const TriGroup& G) {
       if (fs.empty()) continue;
       // the panel ends at column j for these matrices
       const int kp = jj + 1, jl = j, jn = j + 1, s0 = p + kp;
+      counts(fs, [&](int) { return 1; });
+      launch_group<T>(s, k_tri_scal_b<T>, dargs, fs, nbk, 256, jn, kp);
       counts(fs, [&](int i) { return (N[i] - jn + RPW - 1) / RPW; });
       launch_group<T>(s, k_tri_rowstep_b<T>, dargs, fs, nbk, 256, jn, kp, 0);
       counts(fs, [&](int i) { return nblk[i] - (jl + 1) / TB; });

@@ -75,6 +75,7 @@ struct TridiagWork {
   zc* xp = nullptr;       // panel-updated column before the deferred term
   zc* puv = nullptr;      // per-workgroup u^H v partials
   zc* gs = nullptr;       // saved row of the tracked vectors
+  zc* scal = nullptr;     // one reflector's scalars (batched form)
   // optional per-launch timing of the Hermitian GEMV kernel: when kev is set,
   // launch c is bracketed by events kev[2c], kev[2c+1] (c < kev_cap)
   hipEvent_t* kev = nullptr;

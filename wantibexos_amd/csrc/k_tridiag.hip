@@ -417,13 +417,17 @@ __device__ __forceinline__ void tri_scal(const TriArgs<T>& a, int j, int jj) {
   if (tid >= 64 && tid < 64 + a.nrhs) a.scal[66 + tid - 64] = svg[tid - 64];
 }
 
-// ---- hemvstep: one 64 x 64 tile per workgroup ----------------------------------------
+
+// ---- hemvstep: 64 x 64 tiles ------------------------------------------------------------
 // 4 waves, wave w owns 16 columns, lane = row: every load moves one column segment.
-// waves per SIMD that the registers allow without spilling: 4 for the FP64 instance (127
-// registers), 6 for the COMPLEX*8 one (80: FP32 tile sums, 8-quantity epilogue groups);
-// more waves keep more tiles' loads in flight
+// FP64 matrix: one tile per workgroup.  COMPLEX*8 matrix: the diagonal tiles one per
+// workgroup, then the off-diagonal ones two per workgroup, both tiles' loads in flight
+// together (a COMPLEX*8 tile alone is too little data to cover a workgroup's fixed
+// costs); each tile's sums are formed exactly as for one tile.
+// Waves per SIMD that the registers allow without spilling: 4 (127 registers for FP64,
+// two tiles' loads for COMPLEX*8).
 template <class T>
-__device__ __forceinline__ void tri_hemv(const TriArgs<T>& a, int bx, int j, int jj, int nprev) {
+__device__ __forceinline__ void tri_hemv_tile(const TriArgs<T>& a, int ti, int tj, int j, int jj, int nprev) {
   constexpr int XRN = TB;                   // rows of x~ a workgroup needs
   constexpr int XT = XRN + TB;              // threads loading x~ (rows, then columns)
   __shared__ zc xR[XRN], xC[TB];
@@ -433,11 +437,6 @@ __device__ __forceinline__ void tri_hemv(const TriArgs<T>& a, int bx, int j, int
   __shared__ zc s_ap;
   const int N = a.N;
   const int b0 = (j + 1) / TB;
-  const int t = bx;
-  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-  while (ti * (ti + 1) / 2 > t) --ti;
-  const int tj = t - ti * (ti + 1) / 2;
   const int bi = b0 + ti, bj = b0 + tj;
   const int tid = threadIdx.x;
   const int R0 = bi * TB, C0 = bj * TB;
@@ -646,8 +645,160 @@ __device__ __forceinline__ void tri_hemv(const TriArgs<T>& a, int bx, int j, int
     }
   }
 }
+// lower-triangle tile index t -> (ti, tj), tj <= ti
+__device__ inline void tri_tile(int t, int& ti, int& tj) {
+  ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  tj = t - ti * (ti + 1) / 2;
+}
+
+// two off-diagonal tiles o0 = 2 pb, o1 = o0 + 1 (strictly-lower index o -> ti >= 1,
+// tj < ti) of the COMPLEX*8 matrix: the off-diagonal branch of tri_hemv_tile for both
 template <class T>
-__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
+__device__ __forceinline__ void tri_hemv_pair(const TriArgs<T>& a, int pb, int noff, int j, int jj, int nprev) {
+  __shared__ zc xR[2][TB];
+  __shared__ zf xCf[2][TB];
+  __shared__ zc y1s[2][4][TB], y2s[2][TB];
+  __shared__ zc s_ap;
+  const int N = a.N;
+  const int b0 = (j + 1) / TB;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const bool two = 2 * pb + 1 < noff;
+  int bi[2], bj[2];
+  // strictly-lower index o: o = ti (ti - 1) / 2 + tj with 0 <= tj < ti
+  for (int u = 0; u < 2; ++u) {
+    const int o = min(2 * pb + u, noff - 1);
+    int ti = (int)((sqrt(8.0 * o + 1.0) + 1.0) * 0.5);
+    while (ti * (ti - 1) / 2 > o) --ti;
+    while ((ti + 1) * ti / 2 <= o) ++ti;
+    bi[u] = b0 + ti;
+    bj[u] = b0 + (o - ti * (ti - 1) / 2);
+  }
+  T hraw[2][16];
+  unsigned okm[2] = {0xffffu, 0xffffu};
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int R0 = bi[u] * TB, C0 = bj[u] * TB;
+    const int r = R0 + lane;
+    const bool interior = C0 > j && R0 + TB <= N && C0 + TB <= N;
+    if (interior) {
+      const T* col = a.H + (size_t)r + (size_t)a.ld * (C0 + 16 * w);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) hraw[u][k] = col[(size_t)a.ld * k];
+    } else {
+      const bool rvalid = r > j && r < N;
+      okm[u] = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int c = C0 + 16 * w + k;
+        const bool ok = rvalid && c > j && c < N;
+        const size_t off = ok ? (size_t)r + (size_t)a.ld * c : 0;
+        hraw[u][k] = a.H[off];
+        okm[u] |= (unsigned)ok << k;
+      }
+    }
+  }
+  // x~ of both tiles' rows and columns: thread tid loads row/column (tid & 127) of tile
+  // tid >> 7
+  const int ux = tid >> 7, lx = tid & 127;
+  const int xrow = (lx < TB) ? bi[ux] * TB + lx : bj[ux] * TB + lx - TB;
+  const bool xok = xrow >= j + 1 && xrow < N;
+  zc x0 = mkz(0, 0), x1 = mkz(0, 0);
+  if (xok) {
+    if (jj == 0) {
+      x0 = ldz(a.H[(size_t)xrow + (size_t)a.ld * j]);
+    } else {
+      x0 = a.xp[xrow];
+      x1 = a.Vp[(size_t)(jj - 1) * N + xrow];
+    }
+  }
+  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
+  if (jj > 0) {
+    const zc al = alpha_from_puv(a.puv, nprev, taup);
+    if (tid == 0) s_ap = al;
+  } else if (tid == 0) {
+    s_ap = mkz(0, 0);
+  }
+  __syncthreads();
+  const zc ap = s_ap;
+  const double sig = 2.0 * ap.re;
+  const zc xv = x0 - sig * x1;
+  {
+    const zc xt = (xok && xrow >= j + 2) ? xv : mkz(0, 0);
+    if (lx < TB) xR[ux][lx] = xt;
+    else stz(xCf[ux][lx - TB], xt);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const zc xr = xR[u][lane];
+    zf hf[16];
+    zf xrf, y1f;
+    xrf.re = (float)xr.re;
+    xrf.im = (float)xr.im;
+    y1f.re = 0.0f;
+    y1f.im = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int cl = 16 * w + k;
+      const bool on = (okm[u] >> k) & 1u;
+      const zf h = hraw[u][k];
+      zf x;
+      x.re = on ? h.re : 0.0f;
+      x.im = on ? h.im : 0.0f;
+      const zf xc = xCf[u][cl];
+      y1f.re = fmaf(x.re, xc.re, fmaf(-x.im, xc.im, y1f.re));
+      y1f.im = fmaf(x.re, xc.im, fmaf(x.im, xc.re, y1f.im));
+      zf t;
+      t.re = h.re * xrf.re + h.im * xrf.im;               // conj(h) x~(row)
+      t.im = h.re * xrf.im - h.im * xrf.re;
+      hf[k].re = on ? t.re : 0.0f;
+      hf[k].im = on ? t.im : 0.0f;
+    }
+    y1s[u][w][lane] = ldz(y1f);
+    BUTTERFLY16F(hf, lane);
+    if ((lane & 3) == 0) y2s[u][16 * w + ((lane >> 2) & 15)] = ldz(hf[0]);
+  }
+  __syncthreads();
+  // thread tid: tile tid >> 7; rows (A x~) for (tid & 127) < 64, columns (A^H x~) above
+  const int u = tid >> 7, i = tid & 63;
+  if (u == 1 && !two) return;
+  zc* yp = a.ypart;
+  if ((tid & 64) == 0) {
+    const zc sm = y1s[u][0][i] + y1s[u][1][i] + y1s[u][2][i] + y1s[u][3][i];
+    yp[((size_t)bi[u] * a.nblk + bj[u]) * TB + i] = sm;
+  } else {
+    yp[((size_t)bj[u] * a.nblk + bi[u]) * TB + i] = y2s[u][i];
+  }
+}
+
+// workgroups of the GEMV at column j (nt = row blocks of the trailing matrix)
+template <class T>
+__host__ __device__ inline int hemv_blocks(int nt) {
+  if (sizeof(T) == sizeof(zc)) return nt * (nt + 1) / 2;
+  const int noff = nt * (nt - 1) / 2;
+  return nt + (noff + 1) / 2;
+}
+
+template <class T>
+__device__ __forceinline__ void tri_hemv(const TriArgs<T>& a, int bx, int j, int jj, int nprev) {
+  if constexpr (std::is_same<T, zf>::value) {
+    const int nt = a.nblk - (j + 1) / TB;
+    if (bx < nt) {
+      tri_hemv_tile(a, bx, bx, j, jj, nprev);
+    } else {
+      tri_hemv_pair(a, bx - nt, nt * (nt - 1) / 2, j, jj, nprev);
+    }
+  } else {
+    int ti, tj;
+    tri_tile(bx, ti, tj);
+    tri_hemv_tile(a, ti, tj, j, jj, nprev);
+  }
+}
+template <class T>
+__global__ __launch_bounds__(256, 4) void k_tri_hemv(TriArgs<T> a, int j, int jj, int nprev) {
   tri_hemv(a, blockIdx.x, j, jj, nprev);
 }
 
@@ -832,7 +983,7 @@ __global__ __launch_bounds__(256) void k_tri_rowstep_b(TriBatch<T> B, int j, int
   tri_rowstep<T, true>(a, (int)blockIdx.x - B.pre[i], j, jj, update_col);
 }
 template <class T>
-__global__ __launch_bounds__(256, sizeof(T) == 16 ? 4 : 6) void k_tri_hemv_b(TriBatch<T> B, int j, int jj) {
+__global__ __launch_bounds__(256, 4) void k_tri_hemv_b(TriBatch<T> B, int j, int jj) {
   const int i = tri_slot(B.pre, B.n, blockIdx.x);
   const TriArgs<T> a = B.args[B.m[i]];
   const int nprev = jj > 0 ? (a.N - j + RPW - 1) / RPW : 0;
@@ -970,7 +1121,7 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
       const int nt = a.nblk - (j + 1) / TB;
       const bool tev = tw.kev && tw.kev_used < tw.kev_cap;
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used], s);
-      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, a, j, jj, nprev);
+      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(hemv_blocks<T>(nt)), dim3(256), 0, s, a, j, jj, nprev);
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used++ + 1], s);
     }
     // finish the panel's last reflector (no further column in this panel)
@@ -1061,7 +1212,7 @@ static int run_tridiag_group_t(hipStream_t s, const TriGroup& G) {
       }
       counts(ms, [&](int i) {
         const int nt = nblk[i] - (j + 1) / TB;
-        return nt * (nt + 1) / 2;
+        return hemv_blocks<T>(nt);
       });
       launch_group<T>(s, k_tri_hemv_b<T>, dargs, ms, nbk, 256, j, jj);
       if (fs.empty()) continue;

@@ -129,7 +129,7 @@ int small_heev(hipStream_t s, SmallEig& e, int m, const zc* G, int ldg, double* 
   int rc = dc_prepare(e.dcw, m, 0, tree, 1);
   if (rc) return rc;
   if ((rc = run_tridiag(s, tw, e.A, m, nullptr))) return rc;
-  if ((rc = run_dc(s, e.dcw, tree, tw.d, tw.e, nullptr, theta, e.Z))) return rc;
+  if ((rc = run_dc(s, e.dcw, tree, tw.d, tw.e, nullptr, theta, e.Z, k))) return rc;
   return run_backtr(s, e.A, m, m, tw.tau, e.Z, 0, k, S, lds, e.btw);
 }
 

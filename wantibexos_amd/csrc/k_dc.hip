@@ -71,6 +71,7 @@ struct DCArgs {
   double* rotc;
   double* rots;
   int* nrot;            // [nnodes]
+  int zcols;            // eigenvector columns formed at this level: sorted positions < zcols
 };
 
 __device__ __constant__ double kEps = 1.1102230246251565e-16;   // DLAMCH('E')
@@ -661,6 +662,9 @@ __global__ __launch_bounds__(256) void k_dcv_gemm(DCArgs a) {
   const int n = nd.hi - nd.lo, K = a.K[node], lo = nd.lo;
   const int r0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
   if (r0 >= n || j0 >= K) return;
+  // the secular roots' sorted positions increase with the slot: a tile whose first
+  // column lies past the wanted ones has no wanted column
+  if (a.rank[lo + j0] >= a.zcols) return;
   const size_t N = a.N;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -703,6 +707,7 @@ __global__ __launch_bounds__(256) void k_dcv_copy(DCArgs a) {
   const DCNode nd = a.nodes[node];
   const int n = nd.hi - nd.lo, j = blockIdx.x;
   if (j >= n || j < a.K[node]) return;
+  if (a.rank[nd.lo + j] >= a.zcols) return;
   const size_t N = a.N;
   const double* src = a.Z2 + (size_t)(nd.lo + a.cmap[nd.lo + j]) * N;
   double* dst = a.Z + (size_t)(nd.lo + a.rank[nd.lo + j]) * N;
@@ -834,7 +839,7 @@ int dc_prepare(void* work, int N, int ntrack, DCTree& tree, int vectors) {
 }
 
 int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, const double* d_e,
-           double* d_track, double* d_lambda, double* d_Z) {
+           double* d_track, double* d_lambda, double* d_Z, int zcols) {
   const int N = tree.N;
   if (tree.ntrack > 12) {
     set_error("divide and conquer: at most 12 tracked vectors");
@@ -858,6 +863,8 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
     const int cnt = tree.level_cnt[l], maxn = tree.level_maxn[l];
     a.nodes = lay.dnodes + tree.level_off[l];
     a.nnodes = cnt;
+    // below the root every column feeds the parent
+    a.zcols = (l + 1 == tree.level_off.size() && zcols >= 0) ? zcols : N;
     dim3 g1((maxn + 255) / 256, cnt);
     dim3 g16((maxn * 16 + 255) / 256, cnt);
     dim3 g64((maxn * 64 + 255) / 256, cnt);

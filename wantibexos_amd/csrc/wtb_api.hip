@@ -705,7 +705,7 @@ static int plan_enqueue_q(wtb_bse_plan* p, bool build_only) {
   if ((rc = plan_reduce(p, nullptr, p->status + 1))) return rc;
   mark(4);
   if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, nullptr, p->lambda,
-                   p->wf_count > 0 ? p->Zv : nullptr)))
+                   p->wf_count > 0 ? p->Zv : nullptr, p->wf_first + p->wf_count)))
     return rc;
   // BSE_WF along the path (bse_kpath-tool.f90:359-366, 402-410): the exciton vectors
   if (p->wf_count > 0) {
@@ -792,8 +792,10 @@ static int plan_enqueue_post(wtb_bse_plan* p) {
   mark(4);
   // tridiagonal divide and conquer: eigenvalues and Z^T (Q^T h)
   hipLaunchKernelGGL(k_split_g, dim3((N + 255) / 256), dim3(256), 0, s, N, p->nrhs, p->g, p->track);
+  // only the eigenvector columns the refinement and BSE_WF read are formed
+  const int zcols = std::max(refine ? p->nref : 0, p->wf_count > 0 ? p->wf_first + p->wf_count : 0);
   if ((rc = run_dc(s, p->dcwork, p->tree, p->tw.d, p->tw.e, p->track, p->lambda,
-                   (p->wf_count > 0 || refine) ? p->Zv : nullptr)))
+                   (p->wf_count > 0 || refine) ? p->Zv : nullptr, zcols)))
     return rc;
   if (refine) {
     // lowest nref exciton vectors x = Q z (FP32 reflectors, FP64 arithmetic), H x with the

@@ -151,9 +151,10 @@ size_t dc_work_bytes(int N, int ntrack, int vectors = 0);
 int dc_prepare(void* work, int N, int ntrack, DCTree& tree, int vectors = 0);
 // d_track: ntrack x N, in: the tridiagonal-basis vectors, out: Z^T (.)
 // d_Z (optional, N x N column-major): the eigenvectors, column j for d_lambda[j];
-// needs a tree prepared with vectors = 1
+// needs a tree prepared with vectors = 1.  zcols >= 0: only columns j < zcols are
+// formed (the others of d_Z are left undefined)
 int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, const double* d_e,
-           double* d_track, double* d_lambda, double* d_Z = nullptr);
+           double* d_track, double* d_lambda, double* d_Z = nullptr, int zcols = -1);
 
 // stage (d3): X(:, 0:m) = Q Z(:, c0:c0+m) from the stored reflectors (store_v) and tau;
 // the last component of each column is made real and >= 0

@@ -133,21 +133,50 @@ __global__ void k_dc_gather(DCArgs a) {
   if (a.Z) a.perm[dst] = src;
 }
 
-// M2: deflation scan (DLAED2), one workgroup per node.  The poles and weights
-// are staged in LDS (when they fit) for the sequential scan of thread 0, which
-// decides each sorted position's slot (non-deflated first), applies the Givens
-// rotations of close-pole deflation to the tracked sources in place, and writes
-// the poles and weights of the secular equation.
+// exclusive prefix count of f over the workgroup's 256 threads (every thread calls it);
+// *total receives the count over all of them
+__device__ inline int block_prefix256(bool f, int* wsum, int* total) {
+  const unsigned long long b = __ballot(f);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int inw = __popcll(b & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[wv] = __popcll(b);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wv) off += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inw;
+}
+
+// M2: deflation (DLAED2), one workgroup of 256 per node.  The decisions and values are
+// those of the sequential scan over the sorted positions (small weights deflate; a
+// close pair (pj, nj) of kept candidates is rotated so that pj deflates, and the rotated
+// nj becomes the next pair's pj), found in three parts:
+//   - the small-weight test of every position and the close-pole test of every pair of
+//     consecutive candidates, on the original values, in parallel;
+//   - one wave walks the pairs in order, 64 at a time by ballot while no rotation is due,
+//     and applies the rotations one by one, testing the pair after a rotation with the
+//     rotated values (the only values a test can see changed);
+//   - the slots from prefix counts: kept positions first, in order; deflated ones from the
+//     end, in the order of deflation (a small weight at its own position, a rotated pj at
+//     its nj's).
+// The Givens rotations then act on the tracked sources in their order.
 __global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
   extern __shared__ double lds_dz[];
   __shared__ double red[2][256];
+  __shared__ int wsum[4];
+  __shared__ int s_nr;
   const int nodeid = blockIdx.x;
   const DCNode nd = a.nodes[nodeid];
   const int lo = nd.lo, n = nd.hi - nd.lo;
   const int N = a.N, nx = 2 + a.nt;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const double rho = fabs(2.0 * a.e[nd.mid - 1]);
   double zm = 0.0, dm = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = tid; i < n; i += 256) {
     double zi = a.sz[lo + i], di = a.sD[lo + i];
     zm = fmax(zm, fabs(zi));
     dm = fmax(dm, fabs(di));
@@ -156,95 +185,149 @@ __global__ __launch_bounds__(256) void k_dc_deflate(DCArgs a, int use_lds) {
       lds_dz[n + i] = zi;
     }
   }
-  red[0][threadIdx.x] = zm;
-  red[1][threadIdx.x] = dm;
+  red[0][tid] = zm;
+  red[1][tid] = dm;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + st]);
-      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+    if (tid < st) {
+      red[0][tid] = fmax(red[0][tid], red[0][tid + st]);
+      red[1][tid] = fmax(red[1][tid], red[1][tid + st]);
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
   double* D = use_lds ? lds_dz : a.sD + lo;
   double* Z = use_lds ? lds_dz + n : a.sz + lo;
   const double zmax = red[0][0], dmax = red[1][0];
   const double tol = 8.0 * kEps * fmax(dmax, zmax);
-  a.rho[nodeid] = rho;
-  int K = 0, nd_ = 0, nr = 0;
-  auto deflate = [&](int i) {
-    int s = n - 1 - nd_;
-    ++nd_;
-    a.slot[lo + i] = s;
-    a.cD[lo + s] = D[i];
-  };
-  auto keep = [&](int i) {
-    int s = K;
-    ++K;
-    a.slot[lo + i] = s;
-    a.cD[lo + s] = D[i];
-    a.cw[lo + s] = Z[i];
-  };
+  if (tid == 0) a.rho[nodeid] = rho;
   if (rho * zmax <= tol) {
-    for (int i = 0; i < n; ++i) deflate(i);
-    a.K[nodeid] = 0;
-    if (a.Z) a.nrot[nodeid] = 0;
+    // every position deflates, in order
+    for (int i = tid; i < n; i += 256) {
+      const int sl = n - 1 - i;
+      a.slot[lo + i] = sl;
+      a.cD[lo + sl] = D[i];
+    }
+    if (tid == 0) {
+      a.K[nodeid] = 0;
+      a.nrot[nodeid] = 0;
+    }
     return;
   }
-  int j = 0, pj = -1;
-  for (; j < n; ++j) {
-    if (rho * fabs(Z[j]) <= tol) deflate(j);
-    else {
-      pj = j;
-      ++j;
-      break;
-    }
+  // flags per position: bit 0 small weight, bit 1 the pair ending here passes the
+  // close-pole test on original values, bit 2 nj of a rotation, bit 3 pj of a rotation,
+  // bits 4.. the pj of the rotation at this nj.  P: the candidates (not small), in order.
+  int* flag = a.org + lo;
+  int* P = reinterpret_cast<int*>(a.tau + lo);
+  int m = 0;
+  for (int c0 = 0; c0 < n; c0 += 256) {
+    const int i = c0 + tid;
+    const bool small = i < n && rho * fabs(Z[i]) <= tol;
+    if (i < n) flag[i] = small ? 1 : 0;
+    int tot;
+    const int pos = block_prefix256(i < n && !small, wsum, &tot);
+    if (i < n && !small) P[m + pos] = i;
+    m += tot;
   }
-  if (pj >= 0) {
-    for (; j < n; ++j) {
-      const int nj = j;
-      const double znj = Z[nj];
-      if (rho * fabs(znj) <= tol) {
-        deflate(nj);
-        continue;
+  __syncthreads();
+  for (int k = 1 + tid; k < m; k += 256) {
+    const int pj = P[k - 1], nj = P[k];
+    double s = Z[pj], c = Z[nj];
+    const double tau = hypot(c, s);
+    const double t = D[nj] - D[pj];
+    c = c / tau;
+    s = -s / tau;
+    if (fabs(t * c * s) <= tol) flag[nj] |= 2;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    int k = 1, nr = 0;
+    bool kmod = false;          // P[k-1] carries a rotation's values
+    double dmod = 0.0, zmod = 0.0;
+    while (k < m) {
+      if (!kmod) {
+        const int kk = k + lane;
+        const unsigned long long b = __ballot(kk < m && (flag[P[kk < m ? kk : 0]] & 2));
+        if (b == 0ull) {
+          k += 64;
+          continue;
+        }
+        k += __ffsll((long long)b) - 1;
       }
-      double s = Z[pj], c = znj;
+      const int pj = P[k - 1], nj = P[k];
+      const double dp = kmod ? dmod : D[pj], zp = kmod ? zmod : Z[pj];
+      const double dn = D[nj];
+      double s = zp, c = Z[nj];
       const double tau = hypot(c, s);
-      const double t = D[nj] - D[pj];
+      const double t = dn - dp;
       c = c / tau;
       s = -s / tau;
       if (fabs(t * c * s) <= tol) {
-        Z[nj] = tau;
-        Z[pj] = 0.0;
-        for (int q = 0; q < nx; ++q) {
-          double* x = a.sx + (size_t)q * N + lo;
-          double xp = x[pj], xn = x[nj];
-          x[pj] = c * xp + s * xn;
-          x[nj] = c * xn - s * xp;
-        }
-        if (a.Z) {
+        const double t2 = dp * c * c + dn * s * s;
+        const double dnew = dp * s * s + dn * c * c;
+        if (lane == 0) {
+          Z[nj] = tau;
+          Z[pj] = 0.0;
+          D[nj] = dnew;
+          D[pj] = t2;
+          flag[nj] |= 4 | (pj << 4);
+          flag[pj] |= 8;
           a.rotp[lo + nr] = pj;
           a.rotn[lo + nr] = nj;
           a.rotc[lo + nr] = c;
           a.rots[lo + nr] = s;
-          ++nr;
         }
-        const double dp = D[pj], dn = D[nj];
-        const double t2 = dp * c * c + dn * s * s;
-        D[nj] = dp * s * s + dn * c * c;
-        D[pj] = t2;
-        deflate(pj);
-        pj = nj;
+        ++nr;
+        kmod = true;
+        dmod = dnew;
+        zmod = tau;
       } else {
-        keep(pj);
-        pj = nj;
+        kmod = false;
       }
+      ++k;
     }
-    keep(pj);
+    if (lane == 0) s_nr = nr;
   }
-  a.K[nodeid] = K;
-  if (a.Z) a.nrot[nodeid] = nr;
+  __syncthreads();
+  const int nr = s_nr;
+  int K = 0, E = 0;
+  for (int c0 = 0; c0 < n; c0 += 256) {
+    const int i = c0 + tid;
+    const int fl = i < n ? flag[i] : 0;
+    const bool kept = i < n && (fl & 9) == 0;
+    const bool ev = i < n && (fl & 5) != 0;
+    int tk, te;
+    const int pk = block_prefix256(kept, wsum, &tk);
+    const int pe = block_prefix256(ev, wsum, &te);
+    if (kept) {
+      const int sl = K + pk;
+      a.slot[lo + i] = sl;
+      a.cD[lo + sl] = D[i];
+      a.cw[lo + sl] = Z[i];
+    }
+    if (ev) {
+      const int e = (fl & 1) ? i : (fl >> 4);
+      const int sl = n - 1 - (E + pe);
+      a.slot[lo + e] = sl;
+      a.cD[lo + sl] = D[e];
+    }
+    K += tk;
+    E += te;
+  }
+  if (tid == 0) {
+    a.K[nodeid] = K;
+    a.nrot[nodeid] = nr;
+  }
+  // the rotations on the tracked sources, in order (a source per thread)
+  if (nr > 0 && tid < nx) {
+    double* x = a.sx + (size_t)tid * N + lo;
+    for (int r = 0; r < nr; ++r) {
+      const int pj = a.rotp[lo + r], nj = a.rotn[lo + r];
+      const double c = a.rotc[lo + r], s = a.rots[lo + r];
+      const double xp = x[pj], xn = x[nj];
+      x[pj] = c * xp + s * xn;
+      x[nj] = c * xn - s * xp;
+    }
+  }
 }
 
 // M2b: move the tracked sources to their slots
@@ -742,10 +825,11 @@ size_t dc_work_bytes(int N, int ntrack, int vectors) {
   size_t b = 0;
   if (vectors) {
     b += sizeof(double) * (size_t)N * N * 2;   // Z2 U
-    b += sizeof(int) * (size_t)N * 5;          // perm cmap rank rotp rotn
-    b += sizeof(double) * (size_t)N * 2;       // rotc rots
-    b += sizeof(int) * (N / 2 + 2) + 8 * 256;  // nrot
+    b += sizeof(int) * (size_t)N * 3;          // perm cmap rank
   }
+  b += sizeof(int) * (size_t)N * 2;            // rotp rotn
+  b += sizeof(double) * (size_t)N * 2;         // rotc rots
+  b += sizeof(int) * (N / 2 + 2) + 8 * 256;    // nrot
   b += sizeof(double) * N * 3;             // lam FR LR
   b += sizeof(double) * N * 2;             // sD sz
   b += sizeof(double) * N * nx * 3;        // sx cx nx
@@ -792,19 +876,19 @@ static DCLayout dc_layout(void* work, int N, int ntrack, int vectors) {
   L.dnodes = (DCNode*)take(sizeof(DCNode) * (N + 2));
   a.Z = nullptr;
   a.Z2 = a.U = nullptr;
-  a.perm = a.cmap = a.rank = a.rotp = a.rotn = a.nrot = nullptr;
-  a.rotc = a.rots = nullptr;
+  a.perm = a.cmap = a.rank = nullptr;
+  // the deflation's rotations (sources of both modes, eigenvector columns in vector mode)
+  a.rotp = (int*)take(sizeof(int) * N);
+  a.rotn = (int*)take(sizeof(int) * N);
+  a.rotc = (double*)take(sizeof(double) * N);
+  a.rots = (double*)take(sizeof(double) * N);
+  a.nrot = (int*)take(sizeof(int) * (N / 2 + 2));
   if (vectors) {
     a.Z2 = (double*)take(sizeof(double) * (size_t)N * N);
     a.U = (double*)take(sizeof(double) * (size_t)N * N);
     a.perm = (int*)take(sizeof(int) * N);
     a.cmap = (int*)take(sizeof(int) * N);
     a.rank = (int*)take(sizeof(int) * N);
-    a.rotp = (int*)take(sizeof(int) * N);
-    a.rotn = (int*)take(sizeof(int) * N);
-    a.rotc = (double*)take(sizeof(double) * N);
-    a.rots = (double*)take(sizeof(double) * N);
-    a.nrot = (int*)take(sizeof(int) * (N / 2 + 2));
   }
   return L;
 }

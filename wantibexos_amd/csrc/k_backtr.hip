@@ -128,27 +128,24 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
   }
 }
 
-// Y(:, col) = T sum_chunks Ypart(:, col) for 64 columns per workgroup: the chunk sums go
-// to LDS (8 per thread, chunks added in order), then thread (col, 8 rows) applies T.
+// Y(:, col) = T sum_chunks Ypart(:, col) for 8 columns per workgroup: thread (c, col)
+// adds its chunk sums in order into LDS, then thread (r, col) applies row r of T
 __global__ __launch_bounds__(256) void k_bt_ty(const zc* __restrict__ Ypart, int nchunk, int m, int kp,
                                                const zc* __restrict__ Tf, zc* __restrict__ Y) {
   __shared__ zc Ts[BT_NB * BT_NB];
-  __shared__ zc Ss[BT_NB][64];
+  __shared__ zc Ss[BT_NB][8];
   const int tid = threadIdx.x;
   for (int e = tid; e < BT_NB * BT_NB; e += 256) Ts[e] = Tf[e];
-  const int cl = tid & 63, col = blockIdx.x * 64 + cl;
-  for (int c = tid >> 6; c < BT_NB; c += 4) {
-    zc acc = mkz(0, 0);
-    if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
-    Ss[c][cl] = acc;
-  }
+  const int c = tid >> 3, cl = tid & 7, col = blockIdx.x * 8 + cl;
+  zc acc = mkz(0, 0);
+  if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
+  Ss[c][cl] = acc;
   __syncthreads();
   if (col >= m) return;
-  for (int r = (tid >> 6) * 8; r < (tid >> 6) * 8 + 8; ++r) {
-    zc acc = mkz(0, 0);
-    for (int c = r; c < kp; ++c) acc = fma_z(Ts[r + BT_NB * c], Ss[c][cl], acc);   // T column-major
-    Y[(size_t)r * m + col] = acc;
-  }
+  const int r = c;
+  zc y = mkz(0, 0);
+  for (int k = r; k < kp; ++k) y = fma_z(Ts[r + BT_NB * k], Ss[k][cl], y);   // T column-major
+  Y[(size_t)r * m + col] = y;
 }
 
 // rows per V^H X chunk: 256, or down to 64 when the columns alone give few workgroups
@@ -265,7 +262,7 @@ static int run_backtr_t(hipStream_t s, const T* H, int ld, int N, const zc* tau,
       const int nch = (rows + chunk - 1) / chunk;
       hipLaunchKernelGGL(k_bt_vhx<T>, dim3(ncol, nch), dim3(256), 0, s, H, ld, N, p, kp, X, ldx, m, chunk,
                          Ypart);
-      hipLaunchKernelGGL(k_bt_ty, dim3(ncol), dim3(256), 0, s, Ypart, nch, m, kp,
+      hipLaunchKernelGGL(k_bt_ty, dim3((m + 7) / 8), dim3(256), 0, s, Ypart, nch, m, kp,
                          Tf + (size_t)b * BT_NB * BT_NB, Y);
       hipLaunchKernelGGL(k_bt_update<T>, dim3((rows + 63) / 64, ncol), dim3(256), 0, s, H, ld, N, p, kp,
                          Y, X, ldx, m);

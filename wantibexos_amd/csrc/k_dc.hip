@@ -414,17 +414,21 @@ __device__ inline SecSums secular_sums(const double* __restrict__ dl, const doub
   return r;
 }
 
-// M3: roots of the secular equation, 16 lanes per root.  Root j lies between poles j and
-// j + 1 (above the last pole for j = K - 1); its offset tau from the nearer pole o is
-// bracketed by the bit patterns of |tau|, [lob, hib], with f on the two sides of zero.
-// Every evaluation narrows the bracket.  The next point is the middle-way step
-// (rational interpolation of the two nearest poles' parts of f, with value and
-// derivative at the last point) when it falls strictly inside the bracket, else the
-// bracket's bit midpoint.  Once a step moves by at most two bit patterns the iterate has
-// converged, and the other end is found by galloping from it (1, 2, 4, ... patterns)
-// until the sign flips.  The iteration ends where the plain bisection over the bit
-// patterns ends, at adjacent patterns: a CPU emulation over 7435 random roots gave the
-// same bits in every case, in 5.5 evaluations on average instead of 62.
+// M3: roots of the secular equation, G lanes per root (16, or a wave for merges above
+// 256).  Root j lies between poles j and j + 1 (above the last pole for j = K - 1); its
+// offset tau from the nearer pole o is bracketed by the bit patterns of |tau|, [lob, hib],
+// with f on the two sides of zero, and every evaluation narrows the bracket.  The first
+// point is the midpoint of the bracket in value.  The next point is the middle-way step
+// (rational interpolation of the two nearest poles' parts of f, with value and derivative
+// at the last point) when it falls strictly inside the bracket, else the bracket's bit
+// midpoint.  The iterate has converged when, after a first middle-way step, a step moves
+// by at most 8 bit patterns or lands just past the end the last point set; after 12
+// middle-way steps it is taken as converged too.  From there the other end is found by
+// galloping (1, 2, 4, ... patterns) until the sign flips, and the patterns left are
+// bisected.  The iteration ends where the plain bisection over the bit patterns ends, at
+// adjacent patterns.  A CPU emulation on the merges of an N = 5184 BSE tridiagonal gave
+// the bisection's bits on every root, in 8.4 evaluations on average at the root merge
+// (at most 30) instead of 62.
 template <int G>
 __global__ __launch_bounds__(256) void k_dc_secular(DCArgs a) {
   const int node = blockIdx.y;
@@ -471,11 +475,25 @@ __global__ __launch_bounds__(256) void k_dc_secular(DCArgs a) {
     o = K - 1;
     neg = false;
     hi = rho * ws;
+    // a first point at the bracket's midpoint in value (the bit midpoint of [0, hi] is
+    // a tiny number, from which the middle-way model says nothing)
+    tc = 0.5 * hi;
+    sm = secular_sums<G>(dl, w, K, o, js, tc, sub);
+    have = true;
   }
   // the poles bounding the two-pole model, relative to the origin
   const double dlo = mw ? dl[js] - dl[o] : 0.0, dhi = mw ? dl[js + 1] - dl[o] : 0.0;
   unsigned long long lob = 0ull, hib = __double_as_longlong(hi);
   int side = 0;                                      // last evaluation moved lob (+1) or hib (-1)
+  if (last) {
+    if (1.0 + rho * (sm.psi + sm.phi) < 0.0) {
+      lob = __double_as_longlong(tc);
+      side = 1;
+    } else {
+      hib = __double_as_longlong(tc);
+      side = -1;
+    }
+  }
   int nmw = 0, gdir = 0;
   bool conv = false;
   unsigned long long step = 1ull;
@@ -494,7 +512,15 @@ __global__ __launch_bounds__(256) void k_dc_secular(DCArgs a) {
         step <<= 1;
         if (pb > lob && pb < hib) mb = pb;
       }
-    } else if (mw && have && nmw < 12) {
+    } else if (mw && have && nmw >= 12) {
+      // twelve middle-way steps without convergence (the steps creep): gallop from the
+      // last point
+      conv = true;
+      gdir = side;
+      const unsigned long long pb = gdir > 0 ? lob + 1ull : hib - 1ull;
+      step = 2ull;
+      if (pb > lob && pb < hib) mb = pb;
+    } else if (mw && have) {
       // middle way: f / rho = 1 / rho + psi + phi; c eta^2 - a eta + b = 0 with the
       // two poles' distances from the last point
       const double W = 1.0 / rho + sm.psi + sm.phi;
@@ -515,7 +541,8 @@ __global__ __launch_bounds__(256) void k_dc_secular(DCArgs a) {
       if (isfinite(mn) && mn > 0.0) {
         unsigned long long pb = __double_as_longlong(mn);
         const unsigned long long dist = pb > lastb ? pb - lastb : lastb - pb;
-        if (dist <= 2ull) {
+        // converged: a step (after a first middle-way step) of at most 8 bit patterns
+        if (nmw >= 1 && dist <= 8ull) {
           conv = true;
           gdir = side;
           pb = gdir > 0 ? lob + 1ull : hib - 1ull;
@@ -524,7 +551,7 @@ __global__ __launch_bounds__(256) void k_dc_secular(DCArgs a) {
         } else if (pb > lob && pb < hib) {
           mb = pb;
           ++nmw;
-        } else if (dist <= (1ull << 20)) {
+        } else if (nmw >= 1 && dist <= (1ull << 20)) {
           // a proposal just past the end the last point set: the steps have converged
           // there, so gallop from it (a bisection would restart from the far end)
           conv = true;
@@ -679,18 +706,42 @@ __global__ __launch_bounds__(256) void k_dc_apply(DCArgs a) {
   }
 }
 
-// M6: sort the node's eigenvalues ascending and write lam/FR/LR/T
-__global__ void k_dc_sort(DCArgs a) {
+// M6: sort the node's eigenvalues ascending (stable in the slot order) and write
+// lam/FR/LR/T.  The rank of value i counts the values below it: 8 lanes share an element,
+// each counting every 8th value of the node as it passes through LDS in chunks, and their
+// counts are added.
+constexpr int kSortLanes = 8;
+__global__ __launch_bounds__(256) void k_dc_sort(DCArgs a) {
+  constexpr int CH = 4096;
+  __shared__ double buf[CH];
   const DCNode nd = a.nodes[blockIdx.y];
   const int n = nd.hi - nd.lo, lo = nd.lo;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double v = a.nl[lo + i];
+  constexpr int EPB = 256 / kSortLanes;                 // elements per workgroup
+  if ((int)(blockIdx.x * EPB) >= n) return;             // whole block idle (uniform)
+  const int part = threadIdx.x & (kSortLanes - 1);
+  const int i = blockIdx.x * EPB + threadIdx.x / kSortLanes;
+  const double v = i < n ? a.nl[lo + i] : 0.0;
   int rank = 0;
-  for (int k = 0; k < n; ++k) {
-    double u = a.nl[lo + k];
-    rank += (u < v) || (u == v && k < i);
+  for (int k0 = 0; k0 < n; k0 += CH) {
+    const int len = min(CH, n - k0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < len; k += blockDim.x) buf[k] = a.nl[lo + k0 + k];
+    __syncthreads();
+    int k = part;
+    for (; k + 3 * kSortLanes < len; k += 4 * kSortLanes) {
+      double u[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) u[q] = buf[k + q * kSortLanes];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rank += (u[q] < v) || (u[q] == v && k0 + k + q * kSortLanes < i);
+    }
+    for (; k < len; k += kSortLanes) {
+      const double u = buf[k];
+      rank += (u < v) || (u == v && k0 + k < i);
+    }
   }
+  for (int off = kSortLanes / 2; off > 0; off >>= 1) rank += __shfl_xor(rank, off, kSortLanes);
+  if (i >= n || part != 0) return;
   const int N = a.N, dst = lo + rank;
   if (a.Z) a.rank[lo + i] = rank;
   a.lam[dst] = v;
@@ -969,7 +1020,8 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
       if (tree.ntrack <= 8) hipLaunchKernelGGL((k_dc_apply<10, 16>), g16, dim3(256), 0, s, a);
       else hipLaunchKernelGGL((k_dc_apply<14, 16>), g16, dim3(256), 0, s, a);
     }
-    hipLaunchKernelGGL(k_dc_sort, g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dc_sort, dim3((maxn + 256 / kSortLanes - 1) / (256 / kSortLanes), cnt), dim3(256), 0, s,
+                       a);
     if (d_Z) {
       hipLaunchKernelGGL(k_dcv_gather, dim3(maxn, cnt), dim3(256), 0, s, a);
       hipLaunchKernelGGL(k_dcv_rotate, dim3((maxn + 255) / 256, cnt), dim3(256), 0, s, a);

@@ -805,12 +805,17 @@ __global__ __launch_bounds__(256) void k_hmm_sum(int m, int ns, const zc* __rest
 template <class T, int B>
 __global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, int m, int r0, int N,
                                                     const zc* __restrict__ V, const zc* __restrict__ W) {
-  if (blockIdx.y > blockIdx.x) return;                  // tiles above the diagonal: never read
+  // one workgroup per tile on or below the diagonal: t -> tile row ti >= tile column tj
+  const int t = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
   constexpr bool F64 = sizeof(T) == sizeof(zc);
   constexpr int K = 2 * B;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ci0 = blockIdx.y * 64 + (wave >> 1) * 32;   // tile columns (MFMA rows)
-  const int rj0 = blockIdx.x * 64 + (wave & 1) * 32;    // tile rows (MFMA columns)
+  const int ci0 = tj * 64 + (wave >> 1) * 32;           // tile columns (MFMA rows)
+  const int rj0 = ti * 64 + (wave & 1) * 32;            // tile rows (MFMA columns)
   const int lr = lane & 15, lk = lane >> 4;
   typedef typename std::conditional<F64, d4, f4>::type acc_t;
   // COMPLEX*8: the tile's current values are loaded first, so that their latency overlaps
@@ -1293,7 +1298,8 @@ static int stage1(hipStream_t s, BandWork& w, T* H, int ld, int nrhs) {
     const int nt = (m + 63) / 64;
     const bool tev = w.kev && w.kev_used < w.kev_cap;
     if (tev) hipEventRecord(w.kev[2 * w.kev_used], s);
-    hipLaunchKernelGGL((k_her2k_full<T, B>), dim3(nt, nt), dim3(256), 0, s, H, ld, m, r0, N, w.V, w.X);
+    hipLaunchKernelGGL((k_her2k_full<T, B>), dim3(nt * (nt + 1) / 2), dim3(256), 0, s, H, ld, m, r0, N, w.V,
+                       w.X);
     if (tev) {
       hipEventRecord(w.kev[2 * w.kev_used++ + 1], s);
       w.kev_flops += 8.0 * 0.5 * (double)m * (double)(m + 1) * (2.0 * k);   // lower triangle, 2k products each

@@ -803,7 +803,7 @@ __global__ __launch_bounds__(256) void k_hmm_sum(int m, int ns, const zc* __rest
 // Hermitian, so that a lane's result lands next to its neighbours' in the column-major
 // matrix.
 template <class T, int B>
-__global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, int m, int r0, int N,
+__global__ __launch_bounds__(256, 4) void k_her2k_full(T* __restrict__ A, int ld, int m, int r0, int N,
                                                     const zc* __restrict__ V, const zc* __restrict__ W) {
   // one workgroup per tile on or below the diagonal: t -> tile row ti >= tile column tj
   const int t = blockIdx.x;
@@ -839,6 +839,50 @@ __global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, i
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = acc_t{0, 0, 0, 0};
+  if constexpr (!F64) {
+    // COMPLEX*8: the tile's operands go through LDS once per workgroup, in FP32, half of K
+    // at a time (a wave pair shares each operand); the products are those below, in the
+    // same order
+    __shared__ zf As[B][64], Bs[B][64];
+    const int cl0 = tj * 64, rl0 = ti * 64;
+    const int ciw = (wave >> 1) * 32, rjw = (wave & 1) * 32;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      // half 0: A = V(c, k), B = W(r, k); half 1: A = W(c, k), B = V(r, k)
+      const zc* Asrc = h == 0 ? V : W;
+      const zc* Bsrc = h == 0 ? W : V;
+      if (h > 0) __syncthreads();
+#pragma unroll 4
+      for (int u = 0; u < B * 64 / 256; ++u) {
+        const int e = threadIdx.x + 256 * u, kk = e >> 6, cc = e & 63;
+        const int c = cl0 + cc, r = rl0 + cc;
+        const zc a = (c < m) ? Asrc[(size_t)kk * N + c] : mkz(0, 0);
+        const zc b = (r < m) ? Bsrc[(size_t)kk * N + r] : mkz(0, 0);
+        As[kk][cc] = zf{(float)a.re, (float)a.im};
+        Bs[kk][cc] = zf{(float)b.re, (float)b.im};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k0 = 0; k0 < B; k0 += 4) {
+        const int kk = k0 + lk;
+        zf af[2], bf[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          af[x] = As[kk][ciw + x * 16 + lr];
+          bf[x] = Bs[kk][rjw + x * 16 + lr];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            cr[x][y] = mfma16x16x4f32(af[x].re, bf[y].re, cr[x][y]);
+            cr[x][y] = mfma16x16x4f32(af[x].im, bf[y].im, cr[x][y]);
+            ci[x][y] = mfma16x16x4f32(af[x].im, bf[y].re, ci[x][y]);
+            ci[x][y] = mfma16x16x4f32(-af[x].re, bf[y].im, ci[x][y]);
+          }
+      }
+    }
+  } else {
   // A operand (i = column c, k): [V | W](c, k); B operand (k, j = row r): conj([W | V](r, k));
   // the operands of step k0 + 4 are loaded while step k0 is multiplied
   zc av[2], bv[2];
@@ -881,6 +925,7 @@ __global__ __launch_bounds__(256) void k_her2k_full(T* __restrict__ A, int ld, i
         bv[x] = bn[x];
       }
     }
+  }
   }
 #pragma unroll
   for (int x = 0; x < 2; ++x)

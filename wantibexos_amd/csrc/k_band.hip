@@ -841,8 +841,7 @@ __global__ __launch_bounds__(256, 4) void k_her2k_full(T* __restrict__ A, int ld
     for (int y = 0; y < 2; ++y) cr[x][y] = ci[x][y] = acc_t{0, 0, 0, 0};
   if constexpr (!F64) {
     // COMPLEX*8: the tile's operands go through LDS once per workgroup, in FP32, half of K
-    // at a time (a wave pair shares each operand); the products are those below, in the
-    // same order
+    // at a time (a wave pair shares each operand); the products run in the order of k
     __shared__ zf As[B][64], Bs[B][64];
     const int cl0 = tj * 64, rl0 = ti * 64;
     const int ciw = (wave >> 1) * 32, rjw = (wave & 1) * 32;

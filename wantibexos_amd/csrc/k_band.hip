@@ -675,8 +675,10 @@ __global__ __launch_bounds__(256) void k_hmm_b(int m, const T* __restrict__ A, i
   constexpr int NA = KS * HB_R / 256;               // matrix elements per thread and step
   constexpr int CT = BC / 16;                       // 16-column tiles
   constexpr int YP = KS * BC / 256;                 // Y elements per thread and step
-  __shared__ ct As[KS][HB_R];
-  __shared__ ct Ys[KS][BC];
+  // rows padded by one element: the four k rows a wave reads at once (and the 16 rows the
+  // transposed loads store at once) fall in different LDS banks
+  __shared__ ct As[KS][HB_R + 1];
+  __shared__ ct Ys[KS][BC + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int R0 = blockIdx.x * HB_R;
@@ -842,7 +844,7 @@ __global__ __launch_bounds__(256, 4) void k_her2k_full(T* __restrict__ A, int ld
   if constexpr (!F64) {
     // COMPLEX*8: the tile's operands go through LDS once per workgroup, in FP32, half of K
     // at a time (a wave pair shares each operand); the products run in the order of k
-    __shared__ zf As[B][64], Bs[B][64];
+    __shared__ zf As[B][65], Bs[B][65];     // padded rows: no bank conflicts between k rows
     const int cl0 = tj * 64, rl0 = ti * 64;
     const int ciw = (wave >> 1) * 32, rjw = (wave & 1) * 32;
 #pragma unroll

@@ -354,8 +354,10 @@ __global__ __launch_bounds__(256) void k_hmm32(int N, int nc, const zf* __restri
                                                int kc, zc* __restrict__ part) {
   typedef typename std::conditional<F64, double2, float2>::type bt;
   typedef typename std::conditional<F64, d4, f4>::type acc_t;
-  __shared__ float2 As[16][H32_R];
-  __shared__ bt Bs[16][H32_C];
+  // rows padded by one element: the four k rows a wave reads at once, and the 16 rows the
+  // X loader stores at once, fall in different LDS banks
+  __shared__ float2 As[16][H32_R + 1];
+  __shared__ bt Bs[16][H32_C + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i16 = lane & 15, kq = lane >> 4;
   const int R0 = blockIdx.x * H32_R, C0 = blockIdx.y * H32_C;

@@ -419,6 +419,84 @@ __device__ __forceinline__ void tri_scal(const TriArgs<T>& a, int j, int jj) {
 
 
 // ---- hemvstep: 64 x 64 tiles ------------------------------------------------------------
+// The diagonal tile's dot products over its 64 rows (quantity q: 0 = |x~|^2, 1..jj = W^H x~,
+// jj+1..2jj = V^H x~, then x~^T g), 4 waves x EPQ quantities per pass: 8 for the COMPLEX*8
+// form, 16 for FP64.  Pass 0 runs in the diagonal tile's workgroup; for the COMPLEX*8
+// form later passes run in workgroups of their own (tri_hemv_quant), each forming x~ and
+// W(:, jj-1) of its rows the same way, so that the diagonal tile's chain of dependent
+// loads is one pass long (FP64: all passes in the diagonal tile, whose registers an extra
+// path would overflow).
+template <class T>
+__host__ __device__ constexpr int hemv_epq() { return sizeof(T) == sizeof(zc) ? 16 : 8; }
+template <class T>
+__host__ __device__ inline int hemv_split_passes(int jj, int nrhs) {
+  if (sizeof(T) == sizeof(zc)) return 1;
+  const int nq = 1 + 2 * jj + nrhs;
+  return (nq + 4 * hemv_epq<T>() - 1) / (4 * hemv_epq<T>());
+}
+
+// one pass of the dot products of row block bi: wave w takes quantities q0 = EPQ w + 4 EPQ
+// pass .. + EPQ - 1, lane = row r (x~(r) = xr, W(r, jj-1) = wl), from the pass's operands
+// loaded beforehand (hemv_quant_load)
+template <class T>
+__device__ __forceinline__ const zc* hemv_operand(const TriArgs<T>& a, int q, int jj, int nq, int r) {
+  // the loaded operand: W(r, q-1), V(r, q-1-jj) or g(q-1-2jj, r); W(r, jj-1) comes from LDS
+  const int N = a.N;
+  if (q >= 1 && q < jj) return a.W + (size_t)(q - 1) * N + r;
+  if (q > jj && q <= 2 * jj) return a.Vp + (size_t)(q - 1 - jj) * N + r;
+  if (q > 2 * jj && q < nq) return a.g + (size_t)(q - 1 - 2 * jj) * N + r;
+  return nullptr;
+}
+template <int EPQ>
+struct HemvPV {
+  zc v[EPQ];
+};
+template <class T>
+__device__ __forceinline__ HemvPV<hemv_epq<T>()> hemv_quant_load(const TriArgs<T>& a, int q0, int jj, int nq, int r,
+                                                                 bool rok) {
+  HemvPV<hemv_epq<T>()> o;
+#pragma unroll
+  for (int k = 0; k < hemv_epq<T>(); ++k) {
+    const zc* src = hemv_operand(a, q0 + k, jj, nq, r);
+    o.v[k] = (src && rok) ? *src : mkz(0, 0);
+  }
+  return o;
+}
+template <class T>
+__device__ __forceinline__ void hemv_quant_pass(const TriArgs<T>& a, HemvPV<hemv_epq<T>()> in, int q0, int bi,
+                                                int jj, int nq, int r, bool rok, zc xr, zc wl, int lane) {
+  constexpr int EPQ = hemv_epq<T>();
+  zc pv[EPQ];
+#pragma unroll
+  for (int k = 0; k < EPQ; ++k) pv[k] = in.v[k];
+  const double xr2 = norm2(xr);
+#pragma unroll
+  for (int k = 0; k < EPQ; ++k) {
+    const int q = q0 + k;
+    const zc val = (q == jj) ? wl : pv[k];
+    zc v = (q <= 2 * jj) ? fma_conj(val, xr, mkz(0, 0)) : fma_z(xr, val, mkz(0, 0));
+    if (q == 0) v = mkz(xr2, 0.0);
+    pv[k] = (q < nq && rok) ? v : mkz(0, 0);
+  }
+  int q;
+  bool head;
+  if constexpr (EPQ == 8) {
+    BUTTERFLY8(pv, lane);
+    q = q0 + ((lane >> 3) & 7);
+    head = (lane & 7) == 0;
+  } else {
+    BUTTERFLY16(pv, lane);
+    q = q0 + ((lane >> 2) & 15);
+    head = (lane & 3) == 0;
+  }
+  if (head && q < nq) {
+    if (q == 0) a.pn[bi] = pv[0].re;
+    else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = pv[0];
+    else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = pv[0];
+    else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = pv[0];
+  }
+}
+
 // 4 waves, wave w owns 16 columns, lane = row: every load moves one column segment.
 // FP64 matrix: one tile per workgroup.  COMPLEX*8 matrix: the diagonal tiles one per
 // workgroup, then the off-diagonal ones two per workgroup, both tiles' loads in flight
@@ -566,10 +644,9 @@ __device__ __forceinline__ void tri_hemv_tile(const TriArgs<T>& a, int ti, int t
     // lane holds the sum for column 16w + ((lane >> 2) & 15)
     if ((lane & 3) == 0) y2s[0][16 * w + ((lane >> 2) & 15)] = hv[0];
   }
-  // quantity q: 0 = |x~|^2, 1..jj = W^H x~, jj+1..2jj = V^H x~, then x~^T g.  Wave w takes
-  // groups of EPQ quantities (q0 = EPQ w, EPQ w + 4 EPQ, ...): lane = row.  The first
-  // group's operands are loaded here, before the barrier, so that the loads overlap the
-  // tile's epilogue.
+  // the first pass of the dot products (its operands are loaded here, before the barrier,
+  // so that the loads overlap the tile's epilogue); later passes run in workgroups of
+  // their own
   const int nq = 1 + 2 * jj + a.nrhs;
   const bool rok = r >= j + 2 && r < N;
   auto operand = [&](int q) -> const zc* {
@@ -579,9 +656,7 @@ __device__ __forceinline__ void tri_hemv_tile(const TriArgs<T>& a, int ti, int t
     if (q > 2 * jj && q < nq) return a.g + (size_t)(q - 1 - 2 * jj) * N + r;
     return nullptr;
   };
-  // quantities per wave and pass: 8 for the COMPLEX*8 form, whose registers then fit 4
-  // waves per SIMD, 16 for FP64
-  constexpr int EPQ = std::is_same<T, zf>::value ? 8 : 16;
+  constexpr int EPQ = hemv_epq<T>();
   zc pv[EPQ];
   if (diag) {
 #pragma unroll
@@ -600,7 +675,7 @@ __device__ __forceinline__ void tri_hemv_tile(const TriArgs<T>& a, int ti, int t
     yp[((size_t)bj * a.nblk + bi) * TB + (tid - TB)] = y2s[0][tid - TB];
   }
   if (!diag) return;
-  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, g(j+1), partial dot products ------------
+  // ---- diagonal tiles: x_new, W(:,jj-1), d_j, g(j+1), the first pass of dot products ----
   if (xok && tid < TB) {
     a.xn[xrow] = xv;
     if (jj > 0) a.W[(size_t)(jj - 1) * N + xrow] = wo;
@@ -609,42 +684,91 @@ __device__ __forceinline__ void tri_hemv_tile(const TriArgs<T>& a, int ti, int t
   // row j+1 (block b0) of g, for the next rowstep's v^T g, which also updates that row
   if (gsl) a.gs[tid - XT] = x0;
   const zc wl = rok ? sWl[lane] : mkz(0, 0);
-  const double xr2 = norm2(xr);
-  for (int q0 = EPQ * w; q0 < nq; q0 += 4 * EPQ) {
-    if (q0 > EPQ * w) {
+  if constexpr (sizeof(T) != sizeof(zc)) {
+    HemvPV<EPQ> in;
+#pragma unroll
+    for (int k = 0; k < EPQ; ++k) in.v[k] = pv[k];
+    hemv_quant_pass(a, in, EPQ * w, bi, jj, nq, r, rok, xr, wl, lane);
+  } else {
+    // FP64: every pass here (the operands of the first were loaded above)
+    const double xr2 = norm2(xr);
+    for (int q0 = EPQ * w; q0 < nq; q0 += 4 * EPQ) {
+      if (q0 > EPQ * w) {
+#pragma unroll
+        for (int k = 0; k < EPQ; ++k) {
+          const zc* src = operand(q0 + k);
+          pv[k] = (src && rok) ? *src : mkz(0, 0);
+        }
+      }
 #pragma unroll
       for (int k = 0; k < EPQ; ++k) {
-        const zc* src = operand(q0 + k);
-        pv[k] = (src && rok) ? *src : mkz(0, 0);
+        const int q = q0 + k;
+        const zc val = (q == jj) ? wl : pv[k];
+        zc v = (q <= 2 * jj) ? fma_conj(val, xr, mkz(0, 0)) : fma_z(xr, val, mkz(0, 0));
+        if (q == 0) v = mkz(xr2, 0.0);
+        pv[k] = (q < nq && rok) ? v : mkz(0, 0);
       }
-    }
-#pragma unroll
-    for (int k = 0; k < EPQ; ++k) {
-      const int q = q0 + k;
-      const zc val = (q == jj) ? wl : pv[k];
-      zc v = (q <= 2 * jj) ? fma_conj(val, xr, mkz(0, 0)) : fma_z(xr, val, mkz(0, 0));
-      if (q == 0) v = mkz(xr2, 0.0);
-      pv[k] = (q < nq && rok) ? v : mkz(0, 0);
-    }
-    int q;
-    bool head;
-    if constexpr (EPQ == 8) {
-      BUTTERFLY8(pv, lane);
-      q = q0 + ((lane >> 3) & 7);
-      head = (lane & 7) == 0;
-    } else {
       BUTTERFLY16(pv, lane);
-      q = q0 + ((lane >> 2) & 15);
-      head = (lane & 3) == 0;
-    }
-    if (head && q < nq) {
-      if (q == 0) a.pn[bi] = pv[0].re;
-      else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = pv[0];
-      else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = pv[0];
-      else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = pv[0];
+      const int q = q0 + ((lane >> 2) & 15);
+      if ((lane & 3) == 0 && q < nq) {
+        if (q == 0) a.pn[bi] = pv[0].re;
+        else if (q <= jj) a.pt1[(size_t)bi * a.nb + (q - 1)] = pv[0];
+        else if (q <= 2 * jj) a.pt2[(size_t)bi * a.nb + (q - 1 - jj)] = pv[0];
+        else a.pvg[(size_t)bi * a.nrhs + (q - 1 - 2 * jj)] = pv[0];
+      }
     }
   }
 }
+
+// later passes of row block b0 + ti's dot products: x~ and W(:, jj-1) of its rows formed
+// as the diagonal tile forms them
+template <class T>
+__device__ __forceinline__ void tri_hemv_quant(const TriArgs<T>& a, int ti, int pass, int j, int jj, int nprev) {
+  __shared__ zc qxR[TB], qW[TB];
+  __shared__ zc q_ap;
+  const int N = a.N;
+  const int bi = (j + 1) / TB + ti;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int R0 = bi * TB;
+  const int xrow = R0 + tid;
+  const bool xok = tid < TB && xrow >= j + 1 && xrow < N;
+  zc x0 = mkz(0, 0), x1 = mkz(0, 0), uo = mkz(0, 0);
+  if (xok) {
+    if (jj == 0) {
+      x0 = ldz(a.H[(size_t)xrow + (size_t)a.ld * j]);
+    } else {
+      x0 = a.xp[xrow];
+      x1 = a.Vp[(size_t)(jj - 1) * N + xrow];
+      uo = a.u[xrow];
+    }
+  }
+  const int nq = 1 + 2 * jj + a.nrhs;
+  constexpr int EPQ = hemv_epq<T>();
+  const int q0 = EPQ * w + 4 * EPQ * pass;
+  const int r = R0 + lane;
+  const bool rok = r >= j + 2 && r < N;
+  const HemvPV<EPQ> pv0 = hemv_quant_load(a, q0, jj, nq, r, rok);
+  const zc taup = (jj > 0) ? a.tau[j - 1] : mkz(0, 0);
+  if (jj > 0) {
+    const zc al = alpha_from_puv(a.puv, nprev, taup);
+    if (tid == 0) q_ap = al;
+  } else if (tid == 0) {
+    q_ap = mkz(0, 0);
+  }
+  __syncthreads();
+  const zc ap = q_ap;
+  const double sig = 2.0 * ap.re;
+  const zc xv = x0 - sig * x1;
+  if (tid < TB) {
+    qxR[tid] = (xok && xrow >= j + 2) ? xv : mkz(0, 0);
+    qW[tid] = (jj > 0 && xok) ? taup * uo + ap * x1 : mkz(0, 0);
+  }
+  __syncthreads();
+  const zc xr = qxR[lane];
+  const zc wl = rok ? qW[lane] : mkz(0, 0);
+  hemv_quant_pass(a, pv0, q0, bi, jj, nq, r, rok, xr, wl, lane);
+}
+
 // lower-triangle tile index t -> (ti, tj), tj <= ti
 __device__ inline void tri_tile(int t, int& ti, int& tj) {
   ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -774,19 +898,29 @@ __device__ __forceinline__ void tri_hemv_pair(const TriArgs<T>& a, int pb, int n
   }
 }
 
-// workgroups of the GEMV at column j (nt = row blocks of the trailing matrix)
+// workgroups of the GEMV at column j (nt = row blocks of the trailing matrix): the tiles
+// (COMPLEX*8: diagonal ones alone, off-diagonal ones in pairs), then nt workgroups for each
+// pass of dot products after the first
 template <class T>
-__host__ __device__ inline int hemv_blocks(int nt) {
+__host__ __device__ inline int hemv_tile_blocks(int nt) {
   if (sizeof(T) == sizeof(zc)) return nt * (nt + 1) / 2;
   const int noff = nt * (nt - 1) / 2;
   return nt + (noff + 1) / 2;
 }
+template <class T>
+__host__ __device__ inline int hemv_blocks(int nt, int jj, int nrhs) {
+  return hemv_tile_blocks<T>(nt) + nt * (hemv_split_passes<T>(jj, nrhs) - 1);
+}
 
 template <class T>
 __device__ __forceinline__ void tri_hemv(const TriArgs<T>& a, int bx, int j, int jj, int nprev) {
+  const int nt = a.nblk - (j + 1) / TB;
   if constexpr (std::is_same<T, zf>::value) {
-    const int nt = a.nblk - (j + 1) / TB;
-    if (bx < nt) {
+    const int ntile = hemv_tile_blocks<T>(nt);
+    if (bx >= ntile) {
+      const int q = bx - ntile;
+      tri_hemv_quant(a, q % nt, 1 + q / nt, j, jj, nprev);
+    } else if (bx < nt) {
       tri_hemv_tile(a, bx, bx, j, jj, nprev);
     } else {
       tri_hemv_pair(a, bx - nt, nt * (nt - 1) / 2, j, jj, nprev);
@@ -1121,7 +1255,7 @@ static int run_tridiag_t(hipStream_t s, TridiagWork& tw, T* d_H, int ldh, zc* d_
       const int nt = a.nblk - (j + 1) / TB;
       const bool tev = tw.kev && tw.kev_used < tw.kev_cap;
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used], s);
-      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(hemv_blocks<T>(nt)), dim3(256), 0, s, a, j, jj, nprev);
+      hipLaunchKernelGGL(k_tri_hemv<T>, dim3(hemv_blocks<T>(nt, jj, tw.nrhs)), dim3(256), 0, s, a, j, jj, nprev);
       if (tev) hipEventRecord(tw.kev[2 * tw.kev_used++ + 1], s);
     }
     // finish the panel's last reflector (no further column in this panel)
@@ -1212,7 +1346,7 @@ static int run_tridiag_group_t(hipStream_t s, const TriGroup& G) {
       }
       counts(ms, [&](int i) {
         const int nt = nblk[i] - (j + 1) / TB;
-        return hemv_blocks<T>(nt);
+        return hemv_blocks<T>(nt, jj, G.items[i].tw->nrhs);
       });
       launch_group<T>(s, k_tri_hemv_b<T>, dargs, ms, nbk, 256, j, jj);
       if (fs.empty()) continue;

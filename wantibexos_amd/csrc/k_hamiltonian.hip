@@ -6,6 +6,7 @@
 // subroutines/optics.f90:1-156 (hlm, optsp), general_subs.f90:69-112
 // (sandwich, prodintsq), subprograms/bse_solver-tool-diel.f90:257-377.
 #include "wtb_kernels.h"
+#include <algorithm>
 
 namespace wtb {
 
@@ -152,11 +153,11 @@ __device__ inline int rr_player(int pos, int round, int ne) {
   return pos == 0 ? 0 : ((pos - 1 + round) % (ne - 1)) + 1;
 }
 
-__global__ __launch_bounds__(64) void k_jacobi(const zc* __restrict__ Hk, int nk, int nw,
-                                               double ef32, double scs, int nc, int nv,
-                                               double* __restrict__ Eall, zc* __restrict__ Uout,
-                                               int* __restrict__ nocp_out, double* __restrict__ eigv,
-                                               zc* __restrict__ vec, int* __restrict__ status) {
+__device__ __forceinline__ void jacobi_body(int k, const zc* __restrict__ Hk, int nk, int nw, double ef32,
+                                            double scs, int nc, int nv, double* __restrict__ Eall,
+                                            zc* __restrict__ Uout, int* __restrict__ nocp_out,
+                                            double* __restrict__ eigv, zc* __restrict__ vec,
+                                            int* __restrict__ status) {
   extern __shared__ unsigned char smem[];
   const int n = nw;
   const int ne = n + (n & 1);
@@ -172,7 +173,6 @@ __global__ __launch_bounds__(64) void k_jacobi(const zc* __restrict__ Hk, int nk
   double* lam = reinterpret_cast<double*>(flag + 2);
   int* rank = reinterpret_cast<int*>(lam + n);
 
-  const int k = blockIdx.x;
   const int lane = threadIdx.x;
   const zc* H = Hk + (size_t)k * n * n;
   double amax = 0.0;
@@ -299,6 +299,29 @@ __global__ __launch_bounds__(64) void k_jacobi(const zc* __restrict__ Hk, int nk
     }
   }
 }
+__global__ __launch_bounds__(64) void k_jacobi(const zc* __restrict__ Hk, int nk, int nw,
+                                               double ef32, double scs, int nc, int nv,
+                                               double* __restrict__ Eall, zc* __restrict__ Uout,
+                                               int* __restrict__ nocp_out, double* __restrict__ eigv,
+                                               zc* __restrict__ vec, int* __restrict__ status) {
+  jacobi_body(blockIdx.x, Hk, nk, nw, ef32, scs, nc, nv, Eall, Uout, nocp_out, eigv, vec, status);
+}
+
+// the k-points of up to 64 models in one launch: workgroup b belongs to the model whose
+// range [pre[i], pre[i+1]) holds it and runs that model's k-point b - pre[i]
+struct JacobiBatch {
+  const JacobiItem* items;   // device array
+  int n;
+  int m[64];
+  int pre[65];
+};
+__global__ __launch_bounds__(64) void k_jacobi_b(JacobiBatch B) {
+  int i = 0;
+  while (i + 1 < B.n && B.pre[i + 1] <= (int)blockIdx.x) ++i;
+  const JacobiItem it = B.items[B.m[i]];
+  jacobi_body((int)blockIdx.x - B.pre[i], it.hk, it.nk, it.nw, it.ef32, it.scs, it.nc, it.nv, it.eall, it.uout,
+              it.nocp, it.eigv, it.vec, it.status);
+}
 
 // ---------------------------------------------------------------------------
 // Dipoles (optsp, optics.f90:102-156): for every BSE state I = (k,c,v),
@@ -390,6 +413,28 @@ int launch_jacobi(hipStream_t s, const zc* d_hk, int nk, int nw, double ef32, do
   size_t lds = jacobi_lds_bytes(nw);
   hipLaunchKernelGGL(k_jacobi, dim3(nk), dim3(64), lds, s, d_hk, nk, nw, ef32, scs, nc, nv, d_eall,
                      d_uout, d_nocp, d_eigv, d_vec, d_status);
+  WTB_LAUNCH_CHECK();
+  return WTB_OK;
+}
+
+int launch_jacobi_batch(hipStream_t s, const JacobiItem* d_items, const std::vector<JacobiItem>& items) {
+  int nwmax = 0;
+  for (const auto& it : items) nwmax = std::max(nwmax, it.nw);
+  const size_t lds = jacobi_lds_bytes(nwmax);
+  for (size_t c0 = 0; c0 < items.size(); c0 += 64) {
+    JacobiBatch B;
+    B.items = d_items;
+    B.n = 0;
+    int tot = 0;
+    for (size_t i = c0; i < items.size() && i < c0 + 64; ++i) {
+      B.m[B.n] = (int)i;
+      B.pre[B.n] = tot;
+      tot += items[i].nk;
+      ++B.n;
+    }
+    B.pre[B.n] = tot;
+    if (tot > 0) hipLaunchKernelGGL(k_jacobi_b, dim3(tot), dim3(64), lds, s, B);
+  }
   WTB_LAUNCH_CHECK();
   return WTB_OK;
 }

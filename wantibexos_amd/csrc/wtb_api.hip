@@ -729,8 +729,26 @@ static int qmode_status(const int st[2]) {
   return k_bits | q_bits;
 }
 
-// stages (a)-(c) and the tracked vectors' copy: everything before the reduction
-static int plan_enqueue_pre(wtb_bse_plan* p, bool build_only) {
+// the plan's per-k Jacobi eigensolves (stage b) as a batch item
+static JacobiItem plan_jacobi_item(const wtb_bse_plan* p) {
+  return JacobiItem{p->hk4, p->nk, p->nw, p->ef32, p->scs, p->prm.nc, p->prm.nv, p->eall, nullptr,
+                    p->nocp, p->eigv, p->vec, p->status};
+}
+
+// stage (a): the k grid and the H(k) phase GEMM
+static int plan_enqueue_pre_a(wtb_bse_plan* p) {
+  hipStream_t s = p->stream;
+  const wtb_bse_params& P = p->prm;
+  int rc;
+  if (p->timing) hipEventRecord(p->ev[0], s);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p->status, 1);
+  if ((rc = launch_kgrid(s, p->dm.rlat, P.ngrid, P.mshift, p->kpt))) return rc;
+  return launch_hk(s, p->dm, p->kpt, p->nk, p->phase, p->hk4);
+}
+
+// stages (a)-(c) and the tracked vectors' copy: everything before the reduction (stages
+// (a) and (b) only when `after_b`: the batch runs them itself)
+static int plan_enqueue_pre(wtb_bse_plan* p, bool build_only, bool after_b = false) {
   hipStream_t s = p->stream;
   const wtb_bse_params& P = p->prm;
   const int nk = p->nk, nw = p->nw, N = p->N;
@@ -738,14 +756,14 @@ static int plan_enqueue_pre(wtb_bse_plan* p, bool build_only) {
   auto mark = [&](int i) {
     if (p->timing) hipEventRecord(p->ev[i], s);
   };
-  mark(0);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p->status, 1);
-  // (a) k grid + H(k) phase GEMM, (b) per-k Jacobi
-  if ((rc = launch_kgrid(s, p->dm.rlat, P.ngrid, P.mshift, p->kpt))) return rc;
-  if ((rc = launch_hk(s, p->dm, p->kpt, nk, p->phase, p->hk4))) return rc;
-  if ((rc = launch_jacobi(s, p->hk4, nk, nw, p->ef32, p->scs, P.nc, P.nv, p->eall, nullptr, p->nocp,
-                          p->eigv, p->vec, p->status)))
-    return rc;
+  if (!after_b) {
+    // (a) k grid + H(k) phase GEMM, (b) per-k Jacobi
+    if ((rc = plan_enqueue_pre_a(p))) return rc;
+    const JacobiItem j = plan_jacobi_item(p);
+    if ((rc = launch_jacobi(s, j.hk, j.nk, j.nw, j.ef32, j.scs, j.nc, j.nv, j.eall, j.uout, j.nocp, j.eigv,
+                            j.vec, j.status)))
+      return rc;
+  }
   mark(1);
   if ((rc = launch_dipoles(s, p->hk4, nk, nw, P.nc, P.nv, p->sme32, p->eigv, p->vec, p->dip, p->gapk)))
     return rc;
@@ -1720,8 +1738,11 @@ struct wtb_bse_batch {
   std::vector<hipEvent_t> ev_done;      // one per lane
   hipEvent_t ev_start = nullptr;
   hipEvent_t ev_mid = nullptr;          // the groups' reductions are done
+  hipEvent_t ev_jac = nullptr;          // the batched Jacobi eigensolves are done
   std::vector<TriGroup> groups;
   std::vector<char> grouped;            // plan -> reduced in a group
+  std::vector<JacobiItem> jitems;       // the grouped plans' Jacobi eigensolves (stage b)
+  JacobiItem* d_jitems = nullptr;
   bool stale = true;                    // groups and graph to be (re)built
   hipGraphExec_t gexec = nullptr;
 };
@@ -1731,6 +1752,9 @@ static void batch_drop(wtb_bse_batch* b) {
   b->gexec = nullptr;
   for (auto& g : b->groups) tri_group_free(g);
   b->groups.clear();
+  if (b->d_jitems) hipFree(b->d_jitems);
+  b->d_jitems = nullptr;
+  b->jitems.clear();
   b->stale = true;
 }
 
@@ -1760,6 +1784,13 @@ static int batch_group(wtb_bse_batch* b) {
     const int rc = tri_group_prepare(g);
     if (rc) return rc;
   }
+  for (int i = 0; i < n; ++i)
+    if (b->grouped[i]) b->jitems.push_back(plan_jacobi_item(b->plans[i]));
+  if (!b->jitems.empty()) {
+    WTB_HIP(hipMalloc(&b->d_jitems, sizeof(JacobiItem) * b->jitems.size()));
+    WTB_HIP(hipMemcpy(b->d_jitems, b->jitems.data(), sizeof(JacobiItem) * b->jitems.size(),
+                      hipMemcpyHostToDevice));
+  }
   b->stale = false;
   return WTB_OK;
 }
@@ -1781,12 +1812,19 @@ static int batch_enqueue(wtb_bse_batch* b) {
     return WTB_OK;
   };
   int rc;
+  // stage (a) of the grouped matrices on the lanes, every model's Jacobi eigensolves in one
+  // batched launch, then stages (b')-(c) on the lanes again; the other plans run whole
   if ((rc = fork(b->ev_start))) return rc;
   for (int i = 0; i < n; ++i) {
     wtb_bse_plan* p = b->plans[i];
-    rc = b->grouped[i] ? plan_enqueue_pre(p, false) : plan_enqueue(p);
+    rc = b->grouped[i] ? plan_enqueue_pre_a(p) : plan_enqueue(p);
     if (rc) return rc;
   }
+  if ((rc = join())) return rc;
+  if (!b->jitems.empty() && (rc = launch_jacobi_batch(s, b->d_jitems, b->jitems))) return rc;
+  if ((rc = fork(b->ev_jac))) return rc;
+  for (int i = 0; i < n; ++i)
+    if (b->grouped[i] && (rc = plan_enqueue_pre(b->plans[i], false, true))) return rc;
   if ((rc = join())) return rc;
   for (const auto& g : b->groups)
     if ((rc = run_tridiag_group(s, g))) return rc;
@@ -1823,7 +1861,8 @@ int wtb_bse_batch_create(wtb_ctx* c, int n, const wtb_tb_model* models, const wt
     b->ev_done.push_back(e);
   }
   if (!rc && (hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming) != hipSuccess ||
-              hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming) != hipSuccess))
+              hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_jac, hipEventDisableTiming) != hipSuccess))
     rc = WTB_ERR_HIP;
   if (rc) {
     wtb_bse_batch_destroy(b);
@@ -1914,6 +1953,7 @@ int wtb_bse_batch_destroy(wtb_bse_batch* b) {
   hipStreamSynchronize(b->ctx->stream);
   batch_drop(b);
   if (b->ev_mid) hipEventDestroy(b->ev_mid);
+  if (b->ev_jac) hipEventDestroy(b->ev_jac);
   for (auto* p : b->plans) wtb_bse_plan_destroy(p);
   for (auto l : b->lanes) {
     hipStreamSynchronize(l);

@@ -31,6 +31,21 @@ int launch_hr_stack(hipStream_t s, const double* hop, const double* ihop, const 
 int launch_hk(hipStream_t s, const DevModel& dm, const double* d_kpt, int nk, zc* d_phase,
               zc* d_hk4);
 size_t jacobi_lds_bytes(int nw);
+// one model's Jacobi eigensolves (the arguments of launch_jacobi) for a batched launch
+struct JacobiItem {
+  const zc* hk;
+  int nk, nw;
+  double ef32, scs;
+  int nc, nv;
+  double* eall;
+  zc* uout;
+  int* nocp;
+  double* eigv;
+  zc* vec;
+  int* status;
+};
+// every model's k-points in launches of up to 64 models (d_items: the items on the device)
+int launch_jacobi_batch(hipStream_t s, const JacobiItem* d_items, const std::vector<JacobiItem>& items);
 int launch_jacobi(hipStream_t s, const zc* d_hk, int nk, int nw, double ef32, double scs, int nc,
                   int nv, double* d_eall, zc* d_uout, int* d_nocp, double* d_eigv, zc* d_vec,
                   int* d_status);

@@ -964,10 +964,12 @@ __device__ __forceinline__ void tri_her2k(const TriArgs<T>& a, int bx, int s0, i
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row0 = bi * TB + (wave >> 1) * 32;
   const int col0 = bj * TB + (wave & 1) * 32;
-  if (bi == bj && (wave >> 1) < (wave & 1)) return;   // strictly upper quarter of a diagonal tile
+  // the strictly upper quarter of a diagonal tile is never stored
+  const bool upper = bi == bj && (wave >> 1) < (wave & 1);
   const int lr = lane & 15, lk = lane >> 4;
   const int K = 2 * kp;
   if constexpr (sizeof(T) == sizeof(zc)) {
+    if (upper) return;
     d4 cr[2][2], ci[2][2];
     for (int x = 0; x < 2; ++x)
       for (int y = 0; y < 2; ++y) {
@@ -1011,37 +1013,56 @@ __device__ __forceinline__ void tri_her2k(const TriArgs<T>& a, int bx, int s0, i
           }
         }
   } else {
-    // precision form: FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32)
+    // precision form: FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32).  The
+    // tile's operands, [Vp | W] for its rows and [W | Vp] for its columns, go through LDS
+    // in FP32 once per workgroup, 32 of K at a time (rows padded against bank conflicts);
+    // the products run in the order of k.  Every wave takes part in the staging.
+    __shared__ zf As[32][TB + 1], Bs[32][TB + 1];
+    const int rl0 = bi * TB, cl0 = bj * TB;
+    const int rw = (wave >> 1) * 32, cw = (wave & 1) * 32;
     f4 cr[2][2], ci[2][2];
     for (int x = 0; x < 2; ++x)
       for (int y = 0; y < 2; ++y) {
         cr[x][y] = f4{0, 0, 0, 0};
         ci[x][y] = f4{0, 0, 0, 0};
       }
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      int k = k0 + lk;
-      float ar[2], ai[2], br[2], bi[2];
-      for (int x = 0; x < 2; ++x) {
-        int r = row0 + x * 16 + lr;
-        int c = col0 + x * 16 + lr;
+    for (int kb = 0; kb < K; kb += 32) {
+      if (kb > 0) __syncthreads();
+#pragma unroll 4
+      for (int u = 0; u < 32 * TB / 256; ++u) {
+        const int e = threadIdx.x + 256 * u, kk = e >> 6, cc = e & 63, k = kb + kk;
+        const int r = rl0 + cc, c = cl0 + cc;
         zc va = mkz(0, 0), vb = mkz(0, 0);
         if (k < K) {
           if (r < N) va = (k < kp) ? a.Vp[(size_t)k * N + r] : a.W[(size_t)(k - kp) * N + r];
           if (c < N) vb = (k < kp) ? a.W[(size_t)k * N + c] : a.Vp[(size_t)(k - kp) * N + c];
         }
-        ar[x] = (float)va.re;
-        ai[x] = (float)va.im;
-        br[x] = (float)vb.re;
-        bi[x] = (float)vb.im;
+        As[kk][cc] = zf{(float)va.re, (float)va.im};
+        Bs[kk][cc] = zf{(float)vb.re, (float)vb.im};
       }
-      for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y) {
-          cr[x][y] = mfma16x16x4f32(ar[x], br[y], cr[x][y]);
-          cr[x][y] = mfma16x16x4f32(ai[x], bi[y], cr[x][y]);
-          ci[x][y] = mfma16x16x4f32(ai[x], br[y], ci[x][y]);
-          ci[x][y] = mfma16x16x4f32(-ar[x], bi[y], ci[x][y]);
+      __syncthreads();
+      if (!upper) {
+        for (int k0 = 0; k0 < 32 && kb + k0 < K; k0 += 4) {
+          const int kk = k0 + lk;
+          float ar[2], ai[2], br[2], bi[2];
+          for (int x = 0; x < 2; ++x) {
+            const zf va = As[kk][rw + x * 16 + lr], vb = Bs[kk][cw + x * 16 + lr];
+            ar[x] = va.re;
+            ai[x] = va.im;
+            br[x] = vb.re;
+            bi[x] = vb.im;
+          }
+          for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y) {
+              cr[x][y] = mfma16x16x4f32(ar[x], br[y], cr[x][y]);
+              cr[x][y] = mfma16x16x4f32(ai[x], bi[y], cr[x][y]);
+              ci[x][y] = mfma16x16x4f32(ai[x], br[y], ci[x][y]);
+              ci[x][y] = mfma16x16x4f32(-ar[x], bi[y], ci[x][y]);
+            }
         }
+      }
     }
+    if (upper) return;
     for (int x = 0; x < 2; ++x)
       for (int y = 0; y < 2; ++y)
         for (int q = 0; q < 4; ++q) {

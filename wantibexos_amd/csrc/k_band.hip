@@ -752,20 +752,30 @@ __global__ __launch_bounds__(256) void k_hmm_b(int m, const T* __restrict__ A, i
       for (int a = 0; a < 2; ++a) av[a] = As[k][32 * w + 16 * a + i16];
 #pragma unroll
       for (int b = 0; b < CT; ++b) yv[b] = Ys[k][16 * b + i16];
+      // (ar + i ai)(yr + i yi) = (ar yr - ai yi) + i (ar yi + ai yr).  Each accumulator
+      // takes its two products in this order; the first products of all accumulators go
+      // before the second ones, so that no MFMA waits on the one just issued
 #pragma unroll
       for (int b = 0; b < CT; ++b)
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-          // (ar + i ai)(yr + i yi) = (ar yr - ai yi) + i (ar yi + ai yr)
           if constexpr (F64) {
             cr[a][b] = mfma16x16x4(av[a].x, yv[b].x, cr[a][b]);
-            cr[a][b] = mfma16x16x4(-av[a].y, yv[b].y, cr[a][b]);
             ci[a][b] = mfma16x16x4(av[a].x, yv[b].y, ci[a][b]);
-            ci[a][b] = mfma16x16x4(av[a].y, yv[b].x, ci[a][b]);
           } else {
             cr[a][b] = mfma16x16x4f32(av[a].x, yv[b].x, cr[a][b]);
-            cr[a][b] = mfma16x16x4f32(-av[a].y, yv[b].y, cr[a][b]);
             ci[a][b] = mfma16x16x4f32(av[a].x, yv[b].y, ci[a][b]);
+          }
+        }
+#pragma unroll
+      for (int b = 0; b < CT; ++b)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          if constexpr (F64) {
+            cr[a][b] = mfma16x16x4(-av[a].y, yv[b].y, cr[a][b]);
+            ci[a][b] = mfma16x16x4(av[a].y, yv[b].x, ci[a][b]);
+          } else {
+            cr[a][b] = mfma16x16x4f32(-av[a].y, yv[b].y, cr[a][b]);
             ci[a][b] = mfma16x16x4f32(av[a].y, yv[b].x, ci[a][b]);
           }
         }
@@ -872,13 +882,20 @@ __global__ __launch_bounds__(256, 4) void k_her2k_full(T* __restrict__ A, int ld
           af[x] = As[kk][ciw + x * 16 + lr];
           bf[x] = Bs[kk][rjw + x * 16 + lr];
         }
+        // first products of all accumulators, then the second ones (the order within
+        // each accumulator is kept)
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
           for (int y = 0; y < 2; ++y) {
             cr[x][y] = mfma16x16x4f32(af[x].re, bf[y].re, cr[x][y]);
-            cr[x][y] = mfma16x16x4f32(af[x].im, bf[y].im, cr[x][y]);
             ci[x][y] = mfma16x16x4f32(af[x].im, bf[y].re, ci[x][y]);
+          }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            cr[x][y] = mfma16x16x4f32(af[x].im, bf[y].im, cr[x][y]);
             ci[x][y] = mfma16x16x4f32(-af[x].re, bf[y].im, ci[x][y]);
           }
       }
@@ -901,23 +918,21 @@ __global__ __launch_bounds__(256, 4) void k_her2k_full(T* __restrict__ A, int ld
   for (int k0 = 0; k0 < K; k0 += 4) {
     zc an[2], bn[2];
     if (k0 + 4 < K) oload(k0 + 4, an, bn);
+    // a * conj(b): the first products of all accumulators, then the second ones (the order
+    // within each accumulator is kept)
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
       for (int y = 0; y < 2; ++y) {
-        // a * conj(b)
-        if constexpr (F64) {
-          cr[x][y] = mfma16x16x4(av[x].re, bv[y].re, cr[x][y]);
-          cr[x][y] = mfma16x16x4(av[x].im, bv[y].im, cr[x][y]);
-          ci[x][y] = mfma16x16x4(av[x].im, bv[y].re, ci[x][y]);
-          ci[x][y] = mfma16x16x4(-av[x].re, bv[y].im, ci[x][y]);
-        } else {
-          const float ar = (float)av[x].re, ai = (float)av[x].im, br = (float)bv[y].re, bi = (float)bv[y].im;
-          cr[x][y] = mfma16x16x4f32(ar, br, cr[x][y]);
-          cr[x][y] = mfma16x16x4f32(ai, bi, cr[x][y]);
-          ci[x][y] = mfma16x16x4f32(ai, br, ci[x][y]);
-          ci[x][y] = mfma16x16x4f32(-ar, bi, ci[x][y]);
-        }
+        cr[x][y] = mfma16x16x4(av[x].re, bv[y].re, cr[x][y]);
+        ci[x][y] = mfma16x16x4(av[x].im, bv[y].re, ci[x][y]);
+      }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        cr[x][y] = mfma16x16x4(av[x].im, bv[y].im, cr[x][y]);
+        ci[x][y] = mfma16x16x4(-av[x].re, bv[y].im, ci[x][y]);
       }
     if (k0 + 4 < K) {
 #pragma unroll

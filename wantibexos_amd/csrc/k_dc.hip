@@ -788,6 +788,8 @@ __global__ __launch_bounds__(256) void k_dcv_rotate(DCArgs a) {
   }
 }
 
+constexpr int kDcvSteps = 8;
+
 // Z(:, lo + rank(j)) = Z2(:, lo + cmap(0..K-1)) U(0..K-1, j) for the non-deflated
 // slots j < K; 64 x 64 output tiles, four waves of 32 x 32 on v_mfma_f64_16x16x4
 __global__ __launch_bounds__(256) void k_dcv_gemm(DCArgs a) {
@@ -806,22 +808,45 @@ __global__ __launch_bounds__(256) void k_dcv_gemm(DCArgs a) {
   d4 acc[2][2];
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) acc[x][y] = d4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const int k = k0 + lk;
-    double av[2] = {0.0, 0.0}, bv[2] = {0.0, 0.0};
-    if (k < K) {
-      const double* zc_ = a.Z2 + (size_t)(lo + a.cmap[lo + k]) * N + lo;
+  // K in groups of kDcvSteps MFMA steps: a group's operand loads are all issued before its
+  // products, and the next group's column map is loaded while they run (each accumulator
+  // still takes its steps in ascending k)
+  int cm[kDcvSteps];
+#pragma unroll
+  for (int t = 0; t < kDcvSteps; ++t) {
+    const int k = 4 * t + lk;
+    cm[t] = (k < K) ? a.cmap[lo + k] : -1;
+  }
+  for (int k0 = 0; k0 < K; k0 += 4 * kDcvSteps) {
+    double av[kDcvSteps][2], bv[kDcvSteps][2];
+#pragma unroll
+    for (int t = 0; t < kDcvSteps; ++t) {
+      const int k = k0 + 4 * t + lk;
+      const double* zc_ = a.Z2 + (size_t)(lo + (cm[t] < 0 ? 0 : cm[t])) * N + lo;
+#pragma unroll
       for (int x = 0; x < 2; ++x) {
         const int r = rb + x * 16 + lr;
-        if (r < n) av[x] = zc_[r];
+        av[t][x] = (cm[t] >= 0 && r < n) ? zc_[r] : 0.0;
       }
+#pragma unroll
       for (int y = 0; y < 2; ++y) {
         const int c = cb + y * 16 + lr;
-        if (c < K) bv[y] = a.U[(size_t)(lo + c) * N + lo + k];
+        bv[t][y] = (k < K && c < K) ? a.U[(size_t)(lo + c) * N + lo + k] : 0.0;
       }
     }
-    for (int x = 0; x < 2; ++x)
-      for (int y = 0; y < 2; ++y) acc[x][y] = mfma16x16x4(av[x], bv[y], acc[x][y]);
+#pragma unroll
+    for (int t = 0; t < kDcvSteps; ++t) {
+      const int k = k0 + 4 * kDcvSteps + 4 * t + lk;
+      cm[t] = (k < K) ? a.cmap[lo + k] : -1;
+    }
+#pragma unroll
+    for (int t = 0; t < kDcvSteps; ++t) {
+      if (k0 + 4 * t >= K) break;                  // the same steps as one step per 4 of K
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = mfma16x16x4(av[t][x], bv[t][y], acc[x][y]);
+    }
   }
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) {

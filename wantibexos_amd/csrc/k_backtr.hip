@@ -12,6 +12,7 @@
 namespace wtb {
 
 #define BT_NB 32      // reflectors per panel (the reduction's panel width)
+constexpr int kBtSteps = 4;   // V^H X: MFMA steps (4 rows each) per group of loads
 
 template <class T>
 __device__ inline zc vref(const T* H, int ld, int r, int j) {
@@ -99,22 +100,32 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
     yr[y] = d4{0, 0, 0, 0};
     yi[y] = d4{0, 0, 0, 0};
   }
-  for (int r0 = rbeg; r0 < rend; r0 += 4) {
-    const int r = r0 + lk;
-    zc v = mkz(0, 0);
-    const int c = cb + lr;
-    if (r < rend && c < kp) v = vref(H, ld, r, p + c);
-    zc xv[2];
-    for (int y = 0; y < 2; ++y) {
-      const int col = nb0 + y * 16 + lr;
-      xv[y] = (r < rend && col < m) ? X[(size_t)r + (size_t)ldx * col] : mkz(0, 0);
+  // rows in groups of kBtSteps MFMA steps: the group's loads are all issued before its
+  // products (each accumulator still takes its steps in ascending row order)
+  const int c = cb + lr;
+  for (int g0 = rbeg; g0 < rend; g0 += 4 * kBtSteps) {
+    zc v[kBtSteps], xv[kBtSteps][2];
+#pragma unroll
+    for (int t = 0; t < kBtSteps; ++t) {
+      const int r = g0 + 4 * t + lk;
+      v[t] = (r < rend && c < kp) ? vref(H, ld, r, p + c) : mkz(0, 0);
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int col = nb0 + y * 16 + lr;
+        xv[t][y] = (r < rend && col < m) ? X[(size_t)r + (size_t)ldx * col] : mkz(0, 0);
+      }
     }
-    for (int y = 0; y < 2; ++y) {
-      // conj(v) x = (vr xr + vi xi) + i (vr xi - vi xr)
-      yr[y] = mfma16x16x4(v.re, xv[y].re, yr[y]);
-      yr[y] = mfma16x16x4(v.im, xv[y].im, yr[y]);
-      yi[y] = mfma16x16x4(v.re, xv[y].im, yi[y]);
-      yi[y] = mfma16x16x4(-v.im, xv[y].re, yi[y]);
+#pragma unroll
+    for (int t = 0; t < kBtSteps; ++t) {
+      if (g0 + 4 * t >= rend) break;               // the same steps as one step per 4 rows
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        // conj(v) x = (vr xr + vi xi) + i (vr xi - vi xr)
+        yr[y] = mfma16x16x4(v[t].re, xv[t][y].re, yr[y]);
+        yr[y] = mfma16x16x4(v[t].im, xv[t][y].im, yr[y]);
+        yi[y] = mfma16x16x4(v[t].re, xv[t][y].im, yi[y]);
+        yi[y] = mfma16x16x4(-v[t].im, xv[t][y].re, yi[y]);
+      }
     }
   }
   zc* out = Ypart + (size_t)blockIdx.y * BT_NB * m;
@@ -172,24 +183,38 @@ __global__ __launch_bounds__(256) void k_bt_update(const T* __restrict__ H, int 
       ar[x][y] = d4{0, 0, 0, 0};
       ai[x][y] = d4{0, 0, 0, 0};
     }
-  for (int k0 = 0; k0 < kp; k0 += 4) {
-    const int k = k0 + lk;
-    zc v[2], yv[2];
-    for (int x = 0; x < 2; ++x) {
-      const int r = rb + x * 16 + lr;
-      v[x] = (k < kp && r < N) ? vref(H, ld, r, p + k) : mkz(0, 0);
-    }
-    for (int y = 0; y < 2; ++y) {
-      const int col = cb + y * 16 + lr;
-      yv[y] = (k < kp && col < m) ? Y[(size_t)k * m + col] : mkz(0, 0);
-    }
-    for (int x = 0; x < 2; ++x)
-      for (int y = 0; y < 2; ++y) {
-        ar[x][y] = mfma16x16x4(v[x].re, yv[y].re, ar[x][y]);
-        ar[x][y] = mfma16x16x4(-v[x].im, yv[y].im, ar[x][y]);
-        ai[x][y] = mfma16x16x4(v[x].re, yv[y].im, ai[x][y]);
-        ai[x][y] = mfma16x16x4(v[x].im, yv[y].re, ai[x][y]);
+  // the panel's reflectors in two halves of four MFMA steps: a half's loads are all issued
+  // before its products (each accumulator still takes its steps in ascending k)
+#pragma unroll
+  for (int h = 0; h < BT_NB / 16; ++h) {
+    zc v[4][2], yv[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = 16 * h + 4 * t + lk;
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int r = rb + x * 16 + lr;
+        v[t][x] = (k < kp && r < N) ? vref(H, ld, r, p + k) : mkz(0, 0);
       }
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int col = cb + y * 16 + lr;
+        yv[t][y] = (k < kp && col < m) ? Y[(size_t)k * m + col] : mkz(0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (16 * h + 4 * t >= kp) break;             // the same steps as one step per 4 of k
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          ar[x][y] = mfma16x16x4(v[t][x].re, yv[t][y].re, ar[x][y]);
+          ar[x][y] = mfma16x16x4(-v[t][x].im, yv[t][y].im, ar[x][y]);
+          ai[x][y] = mfma16x16x4(v[t][x].re, yv[t][y].im, ai[x][y]);
+          ai[x][y] = mfma16x16x4(v[t][x].im, yv[t][y].re, ai[x][y]);
+        }
+    }
   }
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) {

@@ -280,24 +280,47 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
       overlap_block<F32>(vec, ba, kr0, kc0, ba.nv, 0, rv_, b2 / bv_, b2 % bv_, Ov, rv_);
     }
   }
-  __syncthreads();
   const int side = TK * ba.ncv;
-  const int I0 = kr0 * ba.ncv, J0 = kc0 * ba.ncv;
+  // per row and per column of the tile, once per workgroup: the W-table index splits as
+  // RW[i] - CW[j] (it is linear in the two k-points' grid coordinates), and the overlaps'
+  // row and column offsets; the element loop then needs no integer division
+  int* RW = reinterpret_cast<int*>(Ov + rv_ * rv_);
+  int* CW = RW + side;
+  int* ROc = CW + side;
+  int* COc = ROc + side;
+  int* ROv = COc + side;
+  int* COv = ROv + side;
   const int c0 = ba.n2 * ba.n3;
+  for (int x = threadIdx.x; x < side; x += 256) {
+    const int kl = x / ba.ncv, cx = (x % ba.ncv) / ba.nv, vx = x % ba.nv;
+    const int kr = kr0 + kl, kc = kc0 + kl;
+    RW[x] = ((kr / c0) * ba.D2 + (kr / ba.n3) % ba.n2) * ba.D3 + kr % ba.n3 +
+            ((ba.n1 - 1) * ba.D2 + (ba.n2 - 1)) * ba.D3 + (ba.n3 - 1);
+    CW[x] = ((kc / c0) * ba.D2 + (kc / ba.n3) % ba.n2) * ba.D3 + kc % ba.n3;
+    ROc[x] = (kl * ba.nc + cx) * rc_;
+    COc[x] = kl * ba.nc + cx;
+    ROv[x] = (kl * ba.nv + vx) * rv_;
+    COv[x] = kl * ba.nv + vx;
+  }
+  __syncthreads();
+  const int I0 = kr0 * ba.ncv, J0 = kc0 * ba.ncv;
   const double vq = xa ? *dvq : 0.0;
-  for (int e = threadIdx.x; e < side * side; e += 256) {
-    int i = e % side, j = e / side;
-    int I = I0 + i, J = J0 + j;
+  // element e = i + side j, i the row (consecutive threads, consecutive addresses)
+  const int di = 256 % side, dj = 256 / side;
+  int i = threadIdx.x % side, j = threadIdx.x / side;
+  for (; j < side; i += di, j += dj) {
+    if (i >= side) {
+      i -= side;
+      ++j;
+      if (j >= side) break;
+    }
+    const int I = I0 + i, J = J0 + j;
     if (I >= ba.N || J >= ba.N || I < J) continue;
-    int kl = i / ba.ncv, ci = (i % ba.ncv) / ba.nv, vi = i % ba.nv;
-    int kl2 = j / ba.ncv, cj = (j % ba.ncv) / ba.nv, vj = j % ba.nv;
-    int k = kr0 + kl, k2 = kc0 + kl2;
-    int d1 = k / c0 - k2 / c0, d2 = (k / ba.n3) % ba.n2 - (k2 / ba.n3) % ba.n2,
-        d3 = k % ba.n3 - k2 % ba.n3;
-    int widx = ((d1 + ba.n1 - 1) * ba.D2 + (d2 + ba.n2 - 1)) * ba.D3 + (d3 + ba.n3 - 1);
-    double wv = wtab[widx];
+    const double wv = wtab[RW[i] - CW[j]];
     zc val;
     if (I == J) {
+      const int kl = i / ba.ncv, ci = (i % ba.ncv) / ba.nv, vi = i % ba.nv;
+      const int k = kr0 + kl;
       double ec = eigv[(size_t)k * ba.nb + ba.nv + ci], ev = eigv[(size_t)k * ba.nb + vi];
       if (fdeh) {
         // finite temperature: (Ec - Ev) + V(0) f_I (bse_subs_temp.f90:105-108); with finite Q
@@ -311,8 +334,8 @@ __global__ __launch_bounds__(256) void k_bse_build(BuildArgs ba, const double* _
         if (xa) val = val - (vq * xa[I]) * conj(xa[I]);
       }
     } else {
-      zc oc = Oc[(kl * ba.nc + ci) * rc_ + (kl2 * ba.nc + cj)];
-      zc ov = Ov[(kl * ba.nv + vi) * rv_ + (kl2 * ba.nv + vj)];
+      zc oc = Oc[ROc[i] + COc[j]];
+      zc ov = Ov[ROv[i] + COv[j]];
       val = (wv * oc) * ov;
       // finite Q: the reference's (J, I) element subtracts V(q) a_J conj(a_I)
       // (bse_subs_kpath.f90:178-191); conjugated: V(q) conj(a_J) a_I
@@ -358,7 +381,9 @@ int launch_bse_build(hipStream_t s, int nk, int nc, int nv, int nw, const int ng
   int side = (nk + tk - 1) / tk;
   ba.ntiles_side = side;
   int ntiles = side * (side + 1) / 2;
-  size_t lds = sizeof(zc) * ((size_t)tk * nc * tk * nc + (size_t)tk * nv * tk * nv);
+  // the two overlap blocks, then six int tables of the tile's side (rows and columns)
+  size_t lds = sizeof(zc) * ((size_t)tk * nc * tk * nc + (size_t)tk * nv * tk * nv) +
+               sizeof(int) * 6 * (size_t)tk * nc * nv;
   if (d_H)
     hipLaunchKernelGGL(k_bse_build<false>, dim3(ntiles), dim3(256), lds, s, ba, d_wtab, d_eigv, d_vec,
                        d_H, (zf*)nullptr, d_fdeh, d_xa, d_vq);

@@ -243,3 +243,22 @@ def test_single_particle_table(ctx, name):
     assert np.array_equal(a[:, :6], b[:, :6])
     scale = np.maximum(np.abs(b).max(axis=0), 1e-30)
     assert (np.abs(a - b) / scale).max() < 1e-6
+
+
+@pytest.mark.parametrize("nc,nv", [(3, 1), (1, 3), (3, 3), (4, 3)])
+def test_assembly_band_windows(ctx, nc, nv):
+    """The assembly on band windows whose tile side (TK nc nv: 48, 48, 63, 60 here) does not
+    divide the workgroup's 256 threads, so that its element walk carries from row to
+    column; lower triangle against the oracle's matrix (bse_subs.f90:2-124)."""
+    from wantibexos_amd import bse, tbmodel
+    g = (4, 4, 1) if nc * nv < 9 else (5, 5, 1)
+    m = tbmodel.hex2d(17 + nc + 3 * nv, 8, g)
+    kw = dict(nc=nc, nv=nv, coultype="V2DRK", r0=40.0, ktol=0.001, cshift=0.05, ebse0=0.0, ebsef=6.0,
+              numbse=1001, dtfull=True)
+    print(f"assembly: hex2d nw=8 grid {g} nc={nc} nv={nv}", flush=True)
+    with bse.BSEPlan(ctx, m, bse.BSEParams(ngrid=g, cpol=True, **kw)) as plan:
+        plan.build()
+        H = plan.fetch("hbse")
+    o = O.run_pipeline(m, g, **kw)
+    assert H.shape == o["H"].shape
+    assert np.abs(np.tril(H) - np.tril(o["H"])).max() < 1e-12

@@ -860,6 +860,91 @@ __global__ __launch_bounds__(256) void k_dcv_gemm(DCArgs a) {
     }
 }
 
+// The same product for the large merges: 128 x 128 output tiles (four waves of 64 x 64)
+// with each 16-deep slice of K staged once per workgroup in LDS, so the workgroup reads
+// its rows of Z2 and columns of U once instead of once per wave pair (half the traffic of
+// the 64 x 64 tiles).  Every output block takes its K steps in the same order as there, so
+// the results are bitwise those of k_dcv_gemm.
+constexpr int kDcvT = 128;     // output tile edge
+constexpr int kDcvK = 16;      // K per LDS slice (four MFMA steps)
+__global__ __launch_bounds__(256) void k_dcv_gemm128(DCArgs a) {
+  const int node = blockIdx.z;
+  const DCNode nd = a.nodes[node];
+  const int n = nd.hi - nd.lo, K = a.K[node], lo = nd.lo;
+  const int r0 = blockIdx.x * kDcvT, j0 = blockIdx.y * kDcvT;
+  if (r0 >= n || j0 >= K) return;
+  if (a.rank[lo + j0] >= a.zcols) return;          // no wanted column in the tile
+  const size_t N = a.N;
+  // one row of padding: the four k rows a wave reads together fall in different banks
+  __shared__ double As[kDcvK][kDcvT + 1];
+  __shared__ double Bs[kDcvK][kDcvT + 1];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int rw = (wave >> 1) * 64, cw = (wave & 1) * 64;      // the wave's 64 x 64 block
+  // loaders: A(r, k) for r = t % 128 and k = t / 128 + 2u; B(k, c) for k = t % 16 and
+  // c = t / 16 + 16u
+  const int ar = t & (kDcvT - 1), ak = t >> 7;
+  const int bk = t & (kDcvK - 1), bc = t >> 4;
+  constexpr int NA = kDcvT * kDcvK / 256, NB = kDcvT * kDcvK / 256;
+  double ra[NA], rb[NB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int k = k0 + ak + 2 * u;
+      const int r = r0 + ar;
+      ra[u] = (k < K && r < n) ? a.Z2[(size_t)(lo + a.cmap[lo + k]) * N + lo + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int k = k0 + bk;
+      const int c = j0 + bc + 16 * u;
+      rb[u] = (k < K && c < K) ? a.U[(size_t)(lo + c) * N + lo + k] : 0.0;
+    }
+  };
+  d4 acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = d4{0, 0, 0, 0};
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += kDcvK) {
+    __syncthreads();                                 // the previous slice is consumed
+#pragma unroll
+    for (int u = 0; u < NA; ++u) As[ak + 2 * u][ar] = ra[u];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) Bs[bk][bc + 16 * u] = rb[u];
+    __syncthreads();
+    if (k0 + kDcvK < K) gload(k0 + kDcvK);
+#pragma unroll
+    for (int s4 = 0; s4 < kDcvK / 4; ++s4) {
+      if (k0 + 4 * s4 >= K) break;                   // the same steps as one step per 4 of K
+      const int kr = 4 * s4 + lk;
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) av[x] = As[kr][rw + 16 * x + lr];
+#pragma unroll
+      for (int y = 0; y < 4; ++y) bv[y] = Bs[kr][cw + 16 * y + lr];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = mfma16x16x4(av[x], bv[y], acc[x][y]);
+    }
+  }
+#pragma unroll
+  for (int y = 0; y < 4; ++y) {
+    const int c = j0 + cw + 16 * y + lr;
+    if (c >= K) continue;
+    double* dst = a.Z + (size_t)(lo + a.rank[lo + c]) * N + lo;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rw + 16 * x + lk + 4 * q;     // f64 MFMA: register q holds row lk + 4q
+        if (r < n) dst[r] = acc[x][y][q];
+      }
+  }
+}
+
 // deflated slots j >= K: the gathered, rotated column moves to its sorted position
 __global__ __launch_bounds__(256) void k_dcv_copy(DCArgs a) {
   const int node = blockIdx.y;
@@ -1050,8 +1135,13 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
     if (d_Z) {
       hipLaunchKernelGGL(k_dcv_gather, dim3(maxn, cnt), dim3(256), 0, s, a);
       hipLaunchKernelGGL(k_dcv_rotate, dim3((maxn + 255) / 256, cnt), dim3(256), 0, s, a);
-      const int nt64 = (maxn + 63) / 64;
-      hipLaunchKernelGGL(k_dcv_gemm, dim3(nt64, nt64, cnt), dim3(256), 0, s, a);
+      if (maxn > 512) {
+        const int nt = (maxn + kDcvT - 1) / kDcvT;
+        hipLaunchKernelGGL(k_dcv_gemm128, dim3(nt, nt, cnt), dim3(256), 0, s, a);
+      } else {
+        const int nt64 = (maxn + 63) / 64;
+        hipLaunchKernelGGL(k_dcv_gemm, dim3(nt64, nt64, cnt), dim3(256), 0, s, a);
+      }
       hipLaunchKernelGGL(k_dcv_copy, dim3(maxn, cnt), dim3(256), 0, s, a);
     }
     WTB_LAUNCH_CHECK();

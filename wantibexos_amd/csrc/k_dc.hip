@@ -881,23 +881,25 @@ __global__ __launch_bounds__(256) void k_dcv_gemm128(DCArgs a) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int rw = (wave >> 1) * 64, cw = (wave & 1) * 64;      // the wave's 64 x 64 block
-  // loaders: A(r, k) for r = t % 128 and k = t / 128 + 2u; B(k, c) for k = t % 16 and
-  // c = t / 16 + 16u
-  const int ar = t & (kDcvT - 1), ak = t >> 7;
-  const int bk = t & (kDcvK - 1), bc = t >> 4;
+  // loaders: A(r, k) for r = t % T and k = t / T + (256 / T) u; B(k, c) for k = t % KS and
+  // c = t / KS + (256 / KS) u (T the tile edge, KS the slice depth)
+  static_assert(256 % kDcvT == 0 && 256 % kDcvK == 0 && (kDcvT * kDcvK) % 256 == 0 && kDcvK % 4 == 0,
+                "loader shapes");
+  const int ar = t % kDcvT, ak = t / kDcvT;
+  const int bk = t % kDcvK, bc = t / kDcvK;
   constexpr int NA = kDcvT * kDcvK / 256, NB = kDcvT * kDcvK / 256;
   double ra[NA], rb[NB];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
-      const int k = k0 + ak + 2 * u;
+      const int k = k0 + ak + (256 / kDcvT) * u;
       const int r = r0 + ar;
       ra[u] = (k < K && r < n) ? a.Z2[(size_t)(lo + a.cmap[lo + k]) * N + lo + r] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       const int k = k0 + bk;
-      const int c = j0 + bc + 16 * u;
+      const int c = j0 + bc + (256 / kDcvK) * u;
       rb[u] = (k < K && c < K) ? a.U[(size_t)(lo + c) * N + lo + k] : 0.0;
     }
   };
@@ -910,9 +912,9 @@ __global__ __launch_bounds__(256) void k_dcv_gemm128(DCArgs a) {
   for (int k0 = 0; k0 < K; k0 += kDcvK) {
     __syncthreads();                                 // the previous slice is consumed
 #pragma unroll
-    for (int u = 0; u < NA; ++u) As[ak + 2 * u][ar] = ra[u];
+    for (int u = 0; u < NA; ++u) As[ak + (256 / kDcvT) * u][ar] = ra[u];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) Bs[bk][bc + 16 * u] = rb[u];
+    for (int u = 0; u < NB; ++u) Bs[bk][bc + (256 / kDcvK) * u] = rb[u];
     __syncthreads();
     if (k0 + kDcvK < K) gload(k0 + kDcvK);
 #pragma unroll

@@ -1137,7 +1137,7 @@ int run_dc(hipStream_t s, void* work, const DCTree& tree, const double* d_d, con
     if (d_Z) {
       hipLaunchKernelGGL(k_dcv_gather, dim3(maxn, cnt), dim3(256), 0, s, a);
       hipLaunchKernelGGL(k_dcv_rotate, dim3((maxn + 255) / 256, cnt), dim3(256), 0, s, a);
-      if (maxn > 512) {
+      if (maxn > 1024) {   // below that, 128-wide tiles leave too few workgroups
         const int nt = (maxn + kDcvT - 1) / kDcvT;
         hipLaunchKernelGGL(k_dcv_gemm128, dim3(nt, nt, cnt), dim3(256), 0, s, a);
       } else {

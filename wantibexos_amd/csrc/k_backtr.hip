@@ -11,7 +11,12 @@
 
 namespace wtb {
 
-#define BT_NB 32      // reflectors per panel (the reduction's panel width)
+// reflectors per block: consecutive reflectors in any grouping form a block I - V T V^H,
+// so the blocks need not match the reduction's panels; 64 (two 32-column panels) halves
+// the launches, whose fixed costs dominate each block's time
+#define BT_NB 64
+constexpr int kBtVR = 16;                                    // T factor: V rows per LDS slice
+constexpr int kBtPairs = (BT_NB * (BT_NB - 1) / 2 + 255) / 256;   // Gram pairs per thread
 constexpr int kBtSteps = 4;   // V^H X: MFMA steps (4 rows each) per group of loads
 
 template <class T>
@@ -22,46 +27,76 @@ __device__ inline zc vref(const T* H, int ld, int r, int j) {
   return ldz(H[(size_t)r + (size_t)ld * j]);
 }
 
-// T factor of panel blockIdx.x (reflectors p .. p+kp-1), one workgroup per panel
+// Gram matrix of block blockIdx.x (reflectors p .. p+kp-1), G(i', i) = v_{p+i'}^H v_{p+i}
+// for i' < i, in kBtSplit row ranges (blockIdx.y), each summed by its own workgroup into
+// Gp; pair t of the strictly upper triangle (row i holds pairs i(i-1)/2 .. i(i+1)/2 - 1)
+// is thread t % 256, slot t / 256
+constexpr int kBtSplit = 8;
+constexpr int kBtPairSlots = 256 * kBtPairs;
+__device__ inline void bt_pair(int t, int& pi, int& pj) {
+  int i = 1;
+  while (t >= i && i < BT_NB) {
+    t -= i;
+    ++i;
+  }
+  pi = t;
+  pj = i;
+}
 template <class T>
-__global__ __launch_bounds__(256) void k_bt_tfactor(const T* __restrict__ H, int ld, int N,
-                                                    const zc* __restrict__ tau, zc* __restrict__ Tf) {
-  __shared__ zc Vs[32][BT_NB + 1];
+__global__ __launch_bounds__(256) void k_bt_gram(const T* __restrict__ H, int ld, int N, zc* __restrict__ Gp) {
+  __shared__ zc Vs[kBtVR][BT_NB + 1];
+  const int p = blockIdx.x * BT_NB;
+  const int kp = min(BT_NB, N - 1 - p);
+  const int tid = threadIdx.x;
+  const int npair = kp * (kp - 1) / 2;
+  const int rtot = N - 1 - p;
+  const int per = ((rtot + kBtSplit - 1) / kBtSplit + kBtVR - 1) / kBtVR * kBtVR;
+  const int rlo = p + 1 + (int)blockIdx.y * per, rhi = min(N, rlo + per);
+  zc acc[kBtPairs];
+  int pi[kBtPairs], pj[kBtPairs];
+  for (int s = 0; s < kBtPairs; ++s) {
+    acc[s] = mkz(0, 0);
+    bt_pair(tid + 256 * s, pi[s], pj[s]);
+  }
+  for (int r0 = rlo; r0 < rhi; r0 += kBtVR) {
+    for (int e = tid; e < kBtVR * BT_NB; e += 256) {
+      const int rl = e % kBtVR, c = e / kBtVR;
+      const int r = r0 + rl;
+      Vs[rl][c] = (r < rhi && c < kp) ? vref(H, ld, r, p + c) : mkz(0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kBtPairs; ++s) {
+      if (tid + 256 * s >= npair) continue;
+      for (int rl = 0; rl < kBtVR; ++rl) acc[s] = fma_conj(Vs[rl][pi[s]], Vs[rl][pj[s]], acc[s]);
+    }
+    __syncthreads();
+  }
+  zc* out = Gp + ((size_t)blockIdx.x * kBtSplit + blockIdx.y) * kBtPairSlots;
+  for (int s = 0; s < kBtPairs; ++s)
+    if (tid + 256 * s < npair) out[tid + 256 * s] = acc[s];
+}
+
+// T factor of block blockIdx.x from its Gram partials (summed in range order), one
+// workgroup per block
+__global__ __launch_bounds__(256) void k_bt_tfactor(int N, const zc* __restrict__ tau, const zc* __restrict__ Gp,
+                                                    zc* __restrict__ Tf) {
   __shared__ zc G[BT_NB][BT_NB + 1];
   __shared__ zc Ts[BT_NB][BT_NB + 1];
   const int p = blockIdx.x * BT_NB;
   const int kp = min(BT_NB, N - 1 - p);
   const int tid = threadIdx.x;
-  // G(i', i) = v_{p+i'}^H v_{p+i} for i' < i: pairs spread over the threads
   const int npair = kp * (kp - 1) / 2;
-  zc acc[2] = {mkz(0, 0), mkz(0, 0)};
-  int pi[2], pj[2];
-  for (int s = 0; s < 2; ++s) {
-    int t = tid + 256 * s;
-    // pair t -> (i' < i): row i has i pairs
-    int i = 1;
-    while (t >= i && i < BT_NB) {
-      t -= i;
-      ++i;
-    }
-    pi[s] = t;
-    pj[s] = i;
+  const zc* gp = Gp + (size_t)blockIdx.x * kBtSplit * kBtPairSlots;
+  for (int s = 0; s < kBtPairs; ++s) {
+    const int t = tid + 256 * s;
+    if (t >= npair) continue;
+    zc g = mkz(0, 0);
+    for (int q = 0; q < kBtSplit; ++q) g += gp[(size_t)q * kBtPairSlots + t];
+    int pi, pj;
+    bt_pair(t, pi, pj);
+    G[pi][pj] = g;
   }
-  for (int r0 = p + 1; r0 < N; r0 += 32) {
-    for (int e = tid; e < 32 * BT_NB; e += 256) {
-      const int rl = e & 31, c = e >> 5;
-      const int r = r0 + rl;
-      Vs[rl][c] = (r < N && c < kp) ? vref(H, ld, r, p + c) : mkz(0, 0);
-    }
-    __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-      if (tid + 256 * s >= npair) continue;
-      for (int rl = 0; rl < 32; ++rl) acc[s] = fma_conj(Vs[rl][pi[s]], Vs[rl][pj[s]], acc[s]);
-    }
-    __syncthreads();
-  }
-  for (int s = 0; s < 2; ++s)
-    if (tid + 256 * s < npair) G[pi[s]][pj[s]] = acc[s];
   __syncthreads();
   // T column by column; thread r < i computes T(r, i) = -tau_i sum_{k=r}^{i-1} T(r,k) G(k,i)
   for (int i = 0; i < kp; ++i) {
@@ -83,7 +118,7 @@ __global__ __launch_bounds__(256) void k_bt_tfactor(const T* __restrict__ H, int
 }
 
 // Ypart[chunk][c][col] = sum over the chunk's rows of conj(V(r, c)) X(r, col).
-// Workgroup: 32 reflectors x 64 columns; wave w: reflectors 16 (w >> 1), columns 32 (w & 1).
+// Workgroup: 64 reflectors x 64 columns; wave w: reflectors 16w .. 16w + 15, all 64 columns.
 template <class T>
 __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld, int N, int p, int kp,
                                                 const zc* __restrict__ X, int ldx, int m, int chunk,
@@ -93,10 +128,12 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
   const int rend = min(N, rbeg + chunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int cb = (wave >> 1) * 16;           // reflector block
-  const int nb0 = col0 + (wave & 1) * 32;    // column block
-  d4 yr[2], yi[2];
-  for (int y = 0; y < 2; ++y) {
+  static_assert(BT_NB == 64, "one reflector block of 16 per wave");
+  constexpr int YC = 4;                      // column blocks of 16 per wave
+  const int cb = wave * 16;                  // reflector block
+  const int nb0 = col0;                      // column block
+  d4 yr[YC], yi[YC];
+  for (int y = 0; y < YC; ++y) {
     yr[y] = d4{0, 0, 0, 0};
     yi[y] = d4{0, 0, 0, 0};
   }
@@ -104,13 +141,13 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
   // products (each accumulator still takes its steps in ascending row order)
   const int c = cb + lr;
   for (int g0 = rbeg; g0 < rend; g0 += 4 * kBtSteps) {
-    zc v[kBtSteps], xv[kBtSteps][2];
+    zc v[kBtSteps], xv[kBtSteps][YC];
 #pragma unroll
     for (int t = 0; t < kBtSteps; ++t) {
       const int r = g0 + 4 * t + lk;
       v[t] = (r < rend && c < kp) ? vref(H, ld, r, p + c) : mkz(0, 0);
 #pragma unroll
-      for (int y = 0; y < 2; ++y) {
+      for (int y = 0; y < YC; ++y) {
         const int col = nb0 + y * 16 + lr;
         xv[t][y] = (r < rend && col < m) ? X[(size_t)r + (size_t)ldx * col] : mkz(0, 0);
       }
@@ -119,7 +156,7 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
     for (int t = 0; t < kBtSteps; ++t) {
       if (g0 + 4 * t >= rend) break;               // the same steps as one step per 4 rows
 #pragma unroll
-      for (int y = 0; y < 2; ++y) {
+      for (int y = 0; y < YC; ++y) {
         // conj(v) x = (vr xr + vi xi) + i (vr xi - vi xr)
         yr[y] = mfma16x16x4(v[t].re, xv[t][y].re, yr[y]);
         yr[y] = mfma16x16x4(v[t].im, xv[t][y].im, yr[y]);
@@ -129,7 +166,7 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
     }
   }
   zc* out = Ypart + (size_t)blockIdx.y * BT_NB * m;
-  for (int y = 0; y < 2; ++y) {
+  for (int y = 0; y < YC; ++y) {
     const int col = nb0 + y * 16 + lr;
     if (col >= m) continue;
     for (int q = 0; q < 4; ++q) {
@@ -147,16 +184,19 @@ __global__ __launch_bounds__(256) void k_bt_ty(const zc* __restrict__ Ypart, int
   __shared__ zc Ss[BT_NB][8];
   const int tid = threadIdx.x;
   for (int e = tid; e < BT_NB * BT_NB; e += 256) Ts[e] = Tf[e];
-  const int c = tid >> 3, cl = tid & 7, col = blockIdx.x * 8 + cl;
-  zc acc = mkz(0, 0);
-  if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
-  Ss[c][cl] = acc;
+  const int cl = tid & 7, col = blockIdx.x * 8 + cl;
+  for (int c = tid >> 3; c < BT_NB; c += 32) {       // rows c and c + 32 of the block
+    zc acc = mkz(0, 0);
+    if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
+    Ss[c][cl] = acc;
+  }
   __syncthreads();
   if (col >= m) return;
-  const int r = c;
-  zc y = mkz(0, 0);
-  for (int k = r; k < kp; ++k) y = fma_z(Ts[r + BT_NB * k], Ss[k][cl], y);   // T column-major
-  Y[(size_t)r * m + col] = y;
+  for (int r = tid >> 3; r < BT_NB; r += 32) {
+    zc y = mkz(0, 0);
+    for (int k = r; k < kp; ++k) y = fma_z(Ts[r + BT_NB * k], Ss[k][cl], y);   // T column-major
+    Y[(size_t)r * m + col] = y;
+  }
 }
 
 // rows per V^H X chunk: 256, or down to 64 when the columns alone give few workgroups
@@ -259,7 +299,9 @@ __global__ __launch_bounds__(256) void k_bt_phase(int N, int m, zc* __restrict__
 size_t backtr_work_bytes(int N, int m) {
   const size_t np = (N + BT_NB - 1) / BT_NB + 1;
   const size_t nch = (N + 63) / 64 + 1;          // chunks of at least 64 rows
-  return sizeof(zc) * (np * BT_NB * BT_NB + nch * BT_NB * (size_t)m + BT_NB * (size_t)m) + 3 * 256;
+  return sizeof(zc) * (np * BT_NB * BT_NB + nch * BT_NB * (size_t)m + BT_NB * (size_t)m +
+                       np * kBtSplit * (size_t)kBtPairSlots) +
+         4 * 256;
 }
 
 template <class T>
@@ -274,10 +316,13 @@ static int run_backtr_t(hipStream_t s, const T* H, int ld, int N, const zc* tau,
   const size_t nchmax = (N + 63) / 64 + 1;
   w += ((sizeof(zc) * nchmax * BT_NB * (size_t)m + 255) & ~size_t(255));
   zc* Y = (zc*)w;
+  w += ((sizeof(zc) * BT_NB * (size_t)m + 255) & ~size_t(255));
+  zc* Gp = (zc*)w;                                // Gram partials, kBtSplit per block
   const int gl = (int)std::min<size_t>(((size_t)N * m + 255) / 256, 65536);
   hipLaunchKernelGGL(k_bt_load, dim3(gl), dim3(256), 0, s, N, m, Z, c0, X, ldx);
   if (np > 0) {
-    hipLaunchKernelGGL(k_bt_tfactor<T>, dim3(np), dim3(256), 0, s, H, ld, N, tau, Tf);
+    hipLaunchKernelGGL(k_bt_gram<T>, dim3(np, kBtSplit), dim3(256), 0, s, H, ld, N, Gp);
+    hipLaunchKernelGGL(k_bt_tfactor, dim3(np), dim3(256), 0, s, N, tau, Gp, Tf);
     const int ncol = (m + 63) / 64;
     for (int b = np - 1; b >= 0; --b) {
       const int p = b * BT_NB;

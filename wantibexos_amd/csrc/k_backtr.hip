@@ -16,6 +16,7 @@ namespace wtb {
 // the launches, whose fixed costs dominate each block's time
 #define BT_NB 64
 constexpr int kBtVR = 16;                                    // T factor: V rows per LDS slice
+constexpr int kBtTyCols = 256 / BT_NB;                       // Y = T Y: columns per workgroup
 constexpr int kBtPairs = (BT_NB * (BT_NB - 1) / 2 + 255) / 256;   // Gram pairs per thread
 constexpr int kBtSteps = 4;   // V^H X: MFMA steps (4 rows each) per group of loads
 
@@ -176,27 +177,26 @@ __global__ __launch_bounds__(256) void k_bt_vhx(const T* __restrict__ H, int ld,
   }
 }
 
-// Y(:, col) = T sum_chunks Ypart(:, col) for 8 columns per workgroup: thread (c, col)
+// Y(:, col) = T sum_chunks Ypart(:, col) for kBtTyCols columns per workgroup: thread (c, col)
 // adds its chunk sums in order into LDS, then thread (r, col) applies row r of T
 __global__ __launch_bounds__(256) void k_bt_ty(const zc* __restrict__ Ypart, int nchunk, int m, int kp,
                                                const zc* __restrict__ Tf, zc* __restrict__ Y) {
+  static_assert(BT_NB * kBtTyCols == 256, "one (row, column) pair per thread");
   __shared__ zc Ts[BT_NB * BT_NB];
-  __shared__ zc Ss[BT_NB][8];
+  __shared__ zc Ss[BT_NB][kBtTyCols];
   const int tid = threadIdx.x;
   for (int e = tid; e < BT_NB * BT_NB; e += 256) Ts[e] = Tf[e];
-  const int cl = tid & 7, col = blockIdx.x * 8 + cl;
-  for (int c = tid >> 3; c < BT_NB; c += 32) {       // rows c and c + 32 of the block
-    zc acc = mkz(0, 0);
-    if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
-    Ss[c][cl] = acc;
-  }
+  const int cl = tid % kBtTyCols, col = blockIdx.x * kBtTyCols + cl;
+  const int c = tid / kBtTyCols;                       // row of the block
+  zc acc = mkz(0, 0);
+  if (c < kp && col < m) acc = strided_sum(Ypart + (size_t)c * m + col, (size_t)BT_NB * m, 0, nchunk, 1, acc);
+  Ss[c][cl] = acc;
   __syncthreads();
   if (col >= m) return;
-  for (int r = tid >> 3; r < BT_NB; r += 32) {
-    zc y = mkz(0, 0);
-    for (int k = r; k < kp; ++k) y = fma_z(Ts[r + BT_NB * k], Ss[k][cl], y);   // T column-major
-    Y[(size_t)r * m + col] = y;
-  }
+  const int r = c;
+  zc y = mkz(0, 0);
+  for (int k = r; k < kp; ++k) y = fma_z(Ts[r + BT_NB * k], Ss[k][cl], y);   // T column-major
+  Y[(size_t)r * m + col] = y;
 }
 
 // rows per V^H X chunk: 256, or down to 64 when the columns alone give few workgroups
@@ -332,7 +332,7 @@ static int run_backtr_t(hipStream_t s, const T* H, int ld, int N, const zc* tau,
       const int nch = (rows + chunk - 1) / chunk;
       hipLaunchKernelGGL(k_bt_vhx<T>, dim3(ncol, nch), dim3(256), 0, s, H, ld, N, p, kp, X, ldx, m, chunk,
                          Ypart);
-      hipLaunchKernelGGL(k_bt_ty, dim3((m + 7) / 8), dim3(256), 0, s, Ypart, nch, m, kp,
+      hipLaunchKernelGGL(k_bt_ty, dim3((m + kBtTyCols - 1) / kBtTyCols), dim3(256), 0, s, Ypart, nch, m, kp,
                          Tf + (size_t)b * BT_NB * BT_NB, Y);
       hipLaunchKernelGGL(k_bt_update<T>, dim3((rows + 63) / 64, ncol), dim3(256), 0, s, H, ld, N, p, kp,
                          Y, X, ldx, m);
